@@ -1,0 +1,232 @@
+/*
+ * mpf.h -- C ABI of the MI355X (gfx950) femSolidmechanics assembly and
+ * Newton-Krylov path.  Plain pointers and sizes only; no C++ types cross it.
+ *
+ * Conventions (same as the reference, chennachaos/mphyfem, paths under src/):
+ *   - nodal vectors are interleaved u[node*3 + d]            (ElementBase.cpp:341-345)
+ *   - element dof order is 3*a + d                            (Elem_Solid_3D_Disp.cpp:322-324)
+ *   - Hex8 node order as LagrangeBasisFunsHexa                (BasisFunctionsLagrange.cpp:1202-1279)
+ *   - Tet10 node order as LagrangeBasisFunsTetra, edges 4:(0,1) 5:(1,2) 6:(0,2) 7:(0,3)
+ *     8:(2,3) 9:(1,3); P1 pressure on nodes 0..3             (BasisFunctionsLagrange.cpp:870-956,
+ *                                                              Elem_Solid_3D_Mixed1.cpp:66-70)
+ *   - sigma[9] and C[81] use the column-major index map (0,0),(1,0),(2,0),(0,1),...
+ *                                                             (utilitiesmaterialHyperelastic.cpp:191)
+ *   - dense matrices crossing this ABI are ROW-major (Eigen's default is column-major:
+ *     the C++ wrapper in mpf_host.hpp transposes).
+ *   - global numbering: free displacement dofs node-major skipping prescribed dofs, then
+ *     pressure dofs at offset dispDOF in increasing node id; CSR rows hold sorted unique
+ *     columns                                                 (femSolidmechanics2.cpp:121-478)
+ *
+ * Errors are returned as mpf_status; nothing throws across this boundary.
+ * "Device pointer" arguments must point to memory on the current HIP device.
+ * All calls taking a hipStream_t are stream-ordered; the problem-level calls run on the
+ * problem's own stream and synchronise before returning where they return host values.
+ */
+#ifndef MPF_H
+#define MPF_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* mpf_stream_t; /* == hipStream_t */
+
+typedef enum {
+  MPF_OK = 0,
+  MPF_ERR_ARG = 1,            /* invalid argument or size */
+  MPF_ERR_HIP = 2,            /* HIP runtime error (allocation, launch) */
+  MPF_ERR_NEG_JACOBIAN = 3,   /* det F < 0 in an element: runtime_error("Negative Jacobian in the element")
+                                 (Elem_Solid_3D_Disp.cpp:273-278; ElementBase2.cpp:382-385) */
+  MPF_ERR_STATE = 4,          /* call out of order, e.g. solve before assemble (SolverPetsc.cpp:321-323) */
+  MPF_ERR_NOT_CONVERGED = 5,  /* Krylov or Newton did not reach its tolerance */
+  MPF_ERR_DIVERGED = 6,       /* NaN residual or ratio > 1e7 after iteration 3 (femSolidmechanics1.cpp:627-636) */
+  MPF_ERR_NO_DEVICE = 7       /* no HIP device visible */
+} mpf_status;
+
+/* Element type ids (npElem). */
+enum { MPF_HEX8 = 8, MPF_TET10P1 = 10 };
+
+/* Neo-Hookean parameters: Matl_NeoHookean (Matl_NeoHookean.cpp:45-53) reads mu = data_deviatoric[0],
+ * Utype and Kinv (MaterialBase.cpp:91-115); K = 1/Kinv. */
+typedef struct {
+  double mu;
+  double Kinv;
+  int32_t Utype;   /* -1 or 1..8, volumetric energy (utilitiesmaterialHyperelastic.cpp:7-41) */
+  int32_t mixed;   /* 1 for the u-p element: sigma uses the pressure field */
+  int32_t tangent; /* displacement element only.  MPF_TANGENT_REFERENCE: the spatial tangent exactly as
+                      computeStressAndTangent_NeoHooke codes it, whose volumetric part omits its stress
+                      term U'(J) d_ik d_jl (utilitiesmaterialHyperelastic.cpp:280-285), so K != -dR/du.
+                      MPF_TANGENT_CONSISTENT: that term included (K = -dR/du); same residual. */
+  int32_t pad_;
+} mpf_matl;
+enum { MPF_TANGENT_REFERENCE = 0, MPF_TANGENT_CONSISTENT = 1 };
+
+const char* mpf_status_string(int32_t status);
+/* Number of visible HIP devices (0 when none). */
+int32_t mpf_device_count(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Material operator.  Replaces MaterialBase::computeStressAndTangent (MaterialBase.h:94) for    */
+/* Matl_NeoHookean -> computeStressAndTangent_NeoHooke (utilitiesmaterialHyperelastic.cpp:186).  */
+/* F: [n][9] ROW-major 3x3 (device), p: [n] pressure (used when m->mixed), sig: [n][9], C: [n][81]*/
+/* in the reference's 9-index map, C row-major over that map.                                    */
+mpf_status mpf_neohooke_stress_tangent_batch(const double* F, const double* p, const mpf_matl* m,
+                                             int32_t n, double* sig, double* C, mpf_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Element operators (dense K_e / R_e per element, device buffers).                              */
+/* Hex8 displacement element: Elem_Solid_3D_Disp::calcStiffnessAndResidual                     */
+/* (Elem_Solid_3D_Disp.cpp:196-374; ElementBase.h:149).  X: [nNode][3], u: [nNode*3],            */
+/* conn: [nElem][8].  Elements [first, first+count): Ke [count][24][24] row-major, Re [count][24]. */
+/* bad: one int32 (device); set to 1 + (smallest element id) with det F < 0, else left unchanged */
+/* (caller zeroes it).                                                                           */
+mpf_status mpf_elem_hex8_nh_batch(const double* X, const double* u, const int32_t* conn,
+                                  const mpf_matl* m, int32_t first, int32_t count,
+                                  double* Ke, double* Re, int32_t* bad, mpf_stream_t stream);
+
+/* Tet10/P1 mixed element: ElementBase::calcStiffnessAndResidualMixed3D via                     */
+/* Elem_Solid_3D_Mixed1::calcStiffnessAndResidualMixed (ElementBase2.cpp:278-552;               */
+/* Elem_Solid_3D_Mixed1.cpp:89-97; ElementBase.h:171).  u_prev: previous converged displacement  */
+/* (Fn, ElementBase.cpp:302-329); pres: [nNode] nodal pressure indexed by node id.               */
+/* Kuu [count][30][30], Kup [count][30][4], Kpu [count][4][30], Kpp [count][4][4],              */
+/* Fu [count][30], Fp [count][4].                                                                */
+mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const double* u_prev,
+                                     const double* pres, const int32_t* conn, const mpf_matl* m,
+                                     int32_t first, int32_t count,
+                                     double* Kuu, double* Kup, double* Kpu, double* Kpp,
+                                     double* Fu, double* Fp, int32_t* bad, mpf_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Problem: the femSolidmechanics-shaped driver state on one device (femSolidmechanics.h).       */
+typedef struct mpf_problem mpf_problem;
+
+/* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8 or MPF_TET10P1. */
+mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNode, int32_t nElem,
+                              const double* X, const int32_t* conn, const mpf_matl* m);
+mpf_status mpf_problem_destroy(mpf_problem* p);
+
+/* Dirichlet patch (BoundaryCondition 'fixed' when dof < 0, else 'specified' for dof 0..2 with  */
+/* value[i] * lambda(t)); femSolidmechanics1.cpp:1479-1543 and :813-915.  value may be NULL (0). */
+mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, int32_t dof,
+                             const double* value);
+/* Quad4 dead-load traction patch t = (v0*normal + v1*tangent1) * lambda                        */
+/* (femSolidmechanics1.cpp:922-1119; BasisFunctionsLagrange.cpp:1688-1732).  faces [nf][4].     */
+mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1);
+
+/* prepareMatrixPattern (femSolidmechanics2.cpp:121-478) on the device: DOF numbering, pressure   */
+/* dofs, CSR pattern (rowptr/colidx, int32, sorted), element colouring, per-element CSR slots.   */
+mpf_status mpf_prepare_pattern(mpf_problem* p);
+mpf_status mpf_get_sizes(const mpf_problem* p, int32_t* dispDOF, int32_t* presDOF, int64_t* nnz,
+                         int32_t* ncolors);
+/* Host copies of the pattern; any pointer may be NULL.  rowptr [n+1], colidx [nnz],             */
+/* NodeDofArray [nNode][3] (-1 = prescribed), ID_pres [nNode] (-1 = no pressure dof),            */
+/* color [nElem].                                                                                 */
+mpf_status mpf_get_pattern(const mpf_problem* p, int32_t* rowptr, int32_t* colidx,
+                           int32_t* NodeDofArray, int32_t* ID_pres, int32_t* color);
+
+/* Solution state (host copies).  disp/disp_prev [nNode*3]; pres/pres_prev [nNode].             */
+mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_prev,
+                         const double* pres, const double* pres_prev);
+mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres);
+/* setBoundaryConditions for load factor lam (femSolidmechanics1.cpp:813-915).                  */
+mpf_status mpf_set_bc(mpf_problem* p, double lam);
+
+/* calcStiffnessAndResidual + addExternalForces (femSolidmechanics2.cpp:1047-1120, :785-789):    */
+/* zeroMtx, coloured atomic-free element scatter into CSR, Dirichlet lift when first_iter.        */
+/* Returns MPF_ERR_NEG_JACOBIAN if any element has det F < 0.                                    */
+mpf_status mpf_assemble(mpf_problem* p, int32_t first_iter, double lam);
+/* Device pointers of the assembled system (valid until destroy): vals [nnz], rhs [n].          */
+mpf_status mpf_system_device_ptrs(mpf_problem* p, double** vals, double** rhs,
+                                  const int32_t** rowptr, const int32_t** colidx);
+/* Host copies of the assembled system. */
+mpf_status mpf_get_system(const mpf_problem* p, double* vals, double* rhs);
+/* ||rhs||_2 of the assembled system (VecNorm, femSolidmechanics2.cpp:801). */
+mpf_status mpf_rhs_norm(mpf_problem* p, double* norm);
+
+/* Krylov solve of the assembled system (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:321;   */
+/* KSPSolve with zero initial guess, :290-292).                                                  */
+enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1 };
+enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2 };
+typedef struct {
+  int32_t ksp;       /* MPF_KSP_* */
+  int32_t pc;        /* MPF_PC_* */
+  int32_t restart;   /* GMRES(m) */
+  int32_t maxit;
+  double rtol;       /* ||b - A x|| / ||b|| */
+} mpf_krylov_opts;
+typedef struct {
+  int32_t iterations;
+  int32_t converged;
+  double relres;
+  double seconds;
+} mpf_krylov_report;
+/* Solves K du = rhs; du stays on the device (mpf_get_solution copies it). */
+mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);
+mpf_status mpf_get_solution(const mpf_problem* p, double* du);
+/* factoriseSolveAndUpdate's update (femSolidmechanics2.cpp:1124-1217): on the first iteration   */
+/* prescribed dofs take dispApplied, free dofs += du, pressure dofs += du[dispDOF + k].          */
+mpf_status mpf_update(mpf_problem* p, int32_t first_iter);
+
+/* Newton-Raphson load stepping (solveWithNewtonRaphson, femSolidmechanics2.cpp:730-851).        */
+/* Step k uses lambda_k = sum of k dt's (MyTime::update, MyTime.cpp:26-37) with lambda(t) = t.   */
+/* Converged when ||R_free|| / R0 < rtol with R0 the iteration-1 residual of the step.           */
+typedef struct {
+  int32_t nsteps;
+  double dt;
+  int32_t itmax;           /* Newton iterations (assemblies) per step */
+  double rtol;             /* relative Newton tolerance, e.g. 1e-9 */
+  mpf_krylov_opts lin;     /* linear solver options */
+  int32_t forcing;         /* 1: inexact Newton, linear rtol = max(lin.rtol, min(0.1, 0.1*||R||/R0)) */
+} mpf_newton_opts;
+typedef struct {
+  int32_t steps_done;
+  int32_t newton_iters;    /* total assemblies */
+  int32_t krylov_iters;    /* total Krylov iterations */
+  double final_relres;
+  double seconds;
+  double seconds_assembly;
+  double seconds_solve;
+} mpf_newton_report;
+/* Runs opts->nsteps load steps from the current state; iters [nsteps] (may be NULL) receives the */
+/* Newton iterations per step.                                                                   */
+mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* iters,
+                            mpf_newton_report* rep);
+/* One load step to load factor lam from the current converged state.  guess (host, [nNode*3],   */
+/* may be NULL): if given, Newton starts from it (prescribed dofs are set to their targets) and  */
+/* the tolerance is still measured against R0 of the plain start (previous converged state with  */
+/* the lifted increment).  iters receives the Newton iterations of this step.                    */
+mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, const double* guess,
+                           int32_t* iters, mpf_newton_report* rep);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Timing helpers: average device time (ms) of one assembly (element kernels + scatter, no       */
+/* pattern build) over reps launches, measured with HIP events on the problem's stream.          */
+mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, double* ms_per_assembly,
+                             double* ms_scatter_kernels);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Synthetic meshes (seeded, SplitMix64 with u = (x >> 11) * 2^-53).  Output buffers are host     */
+/* memory sized by the *_sizes call.                                                              */
+/* Structured Hex8 block nx*ny*nz on [0,Lx]x[0,Ly]x[0,Lz]; node id i + (nx+1)*(j + (ny+1)*k);      */
+/* coordinate d of a node is jittered by U(-jitter,+jitter)*h_d when its grid index along d is    */
+/* interior.                                                                                     */
+void mpf_mesh_hex_sizes(int32_t nx, int32_t ny, int32_t nz, int32_t* nNode, int32_t* nElem);
+void mpf_mesh_hex(int32_t nx, int32_t ny, int32_t nz, double Lx, double Ly, double Lz, double jitter,
+                  uint64_t seed, double* X, int32_t* conn);
+/* Kuhn 6-tet split of an n^3 cube grid into Tet10 on the doubled (2n+1)^3 node grid; vertex     */
+/* jitter as above, mid-edge nodes at edge midpoints.                                             */
+void mpf_mesh_tet10_sizes(int32_t n, int32_t* nNode, int32_t* nElem);
+void mpf_mesh_tet10(int32_t n, double L, double jitter, uint64_t seed, double* X, int32_t* conn);
+/* Node ids of a face of a structured block: face 0..5 = x=0, x=max, y=0, y=max, z=0, z=max.     */
+int32_t mpf_mesh_face_nodes(int32_t elem_type, int32_t nx, int32_t ny, int32_t nz, int32_t face,
+                            int32_t* nodes);
+/* Quad4 faces (outward normal under the reference's n = dv x du) of a Hex8 block face.          */
+int32_t mpf_mesh_hex_face_quads(int32_t nx, int32_t ny, int32_t nz, int32_t face, int32_t* quads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,409 @@
+"""mphyfem_amd -- MI355X (gfx950) femSolidmechanics assembly and Newton-Krylov path.
+
+Python view of the C ABI in include/mpf.h (libmpf.so, built in-tree by
+``python -m mphyfem_amd.build``).  The names mirror the reference's API
+(chennachaos/mphyfem src/femSolidmechanics/femSolidmechanics.h,
+src/utilities/ElementBase.h, src/mySolvers/SolverPetsc.h).
+
+There is no CPU fallback: importing works without a GPU (the library loads and
+exports its symbols), but every compute call raises ``MpfError`` when no HIP
+device is present, and ``lib()`` raises when libmpf.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libmpf.so"
+
+HEX8 = 8
+TET10P1 = 10
+KSP_CG, KSP_GMRES = 0, 1
+PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NEG_JACOBIAN, ERR_STATE, ERR_NOT_CONVERGED, ERR_DIVERGED, ERR_NO_DEVICE = range(1, 8)
+
+
+class MpfError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} (status {status})")
+
+
+class NegativeJacobian(MpfError):
+    """runtime_error("Negative Jacobian in the element") of the reference (Elem_Solid_3D_Disp.cpp:277)."""
+
+
+TANGENT_REFERENCE, TANGENT_CONSISTENT = 0, 1
+
+
+class Matl(C.Structure):
+    _fields_ = [("mu", C.c_double), ("Kinv", C.c_double), ("Utype", C.c_int32), ("mixed", C.c_int32),
+                ("tangent", C.c_int32), ("pad_", C.c_int32)]
+
+
+class KrylovOpts(C.Structure):
+    _fields_ = [("ksp", C.c_int32), ("pc", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
+                ("rtol", C.c_double)]
+
+
+class KrylovReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double),
+                ("seconds", C.c_double)]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("dt", C.c_double), ("itmax", C.c_int32), ("rtol", C.c_double),
+                ("lin", KrylovOpts), ("forcing", C.c_int32)]
+
+
+class NewtonReport(C.Structure):
+    _fields_ = [("steps_done", C.c_int32), ("newton_iters", C.c_int32), ("krylov_iters", C.c_int32),
+                ("final_relres", C.c_double), ("seconds", C.c_double), ("seconds_assembly", C.c_double),
+                ("seconds_solve", C.c_double)]
+
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+
+# (name, restype, argtypes) for every symbol of include/mpf.h
+SIGNATURES = [
+    ("mpf_status_string", C.c_char_p, [C.c_int32]),
+    ("mpf_device_count", C.c_int32, []),
+    ("mpf_neohooke_stress_tangent_batch", C.c_int, [_P, _P, C.POINTER(Matl), C.c_int32, _P, _P, _P]),
+    ("mpf_elem_hex8_nh_batch", C.c_int, [_P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    ("mpf_elem_tet10p1_nh_batch", C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32,
+                                             _P, _P, _P, _P, _P, _P, _P, _P]),
+    ("mpf_problem_create", C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32, _DP, _I32P,
+                                     C.POINTER(Matl)]),
+    ("mpf_problem_destroy", C.c_int, [_P]),
+    ("mpf_add_dirichlet", C.c_int, [_P, C.c_int32, _I32P, C.c_int32, _DP]),
+    ("mpf_add_traction", C.c_int, [_P, C.c_int32, _I32P, C.c_double, C.c_double]),
+    ("mpf_prepare_pattern", C.c_int, [_P]),
+    ("mpf_get_sizes", C.c_int, [_P, _I32P, _I32P, _I64P, _I32P]),
+    ("mpf_get_pattern", C.c_int, [_P, _I32P, _I32P, _I32P, _I32P, _I32P]),
+    ("mpf_set_state", C.c_int, [_P, _DP, _DP, _DP, _DP]),
+    ("mpf_get_state", C.c_int, [_P, _DP, _DP]),
+    ("mpf_set_bc", C.c_int, [_P, C.c_double]),
+    ("mpf_assemble", C.c_int, [_P, C.c_int32, C.c_double]),
+    ("mpf_system_device_ptrs", C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    ("mpf_get_system", C.c_int, [_P, _DP, _DP]),
+    ("mpf_rhs_norm", C.c_int, [_P, _DP]),
+    ("mpf_solve", C.c_int, [_P, C.POINTER(KrylovOpts), C.POINTER(KrylovReport)]),
+    ("mpf_get_solution", C.c_int, [_P, _DP]),
+    ("mpf_update", C.c_int, [_P, C.c_int32]),
+    ("mpf_newton_solve", C.c_int, [_P, C.POINTER(NewtonOpts), _I32P, C.POINTER(NewtonReport)]),
+    ("mpf_newton_step", C.c_int, [_P, C.c_double, C.POINTER(NewtonOpts), _DP, _I32P, C.POINTER(NewtonReport)]),
+    ("mpf_time_assembly", C.c_int, [_P, C.c_int32, _DP, _DP]),
+    ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
+    ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_uint64, _DP, _I32P]),
+    ("mpf_mesh_tet10_sizes", None, [C.c_int32, _I32P, _I32P]),
+    ("mpf_mesh_tet10", None, [C.c_int32, C.c_double, C.c_double, C.c_uint64, _DP, _I32P]),
+    ("mpf_mesh_face_nodes", C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32P]),
+    ("mpf_mesh_hex_face_quads", C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32P]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libmpf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m mphyfem_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(st: int) -> str:
+    return lib().mpf_status_string(st).decode()
+
+
+def _check(st: int, what: str) -> None:
+    if st == OK:
+        return
+    if st == ERR_NEG_JACOBIAN:
+        raise NegativeJacobian(st, what)
+    raise MpfError(st, what)
+
+
+def device_count() -> int:
+    return int(lib().mpf_device_count())
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _dp(a):
+    return None if a is None else _ptr(a, C.c_double)
+
+
+def _ip(a):
+    return None if a is None else _ptr(a, C.c_int32)
+
+
+def matl(mu: float, Kinv: float, Utype: int = 3, mixed: bool = False, tangent: int = TANGENT_REFERENCE) -> Matl:
+    return Matl(float(mu), float(Kinv), int(Utype), int(mixed), int(tangent), 0)
+
+
+def matl_from_E_nu(E: float, nu: float, Utype: int = 3, mixed: bool = False,
+                   tangent: int = TANGENT_REFERENCE) -> Matl:
+    """mu = E / (2 (1 + nu)), Kinv = 3 (1 - 2 nu) / E."""
+    return matl(E / (2.0 * (1.0 + nu)), 3.0 * (1.0 - 2.0 * nu) / E, Utype, mixed, tangent)
+
+
+# ---------------------------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------------------------
+def mesh_hex(nx: int, ny: int, nz: int, L=(1.0, 1.0, 1.0), jitter: float = 0.0, seed: int = 1):
+    nn, ne = C.c_int32(), C.c_int32()
+    lib().mpf_mesh_hex_sizes(nx, ny, nz, C.byref(nn), C.byref(ne))
+    X = np.zeros((nn.value, 3))
+    conn = np.zeros((ne.value, 8), dtype=np.int32)
+    lib().mpf_mesh_hex(nx, ny, nz, float(L[0]), float(L[1]), float(L[2]), float(jitter), seed, _dp(X), _ip(conn))
+    return X, conn
+
+
+def mesh_tet10(n: int, L: float = 1.0, jitter: float = 0.0, seed: int = 2):
+    nn, ne = C.c_int32(), C.c_int32()
+    lib().mpf_mesh_tet10_sizes(n, C.byref(nn), C.byref(ne))
+    X = np.zeros((nn.value, 3))
+    conn = np.zeros((ne.value, 10), dtype=np.int32)
+    lib().mpf_mesh_tet10(n, float(L), float(jitter), seed, _dp(X), _ip(conn))
+    return X, conn
+
+
+def face_nodes(elem_type: int, nx: int, ny: int, nz: int, face: int) -> np.ndarray:
+    cnt = lib().mpf_mesh_face_nodes(elem_type, nx, ny, nz, face, None)
+    out = np.zeros(cnt, dtype=np.int32)
+    lib().mpf_mesh_face_nodes(elem_type, nx, ny, nz, face, _ip(out))
+    return out
+
+
+def hex_face_quads(nx: int, ny: int, nz: int, face: int) -> np.ndarray:
+    cnt = lib().mpf_mesh_hex_face_quads(nx, ny, nz, face, None)
+    out = np.zeros((cnt, 4), dtype=np.int32)
+    lib().mpf_mesh_hex_face_quads(nx, ny, nz, face, _ip(out))
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# element and material operators on device tensors (torch provides the device memory)
+# ---------------------------------------------------------------------------------------------
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def elem_hex8_nh_batch(X, u, conn, m: Matl, first: int = 0, count: int | None = None):
+    """Elem_Solid_3D_Disp::calcStiffnessAndResidual for a batch of Hex8 elements.
+
+    X [nNode,3], u [nNode*3] float64 and conn [nElem,8] int32 are CUDA tensors.  Returns
+    (Ke [count,24,24], Re [count,24], bad) where bad = 1 + first element id with det F < 0 or 0.
+    """
+    import torch
+    count = conn.shape[0] - first if count is None else count
+    dev = X.device
+    Ke = torch.empty((count, 24, 24), dtype=torch.float64, device=dev)
+    Re = torch.empty((count, 24), dtype=torch.float64, device=dev)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = lib().mpf_elem_hex8_nh_batch(X.data_ptr(), u.data_ptr(), conn.data_ptr(), C.byref(m), first, count,
+                                      Ke.data_ptr(), Re.data_ptr(), bad.data_ptr(), _stream())
+    _check(st, "mpf_elem_hex8_nh_batch")
+    return Ke, Re, int(bad.item())
+
+
+def elem_tet10p1_nh_batch(X, u, u_prev, pres, conn, m: Matl, first: int = 0, count: int | None = None):
+    """ElementBase::calcStiffnessAndResidualMixed3D (Tet10/P1) for a batch of elements."""
+    import torch
+    count = conn.shape[0] - first if count is None else count
+    dev = X.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    Kuu = torch.empty((count, 30, 30), **f64)
+    Kup = torch.empty((count, 30, 4), **f64)
+    Kpu = torch.empty((count, 4, 30), **f64)
+    Kpp = torch.empty((count, 4, 4), **f64)
+    Fu = torch.empty((count, 30), **f64)
+    Fp = torch.empty((count, 4), **f64)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = lib().mpf_elem_tet10p1_nh_batch(X.data_ptr(), u.data_ptr(), u_prev.data_ptr(), pres.data_ptr(),
+                                         conn.data_ptr(), C.byref(m), first, count, Kuu.data_ptr(), Kup.data_ptr(),
+                                         Kpu.data_ptr(), Kpp.data_ptr(), Fu.data_ptr(), Fp.data_ptr(),
+                                         bad.data_ptr(), _stream())
+    _check(st, "mpf_elem_tet10p1_nh_batch")
+    return Kuu, Kup, Kpu, Kpp, Fu, Fp, int(bad.item())
+
+
+def neohooke_stress_tangent(F, p, m: Matl):
+    """computeStressAndTangent_NeoHooke for a batch: F [n,3,3] (row-major) -> (sig [n,9], C [n,9,9])."""
+    import torch
+    n = F.shape[0]
+    sig = torch.empty((n, 9), dtype=torch.float64, device=F.device)
+    Cm = torch.empty((n, 9, 9), dtype=torch.float64, device=F.device)
+    st = lib().mpf_neohooke_stress_tangent_batch(F.data_ptr(), p.data_ptr() if p is not None else None,
+                                                 C.byref(m), n, sig.data_ptr(), Cm.data_ptr(), _stream())
+    _check(st, "mpf_neohooke_stress_tangent_batch")
+    return sig, Cm
+
+
+# ---------------------------------------------------------------------------------------------
+# problem (femSolidmechanics-shaped)
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class Pattern:
+    rowptr: np.ndarray
+    colidx: np.ndarray
+    NodeDofArray: np.ndarray
+    ID_pres: np.ndarray
+    color: np.ndarray
+    dispDOF: int
+    presDOF: int
+    ncolors: int
+
+
+class Problem:
+    """Device-resident problem mirroring femSolidmechanics' driver calls."""
+
+    def __init__(self, elem_type: int, X: np.ndarray, conn: np.ndarray, m: Matl):
+        self.elem_type = elem_type
+        self.X = np.ascontiguousarray(X, dtype=np.float64)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.nNode, self.nElem = self.X.shape[0], self.conn.shape[0]
+        self._h = C.c_void_p()
+        st = lib().mpf_problem_create(C.byref(self._h), elem_type, self.nNode, self.nElem, _dp(self.X),
+                                      _ip(self.conn), C.byref(m))
+        _check(st, "mpf_problem_create")
+
+    def close(self):
+        if self._h:
+            lib().mpf_problem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # setSpecifiedDOFs_Displacement / setBoundaryConditions
+    def add_dirichlet(self, nodes, dof: int = -1, value=None):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        val = None if value is None else np.ascontiguousarray(np.broadcast_to(value, nodes.shape), dtype=np.float64)
+        _check(lib().mpf_add_dirichlet(self._h, len(nodes), _ip(nodes), dof, _dp(val)), "mpf_add_dirichlet")
+
+    def add_traction(self, faces, v0: float, v1: float = 0.0):
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        _check(lib().mpf_add_traction(self._h, faces.shape[0], _ip(faces), v0, v1), "mpf_add_traction")
+
+    def prepare_matrix_pattern(self):
+        _check(lib().mpf_prepare_pattern(self._h), "mpf_prepare_pattern")
+
+    def sizes(self):
+        d, p, c = C.c_int32(), C.c_int32(), C.c_int32()
+        nnz = C.c_int64()
+        _check(lib().mpf_get_sizes(self._h, C.byref(d), C.byref(p), C.byref(nnz), C.byref(c)), "mpf_get_sizes")
+        return d.value, p.value, nnz.value, c.value
+
+    def pattern(self) -> Pattern:
+        d, p, nnz, nc = self.sizes()
+        rowptr = np.zeros(d + p + 1, dtype=np.int32)
+        colidx = np.zeros(nnz, dtype=np.int32)
+        nda = np.zeros((self.nNode, 3), dtype=np.int32)
+        idp = np.zeros(self.nNode, dtype=np.int32)
+        color = np.zeros(self.nElem, dtype=np.int32)
+        _check(lib().mpf_get_pattern(self._h, _ip(rowptr), _ip(colidx), _ip(nda), _ip(idp), _ip(color)),
+               "mpf_get_pattern")
+        return Pattern(rowptr, colidx, nda, idp, color, d, p, nc)
+
+    def set_state(self, disp=None, disp_prev=None, pres=None, pres_prev=None):
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        a = [c(disp), c(disp_prev), c(pres), c(pres_prev)]
+        _check(lib().mpf_set_state(self._h, *[_dp(x) for x in a]), "mpf_set_state")
+
+    def state(self):
+        disp = np.zeros(self.nNode * 3)
+        pres = np.zeros(self.nNode)
+        _check(lib().mpf_get_state(self._h, _dp(disp), _dp(pres)), "mpf_get_state")
+        return disp, pres
+
+    def set_boundary_conditions(self, lam: float):
+        _check(lib().mpf_set_bc(self._h, lam), "mpf_set_bc")
+
+    def calc_stiffness_and_residual(self, first_iter: bool, lam: float):
+        _check(lib().mpf_assemble(self._h, int(first_iter), lam), "mpf_assemble")
+
+    def system(self):
+        d, p, nnz, _ = self.sizes()
+        vals = np.zeros(nnz)
+        rhs = np.zeros(d + p)
+        _check(lib().mpf_get_system(self._h, _dp(vals), _dp(rhs)), "mpf_get_system")
+        return vals, rhs
+
+    def rhs_norm(self) -> float:
+        v = C.c_double()
+        _check(lib().mpf_rhs_norm(self._h, C.byref(v)), "mpf_rhs_norm")
+        return v.value
+
+    def factorise_and_solve(self, ksp=KSP_CG, pc=PC_BJACOBI3, rtol=1e-10, maxit=10000, restart=50):
+        o = KrylovOpts(ksp, pc, restart, maxit, rtol)
+        r = KrylovReport()
+        st = lib().mpf_solve(self._h, C.byref(o), C.byref(r))
+        if st not in (OK, ERR_NOT_CONVERGED):
+            _check(st, "mpf_solve")
+        return r
+
+    def solution(self):
+        d, p, _, _ = self.sizes()
+        du = np.zeros(d + p)
+        _check(lib().mpf_get_solution(self._h, _dp(du)), "mpf_get_solution")
+        return du
+
+    def update(self, first_iter: bool):
+        _check(lib().mpf_update(self._h, int(first_iter)), "mpf_update")
+
+    @staticmethod
+    def newton_opts(nsteps=10, dt=0.1, itmax=20, rtol=1e-9, ksp=KSP_CG, pc=PC_BJACOBI3, lin_rtol=1e-10,
+                    maxit=20000, restart=50, forcing=False) -> NewtonOpts:
+        return NewtonOpts(nsteps, dt, itmax, rtol, KrylovOpts(ksp, pc, restart, maxit, lin_rtol), int(forcing))
+
+    def solve_with_newton_raphson(self, opts: NewtonOpts):
+        iters = np.zeros(max(opts.nsteps, 1), dtype=np.int32)
+        rep = NewtonReport()
+        st = lib().mpf_newton_solve(self._h, C.byref(opts), _ip(iters), C.byref(rep))
+        _check(st, "mpf_newton_solve")
+        return iters, rep
+
+    def newton_step(self, lam: float, opts: NewtonOpts, guess=None):
+        it = C.c_int32()
+        rep = NewtonReport()
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        st = lib().mpf_newton_step(self._h, lam, C.byref(opts), _dp(g), C.byref(it), C.byref(rep))
+        _check(st, "mpf_newton_step")
+        return it.value, rep
+
+    def time_assembly(self, reps: int = 20):
+        a, s = C.c_double(), C.c_double()
+        _check(lib().mpf_time_assembly(self._h, reps, C.byref(a), C.byref(s)), "mpf_time_assembly")
+        return a.value, s.value

@@ -1,0 +1,170 @@
+// mpf_internal.hpp -- shared internals of libmpf (gfx950).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mpf.h"
+
+namespace mpf {
+
+// Status carried through host code as an exception-free early return.
+#define MPF_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      std::fprintf(stderr, "libmpf: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                                 \
+      return MPF_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define MPF_TRY(call)                        \
+  do {                                       \
+    mpf_status s_ = (call);                  \
+    if (s_ != MPF_OK) return s_;             \
+  } while (0)
+
+// Owned device allocation.
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  mpf_status alloc(size_t count) {
+    if (count == n && ptr) return MPF_OK;
+    release();
+    if (count == 0) return MPF_OK;
+    if (hipMalloc(&ptr, count * sizeof(T)) != hipSuccess) {
+      std::fprintf(stderr, "libmpf: hipMalloc of %zu bytes failed\n", count * sizeof(T));
+      ptr = nullptr;
+      return MPF_ERR_HIP;
+    }
+    n = count;
+    return MPF_OK;
+  }
+  mpf_status upload(const T* h, size_t count, hipStream_t s) {
+    MPF_TRY(alloc(count));
+    if (count) MPF_HIP(hipMemcpyAsync(ptr, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return MPF_OK;
+  }
+  mpf_status download(T* h, size_t count, hipStream_t s) const {
+    if (count > n) return MPF_ERR_ARG;
+    if (count) MPF_HIP(hipMemcpyAsync(h, ptr, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    return MPF_OK;
+  }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------------------------
+// Element-kernel launch description shared by the dense and the CSR-scatter variants.
+// ---------------------------------------------------------------------------------------------
+struct ElemArgs {
+  const double* X;        // [nNode][3]
+  const double* u;        // [nNode*3]
+  const double* uprev;    // [nNode*3] (mixed)
+  const double* pres;     // [nNode]   (mixed)
+  const int32_t* conn;    // [nElem][npe]
+  double mu, Kinv;
+  int32_t Utype;
+  int32_t consistent;     // displacement element tangent mode (MPF_TANGENT_CONSISTENT)
+  int32_t* bad;           // 1 + element id with det F < 0 (smallest id wins)
+};
+
+// Per-element CSR scatter description.
+struct ScatterArgs {
+  const int32_t* elems;     // element ids of this launch (colour slice of the permutation)
+  int32_t count;
+  const int32_t* slot;      // per-element slot bases (layout: see pattern.hip)
+  const int32_t* nodeinfo;  // per node: (row length << 4) | (pressure flag << 3) | free mask (bits 0..2)
+  const int32_t* nda;       // NodeDofArray [nNode][3]
+  const int32_t* idp;       // ID_pres [nNode]
+  int32_t dispDOF;
+  double* vals;             // CSR values
+  double* rhs;              // rhs [n]
+  const double* dispApplied;  // [nNode*3]; non-null -> Dirichlet lift (first iteration)
+  int32_t scatterK;           // 0: residual only (K kept in registers for the lift)
+};
+
+// Kernel launchers (elem.hip).
+mpf_status launch_hex8_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,
+                             hipStream_t s);
+mpf_status launch_tet10_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kuu, double* Kup,
+                              double* Kpu, double* Kpp, double* Fu, double* Fp, hipStream_t s);
+mpf_status launch_hex8_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
+mpf_status launch_tet10_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
+
+// Slot-map sizes per element.
+constexpr int kHex8Slots = 64;                    // (a,b) node pairs
+constexpr int kTet10Slots = 100 + 40 + 40 + 16;   // uu (a,b), up (a,q), pu (q,b), pp (q1,q2)
+
+// ---------------------------------------------------------------------------------------------
+// Problem state (the femSolidmechanics-shaped driver state on one device).
+// ---------------------------------------------------------------------------------------------
+struct PatternBufs {
+  DevBuf<int32_t> nda;        // NodeDofArray [nNode][3]
+  DevBuf<int32_t> assy;       // assyForSoln [dispDOF]
+  DevBuf<int32_t> idp;        // ID_pres [nNode]
+  DevBuf<int32_t> assyP;      // assyForSolnPres [presDOF]
+  DevBuf<int32_t> nptr, nel;  // node -> element adjacency (CSR)
+  DevBuf<int32_t> nbptr, nb;  // node neighbourhoods NB(n), sorted
+  DevBuf<int32_t> rowptr, colidx;
+  DevBuf<int32_t> nodeinfo;   // (row length << 4) | (pressure flag << 3) | free mask
+  DevBuf<int32_t> color, perm;
+  DevBuf<int32_t> slot;       // per-element slot bases
+  std::vector<int32_t> color_off;  // host: [ncolors + 1] offsets into perm
+  int32_t dispDOF = 0, presDOF = 0, ncolors = 0;
+  int64_t nnz = 0;
+};
+
+struct KrylovWork;  // krylov.hip
+
+}  // namespace mpf
+
+struct mpf_problem {
+  int32_t etype = 0, npe = 0, nNode = 0, nElem = 0;
+  mpf_matl m{};
+  hipStream_t stream = nullptr;
+  mpf::DevBuf<double> X;
+  mpf::DevBuf<int32_t> conn;
+  // Dirichlet data (host lists, merged last-wins per dof like setBoundaryConditions)
+  std::vector<int32_t> bc_dof_h;
+  std::vector<double> bc_val_h;
+  mpf::DevBuf<int32_t> bc_dof;
+  mpf::DevBuf<double> bc_val;
+  // traction patches (host) and the unit-load external force vector (device)
+  struct Trac { std::vector<int32_t> faces; double v0, v1; };
+  std::vector<Trac> trac;
+  mpf::DevBuf<double> fext_unit;  // [nNode*3] for lambda = 1
+  bool has_fext = false;
+  mpf::PatternBufs pat;
+  bool prepared = false;
+  // state
+  mpf::DevBuf<double> disp, dispPrev, pres, presPrev, dispApplied;
+  // system
+  mpf::DevBuf<double> vals, rhs, du;
+  mpf::DevBuf<int32_t> bad;
+  bool assembled = false;
+  bool solved = false;
+  mpf::KrylovWork* kw = nullptr;
+};
+
+namespace mpf {
+mpf_status build_pattern(mpf_problem* p);          // pattern.hip
+mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
+void krylov_free(mpf_problem* p);
+mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);
+}  // namespace mpf

@@ -1,0 +1,312 @@
+"""GPU parity: the HIP path, called through the C ABI, against the oracle on the same seeded inputs.
+
+Bars (set by the workload definition): CSR row pointers, column indices, DOF numbering and the
+colouring bit-exact; K_e, R_e and the global K / R within 1e-12 normwise relative
+(max |a - b| / max |b|) in FP64; converged displacements within 1e-9 relative.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, random_state, tet_compress_bcs
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+TOL_K = 1e-12
+
+
+def _t(gpu, a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device=gpu, dtype=dtype)
+
+
+def test_hex8_element_golden(mpf, orc, gpu):
+    import torch
+    g = np.load(GOLD / "hex8_element.npz")
+    X, u = g["X"], g["u"]
+    conn = np.arange(8, dtype=np.int32)[None]
+    for tangent, Kexp in [(mpf.TANGENT_REFERENCE, g["Ke_consistent"] - g["Ke_volgeo"]),
+                          (mpf.TANGENT_CONSISTENT, g["Ke_consistent"])]:
+        m = mpf.matl(float(g["mu"]), float(g["Kinv"]), 3, tangent=tangent)
+        Ke, Re, bad = mpf.elem_hex8_nh_batch(_t(gpu, X), _t(gpu, u.ravel()), _t(gpu, conn, torch.int32), m)
+        assert bad == 0
+        assert normwise_rel(Re.cpu().numpy()[0], g["Re"]) < 1e-12
+        assert normwise_rel(Ke.cpu().numpy()[0], Kexp) < 1e-7  # FD fixture accuracy
+        Ko, Ro, _ = (orc.elem_disp if tangent == 0 else orc.elem_disp_consistent)(X, u, m.mu, m.Kinv, 3)
+        assert normwise_rel(Ke.cpu().numpy()[0], Ko) < TOL_K
+        assert normwise_rel(Re.cpu().numpy()[0], Ro) < TOL_K
+
+
+@pytest.mark.parametrize("tangent", [0, 1])
+def test_hex8_batch_vs_oracle(mpf, orc, gpu, tangent):
+    import torch
+    n = 10
+    X, conn = cube_hex(n)
+    u = random_state(len(X), 0.01, seed=11)
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=tangent)
+    Ke, Re, bad = mpf.elem_hex8_nh_batch(_t(gpu, X), _t(gpu, u), _t(gpu, conn, torch.int32), m)
+    assert bad == 0
+    Ke, Re = Ke.cpu().numpy(), Re.cpu().numpy()
+    f = orc.elem_disp if tangent == 0 else orc.elem_disp_consistent
+    worst_k = worst_r = 0.0
+    for e in range(0, len(conn), 7):
+        Ko, Ro, st = f(X[conn[e]], u.reshape(-1, 3)[conn[e]], m.mu, m.Kinv, 3)
+        worst_k = max(worst_k, normwise_rel(Ke[e], Ko))
+        worst_r = max(worst_r, normwise_rel(Re[e], Ro))
+    assert worst_k < TOL_K and worst_r < TOL_K, (worst_k, worst_r)
+    # first / count sub-range
+    Ke2, Re2, _ = mpf.elem_hex8_nh_batch(_t(gpu, X), _t(gpu, u), _t(gpu, conn, torch.int32), m, first=13, count=5)
+    assert np.array_equal(Ke2.cpu().numpy(), Ke[13:18])
+
+
+def test_tet10p1_golden_and_batch(mpf, orc, gpu):
+    import torch
+    g = np.load(GOLD / "tet10p1_element.npz")
+    conn1 = np.arange(10, dtype=np.int32)[None]
+    m = mpf.matl(float(g["mu"]), float(g["Kinv"]), 3, mixed=True)
+    pres = np.concatenate([g["pres"], np.zeros(6)])
+    out = mpf.elem_tet10p1_nh_batch(_t(gpu, g["X"]), _t(gpu, g["u"].ravel()), _t(gpu, g["uprev"].ravel()),
+                                    _t(gpu, pres), _t(gpu, conn1, torch.int32), m)
+    Kuu, Kup, Kpu, Kpp, Fu, Fp, bad = [o.cpu().numpy() if hasattr(o, "cpu") else o for o in out]
+    assert bad == 0
+    R = np.concatenate([Fu[0], Fp[0]])
+    K = np.block([[Kuu[0], Kup[0]], [Kpu[0], Kpp[0]]])
+    assert normwise_rel(R, g["R"]) < 1e-12
+    assert normwise_rel(K, g["K"]) < 1e-6
+    # batch vs oracle
+    n = 3
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    u = random_state(len(X), 0.01, seed=5)
+    up = 0.6 * u
+    p = np.random.default_rng(6).uniform(-1, 1, len(X))
+    out = mpf.elem_tet10p1_nh_batch(_t(gpu, X), _t(gpu, u), _t(gpu, up), _t(gpu, p), _t(gpu, conn, torch.int32),
+                                    mpf.matl(1.0, 1e-3, 3, mixed=True))
+    Kuu, Kup, Kpu, Kpp, Fu, Fp, bad = [o.cpu().numpy() if hasattr(o, "cpu") else o for o in out]
+    assert bad == 0
+    worst = 0.0
+    for e in range(0, len(conn), 5):
+        c = conn[e]
+        o = orc.elem_tet10p1(X[c], u.reshape(-1, 3)[c], up.reshape(-1, 3)[c], p[c[:4]], 1.0, 1e-3, 3)
+        Ko = np.block([[o[0], o[1]], [o[2], o[3]]])
+        Kg = np.block([[Kuu[e], Kup[e]], [Kpu[e], Kpp[e]]])
+        worst = max(worst, normwise_rel(Kg, Ko), normwise_rel(np.concatenate([Fu[e], Fp[e]]),
+                                                              np.concatenate([o[4], o[5]])))
+    assert worst < TOL_K, worst
+
+
+def test_negative_jacobian_flag(mpf, gpu):
+    import torch
+    X, conn = cube_hex(3, jitter=0.0)
+    u = np.zeros((len(X), 3))
+    e_bad = 13
+    # collapse element 13 through itself: move its top nodes below its bottom nodes
+    for a in range(4, 8):
+        u[conn[e_bad][a], 2] = -1.5 / 3
+    Ke, Re, bad = mpf.elem_hex8_nh_batch(_t(gpu, X), _t(gpu, u.ravel()), _t(gpu, conn, torch.int32),
+                                         mpf.matl(10.0, 1e-2))
+    assert bad >= 1 and bad - 1 <= e_bad
+    P = mpf.Problem(mpf.HEX8, X, conn, mpf.matl(10.0, 1e-2))
+    P.prepare_matrix_pattern()
+    P.set_state(disp=u.ravel())
+    with pytest.raises(mpf.NegativeJacobian):
+        P.calc_stiffness_and_residual(False, 1.0)
+
+
+def test_material_batch_vs_oracle(mpf, orc, gpu):
+    rng = np.random.default_rng(4)
+    Fs = np.eye(3)[None] + rng.uniform(-0.1, 0.1, (32, 3, 3))
+    ps = rng.uniform(-3, 3, 32)
+    for mixed, tangent, Ut in [(False, 0, 3), (False, 1, 3), (True, 0, 3), (False, 0, 1), (False, 0, 4), (False, 0, 7)]:
+        m = mpf.matl(12.0, 1e-2, Ut, mixed=mixed, tangent=tangent)
+        sig, Cm = mpf.neohooke_stress_tangent(_t(gpu, Fs), _t(gpu, ps), m)
+        sig, Cm = sig.cpu().numpy(), Cm.cpu().numpy()
+        for i in range(0, 32, 3):
+            so, Co = orc.neohooke(Fs[i], 12.0, 1e-2, Ut, mixed=mixed, pres=ps[i])
+            if tangent == 1:
+                J = np.linalg.det(Fs[i])
+                U1 = orc.lib().orc_dUdJ(Ut, 100.0, J)
+                mp = [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1), (2, 1), (0, 2), (1, 2), (2, 2)]
+                Co = Co + np.array([[U1 * (j == l) * (i_ == k) for (k, l) in mp] for (i_, j) in mp])
+            assert normwise_rel(sig[i], so) < TOL_K
+            assert normwise_rel(Cm[i], Co) < TOL_K
+
+
+def _hex_problem(mpf, n, m, traction=None):
+    X, conn = cube_hex(n)
+    P = mpf.Problem(mpf.HEX8, X, conn, m)
+    apply_bcs(P, hex_stretch_bcs(n))
+    if traction is not None:
+        P.add_traction(mpf.hex_face_quads(n, n, n, 5), *traction)
+    P.prepare_matrix_pattern()
+    return X, conn, P
+
+
+def _orc_hex_problem(orc, mpf, X, conn, n, m, traction=None):
+    O = orc.Problem(8, X, conn, m.mu, m.Kinv, m.Utype)
+    O.set_tangent(m.tangent == 1)
+    apply_bcs(O, hex_stretch_bcs(n))
+    if traction is not None:
+        O.add_traction(mpf.hex_face_quads(n, n, n, 5), *traction)
+    O.prepare_matrix_pattern()
+    return O
+
+
+def test_pattern_and_colouring_bitexact_hex(mpf, orc, gpu):
+    n = 10
+    m = mpf.matl_from_E_nu(1e3, 0.3)
+    X, conn, P = _hex_problem(mpf, n, m)
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    pat = P.pattern()
+    rowptr, colidx, nda, idp = O.pattern()
+    assert (pat.dispDOF, pat.presDOF, len(pat.colidx)) == (3509, 0, 231601)
+    assert np.array_equal(pat.rowptr, rowptr)
+    assert np.array_equal(pat.colidx, colidx)
+    assert np.array_equal(pat.NodeDofArray, nda)
+    col, ncol = O.color()
+    assert pat.ncolors == ncol
+    assert np.array_equal(pat.color, col)
+
+
+def test_pattern_and_colouring_bitexact_tet(mpf, orc, gpu):
+    n = 4
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    m = mpf.matl(1.0, 1e-3, 3, mixed=True)
+    P = mpf.Problem(mpf.TET10P1, X, conn, m)
+    O = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    for Q in (P, O):
+        apply_bcs(Q, tet_compress_bcs(n))
+        Q.prepare_matrix_pattern()
+    pat = P.pattern()
+    rowptr, colidx, nda, idp = O.pattern()
+    assert np.array_equal(pat.rowptr, rowptr) and np.array_equal(pat.colidx, colidx)
+    assert np.array_equal(pat.NodeDofArray, nda) and np.array_equal(pat.ID_pres, idp)
+    col, ncol = O.color()
+    assert pat.ncolors == ncol and np.array_equal(pat.color, col)
+
+
+@pytest.mark.parametrize("tangent,traction", [(0, None), (1, (-20.0, 5.0))])
+def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction):
+    n = 10
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=tangent)
+    X, conn, P = _hex_problem(mpf, n, m, traction)
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m, traction)
+    u = random_state(len(X), 0.005, seed=21)
+    for Q in (P, O):
+        Q.set_state(disp=u, disp_prev=u)
+        Q.set_boundary_conditions(0.7)
+    for first in (True, False):
+        P.calc_stiffness_and_residual(first, 0.7)
+        vals, rhs = P.system()
+        ov, orhs, st = O.assemble(first, 0.7)
+        assert st == 0
+        assert normwise_rel(vals, ov) < TOL_K
+        assert normwise_rel(rhs, orhs) < TOL_K
+        assert abs(P.rhs_norm() - np.linalg.norm(orhs)) < 1e-12 * np.linalg.norm(orhs)
+
+
+def test_global_assembly_tet_vs_oracle(mpf, orc, gpu):
+    n = 4
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    m = mpf.matl(1.0, 1e-3, 3, mixed=True)
+    P = mpf.Problem(mpf.TET10P1, X, conn, m)
+    O = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    u = random_state(len(X), 0.004, seed=31)
+    up = 0.5 * u
+    p = np.random.default_rng(32).uniform(-0.5, 0.5, len(X))
+    for Q in (P, O):
+        apply_bcs(Q, tet_compress_bcs(n))
+        Q.prepare_matrix_pattern()
+        Q.set_state(disp=u, disp_prev=up, pres=p, pres_prev=p)
+        Q.set_boundary_conditions(1.0)
+    for first in (True, False):
+        P.calc_stiffness_and_residual(first, 1.0)
+        vals, rhs = P.system()
+        ov, orhs, st = O.assemble(first, 1.0)
+        assert normwise_rel(vals, ov) < TOL_K
+        assert normwise_rel(rhs, orhs) < TOL_K
+
+
+def test_krylov_vs_oracle(mpf, orc, gpu):
+    n = 8
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
+    X, conn, P = _hex_problem(mpf, n, m)
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    for Q in (P, O):
+        Q.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    vals, rhs = P.system()
+    x_ref, st = O.direct_solve(vals, rhs)
+    assert st == 0
+    for ksp, pc in [(mpf.KSP_CG, mpf.PC_BJACOBI3), (mpf.KSP_CG, mpf.PC_JACOBI), (mpf.KSP_GMRES, mpf.PC_BJACOBI3)]:
+        r = P.factorise_and_solve(ksp=ksp, pc=pc, rtol=1e-11, maxit=5000, restart=40)
+        assert r.converged, (ksp, pc, r.iterations, r.relres)
+        du = P.solution()
+        assert normwise_rel(du, x_ref) < 1e-8
+        assert np.linalg.norm(O.spmv(vals, du) - rhs) / np.linalg.norm(rhs) < 2e-11
+
+
+def test_gmres_mixed_vs_oracle(mpf, orc, gpu):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    n = 3
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
+    apply_bcs(P, tet_compress_bcs(n))
+    P.prepare_matrix_pattern()
+    P.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    vals, rhs = P.system()
+    pat = P.pattern()
+    A = sp.csr_matrix((vals, pat.colidx, pat.rowptr), shape=(len(rhs),) * 2)
+    x_ref = spl.spsolve(A.tocsc(), rhs)
+    r = P.factorise_and_solve(ksp=mpf.KSP_GMRES, pc=mpf.PC_BJACOBI3, rtol=1e-10, maxit=20000, restart=100)
+    assert r.converged, (r.iterations, r.relres)
+    du = P.solution()
+    assert np.linalg.norm(A @ du - rhs) / np.linalg.norm(rhs) < 1e-9
+    assert normwise_rel(du, x_ref) < 1e-6
+
+
+def test_newton_cfg1_vs_oracle(mpf, orc, gpu):
+    """10^3 Hex8 cube, 10% stretch over 10 steps, converged to ||R||/||R0|| < 1e-9 each step."""
+    n = 10
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=mpf.TANGENT_CONSISTENT)
+    X, conn, P = _hex_problem(mpf, n, m)
+    opts = P.newton_opts(nsteps=10, dt=0.1, itmax=25, rtol=1e-9, ksp=mpf.KSP_CG, pc=mpf.PC_BJACOBI3,
+                         lin_rtol=1e-12)
+    iters, rep = P.solve_with_newton_raphson(opts)
+    assert rep.steps_done == 10 and np.all(iters <= 6), iters
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    st, oit, rel = O.newton(10, 0.1, solver=0, rtol=1e-9, itmax=25)
+    assert st == 0
+    ud, _ = P.state()
+    uo, _ = O.state()
+    assert normwise_rel(ud, uo) < 1e-9
+    assert abs(ud[3 * M_face1(mpf, n)].mean() - 0.1) < 1e-12
+
+
+def M_face1(mpf, n):
+    return mpf.face_nodes(mpf.HEX8, n, n, n, 1)
+
+
+def test_newton_tet_one_step_vs_oracle(mpf, orc, gpu):
+    n = 3
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    m = mpf.matl(1.0, 1e-3, 3, mixed=True)
+    P = mpf.Problem(mpf.TET10P1, X, conn, m)
+    O = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    for Q in (P, O):
+        apply_bcs(Q, tet_compress_bcs(n))
+        Q.prepare_matrix_pattern()
+    opts = P.newton_opts(nsteps=2, dt=0.5, itmax=20, rtol=1e-9, ksp=mpf.KSP_GMRES, pc=mpf.PC_BJACOBI3,
+                         lin_rtol=1e-12, maxit=50000, restart=100)
+    iters, rep = P.solve_with_newton_raphson(opts)
+    assert rep.steps_done == 2
+    st, oit, _ = O.newton(2, 0.5, solver=2, precond=2, lin_rtol=1e-12, rtol=1e-9, itmax=20)
+    assert st == 0
+    ud, pd = P.state()
+    uo, po = O.state()
+    assert normwise_rel(ud, uo) < 1e-9
+    assert normwise_rel(pd, po) < 1e-8
