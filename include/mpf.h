@@ -204,8 +204,8 @@ mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o,
 /* ------------------------------------------------------------------------------------------ */
 /* Timing helpers: average device time (ms) of one assembly (element kernels + scatter, no       */
 /* pattern build) over reps launches, measured with HIP events on the problem's stream.          */
-mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, double* ms_per_assembly,
-                             double* ms_scatter_kernels);
+mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, int32_t first_iter, double lam,
+                             double* ms_per_assembly, double* ms_scatter_kernels);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Synthetic meshes (seeded, SplitMix64 with u = (x >> 11) * 2^-53).  Output buffers are host     */

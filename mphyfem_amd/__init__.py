@@ -101,7 +101,7 @@ SIGNATURES = [
     ("mpf_update", C.c_int, [_P, C.c_int32]),
     ("mpf_newton_solve", C.c_int, [_P, C.POINTER(NewtonOpts), _I32P, C.POINTER(NewtonReport)]),
     ("mpf_newton_step", C.c_int, [_P, C.c_double, C.POINTER(NewtonOpts), _DP, _I32P, C.POINTER(NewtonReport)]),
-    ("mpf_time_assembly", C.c_int, [_P, C.c_int32, _DP, _DP]),
+    ("mpf_time_assembly", C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _DP, _DP]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                             C.c_uint64, _DP, _I32P]),
@@ -120,6 +120,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m mphyfem_amd.build`")
+        # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 with the same sonames as
+        # /opt/rocm's.  Whichever loads first serves the whole process; load torch's first so
+        # one HIP runtime owns the device and torch tensors can be passed to libmpf.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, res, args in SIGNATURES:
             f = getattr(L, name)
@@ -403,7 +410,9 @@ class Problem:
         _check(st, "mpf_newton_step")
         return it.value, rep
 
-    def time_assembly(self, reps: int = 20):
+    def time_assembly(self, reps: int = 20, first_iter: bool = True, lam: float = 1.0):
+        """(ms per full assembly, ms of the element-kernel launches per assembly), HIP events."""
         a, s = C.c_double(), C.c_double()
-        _check(lib().mpf_time_assembly(self._h, reps, C.byref(a), C.byref(s)), "mpf_time_assembly")
+        _check(lib().mpf_time_assembly(self._h, reps, int(first_iter), lam, C.byref(a), C.byref(s)),
+               "mpf_time_assembly")
         return a.value, s.value

@@ -698,16 +698,17 @@ mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* i
   return MPF_OK;
 }
 
-mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, double* ms_per_assembly, double* ms_scatter_kernels) {
+mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, int32_t first_iter, double lam, double* ms_per_assembly,
+                             double* ms_scatter_kernels) {
   if (!p || reps <= 0) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   hipStream_t s = p->stream;
   hipEvent_t e0, e1;
   MPF_HIP(hipEventCreate(&e0));
   MPF_HIP(hipEventCreate(&e1));
-  MPF_TRY(enqueue_assembly(p, false, 1.0, 1));  // warm-up
+  MPF_TRY(enqueue_assembly(p, first_iter != 0, lam, 1));  // warm-up
   MPF_HIP(hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r) MPF_TRY(enqueue_assembly(p, false, 1.0, 1));
+  for (int r = 0; r < reps; ++r) MPF_TRY(enqueue_assembly(p, first_iter != 0, lam, 1));
   MPF_HIP(hipEventRecord(e1, s));
   MPF_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -719,7 +720,8 @@ mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, double* ms_per_assemb
     ScatterArgs sc{};
     const PatternBufs& P = p->pat;
     sc.slot = P.slot.ptr; sc.nodeinfo = P.nodeinfo.ptr; sc.nda = P.nda.ptr; sc.idp = P.idp.ptr;
-    sc.dispDOF = P.dispDOF; sc.vals = p->vals.ptr; sc.rhs = p->rhs.ptr; sc.dispApplied = nullptr; sc.scatterK = 1;
+    sc.dispDOF = P.dispDOF; sc.vals = p->vals.ptr; sc.rhs = p->rhs.ptr; sc.scatterK = 1;
+    sc.dispApplied = first_iter ? p->dispApplied.ptr : nullptr;
     MPF_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r)
       for (int c = 0; c < P.ncolors; ++c) {
@@ -732,6 +734,9 @@ mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, double* ms_per_assemb
     MPF_HIP(hipEventSynchronize(e1));
     MPF_HIP(hipEventElapsedTime(&ms, e0, e1));
     *ms_scatter_kernels = ms / reps;
+    // leave a complete system behind (the kernel-only sweeps accumulate into vals / rhs)
+    MPF_TRY(enqueue_assembly(p, first_iter != 0, lam, 1));
+    MPF_HIP(hipStreamSynchronize(s));
   }
   MPF_HIP(hipEventDestroy(e0));
   MPF_HIP(hipEventDestroy(e1));
