@@ -1,14 +1,17 @@
-"""Benchmark: Tet10/P1 mixed Neo-Hookean tangent + residual assembly into FP64 CSR on one MI355X.
+"""Benchmark: Neo-Hookean tangent + residual assembly into FP64 CSR on one MI355X.
 
-Workload (the headline configuration): 60^3 cubes split into 6 Kuhn tets each = 1 296 000 Tet10
-elements (1 771 561 nodes), P1 pressure on the 226 981 vertices, interior vertices jittered
-+-0.1h (seed 2), mu = 1, K = 1e3 (Kinv = 1e-3, Utype 3), 4-point rule; z=0 fixed, z=1 given
-UZ = -0.05 (5% compression) applied in one load step.  One "step" = one full Newton tangent +
-residual assembly (zeroMtx, every colour of the atomic-free element scatter, Dirichlet lift) into
-the CSR matrix, inputs resident in HBM.  After the timed steps the block-Jacobi GMRES solve of
-that system is run once and reported beside it.
+Default workload (the headline configuration, Tet10/P1): 60^3 cubes split into 6 Kuhn tets each =
+1 296 000 Tet10 elements (1 771 561 nodes), P1 pressure on the 226 981 vertices, interior
+vertices jittered +-0.1h (seed 2), mu = 1, K = 1e3 (Kinv = 1e-3, Utype 3), 4-point rule; z=0 fixed,
+z=1 given UZ = -0.05 (5% compression) applied in one load step.  One "step" = one full Newton
+tangent + residual assembly (zeroMtx, every colour of the atomic-free element scatter, Dirichlet
+lift) into the CSR matrix, inputs resident in HBM.  After the timed steps the block-Jacobi GMRES
+solve of that system is run once and reported beside it.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+--workload hex8: the 128^3 Hex8 block (2 097 152 elements, nu = 0.49, bottom fixed, top UZ = -0.3),
+block-Jacobi PCG beside it.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload tet10|hex8]
 
 Prints one JSON line.  Only N = 1 is supported (the path is single-GPU).
 """
@@ -26,7 +29,9 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-BYTES_PER_ELEM_TET10P1 = 24232   # npe*(4+24+24) + 1156*(8+8+4) + 34*16 + 4*12
+# Algorithmic bytes per element (element-scatter model): npe*(4 conn + 24 X + 24 u)
+# + nK*(8 read + 8 write CSR value + 4 CSR-slot index) + nR*16 (+ nP*12 pressure gather/slot).
+BYTES_PER_ELEM = {"tet10": 24232, "hex8": 12320}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -34,21 +39,25 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_problem(M, n: int):
-    X, conn = M.mesh_tet10(n, L=1.0, jitter=0.1, seed=2)
-    P = M.Problem(M.TET10P1, X, conn, M.matl(1.0, 1e-3, 3, mixed=True))
-    P.add_dirichlet(M.face_nodes(M.TET10P1, n, n, n, 4), -1)
-    P.add_dirichlet(M.face_nodes(M.TET10P1, n, n, n, 5), 2, -0.05)
-    return X, conn, P
+def build_problem(M, n: int, workload: str):
+    X, conn, _, bcs = build_problem_host(M, n, workload)
+    if workload == "hex8":
+        P = M.Problem(M.HEX8, X, conn, M.matl_from_E_nu(1e3, 0.49, tangent=M.TANGENT_CONSISTENT))
+    else:
+        P = M.Problem(M.TET10P1, X, conn, M.matl(1.0, 1e-3, 3, mixed=True))
+    for nodes, dof, val in bcs:
+        P.add_dirichlet(nodes, dof, val)
+    return X, conn, P, bcs
 
 
-def cpu_baseline(M, n_sample: int, budget_s: float):
-    """Oracle (serial C++ restatement, -O3) assembling a bounded sample of the same element family."""
+def cpu_baseline(M, workload: str, budget_s: float):
+    """Oracle (serial C++ restatement, -O3, one core) assembling a bounded sample of the same workload."""
     from oracle import oracle as O
-    X, conn = M.mesh_tet10(n_sample, L=1.0, jitter=0.1, seed=2)
-    Q = O.Problem(10, X, conn, 1.0, 1e-3, 3)
-    Q.add_dirichlet(M.face_nodes(M.TET10P1, n_sample, n_sample, n_sample, 4), -1)
-    Q.add_dirichlet(M.face_nodes(M.TET10P1, n_sample, n_sample, n_sample, 5), 2, -0.05)
+    n = 16 if workload == "tet10" else 32
+    X, conn, _, bcs = build_problem_host(M, n, workload)
+    Q = O.Problem(10 if workload == "tet10" else 8, X, conn, *material_host(M, workload))
+    for nodes, dof, val in bcs:
+        Q.add_dirichlet(nodes, dof, val)
     Q.prepare_matrix_pattern()
     Q.set_boundary_conditions(1.0)
     d, p, nnz = Q.sizes()
@@ -60,10 +69,27 @@ def cpu_baseline(M, n_sample: int, budget_s: float):
         Q.assemble_range(0, len(conn), vals, rhs)
         done += len(conn)
     dt = time.perf_counter() - t0
-    return {"value": done / dt / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++, g++ -O3) assembling {done} Tet10/P1 elements: repeated sweeps of a "
-                      f"{n_sample}^3x6 = {len(conn)}-element mesh of the same generator, material and load, "
-                      f"{dt:.1f} s"}
+    shape = f"{n}^3x6 = {len(conn)}-element Tet10/P1" if workload == "tet10" else f"{n}^3 = {len(conn)}-element Hex8"
+    return {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (serial C++, g++ -O3) assembling {done} elements: repeated full sweeps of a "
+                      f"{shape} mesh of the same generator, material and boundary conditions, {dt:.1f} s"}
+
+
+def build_problem_host(M, n, workload):
+    if workload == "hex8":
+        X, conn = M.mesh_hex(n, n, n, jitter=0.0, seed=5)
+        bcs = [(M.face_nodes(M.HEX8, n, n, n, 4), -1, None), (M.face_nodes(M.HEX8, n, n, n, 5), 2, -0.3)]
+    else:
+        X, conn = M.mesh_tet10(n, L=1.0, jitter=0.1, seed=2)
+        bcs = [(M.face_nodes(M.TET10P1, n, n, n, 4), -1, None), (M.face_nodes(M.TET10P1, n, n, n, 5), 2, -0.05)]
+    return X, conn, None, bcs
+
+
+def material_host(M, workload):
+    if workload == "hex8":
+        m = M.matl_from_E_nu(1e3, 0.49)
+        return m.mu, m.Kinv, 3
+    return 1.0, 1e-3, 3
 
 
 def read_traffic(name: str):
@@ -81,11 +107,14 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=60, help="cubes per edge (60 -> 1 296 000 Tet10)")
-    ap.add_argument("--gmres-maxit", type=int, default=300)
-    ap.add_argument("--no-gmres", action="store_true")
+    ap.add_argument("--workload", choices=["tet10", "hex8"], default="tet10")
+    ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
+    ap.add_argument("--krylov-maxit", type=int, default=300)
+    ap.add_argument("--no-krylov", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--assembly", choices=["gather", "scatter"], default="gather",
+                    help="row-owner gather assembly or coloured element scatter")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this path runs on one GPU only (no multi-GPU code); run with --gpus 1", file=sys.stderr)
@@ -99,11 +128,13 @@ def main(argv=None) -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
     torch.cuda.init()
-    n = a.n
-    log(f"mesh {n}^3 x 6 Tet10")
-    X, conn, P = build_problem(M, n)
+    wl = a.workload
+    n = a.n or (60 if wl == "tet10" else 128)
+    log(f"workload {wl}, n = {n}")
+    X, conn, P, _ = build_problem(M, n, wl)
     nElem = len(conn)
     t0 = time.perf_counter()
+    P.set_assembly_mode(M.ASSEMBLY_GATHER if a.assembly == "gather" else M.ASSEMBLY_COLORED_SCATTER)
     P.prepare_matrix_pattern()
     t_pattern = time.perf_counter() - t0
     dispDOF, presDOF, nnz, ncolors = P.sizes()
@@ -122,31 +153,45 @@ def main(argv=None) -> int:
     log(f"assembly {ms_step:.3f} ms/step -> {value:.1f} Melem/s")
     # live HIP-event timing of the element-kernel launches on the problem's stream
     ms_all, ms_kern = P.time_assembly(max(5, a.steps), first_iter=True, lam=1.0)
-    achieved = BYTES_PER_ELEM_TET10P1 * nElem / (ms_kern * 1e-3) / 1e9
-    traffic = read_traffic("tet10_scatter_traffic.json")
+    bpe = BYTES_PER_ELEM[wl]
+    achieved = bpe * nElem / (ms_kern * 1e-3) / 1e9
+    if a.assembly == "gather":
+        kname = "k_stage + k_rows (row-owner gather, 2 launches"
+    else:
+        kname = ("tet10_kernel<4,false>" if wl == "tet10" else "hex8_kernel<8,false>") + f" ({ncolors} colour launches"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "tet10_kernel<EPW,false> (all colour launches of one assembly)",
-                "kernel_ms_per_assembly": round(ms_kern, 4), "launches_per_assembly": ncolors,
-                "bytes_per_elem": BYTES_PER_ELEM_TET10P1}
-    out = {"metric": "Melem/s assembly (Tet10/P1 tangent+residual into FP64 CSR)", "value": round(value, 2),
-           "unit": "Melem/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"Tet10/P1 mixed Neo-Hookean, {n}^3x6 Kuhn tets ({nElem} elements), "
-                                  f"one tangent+residual assembly at 5% compression",
-                      "elements": nElem, "dofs": dispDOF + presDOF, "nnz": nnz, "colours": ncolors,
-                      "pattern_seconds": round(t_pattern, 3)},
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(f"{wl}_scatter_traffic.json"),
+                "kernel": f"{kname} per assembly)", "assembly": a.assembly,
+                "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe}
+    if wl == "tet10":
+        metric = "Melem/s assembly (Tet10/P1 tangent+residual into FP64 CSR)"
+        desc = f"Tet10/P1 mixed Neo-Hookean, {n}^3x6 Kuhn tets ({nElem} elements), one tangent+residual " \
+               f"assembly at 5% compression"
+    else:
+        metric = "Melem/s assembly (Hex8 tangent+residual into FP64 CSR)"
+        desc = f"Hex8 Neo-Hookean nu=0.49, {n}^3 block ({nElem} elements), one tangent+residual assembly"
+    out = {"metric": metric, "value": round(value, 2), "unit": "Melem/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": desc, "elements": nElem, "dofs": dispDOF + presDOF, "nnz": nnz,
+                      "colours": ncolors, "pattern_seconds": round(t_pattern, 3)},
            "roofline": roofline}
-    if not a.no_gmres:
-        r = P.factorise_and_solve(ksp=M.KSP_GMRES, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.gmres_maxit, restart=50)
-        out["gmres"] = {"pc": "block-Jacobi (3x3 node blocks, 1x1 pressure)", "restart": 50,
-                        "iterations": r.iterations, "relres": r.relres, "converged": bool(r.converged),
-                        "seconds": round(r.seconds, 3),
-                        "ms_per_iteration": round(r.seconds / max(r.iterations, 1) * 1e3, 3)}
-        log(f"gmres {r.iterations} its relres {r.relres:.2e} {r.seconds:.2f} s")
+    if not a.no_krylov:
+        if wl == "tet10":
+            r = P.factorise_and_solve(ksp=M.KSP_GMRES, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit, restart=50)
+            name = "gmres"
+            pc = "block-Jacobi (3x3 node blocks, 1x1 pressure), GMRES(50)"
+        else:
+            r = P.factorise_and_solve(ksp=M.KSP_CG, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit)
+            name = "pcg"
+            pc = "block-Jacobi (3x3 node blocks)"
+        out[name] = {"pc": pc, "iterations": r.iterations, "relres": r.relres, "converged": bool(r.converged),
+                     "seconds": round(r.seconds, 4),
+                     "ms_per_iteration": round(r.seconds / max(r.iterations, 1) * 1e3, 4)}
+        log(f"{name} {r.iterations} its relres {r.relres:.2e} {r.seconds:.2f} s")
     if not a.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(M, 16, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(M, wl, a.cpu_seconds)
     P.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -116,6 +116,17 @@ mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, in
 /* (femSolidmechanics1.cpp:922-1119; BasisFunctionsLagrange.cpp:1688-1732).  faces [nf][4].     */
 mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1);
 
+/* Assembly algorithm used by mpf_assemble (same K and R either way):                           */
+/*   MPF_ASSEMBLY_GATHER (default): per-element Gauss-point staging, then one wavefront per node    */
+/*     sums its rows' contributions in element order and writes each CSR value once.             */
+/*   MPF_ASSEMBLY_COLORED_SCATTER: elements of one colour per launch add their K_e/R_e blocks into  */
+/*     CSR through the per-element slot map (no two elements of a launch share a node).           */
+/* The gather kernel handles up to 64 elements and 128 coupled nodes per node; beyond that the   */
+/* problem switches itself to the coloured scatter.                                              */
+enum { MPF_ASSEMBLY_GATHER = 0, MPF_ASSEMBLY_COLORED_SCATTER = 1 };
+mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode);
+mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode);
+
 /* prepareMatrixPattern (femSolidmechanics2.cpp:121-478) on the device: DOF numbering, pressure   */
 /* dofs, CSR pattern (rowptr/colidx, int32, sorted), element colouring, per-element CSR slots.   */
 mpf_status mpf_prepare_pattern(mpf_problem* p);

@@ -24,6 +24,7 @@ HEX8 = 8
 TET10P1 = 10
 KSP_CG, KSP_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
+ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER = 0, 1
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NEG_JACOBIAN, ERR_STATE, ERR_NOT_CONVERGED, ERR_DIVERGED, ERR_NO_DEVICE = range(1, 8)
@@ -102,6 +103,8 @@ SIGNATURES = [
     ("mpf_newton_solve", C.c_int, [_P, C.POINTER(NewtonOpts), _I32P, C.POINTER(NewtonReport)]),
     ("mpf_newton_step", C.c_int, [_P, C.c_double, C.POINTER(NewtonOpts), _DP, _I32P, C.POINTER(NewtonReport)]),
     ("mpf_time_assembly", C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _DP, _DP]),
+    ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
+    ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                             C.c_uint64, _DP, _I32P]),
@@ -409,6 +412,15 @@ class Problem:
         st = lib().mpf_newton_step(self._h, lam, C.byref(opts), _dp(g), C.byref(it), C.byref(rep))
         _check(st, "mpf_newton_step")
         return it.value, rep
+
+    def set_assembly_mode(self, mode: int):
+        """ASSEMBLY_GATHER (row-owner, default) or ASSEMBLY_COLORED_SCATTER."""
+        _check(lib().mpf_set_assembly_mode(self._h, mode), "mpf_set_assembly_mode")
+
+    def assembly_mode(self) -> int:
+        v = C.c_int32()
+        _check(lib().mpf_get_assembly_mode(self._h, C.byref(v)), "mpf_get_assembly_mode")
+        return v.value
 
     def time_assembly(self, reps: int = 20, first_iter: bool = True, lam: float = 1.0):
         """(ms per full assembly, ms of the element-kernel launches per assembly), HIP events."""

@@ -1,174 +1,48 @@
-// elem.hip -- FP64 Neo-Hookean element kernels for gfx950 (Hex8 displacement, Tet10/P1 mixed).
+// elem.hip -- element operators (dense K_e / R_e per element) and the coloured-scatter assembly.
 //
 // Reference path (chennachaos/mphyfem, src/):
 //   Elem_Solid_3D_Disp::calcStiffnessAndResidual      Elem_Solid_3D_Disp.cpp:196-374
 //   ElementBase::calcStiffnessAndResidualMixed3D      ElementBase2.cpp:278-552
-//   GeomDataLagrange::computeBasisFunctions3D          GeomDataLagrange.cpp:349-458
-//   computeStressAndTangent_NeoHooke                   utilitiesmaterialHyperelastic.cpp:186-295
 //   applyDirichletBCs / applyDirichletBCs2field        ElementBase.cpp:567-594, 717-782
 //   SolverPetsc::assembleMatrixAndVector(2field)       SolverPetsc.cpp:357-378, 434-502
 //
-// Work split.  One 64-lane wavefront owns EPW elements.
-//   Phase 1: lane = (element slot k, Gauss point gp).  The lane builds dN/dxi at gp, the
-//            reference-configuration Jacobian, F = I + sum u_a (x) grad_X N_a, the current
-//            Jacobian (x = X + u), grad_x N for all nodes, and the Neo-Hookean state, and
-//            stages g_a = grad_x N_a, (b g)_a and five tangent scalars in LDS.  Residual
-//            contributions are reduced over Gauss points with lane shuffles.
-//   Phase 2: lane = node pair (a,b) (Tet10: two passes plus u-p lanes).  The 3x3 block
-//            K_ab = sum_gp [ s1 g^a_k g^b_i + (s2 g^a_i + s3 (bg^a)_i) g^b_k
-//                            + s3 g^a_i (bg^b)_k + delta_ik (s4 bg^a + s5 g^a).g^b ]
-//            is the closed-form contraction  sum_{m,n} g^a_m C[(i,m),(k,n)] g^b_n  of the
-//            reference's 9x9 spatial tangent (same terms, fewer flops), with
-//              disp : s1 = dv (f1 - U'), s2 = dv (f3 + J U'' + U'), s3 = dv f2, s4 = dv f4, s5 = -dv U'
-//                     (s5 = 0 for the consistent tangent, which adds the volumetric stress term)
-//              mixed: s1 = dv (f1 - p),  s2 = dv (f3 + p),          s3 = dv f2, s4 = dv f4, s5 = 0
-//            where f1 = mu J^-5/3 I1/3, f2 = -2/3 mu J^-5/3, f3 = 2/9 mu J^-5/3 I1, f4 = mu J^-5/3.
-//            The block is either written densely (element operator) or added into the CSR
-//            values through the per-element slot map (one colour per launch: no two elements
-//            of a launch share a node, so the FP64 read-modify-writes never race).
+// One 64-lane wavefront (one workgroup) owns EPW elements (of one colour when scattering).
+//   Phase 1: lane = (element slot k, Gauss point gp) runs dev::element_point (fem_device.hpp) and
+//            stages grad_x N, b grad_x N and the Gauss-point scalars in LDS.
+//   Phase 2: lane = node pair (a,b) (Tet10: 100 pairs as two blocks per lane, plus u-p lanes)
+//            forms its 3x3 block with dev::tangent_block and either writes it densely (element
+//            operator) or adds it into the CSR values through the per-element slot map.  One colour
+//            per launch: no two elements of a launch share a node, so the FP64 read-modify-writes
+//            never race.  Each lane issues all loads of its blocks before any store.
 
-#include "mpf_internal.hpp"
+#include "fem_device.hpp"
 
 namespace mpf {
 
 namespace {
 
-constexpr double kGP2 = 0.577350269189626;  // QuadratureUtil.cpp:24-25
+using namespace dev;
 
-__device__ __forceinline__ double det3(const double A[3][3]) {
-  return A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
-         A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
-}
-
-// Inverse by the adjugate; returns det.
-__device__ __forceinline__ double inv3(const double A[3][3], double R[3][3]) {
-  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
-  const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
-  const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
-  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
-  const double id = 1.0 / det;
-  R[0][0] = c00 * id;
-  R[1][0] = c01 * id;
-  R[2][0] = c02 * id;
-  R[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
-  R[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
-  R[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
-  R[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
-  R[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
-  R[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
-  return det;
-}
-
-// volumetricfunctions_getFirstDerivative / getSecondDerivative (utilitiesmaterialHyperelastic.cpp:45-140)
-__device__ __forceinline__ void vol_derivs(int type, double K, double J, double& d1, double& d2) {
-  switch (type) {
-    case 1: d1 = 0.1 * K * (J * J * J * J - 1.0 / (J * J * J * J * J * J));
-            d2 = 0.2 * K * (2.0 * J * J * J + 3.0 / (J * J * J * J * J * J * J)); break;
-    case 2: d1 = 0.125 * K * (J * J - 1.0 / J / J) * (J + 1.0 / J / J / J);
-            d2 = 0.125 * K * (3.0 * J * J + 5.0 / (J * J * J * J * J * J)); break;
-    case 3: d1 = K * (J - 1.0); d2 = K; break;
-    case 4: d1 = 0.5 * K * (J - 1.0 / J); d2 = 0.5 * K * (1.0 + 1.0 / J / J); break;
-    case 5: d1 = 0.5 * K * (J - 1.0 + log(J) / J); d2 = 0.5 * K * (J * J + 1.0 - log(J)) / J / J; break;
-    case 6: d1 = K * log(J); d2 = K / log(J); break;
-    case 7: d1 = K * (1.0 - 1.0 / J); d2 = K / J / J; break;
-    case 8: d1 = K * log(J) / J; d2 = K * (1.0 - log(J)) / J / J; break;
-    default: d1 = 0.0; d2 = 0.0; break;
-  }
-}
-
-__device__ void flag_bad(int32_t* bad, int32_t e) {
-  if (!bad) return;
-  const int32_t v = e + 1;
-  int32_t old = __hip_atomic_load(bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  while (old == 0 || v < old) {
-    const int32_t prev = atomicCAS(bad, old, v);
-    if (prev == old) break;
-    old = prev;
-  }
-}
-
-// Neo-Hookean state at one Gauss point: Cauchy stress (symmetric, 6 stored) and the five
-// tangent scalars of the closed-form contraction (multiplied by dv).  See file header.
-struct NHPoint {
-  double sig[3][3];
-  double s1, s2, s3, s4, s5;
-  double b[3][3];
-};
-
-__device__ __forceinline__ void neohooke_point(const double F[3][3], double J, double mu, double K, int Utype,
-                                               bool mixed, double pres, double dv, bool consistent, NHPoint& P) {
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) P.b[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
-  const double r1d3 = 1.0 / 3.0, r2d3 = 2.0 / 3.0;
-  const double Jm2d3 = pow(J, -r2d3);
-  const double Jm5d3 = Jm2d3 / J;
-  const double Ib = P.b[0][0] + P.b[1][1] + P.b[2][2];
-  double U1 = 0.0, U2 = 0.0;
-  if (!mixed) vol_derivs(Utype, K, J, U1, U2);
-  const double fs1 = mu * Jm5d3;
-  const double fs2 = -fs1 * r1d3 * Ib + (mixed ? pres : U1);
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) P.sig[i][j] = fs1 * P.b[i][j] + (i == j ? fs2 : 0.0);
-  const double f1 = r1d3 * mu * Jm5d3 * Ib;
-  const double f2 = -r2d3 * mu * Jm5d3;
-  const double f3 = r2d3 * r1d3 * mu * Jm5d3 * Ib;
-  const double f4 = mu * Jm5d3;
-  if (mixed) {
-    P.s1 = dv * (f1 - pres);
-    P.s2 = dv * (f3 + pres);
-    P.s5 = 0.0;
-  } else {
-    P.s1 = dv * (f1 - U1);
-    P.s2 = dv * (f3 + (J * U2 + U1));
-    P.s5 = consistent ? 0.0 : -dv * U1;
-  }
-  P.s3 = dv * f2;
-  P.s4 = dv * f4;
-}
-
-// dN/dxi of the 8 Hex8 nodes at Gauss point gp (LagrangeBasisFunsHexa, BasisFunctionsLagrange.cpp:1202-1279;
-// getGaussPointsHexa ordering QuadratureUtil.cpp:453-492).
-__device__ __forceinline__ void hex8_dN(int gp, double d[3][8]) {
-  const int p0 = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? 1 : 0;
-  const int p1 = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? 1 : 0;
-  const int p2 = (gp >= 4) ? 1 : 0;
-  const double x1 = p0 ? kGP2 : -kGP2, x2 = p1 ? kGP2 : -kGP2, x3 = p2 ? kGP2 : -kGP2;
-  const double Nu[2] = {0.5 * (1.0 - x1), 0.5 * (1.0 + x1)};
-  const double Nv[2] = {0.5 * (1.0 - x2), 0.5 * (1.0 + x2)};
-  const double Nw[2] = {0.5 * (1.0 - x3), 0.5 * (1.0 + x3)};
-  const double dn[2] = {-0.5, 0.5};
-  const int iu[8] = {0, 1, 1, 0, 0, 1, 1, 0}, iv[8] = {0, 0, 1, 1, 0, 0, 1, 1}, iw[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+// CSR slots of a 3x3 node block: rows of node a (free mask ma, row length La) starting at slot s0,
+// columns of node b (free mask mb).  -1 where a dof is prescribed.
+__device__ __forceinline__ void block_slots(int s0, int ma, int mb, int La, int addr[9]) {
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    d[0][a] = dn[iu[a]] * Nv[iv[a]] * Nw[iw[a]];
-    d[1][a] = Nu[iu[a]] * dn[iv[a]] * Nw[iw[a]];
-    d[2][a] = Nu[iu[a]] * Nv[iv[a]] * dn[iw[a]];
-  }
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      addr[3 * i + kk] = (s0 >= 0 && ((ma >> i) & 1) && ((mb >> kk) & 1)) ? s0 + rank3(ma, i) * La + rank3(mb, kk) : -1;
 }
 
-// Tet Gauss point gp of the 4-point rule (QuadratureUtil.cpp:690-712) and Tet10 / Tet4 bases
-// (BasisFunctionsLagrange.cpp:870-956).
-__device__ __forceinline__ void tet_gp(int gp, double& x1, double& x2, double& x3) {
-  const double a = 0.1381966011250105, b = 0.5854101966249685;
-  x1 = (gp == 1) ? b : a;
-  x2 = (gp == 2) ? b : a;
-  x3 = (gp == 3) ? b : a;
+// V[addr[t]] += v[t] for N distinct slots: every load is issued before the first store.
+template <int N>
+__device__ __forceinline__ void rmw_add(double* __restrict__ V, const int addr[N], const double v[N]) {
+  double old[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) old[t] = addr[t] >= 0 ? V[addr[t]] : 0.0;
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if (addr[t] >= 0) V[addr[t]] = old[t] + v[t];
 }
-
-__device__ __forceinline__ void tet10_dN(double xi1, double xi2, double xi3, double d1[10], double d2[10],
-                                         double d3[10]) {
-  const double xi4 = 1.0 - xi1 - xi2 - xi3;
-  d1[0] = -4 * xi4 + 1; d1[1] = 4 * xi1 - 1; d1[2] = 0; d1[3] = 0; d1[4] = 4 * (xi4 - xi1);
-  d1[5] = 4 * xi2; d1[6] = -4 * xi2; d1[7] = -4 * xi3; d1[8] = 0; d1[9] = 4 * xi3;
-  d2[0] = -4 * xi4 + 1; d2[1] = 0; d2[2] = 4 * xi2 - 1; d2[3] = 0; d2[4] = -4 * xi1;
-  d2[5] = 4 * xi1; d2[6] = 4 * (xi4 - xi2); d2[7] = -4 * xi3; d2[8] = 4 * xi3; d2[9] = 0;
-  d3[0] = -4 * xi4 + 1; d3[1] = 0; d3[2] = 0; d3[3] = 4 * xi3 - 1; d3[4] = -4 * xi1;
-  d3[5] = 0; d3[6] = -4 * xi2; d3[7] = 4 * (xi4 - xi3); d3[8] = 4 * xi2; d3[9] = 4 * xi1;
-}
-
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-
-__device__ __forceinline__ int rank3(int mask, int i) { return __popc(mask & ((1 << i) - 1)); }
 
 // ============================================================================================
 // Hex8 displacement element
@@ -180,193 +54,121 @@ struct Hex8Smem {
   double G[EPW][8][8][6];   // [k][gp][a]: g_a (3), (b g)_a (3)
   double S[EPW][8][5];      // [k][gp]: s1..s5
   double R[EPW][8][3];      // [k][a]: element residual
+  double dN[8][3][8];       // [gp][r][a]: dN_a/dxi_r at the Gauss points
 };
 
 template <int EPW, bool DENSE>
 __global__ void __launch_bounds__(64) hex8_kernel(ElemArgs A, int32_t first, int32_t count, double* __restrict__ Ke,
                                                   double* __restrict__ Re, ScatterArgs sc) {
+  static_assert(EPW == 8, "one wave holds 8 elements x 8 Gauss points");
   __shared__ Hex8Smem<EPW> sm;
   const int lane = threadIdx.x;
   const int base_idx = blockIdx.x * EPW;
-  const double K = 1.0 / A.Kinv;
-
-  // ---- stage X and u of the EPW elements (lane loads node (k, a) triples) ----
+  if (lane < 8) {
+    double d[3][8];
+    hex8_dN(lane, d);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int a = 0; a < 8; ++a) sm.dN[lane][r][a] = d[r][a];
+  }
+  // X, u of the EPW elements (idle slots of a partial block repeat the block's first element)
   for (int t = lane; t < EPW * 8; t += 64) {
     const int k = t >> 3, a = t & 7;
-    const int idx = base_idx + k;
-    double x0 = 0, x1 = 0, x2 = 0, u0 = 0, u1 = 0, u2 = 0;
-    if (idx < count) {
-      const int e = DENSE ? first + idx : sc.elems[idx];
-      const int nd = A.conn[(size_t)e * 8 + a];
-      x0 = A.X[3 * (size_t)nd]; x1 = A.X[3 * (size_t)nd + 1]; x2 = A.X[3 * (size_t)nd + 2];
-      u0 = A.u[3 * (size_t)nd]; u1 = A.u[3 * (size_t)nd + 1]; u2 = A.u[3 * (size_t)nd + 2];
-    }
-    sm.X[k][a][0] = x0; sm.X[k][a][1] = x1; sm.X[k][a][2] = x2;
-    sm.U[k][a][0] = u0; sm.U[k][a][1] = u1; sm.U[k][a][2] = u2;
+    const int idx = (base_idx + k < count) ? base_idx + k : base_idx;
+    const int e = DENSE ? first + idx : sc.elems[idx];
+    const size_t nd = (size_t)A.conn[(size_t)e * 8 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { sm.X[k][a][c] = A.X[3 * nd + c]; sm.U[k][a][c] = A.u[3 * nd + c]; }
   }
   __syncthreads();
-
-  // ---- phase 1: lane = (k, gp) ----
   {
     const int k = lane >> 3, gp = lane & 7;
-    const bool act = (k < EPW) && (base_idx + k < count);
-    double R[8][3];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) R[a][0] = R[a][1] = R[a][2] = 0.0;
-    if (k < EPW) {
-      double d[3][8];
-      hex8_dN(gp, d);
-      // reference Jacobian B(r,c) = sum_a dN_a/dxi_r X_a[c]
-      double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Bi[3][3];
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double xc = sm.X[k][a][c];
-          B[0][c] += xc * d[0][a]; B[1][c] += xc * d[1][a]; B[2][c] += xc * d[2][a];
-        }
-      inv3(B, Bi);
-      // F = I + sum_a u_a (x) grad_X N_a
-      double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        const double gx = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
-        const double gy = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
-        const double gz = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double ui = sm.U[k][a][i];
-          F[i][0] += ui * gx; F[i][1] += ui * gy; F[i][2] += ui * gz;
-        }
-      }
-      const double J = det3(F);
-      if (act && J < 0.0) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
-      // current Jacobian with x = X + u
-      double Bc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double xc = sm.X[k][a][c] + sm.U[k][a][c];
-          Bc[0][c] += xc * d[0][a]; Bc[1][c] += xc * d[1][a]; Bc[2][c] += xc * d[2][a];
-        }
-      const double Jac = inv3(Bc, Bi);
-      const double dv = 1.0 * Jac;  // Hex8 weights are 1
-      NHPoint P;
-      neohooke_point(F, J, A.mu, K, A.Utype, false, 0.0, dv, A.consistent != 0, P);
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        const double g0 = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
-        const double g1 = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
-        const double g2 = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-        double* G = sm.G[k][gp][a];
-        G[0] = g0; G[1] = g1; G[2] = g2;
-        G[3] = P.b[0][0] * g0 + P.b[0][1] * g1 + P.b[0][2] * g2;
-        G[4] = P.b[1][0] * g0 + P.b[1][1] * g1 + P.b[1][2] * g2;
-        G[5] = P.b[2][0] * g0 + P.b[2][1] * g1 + P.b[2][2] * g2;
-        const double b0 = dv * g0, b1 = dv * g1, b2 = dv * g2;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) R[a][i] = -(b0 * P.sig[i][0] + b1 * P.sig[i][1] + b2 * P.sig[i][2]);
-      }
-      double* S = sm.S[k][gp];
-      S[0] = P.s1; S[1] = P.s2; S[2] = P.s3; S[3] = P.s4; S[4] = P.s5;
-    }
-    // reduce R over the 8 Gauss points of element k (lanes k*8 .. k*8+7)
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        double v = R[a][i];
-        v += shfl_xor_d(v, 1);
-        v += shfl_xor_d(v, 2);
-        v += shfl_xor_d(v, 4);
-        R[a][i] = v;
-      }
-    if (k < EPW) {
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-        if (a == gp) { sm.R[k][a][0] = R[a][0]; sm.R[k][a][1] = R[a][1]; sm.R[k][a][2] = R[a][2]; }
-    }
+    const bool act = base_idx + k < count;
+    bool negJ = false;
+    element_point<8, 8, false>(sm.X[k], sm.U[k], nullptr, nullptr, sm.dN[gp], gp, 1.0, A.mu, 1.0 / A.Kinv, A.Utype,
+                               A.consistent != 0, true, &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
+    if (act && negJ) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
   }
   __syncthreads();
 
   // ---- phase 2: lane = (a, b) ----
   const int a = lane >> 3, b = lane & 7;
-  for (int k = 0; k < EPW; ++k) {
-    const int idx = base_idx + k;
-    if (idx >= count) break;  // uniform across the wave
-    double Kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll 2
-    for (int gp = 0; gp < 8; ++gp) {
-      const double* Ga = sm.G[k][gp][a];
-      const double* Gb = sm.G[k][gp][b];
-      const double* S = sm.S[k][gp];
-      const double s1 = S[0], s2 = S[1], s3 = S[2], s4 = S[3], s5 = S[4];
-      const double ga0 = Ga[0], ga1 = Ga[1], ga2 = Ga[2], ba0 = Ga[3], ba1 = Ga[4], ba2 = Ga[5];
-      const double gb0 = Gb[0], gb1 = Gb[1], gb2 = Gb[2], bb0 = Gb[3], bb1 = Gb[4], bb2 = Gb[5];
-      const double P0 = s1 * ga0, P1 = s1 * ga1, P2 = s1 * ga2;
-      const double Q0 = s2 * ga0 + s3 * ba0, Q1 = s2 * ga1 + s3 * ba1, Q2 = s2 * ga2 + s3 * ba2;
-      const double T0 = s3 * ga0, T1 = s3 * ga1, T2 = s3 * ga2;
-      const double y = (s4 * ba0 + s5 * ga0) * gb0 + (s4 * ba1 + s5 * ga1) * gb1 + (s4 * ba2 + s5 * ga2) * gb2;
-      const double Pv[3] = {P0, P1, P2}, Qv[3] = {Q0, Q1, Q2}, Tv[3] = {T0, T1, T2};
-      const double gbv[3] = {gb0, gb1, gb2}, bbv[3] = {bb0, bb1, bb2};
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk)
-          Kb[i][kk] += Pv[kk] * gbv[i] + Qv[i] * gbv[kk] + Tv[i] * bbv[kk] + (i == kk ? y : 0.0);
-    }
-    if (DENSE) {
+  if (DENSE) {
+    for (int k = 0; k < EPW; ++k) {
+      const int idx = base_idx + k;
+      if (idx >= count) break;
+      double Kb[9];
+      tangent_block<8, true>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 48, &sm.S[k][0][0], 5, Kb);
       double* Ko = Ke + (size_t)idx * 576;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) Ko[(3 * a + i) * 24 + 3 * b + kk] = Kb[i][kk];
+        for (int kk = 0; kk < 3; ++kk) Ko[(3 * a + i) * 24 + 3 * b + kk] = Kb[3 * i + kk];
       if (b == 0) {
         double* Ro = Re + (size_t)idx * 24;
         Ro[3 * a] = sm.R[k][a][0]; Ro[3 * a + 1] = sm.R[k][a][1]; Ro[3 * a + 2] = sm.R[k][a][2];
       }
-    } else {
-      const int e = sc.elems[idx];
+    }
+    return;
+  }
+  double* __restrict__ V = sc.vals;
+  double* __restrict__ rhs = sc.rhs;
+  // two elements per batch: 18 independent loads per lane
+  for (int k0 = 0; k0 < EPW; k0 += 2) {
+    if (base_idx + k0 >= count) break;  // uniform across the wave
+    int addr[18];
+    double val[18];
+    double cl[2][3];
+    int nar[2];
+    int mar[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + h;
+      const int idx = base_idx + k;
+      const bool live = idx < count;
+      double Kb[9];
+      tangent_block<8, true>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 48, &sm.S[k][0][0], 5, Kb);
+      const int e = live ? sc.elems[idx] : 0;
       const int na = A.conn[(size_t)e * 8 + a], nb = A.conn[(size_t)e * 8 + b];
       const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
       const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
+      nar[h] = na;
+      mar[h] = live ? ma : 0;
       // Dirichlet lift: F_a -= K_ab * dubar_b over prescribed components of b (first iteration)
-      double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+      double c[3] = {0.0, 0.0, 0.0};
       if (sc.dispApplied) {
         double db[3];
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
-        c0 = Kb[0][0] * db[0] + Kb[0][1] * db[1] + Kb[0][2] * db[2];
-        c1 = Kb[1][0] * db[0] + Kb[1][1] * db[1] + Kb[1][2] * db[2];
-        c2 = Kb[2][0] * db[0] + Kb[2][1] * db[1] + Kb[2][2] * db[2];
-        c0 += shfl_xor_d(c0, 1); c0 += shfl_xor_d(c0, 2); c0 += shfl_xor_d(c0, 4);
-        c1 += shfl_xor_d(c1, 1); c1 += shfl_xor_d(c1, 2); c1 += shfl_xor_d(c1, 4);
-        c2 += shfl_xor_d(c2, 1); c2 += shfl_xor_d(c2, 2); c2 += shfl_xor_d(c2, 4);
-      }
-      const int s0 = (sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
-      if (s0 >= 0) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          if (!((ma >> i) & 1)) continue;
-          const int rowoff = s0 + rank3(ma, i) * La;
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) {
-            if (!((mb >> kk) & 1)) continue;
-            double* v = sc.vals + rowoff + rank3(mb, kk);
-            *v += Kb[i][kk];
-          }
+          double v = Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
+          v += shfl_xor_d(v, 1);
+          v += shfl_xor_d(v, 2);
+          v += shfl_xor_d(v, 4);
+          c[i] = v;
         }
       }
-      if (b == 0 && ma) {
-        const double cv[3] = {c0, c1, c2};
+      cl[h][0] = c[0]; cl[h][1] = c[1]; cl[h][2] = c[2];
+      const int s0 = (live && sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
+      block_slots(s0, ma, mb, La, addr + 9 * h);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) val[9 * h + t] = Kb[t];
+    }
+    rmw_add<18>(V, addr, val);
+    if (b == 0) {
+      int ra[6];
+      double rv[6];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          if (!((ma >> i) & 1)) continue;
-          const int r = sc.nda[3 * (size_t)na + i];
-          sc.rhs[r] += sm.R[k][a][i] - cv[i];
+          const bool fr = (mar[h] >> i) & 1;
+          ra[3 * h + i] = fr ? sc.nda[3 * (size_t)nar[h] + i] : -1;
+          rv[3 * h + i] = sm.R[k0 + h][a][i] - cl[h][i];
         }
-      }
+      rmw_add<6>(rhs, ra, rv);
     }
   }
 }
@@ -385,36 +187,12 @@ struct Tet10Smem {
   double R[EPW][10][3];     // Fu
   double FpS[EPW][4];       // Fp
   double LP[EPW][10][4];    // first-iteration lift of Fp from the prescribed dofs of node a: Kpu(q, a) dubar_a
+  double dN[4][3][10];      // [gp][r][a]: dN_a/dxi_r at the Gauss points
 };
 
-// Tet4 basis at the Gauss points of the 4-point rule: N0 = xi4, N1 = xi1, N2 = xi2, N3 = xi3.
-__device__ __forceinline__ void tet_Np(int gp, double Np[4]) {
-  double x1, x2, x3;
-  tet_gp(gp, x1, x2, x3);
-  Np[0] = 1.0 - x1 - x2 - x3; Np[1] = x1; Np[2] = x2; Np[3] = x3;
-}
-
-// Kuu block (a,b) of element slot k, summed over the 4 Gauss points.
 template <int EPW>
-__device__ __forceinline__ void tet10_uu_block(const Tet10Smem<EPW>& sm, int k, int a, int b, double Kb[3][3]) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i) Kb[i][0] = Kb[i][1] = Kb[i][2] = 0.0;
-#pragma unroll
-  for (int gp = 0; gp < 4; ++gp) {
-    const double* Ga = sm.G[k][gp][a];
-    const double* Gb = sm.G[k][gp][b];
-    const double* S = sm.S[k][gp];
-    const double s1 = S[0], s2 = S[1], s3 = S[2], s4 = S[3];
-    const double ga[3] = {Ga[0], Ga[1], Ga[2]}, ba[3] = {Ga[3], Ga[4], Ga[5]};
-    const double gb[3] = {Gb[0], Gb[1], Gb[2]}, bb[3] = {Gb[3], Gb[4], Gb[5]};
-    const double y = s4 * (ba[0] * gb[0] + ba[1] * gb[1] + ba[2] * gb[2]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk)
-        Kb[i][kk] += (s1 * ga[kk]) * gb[i] + (s2 * ga[i] + s3 * ba[i]) * gb[kk] + (s3 * ga[i]) * bb[kk] +
-                     (i == kk ? y : 0.0);
-  }
+__device__ __forceinline__ void tet10_uu_block(const Tet10Smem<EPW>& sm, int k, int a, int b, double Kb[9]) {
+  tangent_block<4, false>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 60, &sm.S[k][0][0], 8, Kb);
 }
 
 template <int EPW, bool DENSE>
@@ -422,253 +200,183 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
                                                    double* __restrict__ Kup, double* __restrict__ Kpu,
                                                    double* __restrict__ Kpp, double* __restrict__ Fu,
                                                    double* __restrict__ Fp, ScatterArgs sc) {
+  static_assert(EPW == 4 || EPW == 8 || EPW == 16, "phase 1 maps (k, gp) onto the 64 lanes");
   __shared__ Tet10Smem<EPW> sm;
   const int lane = threadIdx.x;
   const int base_idx = blockIdx.x * EPW;
-  const double K = 1.0 / A.Kinv;
   const double wq = 0.2500000000000000 * (1.0 / 6.0);  // QuadratureUtil.cpp:706-709
-
+  if (lane < 4) {
+    double x1, x2, x3, d[3][10];
+    tet_gp(lane, x1, x2, x3);
+    tet10_dN(x1, x2, x3, d[0], d[1], d[2]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int a = 0; a < 10; ++a) sm.dN[lane][r][a] = d[r][a];
+  }
   for (int t = lane; t < EPW * 10; t += 64) {
     const int k = t / 10, a = t % 10;
-    const int idx = base_idx + k;
-    double x[3] = {0, 0, 0}, u[3] = {0, 0, 0}, up[3] = {0, 0, 0}, pn = 0.0;
-    if (idx < count) {
-      const int e = DENSE ? first + idx : sc.elems[idx];
-      const size_t nd = (size_t)A.conn[(size_t)e * 10 + a];
+    const int idx = (base_idx + k < count) ? base_idx + k : base_idx;
+    const int e = DENSE ? first + idx : sc.elems[idx];
+    const size_t nd = (size_t)A.conn[(size_t)e * 10 + a];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) { x[d] = A.X[3 * nd + d]; u[d] = A.u[3 * nd + d]; up[d] = A.uprev[3 * nd + d]; }
-      if (a < 4) pn = A.pres[nd];
+    for (int c = 0; c < 3; ++c) {
+      sm.X[k][a][c] = A.X[3 * nd + c];
+      sm.U[k][a][c] = A.u[3 * nd + c];
+      sm.UP[k][a][c] = A.uprev[3 * nd + c];
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) { sm.X[k][a][d] = x[d]; sm.U[k][a][d] = u[d]; sm.UP[k][a][d] = up[d]; }
-    if (a < 4) sm.Pn[k][a] = pn;
+    if (a < 4) sm.Pn[k][a] = A.pres[nd];
   }
   __syncthreads();
-
-  // ---- phase 1: lane = (k, gp), 4 Gauss points per element ----
   {
-    const int k = lane >> 2, gp = lane & 3;
-    const bool act = (k < EPW) && (base_idx + k < count);
-    double R[10][3];
-#pragma unroll
-    for (int a = 0; a < 10; ++a) R[a][0] = R[a][1] = R[a][2] = 0.0;
-    if (k < EPW) {
-      double x1, x2, x3;
-      tet_gp(gp, x1, x2, x3);
-      double d[3][10];
-      tet10_dN(x1, x2, x3, d[0], d[1], d[2]);
-      double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Bi[3][3];
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double xc = sm.X[k][a][c];
-          B[0][c] += xc * d[0][a]; B[1][c] += xc * d[1][a]; B[2][c] += xc * d[2][a];
-        }
-      const double Jac0 = inv3(B, Bi);
-      const double dv0 = wq * Jac0;
-      double F[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, Fn[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-#pragma unroll
-      for (int a = 0; a < 10; ++a) {
-        const double gx = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
-        const double gy = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
-        const double gz = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double ui = sm.U[k][a][i], upi = sm.UP[k][a][i];
-          F[i][0] += ui * gx; F[i][1] += ui * gy; F[i][2] += ui * gz;
-          Fn[i][0] += upi * gx; Fn[i][1] += upi * gy; Fn[i][2] += upi * gz;
-        }
-      }
-      // compute_constants_volumetric_functions, finite strain (utilitiesmaterialHyperelastic.cpp:144-173)
-      const double Jn = det3(Fn);
-      double Jhat = 1.0, thetahat = 0.0;
-      if (A.Utype != -1) {
-        double d1, d2;
-        vol_derivs(A.Utype, K, Jn, d1, d2);
-        Jhat = Jn - d1 / d2;
-        thetahat = 1.0 / d2;
-      }
-      const double J = det3(F);
-      if (act && J < 0.0) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
-      double Bc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double xc = sm.X[k][a][c] + sm.U[k][a][c];
-          Bc[0][c] += xc * d[0][a]; Bc[1][c] += xc * d[1][a]; Bc[2][c] += xc * d[2][a];
-        }
-      const double Jac = inv3(Bc, Bi);
-      const double dv = wq * Jac;
-      double Np[4];
-      tet_Np(gp, Np);
-      double pres = 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) pres += Np[q] * sm.Pn[k][q];
-      NHPoint P;
-      neohooke_point(F, J, A.mu, K, A.Utype, true, pres, dv, false, P);
-#pragma unroll
-      for (int a = 0; a < 10; ++a) {
-        const double g0 = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
-        const double g1 = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
-        const double g2 = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-        double* G = sm.G[k][gp][a];
-        G[0] = g0; G[1] = g1; G[2] = g2;
-        G[3] = P.b[0][0] * g0 + P.b[0][1] * g1 + P.b[0][2] * g2;
-        G[4] = P.b[1][0] * g0 + P.b[1][1] * g1 + P.b[1][2] * g2;
-        G[5] = P.b[2][0] * g0 + P.b[2][1] * g1 + P.b[2][2] * g2;
-        const double b0 = dv * g0, b1 = dv * g1, b2 = dv * g2;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) R[a][i] = -(b0 * P.sig[i][0] + b1 * P.sig[i][1] + b2 * P.sig[i][2]);
-      }
-      double* S = sm.S[k][gp];
-      S[0] = P.s1; S[1] = P.s2; S[2] = P.s3; S[3] = P.s4;
-      S[4] = dv; S[5] = dv0; S[6] = thetahat; S[7] = J - Jhat - thetahat * pres;
-    }
-#pragma unroll
-    for (int a = 0; a < 10; ++a)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        double v = R[a][i];
-        v += shfl_xor_d(v, 1);
-        v += shfl_xor_d(v, 2);
-        R[a][i] = v;
-      }
-    if (k < EPW) {
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-        if ((a & 3) == gp) { sm.R[k][a][0] = R[a][0]; sm.R[k][a][1] = R[a][1]; sm.R[k][a][2] = R[a][2]; }
-    }
+    // lane = (k, gp); lanes beyond 4*EPW mirror the lower ones and do not write
+    const int k = (lane >> 2) & (EPW - 1), gp = lane & 3;
+    const bool writer = (lane >> 2) < EPW;
+    const bool act = writer && (base_idx + k < count);
+    bool negJ = false;
+    element_point<10, 4, true>(sm.X[k], sm.U[k], sm.UP[k], sm.Pn[k], sm.dN[gp], gp, wq, A.mu, 1.0 / A.Kinv, A.Utype,
+                               false, writer, &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
+    if (act && negJ) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
   }
   __syncthreads();
 
-  // ---- phase 2: items 0..99 = Kuu blocks (a,b); 100..109 = Kup/Kpu of node a; 110..113 = Kpp row q ----
+  // ---- phase 2: Kuu blocks (a,b) = items lane and lane + 64 (< 100); lanes 36..45 also take the
+  //      Kup/Kpu coupling of node a = lane - 36; lanes 46..49 the Kpp row q = lane - 46 ----
+  const bool lift = !DENSE && sc.dispApplied != nullptr;
+  double* __restrict__ V = sc.vals;
   for (int k = 0; k < EPW; ++k) {
     const int idx = base_idx + k;
     if (idx >= count) break;  // uniform across the wave
     const int e = DENSE ? first + idx : sc.elems[idx];
-    const bool lift = !DENSE && sc.dispApplied != nullptr;
-    for (int pass = 0; pass < 2; ++pass) {
-      const int item = pass * 64 + lane;
-      if (item < 100) {
-        const int a = item / 10, b = item % 10;
-        double Kb[3][3];
-        tet10_uu_block(sm, k, a, b, Kb);
-        if (DENSE) {
-          double* Ko = Kuu + (size_t)idx * 900;
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) Ko[(3 * a + i) * 30 + 3 * b + kk] = Kb[i][kk];
-        } else if (sc.scatterK) {
-          const int na = A.conn[(size_t)e * 10 + a], nb = A.conn[(size_t)e * 10 + b];
-          const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
-          const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
-          const int s0 = (ma && mb) ? sc.slot[(size_t)e * kTet10Slots + a * 10 + b] : -1;
-          if (s0 >= 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              if (!((ma >> i) & 1)) continue;
-              const int rowoff = s0 + rank3(ma, i) * La;
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk) {
-                if (!((mb >> kk) & 1)) continue;
-                sc.vals[rowoff + rank3(mb, kk)] += Kb[i][kk];
-              }
-            }
-          }
-        }
-      } else if (item < 110) {
-        // Kup(3a+i, q) += (dv g_a,i) Np_q ;  Kpu(q, 3a+j) += (dv Np_q) g_a,j   (ElementBase2.cpp:500-505, 524-533)
-        const int a = item - 100;
-        double up[3][4], pu[4][3];
+    {
+      const int a1 = lane / 10, b1 = lane % 10;
+      const int item2 = lane + 64;
+      const bool has2 = item2 < 100;
+      const int a2 = has2 ? item2 / 10 : 0, b2 = has2 ? item2 % 10 : 0;
+      double Kb[18];
+      tet10_uu_block(sm, k, a1, b1, Kb);
+      tet10_uu_block(sm, k, a2, b2, Kb + 9);
+      if (DENSE) {
+        double* Ko = Kuu + (size_t)idx * 900;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) { up[i][q] = 0.0; pu[q][i] = 0.0; }
+          for (int kk = 0; kk < 3; ++kk) {
+            Ko[(3 * a1 + i) * 30 + 3 * b1 + kk] = Kb[3 * i + kk];
+            if (has2) Ko[(3 * a2 + i) * 30 + 3 * b2 + kk] = Kb[9 + 3 * i + kk];
+          }
+      } else {
+        int addr[18];
+        const int na1 = A.conn[(size_t)e * 10 + a1], nb1 = A.conn[(size_t)e * 10 + b1];
+        const int i1a = sc.nodeinfo[na1], i1b = sc.nodeinfo[nb1];
+        const int s1 = (sc.scatterK && (i1a & 7) && (i1b & 7)) ? sc.slot[(size_t)e * kTet10Slots + a1 * 10 + b1] : -1;
+        block_slots(s1, i1a & 7, i1b & 7, i1a >> 4, addr);
+        int s2 = -1, i2a = 0, i2b = 0;
+        if (has2) {
+          const int na2 = A.conn[(size_t)e * 10 + a2], nb2 = A.conn[(size_t)e * 10 + b2];
+          i2a = sc.nodeinfo[na2];
+          i2b = sc.nodeinfo[nb2];
+          s2 = (sc.scatterK && (i2a & 7) && (i2b & 7)) ? sc.slot[(size_t)e * kTet10Slots + a2 * 10 + b2] : -1;
+        }
+        block_slots(s2, i2a & 7, i2b & 7, i2a >> 4, addr + 9);
+        rmw_add<18>(V, addr, Kb);
+      }
+    }
+    if (lane >= 36 && lane < 46) {
+      // Kup(3a+i, q) += (dv g_a,i) Np_q ;  Kpu(q, 3a+j) += (dv Np_q) g_a,j   (ElementBase2.cpp:500-505, 524-533)
+      const int a = lane - 36;
+      double up[3][4], pu[4][3];
 #pragma unroll
-        for (int gp = 0; gp < 4; ++gp) {
-          const double* Ga = sm.G[k][gp][a];
-          const double dv = sm.S[k][gp][4];
-          double Np[4];
-          tet_Np(gp, Np);
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { up[i][q] = 0.0; pu[q][i] = 0.0; }
+#pragma unroll
+      for (int gp = 0; gp < 4; ++gp) {
+        const double* Ga = sm.G[k][gp][a];
+        const double dv = sm.S[k][gp][4];
+        double Np[4];
+        tet_Np(gp, Np);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double bb4 = dv * Np[q];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            up[i][q] += (dv * Ga[i]) * Np[q];
+            pu[q][i] += bb4 * Ga[i];
+          }
+        }
+      }
+      if (DENSE) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const double bb4 = dv * Np[q];
+            Kup[(size_t)idx * 120 + (3 * a + i) * 4 + q] = up[i][q];
+            Kpu[(size_t)idx * 120 + q * 30 + 3 * a + i] = pu[q][i];
+          }
+      } else {
+        const int na = A.conn[(size_t)e * 10 + a];
+        const int infa = sc.nodeinfo[na];
+        const int ma = infa & 7, La = infa >> 4;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              up[i][q] += (dv * Ga[i]) * Np[q];
-              pu[q][i] += bb4 * Ga[i];
-            }
+        for (int q = 0; q < 4; ++q) {
+          double c = 0.0;
+          if (lift) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              if (!((ma >> j) & 1)) c += pu[q][j] * sc.dispApplied[3 * (size_t)na + j];
+          }
+          sm.LP[k][a][q] = c;
+        }
+        int addr[12];
+        double v[12];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int sup = (sc.scatterK && ma) ? sc.slot[(size_t)e * kTet10Slots + 100 + a * 4 + q] : -1;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            addr[3 * q + i] = (sup >= 0 && ((ma >> i) & 1)) ? sup + rank3(ma, i) * La : -1;
+            v[3 * q + i] = up[i][q];
           }
         }
-        if (DENSE) {
+        rmw_add<12>(V, addr, v);
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 4; ++q) {
+          const int spu = (sc.scatterK && ma) ? sc.slot[(size_t)e * kTet10Slots + 140 + q * 10 + a] : -1;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              Kup[(size_t)idx * 120 + (3 * a + i) * 4 + q] = up[i][q];
-              Kpu[(size_t)idx * 120 + q * 30 + 3 * a + i] = pu[q][i];
-            }
-        } else {
-          const int na = A.conn[(size_t)e * 10 + a];
-          const int infa = sc.nodeinfo[na];
-          const int ma = infa & 7, La = infa >> 4;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            double c = 0.0;
-            if (lift) {
-#pragma unroll
-              for (int j = 0; j < 3; ++j)
-                if (!((ma >> j) & 1)) c += pu[q][j] * sc.dispApplied[3 * (size_t)na + j];
-            }
-            sm.LP[k][a][q] = c;
-          }
-          if (sc.scatterK && ma) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int sup = sc.slot[(size_t)e * kTet10Slots + 100 + a * 4 + q];
-              const int spu = sc.slot[(size_t)e * kTet10Slots + 140 + q * 10 + a];
-              if (sup < 0 || spu < 0) continue;
-#pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                if (!((ma >> i) & 1)) continue;
-                sc.vals[sup + rank3(ma, i) * La] += up[i][q];
-                sc.vals[spu + rank3(ma, i)] += pu[q][i];
-              }
-            }
+          for (int i = 0; i < 3; ++i) {
+            addr[3 * q + i] = (spu >= 0 && ((ma >> i) & 1)) ? spu + rank3(ma, i) : -1;
+            v[3 * q + i] = pu[q][i];
           }
         }
-      } else if (item < 114) {
-        // Kpp(q, q2) -= (dv0 Np_q thetahat) Np_q2 ;  Fp_q -= (dv0 Np_q) resip   (ElementBase2.cpp:510-539)
-        const int q = item - 110;
-        double pp[4] = {0, 0, 0, 0}, fp = 0.0;
+        rmw_add<12>(V, addr, v);
+      }
+    } else if (lane >= 46 && lane < 50) {
+      // Kpp(q, q2) -= (dv0 Np_q thetahat) Np_q2 ;  Fp_q -= (dv0 Np_q) resip   (ElementBase2.cpp:510-539)
+      const int q = lane - 46;
+      double pp[4] = {0, 0, 0, 0}, fp = 0.0;
 #pragma unroll
-        for (int gp = 0; gp < 4; ++gp) {
-          double Np[4];
-          tet_Np(gp, Np);
-          const double* S = sm.S[k][gp];
-          double bb5 = S[5] * Np[q];
-          fp -= bb5 * S[7];
-          bb5 *= S[6];
+      for (int gp = 0; gp < 4; ++gp) {
+        double Np[4];
+        tet_Np(gp, Np);
+        const double Nq = q == 0 ? Np[0] : q == 1 ? Np[1] : q == 2 ? Np[2] : Np[3];
+        const double* S = sm.S[k][gp];
+        double bb5 = S[5] * Nq;
+        fp -= bb5 * S[7];
+        bb5 *= S[6];
 #pragma unroll
-          for (int q2 = 0; q2 < 4; ++q2) pp[q2] -= bb5 * Np[q2];
-        }
-        if (DENSE) {
+        for (int q2 = 0; q2 < 4; ++q2) pp[q2] -= bb5 * Np[q2];
+      }
+      if (DENSE) {
 #pragma unroll
-          for (int q2 = 0; q2 < 4; ++q2) Kpp[(size_t)idx * 16 + q * 4 + q2] = pp[q2];
-          Fp[(size_t)idx * 4 + q] = fp;
-        } else {
-          if (sc.scatterK) {
+        for (int q2 = 0; q2 < 4; ++q2) Kpp[(size_t)idx * 16 + q * 4 + q2] = pp[q2];
+        Fp[(size_t)idx * 4 + q] = fp;
+      } else {
+        int addr[4];
 #pragma unroll
-            for (int q2 = 0; q2 < 4; ++q2) {
-              const int sp = sc.slot[(size_t)e * kTet10Slots + 180 + q * 4 + q2];
-              if (sp >= 0) sc.vals[sp] += pp[q2];
-            }
-          }
-          sm.FpS[k][q] = fp;
-        }
+        for (int q2 = 0; q2 < 4; ++q2) addr[q2] = sc.scatterK ? sc.slot[(size_t)e * kTet10Slots + 180 + q * 4 + q2] : -1;
+        rmw_add<4>(V, addr, pp);
+        sm.FpS[k][q] = fp;
       }
     }
     if (DENSE) {
@@ -690,17 +398,20 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
           double db[3];
 #pragma unroll
           for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
-          double Kb[3][3];
+          double Kb[9];
           tet10_uu_block(sm, k, a, b, Kb);
 #pragma unroll
-          for (int i = 0; i < 3; ++i) c[i] += Kb[i][0] * db[0] + Kb[i][1] * db[1] + Kb[i][2] * db[2];
+          for (int i = 0; i < 3; ++i) c[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
         }
       }
+      int ra[3];
+      double rv[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        if (!((ma >> i) & 1)) continue;
-        sc.rhs[sc.nda[3 * (size_t)na + i]] += sm.R[k][a][i] - c[i];
+        ra[i] = ((ma >> i) & 1) ? sc.nda[3 * (size_t)na + i] : -1;
+        rv[i] = sm.R[k][a][i] - c[i];
       }
+      rmw_add<3>(sc.rhs, ra, rv);
     } else if (lane < 14) {
       const int q = lane - 10;
       const int nq = A.conn[(size_t)e * 10 + q];
@@ -715,10 +426,13 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
 }  // namespace
 
 // --------------------------------------------------------------------------------------------
-// Launchers
+// Launchers (elements per wavefront: Hex8 8, Tet10 MPF_TET_EPW, default 4).
 // --------------------------------------------------------------------------------------------
+#ifndef MPF_TET_EPW
+#define MPF_TET_EPW 4
+#endif
 constexpr int kHexEPW = 8;
-constexpr int kTetEPW = 8;
+constexpr int kTetEPW = MPF_TET_EPW;
 
 mpf_status launch_hex8_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,
                              hipStream_t s) {

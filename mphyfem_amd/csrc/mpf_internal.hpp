@@ -125,6 +125,9 @@ struct PatternBufs {
   DevBuf<int32_t> nodeinfo;   // (row length << 4) | (pressure flag << 3) | free mask
   DevBuf<int32_t> color, perm;
   DevBuf<int32_t> slot;       // per-element slot bases
+  DevBuf<int32_t> g_nelA;     // gather tables: elements around each node (ascending) with the node's local index
+  DevBuf<int32_t> g_items;    // gather tables: per node, (column, element, node) items in (column, element) order
+  bool gather_ready = false;
   std::vector<int32_t> color_off;  // host: [ncolors + 1] offsets into perm
   int32_t dispDOF = 0, presDOF = 0, ncolors = 0;
   int64_t nnz = 0;
@@ -160,9 +163,18 @@ struct mpf_problem {
   bool assembled = false;
   bool solved = false;
   mpf::KrylovWork* kw = nullptr;
+  // assembly algorithm (MPF_ASSEMBLY_*) and the gather path's staging records
+  int32_t asm_mode = MPF_ASSEMBLY_GATHER;
+  mpf::DevBuf<double> GS;
+  mpf::DevBuf<double> scratch;  // reductions (norms)
 };
 
 namespace mpf {
+size_t stage_doubles_per_elem(int etype);          // gather.hip
+mpf_status ensure_gather_tables(mpf_problem* p);   // gather.hip (falls back to the scatter when unsupported)
+mpf_status launch_gather_assembly(int etype, const ElemArgs& A, int32_t nElem, int32_t nNode, const PatternBufs& P,
+                                  double* GS, double* vals, double* rhs, const double* dispApplied, int writeK,
+                                  int32_t* overflow, hipStream_t s);
 mpf_status build_pattern(mpf_problem* p);          // pattern.hip
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
 void krylov_free(mpf_problem* p);

@@ -204,14 +204,15 @@ std::map<const mpf_problem*, std::vector<double>>& hostX_store() {
   return m;
 }
 
-// Enqueue the assembly kernels (no synchronisation).  scatterK = 0 assembles the residual only.
-mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK) {
+// Enqueue the element kernels of one assembly (no zero fill, no external forces).
+mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK) {
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
-  const int64_t n = (int64_t)P.dispDOF + P.presDOF;
-  if (scatterK) hipLaunchKernelGGL(k_fill_d, dim3(g256(P.nnz)), dim3(256), 0, s, p->vals.ptr, P.nnz, 0.0);
-  hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->rhs.ptr, n, 0.0);
   ElemArgs A = elem_args(p);
+  if (p->asm_mode == MPF_ASSEMBLY_GATHER) {
+    return launch_gather_assembly(p->etype, A, p->nElem, p->nNode, P, p->GS.ptr, p->vals.ptr, p->rhs.ptr,
+                                  first_iter ? p->dispApplied.ptr : nullptr, scatterK, p->bad.ptr + 1, s);
+  }
   ScatterArgs sc{};
   sc.slot = P.slot.ptr;
   sc.nodeinfo = P.nodeinfo.ptr;
@@ -228,6 +229,21 @@ mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int sca
     if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
     else MPF_TRY(launch_tet10_scatter(A, sc, s));
   }
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+// Enqueue one full assembly (no synchronisation).  scatterK = 0 assembles the residual only.
+// zeroMtx (SolverPetsc.cpp:145-163) is only needed by the scatter, which accumulates.
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK) {
+  hipStream_t s = p->stream;
+  const PatternBufs& P = p->pat;
+  const int64_t n = (int64_t)P.dispDOF + P.presDOF;
+  if (p->asm_mode == MPF_ASSEMBLY_COLORED_SCATTER) {
+    if (scatterK) hipLaunchKernelGGL(k_fill_d, dim3(g256(P.nnz)), dim3(256), 0, s, p->vals.ptr, P.nnz, 0.0);
+    hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->rhs.ptr, n, 0.0);
+  }
+  MPF_TRY(enqueue_element_kernels(p, first_iter, scatterK));
   if (p->has_fext && P.dispDOF > 0)
     hipLaunchKernelGGL(k_add_fext, dim3(grid_for(P.dispDOF, 256)), dim3(256), 0, s, P.dispDOF, P.assy.ptr,
                        p->fext_unit.ptr, lam, p->rhs.ptr);
@@ -235,14 +251,21 @@ mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int sca
   return MPF_OK;
 }
 
+// bad[0]: 1 + element with det F < 0; bad[1]: gather-kernel adjacency overflow.
 mpf_status assemble_sync(mpf_problem* p, bool first_iter, double lam, int scatterK) {
-  MPF_HIP(hipMemsetAsync(p->bad.ptr, 0, sizeof(int32_t), p->stream));
+  MPF_HIP(hipMemsetAsync(p->bad.ptr, 0, 2 * sizeof(int32_t), p->stream));
   MPF_TRY(enqueue_assembly(p, first_iter, lam, scatterK));
-  int32_t bad = 0;
-  MPF_HIP(hipMemcpyAsync(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  int32_t bad[2] = {0, 0};
+  MPF_HIP(hipMemcpyAsync(bad, p->bad.ptr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
   MPF_HIP(hipStreamSynchronize(p->stream));
+  if (bad[1]) {
+    // a node has more neighbours than the row kernel holds: use the coloured scatter from now on
+    std::fprintf(stderr, "libmpf: node adjacency exceeds the gather kernel's limits; using the coloured scatter\n");
+    p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
+    return assemble_sync(p, first_iter, lam, scatterK);
+  }
   p->assembled = true;
-  if (bad) return MPF_ERR_NEG_JACOBIAN;
+  if (bad[0]) return MPF_ERR_NEG_JACOBIAN;
   return MPF_OK;
 }
 
@@ -272,13 +295,9 @@ mpf_status do_set_bc(mpf_problem* p, double lam) {
 }
 
 mpf_status rhs_norm(mpf_problem* p, double* out) {
-  KrylovWork* kw = p->kw;
-  (void)kw;
-  static thread_local DevBuf<double>* scratch = nullptr;
-  if (!scratch) scratch = new DevBuf<double>();
-  MPF_TRY(scratch->alloc(4096));
+  if (p->scratch.n < 4096) MPF_TRY(p->scratch.alloc(4096));
   const int64_t n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
-  return dev_norm2(p->rhs.ptr, n, out, p->stream, scratch->ptr);
+  return dev_norm2(p->rhs.ptr, n, out, p->stream, p->scratch.ptr);
 }
 
 double seconds_since(std::chrono::steady_clock::time_point t0) {
@@ -435,7 +454,7 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   if (st == MPF_OK) st = p->dispApplied.alloc(n3);
   if (st == MPF_OK) st = p->pres.alloc(nNode);
   if (st == MPF_OK) st = p->presPrev.alloc(nNode);
-  if (st == MPF_OK) st = p->bad.alloc(1);
+  if (st == MPF_OK) st = p->bad.alloc(2);
   if (st != MPF_OK) {
     mpf_problem_destroy(p);
     return st;
@@ -506,6 +525,7 @@ mpf_status mpf_prepare_pattern(mpf_problem* p) {
   MPF_TRY(p->vals.alloc(p->pat.nnz > 0 ? p->pat.nnz : 1));
   MPF_TRY(p->rhs.alloc(n > 0 ? n : 1));
   MPF_TRY(p->du.alloc(n > 0 ? n : 1));
+  if (p->asm_mode == MPF_ASSEMBLY_GATHER) MPF_TRY(ensure_gather_tables(p));
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->du.ptr, n, 0.0);
   if (!p->trac.empty()) {
     std::vector<double> F;
@@ -565,6 +585,19 @@ mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
   if (disp) MPF_TRY(p->disp.download(disp, (size_t)p->nNode * 3, s));
   if (pres) MPF_TRY(p->pres.download(pres, p->nNode, s));
   MPF_HIP(hipStreamSynchronize(s));
+  return MPF_OK;
+}
+
+mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
+  if (!p || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER)) return MPF_ERR_ARG;
+  p->asm_mode = mode;
+  if (p->prepared && mode == MPF_ASSEMBLY_GATHER) MPF_TRY(ensure_gather_tables(p));
+  return MPF_OK;
+}
+
+mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode) {
+  if (!p || !mode) return MPF_ERR_ARG;
+  *mode = p->asm_mode;
   return MPF_OK;
 }
 
@@ -715,21 +748,9 @@ mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, int32_t first_iter, d
   MPF_HIP(hipEventElapsedTime(&ms, e0, e1));
   if (ms_per_assembly) *ms_per_assembly = ms / reps;
   if (ms_scatter_kernels) {
-    // element kernels only (no zero-fill / external forces)
-    ElemArgs A = elem_args(p);
-    ScatterArgs sc{};
-    const PatternBufs& P = p->pat;
-    sc.slot = P.slot.ptr; sc.nodeinfo = P.nodeinfo.ptr; sc.nda = P.nda.ptr; sc.idp = P.idp.ptr;
-    sc.dispDOF = P.dispDOF; sc.vals = p->vals.ptr; sc.rhs = p->rhs.ptr; sc.scatterK = 1;
-    sc.dispApplied = first_iter ? p->dispApplied.ptr : nullptr;
+    // element kernels only (no zero fill / external forces)
     MPF_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r)
-      for (int c = 0; c < P.ncolors; ++c) {
-        sc.elems = P.perm.ptr + P.color_off[c];
-        sc.count = P.color_off[c + 1] - P.color_off[c];
-        if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
-        else MPF_TRY(launch_tet10_scatter(A, sc, s));
-      }
+    for (int r = 0; r < reps; ++r) MPF_TRY(enqueue_element_kernels(p, first_iter != 0, 1));
     MPF_HIP(hipEventRecord(e1, s));
     MPF_HIP(hipEventSynchronize(e1));
     MPF_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -187,11 +187,13 @@ def test_pattern_and_colouring_bitexact_tet(mpf, orc, gpu):
     assert pat.ncolors == ncol and np.array_equal(pat.color, col)
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["gather", "colored_scatter"])
 @pytest.mark.parametrize("tangent,traction", [(0, None), (1, (-20.0, 5.0))])
-def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction):
+def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction, mode):
     n = 10
     m = mpf.matl_from_E_nu(1e3, 0.3, tangent=tangent)
     X, conn, P = _hex_problem(mpf, n, m, traction)
+    P.set_assembly_mode(mode)
     O = _orc_hex_problem(orc, mpf, X, conn, n, m, traction)
     u = random_state(len(X), 0.005, seed=21)
     for Q in (P, O):
@@ -207,11 +209,13 @@ def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction):
         assert abs(P.rhs_norm() - np.linalg.norm(orhs)) < 1e-12 * np.linalg.norm(orhs)
 
 
-def test_global_assembly_tet_vs_oracle(mpf, orc, gpu):
+@pytest.mark.parametrize("mode", [0, 1], ids=["gather", "colored_scatter"])
+def test_global_assembly_tet_vs_oracle(mpf, orc, gpu, mode):
     n = 4
     X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
     m = mpf.matl(1.0, 1e-3, 3, mixed=True)
     P = mpf.Problem(mpf.TET10P1, X, conn, m)
+    P.set_assembly_mode(mode)
     O = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
     u = random_state(len(X), 0.004, seed=31)
     up = 0.5 * u
@@ -227,6 +231,29 @@ def test_global_assembly_tet_vs_oracle(mpf, orc, gpu):
         ov, orhs, st = O.assemble(first, 1.0)
         assert normwise_rel(vals, ov) < TOL_K
         assert normwise_rel(rhs, orhs) < TOL_K
+    assert P.assembly_mode() == mode
+
+
+def test_gather_matches_scatter_bitwise_reproducible(mpf, gpu):
+    """The gather path writes each CSR value once in a fixed order: repeated assemblies are
+    bit-identical, and both algorithms agree to rounding."""
+    n = 6
+    X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+    P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
+    apply_bcs(P, tet_compress_bcs(n))
+    P.prepare_matrix_pattern()
+    u = random_state(len(X), 0.003, seed=41)
+    P.set_state(disp=u, disp_prev=0.5 * u)
+    P.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    v1, r1 = P.system()
+    P.calc_stiffness_and_residual(True, 1.0)
+    v2, r2 = P.system()
+    assert np.array_equal(v1, v2) and np.array_equal(r1, r2)
+    P.set_assembly_mode(mpf.ASSEMBLY_COLORED_SCATTER)
+    P.calc_stiffness_and_residual(True, 1.0)
+    v3, r3 = P.system()
+    assert normwise_rel(v3, v1) < TOL_K and normwise_rel(r3, r1) < TOL_K
 
 
 def test_krylov_vs_oracle(mpf, orc, gpu):
