@@ -102,6 +102,20 @@ def read_traffic(name: str):
         return None
 
 
+def copy_calibration() -> int:
+    """Three device-to-device copies of 1 GiB: what FETCH_SIZE / WRITE_SIZE read for known traffic."""
+    import torch
+
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copied 3 x 1 GiB")
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +129,11 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--assembly", choices=["gather", "scatter"], default="gather",
                     help="row-owner gather assembly or coloured element scatter")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (reference run for the HBM byte counters)")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        return copy_calibration()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this path runs on one GPU only (no multi-GPU code); run with --gpus 1", file=sys.stderr)
         return 2

@@ -260,6 +260,25 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   }
 }
 
+// One Gauss point's share of the 3x3 tangent block (a,b):
+//   K_ab^{ik} += s1 g^a_k g^b_i + (s2 g^a_i + s3 (b g^a)_i) g^b_k + s3 g^a_i (b g^b)_k + delta_ik (s4 b g^a + s5 g^a).g^b
+// with g = grad_x N and b the left Cauchy-Green tensor (the closed form of the reference's
+// Gc = grad N . Cmat contraction, Elem_Solid_3D_Disp.cpp:315-366).
+template <bool HAS_S5>
+__device__ __forceinline__ void tangent_gp(const double ga[3], const double ba[3], const double gb[3], const double bb[3],
+                                           double s1, double s2, double s3, double s4, double s5, double Kb[9]) {
+  const double y = HAS_S5 ? (s4 * ba[0] + s5 * ga[0]) * gb[0] + (s4 * ba[1] + s5 * ga[1]) * gb[1] +
+                                (s4 * ba[2] + s5 * ga[2]) * gb[2]
+                          : s4 * (ba[0] * gb[0] + ba[1] * gb[1] + ba[2] * gb[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double Qi = s2 * ga[i] + s3 * ba[i], Ti = s3 * ga[i];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+      Kb[3 * i + kk] += (s1 * ga[kk]) * gb[i] + Qi * gb[kk] + Ti * bb[kk] + (i == kk ? y : 0.0);
+  }
+}
+
 // 3x3 tangent block (a,b) from the staged Gauss-point data of one element: Ga = G of node a,
 // Gb = G of node b (6 doubles each per Gauss point, stride gs between points), S scalars (stride ss).
 template <int NGP, bool HAS_S5>
@@ -272,18 +291,9 @@ __device__ __forceinline__ void tangent_block(const double* Ga, const double* Gb
     const double* ga_ = Ga + gp * gs;
     const double* gb_ = Gb + gp * gs;
     const double* s = S + gp * ss;
-    const double s1 = s[0], s2 = s[1], s3 = s[2], s4 = s[3], s5 = HAS_S5 ? s[4] : 0.0;
     const double ga[3] = {ga_[0], ga_[1], ga_[2]}, ba[3] = {ga_[3], ga_[4], ga_[5]};
     const double gb[3] = {gb_[0], gb_[1], gb_[2]}, bb[3] = {gb_[3], gb_[4], gb_[5]};
-    const double y = (s4 * ba[0] + s5 * ga[0]) * gb[0] + (s4 * ba[1] + s5 * ga[1]) * gb[1] +
-                     (s4 * ba[2] + s5 * ga[2]) * gb[2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double Qi = s2 * ga[i] + s3 * ba[i], Ti = s3 * ga[i];
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk)
-        Kb[3 * i + kk] += (s1 * ga[kk]) * gb[i] + Qi * gb[kk] + Ti * bb[kk] + (i == kk ? y : 0.0);
-    }
+    tangent_gp<HAS_S5>(ga, ba, gb, bb, s[0], s[1], s[2], s[3], HAS_S5 ? s[4] : 0.0, Kb);
   }
 }
 

@@ -18,6 +18,8 @@
 // path, and the result is bitwise reproducible.  Each column block's contributions are summed in
 // element order, the reference's own order for a given row (femSolidmechanics2.cpp:1061).
 
+#include <hipcub/hipcub.hpp>
+
 #include "fem_device.hpp"
 
 namespace mpf {
@@ -89,30 +91,34 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   if (writer && negJ) flag_bad(A.bad, e);
 }
 
-struct RowArgs {
-  int32_t nNode;
-  const int32_t* nptr;      // node -> elements (offsets)
-  const int32_t* nelA;      // elements around each node, ascending: e * 16 + (local index of the node in e)
-  const int32_t* items;     // per node, (column block j << 16) | (s << 4) | b sorted by (j, s)
-  const int32_t* nbptr;     // node -> coupled nodes (sorted)
-  const int32_t* nb;
-  const int32_t* nodeinfo;  // (row length << 4) | (pressure << 3) | free mask
-  const int32_t* nda;
-  const int32_t* idp;
-  int32_t dispDOF;
-  const int32_t* rowptr;
-  const double* GS;
-  const double* dispApplied;  // non-null: first-iteration lift
-  double* vals;
-  double* rhs;
-  int32_t writeK;
-  int32_t* overflow;
-};
+// Morton (Z-curve) code of a node: 21 bits per axis of its position in the bounding box, interleaved.
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
 
-// XCD-aware node order: consecutive workgroups land on different XCDs (round robin), so each XCD
-// is given a contiguous range of nodes and the staging records of neighbouring nodes stay in its L2.
-__device__ __forceinline__ int node_of_block(int b, int nNode) {
-  const int per = (nNode + 7) / 8;
+__global__ void k_morton(int32_t nNode, const double* __restrict__ X, double x0, double y0, double z0, double sx,
+                         double sy, double sz, uint64_t* __restrict__ code, int32_t* __restrict__ id) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNode) return;
+  const double q[3] = {(X[3 * (size_t)i] - x0) * sx, (X[3 * (size_t)i + 1] - y0) * sy, (X[3 * (size_t)i + 2] - z0) * sz};
+  uint64_t c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c[d] = (uint64_t)fmin(fmax(q[d], 0.0), 2097151.0);
+  code[i] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+  id[i] = i;
+}
+
+// XCD-aware chunk order: consecutive workgroups land on different XCDs (round robin), so each XCD is
+// given a contiguous 1/8 of the work.  Work is laid out in Morton order of the nodes, so that share is a
+// compact region of the mesh whose elements' staging records stay in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t nb) {
+  const int64_t per = (nb + 7) / 8;
   return (b % 8) * per + b / 8;
 }
 
@@ -131,28 +137,50 @@ __device__ __forceinline__ void wave_sort(int32_t* key, int n, int lane) {
     }
 }
 
-// Per-node gather tables, built once with the pattern (one wavefront per node):
-//   nelA [nptr[n] + s]          = e * 16 + a: the elements around n in ascending id, a = local index of n in e
-//   items[nptr[n] * NPE + t]    = (j << 16) | (s << 4) | b for every element s around n and node b of it,
-//                                 sorted by (j, s), j = position of node conn[e][b] in the sorted coupled-node
-//                                 list of n, i.e. the column block of n's rows that the (s, b) block lands in.
+constexpr int kItemCap = 1024;  // kMaxAdj * NPE rounded up to a power of two (Tet10: 640 -> 1024)
+constexpr int kFlatCap = 2048;  // laid-out items of one node, padding included
+
+// Layout walk over a node's items sorted by (column j, element): a run of one column is never split
+// across a 64-item wavefront (runs are at most kMaxAdj = 64 long).  Calls emit(position, item) for
+// every item and returns the padded length (a multiple of 64).
+template <typename Emit>
+__device__ __forceinline__ int flat_walk(const int32_t* sK, int nitems, Emit emit) {
+  int pos = 0;
+  for (int t = 0; t < nitems; ++t) {
+    const int j = sK[t] >> 16;
+    if (t == 0 || j != (sK[t - 1] >> 16)) {
+      int L = 1;
+      while (t + L < nitems && (sK[t + L] >> 16) == j) ++L;
+      if ((pos & 63) + L > 64) pos = (pos + 63) & ~63;
+    }
+    emit(pos, t);
+    ++pos;
+  }
+  return (pos + 63) & ~63;
+}
+
+// Build step 1 (one wavefront per node n):
+//   nelA[nptr[n] + s]       = e * 16 + a: the elements around n in ascending id, a = local index of n in e
+//   keys[nptr[n] * NPE + t] = (j << 16) | (s << 4) | b for every element s around n and node b of it,
+//                             sorted by (j, s); j = position of node conn[e][b] in the sorted coupled-node
+//                             list of n, i.e. the column block of n's rows that the (s, b) block lands in
+//   flen[n]                 = laid-out length of n's items (0 for a node without rows)
 template <int NPE>
-__global__ void __launch_bounds__(64) k_gather_tables(int32_t nNode, const int32_t* __restrict__ nptr,
-                                                      const int32_t* __restrict__ nel,
-                                                      const int32_t* __restrict__ nbptr,
-                                                      const int32_t* __restrict__ nb,
-                                                      const int32_t* __restrict__ conn, int32_t* __restrict__ nelA,
-                                                      int32_t* __restrict__ items, int32_t* __restrict__ overflow) {
-  constexpr int KCAP = NPE <= 8 ? 512 : 1024;  // kMaxAdj * NPE rounded up to a power of two
-  static_assert(kMaxAdj * NPE <= KCAP, "item capacity");
-  __shared__ int32_t sE[kMaxAdj], sNB[kMaxNB], sK[KCAP];
+__global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t* __restrict__ nptr,
+                                                   const int32_t* __restrict__ nel, const int32_t* __restrict__ nbptr,
+                                                   const int32_t* __restrict__ nb, const int32_t* __restrict__ conn,
+                                                   const int32_t* __restrict__ nodeinfo, int32_t* __restrict__ nelA,
+                                                   int32_t* __restrict__ keys, int64_t* __restrict__ flen,
+                                                   int32_t* __restrict__ overflow) {
+  static_assert(kMaxAdj * NPE <= kItemCap, "item capacity");
+  __shared__ int32_t sE[kMaxAdj], sNB[kMaxNB], sK[kItemCap];
   const int lane = threadIdx.x;
   const int n = blockIdx.x;
   if (n >= nNode) return;
   const int e0 = nptr[n], m = nptr[n + 1] - e0;
   const int b0 = nbptr[n], nnb = nbptr[n + 1] - b0;
   if (m > kMaxAdj || nnb > kMaxNB) {
-    if (lane == 0) atomicOr(overflow, 1);
+    if (lane == 0) { atomicOr(overflow, 1); flen[n] = 0; }
     return;
   }
   sE[lane] = lane < m ? nel[e0 + lane] : 0x7fffffff;
@@ -166,7 +194,9 @@ __global__ void __launch_bounds__(64) k_gather_tables(int32_t nNode, const int32
       if (c[t] == n) a = t;
     nelA[e0 + lane] = sE[lane] * 16 + a;
   }
-  const int nitems = m * NPE;
+  const int info = nodeinfo[n];
+  const bool rows = (info & 7) || ((info >> 3) & 1);
+  const int nitems = rows ? m * NPE : 0;
   int P2 = 64;
   while (P2 < nitems) P2 <<= 1;
   for (int t = lane; t < P2; t += 64) {
@@ -186,48 +216,97 @@ __global__ void __launch_bounds__(64) k_gather_tables(int32_t nNode, const int32
   }
   __syncthreads();
   wave_sort(sK, P2, lane);
-  for (int t = lane; t < nitems; t += 64) items[(size_t)e0 * NPE + t] = sK[t];
+  for (int t = lane; t < nitems; t += 64) keys[(size_t)e0 * NPE + t] = sK[t];
+  if (lane == 0) {
+    const int len = flat_walk(sK, nitems, [](int, int) {});
+    if (len > kFlatCap) atomicOr(overflow, 4);
+    flen[n] = len;
+  }
 }
 
-// Row kernel: one wavefront per node n, 64 (element, node) items per pass in (column, element) order.
-// Item t computes the 3x3 block K_{n b} of element s (plus the u-p couplings) from s's staging record;
-// the first item of each column run (the run's head) sums the run in element order, carrying a run
-// that continues into the next pass, and writes the column block of n's rows once.
-template <int NPE, int NGP, bool MIXED>
-__global__ void __launch_bounds__(64) k_rows(RowArgs R) {
-  using St = Stage<NPE, NGP, MIXED>;
-  constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
-  constexpr int NL = MIXED ? 4 : 3;   // right-hand-side rows of a node
-  __shared__ int32_t sEA[kMaxAdj];
-  __shared__ int32_t sNB[kMaxNB], sInfo[kMaxNB], sUpos[kMaxNB], sPpos[kMaxNB];
-  __shared__ int32_t sJ[65];
-  __shared__ double sPart[64][NV];
-  __shared__ double sCarry[NV];
-  __shared__ double sLift[64][NL];
+__global__ void k_gather_len(int32_t n, const int32_t* __restrict__ order, const int64_t* __restrict__ flen,
+                             int64_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = flen[order[i]];
+}
+
+__global__ void k_scatter_off(int32_t n, const int32_t* __restrict__ order, const int64_t* __restrict__ in,
+                              int64_t* __restrict__ foff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) foff[order[i]] = in[i];
+}
+
+// Build step 2 (one wavefront per node): the node's laid-out items,
+//   fitem[foff[n] + q] = {e * 16 + a, (block << 4) | b}   block = nbptr[n] + j (a column block of n's rows)
+//   and {-1, -1} in padding slots.
+template <int NPE>
+__global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* __restrict__ nptr,
+                                                  const int32_t* __restrict__ nbptr, const int32_t* __restrict__ nelA,
+                                                  const int32_t* __restrict__ keys, const int64_t* __restrict__ flen,
+                                                  const int64_t* __restrict__ foff, int2* __restrict__ fitem) {
+  __shared__ int32_t sK[kItemCap], sMap[kFlatCap];
   const int lane = threadIdx.x;
-  const int n = node_of_block(blockIdx.x, R.nNode);
-  if (n >= R.nNode) return;
-  const int info_n = R.nodeinfo[n];
-  const int mn = info_n & 7;
-  const bool isPn = MIXED && ((info_n >> 3) & 1);
-  if (!mn && !isPn) return;
-  const int e0 = R.nptr[n], m = R.nptr[n + 1] - e0;
-  const int b0 = R.nbptr[n], nnb = R.nbptr[n + 1] - b0;
-  if (m > kMaxAdj || nnb > kMaxNB) {  // the table build rejects such meshes; never reached
-    if (lane == 0) atomicOr(R.overflow, 1);
-    return;
+  const int n = blockIdx.x;
+  if (n >= nNode) return;
+  const int len = (int)flen[n];
+  if (len == 0) return;
+  const int e0 = nptr[n], m = nptr[n + 1] - e0;
+  const int nitems = m * NPE;
+  for (int t = lane; t < nitems; t += 64) sK[t] = keys[(size_t)e0 * NPE + t];
+  for (int q = lane; q < len; q += 64) sMap[q] = -1;
+  __syncthreads();
+  if (lane == 0) flat_walk(sK, nitems, [&](int pos, int t) { sMap[pos] = t; });
+  __syncthreads();
+  const int blk0 = nbptr[n];
+  int2* out = fitem + foff[n];
+  for (int q = lane; q < len; q += 64) {
+    const int t = sMap[q];
+    int2 it = make_int2(-1, -1);
+    if (t >= 0) {
+      const int key = sK[t];
+      it.x = nelA[e0 + ((key >> 4) & 63)];
+      it.y = ((blk0 + (key >> 16)) << 4) | (key & 15);
+    }
+    out[q] = it;
   }
-  if (lane < m) sEA[lane] = R.nelA[e0 + lane];
-  // coupled nodes, their info and the column offsets of their blocks in n's rows
+}
+
+// Build step 3 (one wavefront per node n): where each column block (n, j) of n's rows lives in the CSR.
+//   Hex8:      {A, (La << 8) | (mn << 3) | mb}
+//   Tet10/P1:  {A, Bp, (D << 16) | La, (isPn << 7) | (isPb << 6) | (mn << 3) | mb}
+// A = start of n's first free u row + column offset of j's u dofs (-1 if n has no free u dof),
+// Bp = the same in n's pressure row (-1 if n carries no pressure), D = offset of j's pressure column
+// relative to A (same in every row of n), La = length of n's rows, mn / mb = free masks of n / j.
+template <bool MIXED>
+__global__ void __launch_bounds__(64) k_block_table(int32_t nNode, const int32_t* __restrict__ nbptr,
+                                                    const int32_t* __restrict__ nb,
+                                                    const int32_t* __restrict__ nodeinfo,
+                                                    const int32_t* __restrict__ nda, const int32_t* __restrict__ idp,
+                                                    int32_t dispDOF, const int32_t* __restrict__ rowptr,
+                                                    int32_t* __restrict__ btab) {
+  const int lane = threadIdx.x;
+  const int n = blockIdx.x;
+  if (n >= nNode) return;
+  const int info_n = nodeinfo[n];
+  const int mn = info_n & 7, La = info_n >> 4;
+  const bool isPn = MIXED && ((info_n >> 3) & 1);
+  const int b0 = nbptr[n], nnb = nbptr[n + 1] - b0;
+  const int fr = (mn & 1) ? 0 : (mn & 2) ? 1 : (mn & 4) ? 2 : -1;
+  const int RU = fr >= 0 ? rowptr[nda[3 * (size_t)n + fr]] : -1;
+  const int RP = isPn ? rowptr[dispDOF + idp[n]] : -1;
+  int Ucols = 0;
+  for (int c = 0; c < nnb; c += 64) {
+    const int j = c + lane;
+    int nu = j < nnb ? __popc(nodeinfo[nb[b0 + j]] & 7) : 0;
+    for (int off = 32; off > 0; off >>= 1) nu += __shfl_xor(nu, off, 64);
+    Ucols += nu;
+  }
   int cu0 = 0, cp0 = 0;
   for (int c = 0; c < nnb; c += 64) {
     const int j = c + lane;
     int inf = 0, nu = 0, np = 0;
     if (j < nnb) {
-      const int v = R.nb[b0 + j];
-      inf = R.nodeinfo[v];
-      sNB[j] = v;
-      sInfo[j] = inf;
+      inf = nodeinfo[nb[b0 + j]];
       nu = __popc(inf & 7);
       np = MIXED ? ((inf >> 3) & 1) : 0;
     }
@@ -236,154 +315,235 @@ __global__ void __launch_bounds__(64) k_rows(RowArgs R) {
       const int tu = __shfl_up(iu, off, 64), tp = __shfl_up(ip, off, 64);
       if (lane >= off) { iu += tu; ip += tp; }
     }
-    if (j < nnb) { sUpos[j] = cu0 + iu - nu; sPpos[j] = cp0 + ip - np; }
+    if (j < nnb) {
+      const int upos = cu0 + iu - nu, ppos = cp0 + ip - np;
+      const int mb = inf & 7;
+      const size_t blk = (size_t)b0 + j;
+      if (MIXED) {
+        const int isPb = (inf >> 3) & 1;
+        btab[4 * blk] = RU >= 0 ? RU + upos : -1;
+        btab[4 * blk + 1] = RP >= 0 ? RP + upos : -1;
+        btab[4 * blk + 2] = ((Ucols + ppos - upos) << 16) | La;
+        btab[4 * blk + 3] = ((int)isPn << 7) | (isPb << 6) | (mn << 3) | mb;
+      } else {
+        btab[2 * blk] = RU >= 0 ? RU + upos : -1;
+        btab[2 * blk + 1] = (La << 8) | (mn << 3) | mb;
+      }
+    }
     cu0 += __shfl(iu, 63, 64);
     cp0 += __shfl(ip, 63, 64);
   }
-  const int Ucols = cu0;
-  __syncthreads();
+}
 
-  double liftacc = 0.0;  // lanes < NL: sum over columns of K_{n j} dubar_j (first iteration)
-  if (R.writeK || R.dispApplied) {
-    const int nitems = m * NPE;
-    const int32_t* items = R.items + (size_t)e0 * NPE;
-    for (int p0 = 0; p0 < nitems; p0 += 64) {
-      const int t = p0 + lane;
-      const bool live = t < nitems;
-      const int key = live ? items[t] : -1;
-      const int j = live ? (key >> 16) : -1;
-      double v[NV];
+// Build step 4: near[n] = 1 if an element around n has a node with a prescribed dof (the first-
+// iteration lift of n's rows is non-zero only there).
+template <int NPE>
+__global__ void k_near(int32_t nNode, const int32_t* __restrict__ nptr, const int32_t* __restrict__ nel,
+                       const int32_t* __restrict__ conn, const int32_t* __restrict__ nodeinfo,
+                       int32_t* __restrict__ near) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nNode) return;
+  int f = 0;
+  for (int s = nptr[n]; s < nptr[n + 1] && !f; ++s) {
+    const int32_t* c = conn + (size_t)nel[s] * NPE;
+    for (int b = 0; b < NPE; ++b)
+      if ((nodeinfo[c[b]] & 7) != 7) f = 1;
+  }
+  near[n] = f;
+}
+
+// One (element, node b) item of node n's rows: the 3x3 block K_{n b} (v[0..8]) and, for the mixed
+// element, Kup(n, b) (v[9..11]), Kpu(n, b) (v[12..14]) and Kpp(n, b) (v[15]) from the element's
+// staging record: node n's g, b g at gA, node b's at gB (stride NPE * 6 per Gauss point, read as
+// 16-byte pairs), the Gauss-point scalars at sS (stride SS).
+template <int NPE, int NGP, bool MIXED, int NV, int SS>
+__device__ __forceinline__ void item_block(const double* gA, const double* sS, const double* gB, int a, int b,
+                                           double v[NV]) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) v[q] = 0.0;
-      if (live) {
-        const int s = (key >> 4) & 63, b = key & 15;
-        const int ea = sEA[s];
-        const int e = ea >> 4, a = ea & 15;
-        const double* rec = R.GS + (size_t)e * St::TS;
-        tangent_block<NGP, !MIXED>(rec + St::G0 + a * 6, rec + St::G0 + b * 6, NPE * 6, rec + St::S0, St::SS, v);
-        if (MIXED) {
-          const double* S = rec + St::S0;
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+#pragma unroll 2
+  for (int gp = 0; gp < NGP; ++gp) {
+    const double2* ga2 = reinterpret_cast<const double2*>(gA + gp * NPE * 6);
+    const double2* gb2 = reinterpret_cast<const double2*>(gB + gp * NPE * 6);
+    const double* S = sS + gp * SS;
+    const double2 p0 = ga2[0], p1 = ga2[1], p2 = ga2[2];
+    const double2 q0 = gb2[0], q1 = gb2[1], q2 = gb2[2];
+    const double ga[3] = {p0.x, p0.y, p1.x}, ba[3] = {p1.y, p2.x, p2.y};
+    const double gb[3] = {q0.x, q0.y, q1.x}, bb[3] = {q1.y, q2.x, q2.y};
+    tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], v);
+    if (MIXED) {
+      double Np[4];
+      tet_Np(gp, Np);
+      const double dv = S[4], dv0 = S[5], th = S[6];
+      const double Nb = b == 0 ? Np[0] : b == 1 ? Np[1] : b == 2 ? Np[2] : Np[3];
+      const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
+      if (b < 4) {  // Kup(3a+i, b) += (dv g_a,i) Np_b   (ElementBase2.cpp:500-505)
+        v[9] += (dv * ga[0]) * Nb; v[10] += (dv * ga[1]) * Nb; v[11] += (dv * ga[2]) * Nb;
+      }
+      if (a < 4) {  // Kpu(a, 3b+k) += (dv Np_a) g_b,k ; Kpp(a, b) -= (dv0 Np_a thetahat) Np_b (:524-539)
+        const double bb4 = dv * Na;
+        v[12] += bb4 * gb[0]; v[13] += bb4 * gb[1]; v[14] += bb4 * gb[2];
+        if (b < 4) v[15] -= ((dv0 * Na) * th) * Nb;
+      }
+    }
+  }
+}
+
+struct FlatArgs {
+  int64_t nwaves;
+  const int2* fitem;
+  const int32_t* btab;
+  const double* GS;
+  double* vals;
+};
+
+// Matrix kernel: one wavefront per 64 laid-out items.  Item lanes compute their blocks; the first
+// lane of each column run (its head) sums the run in element order and writes the column block of
+// n's rows, so every CSR value is written exactly once.
+template <int NPE, int NGP, bool MIXED>
+__global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
+  using St = Stage<NPE, NGP, MIXED>;
+  constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
+  constexpr int BT = MIXED ? 4 : 2;
+  __shared__ int32_t sB[64];
+  __shared__ double sPart[64][NV];
+  const int lane = threadIdx.x;
+  const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
+  if (w >= F.nwaves) return;
+  const int2 it = F.fitem[w * 64 + lane];
+  const bool live = it.x >= 0;
+  const int blk = live ? (it.y >> 4) : -1;
+  int bt[BT];
 #pragma unroll
+  for (int q = 0; q < BT; ++q) bt[q] = live ? F.btab[(size_t)BT * blk + q] : 0;
+  double v[NV];
+  if (live) {
+    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+    const double* rec = F.GS + (size_t)e * St::TS;
+    item_block<NPE, NGP, MIXED, NV, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a, b, v);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  }
+  sB[lane] = blk;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) sPart[lane][q] = v[q];
+  __syncthreads();
+  if (!live || (lane > 0 && sB[lane - 1] == blk)) return;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = v[q];
+  for (int t2 = lane + 1; t2 < 64 && sB[t2] == blk; ++t2)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += sPart[t2][q];
+  const int A = bt[0];
+  const int masks = MIXED ? bt[3] : bt[1];
+  const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 8);
+  const int mn = (masks >> 3) & 7, mb = masks & 7;
+  double* __restrict__ V = F.vals;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (!((mn >> i) & 1)) continue;
+    double* row = V + A + rank3(mn, i) * La;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if ((mb >> k) & 1) row[rank3(mb, k)] = acc[3 * i + k];
+    if (MIXED && ((masks >> 6) & 1)) row[bt[2] >> 16] = acc[9 + i];
+  }
+  if (MIXED && ((masks >> 7) & 1)) {
+    double* row = V + bt[1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
+    if ((masks >> 6) & 1) row[bt[2] >> 16] = acc[15];
+  }
+}
+
+struct RhsArgs {
+  int32_t nNode;
+  const int32_t* nptr;
+  const int32_t* nelA;
+  const int32_t* near;
+  const int32_t* conn;
+  const int32_t* nodeinfo;
+  const int32_t* nda;
+  const int32_t* idp;
+  int32_t dispDOF;
+  const double* GS;
+  const double* dispApplied;  // non-null: first-iteration lift
+  double* rhs;
+};
+
+// Right-hand-side kernel (one lane per node): n's rows of the element residuals, summed in element
+// order, minus the first-iteration lift  sum_e K_e(n, b) dubar_b  over the prescribed dofs of e's nodes
+// (applyDirichletBCs / applyDirichletBCs2field, ElementBase.cpp:567-594, 717-782).
+template <int NPE, int NGP, bool MIXED>
+__global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
+  using St = Stage<NPE, NGP, MIXED>;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= R.nNode) return;
+  const int info = R.nodeinfo[n];
+  const int mn = info & 7;
+  const bool isPn = MIXED && ((info >> 3) & 1);
+  if (!mn && !isPn) return;
+  const int e0 = R.nptr[n], m = R.nptr[n + 1] - e0;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < m; ++s) {
+    const int ea = R.nelA[e0 + s];
+    const int a = ea & 15;
+    const double* rec = R.GS + (size_t)(ea >> 4) * St::TS;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] += rec[St::R0 + a * 3 + i];
+    if (MIXED && isPn && a < 4) {
+      double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
+#pragma unroll
+      for (int gp = 0; gp < NGP; ++gp) {
+        double Np[4];
+        tet_Np(gp, Np);
+        const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
+        fp -= (rec[St::S0 + gp * St::SS + 5] * Na) * rec[St::S0 + gp * St::SS + 7];
+      }
+      r[3] += fp;
+    }
+  }
+  if (R.dispApplied && R.near[n]) {
+    double L[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < m; ++s) {
+      const int ea = R.nelA[e0 + s];
+      const int e = ea >> 4, a = ea & 15;
+      const double* rec = R.GS + (size_t)e * St::TS;
+      for (int b = 0; b < NPE; ++b) {
+        const int vb = R.conn[(size_t)e * NPE + b];
+        const int mb = R.nodeinfo[vb] & 7;
+        if (mb == 7) continue;
+        double db[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) db[k] = ((mb >> k) & 1) ? 0.0 : R.dispApplied[3 * (size_t)vb + k];
+        double Kb[9];
+        tangent_block<NGP, !MIXED>(rec + St::G0 + a * 6, rec + St::G0 + b * 6, NPE * 6, rec + St::S0, St::SS, Kb);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) L[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
+        if (MIXED && isPn && a < 4) {
+          double pu[3] = {0.0, 0.0, 0.0};  // Kpu(a, 3b+k) = sum (dv Np_a) g_b,k
           for (int gp = 0; gp < NGP; ++gp) {
             double Np[4];
             tet_Np(gp, Np);
-            const double* ga = rec + St::G0 + gp * NPE * 6 + a * 6;
-            const double* gb = rec + St::G0 + gp * NPE * 6 + b * 6;
-            const double dv = S[gp * St::SS + 4], dv0 = S[gp * St::SS + 5], th = S[gp * St::SS + 6];
-            const double Nb = b == 0 ? Np[0] : b == 1 ? Np[1] : b == 2 ? Np[2] : Np[3];
             const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
-            if (b < 4) {  // Kup(3a+i, b) += (dv g_a,i) Np_b   (ElementBase2.cpp:500-505)
-              v[9] += (dv * ga[0]) * Nb; v[10] += (dv * ga[1]) * Nb; v[11] += (dv * ga[2]) * Nb;
-            }
-            if (a < 4) {  // Kpu(a, 3b+k) += (dv Np_a) g_b,k ; Kpp(a, b) -= (dv0 Np_a thetahat) Np_b (:524-539)
-              const double bb4 = dv * Na;
-              v[12] += bb4 * gb[0]; v[13] += bb4 * gb[1]; v[14] += bb4 * gb[2];
-              if (b < 4) v[15] -= ((dv0 * Na) * th) * Nb;
-            }
+            const double bb4 = rec[St::S0 + gp * St::SS + 4] * Na;
+            const double* gb = rec + St::G0 + gp * NPE * 6 + b * 6;
+            pu[0] += bb4 * gb[0]; pu[1] += bb4 * gb[1]; pu[2] += bb4 * gb[2];
           }
-        }
-      }
-      sJ[lane] = j;
-      if (lane == 0) sJ[64] = (p0 + 64 < nitems) ? (items[p0 + 64] >> 16) : -1;
-#pragma unroll
-      for (int q = 0; q < NV; ++q) sPart[lane][q] = v[q];
-      __syncthreads();
-      bool hasLift = false;
-      if (live && (lane == 0 || sJ[lane - 1] != j)) {  // head of a column run
-        double acc[NV];
-        const bool carried = lane == 0 && p0 > 0 && (items[p0 - 1] >> 16) == j;
-#pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] = carried ? sCarry[q] + v[q] : v[q];
-        int t2 = lane + 1;
-        for (; t2 < 64 && sJ[t2] == j; ++t2)
-#pragma unroll
-          for (int q = 0; q < NV; ++q) acc[q] += sPart[t2][q];
-        if (t2 == 64 && sJ[64] == j) {
-          // the run continues in the next pass (its lane 0 reads this after the pass-end barrier)
-#pragma unroll
-          for (int q = 0; q < NV; ++q) sCarry[q] = acc[q];
-        } else {
-          const int inf = sInfo[j];
-          const int mb = inf & 7;
-          const bool isPb = MIXED && ((inf >> 3) & 1);
-          if (R.writeK) {
-            const int up = sUpos[j], pp = Ucols + sPpos[j];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              if (!((mn >> i) & 1)) continue;
-              double* row = R.vals + R.rowptr[R.nda[3 * (size_t)n + i]];
-#pragma unroll
-              for (int k = 0; k < 3; ++k)
-                if ((mb >> k) & 1) row[up + rank3(mb, k)] = acc[3 * i + k];
-              if (isPb) row[pp] = acc[9 + i];
-            }
-            if (isPn) {
-              double* row = R.vals + R.rowptr[R.dispDOF + R.idp[n]];
-#pragma unroll
-              for (int k = 0; k < 3; ++k)
-                if ((mb >> k) & 1) row[up + rank3(mb, k)] = acc[12 + k];
-              if (isPb) row[pp] = acc[15];
-            }
-          }
-          if (R.dispApplied && mb != 7) {  // F_n -= K_{n j} dubar_j over the prescribed dofs of j
-            const int vb = sNB[j];
-            double db[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) db[k] = ((mb >> k) & 1) ? 0.0 : R.dispApplied[3 * (size_t)vb + k];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) sLift[lane][i] = acc[3 * i] * db[0] + acc[3 * i + 1] * db[1] + acc[3 * i + 2] * db[2];
-            if (MIXED) sLift[lane][NL - 1] = acc[12] * db[0] + acc[13] * db[1] + acc[14] * db[2];
-            hasLift = true;
-          }
-        }
-      }
-      const uint64_t lm = __ballot(hasLift);
-      __syncthreads();
-      if (lane < NL) {
-        uint64_t mm = lm;
-        while (mm) {
-          const int h = __ffsll((unsigned long long)mm) - 1;
-          liftacc += sLift[h][lane];
-          mm &= mm - 1;
+          L[3] += pu[0] * db[0] + pu[1] * db[1] + pu[2] * db[2];
         }
       }
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] -= L[i];
   }
-
-  // right-hand side of n: element residuals in element order, minus the lift
-  if (lane < NL) {
-    const int i = lane;
-    double r = 0.0;
-    bool own = false;
-    if (i < 3) {
-      own = (mn >> i) & 1;
-      if (own)
-        for (int s = 0; s < m; ++s) {
-          const int ea = sEA[s];
-          r += R.GS[(size_t)(ea >> 4) * St::TS + St::R0 + (ea & 15) * 3 + i];
-        }
-    } else if (isPn) {
-      own = true;
-      for (int s = 0; s < m; ++s) {
-        const int ea = sEA[s];
-        const int a = ea & 15;
-        if (a >= 4) continue;
-        const double* S = R.GS + (size_t)(ea >> 4) * St::TS + St::S0;
-        double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
-        for (int gp = 0; gp < NGP; ++gp) {
-          double Np[4];
-          tet_Np(gp, Np);
-          const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
-          fp -= (S[gp * St::SS + 5] * Na) * S[gp * St::SS + 7];
-        }
-        r += fp;
-      }
-    }
-    if (own) {
-      const int row = i < 3 ? R.nda[3 * (size_t)n + i] : R.dispDOF + R.idp[n];
-      R.rhs[row] = r - liftacc;
-    }
-  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if ((mn >> i) & 1) R.rhs[R.nda[3 * (size_t)n + i]] = r[i];
+  if (isPn) R.rhs[R.dispDOF + R.idp[n]] = r[3];
 }
 
 }  // namespace
@@ -392,65 +552,164 @@ size_t stage_doubles_per_elem(int etype) {
   return etype == MPF_HEX8 ? (size_t)Stage<8, 8, false>::TS : (size_t)Stage<10, 4, true>::TS;
 }
 
+static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
+  std::fprintf(stderr, "libmpf: %s; using the coloured scatter\n", why);
+  PatternBufs& P = p->pat;
+  P.g_order.release();
+  P.g_nelA.release();
+  P.g_fitem.release();
+  P.g_btab.release();
+  P.g_near.release();
+  p->GS.release();
+  p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
+  return MPF_OK;
+}
+
 mpf_status ensure_gather_tables(mpf_problem* p) {
   PatternBufs& P = p->pat;
   if (P.gather_ready) return MPF_OK;
   hipStream_t s = p->stream;
-  const int64_t nadj = (int64_t)p->nElem * p->npe;
-  if (p->nElem >= (1 << 27)) {  // nelA packs e * 16 + a into int32
-    std::fprintf(stderr, "libmpf: more than 2^27 elements; using the coloured scatter\n");
-    p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
-    return MPF_OK;
+  const int npe = p->npe, nNode = p->nNode;
+  const bool mixed = p->etype == MPF_TET10P1;
+  const int64_t nadj = (int64_t)p->nElem * npe;
+  int64_t nbtot = 0;
+  {
+    int32_t last = 0;
+    if (nNode > 0) MPF_HIP(hipMemcpyAsync(&last, P.nbptr.ptr + nNode, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    nbtot = last;
   }
-  MPF_TRY(P.g_nelA.alloc(nadj > 0 ? nadj : 1));
-  MPF_TRY(P.g_items.alloc(nadj > 0 ? (size_t)nadj * p->npe : 1));
-  MPF_TRY(p->GS.alloc((size_t)p->nElem * stage_doubles_per_elem(p->etype)));
-  DevBuf<int32_t> ovf;
+  if (p->nElem >= (1 << 27)) return gather_unsupported(p, "more than 2^27 elements");
+  if (nbtot >= (1 << 27)) return gather_unsupported(p, "more than 2^27 node-pair blocks");
+  if (nNode == 0 || p->nElem == 0) return gather_unsupported(p, "empty mesh");
+
+  // processing order: Morton order of the node coordinates
+  MPF_TRY(P.g_order.alloc(nNode));
+  {
+    DevBuf<uint64_t> code, code2;
+    DevBuf<int32_t> ids;
+    MPF_TRY(code.alloc(nNode));
+    MPF_TRY(code2.alloc(nNode));
+    MPF_TRY(ids.alloc(nNode));
+    double sc[3];
+    for (int c = 0; c < 3; ++c) {
+      const double ext = p->xmax[c] - p->xmin[c];
+      sc[c] = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    }
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, p->X.ptr, p->xmin[0], p->xmin[1],
+                       p->xmin[2], sc[0], sc[1], sc[2], code.ptr, ids.ptr);
+    size_t tmp = 0;
+    MPF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, code.ptr, code2.ptr, ids.ptr, P.g_order.ptr, nNode, 0, 63,
+                                               s));
+    DevBuf<char> t;
+    MPF_TRY(t.alloc(tmp > 0 ? tmp : 1));
+    MPF_HIP(hipcub::DeviceRadixSort::SortPairs(t.ptr, tmp, code.ptr, code2.ptr, ids.ptr, P.g_order.ptr, nNode, 0, 63,
+                                               s));
+    MPF_HIP(hipStreamSynchronize(s));
+  }
+
+  // step 1: sorted items per node, laid-out lengths
+  DevBuf<int32_t> keys, ovf;
+  DevBuf<int64_t> flen, tmp1, tmp2, foff;
+  MPF_TRY(P.g_nelA.alloc(nadj));
+  MPF_TRY(keys.alloc((size_t)nadj * npe));
+  MPF_TRY(flen.alloc(nNode));
   MPF_TRY(ovf.alloc(1));
   MPF_HIP(hipMemsetAsync(ovf.ptr, 0, sizeof(int32_t), s));
-  if (p->nNode > 0) {
-    if (p->etype == MPF_HEX8)
-      hipLaunchKernelGGL(k_gather_tables<8>, dim3(p->nNode), dim3(64), 0, s, p->nNode, P.nptr.ptr, P.nel.ptr,
-                         P.nbptr.ptr, P.nb.ptr, p->conn.ptr, P.g_nelA.ptr, P.g_items.ptr, ovf.ptr);
-    else
-      hipLaunchKernelGGL(k_gather_tables<10>, dim3(p->nNode), dim3(64), 0, s, p->nNode, P.nptr.ptr, P.nel.ptr,
-                         P.nbptr.ptr, P.nb.ptr, p->conn.ptr, P.g_nelA.ptr, P.g_items.ptr, ovf.ptr);
-    MPF_HIP(hipGetLastError());
-  }
+  if (mixed)
+    hipLaunchKernelGGL(k_node_items<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
+                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, flen.ptr, ovf.ptr);
+  else
+    hipLaunchKernelGGL(k_node_items<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
+                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, flen.ptr, ovf.ptr);
+  MPF_HIP(hipGetLastError());
   int32_t o = 0;
   MPF_HIP(hipMemcpyAsync(&o, ovf.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   MPF_HIP(hipStreamSynchronize(s));
-  if (o) {
-    std::fprintf(stderr, "libmpf: a node has more than %d elements or %d coupled nodes; using the coloured scatter\n",
-                 kMaxAdj, kMaxNB);
-    P.g_nelA.release();
-    P.g_items.release();
-    p->GS.release();
-    p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
-    return MPF_OK;
+  if (o & 1) return gather_unsupported(p, "a node has more than 64 elements or 128 coupled nodes");
+  if (o & 2) return MPF_ERR_STATE;
+  if (o & 4) return gather_unsupported(p, "a node's laid-out items exceed the row-kernel capacity");
+
+  // offsets in Morton order of the nodes
+  MPF_TRY(tmp1.alloc(nNode + 1));
+  MPF_TRY(tmp2.alloc(nNode + 1));
+  MPF_TRY(foff.alloc(nNode));
+  hipLaunchKernelGGL(k_gather_len, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.g_order.ptr, flen.ptr,
+                     tmp1.ptr);
+  MPF_HIP(hipMemsetAsync(tmp1.ptr + nNode, 0, sizeof(int64_t), s));
+  {
+    size_t tmp = 0;
+    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, tmp1.ptr, tmp2.ptr, nNode + 1, s));
+    DevBuf<char> t;
+    MPF_TRY(t.alloc(tmp > 0 ? tmp : 1));
+    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(t.ptr, tmp, tmp1.ptr, tmp2.ptr, nNode + 1, s));
   }
+  hipLaunchKernelGGL(k_scatter_off, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.g_order.ptr, tmp2.ptr,
+                     foff.ptr);
+  int64_t total = 0;
+  MPF_HIP(hipMemcpyAsync(&total, tmp2.ptr + nNode, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+
+  // step 2: laid-out items
+  MPF_TRY(P.g_fitem.alloc(total > 0 ? (size_t)total : 64));
+  if (total == 0) MPF_HIP(hipMemsetAsync(P.g_fitem.ptr, 0xff, 64 * sizeof(int2), s));
+  if (mixed)
+    hipLaunchKernelGGL(k_flat_fill<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
+                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr);
+  else
+    hipLaunchKernelGGL(k_flat_fill<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
+                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr);
+  P.g_nwaves = total / 64;
+
+  // step 3: column-block table; step 4: near-Dirichlet flags
+  MPF_TRY(P.g_btab.alloc((size_t)(nbtot > 0 ? nbtot : 1) * (mixed ? 4 : 2)));
+  if (mixed)
+    hipLaunchKernelGGL(k_block_table<true>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
+                       P.nda.ptr, P.idp.ptr, P.dispDOF, P.rowptr.ptr, P.g_btab.ptr);
+  else
+    hipLaunchKernelGGL(k_block_table<false>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
+                       P.nda.ptr, nullptr, P.dispDOF, P.rowptr.ptr, P.g_btab.ptr);
+  MPF_TRY(P.g_near.alloc(nNode));
+  if (mixed)
+    hipLaunchKernelGGL(k_near<10>, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
+                       p->conn.ptr, P.nodeinfo.ptr, P.g_near.ptr);
+  else
+    hipLaunchKernelGGL(k_near<8>, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
+                       p->conn.ptr, P.nodeinfo.ptr, P.g_near.ptr);
+  MPF_HIP(hipGetLastError());
+  MPF_TRY(p->GS.alloc((size_t)p->nElem * stage_doubles_per_elem(p->etype)));
+  MPF_HIP(hipStreamSynchronize(s));
   P.gather_ready = true;
   return MPF_OK;
 }
 
-mpf_status launch_gather_assembly(int etype, const ElemArgs& A, int32_t nElem, int32_t nNode, const PatternBufs& P,
-                                  double* GS, double* vals, double* rhs, const double* dispApplied, int writeK,
-                                  int32_t* overflow, hipStream_t s) {
+mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const double* dispApplied, int writeK,
+                                  hipStream_t s) {
+  const PatternBufs& P = p->pat;
   if (!P.gather_ready) return MPF_ERR_STATE;
-  RowArgs R{};
-  R.nNode = nNode;
-  R.nptr = P.nptr.ptr; R.nelA = P.g_nelA.ptr; R.items = P.g_items.ptr; R.nbptr = P.nbptr.ptr; R.nb = P.nb.ptr;
-  R.nodeinfo = P.nodeinfo.ptr; R.nda = P.nda.ptr; R.idp = P.idp.ptr; R.dispDOF = P.dispDOF;
-  R.rowptr = P.rowptr.ptr; R.GS = GS; R.dispApplied = dispApplied; R.vals = vals; R.rhs = rhs;
-  R.writeK = writeK; R.overflow = overflow;
-  const unsigned nodes_grid = (unsigned)(((nNode + 7) / 8) * 8);
+  const int32_t nElem = p->nElem, nNode = p->nNode;
   if (nElem <= 0 || nNode <= 0) return MPF_OK;
-  if (etype == MPF_HEX8) {
-    hipLaunchKernelGGL((k_stage<8, 8, false, 8>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A, nElem, GS);
-    hipLaunchKernelGGL((k_rows<8, 8, false>), dim3(nodes_grid), dim3(64), 0, s, R);
+  const bool mixed = p->etype == MPF_TET10P1;
+  FlatArgs F{};
+  F.nwaves = P.g_nwaves;
+  F.fitem = P.g_fitem.ptr;
+  F.btab = P.g_btab.ptr;
+  F.GS = p->GS.ptr;
+  F.vals = p->vals.ptr;
+  RhsArgs R{};
+  R.nNode = nNode;
+  R.nptr = P.nptr.ptr; R.nelA = P.g_nelA.ptr; R.near = P.g_near.ptr; R.conn = p->conn.ptr;
+  R.nodeinfo = P.nodeinfo.ptr; R.nda = P.nda.ptr; R.idp = P.idp.ptr; R.dispDOF = P.dispDOF;
+  R.GS = p->GS.ptr; R.dispApplied = dispApplied; R.rhs = p->rhs.ptr;
+  const unsigned flat_grid = (unsigned)(((F.nwaves + 7) / 8) * 8);
+  if (!mixed) {
+    hipLaunchKernelGGL((k_stage<8, 8, false, 8>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A, nElem, p->GS.ptr);
+    if (writeK && F.nwaves > 0) hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
+    hipLaunchKernelGGL((k_rhs<8, 8, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
   } else {
-    hipLaunchKernelGGL((k_stage<10, 4, true, 16>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A, nElem, GS);
-    hipLaunchKernelGGL((k_rows<10, 4, true>), dim3(nodes_grid), dim3(64), 0, s, R);
+    hipLaunchKernelGGL((k_stage<10, 4, true, 16>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A, nElem, p->GS.ptr);
+    if (writeK && F.nwaves > 0) hipLaunchKernelGGL((k_flat<10, 4, true>), dim3(flat_grid), dim3(64), 0, s, F);
+    hipLaunchKernelGGL((k_rhs<10, 4, true>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
   }
   MPF_HIP(hipGetLastError());
   return MPF_OK;

@@ -125,8 +125,13 @@ struct PatternBufs {
   DevBuf<int32_t> nodeinfo;   // (row length << 4) | (pressure flag << 3) | free mask
   DevBuf<int32_t> color, perm;
   DevBuf<int32_t> slot;       // per-element slot bases
-  DevBuf<int32_t> g_nelA;     // gather tables: elements around each node (ascending) with the node's local index
-  DevBuf<int32_t> g_items;    // gather tables: per node, (column, element, node) items in (column, element) order
+  // gather-assembly tables (gather.hip)
+  DevBuf<int32_t> g_order;    // nodes in Morton (Z-curve) order of their coordinates
+  DevBuf<int32_t> g_nelA;     // elements around each node (ascending), with the node's local index
+  DevBuf<int2> g_fitem;       // laid-out (element, node) items of every node's rows, 64 per wavefront
+  DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j)
+  DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
+  int64_t g_nwaves = 0;
   bool gather_ready = false;
   std::vector<int32_t> color_off;  // host: [ncolors + 1] offsets into perm
   int32_t dispDOF = 0, presDOF = 0, ncolors = 0;
@@ -142,6 +147,7 @@ struct mpf_problem {
   mpf_matl m{};
   hipStream_t stream = nullptr;
   mpf::DevBuf<double> X;
+  double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};  // bounding box of the nodes
   mpf::DevBuf<int32_t> conn;
   // Dirichlet data (host lists, merged last-wins per dof like setBoundaryConditions)
   std::vector<int32_t> bc_dof_h;
@@ -172,9 +178,8 @@ struct mpf_problem {
 namespace mpf {
 size_t stage_doubles_per_elem(int etype);          // gather.hip
 mpf_status ensure_gather_tables(mpf_problem* p);   // gather.hip (falls back to the scatter when unsupported)
-mpf_status launch_gather_assembly(int etype, const ElemArgs& A, int32_t nElem, int32_t nNode, const PatternBufs& P,
-                                  double* GS, double* vals, double* rhs, const double* dispApplied, int writeK,
-                                  int32_t* overflow, hipStream_t s);
+mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const double* dispApplied, int writeK,
+                                  hipStream_t s);
 mpf_status build_pattern(mpf_problem* p);          // pattern.hip
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
 void krylov_free(mpf_problem* p);

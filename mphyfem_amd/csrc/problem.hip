@@ -210,8 +210,7 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
   const PatternBufs& P = p->pat;
   ElemArgs A = elem_args(p);
   if (p->asm_mode == MPF_ASSEMBLY_GATHER) {
-    return launch_gather_assembly(p->etype, A, p->nElem, p->nNode, P, p->GS.ptr, p->vals.ptr, p->rhs.ptr,
-                                  first_iter ? p->dispApplied.ptr : nullptr, scatterK, p->bad.ptr + 1, s);
+    return launch_gather_assembly(p, A, first_iter ? p->dispApplied.ptr : nullptr, scatterK, s);
   }
   ScatterArgs sc{};
   sc.slot = P.slot.ptr;
@@ -251,21 +250,15 @@ mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int sca
   return MPF_OK;
 }
 
-// bad[0]: 1 + element with det F < 0; bad[1]: gather-kernel adjacency overflow.
+// bad[0]: 1 + element with det F < 0 (smallest id).
 mpf_status assemble_sync(mpf_problem* p, bool first_iter, double lam, int scatterK) {
-  MPF_HIP(hipMemsetAsync(p->bad.ptr, 0, 2 * sizeof(int32_t), p->stream));
+  MPF_HIP(hipMemsetAsync(p->bad.ptr, 0, sizeof(int32_t), p->stream));
   MPF_TRY(enqueue_assembly(p, first_iter, lam, scatterK));
-  int32_t bad[2] = {0, 0};
-  MPF_HIP(hipMemcpyAsync(bad, p->bad.ptr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  int32_t bad = 0;
+  MPF_HIP(hipMemcpyAsync(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
   MPF_HIP(hipStreamSynchronize(p->stream));
-  if (bad[1]) {
-    // a node has more neighbours than the row kernel holds: use the coloured scatter from now on
-    std::fprintf(stderr, "libmpf: node adjacency exceeds the gather kernel's limits; using the coloured scatter\n");
-    p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
-    return assemble_sync(p, first_iter, lam, scatterK);
-  }
   p->assembled = true;
-  if (bad[0]) return MPF_ERR_NEG_JACOBIAN;
+  if (bad) return MPF_ERR_NEG_JACOBIAN;
   return MPF_OK;
 }
 
@@ -465,6 +458,15 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   hipLaunchKernelGGL(k_fill_d, dim3(g256(nNode)), dim3(256), 0, p->stream, p->pres.ptr, (int64_t)nNode, 0.0);
   hipLaunchKernelGGL(k_fill_d, dim3(g256(nNode)), dim3(256), 0, p->stream, p->presPrev.ptr, (int64_t)nNode, 0.0);
   hostX_store()[p].assign(X, X + n3);
+  for (int c = 0; c < 3; ++c) {
+    p->xmin[c] = nNode > 0 ? X[c] : 0.0;
+    p->xmax[c] = p->xmin[c];
+  }
+  for (int64_t i = 0; i < (int64_t)nNode; ++i)
+    for (int c = 0; c < 3; ++c) {
+      p->xmin[c] = std::min(p->xmin[c], X[3 * i + c]);
+      p->xmax[c] = std::max(p->xmax[c], X[3 * i + c]);
+    }
   if (hipStreamSynchronize(p->stream) != hipSuccess) {
     mpf_problem_destroy(p);
     return MPF_ERR_HIP;
