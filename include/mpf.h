@@ -117,12 +117,13 @@ mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, in
 mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1);
 
 /* Assembly algorithm used by mpf_assemble (same K and R either way):                           */
-/*   MPF_ASSEMBLY_GATHER (default): per-element Gauss-point staging, then one wavefront per node    */
-/*     sums its rows' contributions in element order and writes each CSR value once.             */
+/*   MPF_ASSEMBLY_GATHER (default): per-element Gauss-point staging, then the (element, node)     */
+/*     contributions to each node's rows are summed in element order and every CSR value and    */
+/*     rhs entry is written once (no zero fill, no atomics, bitwise reproducible).              */
 /*   MPF_ASSEMBLY_COLORED_SCATTER: elements of one colour per launch add their K_e/R_e blocks into  */
 /*     CSR through the per-element slot map (no two elements of a launch share a node).           */
-/* The gather kernel handles up to 64 elements and 128 coupled nodes per node; beyond that the   */
-/* problem switches itself to the coloured scatter.                                              */
+/* The gather tables hold up to 64 elements and 128 coupled nodes per node; for a mesh beyond    */
+/* that the problem switches itself to the coloured scatter (mpf_get_assembly_mode reports it).  */
 enum { MPF_ASSEMBLY_GATHER = 0, MPF_ASSEMBLY_COLORED_SCATTER = 1 };
 mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode);
 mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode);
@@ -146,7 +147,8 @@ mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres);
 mpf_status mpf_set_bc(mpf_problem* p, double lam);
 
 /* calcStiffnessAndResidual + addExternalForces (femSolidmechanics2.cpp:1047-1120, :785-789):    */
-/* zeroMtx, coloured atomic-free element scatter into CSR, Dirichlet lift when first_iter.        */
+/* K and R of every element into the CSR system (zeroMtx + assembleMatrixAndVector), Dirichlet  */
+/* lift when first_iter, external load scaled by lam.                                           */
 /* Returns MPF_ERR_NEG_JACOBIAN if any element has det F < 0.                                    */
 mpf_status mpf_assemble(mpf_problem* p, int32_t first_iter, double lam);
 /* Device pointers of the assembled system (valid until destroy): vals [nnz], rhs [n].          */
@@ -211,6 +213,47 @@ mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* i
 /* the lifted increment).  iters receives the Newton iterations of this step.                    */
 mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, const double* guess,
                            int32_t* iters, mpf_newton_report* rep);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Batched independent meshes.  No counterpart in the reference: many femSolidmechanics problems  */
+/* (Hex8, one Neo-Hookean material each) solved together.  The problem's mesh is the union of     */
+/* nmesh meshes: mesh m owns nodes [node_off[m], node_off[m+1]) and elements [elem_off[m],        */
+/* elem_off[m+1]) (no element may reference another mesh's node), with material mu[m], Kinv[m]   */
+/* (Utype and tangent mode from the problem's mpf_matl).  Call after mpf_problem_create.         */
+mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
+                         const double* mu, const double* Kinv);
+
+typedef struct {
+  int32_t nsteps;         /* load steps, lambda(t) = t */
+  double dt;
+  int32_t itmax;          /* Newton iterations per step */
+  double rtol;            /* per mesh: ||R_m|| / ||R0_m|| < rtol, R0_m at the plain start of the step */
+  int32_t cg_maxit;       /* CG iterations per solve */
+  double cg_rtol;         /* lower bound of the inexact-Newton CG tolerance min(0.1, 0.1 ||R||/||R0||) */
+  int32_t use_graph;      /* 1: replay captured HIP graphs; 0: launch the same kernels directly */
+  int32_t chunk;          /* CG iterations per graph launch (0: keep) */
+  int32_t tile;           /* dofs per tile (0: keep the autotuned / default value) */
+  int32_t lanes_per_row;  /* SpMV lanes per row, 4, 8 or 16 (0: keep) */
+} mpf_batch_opts;
+
+typedef struct {
+  int32_t steps_done;
+  int32_t newton_iters;     /* batch Newton iterations (solves), all steps */
+  int64_t cg_iters_total;   /* CG iterations summed over meshes */
+  int64_t cg_iters_max;     /* CG iterations of the slowest mesh, summed over solves */
+  int32_t graph_launches;
+  int32_t tile, lanes_per_row, chunk;
+  double seconds, seconds_assembly, seconds_solve;
+} mpf_batch_report;
+
+/* Jacobi-PCG tile shape by measurement: every (tile, lanes_per_row) candidate is timed over a     */
+/* few CG iterations of the batch's own system; the fastest is kept for mpf_batch_newton_solve.   */
+mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter);
+/* Newton-Raphson over nsteps load steps for every mesh at once (femSolidmechanics2.cpp:730-851  */
+/* per mesh) with Jacobi-PCG per mesh.  mesh_iters [nsteps][nmesh]: assemblies to convergence;    */
+/* mesh_relres [nmesh]: final ||R_m|| / ||R0_m||.  Either may be NULL.                           */
+mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters,
+                                  double* mesh_relres, mpf_batch_report* rep);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Timing helpers: average device time (ms) of one assembly (element kernels + scatter, no       */

@@ -69,6 +69,19 @@ class NewtonReport(C.Structure):
                 ("seconds_solve", C.c_double)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("dt", C.c_double), ("itmax", C.c_int32), ("rtol", C.c_double),
+                ("cg_maxit", C.c_int32), ("cg_rtol", C.c_double), ("use_graph", C.c_int32), ("chunk", C.c_int32),
+                ("tile", C.c_int32), ("lanes_per_row", C.c_int32)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("steps_done", C.c_int32), ("newton_iters", C.c_int32), ("cg_iters_total", C.c_int64),
+                ("cg_iters_max", C.c_int64), ("graph_launches", C.c_int32), ("tile", C.c_int32),
+                ("lanes_per_row", C.c_int32), ("chunk", C.c_int32), ("seconds", C.c_double),
+                ("seconds_assembly", C.c_double), ("seconds_solve", C.c_double)]
+
+
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
@@ -103,6 +116,9 @@ SIGNATURES = [
     ("mpf_newton_solve", C.c_int, [_P, C.POINTER(NewtonOpts), _I32P, C.POINTER(NewtonReport)]),
     ("mpf_newton_step", C.c_int, [_P, C.c_double, C.POINTER(NewtonOpts), _DP, _I32P, C.POINTER(NewtonReport)]),
     ("mpf_time_assembly", C.c_int, [_P, C.c_int32, C.c_int32, C.c_double, _DP, _DP]),
+    ("mpf_set_batch", C.c_int, [_P, C.c_int32, _I32P, _I32P, _DP, _DP]),
+    ("mpf_batch_autotune", C.c_int, [_P, _I32P, _I32P, _DP]),
+    ("mpf_batch_newton_solve", C.c_int, [_P, C.POINTER(BatchOpts), _I32P, _DP, C.POINTER(BatchReport)]),
     ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
@@ -210,6 +226,83 @@ def hex_face_quads(nx: int, ny: int, nz: int, face: int) -> np.ndarray:
     out = np.zeros((cnt, 4), dtype=np.int32)
     lib().mpf_mesh_hex_face_quads(nx, ny, nz, face, _ip(out))
     return out
+
+
+class SplitMix64:
+    """The project's seeded generator: SplitMix64, u = (x >> 11) * 2^-53 (same as csrc/mesh.cpp)."""
+
+    def __init__(self, seed: int):
+        self.x = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.x = (self.x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+@dataclass
+class HexBatch:
+    """Union of independent unit-cube Hex8 blocks (mesh m: nodes node_off[m]:node_off[m+1], ...)."""
+    X: np.ndarray
+    conn: np.ndarray
+    node_off: np.ndarray
+    elem_off: np.ndarray
+    edges: np.ndarray
+    bottom: list          # per mesh: node ids of the z = 0 face (global numbering)
+    top: list             # per mesh: Quad4 faces of the z = 1 face, outward normal (global numbering)
+
+
+def mesh_batch_hex(edges, jitter: float = 0.0, seed: int = 3) -> HexBatch:
+    Xs, Cs, bottom, top = [], [], [], []
+    node_off, elem_off = [0], [0]
+    for m, n in enumerate(edges):
+        n = int(n)
+        X, conn = mesh_hex(n, n, n, jitter=jitter, seed=seed * 1000003 + m)
+        o = node_off[-1]
+        Xs.append(X)
+        Cs.append(conn + o)
+        bottom.append(face_nodes(HEX8, n, n, n, 4) + o)
+        top.append(hex_face_quads(n, n, n, 5) + o)
+        node_off.append(o + len(X))
+        elem_off.append(elem_off[-1] + len(conn))
+    return HexBatch(np.concatenate(Xs), np.concatenate(Cs).astype(np.int32), np.array(node_off, dtype=np.int32),
+                    np.array(elem_off, dtype=np.int32), np.array(edges, dtype=np.int32), bottom, top)
+
+
+def batch_workload(nmesh: int = 256, nmin: int = 8, nmax: int = 32, seed: int = 3):
+    """The batched workload: edge n ~ U{nmin..nmax}, E log-uniform [1e2, 1e4], nu ~ U[0.2, 0.45],
+    top-face dead traction of magnitude tau ~ U[0.01, 0.1] E at angle theta ~ U[0, 30 deg] from the
+    inward normal, in the reference's (normal, tangent1) form: v0 = -tau cos(theta), v1 = tau sin(theta).
+    Returns (edges, E, nu, v0, v1)."""
+    g = SplitMix64(seed)
+    edges, E, nu, v0, v1 = [], [], [], [], []
+    for _ in range(nmesh):
+        edges.append(nmin + min(int(g.uniform() * (nmax - nmin + 1)), nmax - nmin))
+        e = 10.0 ** (2.0 + 2.0 * g.uniform())
+        E.append(e)
+        nu.append(0.2 + 0.25 * g.uniform())
+        tau = (0.01 + 0.09 * g.uniform()) * e
+        th = g.uniform() * np.pi / 6.0
+        v0.append(-tau * np.cos(th))
+        v1.append(tau * np.sin(th))
+    return np.array(edges), np.array(E), np.array(nu), np.array(v0), np.array(v1)
+
+
+def batch_problem(edges, E, nu, v0, v1, tangent: int = TANGENT_CONSISTENT, Utype: int = 3):
+    """Problem holding the batch: bottom faces fixed, top faces under their dead traction."""
+    B = mesh_batch_hex(edges)
+    P = Problem(HEX8, B.X, B.conn, matl_from_E_nu(float(E[0]), float(nu[0]), Utype, False, tangent))
+    E, nu = np.asarray(E, dtype=np.float64), np.asarray(nu, dtype=np.float64)
+    P.set_batch(B.node_off, B.elem_off, E / (2.0 * (1.0 + nu)), 3.0 * (1.0 - 2.0 * nu) / E)
+    for m in range(len(edges)):
+        P.add_dirichlet(B.bottom[m], -1)
+        P.add_traction(B.top[m], float(v0[m]), float(v1[m]))
+    return B, P
 
 
 # ---------------------------------------------------------------------------------------------
@@ -421,6 +514,34 @@ class Problem:
         v = C.c_int32()
         _check(lib().mpf_get_assembly_mode(self._h, C.byref(v)), "mpf_get_assembly_mode")
         return v.value
+
+    # ---- batched independent meshes ----
+    def set_batch(self, node_off, elem_off, mu, Kinv):
+        """The problem's mesh is the union of len(mu) independent meshes (Hex8), one material each."""
+        node_off = np.ascontiguousarray(node_off, dtype=np.int32)
+        elem_off = np.ascontiguousarray(elem_off, dtype=np.int32)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        Kinv = np.ascontiguousarray(Kinv, dtype=np.float64)
+        self.nmesh = len(mu)
+        _check(lib().mpf_set_batch(self._h, self.nmesh, _ip(node_off), _ip(elem_off), _dp(mu), _dp(Kinv)),
+               "mpf_set_batch")
+
+    def batch_autotune(self):
+        """(tile, lanes per row, ms per CG iteration) chosen by timing the candidates on this batch."""
+        t, l, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().mpf_batch_autotune(self._h, C.byref(t), C.byref(l), C.byref(ms)), "mpf_batch_autotune")
+        return t.value, l.value, ms.value
+
+    def batch_newton_solve(self, nsteps=1, dt=1.0, itmax=20, rtol=1e-9, cg_maxit=5000, cg_rtol=1e-10,
+                           use_graph=True, chunk=0, tile=0, lanes_per_row=0):
+        """Newton per mesh over nsteps load steps; returns (iters[nsteps][nmesh], relres[nmesh], BatchReport)."""
+        o = BatchOpts(nsteps, dt, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row)
+        iters = np.zeros((nsteps, self.nmesh), dtype=np.int32)
+        rel = np.zeros(self.nmesh)
+        rep = BatchReport()
+        st = lib().mpf_batch_newton_solve(self._h, C.byref(o), _ip(iters), _dp(rel), C.byref(rep))
+        _check(st, "mpf_batch_newton_solve")
+        return iters, rel, rep
 
     def time_assembly(self, reps: int = 20, first_iter: bool = True, lam: float = 1.0):
         """(ms per full assembly, ms of the element-kernel launches per assembly), HIP events."""

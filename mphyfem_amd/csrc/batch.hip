@@ -1,0 +1,669 @@
+// batch.hip -- many independent meshes solved as one problem: batched Newton with batched
+// Jacobi-PCG, replayed from HIP graphs, with tile shapes chosen by a measuring autotuner.
+//
+// The meshes are concatenated into one problem (disjoint node and element ranges, a material per
+// mesh), so pattern, colouring and assembly are the single-mesh ones (the CSR matrix is block
+// diagonal).  What is per mesh is the solver: every mesh runs its own conjugate-gradient recurrence
+// (its own alpha, beta and stopping test: the same iterates as a CG on that mesh alone) and its own
+// Newton convergence test ||R_m|| / ||R0_m|| < rtol, with R0_m taken at the plain start of the load
+// step (femSolidmechanics2.cpp:730-851 per mesh; SolverPetsc.cpp:250-330 with KSPCG + PCJACOBI).
+// Converged meshes are masked out of the remaining work.
+//
+// Vector work is organised in tiles: contiguous dof ranges inside one mesh (meshes own contiguous dof
+// ranges because free dofs are numbered node-major).  One workgroup per tile; per-tile partial dot
+// products are reduced per mesh in tile order, so every reduction is deterministic.
+//
+// One Newton iteration of the whole batch is three graph launches: assembly + residual norms +
+// Newton test + CG start; CG chunks of `chunk` iterations (repeated until every mesh's CG stopped);
+// the update.  The graphs are captured once per load step (the load factor is a kernel argument).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "mpf_internal.hpp"
+
+// problem.hip
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK);
+mpf_status do_update(mpf_problem* p, bool first_iter);
+
+namespace mpf {
+
+// Batched-solver state (owned by mpf_problem::batch).
+struct BatchWork {
+  int32_t nmesh = 0;
+  std::vector<int32_t> node_off, elem_off, dof_off;  // host, [nmesh + 1]
+  DevBuf<int32_t> emat;                              // element -> mesh
+  DevBuf<double> mtab;                               // [nmesh][2]: mu, K
+  // tiles
+  int32_t T = 1024, LPR = 8, chunk = 16, ntiles = 0;
+  bool tuned = false;
+  DevBuf<int32_t> tlo, thi, tmesh, mtile;  // tile ranges and mesh, mesh -> first tile [nmesh + 1]
+  // CG vectors (x is the problem's du)
+  DevBuf<double> dinv, r, z, pv, q;
+  DevBuf<double> part;  // [ntiles][2]
+  // per-mesh scalars: see the k_b_* kernels for the slots
+  DevBuf<double> ms;    // [nmesh][kMS]
+  DevBuf<int32_t> mf;   // [nmesh][kMF]
+  DevBuf<int32_t> cnt;  // [4]: CG-active meshes, Newton-active meshes, diverged meshes, negative Jacobian
+  int32_t* cnt_h = nullptr;  // pinned host mirror
+  hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr;
+  double g_lam = 0.0;
+  ~BatchWork() { release_graphs(); if (cnt_h) (void)hipHostFree(cnt_h); }
+  void release_graphs() {
+    for (auto& g : g_asm) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& g : g_upd) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    if (g_chunk) { (void)hipGraphExecDestroy(g_chunk); g_chunk = nullptr; }
+  }
+};
+
+// per-mesh double slots
+enum { kRZ = 0, kPQ, kRR, kRR0, kAL, kBE, kTOL2, kR0, kRN, kREL, kMS };
+// per-mesh int slots
+//   kCGA CG running, kNA Newton running, kCGIT CG iterations of the current solve, kNIT Newton
+//   iterations (solves) of the current load step, kDIV diverged, kCGT CG iterations of the load step
+enum { kCGA = 0, kNA, kCGIT, kNIT, kDIV, kCGT, kMF };
+
+namespace {
+
+constexpr int kB = 256;  // threads per tile workgroup
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+// Jacobi: inverse of the diagonal of every row.
+__global__ void k_b_diag(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                         const double* __restrict__ vals, double* __restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = rowptr[i], hi = rowptr[i + 1] - 1;
+  double d = 0.0;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = colidx[mid];
+    if (c == i) { d = vals[mid]; break; }
+    if (c < i) lo = mid + 1; else hi = mid - 1;
+  }
+  dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
+}
+
+// partial[tile][comp] = x . y over the tile (meshes with flag[m*kMF + slot] == 0 contribute 0)
+__global__ void __launch_bounds__(kB) k_b_dot(const int32_t* __restrict__ tlo, const int32_t* __restrict__ thi,
+                                              const double* __restrict__ x, const double* __restrict__ y,
+                                              double* __restrict__ part, int comp) {
+  __shared__ double sh[kB / 64];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (int i = tlo[b] + threadIdx.x; i < thi[b]; i += kB) v += x[i] * y[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) part[2 * b + comp] = v;
+}
+
+// y = A x on the tile's rows (LPR lanes per row) and partial[tile][0] = x . y; skipped for meshes
+// whose CG has stopped.
+template <int LPR>
+__global__ void __launch_bounds__(kB) k_b_spmv(const int32_t* __restrict__ tlo, const int32_t* __restrict__ thi,
+                                               const int32_t* __restrict__ tmesh, const int32_t* __restrict__ mf,
+                                               const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                               const double* __restrict__ vals, const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[kB / 64];
+  const int b = blockIdx.x;
+  if (!mf[tmesh[b] * kMF + kCGA]) {
+    if (threadIdx.x == 0) part[2 * b] = 0.0;
+    return;
+  }
+  const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  double dot = 0.0;
+  for (int row = tlo[b] + g; row < thi[b]; row += kB / LPR) {
+    double s = 0.0;
+    for (int k = rowptr[row] + l; k < rowptr[row + 1]; k += LPR) s += vals[k] * x[colidx[k]];
+    for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+    if (l == 0) {
+      y[row] = s;
+      dot += x[row] * s;
+    }
+  }
+  dot = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[2 * b] = dot;
+}
+
+// CG start: x = 0, r = b, z = D^-1 r, p = z; partials r.r and r.z (meshes not Newton-active: r = 0).
+__global__ void __launch_bounds__(kB) k_b_start(const int32_t* __restrict__ tlo, const int32_t* __restrict__ thi,
+                                                const int32_t* __restrict__ tmesh, const int32_t* __restrict__ mf,
+                                                const double* __restrict__ rhs, const double* __restrict__ dinv,
+                                                double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                double* __restrict__ p, double* __restrict__ part) {
+  __shared__ double sh[kB / 64];
+  const int b = blockIdx.x;
+  const bool on = mf[tmesh[b] * kMF + kNA] != 0;
+  double rr = 0.0, rz = 0.0;
+  for (int i = tlo[b] + threadIdx.x; i < thi[b]; i += kB) {
+    const double ri = on ? rhs[i] : 0.0;
+    const double zi = dinv[i] * ri;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  rr = block_sum(rr, sh);
+  rz = block_sum(rz, sh);
+  if (threadIdx.x == 0) { part[2 * b] = rr; part[2 * b + 1] = rz; }
+}
+
+// x += a p, r -= a q, z = D^-1 r; partials r.r and r.z.
+__global__ void __launch_bounds__(kB) k_b_xrz(const int32_t* __restrict__ tlo, const int32_t* __restrict__ thi,
+                                              const int32_t* __restrict__ tmesh, const int32_t* __restrict__ mf,
+                                              const double* __restrict__ ms, const double* __restrict__ dinv,
+                                              double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                              const double* __restrict__ p, const double* __restrict__ q,
+                                              double* __restrict__ part) {
+  __shared__ double sh[kB / 64];
+  const int b = blockIdx.x;
+  const int m = tmesh[b];
+  if (!mf[m * kMF + kCGA]) {
+    if (threadIdx.x == 0) { part[2 * b] = 0.0; part[2 * b + 1] = 0.0; }
+    return;
+  }
+  const double a = ms[m * kMS + kAL];
+  double rr = 0.0, rz = 0.0;
+  for (int i = tlo[b] + threadIdx.x; i < thi[b]; i += kB) {
+    x[i] += a * p[i];
+    const double ri = r[i] - a * q[i];
+    const double zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  }
+  rr = block_sum(rr, sh);
+  rz = block_sum(rz, sh);
+  if (threadIdx.x == 0) { part[2 * b] = rr; part[2 * b + 1] = rz; }
+}
+
+__global__ void __launch_bounds__(kB) k_b_p(const int32_t* __restrict__ tlo, const int32_t* __restrict__ thi,
+                                            const int32_t* __restrict__ tmesh, const int32_t* __restrict__ mf,
+                                            const double* __restrict__ ms, const double* __restrict__ z,
+                                            double* __restrict__ p) {
+  const int b = blockIdx.x;
+  const int m = tmesh[b];
+  if (!mf[m * kMF + kCGA]) return;
+  const double be = ms[m * kMS + kBE];
+  for (int i = tlo[b] + threadIdx.x; i < thi[b]; i += kB) p[i] = z[i] + be * p[i];
+}
+
+// Per-mesh scalar steps (one workgroup; thread per mesh; partials summed in tile order).
+__device__ __forceinline__ double mesh_sum(const double* part, const int32_t* mtile, int m, int comp) {
+  double s = 0.0;
+  for (int t = mtile[m]; t < mtile[m + 1]; ++t) s += part[2 * t + comp];
+  return s;
+}
+
+__device__ __forceinline__ void count_into(int32_t v, int32_t* out) {
+  __shared__ int32_t sc[kB / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sc[w];
+    *out = s;
+  }
+  __syncthreads();
+}
+
+// after the residual norm partials (comp 0): Newton test per mesh.  first: R0 = ||R||.
+__global__ void __launch_bounds__(kB) k_b_newton(int32_t nmesh, const int32_t* __restrict__ mtile,
+                                                 const double* __restrict__ part, double* __restrict__ ms,
+                                                 int32_t* __restrict__ mf, int32_t first, double rtol, double eta_min,
+                                                 int32_t* __restrict__ cnt) {
+  int32_t act = 0, div = 0;
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    double* S = ms + m * kMS;
+    int32_t* F = mf + m * kMF;
+    const double rn = sqrt(mesh_sum(part, mtile, m, 0));
+    if (first) S[kR0] = rn;
+    S[kRN] = rn;
+    const double R0 = S[kR0];
+    const double rel = R0 > 0.0 ? rn / R0 : 0.0;
+    S[kREL] = rel;
+    if (F[kNA]) {
+      if (rn != rn) { F[kDIV] = 1; F[kNA] = 0; }
+      else if (R0 == 0.0 || rel < rtol) F[kNA] = 0;
+      else F[kNIT] += 1;
+    }
+    // inexact Newton: CG to eta * ||R||, eta = min(0.1, 0.1 rel) bounded below
+    const double eta = fmax(eta_min, fmin(0.1, 0.1 * rel));
+    S[kTOL2] = eta * eta;
+    act += F[kNA] != 0;
+    div += F[kDIV] != 0;
+  }
+  count_into(act, cnt + 1);
+  count_into(div, cnt + 2);
+}
+
+__global__ void __launch_bounds__(kB) k_b_start_scalars(int32_t nmesh, const int32_t* __restrict__ mtile,
+                                                        const double* __restrict__ part, double* __restrict__ ms,
+                                                        int32_t* __restrict__ mf, int32_t* __restrict__ cnt) {
+  int32_t act = 0;
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    double* S = ms + m * kMS;
+    int32_t* F = mf + m * kMF;
+    const double rr = mesh_sum(part, mtile, m, 0), rz = mesh_sum(part, mtile, m, 1);
+    S[kRR0] = rr;
+    S[kRR] = rr;
+    S[kRZ] = rz;
+    F[kCGT] += F[kCGIT];
+    F[kCGIT] = 0;
+    F[kCGA] = (F[kNA] && rr > 0.0) ? 1 : 0;
+    act += F[kCGA];
+  }
+  count_into(act, cnt);
+}
+
+__global__ void __launch_bounds__(kB) k_b_alpha(int32_t nmesh, const int32_t* __restrict__ mtile,
+                                                const double* __restrict__ part, double* __restrict__ ms,
+                                                const int32_t* __restrict__ mf) {
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    double* S = ms + m * kMS;
+    const double pq = mesh_sum(part, mtile, m, 0);
+    S[kPQ] = pq;
+    S[kAL] = (mf[m * kMF + kCGA] && pq != 0.0) ? S[kRZ] / pq : 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kB) k_b_beta(int32_t nmesh, const int32_t* __restrict__ mtile,
+                                               const double* __restrict__ part, double* __restrict__ ms,
+                                               int32_t* __restrict__ mf, int32_t maxit, int32_t* __restrict__ cnt) {
+  int32_t act = 0;
+  for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
+    double* S = ms + m * kMS;
+    int32_t* F = mf + m * kMF;
+    if (F[kCGA]) {
+      const double rr = mesh_sum(part, mtile, m, 0), rz = mesh_sum(part, mtile, m, 1);
+      F[kCGIT] += 1;
+      S[kRR] = rr;
+      S[kBE] = S[kRZ] != 0.0 ? rz / S[kRZ] : 0.0;
+      S[kRZ] = rz;
+      if (rr <= S[kTOL2] * S[kRR0] || F[kCGIT] >= maxit || rr != rr) F[kCGA] = 0;
+    }
+    act += F[kCGA];
+  }
+  count_into(act, cnt);
+}
+
+__global__ void k_b_zero_i32(int32_t* a, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+mpf_status build_tiles(BatchWork& B, int T, hipStream_t s) {
+  std::vector<int32_t> lo, hi, tm, mt(B.nmesh + 1, 0);
+  for (int m = 0; m < B.nmesh; ++m) {
+    mt[m] = (int32_t)lo.size();
+    for (int32_t a = B.dof_off[m]; a < B.dof_off[m + 1]; a += T) {
+      lo.push_back(a);
+      hi.push_back(std::min(a + T, B.dof_off[m + 1]));
+      tm.push_back(m);
+    }
+  }
+  mt[B.nmesh] = (int32_t)lo.size();
+  B.T = T;
+  B.ntiles = (int32_t)lo.size();
+  MPF_TRY(B.tlo.upload(lo.data(), lo.size(), s));
+  MPF_TRY(B.thi.upload(hi.data(), hi.size(), s));
+  MPF_TRY(B.tmesh.upload(tm.data(), tm.size(), s));
+  MPF_TRY(B.mtile.upload(mt.data(), mt.size(), s));
+  MPF_TRY(B.part.alloc((size_t)std::max(B.ntiles, 1) * 2));
+  MPF_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+  return MPF_OK;
+}
+
+// One CG iteration of every active mesh.
+mpf_status enqueue_cg_iter(mpf_problem* p, BatchWork& B, int32_t maxit) {
+  hipStream_t s = p->stream;
+  const PatternBufs& P = p->pat;
+  const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
+  switch (B.LPR) {
+    case 4:
+      hipLaunchKernelGGL(k_b_spmv<4>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_b_spmv<16>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+      break;
+    default:
+      hipLaunchKernelGGL(k_b_spmv<8>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+  }
+  hipLaunchKernelGGL(k_b_alpha, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr);
+  hipLaunchKernelGGL(k_b_xrz, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, B.ms.ptr, B.dinv.ptr,
+                     p->du.ptr, B.r.ptr, B.z.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+  hipLaunchKernelGGL(k_b_beta, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr, maxit,
+                     B.cnt.ptr);
+  hipLaunchKernelGGL(k_b_p, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, B.ms.ptr, B.z.ptr, B.pv.ptr);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+// assembly + residual norms + Newton test + CG start
+mpf_status enqueue_newton_head(mpf_problem* p, BatchWork& B, bool first, double lam, double rtol, double eta_min) {
+  hipStream_t s = p->stream;
+  const PatternBufs& P = p->pat;
+  const int64_t n = P.dispDOF;
+  const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
+  hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, s, p->bad.ptr, 1);
+  MPF_TRY(enqueue_assembly(p, first, lam, 1));
+  hipLaunchKernelGGL(k_b_dot, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, p->rhs.ptr, p->rhs.ptr, B.part.ptr, 0);
+  hipLaunchKernelGGL(k_b_newton, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr,
+                     (int32_t)first, rtol, eta_min, B.cnt.ptr);
+  hipLaunchKernelGGL(k_b_diag, dim3(grid_for(n, 256)), dim3(256), 0, s, n, P.rowptr.ptr, P.colidx.ptr, p->vals.ptr,
+                     B.dinv.ptr);
+  hipLaunchKernelGGL(k_b_start, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, p->rhs.ptr, B.dinv.ptr,
+                     p->du.ptr, B.r.ptr, B.z.ptr, B.pv.ptr, B.part.ptr);
+  hipLaunchKernelGGL(k_b_start_scalars, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr,
+                     B.cnt.ptr);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+// capture the work enqueued by f on the problem's stream into an executable graph
+template <typename Fn>
+mpf_status capture(mpf_problem* p, hipGraphExec_t* out, Fn f) {
+  hipStream_t s = p->stream;
+  if (*out) { (void)hipGraphExecDestroy(*out); *out = nullptr; }
+  MPF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  mpf_status st = f();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (st != MPF_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+  MPF_HIP(e);
+  const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MPF_HIP(e2);
+  return MPF_OK;
+}
+
+mpf_status launch(mpf_problem* p, hipGraphExec_t g, bool use_graph, const std::function<mpf_status()>& eager) {
+  if (use_graph) {
+    MPF_HIP(hipGraphLaunch(g, p->stream));
+    return MPF_OK;
+  }
+  return eager();
+}
+
+mpf_status read_counts(mpf_problem* p, BatchWork& B) {
+  MPF_HIP(hipMemcpyAsync(B.cnt_h, B.cnt.ptr, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  return MPF_OK;
+}
+
+double secs(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+void batch_free(mpf_problem* p) {
+  delete p->batch;
+  p->batch = nullptr;
+}
+
+mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
+                     const double* mu, const double* Kinv) {
+  if (p->etype != MPF_HEX8 || nmesh <= 0 || !node_off || !elem_off || !mu || !Kinv) return MPF_ERR_ARG;
+  if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmesh] != p->nNode || elem_off[nmesh] != p->nElem)
+    return MPF_ERR_ARG;
+  for (int m = 0; m < nmesh; ++m)
+    if (node_off[m + 1] < node_off[m] || elem_off[m + 1] < elem_off[m] || Kinv[m] <= 0.0) return MPF_ERR_ARG;
+  // meshes must not share nodes: element nodes of mesh m lie in its node range
+  std::vector<int32_t> conn((size_t)p->nElem * p->npe);
+  MPF_TRY(p->conn.download(conn.data(), conn.size(), p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  std::vector<int32_t> emat(p->nElem);
+  std::vector<double> tab(2 * (size_t)nmesh);
+  for (int m = 0; m < nmesh; ++m) {
+    tab[2 * m] = mu[m];
+    tab[2 * m + 1] = 1.0 / Kinv[m];
+    for (int32_t e = elem_off[m]; e < elem_off[m + 1]; ++e) {
+      emat[e] = m;
+      for (int a = 0; a < p->npe; ++a) {
+        const int32_t v = conn[(size_t)e * p->npe + a];
+        if (v < node_off[m] || v >= node_off[m + 1]) return MPF_ERR_ARG;
+      }
+    }
+  }
+  if (!p->batch) p->batch = new BatchWork();
+  BatchWork& B = *p->batch;
+  B.nmesh = nmesh;
+  B.node_off.assign(node_off, node_off + nmesh + 1);
+  B.elem_off.assign(elem_off, elem_off + nmesh + 1);
+  MPF_TRY(B.emat.upload(emat.data(), emat.size(), p->stream));
+  MPF_TRY(B.mtab.upload(tab.data(), tab.size(), p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  return MPF_OK;
+}
+
+// after the pattern: dof ranges, vectors, tiles
+static mpf_status batch_prepare(mpf_problem* p) {
+  BatchWork& B = *p->batch;
+  if (!B.dof_off.empty()) return MPF_OK;
+  std::vector<int32_t> nda((size_t)p->nNode * 3);
+  MPF_TRY(p->pat.nda.download(nda.data(), nda.size(), p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  B.dof_off.assign(B.nmesh + 1, 0);
+  int32_t c = 0;
+  for (int m = 0; m < B.nmesh; ++m) {
+    B.dof_off[m] = c;
+    for (int32_t v = B.node_off[m]; v < B.node_off[m + 1]; ++v)
+      for (int d = 0; d < 3; ++d)
+        if (nda[3 * (size_t)v + d] >= 0) {
+          if (nda[3 * (size_t)v + d] != c) return MPF_ERR_STATE;  // free dofs must be node-major
+          ++c;
+        }
+  }
+  B.dof_off[B.nmesh] = c;
+  if (c == 0) return MPF_ERR_ARG;  // nothing to solve
+  const size_t n = c;
+  MPF_TRY(B.dinv.alloc(n));
+  MPF_TRY(B.r.alloc(n));
+  MPF_TRY(B.z.alloc(n));
+  MPF_TRY(B.pv.alloc(n));
+  MPF_TRY(B.q.alloc(n));
+  MPF_TRY(B.ms.alloc((size_t)B.nmesh * kMS));
+  MPF_TRY(B.mf.alloc((size_t)B.nmesh * kMF));
+  MPF_TRY(B.cnt.alloc(4));
+  if (!B.cnt_h) MPF_HIP(hipHostMalloc((void**)&B.cnt_h, 4 * sizeof(int32_t), hipHostMallocDefault));
+  return build_tiles(B, B.T, p->stream);
+}
+
+ElemArgs batch_elem_args(mpf_problem* p, ElemArgs a) {
+  if (p->batch && p->batch->nmesh > 0) {
+    a.emat = p->batch->emat.ptr;
+    a.mtab = p->batch->mtab.ptr;
+  }
+  return a;
+}
+
+mpf_status batch_autotune(mpf_problem* p, int32_t* T_out, int32_t* LPR_out, double* ms_per_iter) {
+  if (!p->batch || !p->prepared) return MPF_ERR_STATE;
+  MPF_TRY(batch_prepare(p));
+  BatchWork& B = *p->batch;
+  hipStream_t s = p->stream;
+  // a realistic system: assemble at the current state with the lift, start CG on every mesh
+  {
+    std::vector<int32_t> init((size_t)B.nmesh * kMF, 0);
+    for (int m = 0; m < B.nmesh; ++m) init[(size_t)m * kMF + kNA] = 1;
+    MPF_HIP(hipMemcpyAsync(B.mf.ptr, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    MPF_HIP(hipStreamSynchronize(s));
+  }
+  const int Ts[] = {512, 1024, 2048, 4096};
+  const int Ls[] = {4, 8, 16};
+  double best = 1e300;
+  int bT = B.T, bL = B.LPR;
+  hipEvent_t e0, e1;
+  MPF_HIP(hipEventCreate(&e0));
+  MPF_HIP(hipEventCreate(&e1));
+  for (int T : Ts) {
+    MPF_TRY(build_tiles(B, T, s));
+    MPF_TRY(enqueue_newton_head(p, B, true, 1.0, 0.0, 1e-300));
+    for (int L : Ls) {
+      B.LPR = L;
+      MPF_TRY(enqueue_cg_iter(p, B, 1 << 30));  // warm-up
+      MPF_HIP(hipEventRecord(e0, s));
+      const int reps = 5;
+      for (int k = 0; k < reps; ++k) MPF_TRY(enqueue_cg_iter(p, B, 1 << 30));
+      MPF_HIP(hipEventRecord(e1, s));
+      MPF_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MPF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) { best = ms / reps; bT = T; bL = L; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MPF_TRY(build_tiles(B, bT, s));
+  B.LPR = bL;
+  B.tuned = true;
+  B.release_graphs();
+  if (T_out) *T_out = bT;
+  if (LPR_out) *LPR_out = bL;
+  if (ms_per_iter) *ms_per_iter = best;
+  return MPF_OK;
+}
+
+mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
+                              mpf_batch_report* rep) {
+  if (!p->batch || !p->prepared || !o || !rep) return MPF_ERR_STATE;
+  if (o->nsteps <= 0 || o->dt <= 0.0 || o->itmax <= 0 || o->cg_maxit <= 0) return MPF_ERR_ARG;
+  MPF_TRY(batch_prepare(p));
+  BatchWork& B = *p->batch;
+  hipStream_t s = p->stream;
+  *rep = mpf_batch_report{};
+  if (o->chunk > 0) B.chunk = o->chunk;
+  if (o->tile > 0 && o->tile != B.T) {
+    MPF_TRY(build_tiles(B, o->tile, s));
+    B.release_graphs();
+  }
+  if (o->lanes_per_row == 4 || o->lanes_per_row == 8 || o->lanes_per_row == 16) {
+    if (o->lanes_per_row != B.LPR) B.release_graphs();
+    B.LPR = o->lanes_per_row;
+  }
+  rep->tile = B.T;
+  rep->lanes_per_row = B.LPR;
+  rep->chunk = B.chunk;
+  const bool G = o->use_graph != 0;
+  const size_t n3 = (size_t)p->nNode * 3;
+  auto t_all = std::chrono::steady_clock::now();
+  mpf_status result = MPF_OK;
+  std::vector<int32_t> mfh((size_t)B.nmesh * kMF);
+  std::vector<double> msh((size_t)B.nmesh * kMS);
+  double t = 0.0;
+  for (int step = 0; step < o->nsteps; ++step) {
+    t += o->dt;
+    const double lam = t;  // lambda(t) = t
+    // saveSolution, setBoundaryConditions
+    MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MPF_TRY(mpf_set_bc(p, lam));
+    {
+      // every mesh starts the step Newton-active, counters at zero
+      std::vector<int32_t> init((size_t)B.nmesh * kMF, 0);
+      for (int m = 0; m < B.nmesh; ++m) init[(size_t)m * kMF + kNA] = 1;
+      MPF_HIP(hipMemcpyAsync(B.mf.ptr, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      MPF_HIP(hipStreamSynchronize(s));
+    }
+    if (G && (B.g_lam != lam || !B.g_asm[0])) {
+      for (int f = 0; f < 2; ++f) {
+        MPF_TRY(capture(p, &B.g_asm[f], [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+        MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
+      }
+      MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
+        for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
+        return MPF_OK;
+      }));
+      B.g_lam = lam;
+    }
+    bool step_conv = false;
+    for (int iter = 1; iter <= o->itmax + 1; ++iter) {
+      const int f = iter == 1 ? 1 : 0;
+      auto ta = std::chrono::steady_clock::now();
+      MPF_TRY(launch(p, B.g_asm[f], G, [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+      ++rep->graph_launches;
+      MPF_TRY(read_counts(p, B));
+      rep->seconds_assembly += secs(ta);
+      int32_t bad = 0;
+      MPF_HIP(hipMemcpy(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
+      if (bad) { result = MPF_ERR_NEG_JACOBIAN; break; }
+      if (B.cnt_h[2] > 0) { result = MPF_ERR_DIVERGED; break; }
+      if (B.cnt_h[1] == 0) { step_conv = true; break; }
+      if (iter > o->itmax) break;
+      ++rep->newton_iters;
+      auto ts = std::chrono::steady_clock::now();
+      int its = 0;
+      while (B.cnt_h[0] > 0 && its < o->cg_maxit) {
+        MPF_TRY(launch(p, B.g_chunk, G, [&]() -> mpf_status {
+          for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
+          return MPF_OK;
+        }));
+        ++rep->graph_launches;
+        its += B.chunk;
+        MPF_TRY(read_counts(p, B));
+      }
+      rep->cg_iters_max += its;
+      MPF_TRY(launch(p, B.g_upd[f], G, [&] { return do_update(p, f == 1); }));
+      ++rep->graph_launches;
+      rep->seconds_solve += secs(ts);
+    }
+    MPF_HIP(hipMemcpyAsync(mfh.data(), B.mf.ptr, mfh.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipMemcpyAsync(msh.data(), B.ms.ptr, msh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    for (int m = 0; m < B.nmesh; ++m) {
+      const int32_t it = mfh[(size_t)m * kMF + kNIT] + 1;  // assemblies to convergence
+      if (mesh_iters) mesh_iters[(size_t)step * B.nmesh + m] = it;
+      if (mesh_relres) mesh_relres[m] = msh[(size_t)m * kMS + kREL];
+      rep->cg_iters_total += mfh[(size_t)m * kMF + kCGT] + mfh[(size_t)m * kMF + kCGIT];
+    }
+    if (result != MPF_OK) break;
+    rep->steps_done = step + 1;
+    if (!step_conv) { result = MPF_ERR_NOT_CONVERGED; break; }
+  }
+  rep->seconds = secs(t_all);
+  return result;
+}
+
+}  // namespace mpf
+
+extern "C" {
+
+mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
+                         const double* mu, const double* Kinv) {
+  if (!p) return MPF_ERR_ARG;
+  return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
+}
+
+mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter) {
+  if (!p) return MPF_ERR_ARG;
+  return mpf::batch_autotune(p, tile, lanes_per_row, ms_per_cg_iter);
+}
+
+mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
+                                  mpf_batch_report* rep) {
+  if (!p) return MPF_ERR_ARG;
+  return mpf::batch_newton_solve(p, o, mesh_iters, mesh_relres, rep);
+}
+
+}  // extern "C"

@@ -86,7 +86,9 @@ __global__ void __launch_bounds__(64) hex8_kernel(ElemArgs A, int32_t first, int
     const int k = lane >> 3, gp = lane & 7;
     const bool act = base_idx + k < count;
     bool negJ = false;
-    element_point<8, 8, false>(sm.X[k], sm.U[k], nullptr, nullptr, sm.dN[gp], gp, 1.0, A.mu, 1.0 / A.Kinv, A.Utype,
+    double mu, K;
+    elem_material(A, DENSE ? first + (act ? base_idx + k : base_idx) : sc.elems[act ? base_idx + k : base_idx], mu, K);
+    element_point<8, 8, false>(sm.X[k], sm.U[k], nullptr, nullptr, sm.dN[gp], gp, 1.0, mu, K, A.Utype,
                                A.consistent != 0, true, &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
     if (act && negJ) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
   }
@@ -234,7 +236,9 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
     const bool writer = (lane >> 2) < EPW;
     const bool act = writer && (base_idx + k < count);
     bool negJ = false;
-    element_point<10, 4, true>(sm.X[k], sm.U[k], sm.UP[k], sm.Pn[k], sm.dN[gp], gp, wq, A.mu, 1.0 / A.Kinv, A.Utype,
+    double mu, K;
+    elem_material(A, DENSE ? first + (act ? base_idx + k : base_idx) : sc.elems[act ? base_idx + k : base_idx], mu, K);
+    element_point<10, 4, true>(sm.X[k], sm.U[k], sm.UP[k], sm.Pn[k], sm.dN[gp], gp, wq, mu, K, A.Utype,
                                false, writer, &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
     if (act && negJ) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
   }

@@ -260,6 +260,18 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   }
 }
 
+// Material parameters of element e: (mu, K) from the per-element table when present.
+__device__ __forceinline__ void elem_material(const ElemArgs& A, int e, double& mu, double& K) {
+  if (A.emat) {
+    const int t = A.emat[e];
+    mu = A.mtab[2 * t];
+    K = A.mtab[2 * t + 1];
+  } else {
+    mu = A.mu;
+    K = 1.0 / A.Kinv;
+  }
+}
+
 // One Gauss point's share of the 3x3 tangent block (a,b):
 //   K_ab^{ik} += s1 g^a_k g^b_i + (s2 g^a_i + s3 (b g^a)_i) g^b_k + s3 g^a_i (b g^b)_k + delta_ik (s4 b g^a + s5 g^a).g^b
 // with g = grad_x N and b the left Cauchy-Green tensor (the closed form of the reference's

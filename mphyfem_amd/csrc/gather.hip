@@ -85,8 +85,10 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   double* rec = GS + (size_t)(writer ? e : base) * St::TS;
   const double wgp = (NPE == 8) ? 1.0 : 0.2500000000000000 * (1.0 / 6.0);  // QuadratureUtil.cpp:466-492, 706-709
   bool negJ = false;
+  double mu, K;
+  elem_material(A, writer ? e : base, mu, K);
   element_point<NPE, NGP, MIXED>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp], gp, wgp,
-                                 A.mu, 1.0 / A.Kinv, A.Utype, A.consistent != 0, writer,
+                                 mu, K, A.Utype, A.consistent != 0, writer,
                                  rec + St::G0 + gp * NPE * 6, rec + St::S0 + gp * St::SS, rec + St::R0, negJ);
   if (writer && negJ) flag_bad(A.bad, e);
 }
@@ -237,13 +239,14 @@ __global__ void k_scatter_off(int32_t n, const int32_t* __restrict__ order, cons
 }
 
 // Build step 2 (one wavefront per node): the node's laid-out items,
-//   fitem[foff[n] + q] = {e * 16 + a, (block << 4) | b}   block = nbptr[n] + j (a column block of n's rows)
-//   and {-1, -1} in padding slots.
+//   fitem[foff[n] + q] = {e * 16 + a, (j << 10) | (s << 4) | b}  (column block j of n's rows, element s
+//   around n, node b of it) and {-1, -1} in padding slots; wnode[w] = n for each of n's wavefronts.
 template <int NPE>
 __global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* __restrict__ nptr,
                                                   const int32_t* __restrict__ nbptr, const int32_t* __restrict__ nelA,
                                                   const int32_t* __restrict__ keys, const int64_t* __restrict__ flen,
-                                                  const int64_t* __restrict__ foff, int2* __restrict__ fitem) {
+                                                  const int64_t* __restrict__ foff, int2* __restrict__ fitem,
+                                                  int32_t* __restrict__ wnode) {
   __shared__ int32_t sK[kItemCap], sMap[kFlatCap];
   const int lane = threadIdx.x;
   const int n = blockIdx.x;
@@ -257,7 +260,6 @@ __global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* 
   __syncthreads();
   if (lane == 0) flat_walk(sK, nitems, [&](int pos, int t) { sMap[pos] = t; });
   __syncthreads();
-  const int blk0 = nbptr[n];
   int2* out = fitem + foff[n];
   for (int q = lane; q < len; q += 64) {
     const int t = sMap[q];
@@ -265,10 +267,11 @@ __global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* 
     if (t >= 0) {
       const int key = sK[t];
       it.x = nelA[e0 + ((key >> 4) & 63)];
-      it.y = ((blk0 + (key >> 16)) << 4) | (key & 15);
+      it.y = ((key >> 16) << 10) | (key & 0x3ff);
     }
     out[q] = it;
   }
+  for (int q = lane * 64; q < len; q += 64 * 64) wnode[(foff[n] + q) >> 6] = n;
 }
 
 // Build step 3 (one wavefront per node n): where each column block (n, j) of n's rows lives in the CSR.
@@ -354,16 +357,17 @@ __global__ void k_near(int32_t nNode, const int32_t* __restrict__ nptr, const in
 
 // One (element, node b) item of node n's rows: the 3x3 block K_{n b} (v[0..8]) and, for the mixed
 // element, Kup(n, b) (v[9..11]), Kpu(n, b) (v[12..14]) and Kpp(n, b) (v[15]) from the element's
-// staging record: node n's g, b g at gA, node b's at gB (stride NPE * 6 per Gauss point, read as
-// 16-byte pairs), the Gauss-point scalars at sS (stride SS).
-template <int NPE, int NGP, bool MIXED, int NV, int SS>
+// staged Gauss-point data: node n's g, b g at gA (stride GAS per Gauss point), the Gauss-point
+// scalars at sS (stride SS), node b's g, b g at gB in the staging record (stride NPE * 6); the
+// six-double groups are read as 16-byte pairs.
+template <int NPE, int NGP, bool MIXED, int NV, int GAS, int SS>
 __device__ __forceinline__ void item_block(const double* gA, const double* sS, const double* gB, int a, int b,
                                            double v[NV]) {
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
 #pragma unroll 2
   for (int gp = 0; gp < NGP; ++gp) {
-    const double2* ga2 = reinterpret_cast<const double2*>(gA + gp * NPE * 6);
+    const double2* ga2 = reinterpret_cast<const double2*>(gA + gp * GAS);
     const double2* gb2 = reinterpret_cast<const double2*>(gB + gp * NPE * 6);
     const double* S = sS + gp * SS;
     const double2 p0 = ga2[0], p1 = ga2[1], p2 = ga2[2];
@@ -391,51 +395,93 @@ __device__ __forceinline__ void item_block(const double* gA, const double* sS, c
 
 struct FlatArgs {
   int64_t nwaves;
-  const int2* fitem;
+  const int2* fitem;        // {e * 16 + a, (j << 10) | (s << 4) | b}, {-1, -1} in padding slots
+  const int32_t* wnode;     // node whose rows each wavefront's items belong to
+  const int32_t* nptr;
+  const int32_t* nelA;
+  const int32_t* nbptr;
   const int32_t* btab;
   const double* GS;
   double* vals;
 };
 
-// Matrix kernel: one wavefront per 64 laid-out items.  Item lanes compute their blocks; the first
-// lane of each column run (its head) sums the run in element order and writes the column block of
-// n's rows, so every CSR value is written exactly once.
+// Owner-side staging capacity of the matrix kernel: elements around a node whose node-n data are
+// copied to LDS once per wavefront; a node with more elements reads them from the staging records.
+template <int NPE>
+struct FlatCap {
+  static constexpr int MAXS = NPE == 8 ? 8 : 24;
+};
+
+// Matrix kernel: one wavefront per 64 laid-out items of one node n.  The wavefront first copies
+// node n's share of the staging records of the elements around n (g, b g of n and the Gauss-point
+// scalars) to LDS; item lanes then compute their blocks, and the first lane of each column run (its
+// head) sums the run in element order and writes the column block of n's rows, so every CSR value
+// is written exactly once.
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
   constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
   constexpr int BT = MIXED ? 4 : 2;
+  constexpr int MAXS = FlatCap<NPE>::MAXS;
+  constexpr int SSP = (St::SS + 1) & ~1;                 // scalars per Gauss point, padded to pairs
+  constexpr int STG = MAXS * NGP * (6 + SSP);             // staged doubles
+  constexpr int LDSD = STG > 64 * NV ? STG : 64 * NV;     // the staging area is reused for the run sums
   __shared__ int32_t sB[64];
-  __shared__ double sPart[64][NV];
+  __shared__ double2 sMem[LDSD / 2];
+  double* const sGa = reinterpret_cast<double*>(sMem);   // [MAXS][NGP][6]
+  double* const sSc = sGa + MAXS * NGP * 6;               // [MAXS][NGP][SSP]
+  double* const sPart = reinterpret_cast<double*>(sMem);  // [64][NV], after the items are computed
   const int lane = threadIdx.x;
   const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
   if (w >= F.nwaves) return;
   const int2 it = F.fitem[w * 64 + lane];
+  const int n = F.wnode[w];
+  const int e0 = F.nptr[n], m = F.nptr[n + 1] - e0;
+  const bool staged = m <= MAXS;
+  if (staged) {
+    for (int t = lane; t < m * NGP * 3; t += 64) {
+      const int s = t / (NGP * 3), r = t % (NGP * 3), gp = r / 3, c = r % 3;
+      const int ea = F.nelA[e0 + s];
+      sMem[(s * NGP + gp) * 3 + c] = reinterpret_cast<const double2*>(
+          F.GS + (size_t)(ea >> 4) * St::TS + St::G0 + gp * NPE * 6 + (ea & 15) * 6)[c];
+    }
+    for (int t = lane; t < m * NGP * SSP; t += 64) {
+      const int s = t / (NGP * SSP), r = t % (NGP * SSP), gp = r / SSP, c = r % SSP;
+      sSc[t] = c < St::SS ? F.GS[(size_t)(F.nelA[e0 + s] >> 4) * St::TS + St::S0 + gp * St::SS + c] : 0.0;
+    }
+  }
   const bool live = it.x >= 0;
-  const int blk = live ? (it.y >> 4) : -1;
-  int bt[BT];
-#pragma unroll
-  for (int q = 0; q < BT; ++q) bt[q] = live ? F.btab[(size_t)BT * blk + q] : 0;
+  const int j = live ? (it.y >> 10) : -1;
+  __syncthreads();
   double v[NV];
   if (live) {
-    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+    const int e = it.x >> 4, a = it.x & 15, s = (it.y >> 4) & 63, b = it.y & 15;
     const double* rec = F.GS + (size_t)e * St::TS;
-    item_block<NPE, NGP, MIXED, NV, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a, b, v);
+    if (staged)
+      item_block<NPE, NGP, MIXED, NV, 6, SSP>(sGa + s * NGP * 6, sSc + s * NGP * SSP, rec + St::G0 + b * 6, a, b, v);
+    else
+      item_block<NPE, NGP, MIXED, NV, NPE * 6, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a,
+                                                       b, v);
   } else {
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.0;
   }
-  sB[lane] = blk;
+  __syncthreads();  // staging area becomes the run-sum area
+  sB[lane] = j;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) sPart[lane][q] = v[q];
+  for (int q = 0; q < NV; ++q) sPart[lane * NV + q] = v[q];
   __syncthreads();
-  if (!live || (lane > 0 && sB[lane - 1] == blk)) return;
+  if (!live || (lane > 0 && sB[lane - 1] == j)) return;
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = v[q];
-  for (int t2 = lane + 1; t2 < 64 && sB[t2] == blk; ++t2)
+  for (int t2 = lane + 1; t2 < 64 && sB[t2] == j; ++t2)
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] += sPart[t2][q];
+    for (int q = 0; q < NV; ++q) acc[q] += sPart[t2 * NV + q];
+  const size_t blk = (size_t)F.nbptr[n] + j;
+  int bt[BT];
+#pragma unroll
+  for (int q = 0; q < BT; ++q) bt[q] = F.btab[BT * blk + q];
   const int A = bt[0];
   const int masks = MIXED ? bt[3] : bt[1];
   const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 8);
@@ -558,6 +604,7 @@ static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
   P.g_order.release();
   P.g_nelA.release();
   P.g_fitem.release();
+  P.g_wnode.release();
   P.g_btab.release();
   P.g_near.release();
   p->GS.release();
@@ -652,13 +699,14 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
 
   // step 2: laid-out items
   MPF_TRY(P.g_fitem.alloc(total > 0 ? (size_t)total : 64));
+  MPF_TRY(P.g_wnode.alloc(total > 0 ? (size_t)total / 64 : 1));
   if (total == 0) MPF_HIP(hipMemsetAsync(P.g_fitem.ptr, 0xff, 64 * sizeof(int2), s));
   if (mixed)
     hipLaunchKernelGGL(k_flat_fill<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
-                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr);
+                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr, P.g_wnode.ptr);
   else
     hipLaunchKernelGGL(k_flat_fill<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
-                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr);
+                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr, P.g_wnode.ptr);
   P.g_nwaves = total / 64;
 
   // step 3: column-block table; step 4: near-Dirichlet flags
@@ -693,6 +741,10 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   FlatArgs F{};
   F.nwaves = P.g_nwaves;
   F.fitem = P.g_fitem.ptr;
+  F.wnode = P.g_wnode.ptr;
+  F.nptr = P.nptr.ptr;
+  F.nelA = P.g_nelA.ptr;
+  F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
   F.GS = p->GS.ptr;
   F.vals = p->vals.ptr;

@@ -79,6 +79,8 @@ struct ElemArgs {
   const double* pres;     // [nNode]   (mixed)
   const int32_t* conn;    // [nElem][npe]
   double mu, Kinv;
+  const int32_t* emat;    // optional per-element material index (batched meshes), else mu / Kinv
+  const double* mtab;     // [nmat][2]: mu, K = 1 / Kinv
   int32_t Utype;
   int32_t consistent;     // displacement element tangent mode (MPF_TANGENT_CONSISTENT)
   int32_t* bad;           // 1 + element id with det F < 0 (smallest id wins)
@@ -129,6 +131,7 @@ struct PatternBufs {
   DevBuf<int32_t> g_order;    // nodes in Morton (Z-curve) order of their coordinates
   DevBuf<int32_t> g_nelA;     // elements around each node (ascending), with the node's local index
   DevBuf<int2> g_fitem;       // laid-out (element, node) items of every node's rows, 64 per wavefront
+  DevBuf<int32_t> g_wnode;    // node of each wavefront of items
   DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j)
   DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
   int64_t g_nwaves = 0;
@@ -139,6 +142,7 @@ struct PatternBufs {
 };
 
 struct KrylovWork;  // krylov.hip
+struct BatchWork;   // batch.hip
 
 }  // namespace mpf
 
@@ -173,6 +177,7 @@ struct mpf_problem {
   int32_t asm_mode = MPF_ASSEMBLY_GATHER;
   mpf::DevBuf<double> GS;
   mpf::DevBuf<double> scratch;  // reductions (norms)
+  mpf::BatchWork* batch = nullptr;  // independent meshes solved together (batch.hip)
 };
 
 namespace mpf {
@@ -181,6 +186,14 @@ mpf_status ensure_gather_tables(mpf_problem* p);   // gather.hip (falls back to 
 mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const double* dispApplied, int writeK,
                                   hipStream_t s);
 mpf_status build_pattern(mpf_problem* p);          // pattern.hip
+// batch.hip
+ElemArgs batch_elem_args(mpf_problem* p, ElemArgs a);
+void batch_free(mpf_problem* p);
+mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
+                     const double* mu, const double* Kinv);
+mpf_status batch_autotune(mpf_problem* p, int32_t* T, int32_t* LPR, double* ms_per_iter);
+mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
+                              mpf_batch_report* rep);
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
 void krylov_free(mpf_problem* p);
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);

@@ -181,6 +181,9 @@ mpf_status check_device() {
   return MPF_OK;
 }
 
+}  // namespace
+
+// Per-problem host logic shared with the batched solver (batch.hip).
 ElemArgs elem_args(mpf_problem* p) {
   ElemArgs a{};
   a.X = p->X.ptr;
@@ -193,7 +196,7 @@ ElemArgs elem_args(mpf_problem* p) {
   a.Utype = p->m.Utype;
   a.consistent = p->m.tangent == MPF_TANGENT_CONSISTENT;
   a.bad = p->bad.ptr;
-  return a;
+  return batch_elem_args(p, a);
 }
 
 
@@ -358,7 +361,6 @@ mpf_status newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, con
   return conv ? MPF_OK : MPF_ERR_NOT_CONVERGED;
 }
 
-}  // namespace
 
 // =============================================================================================
 // C ABI
@@ -479,6 +481,7 @@ mpf_status mpf_problem_destroy(mpf_problem* p) {
   if (!p) return MPF_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   krylov_free(p);
+  batch_free(p);
   hostX_store().erase(p);
   hipStream_t s = p->stream;
   delete p;
