@@ -254,6 +254,50 @@ mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_
 /* mesh_relres [nmesh]: final ||R_m|| / ||R0_m||.  Either may be NULL.                           */
 mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters,
                                   double* mesh_relres, mpf_batch_report* rep);
+/* One load step of every mesh at load factor lam (nsteps, dt ignored), from the current converged */
+/* state or, when guess (device, [nNode*3]) is given, from that displacement with the prescribed  */
+/* dofs set to their values; R0_m is the plain-start residual either way.  mesh_iters [nmesh].    */
+/* rep accumulates (zero it before the first step).                                             */
+mpf_status mpf_batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
+                                 int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep);
+
+/* Device pointers of the problem's displacement [nNode*3] and pressure [nNode] (valid until      */
+/* destroy; ordered on the problem's stream).                                                     */
+mpf_status mpf_state_device_ptrs(mpf_problem* p, double** disp, double** pres);
+/* Device-to-device copy of the displacement [nNode*3] into disp_dst (synchronous).                */
+mpf_status mpf_copy_state_device(mpf_problem* p, double* disp_dst);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Learned initial guess.  No counterpart in the reference: a message-passing network on a       */
+/* problem's node graph (nodes sharing an element), 32 input features, 3 layers of width 64, BF16  */
+/* MFMA GEMMs with FP32 accumulation, FP32 master weights with Adam.  It predicts the correction   */
+/* of the linear extrapolation u_k + rho (u_k - u_{k-1}) of the next load step's displacement.    */
+typedef struct mpf_mpnn mpf_mpnn;
+int32_t mpf_mpnn_num_params(void);
+mpf_status mpf_mpnn_create(mpf_mpnn** out, uint64_t seed);
+mpf_status mpf_mpnn_destroy(mpf_mpnn* net);
+/* Host copies of the FP32 parameters / accumulated gradients (either may be NULL); set refreshes  */
+/* the BF16 copies.                                                                               */
+mpf_status mpf_mpnn_get_params(mpf_mpnn* net, float* w, float* g);
+mpf_status mpf_mpnn_set_params(mpf_mpnn* net, const float* w);
+mpf_status mpf_mpnn_zero_grad(mpf_mpnn* net);
+/* Features of load step k+1 from the converged u_k, u_{k-1} (device) into the network's input.    */
+mpf_status mpf_mpnn_features(mpf_mpnn* net, mpf_problem* p, const double* uk, const double* ukm1, double rho,
+                             double lam_next, double lam_ratio);
+/* Forward over the problem's nodes: x [nNode][32] BF16 (device) or NULL for the features above;    */
+/* y [nNode][32] FP32 (device) or NULL.                                                            */
+mpf_status mpf_mpnn_forward(mpf_mpnn* net, mpf_problem* p, const uint16_t* x, float* y);
+/* Mean squared error of the last forward against the converged u_{k+1} over free dofs; leaves     */
+/* dL/dy in the network.                                                                          */
+mpf_status mpf_mpnn_loss(mpf_mpnn* net, mpf_problem* p, const double* unext, const double* uk, const double* ukm1,
+                         double rho, double* loss);
+/* Gradients of the last forward for dL/dy = dy [nNode][32] FP32 (device), or the one mpf_mpnn_loss */
+/* left when dy is NULL; they accumulate until mpf_mpnn_adam / mpf_mpnn_zero_grad.                 */
+mpf_status mpf_mpnn_backward(mpf_mpnn* net, mpf_problem* p, const float* dy);
+mpf_status mpf_mpnn_adam(mpf_mpnn* net, float lr, float beta1, float beta2, float eps);
+/* Newton start point u_k + rho (u_k - u_{k-1}) (+ the network's correction when use_net), device. */
+mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const double* ukm1, double rho,
+                          int32_t use_net, double* guess);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Timing helpers: average device time (ms) of one assembly (element kernels + scatter, no       */

@@ -119,6 +119,22 @@ SIGNATURES = [
     ("mpf_set_batch", C.c_int, [_P, C.c_int32, _I32P, _I32P, _DP, _DP]),
     ("mpf_batch_autotune", C.c_int, [_P, _I32P, _I32P, _DP]),
     ("mpf_batch_newton_solve", C.c_int, [_P, C.POINTER(BatchOpts), _I32P, _DP, C.POINTER(BatchReport)]),
+    ("mpf_batch_newton_step", C.c_int, [_P, C.c_double, C.POINTER(BatchOpts), _P, _I32P, _DP,
+                                        C.POINTER(BatchReport)]),
+    ("mpf_state_device_ptrs", C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mpf_copy_state_device", C.c_int, [_P, _P]),
+    ("mpf_mpnn_num_params", C.c_int32, []),
+    ("mpf_mpnn_create", C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
+    ("mpf_mpnn_destroy", C.c_int, [_P]),
+    ("mpf_mpnn_get_params", C.c_int, [_P, _P, _P]),
+    ("mpf_mpnn_set_params", C.c_int, [_P, _P]),
+    ("mpf_mpnn_zero_grad", C.c_int, [_P]),
+    ("mpf_mpnn_features", C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double, C.c_double]),
+    ("mpf_mpnn_forward", C.c_int, [_P, _P, _P, _P]),
+    ("mpf_mpnn_loss", C.c_int, [_P, _P, _P, _P, _P, C.c_double, _DP]),
+    ("mpf_mpnn_backward", C.c_int, [_P, _P, _P]),
+    ("mpf_mpnn_adam", C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("mpf_mpnn_guess", C.c_int, [_P, _P, _P, _P, C.c_double, C.c_int32, _P]),
     ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
@@ -543,9 +559,114 @@ class Problem:
         _check(st, "mpf_batch_newton_solve")
         return iters, rel, rep
 
+    def batch_newton_step(self, lam, guess=None, rep=None, itmax=20, rtol=1e-9, cg_maxit=5000, cg_rtol=1e-10,
+                          use_graph=True, chunk=0, tile=0, lanes_per_row=0):
+        """One load step of every mesh; guess: device tensor [nNode*3] (float64) or None.
+        Returns (iters[nmesh], relres[nmesh], rep) with rep accumulated across calls."""
+        o = BatchOpts(1, 1.0, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row)
+        iters = np.zeros(self.nmesh, dtype=np.int32)
+        rel = np.zeros(self.nmesh)
+        rep = BatchReport() if rep is None else rep
+        gp = None
+        if guess is not None:
+            import torch
+            assert guess.dtype == torch.float64 and guess.is_contiguous() and guess.numel() == 3 * self.nNode
+            torch.cuda.current_stream().synchronize()  # the problem runs on its own stream
+            gp = C.c_void_p(guess.data_ptr())
+        st = lib().mpf_batch_newton_step(self._h, lam, C.byref(o), gp, _ip(iters), _dp(rel), C.byref(rep))
+        _check(st, "mpf_batch_newton_step")
+        return iters, rel, rep
+
+    def state_device_ptr(self) -> int:
+        """Device address of the displacement [nNode*3] (float64), ordered on the problem's stream."""
+        d, pr = C.c_void_p(), C.c_void_p()
+        _check(lib().mpf_state_device_ptrs(self._h, C.byref(d), C.byref(pr)), "mpf_state_device_ptrs")
+        return d.value
+
+    def copy_state_device(self, dst):
+        """Copy the displacement into a device tensor dst (float64, nNode * 3 entries)."""
+        import torch
+        assert dst.dtype == torch.float64 and dst.is_contiguous() and dst.numel() == 3 * self.nNode
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_copy_state_device(self._h, C.c_void_p(dst.data_ptr())), "mpf_copy_state_device")
+
     def time_assembly(self, reps: int = 20, first_iter: bool = True, lam: float = 1.0):
         """(ms per full assembly, ms of the element-kernel launches per assembly), HIP events."""
         a, s = C.c_double(), C.c_double()
         _check(lib().mpf_time_assembly(self._h, reps, int(first_iter), lam, C.byref(a), C.byref(s)),
                "mpf_time_assembly")
         return a.value, s.value
+
+
+# ---------------------------------------------------------------------------------------------
+# learned initial guess (message-passing network with BF16 MFMA kernels)
+# ---------------------------------------------------------------------------------------------
+def _dptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr() if hasattr(t, "data_ptr") else int(t))
+
+
+class Mpnn:
+    """The learned-guess network on a problem's node graph (csrc/mpnn.hip)."""
+
+    FIN, WIDTH, LAYERS, FOUT = 32, 64, 3, 32
+
+    def __init__(self, seed: int = 4):
+        self._h = C.c_void_p()
+        _check(lib().mpf_mpnn_create(C.byref(self._h), seed), "mpf_mpnn_create")
+        self.nparams = lib().mpf_mpnn_num_params()
+
+    def close(self):
+        if self._h:
+            lib().mpf_mpnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        w = np.zeros(self.nparams, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().mpf_mpnn_get_params(self._h, w.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p)),
+               "mpf_mpnn_get_params")
+        return w, g
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().mpf_mpnn_set_params(self._h, w.ctypes.data_as(C.c_void_p)), "mpf_mpnn_set_params")
+
+    def zero_grad(self):
+        _check(lib().mpf_mpnn_zero_grad(self._h), "mpf_mpnn_zero_grad")
+
+    def features(self, P: Problem, uk, ukm1, rho: float, lam_next: float, lam_ratio: float):
+        _check(lib().mpf_mpnn_features(self._h, P._h, _dptr(uk), _dptr(ukm1), rho, lam_next, lam_ratio),
+               "mpf_mpnn_features")
+
+    def forward(self, P: Problem, x=None, y=None):
+        """x: device int16/bfloat16 tensor [nNode, 32] or None (the features); y: device float32 [nNode, 32]."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_mpnn_forward(self._h, P._h, None if x is None else _dptr(x), None if y is None else _dptr(y)),
+               "mpf_mpnn_forward")
+
+    def loss(self, P: Problem, unext, uk, ukm1, rho: float) -> float:
+        v = C.c_double()
+        _check(lib().mpf_mpnn_loss(self._h, P._h, _dptr(unext), _dptr(uk), _dptr(ukm1), rho, C.byref(v)),
+               "mpf_mpnn_loss")
+        return v.value
+
+    def backward(self, P: Problem, dy=None):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_mpnn_backward(self._h, P._h, None if dy is None else _dptr(dy)), "mpf_mpnn_backward")
+
+    def adam(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        _check(lib().mpf_mpnn_adam(self._h, lr, beta1, beta2, eps), "mpf_mpnn_adam")
+
+    def guess(self, P: Problem, uk, ukm1, rho: float, use_net: bool, out):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_mpnn_guess(self._h, P._h, _dptr(uk), _dptr(ukm1), rho, int(use_net), _dptr(out)),
+               "mpf_mpnn_guess")

@@ -49,13 +49,14 @@ struct BatchWork {
   DevBuf<int32_t> mf;   // [nmesh][kMF]
   DevBuf<int32_t> cnt;  // [4]: CG-active meshes, Newton-active meshes, diverged meshes, negative Jacobian
   int32_t* cnt_h = nullptr;  // pinned host mirror
-  hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr;
+  hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr, g_r0 = nullptr;
   double g_lam = 0.0;
   ~BatchWork() { release_graphs(); if (cnt_h) (void)hipHostFree(cnt_h); }
   void release_graphs() {
     for (auto& g : g_asm) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     for (auto& g : g_upd) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (g_chunk) { (void)hipGraphExecDestroy(g_chunk); g_chunk = nullptr; }
+    if (g_r0) { (void)hipGraphExecDestroy(g_r0); g_r0 = nullptr; }
   }
 };
 
@@ -305,6 +306,13 @@ __global__ void __launch_bounds__(kB) k_b_beta(int32_t nmesh, const int32_t* __r
   count_into(act, cnt);
 }
 
+// prescribed dofs at their values for load factor lam (the start point of a guessed step)
+__global__ void k_b_set_prescribed(int32_t nbc, const int32_t* __restrict__ dof, const double* __restrict__ val,
+                                   double lam, double* __restrict__ disp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbc) disp[dof[i]] = val[i] * lam;
+}
+
 __global__ void k_b_zero_i32(int32_t* a, int32_t n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -417,6 +425,12 @@ double secs(std::chrono::steady_clock::time_point t0) {
 }
 
 }  // namespace
+
+mpf_status batch_node_offsets(const mpf_problem* p, std::vector<int32_t>& off) {
+  if (!p->batch || p->batch->nmesh <= 0) return MPF_ERR_STATE;
+  off = p->batch->node_off;
+  return MPF_OK;
+}
 
 void batch_free(mpf_problem* p) {
   delete p->batch;
@@ -546,17 +560,17 @@ mpf_status batch_autotune(mpf_problem* p, int32_t* T_out, int32_t* LPR_out, doub
   return MPF_OK;
 }
 
-mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
-                              mpf_batch_report* rep) {
+static mpf_status batch_configure(mpf_problem* p, const mpf_batch_opts* o, mpf_batch_report* rep) {
   if (!p->batch || !p->prepared || !o || !rep) return MPF_ERR_STATE;
-  if (o->nsteps <= 0 || o->dt <= 0.0 || o->itmax <= 0 || o->cg_maxit <= 0) return MPF_ERR_ARG;
+  if (o->itmax <= 0 || o->cg_maxit <= 0) return MPF_ERR_ARG;
   MPF_TRY(batch_prepare(p));
   BatchWork& B = *p->batch;
-  hipStream_t s = p->stream;
-  *rep = mpf_batch_report{};
-  if (o->chunk > 0) B.chunk = o->chunk;
+  if (o->chunk > 0 && o->chunk != B.chunk) {
+    B.chunk = o->chunk;
+    B.release_graphs();
+  }
   if (o->tile > 0 && o->tile != B.T) {
-    MPF_TRY(build_tiles(B, o->tile, s));
+    MPF_TRY(build_tiles(B, o->tile, p->stream));
     B.release_graphs();
   }
   if (o->lanes_per_row == 4 || o->lanes_per_row == 8 || o->lanes_per_row == 16) {
@@ -566,83 +580,135 @@ mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* 
   rep->tile = B.T;
   rep->lanes_per_row = B.LPR;
   rep->chunk = B.chunk;
+  return MPF_OK;
+}
+
+// residual at the plain start of a step (with the lift) -> R0 of every mesh
+static mpf_status enqueue_r0(mpf_problem* p, BatchWork& B, double lam) {
+  hipStream_t s = p->stream;
+  const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
+  hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, s, p->bad.ptr, 1);
+  MPF_TRY(enqueue_assembly(p, true, lam, 0));
+  hipLaunchKernelGGL(k_b_dot, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, p->rhs.ptr, p->rhs.ptr, B.part.ptr, 0);
+  hipLaunchKernelGGL(k_b_newton, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr, 1, 0.0,
+                     1.0, B.cnt.ptr);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+static mpf_status reset_mesh_flags(mpf_problem* p, BatchWork& B) {
+  std::vector<int32_t> init((size_t)B.nmesh * kMF, 0);
+  for (int m = 0; m < B.nmesh; ++m) init[(size_t)m * kMF + kNA] = 1;
+  MPF_HIP(hipMemcpyAsync(B.mf.ptr, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  return MPF_OK;
+}
+
+// One load step of every mesh at load factor lam from the current converged state.  guess (device,
+// [nNode*3], may be null): displacement to start Newton from; R0 is still taken at the plain start.
+static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
+                             int32_t* iters_row, double* mesh_relres, mpf_batch_report* rep) {
+  BatchWork& B = *p->batch;
+  hipStream_t s = p->stream;
   const bool G = o->use_graph != 0;
   const size_t n3 = (size_t)p->nNode * 3;
   auto t_all = std::chrono::steady_clock::now();
-  mpf_status result = MPF_OK;
-  std::vector<int32_t> mfh((size_t)B.nmesh * kMF);
-  std::vector<double> msh((size_t)B.nmesh * kMS);
-  double t = 0.0;
-  for (int step = 0; step < o->nsteps; ++step) {
-    t += o->dt;
-    const double lam = t;  // lambda(t) = t
-    // saveSolution, setBoundaryConditions
-    MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MPF_TRY(mpf_set_bc(p, lam));
-    {
-      // every mesh starts the step Newton-active, counters at zero
-      std::vector<int32_t> init((size_t)B.nmesh * kMF, 0);
-      for (int m = 0; m < B.nmesh; ++m) init[(size_t)m * kMF + kNA] = 1;
-      MPF_HIP(hipMemcpyAsync(B.mf.ptr, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      MPF_HIP(hipStreamSynchronize(s));
+  // saveSolution, setBoundaryConditions
+  MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  MPF_TRY(mpf_set_bc(p, lam));
+  MPF_TRY(reset_mesh_flags(p, B));
+  if (G && (B.g_lam != lam || !B.g_asm[0])) {
+    for (int f = 0; f < 2; ++f) {
+      MPF_TRY(capture(p, &B.g_asm[f], [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+      MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
     }
-    if (G && (B.g_lam != lam || !B.g_asm[0])) {
-      for (int f = 0; f < 2; ++f) {
-        MPF_TRY(capture(p, &B.g_asm[f], [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
-        MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
-      }
-      MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
+    MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
+      for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
+      return MPF_OK;
+    }));
+    MPF_TRY(capture(p, &B.g_r0, [&] { return enqueue_r0(p, B, lam); }));
+    B.g_lam = lam;
+  }
+  if (guess) {
+    auto ta = std::chrono::steady_clock::now();
+    MPF_TRY(launch(p, B.g_r0, G, [&] { return enqueue_r0(p, B, lam); }));
+    ++rep->graph_launches;
+    MPF_HIP(hipMemcpyAsync(p->disp.ptr, guess, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int32_t nbc = (int32_t)p->bc_dof_h.size();
+    if (nbc > 0)
+      hipLaunchKernelGGL(k_b_set_prescribed, dim3(grid_for(nbc, 256)), dim3(256), 0, s, nbc, p->bc_dof.ptr,
+                         p->bc_val.ptr, lam, p->disp.ptr);
+    MPF_TRY(reset_mesh_flags(p, B));
+    rep->seconds_assembly += secs(ta);
+  }
+  mpf_status result = MPF_OK;
+  bool step_conv = false;
+  for (int iter = 1; iter <= o->itmax + 1; ++iter) {
+    const int f = (iter == 1 && !guess) ? 1 : 0;
+    auto ta = std::chrono::steady_clock::now();
+    MPF_TRY(launch(p, B.g_asm[f], G, [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+    ++rep->graph_launches;
+    MPF_TRY(read_counts(p, B));
+    rep->seconds_assembly += secs(ta);
+    int32_t bad = 0;
+    MPF_HIP(hipMemcpy(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (bad) { result = MPF_ERR_NEG_JACOBIAN; break; }
+    if (B.cnt_h[2] > 0) { result = MPF_ERR_DIVERGED; break; }
+    if (B.cnt_h[1] == 0) { step_conv = true; break; }
+    if (iter > o->itmax) break;
+    ++rep->newton_iters;
+    auto ts = std::chrono::steady_clock::now();
+    int its = 0;
+    while (B.cnt_h[0] > 0 && its < o->cg_maxit) {
+      MPF_TRY(launch(p, B.g_chunk, G, [&]() -> mpf_status {
         for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
         return MPF_OK;
       }));
-      B.g_lam = lam;
-    }
-    bool step_conv = false;
-    for (int iter = 1; iter <= o->itmax + 1; ++iter) {
-      const int f = iter == 1 ? 1 : 0;
-      auto ta = std::chrono::steady_clock::now();
-      MPF_TRY(launch(p, B.g_asm[f], G, [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
       ++rep->graph_launches;
+      its += B.chunk;
       MPF_TRY(read_counts(p, B));
-      rep->seconds_assembly += secs(ta);
-      int32_t bad = 0;
-      MPF_HIP(hipMemcpy(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
-      if (bad) { result = MPF_ERR_NEG_JACOBIAN; break; }
-      if (B.cnt_h[2] > 0) { result = MPF_ERR_DIVERGED; break; }
-      if (B.cnt_h[1] == 0) { step_conv = true; break; }
-      if (iter > o->itmax) break;
-      ++rep->newton_iters;
-      auto ts = std::chrono::steady_clock::now();
-      int its = 0;
-      while (B.cnt_h[0] > 0 && its < o->cg_maxit) {
-        MPF_TRY(launch(p, B.g_chunk, G, [&]() -> mpf_status {
-          for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
-          return MPF_OK;
-        }));
-        ++rep->graph_launches;
-        its += B.chunk;
-        MPF_TRY(read_counts(p, B));
-      }
-      rep->cg_iters_max += its;
-      MPF_TRY(launch(p, B.g_upd[f], G, [&] { return do_update(p, f == 1); }));
-      ++rep->graph_launches;
-      rep->seconds_solve += secs(ts);
     }
-    MPF_HIP(hipMemcpyAsync(mfh.data(), B.mf.ptr, mfh.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    MPF_HIP(hipMemcpyAsync(msh.data(), B.ms.ptr, msh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    MPF_HIP(hipStreamSynchronize(s));
-    for (int m = 0; m < B.nmesh; ++m) {
-      const int32_t it = mfh[(size_t)m * kMF + kNIT] + 1;  // assemblies to convergence
-      if (mesh_iters) mesh_iters[(size_t)step * B.nmesh + m] = it;
-      if (mesh_relres) mesh_relres[m] = msh[(size_t)m * kMS + kREL];
-      rep->cg_iters_total += mfh[(size_t)m * kMF + kCGT] + mfh[(size_t)m * kMF + kCGIT];
-    }
-    if (result != MPF_OK) break;
-    rep->steps_done = step + 1;
-    if (!step_conv) { result = MPF_ERR_NOT_CONVERGED; break; }
+    rep->cg_iters_max += its;
+    MPF_TRY(launch(p, B.g_upd[f], G, [&] { return do_update(p, f == 1); }));
+    ++rep->graph_launches;
+    rep->seconds_solve += secs(ts);
   }
-  rep->seconds = secs(t_all);
-  return result;
+  std::vector<int32_t> mfh((size_t)B.nmesh * kMF);
+  std::vector<double> msh((size_t)B.nmesh * kMS);
+  MPF_HIP(hipMemcpyAsync(mfh.data(), B.mf.ptr, mfh.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipMemcpyAsync(msh.data(), B.ms.ptr, msh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < B.nmesh; ++m) {
+    if (iters_row) iters_row[m] = mfh[(size_t)m * kMF + kNIT] + 1;  // assemblies to convergence
+    if (mesh_relres) mesh_relres[m] = msh[(size_t)m * kMS + kREL];
+    rep->cg_iters_total += mfh[(size_t)m * kMF + kCGT] + mfh[(size_t)m * kMF + kCGIT];
+  }
+  rep->seconds += secs(t_all);
+  if (result != MPF_OK) return result;
+  if (!step_conv) return MPF_ERR_NOT_CONVERGED;
+  rep->steps_done += 1;
+  return MPF_OK;
+}
+
+mpf_status batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
+                             int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep) {
+  MPF_TRY(batch_configure(p, o, rep));
+  return batch_step(p, lam, o, guess, mesh_iters, mesh_relres, rep);
+}
+
+mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
+                              mpf_batch_report* rep) {
+  if (!rep) return MPF_ERR_ARG;
+  *rep = mpf_batch_report{};
+  MPF_TRY(batch_configure(p, o, rep));
+  if (o->nsteps <= 0 || o->dt <= 0.0) return MPF_ERR_ARG;
+  const int nmesh = p->batch->nmesh;
+  for (int step = 0; step < o->nsteps; ++step) {
+    const double lam = (step + 1) * o->dt;  // lambda(t) = t
+    MPF_TRY(batch_step(p, lam, o, nullptr, mesh_iters ? mesh_iters + (size_t)step * nmesh : nullptr, mesh_relres,
+                       rep));
+  }
+  return MPF_OK;
 }
 
 }  // namespace mpf
@@ -664,6 +730,12 @@ mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32
                                   mpf_batch_report* rep) {
   if (!p) return MPF_ERR_ARG;
   return mpf::batch_newton_solve(p, o, mesh_iters, mesh_relres, rep);
+}
+
+mpf_status mpf_batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
+                                 int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep) {
+  if (!p) return MPF_ERR_ARG;
+  return mpf::batch_newton_step(p, lam, o, guess, mesh_iters, mesh_relres, rep);
 }
 
 }  // extern "C"

@@ -189,11 +189,14 @@ mpf_status build_pattern(mpf_problem* p);          // pattern.hip
 // batch.hip
 ElemArgs batch_elem_args(mpf_problem* p, ElemArgs a);
 void batch_free(mpf_problem* p);
+mpf_status batch_node_offsets(const mpf_problem* p, std::vector<int32_t>& off);
 mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                      const double* mu, const double* Kinv);
 mpf_status batch_autotune(mpf_problem* p, int32_t* T, int32_t* LPR, double* ms_per_iter);
 mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
                               mpf_batch_report* rep);
+mpf_status batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
+                             int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep);
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
 void krylov_free(mpf_problem* p);
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);

@@ -584,6 +584,21 @@ mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_
   return MPF_OK;
 }
 
+mpf_status mpf_state_device_ptrs(mpf_problem* p, double** disp, double** pres) {
+  if (!p) return MPF_ERR_ARG;
+  if (disp) *disp = p->disp.ptr;
+  if (pres) *pres = p->pres.ptr;
+  return MPF_OK;
+}
+
+mpf_status mpf_copy_state_device(mpf_problem* p, double* disp_dst) {
+  if (!p || !disp_dst) return MPF_ERR_ARG;
+  MPF_HIP(hipMemcpyAsync(disp_dst, p->disp.ptr, (size_t)p->nNode * 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                         p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  return MPF_OK;
+}
+
 mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
   if (!p) return MPF_ERR_ARG;
   hipStream_t s = p->stream;

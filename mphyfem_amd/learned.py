@@ -1,0 +1,135 @@
+"""Learned Newton initial guess: cantilever workload, trajectories from the plain batched solver,
+training of the message-passing network (BF16 MFMA kernels, csrc/mpnn.hip) and the comparison of
+plain Newton, linear extrapolation and the learned guess on held-out beams.
+
+Everything numerical runs in libmpf (the batched Newton/PCG solver and the network's kernels);
+torch only holds device memory for the trajectories.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import HEX8, TANGENT_CONSISTENT, HexBatch, Mpnn, Problem, SplitMix64, face_nodes, hex_face_quads, matl, \
+    mesh_hex
+
+H_ELEM, W_ELEM = 8, 4      # beam height (z) and width (y) in elements, element size 1/8
+NSTEPS = 20                # tip load ramped over 20 load steps, lambda = k / 20
+NU = 0.3
+TIP_KAPPA = 0.075          # tip shear tau = TIP_KAPPA * E / length^2 (large-deflection regime)
+
+
+@dataclass
+class Beams:
+    lengths: np.ndarray    # elements along x
+    mu: np.ndarray
+
+
+def cantilever_workload(nbeams: int = 2000, seed: int = 4) -> Beams:
+    """Beams of length L = round(U[4, 12] * 8) elements, mu log-uniform in [1, 100] (SplitMix64)."""
+    g = SplitMix64(seed)
+    L, mu = [], []
+    for _ in range(nbeams):
+        L.append(int(round((4.0 + 8.0 * g.uniform()) * 8.0)))
+        mu.append(10.0 ** (2.0 * g.uniform()))
+    return Beams(np.array(L, dtype=np.int32), np.array(mu))
+
+
+def cantilever_problem(beams: Beams, idx) -> tuple[HexBatch, Problem]:
+    """Batched problem of the selected beams: x = 0 face fixed, tip face (x = length) under a dead
+    shear traction along -z, in the reference's (normal, tangent1) form: on the tip face tangent1 =
+    dx/dxi = +z h/2, so v1 = -2 tau / h gives the shear tau."""
+    h = 1.0 / H_ELEM
+    Xs, Cs, bottom, top, node_off, elem_off = [], [], [], [], [0], [0]
+    mus, kinv, v1 = [], [], []
+    for i in idx:
+        n = int(beams.lengths[i])
+        X, conn = mesh_hex(n, W_ELEM, H_ELEM, L=(n * h, W_ELEM * h, H_ELEM * h))
+        o = node_off[-1]
+        Xs.append(X)
+        Cs.append(conn + o)
+        bottom.append(face_nodes(HEX8, n, W_ELEM, H_ELEM, 0) + o)
+        top.append(hex_face_quads(n, W_ELEM, H_ELEM, 1) + o)
+        node_off.append(o + len(X))
+        elem_off.append(elem_off[-1] + len(conn))
+        mu = float(beams.mu[i])
+        E = 2.0 * mu * (1.0 + NU)
+        mus.append(mu)
+        kinv.append(3.0 * (1.0 - 2.0 * NU) / E)
+        tau = TIP_KAPPA * E / (n * h) ** 2
+        v1.append(-2.0 * tau / h)
+    B = HexBatch(np.concatenate(Xs), np.concatenate(Cs).astype(np.int32), np.array(node_off, dtype=np.int32),
+                 np.array(elem_off, dtype=np.int32), beams.lengths[list(idx)], bottom, top)
+    P = Problem(HEX8, B.X, B.conn, matl(mus[0], kinv[0], 3, False, TANGENT_CONSISTENT))
+    P.set_batch(B.node_off, B.elem_off, mus, kinv)
+    for m in range(len(idx)):
+        P.add_dirichlet(bottom[m], -1)
+        P.add_traction(top[m], 0.0, v1[m])
+    P.prepare_matrix_pattern()
+    return B, P
+
+
+@dataclass
+class StepLog:
+    iters: list = field(default_factory=list)      # per step: assemblies per mesh (np.int32 array)
+    seconds: list = field(default_factory=list)    # per step: wall time of the batched step
+    newton: list = field(default_factory=list)     # per step: batch Newton iterations (slowest mesh)
+
+
+def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTEPS, store=None, **solver):
+    """Ramp the load over nsteps with Newton started from: 'plain' the previous converged solution,
+    'extrap' the linear extrapolation, 'learned' the network's guess.  store: optional device tensor
+    [nsteps + 1, nNode * 3] receiving u_0 .. u_nsteps."""
+    import torch
+
+    n3 = 3 * P.nNode
+    P.set_state(np.zeros(n3))
+    dev = torch.device("cuda")
+    hist = [torch.zeros(n3, dtype=torch.float64, device=dev) for _ in range(2)]  # u_{k-1}, u_k
+    guess = torch.zeros(n3, dtype=torch.float64, device=dev)
+    if store is not None:
+        store[0].zero_()
+    log = StepLog()
+    for k in range(nsteps):
+        lam = (k + 1) / nsteps
+        g = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()  # the guess is part of the step's wall time
+        if mode != "plain" and k >= 1:
+            rho = 1.0  # equal load increments
+            net.features(P, hist[1], hist[0], rho, lam, k / (k + 1.0))
+            if mode == "learned":
+                net.forward(P)
+            net.guess(P, hist[1], hist[0], rho, mode == "learned", guess)
+            g = guess
+        iters, rel, rep = P.batch_newton_step(lam, guess=g, **solver)
+        log.seconds.append(time.perf_counter() - t0)
+        log.iters.append(iters.copy())
+        log.newton.append(rep.newton_iters)
+        hist[0].copy_(hist[1])
+        P.copy_state_device(hist[1])
+        if store is not None:
+            store[k + 1].copy_(hist[1])
+    return log
+
+
+def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, log_every: int = 10):
+    """Full-batch steps over the stored trajectories u_0..u_N: sample k predicts u_{k+1} from u_k,
+    u_{k-1} (k = 1 .. N-1).  Returns the mean loss per epoch."""
+    nsteps = traj.shape[0] - 1
+    losses = []
+    for ep in range(epochs):
+        tot = 0.0
+        for k in range(1, nsteps):
+            lam = (k + 1) / nsteps
+            net.features(P, traj[k], traj[k - 1], 1.0, lam, k / (k + 1.0))
+            net.forward(P)
+            tot += net.loss(P, traj[k + 1], traj[k], traj[k - 1], 1.0)
+            net.backward(P)
+            net.adam(lr=lr * (0.5 ** (ep / max(epochs / 3, 1))))
+        losses.append(tot / (nsteps - 1))
+        if log_every and (ep % log_every == 0 or ep == epochs - 1):
+            print(f"[learned] epoch {ep}: loss {losses[-1]:.4e}", flush=True)
+    return losses
