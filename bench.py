@@ -102,6 +102,58 @@ def read_traffic(name: str):
         return None
 
 
+SOLVER = dict(itmax=25, rtol=1e-9, cg_maxit=20000, cg_rtol=1e-10, chunk=32)
+
+
+def bench_batch(M, a) -> int:
+    """One load step (lambda = 1) of every block of the batched workload, converged per block to
+    ||R||/||R0|| < 1e-9, after the tile autotuner."""
+    import torch
+
+    edges, E, nu, v0, v1 = M.batch_workload(a.nmesh)
+    t0 = time.perf_counter()
+    B, P = M.batch_problem(edges, E, nu, v0, v1)
+    P.prepare_matrix_pattern()
+    t_setup = time.perf_counter() - t0
+    nElem, nNode = len(B.conn), len(B.X)
+    log(f"batch: {a.nmesh} blocks, {nElem} elements, {nNode} nodes, setup {t_setup:.2f} s")
+    T, L, ms_it = P.batch_autotune()
+    log(f"autotune: tile {T}, {L} lanes per row, {ms_it:.3f} ms per CG iteration")
+    torch.cuda.synchronize()
+    iters, rel, rep = P.batch_newton_solve(nsteps=1, dt=1.0, **SOLVER)
+    log(f"newton: {rep.newton_iters} batch iterations, max CG {rep.cg_iters_max}, {rep.seconds:.3f} s")
+    out = {"metric": "s per converged load step (256 Hex8 blocks, batched Newton + Jacobi-PCG)",
+           "value": round(rep.seconds, 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(rep.seconds * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{a.nmesh} unit-cube Hex8 blocks, edge U{{8..32}}, E log-U[1e2,1e4], "
+                                  f"nu U[0.2,0.45], top dead traction U[0.01,0.1] E, bottom fixed, seed 3",
+                      "elements": nElem, "nodes": nNode, "tile": T, "lanes_per_row": L},
+           "newton": {"batch_iterations": rep.newton_iters, "assemblies_per_block_mean": float(iters.mean()),
+                      "assemblies_per_block_max": int(iters.max()), "cg_iterations_slowest": rep.cg_iters_max,
+                      "cg_iterations_all_blocks": rep.cg_iters_total, "graph_launches": rep.graph_launches,
+                      "seconds_assembly": round(rep.seconds_assembly, 4), "seconds_solve": round(rep.seconds_solve, 4),
+                      "max_relres": float(rel.max())}}
+    P.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_learned(M, a) -> int:
+    from mphyfem_amd import learned
+
+    r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs, **SOLVER)
+    out = {"metric": "s per converged load step, learned vs plain Newton start (held-out cantilevers)",
+           "value": round(r["learned"]["seconds"] / max(r["steps"] - 1, 1), 4), "unit": "s",
+           "n_gpus": 1, "steps": r["steps"], "warmup": 0, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 solve, bf16/fp32 network", "data": "synthetic",
+           "config": {"workload": "Hex8 cantilevers H=8 W=4 L=round(U[4,12]*8), mu log-U[1,100], nu=0.3, "
+                                  "tip shear ramped over 20 steps, seed 4"},
+           "learned": r}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration() -> int:
     """Three device-to-device copies of 1 GiB: what FETCH_SIZE / WRITE_SIZE read for known traffic."""
     import torch
@@ -121,7 +173,14 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tet10", "hex8"], default="tet10")
+    ap.add_argument("--workload", choices=["tet10", "hex8", "batch", "learned"], default="tet10",
+                    help="tet10/hex8: assembly; batch: 256 independent Hex8 blocks, batched Newton + Jacobi-PCG; "
+                         "learned: cantilever beams, plain vs learned Newton initial guess")
+    ap.add_argument("--nmesh", type=int, default=256, help="batch: number of blocks")
+    ap.add_argument("--beams", type=int, default=2000, help="learned: beams (the last --test are held out)")
+    ap.add_argument("--train", type=int, default=1800)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
     ap.add_argument("--krylov-maxit", type=int, default=300)
     ap.add_argument("--no-krylov", action="store_true")
@@ -147,6 +206,10 @@ def main(argv=None) -> int:
         return 1
     torch.cuda.init()
     wl = a.workload
+    if wl == "batch":
+        return bench_batch(M, a)
+    if wl == "learned":
+        return bench_learned(M, a)
     n = a.n or (60 if wl == "tet10" else 128)
     log(f"workload {wl}, n = {n}")
     X, conn, P, _ = build_problem(M, n, wl)

@@ -147,6 +147,7 @@ struct BatchWork;   // batch.hip
 }  // namespace mpf
 
 struct mpf_problem {
+  uint64_t uid = 0;  // unique per process (caches keyed by problem must not trust a reused address)
   int32_t etype = 0, npe = 0, nNode = 0, nElem = 0;
   mpf_matl m{};
   hipStream_t stream = nullptr;

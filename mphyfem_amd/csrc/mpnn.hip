@@ -59,7 +59,7 @@ struct mpf_mpnn {
   hipStream_t stream = nullptr;
   int64_t adam_t = 0;
   // per-problem context of the learned-guess data path
-  const mpf_problem* ctx = nullptr;
+  uint64_t ctx = 0;  // uid of the problem the context below was built for
   int32_t nmesh = 0;
   mpf::DevBuf<int32_t> node_off, nmesh_of, fixed;
   mpf::DevBuf<double> xref;
@@ -532,7 +532,7 @@ using namespace mpf;
 namespace {
 // node -> mesh, per-mesh reference frame, prescribed-node flags of a (batched) problem
 mpf_status ensure_ctx(mpf_mpnn* net, mpf_problem* p) {
-  if (net->ctx == p) return MPF_OK;
+  if (net->ctx == p->uid) return MPF_OK;
   if (!p->prepared) return MPF_ERR_STATE;
   const int32_t nNode = p->nNode;
   std::vector<int32_t> off;
@@ -562,7 +562,7 @@ mpf_status ensure_ctx(mpf_mpnn* net, mpf_problem* p) {
   MPF_TRY(net->scale.alloc(nm));
   MPF_HIP(hipStreamSynchronize(net->stream));
   net->nmesh = nm;
-  net->ctx = p;
+  net->ctx = p->uid;
   return MPF_OK;
 }
 
@@ -710,7 +710,7 @@ mpf_status mpf_mpnn_features(mpf_mpnn* net, mpf_problem* p, const double* uk, co
 // left in the network for mpf_mpnn_backward (dy = NULL there).
 mpf_status mpf_mpnn_loss(mpf_mpnn* net, mpf_problem* p, const double* unext, const double* uk, const double* ukm1,
                          double rho, double* loss) {
-  if (!net || !p || !unext || !uk || !ukm1 || net->ctx != p) return MPF_ERR_ARG;
+  if (!net || !p || !unext || !uk || !ukm1 || net->ctx != p->uid) return MPF_ERR_ARG;
   const int64_t n = p->nNode;
   const unsigned nb = g256(n);
   if (net->red.n < nb) MPF_TRY(net->red.alloc(nb));
@@ -730,7 +730,7 @@ mpf_status mpf_mpnn_loss(mpf_mpnn* net, mpf_problem* p, const double* unext, con
 // Newton start point for step k+1: u_k + rho (u_k - u_{k-1}) (+ s_m y of the last forward when use_net).
 mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const double* ukm1, double rho,
                           int32_t use_net, double* guess) {
-  if (!net || !p || !uk || !ukm1 || !guess || net->ctx != p) return MPF_ERR_ARG;
+  if (!net || !p || !uk || !ukm1 || !guess || net->ctx != p->uid) return MPF_ERR_ARG;
   const int64_t n = p->nNode;
   hipLaunchKernelGGL(k_guess, dim3(g256(n)), dim3(256), 0, net->stream, n, net->nmesh_of.ptr, net->scale.ptr, uk,
                      ukm1, rho, use_net, net->y.ptr, guess);

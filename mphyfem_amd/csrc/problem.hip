@@ -10,6 +10,7 @@
 //   converged / diverging           femSolidmechanics1.cpp:616-636
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -429,7 +430,9 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   for (int64_t i = 0; i < (int64_t)nElem * npe; ++i)
     if (conn[i] < 0 || conn[i] >= nNode) return MPF_ERR_ARG;
   MPF_TRY(check_device());
+  static std::atomic<uint64_t> next_uid{1};
   auto* p = new mpf_problem();
+  p->uid = next_uid++;
   p->etype = elem_type;
   p->npe = npe;
   p->nNode = nNode;

@@ -133,3 +133,47 @@ def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, log_e
         if log_every and (ep % log_every == 0 or ep == epochs - 1):
             print(f"[learned] epoch {ep}: loss {losses[-1]:.4e}", flush=True)
     return losses
+
+
+def summarize(log: StepLog, first: int = 1) -> dict:
+    """Mean assemblies per mesh and step, batch Newton iterations and wall time over steps >= first
+    (step 0 has no history, every method starts it from the previous solution)."""
+    it = np.array(log.iters[first:], dtype=np.float64)
+    return {"assemblies_per_mesh_step": round(float(it.mean()), 3),
+            "batch_newton_iters": int(np.sum(log.newton[first:])),
+            "seconds": round(float(np.sum(log.seconds[first:])), 4),
+            "seconds_all_steps": round(float(np.sum(log.seconds)), 4)}
+
+
+def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs: int = 60, lr: float = 2e-3,
+               nsteps: int = NSTEPS, seed: int = 4, autotune: bool = True, **solver) -> dict:
+    """Trajectories of the plain solver on the training beams, training, then plain / extrapolated /
+    learned Newton on the held-out beams.  Returns the measurements."""
+    import torch
+
+    beams = cantilever_workload(nbeams, seed)
+    train_idx = list(range(ntrain))
+    test_idx = list(range(nbeams - ntest, nbeams))
+    out = {"beams": nbeams, "train": ntrain, "test": ntest, "steps": nsteps, "epochs": epochs}
+    t0 = time.perf_counter()
+    _, Ptr = cantilever_problem(beams, train_idx)
+    if autotune:
+        out["tile"], out["lanes_per_row"], _ = Ptr.batch_autotune()
+    traj = torch.zeros((nsteps + 1, 3 * Ptr.nNode), dtype=torch.float64, device="cuda")
+    log_tr = run_steps(Ptr, "plain", None, nsteps, store=traj, **solver)
+    out["train_trajectories_seconds"] = round(time.perf_counter() - t0, 2)
+    out["train_plain"] = summarize(log_tr)
+    net = Mpnn(seed=seed)
+    t1 = time.perf_counter()
+    losses = train(net, Ptr, traj, epochs=epochs, lr=lr)
+    out["train_seconds"] = round(time.perf_counter() - t1, 2)
+    out["loss_first"], out["loss_last"] = losses[0], losses[-1]
+    del traj, Ptr
+    _, Pte = cantilever_problem(beams, test_idx)
+    if autotune:
+        Pte.batch_autotune()
+    for mode in ("plain", "extrap", "learned"):
+        out[mode] = summarize(run_steps(Pte, mode, net, nsteps, **solver))
+    out["speedup_learned_vs_plain"] = round(out["plain"]["seconds"] / max(out["learned"]["seconds"], 1e-12), 3)
+    out["speedup_extrap_vs_plain"] = round(out["plain"]["seconds"] / max(out["extrap"]["seconds"], 1e-12), 3)
+    return out
