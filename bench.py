@@ -142,7 +142,8 @@ def bench_batch(M, a) -> int:
 def bench_learned(M, a) -> int:
     from mphyfem_amd import learned
 
-    r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs, **SOLVER)
+    r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs,
+                           residual_epochs=a.residual_epochs, **SOLVER)
     out = {"metric": "s per converged load step, learned vs plain Newton start (held-out cantilevers)",
            "value": round(r["learned"]["seconds"] / max(r["steps"] - 1, 1), 4), "unit": "s",
            "n_gpus": 1, "steps": r["steps"], "warmup": 0, "higher_is_better": False, "scaling": "weak",
@@ -181,6 +182,8 @@ def main(argv=None) -> int:
     ap.add_argument("--train", type=int, default=1800)
     ap.add_argument("--test", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--residual-epochs", type=int, default=0,
+                    help="learned: extra epochs on the linearised start residual")
     ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
     ap.add_argument("--krylov-maxit", type=int, default=300)
     ap.add_argument("--no-krylov", action="store_true")

@@ -229,7 +229,7 @@ typedef struct {
   int32_t itmax;          /* Newton iterations per step */
   double rtol;            /* per mesh: ||R_m|| / ||R0_m|| < rtol, R0_m at the plain start of the step */
   int32_t cg_maxit;       /* CG iterations per solve */
-  double cg_rtol;         /* lower bound of the inexact-Newton CG tolerance min(0.1, 0.1 ||R||/||R0||) */
+  double cg_rtol;         /* lower bound of the inexact-Newton CG tolerance (Eisenstat-Walker forcing) */
   int32_t use_graph;      /* 1: replay captured HIP graphs; 0: launch the same kernels directly */
   int32_t chunk;          /* CG iterations per graph launch (0: keep) */
   int32_t tile;           /* dofs per tile (0: keep the autotuned / default value) */
@@ -244,6 +244,7 @@ typedef struct {
   int32_t graph_launches;
   int32_t tile, lanes_per_row, chunk;
   double seconds, seconds_assembly, seconds_solve;
+  double start_relres_mean, start_relres_max;  /* ||R_m|| / ||R0_m|| at the start point (last step) */
 } mpf_batch_report;
 
 /* Jacobi-PCG tile shape by measurement: every (tile, lanes_per_row) candidate is timed over a     */
@@ -291,6 +292,12 @@ mpf_status mpf_mpnn_forward(mpf_mpnn* net, mpf_problem* p, const uint16_t* x, fl
 /* dL/dy in the network.                                                                          */
 mpf_status mpf_mpnn_loss(mpf_mpnn* net, mpf_problem* p, const double* unext, const double* uk, const double* ukm1,
                          double rho, double* loss);
+/* Linearised start residual of the last forward's prediction, the quantity Newton's convergence  */
+/* depends on: mean over meshes of ||K e_m||^2 / (dlam^2 ||f_m||^2), e = predicted - converged     */
+/* u_{k+1} on free dofs, K the tangent at the converged u_{k+1} (assembled here; overwrites the     */
+/* problem's state and system), f_m the mesh's unit external load.  Leaves dL/dy in the network.   */
+mpf_status mpf_mpnn_residual_loss(mpf_mpnn* net, mpf_problem* p, double lam_next, double dlam, const double* unext,
+                                  const double* uk, const double* ukm1, double rho, double* loss);
 /* Gradients of the last forward for dL/dy = dy [nNode][32] FP32 (device), or the one mpf_mpnn_loss */
 /* left when dy is NULL; they accumulate until mpf_mpnn_adam / mpf_mpnn_zero_grad.                 */
 mpf_status mpf_mpnn_backward(mpf_mpnn* net, mpf_problem* p, const float* dy);

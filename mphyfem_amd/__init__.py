@@ -79,7 +79,8 @@ class BatchReport(C.Structure):
     _fields_ = [("steps_done", C.c_int32), ("newton_iters", C.c_int32), ("cg_iters_total", C.c_int64),
                 ("cg_iters_max", C.c_int64), ("graph_launches", C.c_int32), ("tile", C.c_int32),
                 ("lanes_per_row", C.c_int32), ("chunk", C.c_int32), ("seconds", C.c_double),
-                ("seconds_assembly", C.c_double), ("seconds_solve", C.c_double)]
+                ("seconds_assembly", C.c_double), ("seconds_solve", C.c_double),
+                ("start_relres_mean", C.c_double), ("start_relres_max", C.c_double)]
 
 
 _P = C.c_void_p
@@ -132,6 +133,7 @@ SIGNATURES = [
     ("mpf_mpnn_features", C.c_int, [_P, _P, _P, _P, C.c_double, C.c_double, C.c_double]),
     ("mpf_mpnn_forward", C.c_int, [_P, _P, _P, _P]),
     ("mpf_mpnn_loss", C.c_int, [_P, _P, _P, _P, _P, C.c_double, _DP]),
+    ("mpf_mpnn_residual_loss", C.c_int, [_P, _P, C.c_double, C.c_double, _P, _P, _P, C.c_double, _DP]),
     ("mpf_mpnn_backward", C.c_int, [_P, _P, _P]),
     ("mpf_mpnn_adam", C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
     ("mpf_mpnn_guess", C.c_int, [_P, _P, _P, _P, C.c_double, C.c_int32, _P]),
@@ -655,6 +657,13 @@ class Mpnn:
         v = C.c_double()
         _check(lib().mpf_mpnn_loss(self._h, P._h, _dptr(unext), _dptr(uk), _dptr(ukm1), rho, C.byref(v)),
                "mpf_mpnn_loss")
+        return v.value
+
+    def residual_loss(self, P: Problem, lam_next: float, dlam: float, unext, uk, ukm1, rho: float) -> float:
+        """Linearised relative start residual (squared, mean over meshes) of the last prediction."""
+        v = C.c_double()
+        _check(lib().mpf_mpnn_residual_loss(self._h, P._h, lam_next, dlam, _dptr(unext), _dptr(uk), _dptr(ukm1),
+                                            rho, C.byref(v)), "mpf_mpnn_residual_loss")
         return v.value
 
     def backward(self, P: Problem, dy=None):

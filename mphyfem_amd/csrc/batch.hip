@@ -61,7 +61,8 @@ struct BatchWork {
 };
 
 // per-mesh double slots
-enum { kRZ = 0, kPQ, kRR, kRR0, kAL, kBE, kTOL2, kR0, kRN, kREL, kMS };
+//   kETA forcing term of the last solve, kRNP residual norm before the last solve
+enum { kRZ = 0, kPQ, kRR, kRR0, kAL, kBE, kTOL2, kR0, kRN, kREL, kETA, kRNP, kMS };
 // per-mesh int slots
 //   kCGA CG running, kNA Newton running, kCGIT CG iterations of the current solve, kNIT Newton
 //   iterations (solves) of the current load step, kDIV diverged, kCGT CG iterations of the load step
@@ -246,8 +247,22 @@ __global__ void __launch_bounds__(kB) k_b_newton(int32_t nmesh, const int32_t* _
       else if (R0 == 0.0 || rel < rtol) F[kNA] = 0;
       else F[kNIT] += 1;
     }
-    // inexact Newton: CG to eta * ||R||, eta = min(0.1, 0.1 rel) bounded below
-    const double eta = fmax(eta_min, fmin(0.1, 0.1 * rel));
+    // Inexact Newton: CG to eta ||R||.  First solve of the step: eta = 0.1 min(1, rel); later solves
+    // follow the observed nonlinear reduction (Eisenstat-Walker, choice 2: 0.9 (|R_k|/|R_k-1|)^2,
+    // kept >= 0.9 eta_prev^2 while that exceeds 0.1); never tighter than the step's target needs
+    // (0.5 rtol / rel) nor than eta_min, never looser than 0.5.
+    double eta;
+    if (F[kNIT] <= 1) {
+      eta = 0.1 * fmin(1.0, rel);
+    } else {
+      const double ratio = S[kRNP] > 0.0 ? rn / S[kRNP] : 1.0;
+      eta = 0.9 * ratio * ratio;
+      const double sg = 0.9 * S[kETA] * S[kETA];
+      if (sg > 0.1) eta = fmax(eta, sg);
+    }
+    if (rel > 0.0) eta = fmax(eta, 0.5 * rtol / rel);
+    eta = fmin(0.5, fmax(eta_min, eta));
+    if (F[kNA]) { S[kETA] = eta; S[kRNP] = rn; }
     S[kTOL2] = eta * eta;
     act += F[kNA] != 0;
     div += F[kDIV] != 0;
@@ -654,6 +669,17 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
     MPF_HIP(hipMemcpy(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (bad) { result = MPF_ERR_NEG_JACOBIAN; break; }
     if (B.cnt_h[2] > 0) { result = MPF_ERR_DIVERGED; break; }
+    if (iter == 1) {  // residual at the start point, relative to the plain-start R0
+      std::vector<double> msh((size_t)B.nmesh * kMS);
+      MPF_HIP(hipMemcpy(msh.data(), B.ms.ptr, msh.size() * sizeof(double), hipMemcpyDeviceToHost));
+      double sum = 0.0, mx = 0.0;
+      for (int m = 0; m < B.nmesh; ++m) {
+        sum += msh[(size_t)m * kMS + kREL];
+        mx = std::max(mx, msh[(size_t)m * kMS + kREL]);
+      }
+      rep->start_relres_mean = sum / B.nmesh;
+      rep->start_relres_max = mx;
+    }
     if (B.cnt_h[1] == 0) { step_conv = true; break; }
     if (iter > o->itmax) break;
     ++rep->newton_iters;

@@ -25,6 +25,13 @@
 
 #include "mpf_internal.hpp"
 
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK);  // problem.hip
+
+namespace mpf {
+mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals, int64_t n, int64_t nnz,
+                const double* x, double* y, double* part, unsigned np, const double* sc, hipStream_t s);  // krylov.hip
+}
+
 typedef short bf16x8_s __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -64,6 +71,10 @@ struct mpf_mpnn {
   mpf::DevBuf<int32_t> node_off, nmesh_of, fixed;
   mpf::DevBuf<double> xref;
   mpf::DevBuf<float> scale;
+  std::vector<int32_t> doff;          // per mesh: first free dof (free dofs of a mesh are contiguous)
+  mpf::DevBuf<int32_t> doff_d;
+  mpf::DevBuf<double> wres;           // per mesh: 1 / ||f_m||^2 over free dofs (unit external load)
+  mpf::DevBuf<double> ebuf, rbuf, gbuf, msq;  // residual-loss work vectors
 };
 
 namespace mpf {
@@ -262,6 +273,11 @@ __global__ void k_relu_bwd(int64_t n, int C, const float* __restrict__ src, int 
   dstb[t] = f2bf(v);
 }
 
+__global__ void k_fill0(int64_t n, double* __restrict__ a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0.0;
+}
+
 __global__ void k_to_bf16(int64_t n, const float* __restrict__ a, uint16_t* __restrict__ b) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = f2bf(a[t]);
@@ -403,6 +419,56 @@ __global__ void k_guess(int64_t n, const int32_t* __restrict__ nmesh_of, const f
   for (int d = 0; d < 3; ++d) {
     const double u = uk[3 * i + d];
     guess[3 * i + d] = u + rho * (u - ukm1[3 * i + d]) + (use_net ? s * (double)y[i * kFout + d] : 0.0);
+  }
+}
+
+// e[dof] = s_m y - (u_next - u_k - rho (u_k - u_{k-1})) on the free dofs (the prediction's error)
+__global__ void k_err_free(int64_t n, const int32_t* __restrict__ nmesh_of, const float* __restrict__ scale,
+                           const int32_t* __restrict__ nda, const float* __restrict__ y,
+                           const double* __restrict__ unext, const double* __restrict__ uk,
+                           const double* __restrict__ ukm1, double rho, double* __restrict__ e) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = scale[nmesh_of[i]];
+  for (int d = 0; d < 3; ++d) {
+    const int dof = nda[3 * i + d];
+    if (dof < 0) continue;
+    const double u = uk[3 * i + d];
+    e[dof] = s * (double)y[i * kFout + d] - (unext[3 * i + d] - u - rho * (u - ukm1[3 * i + d]));
+  }
+}
+
+// per mesh: sum of r^2 over its free-dof range
+__global__ void __launch_bounds__(256) k_mesh_sq(const int32_t* __restrict__ doff, const double* __restrict__ r,
+                                                 double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int m = blockIdx.x;
+  double v = 0.0;
+  for (int64_t i = doff[m] + threadIdx.x; i < doff[m + 1]; i += 256) v += r[i] * r[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = sh[0];
+}
+
+// dy[i][d] = c w_m s_m g[dof] on free dofs, 0 elsewhere
+__global__ void k_dy_from_g(int64_t n, const int32_t* __restrict__ nmesh_of, const float* __restrict__ scale,
+                            const double* __restrict__ w, const int32_t* __restrict__ nda,
+                            const double* __restrict__ g, double c, float* __restrict__ dy) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int m = nmesh_of[i];
+  const double f = c * w[m] * (double)scale[m];
+  for (int d = 0; d < kFout; ++d) {
+    float v = 0.f;
+    if (d < 3) {
+      const int dof = nda[3 * i + d];
+      if (dof >= 0) v = (float)(f * g[dof]);
+    }
+    dy[i * kFout + d] = v;
   }
 }
 
@@ -555,6 +621,40 @@ mpf_status ensure_ctx(mpf_mpnn* net, mpf_problem* p) {
     for (int d = 0; d < 3; ++d) { xr[4 * m + d] = off[m + 1] > off[m] ? lo[d] : 0.0; L = std::max(L, hi[d] - lo[d]); }
     xr[4 * m + 3] = L > 0.0 ? L : 1.0;
   }
+  // free-dof ranges per mesh and the load weights 1 / ||f_m||^2
+  std::vector<int32_t> nda((size_t)nNode * 3);
+  MPF_TRY(p->pat.nda.download(nda.data(), nda.size(), p->stream));
+  std::vector<double> fu;
+  if (p->has_fext) {
+    fu.resize((size_t)nNode * 3);
+    MPF_TRY(p->fext_unit.download(fu.data(), fu.size(), p->stream));
+  }
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  std::vector<int32_t> doff(nm + 1, 0);
+  std::vector<double> wr(nm, 0.0);
+  int32_t cnt = 0;
+  for (int m = 0; m < nm; ++m) {
+    doff[m] = cnt;
+    double f2 = 0.0;
+    for (int32_t v = off[m]; v < off[m + 1]; ++v)
+      for (int d = 0; d < 3; ++d) {
+        const int32_t dof = nda[3 * (size_t)v + d];
+        if (dof < 0) continue;
+        if (dof != cnt) return MPF_ERR_STATE;  // free dofs numbered node-major
+        ++cnt;
+        if (!fu.empty()) f2 += fu[3 * (size_t)v + d] * fu[3 * (size_t)v + d];
+      }
+    wr[m] = f2 > 0.0 ? 1.0 / f2 : 0.0;
+  }
+  doff[nm] = cnt;
+  net->doff = doff;
+  MPF_TRY(net->doff_d.upload(doff.data(), doff.size(), net->stream));
+  MPF_TRY(net->wres.upload(wr.data(), wr.size(), net->stream));
+  const size_t nsys = (size_t)p->pat.dispDOF + p->pat.presDOF;
+  MPF_TRY(net->ebuf.alloc(nsys > 0 ? nsys : 1));
+  MPF_TRY(net->rbuf.alloc(nsys > 0 ? nsys : 1));
+  MPF_TRY(net->gbuf.alloc(nsys > 0 ? nsys : 1));
+  MPF_TRY(net->msq.alloc(nm));
   MPF_TRY(net->node_off.upload(off.data(), off.size(), net->stream));
   MPF_TRY(net->nmesh_of.upload(of.data(), of.size(), net->stream));
   MPF_TRY(net->fixed.upload(fx.data(), fx.size(), net->stream));
@@ -724,6 +824,42 @@ mpf_status mpf_mpnn_loss(mpf_mpnn* net, mpf_problem* p, const double* unext, con
   double sum = 0.0;
   for (double v : h) sum += v;
   if (loss) *loss = sum * inv;
+  return MPF_OK;
+}
+
+// Linearised start residual of the prediction (what Newton's convergence depends on):
+//   loss = mean over meshes of ||K e_m||^2 / (dlam^2 ||f_m||^2),
+// e = predicted - converged u_{k+1} on the free dofs, K = tangent at the converged u_{k+1} (assembled
+// here: the problem's state, system and solution buffers are overwritten).  dL/dy (K symmetric:
+// 2 w_m s_m K K e / (dlam^2 nmesh)) is left in the network for mpf_mpnn_backward.
+mpf_status mpf_mpnn_residual_loss(mpf_mpnn* net, mpf_problem* p, double lam_next, double dlam, const double* unext,
+                                  const double* uk, const double* ukm1, double rho, double* loss) {
+  if (!net || !p || !unext || !uk || !ukm1 || dlam <= 0.0 || net->ctx != p->uid) return MPF_ERR_ARG;
+  const int64_t n = p->nNode;
+  const int64_t nsys = (int64_t)p->pat.dispDOF + p->pat.presDOF;
+  MPF_HIP(hipStreamSynchronize(net->stream));
+  MPF_HIP(hipMemcpyAsync(p->disp.ptr, unext, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+  MPF_TRY(enqueue_assembly(p, false, lam_next, 1));
+  hipLaunchKernelGGL(k_fill0, dim3(g256(nsys)), dim3(256), 0, p->stream, nsys, net->ebuf.ptr);
+  hipLaunchKernelGGL(k_err_free, dim3(g256(n)), dim3(256), 0, p->stream, n, net->nmesh_of.ptr, net->scale.ptr,
+                     p->pat.nda.ptr, net->y.ptr, unext, uk, ukm1, rho, net->ebuf.ptr);
+  MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, nsys, p->pat.nnz, net->ebuf.ptr, net->rbuf.ptr,
+               nullptr, 0, nullptr, p->stream));
+  hipLaunchKernelGGL(k_mesh_sq, dim3(net->nmesh), dim3(256), 0, p->stream, net->doff_d.ptr, net->rbuf.ptr,
+                     net->msq.ptr);
+  MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, nsys, p->pat.nnz, net->rbuf.ptr, net->gbuf.ptr,
+               nullptr, 0, nullptr, p->stream));
+  const double c = 2.0 / (dlam * dlam * net->nmesh);
+  hipLaunchKernelGGL(k_dy_from_g, dim3(g256(n)), dim3(256), 0, p->stream, n, net->nmesh_of.ptr, net->scale.ptr,
+                     net->wres.ptr, p->pat.nda.ptr, net->gbuf.ptr, c, net->dy.ptr);
+  MPF_HIP(hipGetLastError());
+  std::vector<double> sq(net->nmesh), w(net->nmesh);
+  MPF_HIP(hipMemcpyAsync(sq.data(), net->msq.ptr, sq.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  MPF_HIP(hipMemcpyAsync(w.data(), net->wres.ptr, w.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  MPF_HIP(hipStreamSynchronize(p->stream));
+  double sum = 0.0;
+  for (int m = 0; m < net->nmesh; ++m) sum += w[m] * sq[m];
+  if (loss) *loss = sum / (dlam * dlam * net->nmesh);
   return MPF_OK;
 }
 

@@ -74,6 +74,7 @@ def cantilever_problem(beams: Beams, idx) -> tuple[HexBatch, Problem]:
 @dataclass
 class StepLog:
     iters: list = field(default_factory=list)      # per step: assemblies per mesh (np.int32 array)
+    start_rel: list = field(default_factory=list)  # per step: mean ||R|| / ||R0|| at the start point
     seconds: list = field(default_factory=list)    # per step: wall time of the batched step
     newton: list = field(default_factory=list)     # per step: batch Newton iterations (slowest mesh)
 
@@ -108,6 +109,7 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
         log.seconds.append(time.perf_counter() - t0)
         log.iters.append(iters.copy())
         log.newton.append(rep.newton_iters)
+        log.start_rel.append(rep.start_relres_mean)
         hist[0].copy_(hist[1])
         P.copy_state_device(hist[1])
         if store is not None:
@@ -115,23 +117,32 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
     return log
 
 
-def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, log_every: int = 10):
+def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, residual_epochs: int = 0,
+          log_every: int = 10):
     """Full-batch steps over the stored trajectories u_0..u_N: sample k predicts u_{k+1} from u_k,
-    u_{k-1} (k = 1 .. N-1).  Returns the mean loss per epoch."""
+    u_{k-1} (k = 1 .. N-1).  The first `epochs` minimise the displacement error; the following
+    `residual_epochs` the linearised start residual ||K e|| / ||dlam f|| (what Newton's convergence
+    depends on; one tangent assembly per sample).  Returns the mean loss per epoch."""
     nsteps = traj.shape[0] - 1
     losses = []
-    for ep in range(epochs):
+    total = epochs + residual_epochs
+    for ep in range(total):
         tot = 0.0
+        res = ep >= epochs
         for k in range(1, nsteps):
             lam = (k + 1) / nsteps
             net.features(P, traj[k], traj[k - 1], 1.0, lam, k / (k + 1.0))
             net.forward(P)
-            tot += net.loss(P, traj[k + 1], traj[k], traj[k - 1], 1.0)
+            if res:
+                tot += net.residual_loss(P, lam, 1.0 / nsteps, traj[k + 1], traj[k], traj[k - 1], 1.0)
+            else:
+                tot += net.loss(P, traj[k + 1], traj[k], traj[k - 1], 1.0)
             net.backward(P)
-            net.adam(lr=lr * (0.5 ** (ep / max(epochs / 3, 1))))
+            net.adam(lr=lr * (0.5 ** ((ep - (epochs if res else 0)) / max((residual_epochs if res else epochs) / 3, 1))))
         losses.append(tot / (nsteps - 1))
-        if log_every and (ep % log_every == 0 or ep == epochs - 1):
-            print(f"[learned] epoch {ep}: loss {losses[-1]:.4e}", flush=True)
+        if log_every and (ep % log_every == 0 or ep == total - 1 or ep == epochs):
+            kind = "start residual^2" if res else "displacement"
+            print(f"[learned] epoch {ep}: {kind} loss {losses[-1]:.4e}", flush=True)
     return losses
 
 
@@ -142,11 +153,13 @@ def summarize(log: StepLog, first: int = 1) -> dict:
     return {"assemblies_per_mesh_step": round(float(it.mean()), 3),
             "batch_newton_iters": int(np.sum(log.newton[first:])),
             "seconds": round(float(np.sum(log.seconds[first:])), 4),
-            "seconds_all_steps": round(float(np.sum(log.seconds)), 4)}
+            "seconds_all_steps": round(float(np.sum(log.seconds)), 4),
+            "start_relres_geomean": float(np.exp(np.mean(np.log(np.maximum(log.start_rel[first:], 1e-300)))))}
 
 
 def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs: int = 60, lr: float = 2e-3,
-               nsteps: int = NSTEPS, seed: int = 4, autotune: bool = True, **solver) -> dict:
+               nsteps: int = NSTEPS, seed: int = 4, autotune: bool = True, residual_epochs: int = 0,
+               **solver) -> dict:
     """Trajectories of the plain solver on the training beams, training, then plain / extrapolated /
     learned Newton on the held-out beams.  Returns the measurements."""
     import torch
@@ -165,9 +178,12 @@ def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs:
     out["train_plain"] = summarize(log_tr)
     net = Mpnn(seed=seed)
     t1 = time.perf_counter()
-    losses = train(net, Ptr, traj, epochs=epochs, lr=lr)
+    losses = train(net, Ptr, traj, epochs=epochs, lr=lr, residual_epochs=residual_epochs)
     out["train_seconds"] = round(time.perf_counter() - t1, 2)
-    out["loss_first"], out["loss_last"] = losses[0], losses[-1]
+    out["loss_first"], out["loss_last"] = losses[0], losses[epochs - 1]
+    out["residual_epochs"] = residual_epochs
+    if residual_epochs:
+        out["start_residual_loss_first"], out["start_residual_loss_last"] = losses[epochs], losses[-1]
     del traj, Ptr
     _, Pte = cantilever_problem(beams, test_idx)
     if autotune:
