@@ -92,12 +92,14 @@ def material_host(M, workload):
     return 1.0, 1e-3, 3
 
 
-def read_traffic(name: str):
-    f = ROOT / "profiles" / name
+def read_traffic(workload: str):
+    """HBM bytes per assembly of the gather kernels, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE and
+    scaled by a copy calibration (profiles/traffic.json, written from the runs its .md files list)."""
+    f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get("bytes_per_launch")
+        return json.loads(f.read_text()).get(workload, {}).get("bytes_per_launch")
     except Exception:
         return None
 
@@ -240,11 +242,11 @@ def main(argv=None) -> int:
     bpe = BYTES_PER_ELEM[wl]
     achieved = bpe * nElem / (ms_kern * 1e-3) / 1e9
     if a.assembly == "gather":
-        kname = "k_stage + k_rows (row-owner gather, 2 launches"
+        kname = "k_stage + k_flat + k_rhs (row-owner gather, 3 launches"
     else:
         kname = ("tet10_kernel<4,false>" if wl == "tet10" else "hex8_kernel<8,false>") + f" ({ncolors} colour launches"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(f"{wl}_scatter_traffic.json"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(wl) if a.assembly == "gather" else None,
                 "kernel": f"{kname} per assembly)", "assembly": a.assembly,
                 "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe}
     if wl == "tet10":

@@ -254,6 +254,54 @@ __global__ void k_block_setup(int64_t nblk, const int32_t* __restrict__ bs, cons
   Bi[8] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
 }
 
+// Pressure rows of the block-Jacobi preconditioner: instead of the bare Kpp diagonal (tiny for a
+// nearly incompressible material), the diagonal of the Schur complement
+//   S_qq = Kpp_qq - sum over u-node blocks n of  Kpu(q, n) A_n^-1 Kup(n, q),
+// A_n the node's 3x3 Kuu block (already inverted in binv), Kup(n, q) taken as Kpu(q, n)^T (the
+// mixed element's coupling blocks are transposes of each other).  With it, the preconditioner is
+// blockdiag(A, S), whose preconditioned saddle-point operator has clustered eigenvalues.
+__global__ void k_schur_pressure(int32_t presDOF, int32_t dispDOF, const int32_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                 const int32_t* __restrict__ bs, const int32_t* __restrict__ bm, int64_t nub,
+                                 double* __restrict__ binv) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= presDOF) return;
+  const int r = dispDOF + q;
+  double S = 0.0;
+  int k = rowptr[r];
+  const int kend = rowptr[r + 1];
+  while (k < kend) {
+    const int c = colidx[k];
+    if (c >= dispDOF) {
+      if (c == r) S += vals[k];
+      ++k;
+      continue;
+    }
+    // node block holding column c (blocks sorted by first row)
+    int64_t lo = 0, hi = nub - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (bs[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const int b0 = bs[lo], m = bm[lo];
+    double b[3] = {0.0, 0.0, 0.0};
+    int kk = k;
+    while (kk < kend && colidx[kk] < b0 + m) {
+      b[colidx[kk] - b0] = vals[kk];
+      ++kk;
+    }
+    const double* Ai = binv + 9 * lo;
+    double t = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) t += b[i] * Ai[3 * i + j] * b[j];
+    S -= t;
+    k = kk;
+  }
+  double* Bi = binv + 9 * (nub + q);
+  for (int i = 0; i < 9; ++i) Bi[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  Bi[0] = S != 0.0 ? 1.0 / S : 1.0;
+}
+
 // Block list: node blocks of free u dofs (consecutive rows), then 1x1 pressure rows.
 __global__ void k_node_blocks(int32_t nNode, const int32_t* __restrict__ nda, const int32_t* __restrict__ first_blk,
                               int32_t* __restrict__ bs, int32_t* __restrict__ bm) {
@@ -430,6 +478,12 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
   hipLaunchKernelGGL(k_block_setup, dim3(grid_for(W.nblk, 256)), dim3(256), 0, s, W.nblk, W.blk_start.ptr,
                      W.blk_size.ptr, p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, W.blk_inv.ptr,
                      pc == MPF_PC_NONE ? 1 : 0);
+  if (kind == 2 && p->pat.presDOF > 0) {
+    const int64_t nub = W.nblk - p->pat.presDOF;
+    hipLaunchKernelGGL(k_schur_pressure, dim3(grid_for(p->pat.presDOF, 256)), dim3(256), 0, s, p->pat.presDOF,
+                       p->pat.dispDOF, p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, W.blk_start.ptr,
+                       W.blk_size.ptr, nub, W.blk_inv.ptr);
+  }
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
