@@ -275,11 +275,13 @@ __global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* 
 }
 
 // Build step 3 (one wavefront per node n): where each column block (n, j) of n's rows lives in the CSR.
-//   Hex8:      {A, (La << 8) | (mn << 3) | mb}
-//   Tet10/P1:  {A, Bp, (D << 16) | La, (isPn << 7) | (isPb << 6) | (mn << 3) | mb}
+//   Hex8:      {A, (La << 9) | flags}
+//   Tet10/P1:  {A, Bp, (D << 16) | La, flags}
+//   flags = (self << 8) | (isPn << 7) | (isPb << 6) | (mn << 3) | mb
 // A = start of n's first free u row + column offset of j's u dofs (-1 if n has no free u dof),
 // Bp = the same in n's pressure row (-1 if n carries no pressure), D = offset of j's pressure column
-// relative to A (same in every row of n), La = length of n's rows, mn / mb = free masks of n / j.
+// relative to A (same in every row of n), La = length of n's rows, mn / mb = free masks of n / j,
+// self: j is n itself (its column run also sums n's right-hand side).
 template <bool MIXED>
 __global__ void __launch_bounds__(64) k_block_table(int32_t nNode, const int32_t* __restrict__ nbptr,
                                                     const int32_t* __restrict__ nb,
@@ -322,15 +324,16 @@ __global__ void __launch_bounds__(64) k_block_table(int32_t nNode, const int32_t
       const int upos = cu0 + iu - nu, ppos = cp0 + ip - np;
       const int mb = inf & 7;
       const size_t blk = (size_t)b0 + j;
+      const int self = nb[b0 + j] == n;
       if (MIXED) {
         const int isPb = (inf >> 3) & 1;
         btab[4 * blk] = RU >= 0 ? RU + upos : -1;
         btab[4 * blk + 1] = RP >= 0 ? RP + upos : -1;
         btab[4 * blk + 2] = ((Ucols + ppos - upos) << 16) | La;
-        btab[4 * blk + 3] = ((int)isPn << 7) | (isPb << 6) | (mn << 3) | mb;
+        btab[4 * blk + 3] = (self << 8) | ((int)isPn << 7) | (isPb << 6) | (mn << 3) | mb;
       } else {
         btab[2 * blk] = RU >= 0 ? RU + upos : -1;
-        btab[2 * blk + 1] = (La << 8) | (mn << 3) | mb;
+        btab[2 * blk + 1] = (La << 9) | (self << 8) | (mn << 3) | mb;
       }
     }
     cu0 += __shfl(iu, 63, 64);
@@ -401,8 +404,12 @@ struct FlatArgs {
   const int32_t* nelA;
   const int32_t* nbptr;
   const int32_t* btab;
+  const int32_t* nda;
+  const int32_t* idp;
+  int32_t dispDOF;
   const double* GS;
   double* vals;
+  double* rhs;
 };
 
 // Owner-side staging capacity of the matrix kernel: elements around a node whose node-n data are
@@ -427,13 +434,15 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   constexpr int BT = MIXED ? 4 : 2;
   constexpr int MAXS = FlatCap<NPE>::MAXS;
   constexpr int SSP = (St::SS + 1) & ~1;                 // scalars per Gauss point, padded to pairs
+  constexpr int NR = MIXED ? 4 : 3;                       // right-hand-side rows of a node (self items)
+  constexpr int NT = NV + NR;
   constexpr int STG = MAXS * NGP * (6 + SSP);             // staged doubles
-  constexpr int LDSD = STG > 64 * NV ? STG : 64 * NV;     // the staging area is reused for the run sums
+  constexpr int LDSD = STG > 64 * NT ? STG : 64 * NT;     // the staging area is reused for the run sums
   __shared__ int32_t sB[64];
   __shared__ double2 sMem[LDSD / 2];
   double* const sGa = reinterpret_cast<double*>(sMem);   // [MAXS][NGP][6]
   double* const sSc = sGa + MAXS * NGP * 6;               // [MAXS][NGP][SSP]
-  double* const sPart = reinterpret_cast<double*>(sMem);  // [64][NV], after the items are computed
+  double* const sPart = reinterpret_cast<double*>(sMem);  // [64][NT], after the items are computed
   const int lane = threadIdx.x;
   const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
   if (w >= F.nwaves) return;
@@ -460,7 +469,9 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   const bool live = it.x >= 0;
   const int j = live ? (it.y >> 10) : -1;
   __syncthreads();
-  double v[NV];
+  double v[NT];
+#pragma unroll
+  for (int q = NV; q < NT; ++q) v[q] = 0.0;
   if (live) {
     const int e = it.x >> 4, a = it.x & 15, s = (it.y >> 4) & 63, b = it.y & 15;
     const double* rec = F.GS + (size_t)e * St::TS;
@@ -469,6 +480,21 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
     else
       item_block<NPE, NGP, MIXED, NV, NPE * 6, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a,
                                                        b, v);
+    if (b == a) {  // the element's own item of node n carries its residual rows
+#pragma unroll
+      for (int i = 0; i < 3; ++i) v[NV + i] = rec[St::R0 + a * 3 + i];
+      if (MIXED && a < 4) {
+        double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
+#pragma unroll
+        for (int gp = 0; gp < NGP; ++gp) {
+          double Np[4];
+          tet_Np(gp, Np);
+          const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
+          fp -= (rec[St::S0 + gp * St::SS + 5] * Na) * rec[St::S0 + gp * St::SS + 7];
+        }
+        v[NT - 1] = fp;
+      }
+    }
   } else {
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = 0.0;
@@ -476,23 +502,29 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   __syncthreads();  // staging area becomes the run-sum area
   sB[lane] = j;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) sPart[lane * NV + q] = v[q];
+  for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
   __syncthreads();
   if (!live || (lane > 0 && sB[lane - 1] == j)) return;
-  double acc[NV];
+  double acc[NT];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = v[q];
+  for (int q = 0; q < NT; ++q) acc[q] = v[q];
   for (int t2 = lane + 1; t2 < 64 && sB[t2] == j; ++t2)
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] += sPart[t2 * NV + q];
+    for (int q = 0; q < NT; ++q) acc[q] += sPart[t2 * NT + q];
   const size_t blk = (size_t)F.nbptr[n] + j;
   int bt[BT];
 #pragma unroll
   for (int q = 0; q < BT; ++q) bt[q] = F.btab[BT * blk + q];
   const int A = bt[0];
-  const int masks = MIXED ? bt[3] : bt[1];
-  const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 8);
+  const int masks = MIXED ? bt[3] : (bt[1] & 0x1ff);
+  const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 9);
   const int mn = (masks >> 3) & 7, mb = masks & 7;
+  if ((masks >> 8) & 1) {  // self column: n's right-hand side, its elements' residuals in element order
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if ((mn >> i) & 1) F.rhs[F.nda[3 * (size_t)n + i]] = acc[NV + i];
+    if (MIXED && ((masks >> 7) & 1)) F.rhs[F.dispDOF + F.idp[n]] = acc[NT - 1];
+  }
   double* __restrict__ V = F.vals;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -525,11 +557,14 @@ struct RhsArgs {
   const double* GS;
   const double* dispApplied;  // non-null: first-iteration lift
   double* rhs;
+  int32_t lift_only;          // the residual sums are already in rhs (written by the matrix kernel)
 };
 
 // Right-hand-side kernel (one lane per node): n's rows of the element residuals, summed in element
 // order, minus the first-iteration lift  sum_e K_e(n, b) dubar_b  over the prescribed dofs of e's nodes
-// (applyDirichletBCs / applyDirichletBCs2field, ElementBase.cpp:567-594, 717-782).
+// (applyDirichletBCs / applyDirichletBCs2field, ElementBase.cpp:567-594, 717-782).  With lift_only
+// the matrix kernel has written the residual sums and only nodes next to prescribed dofs subtract
+// their lift.
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
   using St = Stage<NPE, NGP, MIXED>;
@@ -540,8 +575,16 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
   const bool isPn = MIXED && ((info >> 3) & 1);
   if (!mn && !isPn) return;
   const int e0 = R.nptr[n], m = R.nptr[n + 1] - e0;
+  // lift only: nodes next to prescribed dofs, and nodes in no element (no items wrote their rows)
+  if (R.lift_only && m > 0 && !(R.dispApplied && R.near[n])) return;
   double r[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int s = 0; s < m; ++s) {
+  if (R.lift_only && m > 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if ((mn >> i) & 1) r[i] = R.rhs[R.nda[3 * (size_t)n + i]];
+    if (isPn) r[3] = R.rhs[R.dispDOF + R.idp[n]];
+  }
+  for (int s = 0; s < m && !R.lift_only; ++s) {
     const int ea = R.nelA[e0 + s];
     const int a = ea & 15;
     const double* rec = R.GS + (size_t)(ea >> 4) * St::TS;
@@ -753,21 +796,28 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   F.nelA = P.g_nelA.ptr;
   F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
+  F.nda = P.nda.ptr;
+  F.idp = P.idp.ptr;
+  F.dispDOF = P.dispDOF;
   F.GS = p->GS.ptr;
   F.vals = p->vals.ptr;
+  F.rhs = p->rhs.ptr;
   RhsArgs R{};
   R.nNode = nNode;
   R.nptr = P.nptr.ptr; R.nelA = P.g_nelA.ptr; R.near = P.g_near.ptr; R.conn = p->conn.ptr;
   R.nodeinfo = P.nodeinfo.ptr; R.nda = P.nda.ptr; R.idp = P.idp.ptr; R.dispDOF = P.dispDOF;
   R.GS = p->GS.ptr; R.dispApplied = dispApplied; R.rhs = p->rhs.ptr;
+  // with the matrix the residual sums come from the matrix kernel; k_rhs then only lifts
+  const bool flat = writeK && F.nwaves > 0;
+  R.lift_only = flat ? 1 : 0;
   const unsigned flat_grid = (unsigned)(((F.nwaves + 7) / 8) * 8);
   if (!mixed) {
     hipLaunchKernelGGL((k_stage<8, 8, false, 8>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A, nElem, p->GS.ptr);
-    if (writeK && F.nwaves > 0) hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
+    if (flat) hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<8, 8, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
   } else {
     hipLaunchKernelGGL((k_stage<10, 4, true, 16>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A, nElem, p->GS.ptr);
-    if (writeK && F.nwaves > 0) hipLaunchKernelGGL((k_flat<10, 4, true>), dim3(flat_grid), dim3(64), 0, s, F);
+    if (flat) hipLaunchKernelGGL((k_flat<10, 4, true>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<10, 4, true>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
   }
   MPF_HIP(hipGetLastError());
