@@ -163,9 +163,10 @@ __device__ __forceinline__ int flat_walk(const int32_t* sK, int nitems, Emit emi
 
 // Build step 1 (one wavefront per node n):
 //   nelA[nptr[n] + s]       = e * 16 + a: the elements around n in ascending id, a = local index of n in e
-//   keys[nptr[n] * NPE + t] = (j << 16) | (s << 4) | b for every element s around n and node b of it,
-//                             sorted by (j, s); j = position of node conn[e][b] in the sorted coupled-node
-//                             list of n, i.e. the column block of n's rows that the (s, b) block lands in
+//   keys[nptr[n] * NPE + t] = (s0 << 23) | (j << 16) | (s << 4) | b for every element s around n and
+//                             node b of it, sorted: j = position of node conn[e][b] in the sorted
+//                             coupled-node list of n (the column block of n's rows the (s, b) block
+//                             lands in), s0 = first element of column j's run
 //   flen[n]                 = laid-out length of n's items (0 for a node without rows)
 template <int NPE>
 __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t* __restrict__ nptr,
@@ -218,6 +219,17 @@ __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t*
   }
   __syncthreads();
   wave_sort(sK, P2, lane);
+  // Runs (columns) are laid out in order of their first element, so a wavefront's items come from
+  // few elements; within a run, items stay in element order.  key |= (first element of the run) << 23.
+  if (lane == 0) {
+    int first = 0;
+    for (int t = 0; t < nitems; ++t) {
+      if (t == 0 || ((sK[t] >> 16) & 127) != ((sK[t - 1] >> 16) & 127)) first = (sK[t] >> 4) & 63;
+      sK[t] |= first << 23;
+    }
+  }
+  __syncthreads();
+  wave_sort(sK, P2, lane);
   for (int t = lane; t < nitems; t += 64) keys[(size_t)e0 * NPE + t] = sK[t];
   if (lane == 0) {
     const int len = flat_walk(sK, nitems, [](int, int) {});
@@ -267,7 +279,7 @@ __global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* 
     if (t >= 0) {
       const int key = sK[t];
       it.x = nelA[e0 + ((key >> 4) & 63)];
-      it.y = ((key >> 16) << 10) | (key & 0x3ff);
+      it.y = (((key >> 16) & 127) << 10) | (key & 0x3ff);
     }
     out[q] = it;
   }
@@ -412,71 +424,91 @@ struct FlatArgs {
   double* rhs;
 };
 
-// Owner-side staging capacity of the matrix kernel: elements around a node whose node-n data are
-// copied to LDS once per wavefront; a node with more elements reads them from the staging records.
-// Hex8 (8 elements around an interior node, one wavefront per node) gains from it.  Tet10/P1 does not
-// stage: a vertex node's items span four wavefronts that would each copy all ~24 elements' data
-// while using a quarter of it (measured 9.9 ms unstaged vs 12.5 ms staged per assembly).
+// Owner-side staging of the matrix kernel: the node's own Gauss-point data (g, b g of node n and the
+// scalars) in each distinct element of a wavefront's items is copied to LDS once per wavefront;
+// each item then gathers only its other node's data from the element's staging record.  Runs are
+// laid out in order of their first element, so a wavefront's items come from few elements.  A
+// wavefront with more than DMAX distinct elements reads everything from the records.  Measured per
+// assembly: Hex8 11.2 ms staged vs 15.3 ms not; Tet10/P1 9.37 ms not staged vs 9.41 (DMAX 12) and
+// 9.54 ms (DMAX 24), so Tet10/P1 does not stage.
+#ifndef MPF_FLAT_DMAX_HEX
+#define MPF_FLAT_DMAX_HEX 8
+#endif
+#ifndef MPF_FLAT_DMAX_TET
+#define MPF_FLAT_DMAX_TET 0
+#endif
 template <int NPE>
 struct FlatCap {
-  static constexpr int MAXS = NPE == 8 ? 8 : 0;
+  static constexpr int DMAX = NPE == 8 ? MPF_FLAT_DMAX_HEX : MPF_FLAT_DMAX_TET;
 };
 
-// Matrix kernel: one wavefront per 64 laid-out items of one node n.  The wavefront first copies
-// node n's share of the staging records of the elements around n (g, b g of n and the Gauss-point
-// scalars) to LDS; item lanes then compute their blocks, and the first lane of each column run (its
-// head) sums the run in element order and writes the column block of n's rows, so every CSR value
-// is written exactly once.
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
+    hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Matrix kernel: one wavefront per 64 laid-out items of one node n.  Item lanes compute their blocks,
+// and the first lane of each column run (its head) sums the run in element order and writes the
+// column block of n's rows, so every CSR value is written exactly once.  The self column's run also
+// sums n's residual rows.
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
   constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
   constexpr int BT = MIXED ? 4 : 2;
-  constexpr int MAXS = FlatCap<NPE>::MAXS;
-  constexpr int SSP = (St::SS + 1) & ~1;                 // scalars per Gauss point, padded to pairs
-  constexpr int NR = MIXED ? 4 : 3;                       // right-hand-side rows of a node (self items)
+  constexpr int NR = MIXED ? 4 : 3;   // right-hand-side rows of a node (self items)
   constexpr int NT = NV + NR;
-  constexpr int STG = MAXS * NGP * (6 + SSP);             // staged doubles
-  constexpr int LDSD = STG > 64 * NT ? STG : 64 * NT;     // the staging area is reused for the run sums
+  constexpr int DMAX = FlatCap<NPE>::DMAX;
+  constexpr int GA = NGP * 6, SC = NGP * St::SS;  // owner data per element (doubles, both even)
+  static_assert(GA % 2 == 0 && SC % 2 == 0, "16-byte staging");
+  constexpr int STG = DMAX * (GA + SC);
+  constexpr int LDSD = STG > 64 * NT ? STG : 64 * NT;  // the staging area is reused for the run sums
   __shared__ int32_t sB[64];
+  __shared__ int32_t sE[DMAX > 0 ? DMAX : 1];
   __shared__ double2 sMem[LDSD / 2];
-  double* const sGa = reinterpret_cast<double*>(sMem);   // [MAXS][NGP][6]
-  double* const sSc = sGa + MAXS * NGP * 6;               // [MAXS][NGP][SSP]
+  double* const sGa = reinterpret_cast<double*>(sMem);    // [DMAX][NGP][6]
+  double* const sSc = sGa + DMAX * GA;                     // [DMAX][NGP][SS]
   double* const sPart = reinterpret_cast<double*>(sMem);  // [64][NT], after the items are computed
   const int lane = threadIdx.x;
   const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
   if (w >= F.nwaves) return;
   const int2 it = F.fitem[w * 64 + lane];
   const int n = F.wnode[w];
-  int e0 = 0, m = 0;
-  if (MAXS > 0) {
-    e0 = F.nptr[n];
-    m = F.nptr[n + 1] - e0;
-  }
-  const bool staged = MAXS > 0 && m <= MAXS;
-  if (staged) {
-    for (int t = lane; t < m * NGP * 3; t += 64) {
-      const int s = t / (NGP * 3), r = t % (NGP * 3), gp = r / 3, c = r % 3;
-      const int ea = F.nelA[e0 + s];
-      sMem[(s * NGP + gp) * 3 + c] = reinterpret_cast<const double2*>(
-          F.GS + (size_t)(ea >> 4) * St::TS + St::G0 + gp * NPE * 6 + (ea & 15) * 6)[c];
-    }
-    for (int t = lane; t < m * NGP * SSP; t += 64) {
-      const int s = t / (NGP * SSP), r = t % (NGP * SSP), gp = r / SSP, c = r % SSP;
-      sSc[t] = c < St::SS ? F.GS[(size_t)(F.nelA[e0 + s] >> 4) * St::TS + St::S0 + gp * St::SS + c] : 0.0;
-    }
-  }
   const bool live = it.x >= 0;
   const int j = live ? (it.y >> 10) : -1;
+  const int s = live ? ((it.y >> 4) & 63) : 0;
+  const uint64_t emask = wave_or64(live ? (1ull << s) : 0ull);
+  const int D = __popcll(emask);
+  const bool staged = DMAX > 0 && D <= DMAX;
+  const int slot = __popcll(emask & ((1ull << s) - 1ull));
+  if (staged) {
+    if (live) sE[slot] = it.x;  // e * 16 + a; every lane of one element writes the same value
+    __syncthreads();
+    for (int t = lane; t < D * (GA / 2); t += 64) {
+      const int d = t / (GA / 2), r = t % (GA / 2), gp = r / 3, c = r % 3;
+      const int ea = sE[d];
+      sMem[t] = reinterpret_cast<const double2*>(F.GS + (size_t)(ea >> 4) * St::TS + St::G0 + gp * NPE * 6 +
+                                                 (ea & 15) * 6)[c];
+    }
+    for (int t = lane; t < D * (SC / 2); t += 64) {
+      const int d = t / (SC / 2), c = t % (SC / 2);
+      sMem[DMAX * (GA / 2) + t] = reinterpret_cast<const double2*>(F.GS + (size_t)(sE[d] >> 4) * St::TS + St::S0)[c];
+    }
+  }
   __syncthreads();
   double v[NT];
 #pragma unroll
   for (int q = NV; q < NT; ++q) v[q] = 0.0;
   if (live) {
-    const int e = it.x >> 4, a = it.x & 15, s = (it.y >> 4) & 63, b = it.y & 15;
+    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
     const double* rec = F.GS + (size_t)e * St::TS;
     if (staged)
-      item_block<NPE, NGP, MIXED, NV, 6, SSP>(sGa + s * NGP * 6, sSc + s * NGP * SSP, rec + St::G0 + b * 6, a, b, v);
+      item_block<NPE, NGP, MIXED, NV, 6, St::SS>(sGa + slot * GA, sSc + slot * SC, rec + St::G0 + b * 6, a, b, v);
     else
       item_block<NPE, NGP, MIXED, NV, NPE * 6, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a,
                                                        b, v);
