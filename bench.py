@@ -157,6 +157,34 @@ def bench_learned(M, a) -> int:
     return 0
 
 
+def bench_full(M, a) -> int:
+    """Full-scale Newton: 128^3 Hex8, E = 1e3, nu = 0.49, bottom fixed, top UZ = -0.3 lambda over 15 load
+    steps; block-Jacobi PCG to 1e-6 per Newton iteration, ||R||/||R0|| < 1e-9 per step.  The first
+    --full-steps steps from the previous converged state (plain), from the extrapolation of the last
+    two converged steps, and from the learned guess (network trained on a 48^3 block's trajectory)."""
+    from mphyfem_amd import learned
+
+    n = a.n or 128
+    r = learned.block_experiment(n_train=48, n_eval=n, eval_steps=a.full_steps, epochs=a.epochs, log=log)
+
+    def per_step(mode):
+        later = r[mode][1:]  # step 1 has no history: every mode starts it plainly
+        return sum(s["seconds"] for s in later) / max(len(later), 1)
+
+    tp, te, tl = per_step("plain"), per_step("extrap"), per_step("learned")
+    out = {"metric": "s per converged load step (128^3 Hex8, nu=0.49, block-Jacobi PCG Newton)",
+           "value": round(tp, 3), "unit": "s", "n_gpus": 1, "steps": a.full_steps, "warmup": 0,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{n}^3 Hex8 block, E=1e3, nu=0.49, bottom fixed, top UZ=-0.3 lambda, 15 steps; "
+                                  f"first {a.full_steps} measured"},
+           "seconds_per_step": {"plain": round(tp, 3), "extrap": round(te, 3), "learned": round(tl, 3)},
+           "speedup_vs_plain": {"extrap": round(tp / te, 3) if te else None,
+                                "learned": round(tp / tl, 3) if tl else None},
+           "full": r}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration() -> int:
     """Three device-to-device copies of 1 GiB: what FETCH_SIZE / WRITE_SIZE read for known traffic."""
     import torch
@@ -176,9 +204,11 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tet10", "hex8", "batch", "learned"], default="tet10",
+    ap.add_argument("--workload", choices=["tet10", "hex8", "batch", "learned", "full"], default="tet10",
                     help="tet10/hex8: assembly; batch: 256 independent Hex8 blocks, batched Newton + Jacobi-PCG; "
-                         "learned: cantilever beams, plain vs learned Newton initial guess")
+                         "learned: cantilever beams, plain vs learned Newton initial guess; "
+                         "full: 128^3 Hex8 nu=0.49 Newton load steps, plain vs extrapolated start")
+    ap.add_argument("--full-steps", type=int, default=2, help="full: load steps measured (of 15)")
     ap.add_argument("--nmesh", type=int, default=256, help="batch: number of blocks")
     ap.add_argument("--beams", type=int, default=2000, help="learned: beams (the last --test are held out)")
     ap.add_argument("--train", type=int, default=1800)
@@ -215,6 +245,8 @@ def main(argv=None) -> int:
         return bench_batch(M, a)
     if wl == "learned":
         return bench_learned(M, a)
+    if wl == "full":
+        return bench_full(M, a)
     n = a.n or (60 if wl == "tet10" else 128)
     log(f"workload {wl}, n = {n}")
     X, conn, P, _ = build_problem(M, n, wl)

@@ -137,6 +137,7 @@ SIGNATURES = [
     ("mpf_mpnn_backward", C.c_int, [_P, _P, _P]),
     ("mpf_mpnn_adam", C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
     ("mpf_mpnn_guess", C.c_int, [_P, _P, _P, _P, C.c_double, C.c_int32, _P]),
+    ("mpf_mpnn_guarded_guess", C.c_int, [_P, _P, C.c_double, _P, _P, C.c_double, _P, _I32P]),
     ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
@@ -679,3 +680,13 @@ class Mpnn:
         torch.cuda.current_stream().synchronize()
         _check(lib().mpf_mpnn_guess(self._h, P._h, _dptr(uk), _dptr(ukm1), rho, int(use_net), _dptr(out)),
                "mpf_mpnn_guess")
+
+    def guarded_guess(self, P: Problem, lam: float, uk, ukm1, rho: float, out) -> int:
+        """Per mesh: the network's start point where it lowers the residual, else the extrapolation.
+        Returns the number of meshes that keep the network's correction."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        k = C.c_int32()
+        _check(lib().mpf_mpnn_guarded_guess(self._h, P._h, lam, _dptr(uk), _dptr(ukm1), rho, _dptr(out), C.byref(k)),
+               "mpf_mpnn_guarded_guess")
+        return k.value

@@ -75,6 +75,8 @@ struct mpf_mpnn {
   mpf::DevBuf<int32_t> doff_d;
   mpf::DevBuf<double> wres;           // per mesh: 1 / ||f_m||^2 over free dofs (unit external load)
   mpf::DevBuf<double> ebuf, rbuf, gbuf, msq;  // residual-loss work vectors
+  mpf::DevBuf<double> usave, gbuf2;           // guarded guess: saved state, second candidate
+  mpf::DevBuf<float> wmesh;                   // guarded guess: per-mesh weight of the correction
 };
 
 namespace mpf {
@@ -410,16 +412,25 @@ __global__ void __launch_bounds__(256) k_loss(int64_t n, const int32_t* __restri
 }
 
 // guess = u_k + rho (u_k - u_{k-1}) + s_m y   (prescribed dofs are reset by the solver)
+// guess = u_k + rho (u_k - u_{k-1}) + w_m s_m y, w_m = wmesh[m] (or use_net when wmesh is null)
 __global__ void k_guess(int64_t n, const int32_t* __restrict__ nmesh_of, const float* __restrict__ scale,
                         const double* __restrict__ uk, const double* __restrict__ ukm1, double rho, int use_net,
-                        const float* __restrict__ y, double* __restrict__ guess) {
+                        const float* __restrict__ wmesh, const float* __restrict__ y, double* __restrict__ guess) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double s = scale[nmesh_of[i]];
+  const int m = nmesh_of[i];
+  const double s = scale[m];
+  const double wgt = wmesh ? (double)wmesh[m] : (use_net ? 1.0 : 0.0);
   for (int d = 0; d < 3; ++d) {
     const double u = uk[3 * i + d];
-    guess[3 * i + d] = u + rho * (u - ukm1[3 * i + d]) + (use_net ? s * (double)y[i * kFout + d] : 0.0);
+    guess[3 * i + d] = u + rho * (u - ukm1[3 * i + d]) + (wgt != 0.0 ? wgt * s * (double)y[i * kFout + d] : 0.0);
   }
+}
+
+__global__ void k_set_bc_values(int32_t nbc, const int32_t* __restrict__ dof, const double* __restrict__ val,
+                                double lam, double* __restrict__ u) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbc) u[dof[i]] = val[i] * lam;
 }
 
 // e[dof] = s_m y - (u_next - u_k - rho (u_k - u_{k-1})) on the free dofs (the prediction's error)
@@ -869,9 +880,55 @@ mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const
   if (!net || !p || !uk || !ukm1 || !guess || net->ctx != p->uid) return MPF_ERR_ARG;
   const int64_t n = p->nNode;
   hipLaunchKernelGGL(k_guess, dim3(g256(n)), dim3(256), 0, net->stream, n, net->nmesh_of.ptr, net->scale.ptr, uk,
-                     ukm1, rho, use_net, net->y.ptr, guess);
+                     ukm1, rho, use_net, nullptr, net->y.ptr, guess);
   MPF_HIP(hipGetLastError());
   MPF_HIP(hipStreamSynchronize(net->stream));
+  return MPF_OK;
+}
+
+// Guarded learned start: per mesh, the network's correction is kept only where the residual at load
+// factor lam (prescribed dofs at their values) is smaller with it than at the plain extrapolation.
+// Two residual-only assemblies; the problem's displacement is restored.  n_learned: meshes kept.
+mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, const double* uk, const double* ukm1,
+                                  double rho, double* guess, int32_t* n_learned) {
+  if (!net || !p || !uk || !ukm1 || !guess || net->ctx != p->uid) return MPF_ERR_ARG;
+  const int64_t n = p->nNode;
+  const size_t n3 = (size_t)n * 3;
+  if (net->usave.n < n3) MPF_TRY(net->usave.alloc(n3));
+  if (net->gbuf2.n < n3) MPF_TRY(net->gbuf2.alloc(n3));
+  MPF_HIP(hipStreamSynchronize(net->stream));
+  hipStream_t s = p->stream;
+  MPF_HIP(hipMemcpyAsync(net->usave.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const int32_t nbc = (int32_t)p->bc_dof_h.size();
+  std::vector<double> sq[2];
+  for (int c = 0; c < 2; ++c) {  // 0: with the network, 1: extrapolation only
+    double* cand = c == 0 ? guess : net->gbuf2.ptr;
+    hipLaunchKernelGGL(k_guess, dim3(g256(n)), dim3(256), 0, s, n, net->nmesh_of.ptr, net->scale.ptr, uk, ukm1, rho,
+                       c == 0 ? 1 : 0, nullptr, net->y.ptr, cand);
+    MPF_HIP(hipMemcpyAsync(p->disp.ptr, cand, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (nbc > 0)
+      hipLaunchKernelGGL(k_set_bc_values, dim3(g256(nbc)), dim3(256), 0, s, nbc, p->bc_dof.ptr, p->bc_val.ptr, lam,
+                         p->disp.ptr);
+    MPF_TRY(enqueue_assembly(p, false, lam, 0));
+    hipLaunchKernelGGL(k_mesh_sq, dim3(net->nmesh), dim3(256), 0, s, net->doff_d.ptr, p->rhs.ptr, net->msq.ptr);
+    sq[c].resize(net->nmesh);
+    MPF_HIP(hipMemcpyAsync(sq[c].data(), net->msq.ptr, net->nmesh * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<float> w(net->nmesh);
+  int32_t kept = 0;
+  for (int m = 0; m < net->nmesh; ++m) {
+    w[m] = sq[0][m] < sq[1][m] ? 1.f : 0.f;
+    kept += w[m] != 0.f;
+  }
+  if (net->wmesh.n < (size_t)net->nmesh) MPF_TRY(net->wmesh.alloc(net->nmesh));
+  MPF_HIP(hipMemcpyAsync(net->wmesh.ptr, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_guess, dim3(g256(n)), dim3(256), 0, s, n, net->nmesh_of.ptr, net->scale.ptr, uk, ukm1, rho, 1,
+                     net->wmesh.ptr, net->y.ptr, guess);
+  MPF_HIP(hipMemcpyAsync(p->disp.ptr, net->usave.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  MPF_HIP(hipGetLastError());
+  MPF_HIP(hipStreamSynchronize(s));
+  if (n_learned) *n_learned = kept;
   return MPF_OK;
 }
 

@@ -75,6 +75,7 @@ def cantilever_problem(beams: Beams, idx) -> tuple[HexBatch, Problem]:
 class StepLog:
     iters: list = field(default_factory=list)      # per step: assemblies per mesh (np.int32 array)
     start_rel: list = field(default_factory=list)  # per step: mean ||R|| / ||R0|| at the start point
+    kept: list = field(default_factory=list)       # per step: meshes keeping the network's correction
     seconds: list = field(default_factory=list)    # per step: wall time of the batched step
     newton: list = field(default_factory=list)     # per step: batch Newton iterations (slowest mesh)
 
@@ -103,7 +104,9 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
             net.features(P, hist[1], hist[0], rho, lam, k / (k + 1.0))
             if mode == "learned":
                 net.forward(P)
-            net.guess(P, hist[1], hist[0], rho, mode == "learned", guess)
+                log.kept.append(net.guarded_guess(P, lam, hist[1], hist[0], rho, guess))
+            else:
+                net.guess(P, hist[1], hist[0], rho, False, guess)
             g = guess
         iters, rel, rep = P.batch_newton_step(lam, guess=g, **solver)
         log.seconds.append(time.perf_counter() - t0)
@@ -154,7 +157,8 @@ def summarize(log: StepLog, first: int = 1) -> dict:
             "batch_newton_iters": int(np.sum(log.newton[first:])),
             "seconds": round(float(np.sum(log.seconds[first:])), 4),
             "seconds_all_steps": round(float(np.sum(log.seconds)), 4),
-            "start_relres_geomean": float(np.exp(np.mean(np.log(np.maximum(log.start_rel[first:], 1e-300)))))}
+            "start_relres_geomean": float(np.exp(np.mean(np.log(np.maximum(log.start_rel[first:], 1e-300))))),
+            "meshes_keeping_correction_per_step": [int(x) for x in log.kept]}
 
 
 def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs: int = 60, lr: float = 2e-3,
@@ -192,4 +196,95 @@ def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs:
         out[mode] = summarize(run_steps(Pte, mode, net, nsteps, **solver))
     out["speedup_learned_vs_plain"] = round(out["plain"]["seconds"] / max(out["learned"]["seconds"], 1e-12), 3)
     out["speedup_extrap_vs_plain"] = round(out["plain"]["seconds"] / max(out["extrap"]["seconds"], 1e-12), 3)
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# single large problem: the 128^3 compression block
+# ---------------------------------------------------------------------------------------------
+BLOCK_STEPS = 15
+
+
+def compression_block(n: int) -> Problem:
+    """n^3 Hex8 unit block, E = 1e3, nu = 0.49, bottom fixed, top UZ = -0.3 lambda (15 load steps)."""
+    from . import TANGENT_CONSISTENT as TC, matl_from_E_nu
+
+    X, conn = mesh_hex(n, n, n, jitter=0.0, seed=5)
+    P = Problem(HEX8, X, conn, matl_from_E_nu(1e3, 0.49, tangent=TC))
+    P.add_dirichlet(face_nodes(HEX8, n, n, n, 4), -1)
+    P.add_dirichlet(face_nodes(HEX8, n, n, n, 5), 2, -0.3)
+    P.prepare_matrix_pattern()
+    return P
+
+
+def block_newton_opts(P: Problem):
+    from . import KSP_CG, PC_BJACOBI3
+    # every linear solve to 1e-6: the first (lifted) iteration must carry the top-face increment into
+    # the block accurately, or the element layer under the face inverts (a loose first solve does)
+    return P.newton_opts(nsteps=1, dt=1.0 / BLOCK_STEPS, itmax=30, rtol=1e-9, ksp=KSP_CG, pc=PC_BJACOBI3,
+                         lin_rtol=1e-6, maxit=100000, forcing=False)
+
+
+def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, store=None, log=print):
+    """Load steps 1..nsteps of the block from 'plain', 'extrap' or 'learned' start points."""
+    import torch
+
+    n3 = 3 * P.nNode
+    P.set_state(np.zeros(n3))
+    opts = block_newton_opts(P)
+    dev = torch.device("cuda")
+    hist = [torch.zeros(n3, dtype=torch.float64, device=dev) for _ in range(2)]
+    guess = torch.zeros(n3, dtype=torch.float64, device=dev)
+    if store is not None:
+        store[0].zero_()
+    steps = []
+    for k in range(nsteps):
+        lam = (k + 1) / BLOCK_STEPS
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = None
+        kept = None
+        if mode != "plain" and k >= 1:
+            net.features(P, hist[1], hist[0], 1.0, lam, k / (k + 1.0))
+            if mode == "learned":
+                net.forward(P)
+                kept = net.guarded_guess(P, lam, hist[1], hist[0], 1.0, guess)
+            else:
+                net.guess(P, hist[1], hist[0], 1.0, False, guess)
+            g = guess.cpu().numpy()
+        it, rep = P.newton_step(lam, opts, g)
+        dt = time.perf_counter() - t0
+        hist[0].copy_(hist[1])
+        P.copy_state_device(hist[1])
+        if store is not None:
+            store[k + 1].copy_(hist[1])
+        steps.append({"step": k + 1, "assemblies": it, "krylov_iterations": rep.krylov_iters, "seconds": round(dt, 3),
+                      "network_correction_kept": kept})
+        if log:
+            log(f"block {mode} step {k + 1}: {it} assemblies, {rep.krylov_iters} PCG iterations, {dt:.2f} s")
+    return steps
+
+
+def block_experiment(n_train: int = 48, n_eval: int = 128, eval_steps: int = 3, epochs: int = 40,
+                     lr: float = 2e-3, log=print) -> dict:
+    """Train the network on the trajectory of an n_train^3 block (all 15 steps, plain Newton), then run
+    the first eval_steps steps of the n_eval^3 block from plain, extrapolated and learned start points."""
+    import torch
+
+    out = {"train_block": n_train, "eval_block": n_eval, "eval_steps": eval_steps, "epochs": epochs}
+    Pt = compression_block(n_train)
+    traj = torch.zeros((BLOCK_STEPS + 1, 3 * Pt.nNode), dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    block_steps(Pt, "plain", BLOCK_STEPS, store=traj, log=None)
+    out["train_trajectory_seconds"] = round(time.perf_counter() - t0, 2)
+    net = Mpnn(seed=5)
+    t1 = time.perf_counter()
+    losses = train(net, Pt, traj, epochs=epochs, lr=lr, log_every=0)
+    out["train_seconds"] = round(time.perf_counter() - t1, 2)
+    out["loss_first"], out["loss_last"] = losses[0], losses[-1]
+    del traj, Pt
+    Pe = compression_block(n_eval)
+    for mode in ("plain", "extrap", "learned"):
+        out[mode] = block_steps(Pe, mode, eval_steps, net, log=log)
+    Pe.close()
     return out
