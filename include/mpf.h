@@ -122,9 +122,12 @@ mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, do
 /*     rhs entry is written once (no zero fill, no atomics, bitwise reproducible).              */
 /*   MPF_ASSEMBLY_COLORED_SCATTER: elements of one colour per launch add their K_e/R_e blocks into  */
 /*     CSR through the per-element slot map (no two elements of a launch share a node).           */
+/*   MPF_ASSEMBLY_GATHER_DIRECT: the gather with every item reading the staging records directly   */
+/*     (no LDS staging of the node's own data; the path a wavefront takes when its items come     */
+/*     from more elements than LDS holds).                                                        */
 /* The gather tables hold up to 64 elements and 128 coupled nodes per node; for a mesh beyond    */
 /* that the problem switches itself to the coloured scatter (mpf_get_assembly_mode reports it).  */
-enum { MPF_ASSEMBLY_GATHER = 0, MPF_ASSEMBLY_COLORED_SCATTER = 1 };
+enum { MPF_ASSEMBLY_GATHER = 0, MPF_ASSEMBLY_COLORED_SCATTER = 1, MPF_ASSEMBLY_GATHER_DIRECT = 2 };
 mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode);
 mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode);
 

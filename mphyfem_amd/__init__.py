@@ -24,7 +24,7 @@ HEX8 = 8
 TET10P1 = 10
 KSP_CG, KSP_GMRES = 0, 1
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
-ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER = 0, 1
+ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER, ASSEMBLY_GATHER_DIRECT = 0, 1, 2
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NEG_JACOBIAN, ERR_STATE, ERR_NOT_CONVERGED, ERR_DIVERGED, ERR_NO_DEVICE = range(1, 8)
@@ -526,7 +526,7 @@ class Problem:
         return it.value, rep
 
     def set_assembly_mode(self, mode: int):
-        """ASSEMBLY_GATHER (row-owner, default) or ASSEMBLY_COLORED_SCATTER."""
+        """ASSEMBLY_GATHER (row-owner, default), ASSEMBLY_COLORED_SCATTER or ASSEMBLY_GATHER_DIRECT."""
         _check(lib().mpf_set_assembly_mode(self._h, mode), "mpf_set_assembly_mode")
 
     def assembly_mode(self) -> int:

@@ -416,6 +416,7 @@ struct FlatArgs {
   const int32_t* nelA;
   const int32_t* nbptr;
   const int32_t* btab;
+  int32_t stage;            // 0: every item reads the staging records directly
   const int32_t* nda;
   const int32_t* idp;
   int32_t dispDOF;
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   const int s = live ? ((it.y >> 4) & 63) : 0;
   const uint64_t emask = wave_or64(live ? (1ull << s) : 0ull);
   const int D = __popcll(emask);
-  const bool staged = DMAX > 0 && D <= DMAX;
+  const bool staged = DMAX > 0 && F.stage && D <= DMAX;
   const int slot = __popcll(emask & ((1ull << s) - 1ull));
   if (staged) {
     if (live) sE[slot] = it.x;  // e * 16 + a; every lane of one element writes the same value
@@ -828,6 +829,7 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   F.nelA = P.g_nelA.ptr;
   F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
+  F.stage = p->asm_mode != MPF_ASSEMBLY_GATHER_DIRECT;
   F.nda = P.nda.ptr;
   F.idp = P.idp.ptr;
   F.dispDOF = P.dispDOF;

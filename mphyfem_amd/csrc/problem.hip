@@ -213,7 +213,7 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   ElemArgs A = elem_args(p);
-  if (p->asm_mode == MPF_ASSEMBLY_GATHER) {
+  if (p->asm_mode == MPF_ASSEMBLY_GATHER || p->asm_mode == MPF_ASSEMBLY_GATHER_DIRECT) {
     return launch_gather_assembly(p, A, first_iter ? p->dispApplied.ptr : nullptr, scatterK, s);
   }
   ScatterArgs sc{};
@@ -533,7 +533,7 @@ mpf_status mpf_prepare_pattern(mpf_problem* p) {
   MPF_TRY(p->vals.alloc(p->pat.nnz > 0 ? p->pat.nnz : 1));
   MPF_TRY(p->rhs.alloc(n > 0 ? n : 1));
   MPF_TRY(p->du.alloc(n > 0 ? n : 1));
-  if (p->asm_mode == MPF_ASSEMBLY_GATHER) MPF_TRY(ensure_gather_tables(p));
+  if (p->asm_mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->du.ptr, n, 0.0);
   if (!p->trac.empty()) {
     std::vector<double> F;
@@ -612,9 +612,11 @@ mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
 }
 
 mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
-  if (!p || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER)) return MPF_ERR_ARG;
+  if (!p || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER &&
+             mode != MPF_ASSEMBLY_GATHER_DIRECT))
+    return MPF_ERR_ARG;
   p->asm_mode = mode;
-  if (p->prepared && mode == MPF_ASSEMBLY_GATHER) MPF_TRY(ensure_gather_tables(p));
+  if (p->prepared && mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
   return MPF_OK;
 }
 

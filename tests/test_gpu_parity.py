@@ -187,7 +187,7 @@ def test_pattern_and_colouring_bitexact_tet(mpf, orc, gpu):
     assert pat.ncolors == ncol and np.array_equal(pat.color, col)
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["gather", "colored_scatter"])
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["gather", "colored_scatter", "gather_direct"])
 @pytest.mark.parametrize("tangent,traction", [(0, None), (1, (-20.0, 5.0))])
 def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction, mode):
     n = 10
@@ -209,7 +209,7 @@ def test_global_assembly_hex_vs_oracle(mpf, orc, gpu, tangent, traction, mode):
         assert abs(P.rhs_norm() - np.linalg.norm(orhs)) < 1e-12 * np.linalg.norm(orhs)
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["gather", "colored_scatter"])
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["gather", "colored_scatter", "gather_direct"])
 def test_global_assembly_tet_vs_oracle(mpf, orc, gpu, mode):
     n = 4
     X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
