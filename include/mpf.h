@@ -123,13 +123,16 @@ mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, do
 /*   MPF_ASSEMBLY_COLORED_SCATTER: elements of one colour per launch add their K_e/R_e blocks into  */
 /*     CSR through the per-element slot map (no two elements of a launch share a node).           */
 /*   MPF_ASSEMBLY_GATHER_DIRECT: the gather with every item reading the staging records directly   */
-/*     (no LDS staging of the node's own data; the path a wavefront takes when its items come     */
-/*     from more elements than LDS holds).                                                        */
+/*     (for Hex8 the default copies the records of each chunk of 8 wavefronts to LDS first).      */
 /* The gather tables hold up to 64 elements and 128 coupled nodes per node; for a mesh beyond    */
 /* that the problem switches itself to the coloured scatter (mpf_get_assembly_mode reports it).  */
 enum { MPF_ASSEMBLY_GATHER = 0, MPF_ASSEMBLY_COLORED_SCATTER = 1, MPF_ASSEMBLY_GATHER_DIRECT = 2 };
 mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode);
 mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode);
+/* Hex8 gather tuning: the largest number of distinct elements (0..32, default 32) whose staging  */
+/* records a chunk of 8 wavefronts copies to LDS; a chunk with more reads them directly (same     */
+/* values).  No counterpart in the reference; 0 makes every chunk take the direct reads.         */
+mpf_status mpf_set_chunk_stage_cap(mpf_problem* p, int32_t cap);
 
 /* prepareMatrixPattern (femSolidmechanics2.cpp:121-478) on the device: DOF numbering, pressure   */
 /* dofs, CSR pattern (rowptr/colidx, int32, sorted), element colouring, per-element CSR slots.   */

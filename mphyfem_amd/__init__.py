@@ -140,6 +140,7 @@ SIGNATURES = [
     ("mpf_mpnn_guarded_guess", C.c_int, [_P, _P, C.c_double, _P, _P, C.c_double, _P, _I32P]),
     ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
+    ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                             C.c_uint64, _DP, _I32P]),
@@ -528,6 +529,10 @@ class Problem:
     def set_assembly_mode(self, mode: int):
         """ASSEMBLY_GATHER (row-owner, default), ASSEMBLY_COLORED_SCATTER or ASSEMBLY_GATHER_DIRECT."""
         _check(lib().mpf_set_assembly_mode(self._h, mode), "mpf_set_assembly_mode")
+
+    def set_chunk_stage_cap(self, cap: int):
+        """Hex8 gather: most distinct elements a chunk of 8 wavefronts stages in LDS (0..32)."""
+        _check(lib().mpf_set_chunk_stage_cap(self._h, cap), "mpf_set_chunk_stage_cap")
 
     def assembly_mode(self) -> int:
         v = C.c_int32()

@@ -172,11 +172,13 @@ __device__ __forceinline__ int rank3(int mask, int i) { return __popc(mask & ((1
 // -dv sum_m g_a,m sigma_i,m, is reduced over the NGP lanes of the element (consecutive lanes,
 // lane & (NGP-1) == gp) and stored by the lane gp == a % NGP into R[a*3 + i].
 // Elem_Solid_3D_Disp.cpp:250-330; ElementBase2.cpp:350-470.
-template <int NPE, int NGP, bool MIXED>
+// COMPACT output (the gather path's Hex8 staging records): per node g (3) at G + a * 3, and the six
+// components b00 b11 b22 b01 b02 b12 of the symmetric b at Bv, instead of g and b g at G + a * 6.
+template <int NPE, int NGP, bool MIXED, bool COMPACT = false>
 __device__ __forceinline__ void element_point(const double (*X)[3], const double (*U)[3], const double (*UP)[3],
                                               const double* Pn, const double (*d)[NPE], int gp, double wgp,
                                               double mu, double K, int Utype, bool consistent, bool writer,
-                                              double* G, double* S, double* R, bool& negJ) {
+                                              double* G, double* S, double* R, bool& negJ, double* Bv = nullptr) {
   double B[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Bi[3][3];
 #pragma unroll
   for (int a = 0; a < NPE; ++a)
@@ -236,7 +238,10 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
     const double g0 = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
     const double g1 = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
     const double g2 = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-    if (writer) {
+    if (writer && COMPACT) {
+      double* Ga = G + a * 3;
+      Ga[0] = g0; Ga[1] = g1; Ga[2] = g2;
+    } else if (writer) {
       double* Ga = G + a * 6;
       Ga[0] = g0; Ga[1] = g1; Ga[2] = g2;
       Ga[3] = P.b[0][0] * g0 + P.b[0][1] * g1 + P.b[0][2] * g2;
@@ -257,7 +262,18 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   if (writer) {
     S[0] = P.s1; S[1] = P.s2; S[2] = P.s3; S[3] = P.s4; S[4] = P.s5;
     if (MIXED) { S[4] = dv; S[5] = dv0; S[6] = thetahat; S[7] = J - Jhat - thetahat * pres; }
+    if (COMPACT) {
+      Bv[0] = P.b[0][0]; Bv[1] = P.b[1][1]; Bv[2] = P.b[2][2];
+      Bv[3] = P.b[0][1]; Bv[4] = P.b[0][2]; Bv[5] = P.b[1][2];
+    }
   }
+}
+
+// b g from the six stored components of the symmetric b (same products and order as element_point).
+__device__ __forceinline__ void bsym_times(const double* B6, const double g[3], double bg[3]) {
+  bg[0] = B6[0] * g[0] + B6[3] * g[1] + B6[4] * g[2];
+  bg[1] = B6[3] * g[0] + B6[1] * g[1] + B6[5] * g[2];
+  bg[2] = B6[4] * g[0] + B6[5] * g[1] + B6[2] * g[2];
 }
 
 // Material parameters of element e: (mu, K) from the per-element table when present.
@@ -282,12 +298,18 @@ __device__ __forceinline__ void tangent_gp(const double ga[3], const double ba[3
   const double y = HAS_S5 ? (s4 * ba[0] + s5 * ga[0]) * gb[0] + (s4 * ba[1] + s5 * ga[1]) * gb[1] +
                                 (s4 * ba[2] + s5 * ga[2]) * gb[2]
                           : s4 * (ba[0] * gb[0] + ba[1] * gb[1] + ba[2] * gb[2]);
+  const double s1ga[3] = {s1 * ga[0], s1 * ga[1], s1 * ga[2]};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const double Qi = s2 * ga[i] + s3 * ba[i], Ti = s3 * ga[i];
 #pragma unroll
-    for (int kk = 0; kk < 3; ++kk)
-      Kb[3 * i + kk] += (s1 * ga[kk]) * gb[i] + Qi * gb[kk] + Ti * bb[kk] + (i == kk ? y : 0.0);
+    for (int kk = 0; kk < 3; ++kk) {  // three fused multiply-adds into the running block entry
+      double t = Kb[3 * i + kk];
+      t = fma(s1ga[kk], gb[i], t);
+      t = fma(Qi, gb[kk], t);
+      t = fma(Ti, bb[kk], t);
+      Kb[3 * i + kk] = i == kk ? t + y : t;
+    }
   }
 }
 

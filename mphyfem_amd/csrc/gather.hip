@@ -8,17 +8,21 @@
 //   stage kernel  (one wavefront per EPW elements, lane = (element, Gauss point)):
 //       per element and Gauss point, grad_x N and b grad_x N of every node and the tangent
 //       scalars, plus the element residual, into a staging record of TS doubles per element.
-//   row kernel    (one wavefront per node n):
+//   matrix kernel (one wavefront per 64 laid-out items of one node n):
 //       for every element e around n and every node b of e, the 3x3 block K_{n b} (and the u-p
 //       couplings) from e's staging record, visited in (column block, element id) order from a table
 //       built once with the pattern; each column block's run of contributions is summed in element
 //       order, then every CSR value of n's rows and n's right-hand-side entries are written once.
+//       Hex8 runs it per chunk of 8 wavefronts whose elements' records are first copied to LDS.
 //
 // Nothing is read-modify-written, so there is no zero fill, no colouring and no atomics on the FP64
 // path, and the result is bitwise reproducible.  Each column block's contributions are summed in
 // element order, the reference's own order for a given row (femSolidmechanics2.cpp:1061).
 
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
 
 #include "fem_device.hpp"
 
@@ -32,17 +36,53 @@ constexpr int kMaxAdj = 64;   // elements around one node
 constexpr int kMaxNB = 128;   // nodes coupled to one node (column blocks of its rows)
 
 // Staging record layout (doubles) per element:
-//   G  [NGP][NPE][6]   g_a (3), (b g)_a (3)
+//   Tet10/P1:  G [NGP][NPE][6]  g_a (3), (b g)_a (3)
+//   Hex8:      G [NGP][NPE][3]  g_a;  B [NGP][6]  b00 b11 b22 b01 b02 b12  (b g formed on use: 304
+//              doubles per element instead of 448)
 //   S  [NGP][SS]       Hex8: s1..s5 (SS = 5); Tet10/P1: s1..s4, dv, dv0, thetahat, resip (SS = 8)
 //   R  [NPE][3]        element residual (no lift)
+// The first SL doubles (G, B, S) are what the matrix kernel reads for a block.
 template <int NPE, int NGP, bool MIXED>
 struct Stage {
+  static constexpr bool COMPACT = !MIXED;
+  static constexpr int GW = COMPACT ? 3 : 6;
   static constexpr int SS = MIXED ? 8 : 5;
   static constexpr int G0 = 0;
-  static constexpr int S0 = NGP * NPE * 6;
+  static constexpr int B0 = NGP * NPE * GW;
+  static constexpr int S0 = B0 + (COMPACT ? NGP * 6 : 0);
   static constexpr int R0 = S0 + NGP * SS;
+  static constexpr int SL = R0;
   static constexpr int TS = ((R0 + NPE * 3) + 7) / 8 * 8;
+  static_assert(B0 % 2 == 0 && S0 % 2 == 0 && SL % 2 == 0, "16-byte groups");
 };
+
+// g, b g of node c at Gauss point gp from a staging record, either layout
+template <int NPE, int NGP, bool MIXED>
+__device__ __forceinline__ void record_g(const double* rec, int gp, int c, double g[3], double bg[3]) {
+  using St = Stage<NPE, NGP, MIXED>;
+  const double* p = rec + St::G0 + (gp * NPE + c) * St::GW;
+  g[0] = p[0]; g[1] = p[1]; g[2] = p[2];
+  if (St::COMPACT) {
+    bsym_times(rec + St::B0 + gp * 6, g, bg);
+  } else {
+    bg[0] = p[3]; bg[1] = p[4]; bg[2] = p[5];
+  }
+}
+
+// 3x3 tangent block (a, b) of one element from its staging record, either layout
+template <int NPE, int NGP, bool MIXED>
+__device__ __forceinline__ void record_block(const double* rec, int a, int b, double Kb[9]) {
+  using St = Stage<NPE, NGP, MIXED>;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) Kb[t] = 0.0;
+  for (int gp = 0; gp < NGP; ++gp) {
+    double ga[3], ba[3], gb[3], bb[3];
+    record_g<NPE, NGP, MIXED>(rec, gp, a, ga, ba);
+    record_g<NPE, NGP, MIXED>(rec, gp, b, gb, bb);
+    const double* S = rec + St::S0 + gp * St::SS;
+    tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], Kb);
+  }
+}
 
 template <int NPE, int NGP, bool MIXED, int EPW>
 __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double* __restrict__ GS) {
@@ -87,9 +127,10 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   bool negJ = false;
   double mu, K;
   elem_material(A, writer ? e : base, mu, K);
-  element_point<NPE, NGP, MIXED>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp], gp, wgp,
-                                 mu, K, A.Utype, A.consistent != 0, writer,
-                                 rec + St::G0 + gp * NPE * 6, rec + St::S0 + gp * St::SS, rec + St::R0, negJ);
+  element_point<NPE, NGP, MIXED, St::COMPACT>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp],
+                                              gp, wgp, mu, K, A.Utype, A.consistent != 0, writer,
+                                              rec + St::G0 + gp * NPE * St::GW, rec + St::S0 + gp * St::SS,
+                                              rec + St::R0, negJ, rec + St::B0 + gp * 6);
   if (writer && negJ) flag_bad(A.bad, e);
 }
 
@@ -111,7 +152,7 @@ __global__ void k_morton(int32_t nNode, const double* __restrict__ X, double x0,
   const double q[3] = {(X[3 * (size_t)i] - x0) * sx, (X[3 * (size_t)i + 1] - y0) * sy, (X[3 * (size_t)i + 2] - z0) * sz};
   uint64_t c[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) c[d] = (uint64_t)fmin(fmax(q[d], 0.0), 2097151.0);
+  for (int d = 0; d < 3; ++d) c[d] = (uint64_t)fmin(fmax(q[d] + 0.5, 0.0), 2097151.0);
   code[i] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
   id[i] = i;
 }
@@ -371,24 +412,38 @@ __global__ void k_near(int32_t nNode, const int32_t* __restrict__ nptr, const in
 }
 
 // One (element, node b) item of node n's rows: the 3x3 block K_{n b} (v[0..8]) and, for the mixed
-// element, Kup(n, b) (v[9..11]), Kpu(n, b) (v[12..14]) and Kpp(n, b) (v[15]) from the element's
-// staged Gauss-point data: node n's g, b g at gA (stride GAS per Gauss point), the Gauss-point
-// scalars at sS (stride SS), node b's g, b g at gB in the staging record (stride NPE * 6); the
-// six-double groups are read as 16-byte pairs.
-template <int NPE, int NGP, bool MIXED, int NV, int GAS, int SS>
-__device__ __forceinline__ void item_block(const double* gA, const double* sS, const double* gB, int a, int b,
-                                           double v[NV]) {
+// element, Kup(n, b) (v[9..11]), Kpu(n, b) (v[12..14]) and Kpp(n, b) (v[15]) from the first SL
+// doubles of the element's staging record at rec (in global memory or copied to LDS; LDS: Gauss
+// points one at a time, since LDS latency needs no unrolling to hide and registers are the limit).
+template <int NPE, int NGP, bool MIXED, int NV, bool IN_LDS = false>
+__device__ __forceinline__ void item_block(const double* rec, int a, int b, double v[NV]) {
+  using St = Stage<NPE, NGP, MIXED>;
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
-#pragma unroll 2
+#ifndef MPF_LDS_UNROLL
+#define MPF_LDS_UNROLL 1
+#endif
+#pragma unroll(IN_LDS ? MPF_LDS_UNROLL : 2)
   for (int gp = 0; gp < NGP; ++gp) {
-    const double2* ga2 = reinterpret_cast<const double2*>(gA + gp * GAS);
-    const double2* gb2 = reinterpret_cast<const double2*>(gB + gp * NPE * 6);
-    const double* S = sS + gp * SS;
-    const double2 p0 = ga2[0], p1 = ga2[1], p2 = ga2[2];
-    const double2 q0 = gb2[0], q1 = gb2[1], q2 = gb2[2];
-    const double ga[3] = {p0.x, p0.y, p1.x}, ba[3] = {p1.y, p2.x, p2.y};
-    const double gb[3] = {q0.x, q0.y, q1.x}, bb[3] = {q1.y, q2.x, q2.y};
+    const double* Gp = rec + St::G0 + gp * NPE * St::GW;
+    const double* S = rec + St::S0 + gp * St::SS;
+    double ga[3], ba[3], gb[3], bb[3];
+    if (St::COMPACT) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
+      const double2* B2 = reinterpret_cast<const double2*>(rec + St::B0 + gp * 6);
+      const double2 c0 = B2[0], c1 = B2[1], c2 = B2[2];
+      const double B6[6] = {c0.x, c0.y, c1.x, c1.y, c2.x, c2.y};
+      bsym_times(B6, ga, ba);
+      bsym_times(B6, gb, bb);
+    } else {
+      const double2* ga2 = reinterpret_cast<const double2*>(Gp + a * 6);
+      const double2* gb2 = reinterpret_cast<const double2*>(Gp + b * 6);
+      const double2 p0 = ga2[0], p1 = ga2[1], p2 = ga2[2];
+      const double2 q0 = gb2[0], q1 = gb2[1], q2 = gb2[2];
+      ga[0] = p0.x; ga[1] = p0.y; ga[2] = p1.x; ba[0] = p1.y; ba[1] = p2.x; ba[2] = p2.y;
+      gb[0] = q0.x; gb[1] = q0.y; gb[2] = q1.x; bb[0] = q1.y; bb[1] = q2.x; bb[2] = q2.y;
+    }
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], v);
     if (MIXED) {
       double Np[4];
@@ -408,142 +463,61 @@ __device__ __forceinline__ void item_block(const double* gA, const double* sS, c
   }
 }
 
+// Node a's residual rows of one element (v[0..2]) and, for the mixed element, its pressure row (v[3])
+template <int NPE, int NGP, bool MIXED>
+__device__ __forceinline__ void item_residual(const double* rec, int a, double* v) {
+  using St = Stage<NPE, NGP, MIXED>;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = rec[St::R0 + a * 3 + i];
+  if (MIXED && a < 4) {
+    double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
+#pragma unroll
+    for (int gp = 0; gp < NGP; ++gp) {
+      double Np[4];
+      tet_Np(gp, Np);
+      const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
+      fp -= (rec[St::S0 + gp * St::SS + 5] * Na) * rec[St::S0 + gp * St::SS + 7];
+    }
+    v[3] = fp;
+  }
+}
+
 struct FlatArgs {
   int64_t nwaves;
   const int2* fitem;        // {e * 16 + a, (j << 10) | (s << 4) | b}, {-1, -1} in padding slots
   const int32_t* wnode;     // node whose rows each wavefront's items belong to
-  const int32_t* nptr;
-  const int32_t* nelA;
   const int32_t* nbptr;
   const int32_t* btab;
-  int32_t stage;            // 0: every item reads the staging records directly
   const int32_t* nda;
   const int32_t* idp;
   int32_t dispDOF;
   const double* GS;
   double* vals;
   double* rhs;
+  // chunks of CW wavefronts (chunked matrix kernel): distinct elements of each chunk's items in
+  // ascending order (ECAP slots per chunk), their number, and each item's slot
+  int64_t nchunks;
+  int32_t ecap;             // chunks with at most ecap (<= ECAP) distinct elements are staged
+  const int32_t* ctab;
+  const int32_t* ccnt;
+  const uint8_t* fslot;
 };
 
-// Owner-side staging of the matrix kernel: the node's own Gauss-point data (g, b g of node n and the
-// scalars) in each distinct element of a wavefront's items is copied to LDS once per wavefront;
-// each item then gathers only its other node's data from the element's staging record.  Runs are
-// laid out in order of their first element, so a wavefront's items come from few elements.  A
-// wavefront with more than DMAX distinct elements reads everything from the records.  Measured per
-// assembly: Hex8 11.2 ms staged vs 15.3 ms not; Tet10/P1 9.37 ms not staged vs 9.41 (DMAX 12) and
-// 9.54 ms (DMAX 24), so Tet10/P1 does not stage.
-#ifndef MPF_FLAT_DMAX_HEX
-#define MPF_FLAT_DMAX_HEX 8
-#endif
-#ifndef MPF_FLAT_DMAX_TET
-#define MPF_FLAT_DMAX_TET 0
-#endif
-template <int NPE>
-struct FlatCap {
-  static constexpr int DMAX = NPE == 8 ? MPF_FLAT_DMAX_HEX : MPF_FLAT_DMAX_TET;
-};
-
-__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
-    hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
-  }
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Matrix kernel: one wavefront per 64 laid-out items of one node n.  Item lanes compute their blocks,
-// and the first lane of each column run (its head) sums the run in element order and writes the
-// column block of n's rows, so every CSR value is written exactly once.  The self column's run also
-// sums n's residual rows.
-template <int NPE, int NGP, bool MIXED>
-__global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
-  using St = Stage<NPE, NGP, MIXED>;
-  constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
+// Column runs of one wavefront's items, after every item lane has stored its values at part[t * NT]
+// and its column at col[t]: the first lane of each run (its head) sums the run in lane order, which
+// is element order, and writes the column block of node n's rows; the self column's head also
+// writes n's right-hand-side rows.  Every CSR value is written exactly once.
+template <bool MIXED, int NV, int NT>
+__device__ __forceinline__ void run_heads(const FlatArgs& F, int n, int lane, int j, const double v[NT],
+                                          const int32_t* col, const double* part) {
   constexpr int BT = MIXED ? 4 : 2;
-  constexpr int NR = MIXED ? 4 : 3;   // right-hand-side rows of a node (self items)
-  constexpr int NT = NV + NR;
-  constexpr int DMAX = FlatCap<NPE>::DMAX;
-  constexpr int GA = NGP * 6, SC = NGP * St::SS;  // owner data per element (doubles, both even)
-  static_assert(GA % 2 == 0 && SC % 2 == 0, "16-byte staging");
-  constexpr int STG = DMAX * (GA + SC);
-  constexpr int LDSD = STG > 64 * NT ? STG : 64 * NT;  // the staging area is reused for the run sums
-  __shared__ int32_t sB[64];
-  __shared__ int32_t sE[DMAX > 0 ? DMAX : 1];
-  __shared__ double2 sMem[LDSD / 2];
-  double* const sGa = reinterpret_cast<double*>(sMem);    // [DMAX][NGP][6]
-  double* const sSc = sGa + DMAX * GA;                     // [DMAX][NGP][SS]
-  double* const sPart = reinterpret_cast<double*>(sMem);  // [64][NT], after the items are computed
-  const int lane = threadIdx.x;
-  const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
-  if (w >= F.nwaves) return;
-  const int2 it = F.fitem[w * 64 + lane];
-  const int n = F.wnode[w];
-  const bool live = it.x >= 0;
-  const int j = live ? (it.y >> 10) : -1;
-  const int s = live ? ((it.y >> 4) & 63) : 0;
-  const uint64_t emask = wave_or64(live ? (1ull << s) : 0ull);
-  const int D = __popcll(emask);
-  const bool staged = DMAX > 0 && F.stage && D <= DMAX;
-  const int slot = __popcll(emask & ((1ull << s) - 1ull));
-  if (staged) {
-    if (live) sE[slot] = it.x;  // e * 16 + a; every lane of one element writes the same value
-    __syncthreads();
-    for (int t = lane; t < D * (GA / 2); t += 64) {
-      const int d = t / (GA / 2), r = t % (GA / 2), gp = r / 3, c = r % 3;
-      const int ea = sE[d];
-      sMem[t] = reinterpret_cast<const double2*>(F.GS + (size_t)(ea >> 4) * St::TS + St::G0 + gp * NPE * 6 +
-                                                 (ea & 15) * 6)[c];
-    }
-    for (int t = lane; t < D * (SC / 2); t += 64) {
-      const int d = t / (SC / 2), c = t % (SC / 2);
-      sMem[DMAX * (GA / 2) + t] = reinterpret_cast<const double2*>(F.GS + (size_t)(sE[d] >> 4) * St::TS + St::S0)[c];
-    }
-  }
-  __syncthreads();
-  double v[NT];
-#pragma unroll
-  for (int q = NV; q < NT; ++q) v[q] = 0.0;
-  if (live) {
-    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-    const double* rec = F.GS + (size_t)e * St::TS;
-    if (staged)
-      item_block<NPE, NGP, MIXED, NV, 6, St::SS>(sGa + slot * GA, sSc + slot * SC, rec + St::G0 + b * 6, a, b, v);
-    else
-      item_block<NPE, NGP, MIXED, NV, NPE * 6, St::SS>(rec + St::G0 + a * 6, rec + St::S0, rec + St::G0 + b * 6, a,
-                                                       b, v);
-    if (b == a) {  // the element's own item of node n carries its residual rows
-#pragma unroll
-      for (int i = 0; i < 3; ++i) v[NV + i] = rec[St::R0 + a * 3 + i];
-      if (MIXED && a < 4) {
-        double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
-#pragma unroll
-        for (int gp = 0; gp < NGP; ++gp) {
-          double Np[4];
-          tet_Np(gp, Np);
-          const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
-          fp -= (rec[St::S0 + gp * St::SS + 5] * Na) * rec[St::S0 + gp * St::SS + 7];
-        }
-        v[NT - 1] = fp;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = 0.0;
-  }
-  __syncthreads();  // staging area becomes the run-sum area
-  sB[lane] = j;
-#pragma unroll
-  for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
-  __syncthreads();
-  if (!live || (lane > 0 && sB[lane - 1] == j)) return;
+  if (lane > 0 && col[lane - 1] == j) return;
   double acc[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) acc[q] = v[q];
-  for (int t2 = lane + 1; t2 < 64 && sB[t2] == j; ++t2)
+  for (int t2 = lane + 1; t2 < 64 && col[t2] == j; ++t2)
 #pragma unroll
-    for (int q = 0; q < NT; ++q) acc[q] += sPart[t2 * NT + q];
+    for (int q = 0; q < NT; ++q) acc[q] += part[t2 * NT + q];
   const size_t blk = (size_t)F.nbptr[n] + j;
   int bt[BT];
 #pragma unroll
@@ -575,6 +549,178 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
       if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
     if ((masks >> 6) & 1) row[bt[2] >> 16] = acc[15];
   }
+}
+
+// Matrix kernel, one wavefront per 64 laid-out items of one node n, every item reading its element's
+// staging record from global memory.
+template <int NPE, int NGP, bool MIXED>
+__global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
+  using St = Stage<NPE, NGP, MIXED>;
+  constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
+  constexpr int NR = MIXED ? 4 : 3;   // right-hand-side rows of a node (self items)
+  constexpr int NT = NV + NR;
+  __shared__ int32_t sCol[64];
+  __shared__ double sPart[64 * NT];
+  const int lane = threadIdx.x;
+  const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
+  if (w >= F.nwaves) return;
+  const int2 it = F.fitem[w * 64 + lane];
+  const int n = F.wnode[w];
+  const bool live = it.x >= 0;
+  const int j = live ? (it.y >> 10) : -1;
+  double v[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) v[q] = 0.0;
+  if (live) {
+    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+    const double* rec = F.GS + (size_t)e * St::TS;
+    item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
+    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);  // the element's own item of n
+  }
+  sCol[lane] = j;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
+  __syncthreads();
+  if (live) run_heads<MIXED, NV, NT>(F, n, lane, j, v, sCol, sPart);
+}
+
+// Chunked matrix kernel: one workgroup per chunk of CW consecutive wavefronts (CW nodes in Morton
+// order for Hex8, an aligned 2x2x2 block of nodes on a structured grid, 27 elements).  The G, B, S
+// part of the chunk's distinct elements' staging records is copied to LDS with coalesced 16-byte
+// loads, and every item then reads LDS: each record is read from L2 once per chunk rather than once
+// per item lane, and the per-lane gathers that bound the per-wavefront kernel on address processing
+// become LDS reads.  A chunk with more than ECAP distinct elements reads the records directly.
+#ifndef MPF_CHUNK_WAVES
+#define MPF_CHUNK_WAVES 8
+#endif
+#ifndef MPF_CHUNK_ELEMS
+#define MPF_CHUNK_ELEMS 32
+#endif
+template <int NPE, int NGP, bool MIXED, int CW, int ECAP>
+__global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
+  using St = Stage<NPE, NGP, MIXED>;
+  constexpr int NV = MIXED ? 16 : 9;
+  constexpr int NR = MIXED ? 4 : 3;
+  constexpr int NT = NV + NR;
+// slot stride padded by 6 doubles: 286 doubles = 2288 bytes per slot, which spreads the slots' rows
+// over the LDS banks (measured per Hex8 assembly on the 128^3 block: 10.24 ms unpadded, 9.97 ms with
+// 2, 9.93 ms with 6, 9.98 ms with 10)
+#ifndef MPF_LDS_PAD
+#define MPF_LDS_PAD 6
+#endif
+  constexpr int SL2 = St::SL / 2;                 // 16-byte groups copied per element
+  constexpr int SLP = St::SL + MPF_LDS_PAD;       // LDS stride per element slot (doubles)
+  static_assert(SLP % 2 == 0, "16-byte slots");
+  constexpr int STG = ECAP * SLP, PART = CW * 64 * NT;
+  constexpr int LDSD = STG > PART ? STG : PART;  // record copies, then the run sums
+  static_assert(ECAP <= 255 && ECAP <= 64 * CW, "slot tables");
+  __shared__ double2 sMem[LDSD / 2];
+  __shared__ int32_t sCol[CW * 64];
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int64_t c = xcd_chunk(blockIdx.x, F.nchunks);
+  if (c >= F.nchunks) return;  // whole workgroup, before any barrier
+  const int cnt = F.ccnt[c];
+  const bool staged = cnt <= ECAP && cnt <= F.ecap;
+  if (staged) {
+    const int32_t* ct = F.ctab + c * ECAP;
+    const int tot = cnt * SL2;
+    // four 16-byte loads in flight per lane; indices past the end re-read the last group
+    auto ld = [&](int t) {
+      t = t < tot ? t : tot - 1;
+      const int k = t / SL2;
+      return reinterpret_cast<const double2*>(F.GS + (size_t)ct[k] * St::TS)[t - k * SL2];
+    };
+    auto dst = [&](int t) { return t + (t / SL2) * (SLP / 2 - SL2); };
+    constexpr int BS = 64 * CW;
+    for (int t0 = tid; t0 < tot; t0 += 4 * BS) {
+      const double2 r0 = ld(t0), r1 = ld(t0 + BS), r2 = ld(t0 + 2 * BS), r3 = ld(t0 + 3 * BS);
+      sMem[dst(t0)] = r0;
+      if (t0 + BS < tot) sMem[dst(t0 + BS)] = r1;
+      if (t0 + 2 * BS < tot) sMem[dst(t0 + 2 * BS)] = r2;
+      if (t0 + 3 * BS < tot) sMem[dst(t0 + 3 * BS)] = r3;
+    }
+  }
+  __syncthreads();
+  const int64_t w = c * CW + wid;
+  const bool wv = w < F.nwaves;
+  const int2 it = wv ? F.fitem[w * 64 + lane] : make_int2(-1, -1);
+  const int n = wv ? F.wnode[w] : 0;
+  const bool live = it.x >= 0;
+  const int j = live ? (it.y >> 10) : -1;
+  double v[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) v[q] = 0.0;
+  if (live) {
+    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+    const double* rec = F.GS + (size_t)e * St::TS;
+    if (staged)
+      item_block<NPE, NGP, MIXED, NV, true>(reinterpret_cast<const double*>(sMem) + F.fslot[w * 64 + lane] * SLP, a,
+                                            b, v);
+    else
+      item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
+    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
+  }
+  __syncthreads();  // the record copies become the run-sum area
+  double* const part = reinterpret_cast<double*>(sMem) + wid * 64 * NT;
+  sCol[tid] = j;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) part[lane * NT + q] = v[q];
+  __syncthreads();
+  if (live) run_heads<MIXED, NV, NT>(F, n, lane, j, v, sCol + wid * 64, part);
+}
+
+// Chunk tables (once per pattern), one workgroup per chunk of CW wavefronts: the chunk's distinct
+// element ids in ascending order (bitonic sort of its 64 CW items' elements in LDS, then a
+// compaction), their number, and each item's slot among them.
+template <int CW, int ECAP>
+__global__ void __launch_bounds__(64 * CW) k_chunk_table(int64_t nwaves, const int2* __restrict__ fitem,
+                                                         int32_t* __restrict__ ctab, int32_t* __restrict__ ccnt,
+                                                         uint8_t* __restrict__ fslot) {
+  constexpr int N = 64 * CW;
+  static_assert((N & (N - 1)) == 0, "power of two");
+  __shared__ int32_t key[N], uq[N], wsum[CW];
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int64_t c = blockIdx.x;
+  const int64_t i = c * N + tid;
+  const bool valid = i < nwaves * 64;
+  const int2 it = valid ? fitem[i] : make_int2(-1, -1);
+  const int32_t none = 0x7fffffff;
+  const int32_t e = it.x >= 0 ? (it.x >> 4) : none;
+  key[tid] = e;
+  __syncthreads();
+  for (int k = 2; k <= N; k <<= 1)
+    for (int h = k >> 1; h > 0; h >>= 1) {
+      const int p = tid ^ h;
+      if (p > tid) {
+        const int32_t x = key[tid], y = key[p];
+        if ((x > y) == ((tid & k) == 0)) { key[tid] = y; key[p] = x; }
+      }
+      __syncthreads();
+    }
+  const int32_t x = key[tid];
+  const bool u = x != none && (tid == 0 || key[tid - 1] != x);
+  const uint64_t bal = __ballot(u);
+  if (lane == 0) wsum[wid] = __popcll(bal);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int q = 0; q < CW; ++q) {
+    if (q < wid) off += wsum[q];
+    tot += wsum[q];
+  }
+  if (u) uq[off + __popcll(bal & ((1ull << lane) - 1ull))] = x;
+  __syncthreads();
+  int slot = 0;
+  if (e != none && tot <= ECAP) {
+    int lo = 0, hi = tot - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (uq[mid] < e) lo = mid + 1; else hi = mid;
+    }
+    slot = lo;
+  }
+  if (valid) fslot[i] = (uint8_t)slot;
+  if (tid < ECAP) ctab[c * ECAP + tid] = tid < tot ? uq[tid] : 0;
+  if (tid == 0) ccnt[c] = tot;
 }
 
 struct RhsArgs {
@@ -619,21 +765,11 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
   }
   for (int s = 0; s < m && !R.lift_only; ++s) {
     const int ea = R.nelA[e0 + s];
-    const int a = ea & 15;
-    const double* rec = R.GS + (size_t)(ea >> 4) * St::TS;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    item_residual<NPE, NGP, MIXED>(R.GS + (size_t)(ea >> 4) * St::TS, ea & 15, t);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) r[i] += rec[St::R0 + a * 3 + i];
-    if (MIXED && isPn && a < 4) {
-      double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
-#pragma unroll
-      for (int gp = 0; gp < NGP; ++gp) {
-        double Np[4];
-        tet_Np(gp, Np);
-        const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
-        fp -= (rec[St::S0 + gp * St::SS + 5] * Na) * rec[St::S0 + gp * St::SS + 7];
-      }
-      r[3] += fp;
-    }
+    for (int i = 0; i < 3; ++i) r[i] += t[i];
+    if (isPn) r[3] += t[3];
   }
   if (R.dispApplied && R.near[n]) {
     double L[4] = {0.0, 0.0, 0.0, 0.0};
@@ -649,7 +785,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) db[k] = ((mb >> k) & 1) ? 0.0 : R.dispApplied[3 * (size_t)vb + k];
         double Kb[9];
-        tangent_block<NGP, !MIXED>(rec + St::G0 + a * 6, rec + St::G0 + b * 6, NPE * 6, rec + St::S0, St::SS, Kb);
+        record_block<NPE, NGP, MIXED>(rec, a, b, Kb);
 #pragma unroll
         for (int i = 0; i < 3; ++i) L[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
         if (MIXED && isPn && a < 4) {
@@ -659,7 +795,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
             tet_Np(gp, Np);
             const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
             const double bb4 = rec[St::S0 + gp * St::SS + 4] * Na;
-            const double* gb = rec + St::G0 + gp * NPE * 6 + b * 6;
+            const double* gb = rec + St::G0 + (gp * NPE + b) * St::GW;
             pu[0] += bb4 * gb[0]; pu[1] += bb4 * gb[1]; pu[2] += bb4 * gb[2];
           }
           L[3] += pu[0] * db[0] + pu[1] * db[1] + pu[2] * db[2];
@@ -690,6 +826,10 @@ static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
   P.g_wnode.release();
   P.g_btab.release();
   P.g_near.release();
+  P.g_ctab.release();
+  P.g_ccnt.release();
+  P.g_fslot.release();
+  P.g_nchunks = 0;
   p->GS.release();
   p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;
   return MPF_OK;
@@ -721,11 +861,17 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
     MPF_TRY(code.alloc(nNode));
     MPF_TRY(code2.alloc(nNode));
     MPF_TRY(ids.alloc(nNode));
-    double sc[3];
-    for (int c = 0; c < 3; ++c) {
-      const double ext = p->xmax[c] - p->xmin[c];
-      sc[c] = ext > 0.0 ? 2097151.0 / ext : 0.0;
-    }
+    // positions quantised to a lattice of spacing h = (bounding-box volume / elements)^(1/3), about
+    // the element size: on a structured grid each node gets its own lattice point, so 8 consecutive
+    // nodes of the order form an aligned 2x2x2 block (the chunked matrix kernel's unit of reuse)
+    double ext[3], h = 0.0;
+    for (int c = 0; c < 3; ++c) ext[c] = p->xmax[c] - p->xmin[c];
+    const double emax = std::max(ext[0], std::max(ext[1], ext[2]));
+    if (ext[0] > 0.0 && ext[1] > 0.0 && ext[2] > 0.0) h = std::cbrt(ext[0] * ext[1] * ext[2] / (double)p->nElem);
+    if (!(h > 0.0)) h = emax / std::cbrt((double)p->nElem);
+    if (!(h > 0.0)) h = 1.0;
+    h = std::max(h, emax / 2097151.0);
+    const double sc[3] = {1.0 / h, 1.0 / h, 1.0 / h};
     hipLaunchKernelGGL(k_morton, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, p->X.ptr, p->xmin[0], p->xmin[1],
                        p->xmin[2], sc[0], sc[1], sc[2], code.ptr, ids.ptr);
     size_t tmp = 0;
@@ -792,6 +938,19 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
                        keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr, P.g_wnode.ptr);
   P.g_nwaves = total / 64;
 
+  // chunk tables of the chunked matrix kernel (Hex8)
+  if (!mixed && P.g_nwaves > 0) {
+    constexpr int CW = MPF_CHUNK_WAVES, ECAP = MPF_CHUNK_ELEMS;
+    static_assert(ECAP == 32, "mpf_set_chunk_stage_cap documents a cap of 32");
+    P.g_nchunks = (P.g_nwaves + CW - 1) / CW;
+    MPF_TRY(P.g_ctab.alloc((size_t)P.g_nchunks * ECAP));
+    MPF_TRY(P.g_ccnt.alloc((size_t)P.g_nchunks));
+    MPF_TRY(P.g_fslot.alloc((size_t)P.g_nwaves * 64));
+    hipLaunchKernelGGL((k_chunk_table<CW, ECAP>), dim3((unsigned)P.g_nchunks), dim3(64 * CW), 0, s, P.g_nwaves,
+                       P.g_fitem.ptr, P.g_ctab.ptr, P.g_ccnt.ptr, P.g_fslot.ptr);
+    MPF_HIP(hipGetLastError());
+  }
+
   // step 3: column-block table; step 4: near-Dirichlet flags
   MPF_TRY(P.g_btab.alloc((size_t)(nbtot > 0 ? nbtot : 1) * (mixed ? 4 : 2)));
   if (mixed)
@@ -825,17 +984,19 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   F.nwaves = P.g_nwaves;
   F.fitem = P.g_fitem.ptr;
   F.wnode = P.g_wnode.ptr;
-  F.nptr = P.nptr.ptr;
-  F.nelA = P.g_nelA.ptr;
   F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
-  F.stage = p->asm_mode != MPF_ASSEMBLY_GATHER_DIRECT;
   F.nda = P.nda.ptr;
   F.idp = P.idp.ptr;
   F.dispDOF = P.dispDOF;
   F.GS = p->GS.ptr;
   F.vals = p->vals.ptr;
   F.rhs = p->rhs.ptr;
+  F.nchunks = P.g_nchunks;
+  F.ecap = p->chunk_cap;
+  F.ctab = P.g_ctab.ptr;
+  F.ccnt = P.g_ccnt.ptr;
+  F.fslot = P.g_fslot.ptr;
   RhsArgs R{};
   R.nNode = nNode;
   R.nptr = P.nptr.ptr; R.nelA = P.g_nelA.ptr; R.near = P.g_near.ptr; R.conn = p->conn.ptr;
@@ -845,9 +1006,15 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   const bool flat = writeK && F.nwaves > 0;
   R.lift_only = flat ? 1 : 0;
   const unsigned flat_grid = (unsigned)(((F.nwaves + 7) / 8) * 8);
+  const unsigned chunk_grid = (unsigned)(((F.nchunks + 7) / 8) * 8);
+  const bool chunked = p->asm_mode != MPF_ASSEMBLY_GATHER_DIRECT && F.nchunks > 0;
   if (!mixed) {
     hipLaunchKernelGGL((k_stage<8, 8, false, 8>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A, nElem, p->GS.ptr);
-    if (flat) hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
+    if (flat && chunked)
+      hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS>), dim3(chunk_grid),
+                         dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
+    else if (flat)
+      hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<8, 8, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
   } else {
     hipLaunchKernelGGL((k_stage<10, 4, true, 16>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A, nElem, p->GS.ptr);

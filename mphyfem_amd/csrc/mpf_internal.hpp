@@ -135,6 +135,10 @@ struct PatternBufs {
   DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j)
   DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
   int64_t g_nwaves = 0;
+  mpf::DevBuf<int32_t> g_ctab;   // chunked matrix kernel: distinct elements per chunk of wavefronts
+  mpf::DevBuf<int32_t> g_ccnt;   // their number per chunk
+  mpf::DevBuf<uint8_t> g_fslot;  // each laid-out item's element slot in its chunk
+  int64_t g_nchunks = 0;
   bool gather_ready = false;
   std::vector<int32_t> color_off;  // host: [ncolors + 1] offsets into perm
   int32_t dispDOF = 0, presDOF = 0, ncolors = 0;
@@ -176,6 +180,7 @@ struct mpf_problem {
   mpf::KrylovWork* kw = nullptr;
   // assembly algorithm (MPF_ASSEMBLY_*) and the gather path's staging records
   int32_t asm_mode = MPF_ASSEMBLY_GATHER;
+  int32_t chunk_cap = 32;  // Hex8 chunked matrix kernel: most distinct elements staged in LDS
   mpf::DevBuf<double> GS;
   mpf::DevBuf<double> scratch;  // reductions (norms)
   mpf::BatchWork* batch = nullptr;  // independent meshes solved together (batch.hip)

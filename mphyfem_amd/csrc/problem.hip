@@ -620,6 +620,12 @@ mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
   return MPF_OK;
 }
 
+mpf_status mpf_set_chunk_stage_cap(mpf_problem* p, int32_t cap) {
+  if (!p || cap < 0 || cap > 32) return MPF_ERR_ARG;
+  p->chunk_cap = cap;
+  return MPF_OK;
+}
+
 mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode) {
   if (!p || !mode) return MPF_ERR_ARG;
   *mode = p->asm_mode;

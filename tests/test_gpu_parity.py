@@ -234,6 +234,38 @@ def test_global_assembly_tet_vs_oracle(mpf, orc, gpu, mode):
     assert P.assembly_mode() == mode
 
 
+@pytest.mark.parametrize("cap", [32, 20, 0], ids=["staged", "mixed", "direct_reads"])
+def test_hex_chunk_staging_vs_oracle(mpf, orc, gpu, cap):
+    """Hex8 chunked matrix kernel on a jittered box that is not a cube: chunks staged in LDS (cap 32),
+    some staged and some reading the records directly (cap 20), every chunk reading directly (0)."""
+    nx, ny, nz = 9, 6, 11
+    X, conn = mpf.mesh_hex(nx, ny, nz, L=(1.0, 0.7, 1.6), jitter=0.1, seed=4)
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
+    bcs = [(mpf.face_nodes(mpf.HEX8, nx, ny, nz, 0), -1, None), (mpf.face_nodes(mpf.HEX8, nx, ny, nz, 1), 0, 0.1)]
+    P = mpf.Problem(mpf.HEX8, X, conn, m)
+    O = orc.Problem(8, X, conn, m.mu, m.Kinv, m.Utype)
+    O.set_tangent(True)
+    u = random_state(len(X), 0.005, seed=23)
+    for Q in (P, O):
+        apply_bcs(Q, bcs)
+        Q.prepare_matrix_pattern()
+        Q.set_state(disp=u, disp_prev=u)
+        Q.set_boundary_conditions(0.6)
+    P.set_chunk_stage_cap(cap)
+    for first in (True, False):
+        P.calc_stiffness_and_residual(first, 0.6)
+        vals, rhs = P.system()
+        P.calc_stiffness_and_residual(first, 0.6)
+        v2, r2 = P.system()
+        assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
+        ov, orhs, st = O.assemble(first, 0.6)
+        assert st == 0
+        assert normwise_rel(vals, ov) < TOL_K
+        assert normwise_rel(rhs, orhs) < TOL_K
+    with pytest.raises(mpf.MpfError):
+        P.set_chunk_stage_cap(33)
+
+
 def test_gather_matches_scatter_bitwise_reproducible(mpf, gpu):
     """The gather path writes each CSR value once in a fixed order: repeated assemblies are
     bit-identical, and both algorithms agree to rounding."""
