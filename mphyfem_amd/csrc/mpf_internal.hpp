@@ -134,6 +134,8 @@ struct PatternBufs {
   DevBuf<int32_t> g_wnode;    // node of each wavefront of items
   DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j)
   DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
+  DevBuf<int32_t> g_nearlist; // the flagged nodes
+  int32_t g_nnear = 0;
   int64_t g_nwaves = 0;
   mpf::DevBuf<int32_t> g_ctab;   // chunked matrix kernel: distinct elements per chunk of wavefronts
   mpf::DevBuf<int32_t> g_ccnt;   // their number per chunk
