@@ -221,8 +221,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-krylov", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--assembly", choices=["gather", "scatter"], default="gather",
-                    help="row-owner gather assembly or coloured element scatter")
+    ap.add_argument("--assembly", choices=["gather", "direct", "scatter"], default="gather",
+                    help="row-owner gather assembly (default; Hex8 stages chunks of element records in LDS), "
+                         "the gather with every item reading the records directly, or coloured element scatter")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device 3 times (reference run for the HBM byte counters)")
     a = ap.parse_args(argv)
@@ -252,7 +253,8 @@ def main(argv=None) -> int:
     X, conn, P, _ = build_problem(M, n, wl)
     nElem = len(conn)
     t0 = time.perf_counter()
-    P.set_assembly_mode(M.ASSEMBLY_GATHER if a.assembly == "gather" else M.ASSEMBLY_COLORED_SCATTER)
+    P.set_assembly_mode({"gather": M.ASSEMBLY_GATHER, "direct": M.ASSEMBLY_GATHER_DIRECT,
+                         "scatter": M.ASSEMBLY_COLORED_SCATTER}[a.assembly])
     P.prepare_matrix_pattern()
     t_pattern = time.perf_counter() - t0
     dispDOF, presDOF, nnz, ncolors = P.sizes()
@@ -273,8 +275,8 @@ def main(argv=None) -> int:
     ms_all, ms_kern = P.time_assembly(max(5, a.steps), first_iter=True, lam=1.0)
     bpe = BYTES_PER_ELEM[wl]
     achieved = bpe * nElem / (ms_kern * 1e-3) / 1e9
-    if a.assembly == "gather":
-        kname = "k_stage + k_flat + k_rhs (row-owner gather, 3 launches"
+    if a.assembly != "scatter":
+        kname = "k_stage + k_flat + k_rhs + k_lift (row-owner gather, 4 launches"
     else:
         kname = ("tet10_kernel<4,false>" if wl == "tet10" else "hex8_kernel<8,false>") + f" ({ncolors} colour launches"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

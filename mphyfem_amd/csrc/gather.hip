@@ -198,42 +198,34 @@ __device__ __forceinline__ void wave_sort(int32_t* key, int n, int lane) {
     }
 }
 
-constexpr int kItemCap = 1024;  // kMaxAdj * NPE rounded up to a power of two (Tet10: 640 -> 1024)
-constexpr int kFlatCap = 2048;  // laid-out items of one node, padding included
+constexpr int kItemCap = 1024;   // kMaxAdj * NPE rounded up to a power of two (Tet10: 640 -> 1024)
+constexpr int kGroupNodes = 8;   // nodes per layout group (consecutive in Morton order)
 
-// Layout walk over a node's items sorted by (column j, element): a run of one column is never split
-// across a 64-item wavefront (runs are at most kMaxAdj = 64 long).  Calls emit(position, item) for
-// every item and returns the padded length (a multiple of 64).
-template <typename Emit>
-__device__ __forceinline__ int flat_walk(const int32_t* sK, int nitems, Emit emit) {
-  int pos = 0;
-  for (int t = 0; t < nitems; ++t) {
-    const int j = sK[t] >> 16;
-    if (t == 0 || j != (sK[t - 1] >> 16)) {
-      int L = 1;
-      while (t + L < nitems && (sK[t + L] >> 16) == j) ++L;
-      if ((pos & 63) + L > 64) pos = (pos + 63) & ~63;
-    }
-    emit(pos, t);
-    ++pos;
-  }
-  return (pos + 63) & ~63;
+// Block ownership.  K is symmetric (K_e(a, b) = K_e(b, a)^T for every element: the Neo-Hookean
+// tangent and the u-p couplings alike), so the pair of blocks (n, m), (m, n) is summed once, by its
+// owner, which writes K_nm into n's rows and its transpose into m's rows.  n owns (n, m) if m is n,
+// or m has no rows, or n < m.  Only nodes with rows own anything.
+template <bool MIXED>
+__device__ __forceinline__ bool has_rows(int info) {
+  return (info & 7) || (MIXED && ((info >> 3) & 1));
 }
 
 // Build step 1 (one wavefront per node n):
 //   nelA[nptr[n] + s]       = e * 16 + a: the elements around n in ascending id, a = local index of n in e
 //   keys[nptr[n] * NPE + t] = (s0 << 23) | (j << 16) | (s << 4) | b for every element s around n and
-//                             node b of it, sorted: j = position of node conn[e][b] in the sorted
-//                             coupled-node list of n (the column block of n's rows the (s, b) block
-//                             lands in), s0 = first element of column j's run
-//   flen[n]                 = laid-out length of n's items (0 for a node without rows)
+//                             node b of it such that n owns the block pair (n, conn[e][b]), sorted:
+//                             j = position of node conn[e][b] in the sorted coupled-node list of n (the
+//                             column block of n's rows the (s, b) block lands in), s0 = first element of
+//                             column j's run
+//   cnt[n]                  = number of those items (0 for a node without rows)
 template <int NPE>
 __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t* __restrict__ nptr,
                                                    const int32_t* __restrict__ nel, const int32_t* __restrict__ nbptr,
                                                    const int32_t* __restrict__ nb, const int32_t* __restrict__ conn,
                                                    const int32_t* __restrict__ nodeinfo, int32_t* __restrict__ nelA,
-                                                   int32_t* __restrict__ keys, int64_t* __restrict__ flen,
+                                                   int32_t* __restrict__ keys, int64_t* __restrict__ cnt,
                                                    int32_t* __restrict__ overflow) {
+  constexpr bool MIXED = NPE == 10;
   static_assert(kMaxAdj * NPE <= kItemCap, "item capacity");
   __shared__ int32_t sE[kMaxAdj], sNB[kMaxNB], sK[kItemCap];
   const int lane = threadIdx.x;
@@ -242,7 +234,7 @@ __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t*
   const int e0 = nptr[n], m = nptr[n + 1] - e0;
   const int b0 = nbptr[n], nnb = nbptr[n + 1] - b0;
   if (m > kMaxAdj || nnb > kMaxNB) {
-    if (lane == 0) { atomicOr(overflow, 1); flen[n] = 0; }
+    if (lane == 0) { atomicOr(overflow, 1); cnt[n] = 0; }
     return;
   }
   sE[lane] = lane < m ? nel[e0 + lane] : 0x7fffffff;
@@ -256,26 +248,31 @@ __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t*
       if (c[t] == n) a = t;
     nelA[e0 + lane] = sE[lane] * 16 + a;
   }
-  const int info = nodeinfo[n];
-  const bool rows = (info & 7) || ((info >> 3) & 1);
-  const int nitems = rows ? m * NPE : 0;
+  const bool rows = has_rows<MIXED>(nodeinfo[n]);
+  const int nall = rows ? m * NPE : 0;
   int P2 = 64;
-  while (P2 < nitems) P2 <<= 1;
+  while (P2 < nall) P2 <<= 1;
+  int own = 0;
   for (int t = lane; t < P2; t += 64) {
     int32_t key = 0x7fffffff;
-    if (t < nitems) {
+    if (t < nall) {
       const int s = t / NPE, b = t % NPE;
       const int vb = conn[(size_t)sE[s] * NPE + b];
-      int lo = 0, hi = nnb;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (sNB[mid] < vb) lo = mid + 1; else hi = mid;
+      if (vb == n || !has_rows<MIXED>(nodeinfo[vb]) || n < vb) {
+        int lo = 0, hi = nnb;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (sNB[mid] < vb) lo = mid + 1; else hi = mid;
+        }
+        if (lo >= nnb || sNB[lo] != vb) atomicOr(overflow, 2);  // not a coupled node: inconsistent tables
+        key = (min(lo, nnb - 1) << 16) | (s << 4) | b;
+        ++own;
       }
-      if (lo >= nnb || sNB[lo] != vb) atomicOr(overflow, 2);  // not a coupled node: inconsistent tables
-      key = (min(lo, nnb - 1) << 16) | (s << 4) | b;
     }
     sK[t] = key;
   }
+  for (int off = 32; off > 0; off >>= 1) own += __shfl_xor(own, off, 64);
+  const int nitems = own;
   __syncthreads();
   wave_sort(sK, P2, lane);
   // Runs (columns) are laid out in order of their first element, so a wavefront's items come from
@@ -290,64 +287,119 @@ __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t*
   __syncthreads();
   wave_sort(sK, P2, lane);
   for (int t = lane; t < nitems; t += 64) keys[(size_t)e0 * NPE + t] = sK[t];
-  if (lane == 0) {
-    const int len = flat_walk(sK, nitems, [](int, int) {});
-    if (len > kFlatCap) atomicOr(overflow, 4);
-    flen[n] = len;
+  if (lane == 0) cnt[n] = nitems;
+}
+
+// Layout walk over layout group g (nodes order[8g .. 8g+7]) and each node's sorted items: items are
+// placed one after the other, except that a run (one column block of one node) never crosses a
+// 64-item wavefront.  Calls emit(position, node, item index, key) for every item and returns the
+// group's length padded to a multiple of 64.
+template <int NPE, typename Emit>
+__device__ __forceinline__ int64_t group_walk(int64_t g, int32_t nNode, const int32_t* __restrict__ order,
+                                              const int32_t* __restrict__ nptr, const int32_t* __restrict__ keys,
+                                              const int64_t* __restrict__ cnt, Emit emit) {
+  int64_t pos = 0;
+  const int64_t q1 = (g + 1) * kGroupNodes < nNode ? (g + 1) * kGroupNodes : nNode;
+  for (int64_t q = g * kGroupNodes; q < q1; ++q) {
+    const int n = order[q];
+    const int c = (int)cnt[n];
+    const int32_t* K = keys + (size_t)nptr[n] * NPE;
+    int jprev = -1;
+    for (int t = 0; t < c; ++t) {
+      const int32_t key = K[t];
+      const int j = (key >> 16) & 127;
+      if (j != jprev) {
+        int L = 1;
+        while (t + L < c && ((K[t + L] >> 16) & 127) == j) ++L;
+        if ((pos & 63) + L > 64) pos = (pos + 63) & ~(int64_t)63;
+        jprev = j;
+      }
+      emit(pos, n, t, key);
+      ++pos;
+    }
+  }
+  return (pos + 63) & ~(int64_t)63;
+}
+
+// Build step 2a (one lane per layout group): padded length of each group.
+template <int NPE>
+__global__ void k_group_len(int64_t ngroups, int32_t nNode, const int32_t* __restrict__ order,
+                            const int32_t* __restrict__ nptr, const int32_t* __restrict__ keys,
+                            const int64_t* __restrict__ cnt, int64_t* __restrict__ glen) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  glen[g] = group_walk<NPE>(g, nNode, order, nptr, keys, cnt, [](int64_t, int, int, int32_t) {});
+}
+
+// Build step 2b (one lane per layout group): the laid-out items,
+//   fitem[goff[g] + q] = {e * 16 + a, (blk << 4) | b}  (element e around node n, a = n's local index
+//   in e, blk = nbptr[n] + j the column block of n's rows node b of e lands in),
+//   fnode[goff[g] + q] = n; padding slots keep {-1, -1} and -1.
+template <int NPE>
+__global__ void k_group_fill(int64_t ngroups, int32_t nNode, const int32_t* __restrict__ order,
+                             const int32_t* __restrict__ nptr, const int32_t* __restrict__ nbptr,
+                             const int32_t* __restrict__ keys, const int64_t* __restrict__ cnt,
+                             const int32_t* __restrict__ nelA, const int64_t* __restrict__ goff,
+                             int2* __restrict__ fitem, int32_t* __restrict__ fnode) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t base = goff[g];
+  group_walk<NPE>(g, nNode, order, nptr, keys, cnt, [&](int64_t pos, int n, int, int32_t key) {
+    const int blk = nbptr[n] + ((key >> 16) & 127);
+    fitem[base + pos] = make_int2(nelA[nptr[n] + ((key >> 4) & 63)], (blk << 4) | (key & 15));
+    fnode[base + pos] = n;
+  });
+}
+
+// Build step 2c (one lane per layout group): the group's wavefronts split into chunks of at most CW
+// (the chunked matrix kernel's workgroups); with count, only the number of chunks.
+__global__ void k_group_chunks(int64_t ngroups, int32_t CW, const int64_t* __restrict__ glen,
+                               const int64_t* __restrict__ goff, const int64_t* __restrict__ coff,
+                               int64_t* __restrict__ ccount, int32_t* __restrict__ cwave, int32_t* __restrict__ cnw) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t waves = glen[g] / 64;
+  const int64_t nch = (waves + CW - 1) / CW;
+  if (ccount) { ccount[g] = nch; return; }
+  for (int64_t k = 0; k < nch; ++k) {
+    cwave[coff[g] + k] = (int32_t)(goff[g] / 64 + k * CW);
+    cnw[coff[g] + k] = (int32_t)(waves - k * CW < CW ? waves - k * CW : CW);
   }
 }
 
-__global__ void k_gather_len(int32_t n, const int32_t* __restrict__ order, const int64_t* __restrict__ flen,
-                             int64_t* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = flen[order[i]];
-}
-
-__global__ void k_scatter_off(int32_t n, const int32_t* __restrict__ order, const int64_t* __restrict__ in,
-                              int64_t* __restrict__ foff) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) foff[order[i]] = in[i];
-}
-
-// Build step 2 (one wavefront per node): the node's laid-out items,
-//   fitem[foff[n] + q] = {e * 16 + a, (j << 10) | (s << 4) | b}  (column block j of n's rows, element s
-//   around n, node b of it) and {-1, -1} in padding slots; wnode[w] = n for each of n's wavefronts.
-template <int NPE>
-__global__ void __launch_bounds__(64) k_flat_fill(int32_t nNode, const int32_t* __restrict__ nptr,
-                                                  const int32_t* __restrict__ nbptr, const int32_t* __restrict__ nelA,
-                                                  const int32_t* __restrict__ keys, const int64_t* __restrict__ flen,
-                                                  const int64_t* __restrict__ foff, int2* __restrict__ fitem,
-                                                  int32_t* __restrict__ wnode) {
-  __shared__ int32_t sK[kItemCap], sMap[kFlatCap];
-  const int lane = threadIdx.x;
+// Build step 3b (one wavefront per node n): for each column block (n, j) with coupled node m, the
+// entry of block (m, n) of m's rows, which receives its transpose when n owns the pair, is copied
+// next to (n, j)'s own entry, so that a run's head reads both with one load; {-1, 0, ..} when the
+// pair has no mirror (m is n, or m has no rows).  btab holds 2 BT ints per block.
+template <bool MIXED>
+__global__ void __launch_bounds__(64) k_mirror(int32_t nNode, const int32_t* __restrict__ nbptr,
+                                               const int32_t* __restrict__ nb, const int32_t* __restrict__ nodeinfo,
+                                               int32_t* __restrict__ btab) {
+  constexpr int BT = MIXED ? 4 : 2;
   const int n = blockIdx.x;
   if (n >= nNode) return;
-  const int len = (int)flen[n];
-  if (len == 0) return;
-  const int e0 = nptr[n], m = nptr[n + 1] - e0;
-  const int nitems = m * NPE;
-  for (int t = lane; t < nitems; t += 64) sK[t] = keys[(size_t)e0 * NPE + t];
-  for (int q = lane; q < len; q += 64) sMap[q] = -1;
-  __syncthreads();
-  if (lane == 0) flat_walk(sK, nitems, [&](int pos, int t) { sMap[pos] = t; });
-  __syncthreads();
-  int2* out = fitem + foff[n];
-  for (int q = lane; q < len; q += 64) {
-    const int t = sMap[q];
-    int2 it = make_int2(-1, -1);
-    if (t >= 0) {
-      const int key = sK[t];
-      it.x = nelA[e0 + ((key >> 4) & 63)];
-      it.y = (((key >> 16) & 127) << 10) | (key & 0x3ff);
+  const int b0 = nbptr[n], nnb = nbptr[n + 1] - b0;
+  for (int j = threadIdx.x; j < nnb; j += 64) {
+    const int mnode = nb[b0 + j];
+    int out = -1;
+    if (mnode != n && has_rows<MIXED>(nodeinfo[mnode])) {
+      int lo = nbptr[mnode], hi = nbptr[mnode + 1];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (nb[mid] < n) lo = mid + 1; else hi = mid;
+      }
+      out = (lo < nbptr[mnode + 1] && nb[lo] == n) ? lo : -1;
     }
-    out[q] = it;
+    int32_t* dst = btab + 2 * BT * (size_t)(b0 + j) + BT;
+#pragma unroll
+    for (int q = 0; q < BT; ++q) dst[q] = out >= 0 ? btab[2 * BT * (size_t)out + q] : (q == 0 ? -1 : 0);
   }
-  for (int q = lane * 64; q < len; q += 64 * 64) wnode[(foff[n] + q) >> 6] = n;
 }
 
-// Build step 3 (one wavefront per node n): where each column block (n, j) of n's rows lives in the CSR.
-//   Hex8:      {A, (La << 9) | flags}
-//   Tet10/P1:  {A, Bp, (D << 16) | La, flags}
+// Build step 3 (one wavefront per node n): where each column block (n, j) of n's rows lives in the CSR,
+// the first BT of the block's 2 BT ints (the mirror's entry follows, step 3b).
+//   Hex8:      {A, (La << 9) | flags}                         (BT = 2)
+//   Tet10/P1:  {A, Bp, (D << 16) | La, flags}                 (BT = 4)
 //   flags = (self << 8) | (isPn << 7) | (isPb << 6) | (mn << 3) | mb
 // A = start of n's first free u row + column offset of j's u dofs (-1 if n has no free u dof),
 // Bp = the same in n's pressure row (-1 if n carries no pressure), D = offset of j's pressure column
@@ -398,13 +450,13 @@ __global__ void __launch_bounds__(64) k_block_table(int32_t nNode, const int32_t
       const int self = nb[b0 + j] == n;
       if (MIXED) {
         const int isPb = (inf >> 3) & 1;
-        btab[4 * blk] = RU >= 0 ? RU + upos : -1;
-        btab[4 * blk + 1] = RP >= 0 ? RP + upos : -1;
-        btab[4 * blk + 2] = ((Ucols + ppos - upos) << 16) | La;
-        btab[4 * blk + 3] = (self << 8) | ((int)isPn << 7) | (isPb << 6) | (mn << 3) | mb;
+        btab[8 * blk] = RU >= 0 ? RU + upos : -1;
+        btab[8 * blk + 1] = RP >= 0 ? RP + upos : -1;
+        btab[8 * blk + 2] = ((Ucols + ppos - upos) << 16) | La;
+        btab[8 * blk + 3] = (self << 8) | ((int)isPn << 7) | (isPb << 6) | (mn << 3) | mb;
       } else {
-        btab[2 * blk] = RU >= 0 ? RU + upos : -1;
-        btab[2 * blk + 1] = (La << 9) | (self << 8) | (mn << 3) | mb;
+        btab[4 * blk] = RU >= 0 ? RU + upos : -1;
+        btab[4 * blk + 1] = (La << 9) | (self << 8) | (mn << 3) | mb;
       }
     }
     cu0 += __shfl(iu, 63, 64);
@@ -429,27 +481,24 @@ __global__ void k_near(int32_t nNode, const int32_t* __restrict__ nptr, const in
   near[n] = f;
 }
 
-// One (element, node b) item of node n's rows: the 3x3 block K_{n b} (v[0..8]) and, for the mixed
-// element, Kup(n, b) (v[9..11]), Kpu(n, b) (v[12..14]) and Kpp(n, b) (v[15]) from the first SL
-// doubles of the element's staging record at rec (in global memory or copied to LDS; LDS: Gauss
-// points one at a time, since LDS latency needs no unrolling to hide and registers are the limit).
-template <int NPE, int NGP, bool MIXED, int NV, bool IN_LDS = false>
-__device__ __forceinline__ void item_block(const double* rec, int a, int b, double v[NV]) {
+// Gauss points gp0 .. gp0 + NG - 1 of one (element, node b) item of node n's rows, added to v: the
+// 3x3 block K_{n b} (v[0..8]) and, for the mixed element, Kup(n, b) (v[9..11]), Kpu(n, b)
+// (v[12..14]) and Kpp(n, b) (v[15]).  G, Bv, S point at the first of those Gauss points' g data
+// (stride NPE * GW), b components (stride 6; compact records) and scalars (stride SS), in a staging
+// record in global memory or in a copy in LDS.
+template <int NPE, int NGP, bool MIXED, int NV, int NG, int UNROLL>
+__device__ __forceinline__ void item_gps(const double* G, const double* Bv, const double* S0p, int gp0, int a, int b,
+                                         double v[NV]) {
   using St = Stage<NPE, NGP, MIXED>;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = 0.0;
-#ifndef MPF_LDS_UNROLL
-#define MPF_LDS_UNROLL 1
-#endif
-#pragma unroll(IN_LDS ? MPF_LDS_UNROLL : 2)
-  for (int gp = 0; gp < NGP; ++gp) {
-    const double* Gp = rec + St::G0 + gp * NPE * St::GW;
-    const double* S = rec + St::S0 + gp * St::SS;
+#pragma unroll UNROLL
+  for (int gp = 0; gp < NG; ++gp) {
+    const double* Gp = G + gp * NPE * St::GW;
+    const double* S = S0p + gp * St::SS;
     double ga[3], ba[3], gb[3], bb[3];
     if (St::COMPACT) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
-      const double2* B2 = reinterpret_cast<const double2*>(rec + St::B0 + gp * 6);
+      const double2* B2 = reinterpret_cast<const double2*>(Bv + gp * 6);
       const double2 c0 = B2[0], c1 = B2[1], c2 = B2[2];
       const double B6[6] = {c0.x, c0.y, c1.x, c1.y, c2.x, c2.y};
       bsym_times(B6, ga, ba);
@@ -465,7 +514,7 @@ __device__ __forceinline__ void item_block(const double* rec, int a, int b, doub
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], v);
     if (MIXED) {
       double Np[4];
-      tet_Np(gp, Np);
+      tet_Np(gp0 + gp, Np);
       const double dv = S[4], dv0 = S[5], th = S[6];
       const double Nb = b == 0 ? Np[0] : b == 1 ? Np[1] : b == 2 ? Np[2] : Np[3];
       const double Na = a == 0 ? Np[0] : a == 1 ? Np[1] : a == 2 ? Np[2] : Np[3];
@@ -479,6 +528,16 @@ __device__ __forceinline__ void item_block(const double* rec, int a, int b, doub
       }
     }
   }
+}
+
+// One (element, node b) item of node n's rows over all Gauss points, from the element's staging
+// record at rec, UNROLL Gauss points at a time.
+template <int NPE, int NGP, bool MIXED, int NV, int UNROLL = 2>
+__device__ __forceinline__ void item_block(const double* rec, int a, int b, double v[NV]) {
+  using St = Stage<NPE, NGP, MIXED>;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  item_gps<NPE, NGP, MIXED, NV, NGP, UNROLL>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
 }
 
 // Node a's residual rows of one element (v[0..2]) and, for the mixed element, its pressure row (v[3])
@@ -503,74 +562,116 @@ __device__ __forceinline__ void item_residual(const double* rec, int a, double* 
 struct FlatArgs {
   int64_t nwaves;
   const int2* fitem;        // {e * 16 + a, (j << 10) | (s << 4) | b}, {-1, -1} in padding slots
-  const int32_t* wnode;     // node whose rows each wavefront's items belong to
+  const int32_t* fnode;     // node whose rows each laid-out item belongs to (-1 in padding slots)
   const int32_t* nbptr;
-  const int32_t* btab;
+  const int32_t* btab;      // per column block: its CSR location, then its mirror's (2 BT ints)
   const int32_t* nda;
   const int32_t* idp;
   int32_t dispDOF;
   const double* GS;
   double* vals;
   double* rhs;
-  // chunks of CW wavefronts (chunked matrix kernel): distinct elements of each chunk's items in
-  // ascending order (ECAP slots per chunk), their number, and each item's slot
+  // chunks of at most CW wavefronts of one layout group (chunked matrix kernel): first wavefront and
+  // count, distinct elements of the chunk's items in ascending order (ECAP slots per chunk), their
+  // number, and each item's slot
   int64_t nchunks;
   int32_t ecap;             // chunks with at most ecap (<= ECAP) distinct elements are staged
+  const int32_t* cwave;
+  const int32_t* cnw;
   const int32_t* ctab;
   const int32_t* ccnt;
   const uint8_t* fslot;
 };
 
-// Column runs of one wavefront's items, after every item lane has stored its values at part[t * NT]
-// and its column at col[t]: the first lane of each run (its head) sums the run in lane order, which
-// is element order, and writes the column block of node n's rows; the self column's head also
-// writes n's right-hand-side rows.  Every CSR value is written exactly once.
-template <bool MIXED, int NV, int NT>
-__device__ __forceinline__ void run_heads(const FlatArgs& F, int n, int lane, int j, const double v[NT],
-                                          const int32_t* col, const double* part) {
+// Writes a summed column block: K_nm (acc) into node n's rows at block blk and, through the mirror
+// table, its transpose into m's rows.  For the mixed element, acc[9..11] = Kup(n, m) (n's u rows,
+// m's pressure column), acc[12..14] = Kpu(n, m), acc[15] = Kpp(n, m); m's rows receive
+// Kup(m, n) = Kpu(n, m)^T, Kpu(m, n) = Kup(n, m)^T, Kpp(m, n) = Kpp(n, m).
+template <bool MIXED>
+__device__ __forceinline__ void write_block(const FlatArgs& F, size_t blk, const double* acc) {
   constexpr int BT = MIXED ? 4 : 2;
-  if (lane > 0 && col[lane - 1] == j) return;
-  double acc[NT];
+  double* __restrict__ V = F.vals;
+  int bt[2 * BT];
+  if (MIXED) {
+    const int4* p = reinterpret_cast<const int4*>(F.btab) + 2 * blk;
+    const int4 x = p[0], y = p[1];
+    bt[0] = x.x; bt[1] = x.y; bt[2] = x.z; bt[3] = x.w; bt[4] = y.x; bt[5] = y.y; bt[6] = y.z; bt[7] = y.w;
+  } else {
+    const int4 x = reinterpret_cast<const int4*>(F.btab)[blk];
+    bt[0] = x.x; bt[1] = x.y; bt[2] = x.z; bt[3] = x.w;
+  }
+  {
+    const int masks = MIXED ? bt[3] : (bt[1] & 0x1ff);
+    const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 9);
+    const int mn = (masks >> 3) & 7, mb = masks & 7;
 #pragma unroll
-  for (int q = 0; q < NT; ++q) acc[q] = v[q];
-  for (int t2 = lane + 1; t2 < 64 && col[t2] == j; ++t2)
+    for (int i = 0; i < 3; ++i) {
+      if (!((mn >> i) & 1)) continue;
+      double* row = V + bt[0] + rank3(mn, i) * La;
 #pragma unroll
-    for (int q = 0; q < NT; ++q) acc[q] += part[t2 * NT + q];
-  const size_t blk = (size_t)F.nbptr[n] + j;
-  int bt[BT];
+      for (int k = 0; k < 3; ++k)
+        if ((mb >> k) & 1) row[rank3(mb, k)] = acc[3 * i + k];
+      if (MIXED && ((masks >> 6) & 1)) row[bt[2] >> 16] = acc[9 + i];
+    }
+    if (MIXED && ((masks >> 7) & 1)) {
+      double* row = V + bt[1];
 #pragma unroll
-  for (int q = 0; q < BT; ++q) bt[q] = F.btab[BT * blk + q];
-  const int A = bt[0];
-  const int masks = MIXED ? bt[3] : (bt[1] & 0x1ff);
-  const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 9);
-  const int mn = (masks >> 3) & 7, mb = masks & 7;
-  if ((masks >> 8) & 1) {  // self column: n's right-hand side, its elements' residuals in element order
+      for (int k = 0; k < 3; ++k)
+        if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
+      if ((masks >> 6) & 1) row[bt[2] >> 16] = acc[15];
+    }
+  }
+  const int* mt = bt + BT;  // the mirror block (m, n): rows of m, columns of n (all masks 0 if none)
+  const int masks = MIXED ? mt[3] : (mt[1] & 0x1ff);
+  const int La = MIXED ? (mt[2] & 0xffff) : (mt[1] >> 9);
+  const int mm = (masks >> 3) & 7, mnn = masks & 7;  // free masks of m (rows) and n (columns)
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (!((mm >> k) & 1)) continue;
+    double* row = V + mt[0] + rank3(mm, k) * La;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-      if ((mn >> i) & 1) F.rhs[F.nda[3 * (size_t)n + i]] = acc[NV + i];
-    if (MIXED && ((masks >> 7) & 1)) F.rhs[F.dispDOF + F.idp[n]] = acc[NT - 1];
-  }
-  double* __restrict__ V = F.vals;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if (!((mn >> i) & 1)) continue;
-    double* row = V + A + rank3(mn, i) * La;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if ((mb >> k) & 1) row[rank3(mb, k)] = acc[3 * i + k];
-    if (MIXED && ((masks >> 6) & 1)) row[bt[2] >> 16] = acc[9 + i];
+      if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[3 * i + k];
+    if (MIXED && ((masks >> 6) & 1)) row[mt[2] >> 16] = acc[12 + k];
   }
   if (MIXED && ((masks >> 7) & 1)) {
-    double* row = V + bt[1];
+    double* row = V + mt[1];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
-    if ((masks >> 6) & 1) row[bt[2] >> 16] = acc[15];
+    for (int i = 0; i < 3; ++i)
+      if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[9 + i];
+    if ((masks >> 6) & 1) row[mt[2] >> 16] = acc[15];
   }
 }
 
-// Matrix kernel, one wavefront per 64 laid-out items of one node n, every item reading its element's
-// staging record from global memory.
+// Column runs of one wavefront's items, after every item lane has stored its values at part[t * NT]
+// and its block index (nbptr[n] + j, unique per node and column) at col[t]: the first lane of each
+// run (its head) sums the run in lane order, which is element order, and writes the block and its
+// mirror; the self column's head also writes n's right-hand-side rows.  Every CSR value is written
+// exactly once.
+template <bool MIXED, int NV, int NT>
+__device__ __forceinline__ void run_heads(const FlatArgs& F, int64_t item, int lane, int blk, const double v[NT],
+                                          const int32_t* col, const double* part) {
+  if (lane > 0 && col[lane - 1] == blk) return;
+  double acc[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) acc[q] = v[q];
+  for (int t2 = lane + 1; t2 < 64 && col[t2] == blk; ++t2)
+#pragma unroll
+    for (int q = 0; q < NT; ++q) acc[q] += part[t2 * NT + q];
+  const int flags = MIXED ? F.btab[8 * (size_t)blk + 3] : (F.btab[4 * (size_t)blk + 1] & 0x1ff);
+  if ((flags >> 8) & 1) {  // self column: n's right-hand side, its elements' residuals in element order
+    const int n = F.fnode[item];
+    const int mn = (flags >> 3) & 7;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if ((mn >> i) & 1) F.rhs[F.nda[3 * (size_t)n + i]] = acc[NV + i];
+    if (MIXED && ((flags >> 7) & 1)) F.rhs[F.dispDOF + F.idp[n]] = acc[NT - 1];
+  }
+  write_block<MIXED>(F, (size_t)blk, acc);
+}
+
+// Matrix kernel, one wavefront per 64 laid-out items, every item reading its element's staging record
+// from global memory.
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
@@ -583,9 +684,8 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
   if (w >= F.nwaves) return;
   const int2 it = F.fitem[w * 64 + lane];
-  const int n = F.wnode[w];
   const bool live = it.x >= 0;
-  const int j = live ? (it.y >> 10) : -1;
+  const int blk = live ? (it.y >> 4) : -1;
   double v[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) v[q] = 0.0;
@@ -595,43 +695,49 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
     item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
     if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);  // the element's own item of n
   }
-  sCol[lane] = j;
+  sCol[lane] = blk;
 #pragma unroll
   for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, n, lane, j, v, sCol, sPart);
+  if (live) run_heads<MIXED, NV, NT>(F, w * 64 + lane, lane, blk, v, sCol, sPart);
 }
 
-// Chunked matrix kernel: one workgroup per chunk of CW consecutive wavefronts (CW nodes in Morton
-// order for Hex8, an aligned 2x2x2 block of nodes on a structured grid, 27 elements).  The G, B, S
-// part of the chunk's distinct elements' staging records is copied to LDS with coalesced 16-byte
-// loads, and every item then reads LDS: each record is read from L2 once per chunk rather than once
-// per item lane, and the per-lane gathers that bound the per-wavefront kernel on address processing
-// become LDS reads.  A chunk with more than ECAP distinct elements reads the records directly.
+// Chunked matrix kernel: one workgroup per chunk (the wavefronts of one layout group of 8 nodes in
+// Morton order, an aligned 2x2x2 block of nodes on a structured grid: 27 elements).  The chunk's
+// distinct elements' Gauss-point data (g, b, scalars) is copied to LDS with coalesced 16-byte loads,
+// in NPH phases of NGP / NPH Gauss points each (so a workgroup holds 1/NPH of it and more
+// workgroups fit on a CU), and every item then reads LDS, keeping its partial block in registers
+// between phases: each record is read from L2 once per chunk rather than once per item lane, and
+// the per-lane gathers that bound the per-wavefront kernel on address processing become LDS reads.
+// A chunk with more than ECAP distinct elements reads the records directly.
 #ifndef MPF_CHUNK_WAVES
 #define MPF_CHUNK_WAVES 8
 #endif
 #ifndef MPF_CHUNK_ELEMS
 #define MPF_CHUNK_ELEMS 32
 #endif
-template <int NPE, int NGP, bool MIXED, int CW, int ECAP>
+#ifndef MPF_CHUNK_PHASES
+#define MPF_CHUNK_PHASES 1
+#endif
+#ifndef MPF_LDS_PAD
+#define MPF_LDS_PAD 6
+#endif
+template <int NPE, int NGP, bool MIXED, int CW, int ECAP, int NPH>
 __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
   constexpr int NV = MIXED ? 16 : 9;
   constexpr int NR = MIXED ? 4 : 3;
   constexpr int NT = NV + NR;
-// slot stride padded by 6 doubles: 286 doubles = 2288 bytes per slot, which spreads the slots' rows
-// over the LDS banks (measured per Hex8 assembly on the 128^3 block: 10.24 ms unpadded, 9.97 ms with
-// 2, 9.93 ms with 6, 9.98 ms with 10)
-#ifndef MPF_LDS_PAD
-#define MPF_LDS_PAD 6
-#endif
-  constexpr int SL2 = St::SL / 2;                 // 16-byte groups copied per element
-  constexpr int SLP = St::SL + MPF_LDS_PAD;       // LDS stride per element slot (doubles)
+  static_assert(NGP % NPH == 0, "whole Gauss points per phase");
+  constexpr int NGp = NGP / NPH;
+  constexpr int GL = NGp * NPE * St::GW, BL = St::COMPACT ? NGp * 6 : 0, SN = NGp * St::SS;  // per phase
+  static_assert(GL % 2 == 0 && BL % 2 == 0 && SN % 2 == 0, "16-byte segments");
+  constexpr int SLq = (GL + BL + SN) / 2;         // 16-byte groups per element and phase
+  constexpr int SLP = GL + BL + SN + MPF_LDS_PAD; // LDS stride per element slot (doubles), padded
   static_assert(SLP % 2 == 0, "16-byte slots");
   constexpr int STG = ECAP * SLP, PART = CW * 64 * NT;
   constexpr int LDSD = STG > PART ? STG : PART;  // record copies, then the run sums
-  static_assert(ECAP <= 255 && ECAP <= 64 * CW, "slot tables");
+  static_assert(ECAP <= 255, "slot tables");
   __shared__ double2 sMem[LDSD / 2];
   __shared__ int32_t sCol[CW * 64];
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
@@ -639,68 +745,76 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   if (c >= F.nchunks) return;  // whole workgroup, before any barrier
   const int cnt = F.ccnt[c];
   const bool staged = cnt <= ECAP && cnt <= F.ecap;
-  if (staged) {
-    const int32_t* ct = F.ctab + c * ECAP;
-    const int tot = cnt * SL2;
-    // four 16-byte loads in flight per lane; indices past the end re-read the last group
-    auto ld = [&](int t) {
-      t = t < tot ? t : tot - 1;
-      const int k = t / SL2;
-      return reinterpret_cast<const double2*>(F.GS + (size_t)ct[k] * St::TS)[t - k * SL2];
-    };
-    auto dst = [&](int t) { return t + (t / SL2) * (SLP / 2 - SL2); };
-    constexpr int BS = 64 * CW;
-    for (int t0 = tid; t0 < tot; t0 += 4 * BS) {
-      const double2 r0 = ld(t0), r1 = ld(t0 + BS), r2 = ld(t0 + 2 * BS), r3 = ld(t0 + 3 * BS);
-      sMem[dst(t0)] = r0;
-      if (t0 + BS < tot) sMem[dst(t0 + BS)] = r1;
-      if (t0 + 2 * BS < tot) sMem[dst(t0 + 2 * BS)] = r2;
-      if (t0 + 3 * BS < tot) sMem[dst(t0 + 3 * BS)] = r3;
-    }
-  }
-  __syncthreads();
-  const int64_t w = c * CW + wid;
-  const bool wv = w < F.nwaves;
+  const int64_t w = (int64_t)F.cwave[c] + wid;
+  const bool wv = wid < F.cnw[c];
   const int2 it = wv ? F.fitem[w * 64 + lane] : make_int2(-1, -1);
-  const int n = wv ? F.wnode[w] : 0;
   const bool live = it.x >= 0;
-  const int j = live ? (it.y >> 10) : -1;
+  const int blk = live ? (it.y >> 4) : -1;
+  const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+  const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
+  const int slot = live && staged ? F.fslot[w * 64 + lane] : 0;
   double v[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) v[q] = 0.0;
-  if (live) {
-    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-    const double* rec = F.GS + (size_t)e * St::TS;
-    if (staged)
-      item_block<NPE, NGP, MIXED, NV, true>(reinterpret_cast<const double*>(sMem) + F.fslot[w * 64 + lane] * SLP, a,
-                                            b, v);
-    else
-      item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
-    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
+  const int32_t* ct = F.ctab + c * ECAP;
+  const int tot = cnt * SLq;
+  for (int ph = 0; ph < NPH; ++ph) {
+    if (staged) {
+      if (ph > 0) __syncthreads();  // every item is done with the previous phase
+      // element k's phase: its G, B, S segments; four 16-byte loads in flight per lane, indices past
+      // the end re-read the last group
+      auto ld = [&](int t) {
+        t = t < tot ? t : tot - 1;
+        const int k = t / SLq, r = t - k * SLq;
+        const double* src = F.GS + (size_t)ct[k] * St::TS;
+        const int off = r < GL / 2 ? St::G0 + ph * GL + 2 * r
+                        : r < (GL + BL) / 2 ? St::B0 + ph * BL + 2 * r - GL
+                                            : St::S0 + ph * SN + 2 * r - GL - BL;
+        return *reinterpret_cast<const double2*>(src + off);
+      };
+      auto dst = [&](int t) { return t + (t / SLq) * (SLP / 2 - SLq); };
+      constexpr int BS = 64 * CW;
+      for (int t0 = tid; t0 < tot; t0 += 4 * BS) {
+        const double2 r0 = ld(t0), r1 = ld(t0 + BS), r2 = ld(t0 + 2 * BS), r3 = ld(t0 + 3 * BS);
+        sMem[dst(t0)] = r0;
+        if (t0 + BS < tot) sMem[dst(t0 + BS)] = r1;
+        if (t0 + 2 * BS < tot) sMem[dst(t0 + 2 * BS)] = r2;
+        if (t0 + 3 * BS < tot) sMem[dst(t0 + 3 * BS)] = r3;
+      }
+      __syncthreads();
+      // one Gauss point at a time: LDS latency needs no unrolling to hide, and registers are the limit
+      const double* sl = reinterpret_cast<const double*>(sMem) + slot * SLP;
+      if (live) item_gps<NPE, NGP, MIXED, NV, NGp, 1>(sl, sl + GL, sl + GL + BL, ph * NGp, a, b, v);
+    } else if (live) {
+      item_gps<NPE, NGP, MIXED, NV, NGp, 1>(rec + St::G0 + ph * GL, rec + St::B0 + ph * BL, rec + St::S0 + ph * SN,
+                                            ph * NGp, a, b, v);
+    }
   }
+  if (live && b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   __syncthreads();  // the record copies become the run-sum area
   double* const part = reinterpret_cast<double*>(sMem) + wid * 64 * NT;
-  sCol[tid] = j;
+  sCol[tid] = blk;
 #pragma unroll
   for (int q = 0; q < NT; ++q) part[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, n, lane, j, v, sCol + wid * 64, part);
+  if (live) run_heads<MIXED, NV, NT>(F, w * 64 + lane, lane, blk, v, sCol + wid * 64, part);
 }
 
-// Chunk tables (once per pattern), one workgroup per chunk of CW wavefronts: the chunk's distinct
-// element ids in ascending order (bitonic sort of its 64 CW items' elements in LDS, then a
+// Chunk tables (once per pattern), one workgroup of 512 lanes per chunk of at most 8 wavefronts: the
+// chunk's distinct element ids in ascending order (bitonic sort of its items' elements in LDS, then a
 // compaction), their number, and each item's slot among them.
-template <int CW, int ECAP>
-__global__ void __launch_bounds__(64 * CW) k_chunk_table(int64_t nwaves, const int2* __restrict__ fitem,
-                                                         int32_t* __restrict__ ctab, int32_t* __restrict__ ccnt,
-                                                         uint8_t* __restrict__ fslot) {
-  constexpr int N = 64 * CW;
-  static_assert((N & (N - 1)) == 0, "power of two");
-  __shared__ int32_t key[N], uq[N], wsum[CW];
+template <int ECAP>
+__global__ void __launch_bounds__(512) k_chunk_table(int64_t nchunks, const int32_t* __restrict__ cwave,
+                                                     const int32_t* __restrict__ cnw, const int2* __restrict__ fitem,
+                                                     int32_t* __restrict__ ctab, int32_t* __restrict__ ccnt,
+                                                     uint8_t* __restrict__ fslot) {
+  constexpr int N = 512, NW = N / 64;
+  __shared__ int32_t key[N], uq[N], wsum[NW];
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int64_t c = blockIdx.x;
-  const int64_t i = c * N + tid;
-  const bool valid = i < nwaves * 64;
+  if (c >= nchunks) return;
+  const bool valid = wid < cnw[c];
+  const int64_t i = ((int64_t)cwave[c] + wid) * 64 + lane;
   const int2 it = valid ? fitem[i] : make_int2(-1, -1);
   const int32_t none = 0x7fffffff;
   const int32_t e = it.x >= 0 ? (it.x >> 4) : none;
@@ -721,7 +835,7 @@ __global__ void __launch_bounds__(64 * CW) k_chunk_table(int64_t nwaves, const i
   if (lane == 0) wsum[wid] = __popcll(bal);
   __syncthreads();
   int off = 0, tot = 0;
-  for (int q = 0; q < CW; ++q) {
+  for (int q = 0; q < NW; ++q) {
     if (q < wid) off += wsum[q];
     tot += wsum[q];
   }
@@ -865,7 +979,9 @@ static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
   P.g_order.release();
   P.g_nelA.release();
   P.g_fitem.release();
-  P.g_wnode.release();
+  P.g_fnode.release();
+  P.g_cwave.release();
+  P.g_cnw.release();
   P.g_btab.release();
   P.g_near.release();
   P.g_nearlist.release();
@@ -928,81 +1044,113 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
     MPF_HIP(hipStreamSynchronize(s));
   }
 
-  // step 1: sorted items per node, laid-out lengths
+  // step 1: sorted owned items per node
   DevBuf<int32_t> keys, ovf;
-  DevBuf<int64_t> flen, tmp1, tmp2, foff;
+  DevBuf<int64_t> cnt;
   MPF_TRY(P.g_nelA.alloc(nadj));
   MPF_TRY(keys.alloc((size_t)nadj * npe));
-  MPF_TRY(flen.alloc(nNode));
+  MPF_TRY(cnt.alloc(nNode));
   MPF_TRY(ovf.alloc(1));
   MPF_HIP(hipMemsetAsync(ovf.ptr, 0, sizeof(int32_t), s));
   if (mixed)
     hipLaunchKernelGGL(k_node_items<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
-                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, flen.ptr, ovf.ptr);
+                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
   else
     hipLaunchKernelGGL(k_node_items<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
-                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, flen.ptr, ovf.ptr);
+                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
   MPF_HIP(hipGetLastError());
   int32_t o = 0;
   MPF_HIP(hipMemcpyAsync(&o, ovf.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   MPF_HIP(hipStreamSynchronize(s));
   if (o & 1) return gather_unsupported(p, "a node has more than 64 elements or 128 coupled nodes");
   if (o & 2) return MPF_ERR_STATE;
-  if (o & 4) return gather_unsupported(p, "a node's laid-out items exceed the row-kernel capacity");
 
-  // offsets in Morton order of the nodes
-  MPF_TRY(tmp1.alloc(nNode + 1));
-  MPF_TRY(tmp2.alloc(nNode + 1));
-  MPF_TRY(foff.alloc(nNode));
-  hipLaunchKernelGGL(k_gather_len, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.g_order.ptr, flen.ptr,
-                     tmp1.ptr);
-  MPF_HIP(hipMemsetAsync(tmp1.ptr + nNode, 0, sizeof(int64_t), s));
-  {
+  // step 2: layout groups of 8 nodes in Morton order, their padded lengths and offsets
+  const int64_t ngroups = ((int64_t)nNode + kGroupNodes - 1) / kGroupNodes;
+  DevBuf<int64_t> glen, goff;
+  MPF_TRY(glen.alloc(ngroups + 1));
+  MPF_TRY(goff.alloc(ngroups + 1));
+  const unsigned ggrid = (unsigned)((ngroups + 255) / 256);
+  if (mixed)
+    hipLaunchKernelGGL(k_group_len<10>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
+                       keys.ptr, cnt.ptr, glen.ptr);
+  else
+    hipLaunchKernelGGL(k_group_len<8>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
+                       keys.ptr, cnt.ptr, glen.ptr);
+  MPF_HIP(hipMemsetAsync(glen.ptr + ngroups, 0, sizeof(int64_t), s));
+  auto exclusive_sum = [&](const int64_t* in, int64_t* out, int64_t n) -> mpf_status {
     size_t tmp = 0;
-    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, tmp1.ptr, tmp2.ptr, nNode + 1, s));
+    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s));
     DevBuf<char> t;
     MPF_TRY(t.alloc(tmp > 0 ? tmp : 1));
-    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(t.ptr, tmp, tmp1.ptr, tmp2.ptr, nNode + 1, s));
-  }
-  hipLaunchKernelGGL(k_scatter_off, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.g_order.ptr, tmp2.ptr,
-                     foff.ptr);
+    MPF_HIP(hipcub::DeviceScan::ExclusiveSum(t.ptr, tmp, in, out, n, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    return MPF_OK;
+  };
+  MPF_TRY(exclusive_sum(glen.ptr, goff.ptr, ngroups + 1));
   int64_t total = 0;
-  MPF_HIP(hipMemcpyAsync(&total, tmp2.ptr + nNode, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipMemcpyAsync(&total, goff.ptr + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   MPF_HIP(hipStreamSynchronize(s));
+  if (total >= ((int64_t)1 << 31) - 64) return gather_unsupported(p, "more than 2^31 laid-out items");
 
-  // step 2: laid-out items
+  // laid-out items
   MPF_TRY(P.g_fitem.alloc(total > 0 ? (size_t)total : 64));
-  MPF_TRY(P.g_wnode.alloc(total > 0 ? (size_t)total / 64 : 1));
-  if (total == 0) MPF_HIP(hipMemsetAsync(P.g_fitem.ptr, 0xff, 64 * sizeof(int2), s));
+  MPF_TRY(P.g_fnode.alloc(total > 0 ? (size_t)total : 64));
+  MPF_HIP(hipMemsetAsync(P.g_fitem.ptr, 0xff, (total > 0 ? (size_t)total : 64) * sizeof(int2), s));
+  MPF_HIP(hipMemsetAsync(P.g_fnode.ptr, 0xff, (total > 0 ? (size_t)total : 64) * sizeof(int32_t), s));
   if (mixed)
-    hipLaunchKernelGGL(k_flat_fill<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
-                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr, P.g_wnode.ptr);
+    hipLaunchKernelGGL(k_group_fill<10>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
+                       P.nbptr.ptr, keys.ptr, cnt.ptr, P.g_nelA.ptr, goff.ptr, P.g_fitem.ptr, P.g_fnode.ptr);
   else
-    hipLaunchKernelGGL(k_flat_fill<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nbptr.ptr, P.g_nelA.ptr,
-                       keys.ptr, flen.ptr, foff.ptr, P.g_fitem.ptr, P.g_wnode.ptr);
+    hipLaunchKernelGGL(k_group_fill<8>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
+                       P.nbptr.ptr, keys.ptr, cnt.ptr, P.g_nelA.ptr, goff.ptr, P.g_fitem.ptr, P.g_fnode.ptr);
+  MPF_HIP(hipGetLastError());
   P.g_nwaves = total / 64;
 
-  // chunk tables of the chunked matrix kernel (Hex8)
+  // chunk tables of the chunked matrix kernel (Hex8): each group's wavefronts in chunks of <= CW
   if (!mixed && P.g_nwaves > 0) {
     constexpr int CW = MPF_CHUNK_WAVES, ECAP = MPF_CHUNK_ELEMS;
     static_assert(ECAP == 32, "mpf_set_chunk_stage_cap documents a cap of 32");
-    P.g_nchunks = (P.g_nwaves + CW - 1) / CW;
-    MPF_TRY(P.g_ctab.alloc((size_t)P.g_nchunks * ECAP));
-    MPF_TRY(P.g_ccnt.alloc((size_t)P.g_nchunks));
+    static_assert(CW >= 1 && CW <= 8, "k_chunk_table covers 8 wavefronts");
+    DevBuf<int64_t> ccount, coff;
+    MPF_TRY(ccount.alloc(ngroups + 1));
+    MPF_TRY(coff.alloc(ngroups + 1));
+    hipLaunchKernelGGL(k_group_chunks, dim3(ggrid), dim3(256), 0, s, ngroups, CW, glen.ptr, goff.ptr, nullptr,
+                       ccount.ptr, nullptr, nullptr);
+    MPF_HIP(hipMemsetAsync(ccount.ptr + ngroups, 0, sizeof(int64_t), s));
+    MPF_TRY(exclusive_sum(ccount.ptr, coff.ptr, ngroups + 1));
+    int64_t nch = 0;
+    MPF_HIP(hipMemcpyAsync(&nch, coff.ptr + ngroups, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    P.g_nchunks = nch;
+    MPF_TRY(P.g_cwave.alloc((size_t)nch));
+    MPF_TRY(P.g_cnw.alloc((size_t)nch));
+    MPF_TRY(P.g_ctab.alloc((size_t)nch * ECAP));
+    MPF_TRY(P.g_ccnt.alloc((size_t)nch));
     MPF_TRY(P.g_fslot.alloc((size_t)P.g_nwaves * 64));
-    hipLaunchKernelGGL((k_chunk_table<CW, ECAP>), dim3((unsigned)P.g_nchunks), dim3(64 * CW), 0, s, P.g_nwaves,
+    hipLaunchKernelGGL(k_group_chunks, dim3(ggrid), dim3(256), 0, s, ngroups, CW, glen.ptr, goff.ptr, coff.ptr,
+                       nullptr, P.g_cwave.ptr, P.g_cnw.ptr);
+    hipLaunchKernelGGL((k_chunk_table<ECAP>), dim3((unsigned)nch), dim3(512), 0, s, nch, P.g_cwave.ptr, P.g_cnw.ptr,
                        P.g_fitem.ptr, P.g_ctab.ptr, P.g_ccnt.ptr, P.g_fslot.ptr);
     MPF_HIP(hipGetLastError());
+    MPF_HIP(hipStreamSynchronize(s));
   }
 
   // step 3: column-block table; step 4: near-Dirichlet flags
-  MPF_TRY(P.g_btab.alloc((size_t)(nbtot > 0 ? nbtot : 1) * (mixed ? 4 : 2)));
+  MPF_TRY(P.g_btab.alloc((size_t)(nbtot > 0 ? nbtot : 1) * (mixed ? 8 : 4)));
   if (mixed)
     hipLaunchKernelGGL(k_block_table<true>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
                        P.nda.ptr, P.idp.ptr, P.dispDOF, P.rowptr.ptr, P.g_btab.ptr);
   else
     hipLaunchKernelGGL(k_block_table<false>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
                        P.nda.ptr, nullptr, P.dispDOF, P.rowptr.ptr, P.g_btab.ptr);
+  if (mixed)
+    hipLaunchKernelGGL(k_mirror<true>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
+                       P.g_btab.ptr);
+  else
+    hipLaunchKernelGGL(k_mirror<false>, dim3(nNode), dim3(64), 0, s, nNode, P.nbptr.ptr, P.nb.ptr, P.nodeinfo.ptr,
+                       P.g_btab.ptr);
+  MPF_HIP(hipGetLastError());
   MPF_TRY(P.g_near.alloc(nNode));
   if (mixed)
     hipLaunchKernelGGL(k_near<10>, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
@@ -1041,7 +1189,7 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   FlatArgs F{};
   F.nwaves = P.g_nwaves;
   F.fitem = P.g_fitem.ptr;
-  F.wnode = P.g_wnode.ptr;
+  F.fnode = P.g_fnode.ptr;
   F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
   F.nda = P.nda.ptr;
@@ -1052,6 +1200,8 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   F.rhs = p->rhs.ptr;
   F.nchunks = P.g_nchunks;
   F.ecap = p->chunk_cap;
+  F.cwave = P.g_cwave.ptr;
+  F.cnw = P.g_cnw.ptr;
   F.ctab = P.g_ctab.ptr;
   F.ccnt = P.g_ccnt.ptr;
   F.fslot = P.g_fslot.ptr;
@@ -1075,8 +1225,8 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
     hipLaunchKernelGGL((k_stage<8, 8, false, 8, MPF_STAGE_LDS_HEX>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A,
                        nElem, p->GS.ptr);
     if (flat && chunked)
-      hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS>), dim3(chunk_grid),
-                         dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
+      hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS, MPF_CHUNK_PHASES>),
+                         dim3(chunk_grid), dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
     else if (flat)
       hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<8, 8, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);

@@ -130,14 +130,16 @@ struct PatternBufs {
   // gather-assembly tables (gather.hip)
   DevBuf<int32_t> g_order;    // nodes in Morton (Z-curve) order of their coordinates
   DevBuf<int32_t> g_nelA;     // elements around each node (ascending), with the node's local index
-  DevBuf<int2> g_fitem;       // laid-out (element, node) items of every node's rows, 64 per wavefront
-  DevBuf<int32_t> g_wnode;    // node of each wavefront of items
-  DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j)
+  DevBuf<int2> g_fitem;       // laid-out (element, node) items of the owned blocks, 64 per wavefront
+  DevBuf<int32_t> g_fnode;    // node whose rows each laid-out item belongs to
+  DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j) and of its mirror (m, n)
   DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
   DevBuf<int32_t> g_nearlist; // the flagged nodes
   int32_t g_nnear = 0;
   int64_t g_nwaves = 0;
-  mpf::DevBuf<int32_t> g_ctab;   // chunked matrix kernel: distinct elements per chunk of wavefronts
+  mpf::DevBuf<int32_t> g_cwave;  // chunked matrix kernel: first wavefront of each chunk
+  mpf::DevBuf<int32_t> g_cnw;    // its number of wavefronts
+  mpf::DevBuf<int32_t> g_ctab;   // distinct elements per chunk
   mpf::DevBuf<int32_t> g_ccnt;   // their number per chunk
   mpf::DevBuf<uint8_t> g_fslot;  // each laid-out item's element slot in its chunk
   int64_t g_nchunks = 0;
