@@ -164,6 +164,27 @@ __device__ __forceinline__ void tet_Np(int gp, double Np[4]) {
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
 
+// A double moved across lanes by a DPP lane pattern (two 32-bit v_mov_dpp; no LDS round trip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over each aligned group of NG (2, 4 or 8) lanes, every lane receiving it: bitwise the same as
+// v += shfl_xor(v, 1); v += shfl_xor(v, 2); v += shfl_xor(v, 4).  The last step takes lane 7 - l of
+// the group (row_half_mirror) instead of l ^ 4; after the first two steps both hold the other
+// quad's sum, and the additions are the same.
+template <int NG>
+__device__ __forceinline__ double group_sum(double v) {
+  static_assert(NG == 2 || NG == 4 || NG == 8, "DPP group");
+  v += dpp_d<0xB1>(v);                   // quad_perm [1, 0, 3, 2]: lane ^ 1
+  if (NG >= 4) v += dpp_d<0x4E>(v);      // quad_perm [2, 3, 0, 1]: lane ^ 2
+  if (NG >= 8) v += dpp_d<0x141>(v);     // row_half_mirror
+  return v;
+}
+
 __device__ __forceinline__ int rank3(int mask, int i) { return __popc(mask & ((1 << i) - 1)); }
 
 // One Gauss point of one element: reference Jacobian from X, F from u (and Fn from u_prev for the
@@ -253,8 +274,7 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double v = -(b0 * P.sig[i][0] + b1 * P.sig[i][1] + b2 * P.sig[i][2]);
-#pragma unroll
-      for (int m = 1; m < NGP; m <<= 1) v += shfl_xor_d(v, m);
+      v = group_sum<NGP>(v);
       r[i] = v;
     }
     if (writer && (a % NGP) == gp) { R[a * 3] = r[0]; R[a * 3 + 1] = r[1]; R[a * 3 + 2] = r[2]; }
