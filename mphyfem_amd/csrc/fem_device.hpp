@@ -136,6 +136,32 @@ __device__ __forceinline__ void hex8_dN(int gp, double d[3][8]) {
   }
 }
 
+// dN/dxi of one Hex8 node a at Gauss point gp: the same products as hex8_dN.
+__device__ __forceinline__ void hex8_dN_node(int gp, int a, double d[3]) {
+  const double x1 = (gp == 1 || gp == 2 || gp == 5 || gp == 6) ? kGP2 : -kGP2;
+  const double x2 = (gp == 2 || gp == 3 || gp == 6 || gp == 7) ? kGP2 : -kGP2;
+  const double x3 = (gp >= 4) ? kGP2 : -kGP2;
+  const int iu = (a == 1 || a == 2 || a == 5 || a == 6) ? 1 : 0;
+  const int iv = (a == 2 || a == 3 || a == 6 || a == 7) ? 1 : 0;
+  const int iw = (a >= 4) ? 1 : 0;
+  const double Nu = iu ? 0.5 * (1.0 + x1) : 0.5 * (1.0 - x1);
+  const double Nv = iv ? 0.5 * (1.0 + x2) : 0.5 * (1.0 - x2);
+  const double Nw = iw ? 0.5 * (1.0 + x3) : 0.5 * (1.0 - x3);
+  const double du = iu ? 0.5 : -0.5, dv = iv ? 0.5 : -0.5, dw = iw ? 0.5 : -0.5;
+  d[0] = du * Nv * Nw;
+  d[1] = Nu * dv * Nw;
+  d[2] = Nu * Nv * dw;
+}
+
+// g = grad_x N from dN/dxi (d) and the inverse Jacobian Bi (row-major, Bi[r][c] = dxi_c/dx_r in the
+// element_point convention), with explicit fused multiply-adds so that every kernel that forms g
+// from a stored Bi gets the same bits as the stage kernel.
+__device__ __forceinline__ void grad_jinv(const double* Bi, const double d[3], double g[3]) {
+  g[0] = fma(d[2], Bi[2], fma(d[1], Bi[1], d[0] * Bi[0]));
+  g[1] = fma(d[2], Bi[5], fma(d[1], Bi[4], d[0] * Bi[3]));
+  g[2] = fma(d[2], Bi[8], fma(d[1], Bi[7], d[0] * Bi[6]));
+}
+
 // Gauss point gp of the 4-point tet rule (QuadratureUtil.cpp:690-712).
 __device__ __forceinline__ void tet_gp(int gp, double& x1, double& x2, double& x3) {
   x1 = (gp == 1) ? kTetB : kTetA;
@@ -193,8 +219,9 @@ __device__ __forceinline__ int rank3(int mask, int i) { return __popc(mask & ((1
 // -dv sum_m g_a,m sigma_i,m, is reduced over the NGP lanes of the element (consecutive lanes,
 // lane & (NGP-1) == gp) and stored by the lane gp == a % NGP into R[a*3 + i].
 // Elem_Solid_3D_Disp.cpp:250-330; ElementBase2.cpp:350-470.
-// COMPACT output (the gather path's Hex8 staging records): per node g (3) at G + a * 3, and the six
-// components b00 b11 b22 b01 b02 b12 of the symmetric b at Bv, instead of g and b g at G + a * 6.
+// COMPACT output (the gather path's Hex8 staging records): the inverse Jacobian Bi (9 doubles,
+// row-major) at G, from which g of every node is formed with grad_jinv, and the six components
+// b00 b11 b22 b01 b02 b12 of the symmetric b at Bv, instead of g and b g of every node at G + a * 6.
 template <int NPE, int NGP, bool MIXED, bool COMPACT = false>
 __device__ __forceinline__ void element_point(const double (*X)[3], const double (*U)[3], const double (*UP)[3],
                                               const double* Pn, const double (*d)[NPE], int gp, double wgp,
@@ -254,15 +281,19 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   }
   NHPoint P;
   neohooke_point(F, J, mu, K, Utype, MIXED, pres, dv, consistent, P);
+  if (writer && COMPACT) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) G[3 * r + c] = Bi[r][c];
+  }
 #pragma unroll
   for (int a = 0; a < NPE; ++a) {
-    const double g0 = d[0][a] * Bi[0][0] + d[1][a] * Bi[0][1] + d[2][a] * Bi[0][2];
-    const double g1 = d[0][a] * Bi[1][0] + d[1][a] * Bi[1][1] + d[2][a] * Bi[1][2];
-    const double g2 = d[0][a] * Bi[2][0] + d[1][a] * Bi[2][1] + d[2][a] * Bi[2][2];
-    if (writer && COMPACT) {
-      double* Ga = G + a * 3;
-      Ga[0] = g0; Ga[1] = g1; Ga[2] = g2;
-    } else if (writer) {
+    double ga[3];
+    const double da[3] = {d[0][a], d[1][a], d[2][a]};
+    grad_jinv(&Bi[0][0], da, ga);
+    const double g0 = ga[0], g1 = ga[1], g2 = ga[2];
+    if (writer && !COMPACT) {
       double* Ga = G + a * 6;
       Ga[0] = g0; Ga[1] = g1; Ga[2] = g2;
       Ga[3] = P.b[0][0] * g0 + P.b[0][1] * g1 + P.b[0][2] * g2;

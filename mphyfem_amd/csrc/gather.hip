@@ -37,18 +37,19 @@ constexpr int kMaxNB = 128;   // nodes coupled to one node (column blocks of its
 
 // Staging record layout (doubles) per element:
 //   Tet10/P1:  G [NGP][NPE][6]  g_a (3), (b g)_a (3)
-//   Hex8:      G [NGP][NPE][3]  g_a;  B [NGP][6]  b00 b11 b22 b01 b02 b12  (b g formed on use: 304
-//              doubles per element instead of 448)
+//   Hex8:      J [NGP][9]       the inverse Jacobian (g_a = grad_jinv(J, dN_a/dxi));
+//              B [NGP][6]       b00 b11 b22 b01 b02 b12   (g and b g formed on use: 184 doubles per
+//                               element instead of 448)
 //   S  [NGP][SS]       Hex8: s1..s5 (SS = 5); Tet10/P1: s1..s4, dv, dv0, thetahat, resip (SS = 8)
 //   R  [NPE][3]        element residual (no lift)
-// The first SL doubles (G, B, S) are what the matrix kernel reads for a block.
+// The first SL doubles (G or J, B, S) are what the matrix kernel reads for a block.
 template <int NPE, int NGP, bool MIXED>
 struct Stage {
   static constexpr bool COMPACT = !MIXED;
-  static constexpr int GW = COMPACT ? 3 : 6;
+  static constexpr int GW = COMPACT ? 3 : 6;      // doubles per node and Gauss point of g data
   static constexpr int SS = MIXED ? 8 : 5;
   static constexpr int G0 = 0;
-  static constexpr int B0 = NGP * NPE * GW;
+  static constexpr int B0 = COMPACT ? NGP * 9 : NGP * NPE * GW;
   static constexpr int S0 = B0 + (COMPACT ? NGP * 6 : 0);
   static constexpr int R0 = S0 + NGP * SS;
   static constexpr int SL = R0;
@@ -60,11 +61,14 @@ struct Stage {
 template <int NPE, int NGP, bool MIXED>
 __device__ __forceinline__ void record_g(const double* rec, int gp, int c, double g[3], double bg[3]) {
   using St = Stage<NPE, NGP, MIXED>;
-  const double* p = rec + St::G0 + (gp * NPE + c) * St::GW;
-  g[0] = p[0]; g[1] = p[1]; g[2] = p[2];
   if (St::COMPACT) {
+    double d[3];
+    hex8_dN_node(gp, c, d);
+    grad_jinv(rec + St::G0 + gp * 9, d, g);
     bsym_times(rec + St::B0 + gp * 6, g, bg);
   } else {
+    const double* p = rec + St::G0 + (gp * NPE + c) * St::GW;
+    g[0] = p[0]; g[1] = p[1]; g[2] = p[2];
     bg[0] = p[3]; bg[1] = p[4]; bg[2] = p[5];
   }
 }
@@ -141,8 +145,8 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   elem_material(A, writer ? e : base, mu, K);
   element_point<NPE, NGP, MIXED, St::COMPACT>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp],
                                               gp, wgp, mu, K, A.Utype, A.consistent != 0, writer,
-                                              rec + St::G0 + gp * NPE * St::GW, rec + St::S0 + gp * St::SS,
-                                              rec + St::R0, negJ, rec + St::B0 + gp * 6);
+                                              rec + St::G0 + gp * (St::COMPACT ? 9 : NPE * St::GW),
+                                              rec + St::S0 + gp * St::SS, rec + St::R0, negJ, rec + St::B0 + gp * 6);
   if (writer && negJ) flag_bad(A.bad, e);
   if (OUT_LDS) {
     __syncthreads();
@@ -484,20 +488,29 @@ __global__ void k_near(int32_t nNode, const int32_t* __restrict__ nptr, const in
 // Gauss points gp0 .. gp0 + NG - 1 of one (element, node b) item of node n's rows, added to v: the
 // 3x3 block K_{n b} (v[0..8]) and, for the mixed element, Kup(n, b) (v[9..11]), Kpu(n, b)
 // (v[12..14]) and Kpp(n, b) (v[15]).  G, Bv, S point at the first of those Gauss points' g data
-// (stride NPE * GW), b components (stride 6; compact records) and scalars (stride SS), in a staging
-// record in global memory or in a copy in LDS.
-template <int NPE, int NGP, bool MIXED, int NV, int NG, int UNROLL>
+// (stride NPE * GW; FROM_J: the inverse Jacobians of a Hex8 record, stride 9, g formed here), b
+// components (stride 6; compact records) and scalars (stride SS), in a staging record in global
+// memory or in a copy in LDS.
+template <int NPE, int NGP, bool MIXED, int NV, int NG, int UNROLL, bool FROM_J = false>
 __device__ __forceinline__ void item_gps(const double* G, const double* Bv, const double* S0p, int gp0, int a, int b,
                                          double v[NV]) {
   using St = Stage<NPE, NGP, MIXED>;
 #pragma unroll UNROLL
   for (int gp = 0; gp < NG; ++gp) {
-    const double* Gp = G + gp * NPE * St::GW;
+    const double* Gp = G + gp * (FROM_J ? 9 : NPE * St::GW);
     const double* S = S0p + gp * St::SS;
     double ga[3], ba[3], gb[3], bb[3];
+    if (St::COMPACT && FROM_J) {  // g from the record's inverse Jacobian
+      double da[3], db[3];
+      hex8_dN_node(gp0 + gp, a, da);
+      hex8_dN_node(gp0 + gp, b, db);
+      grad_jinv(Gp, da, ga);
+      grad_jinv(Gp, db, gb);
+    }
     if (St::COMPACT) {
+      if (!FROM_J)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
+        for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
       const double2* B2 = reinterpret_cast<const double2*>(Bv + gp * 6);
       const double2 c0 = B2[0], c1 = B2[1], c2 = B2[2];
       const double B6[6] = {c0.x, c0.y, c1.x, c1.y, c2.x, c2.y};
@@ -537,7 +550,7 @@ __device__ __forceinline__ void item_block(const double* rec, int a, int b, doub
   using St = Stage<NPE, NGP, MIXED>;
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
-  item_gps<NPE, NGP, MIXED, NV, NGP, UNROLL>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
+  item_gps<NPE, NGP, MIXED, NV, NGP, UNROLL, St::COMPACT>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
 }
 
 // Node a's residual rows of one element (v[0..2]) and, for the mixed element, its pressure row (v[3])
@@ -703,94 +716,116 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
 }
 
 // Chunked matrix kernel: one workgroup per chunk (the wavefronts of one layout group of 8 nodes in
-// Morton order, an aligned 2x2x2 block of nodes on a structured grid: 27 elements).  The chunk's
-// distinct elements' Gauss-point data (g, b, scalars) is copied to LDS with coalesced 16-byte loads,
-// in NPH phases of NGP / NPH Gauss points each (so a workgroup holds 1/NPH of it and more
-// workgroups fit on a CU), and every item then reads LDS, keeping its partial block in registers
-// between phases: each record is read from L2 once per chunk rather than once per item lane, and
-// the per-lane gathers that bound the per-wavefront kernel on address processing become LDS reads.
-// A chunk with more than ECAP distinct elements reads the records directly.
+// Morton order; for Hex8 on a structured grid an aligned 2x2x2 block of nodes and its 27 elements).
+// The chunk's distinct elements' Gauss-point data (g, b, scalars) is put in LDS — the verbatim part
+// of the records with coalesced 16-byte loads, g formed from the records' inverse Jacobians — and
+// every item then reads LDS: each record is read from L2 once per chunk rather than once per item
+// lane, and the per-lane gathers that bound the per-wavefront kernel on address processing become
+// LDS reads.  A chunk with more than ECAP distinct elements (or more than the problem's cap) reads
+// the records directly.  Workgroups of 8 wavefronts although a Hex8 group fills 5: measured per
+// Hex8 128^3 assembly, the kernel takes 13.6, 13.7, 8.3 and 7.0 ms with workgroups of 5, 6, 7 and 8
+// wavefronts for the same chunks.
 #ifndef MPF_CHUNK_WAVES
 #define MPF_CHUNK_WAVES 8
 #endif
 #ifndef MPF_CHUNK_ELEMS
 #define MPF_CHUNK_ELEMS 32
 #endif
-#ifndef MPF_CHUNK_PHASES
-#define MPF_CHUNK_PHASES 1
-#endif
+// LDS slot stride padded by 6 doubles (Hex8: 286 doubles, 2288 bytes per slot) so that the slots'
+// rows spread over the LDS banks: measured per Hex8 128^3 assembly, 10.24 ms unpadded, 9.97 ms
+// with 2, 9.93 ms with 6, 9.98 ms with 10
 #ifndef MPF_LDS_PAD
 #define MPF_LDS_PAD 6
 #endif
-template <int NPE, int NGP, bool MIXED, int CW, int ECAP, int NPH>
+template <int NPE, int NGP, bool MIXED, int CW, int ECAP>
 __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
   constexpr int NV = MIXED ? 16 : 9;
   constexpr int NR = MIXED ? 4 : 3;
   constexpr int NT = NV + NR;
-  static_assert(NGP % NPH == 0, "whole Gauss points per phase");
-  constexpr int NGp = NGP / NPH;
-  constexpr int GL = NGp * NPE * St::GW, BL = St::COMPACT ? NGp * 6 : 0, SN = NGp * St::SS;  // per phase
-  static_assert(GL % 2 == 0 && BL % 2 == 0 && SN % 2 == 0, "16-byte segments");
-  constexpr int SLq = (GL + BL + SN) / 2;         // 16-byte groups per element and phase
-  constexpr int SLP = GL + BL + SN + MPF_LDS_PAD; // LDS stride per element slot (doubles), padded
-  static_assert(SLP % 2 == 0, "16-byte slots");
+  // LDS slot: g of every node and Gauss point (Hex8: formed here from the record's inverse
+  // Jacobians), then the record's B and S copied verbatim (Tet10/P1: the record's G and S verbatim)
+  constexpr int GX = St::COMPACT ? NGP * NPE * 3 : 0;   // g formed in LDS (doubles per slot)
+  constexpr int CP0 = St::COMPACT ? St::B0 : 0;          // record part copied verbatim: [CP0, SL)
+  constexpr int CP2 = (St::SL - CP0) / 2;                // its 16-byte groups
+  constexpr int SLP = GX + (St::SL - CP0) + MPF_LDS_PAD; // LDS stride per element slot (doubles)
+  static_assert(SLP % 2 == 0 && GX % 2 == 0 && CP0 % 2 == 0, "16-byte slots");
   constexpr int STG = ECAP * SLP, PART = CW * 64 * NT;
   constexpr int LDSD = STG > PART ? STG : PART;  // record copies, then the run sums
   static_assert(ECAP <= 255, "slot tables");
   __shared__ double2 sMem[LDSD / 2];
   __shared__ int32_t sCol[CW * 64];
+  __shared__ double sdN[St::COMPACT ? NGP * NPE * 3 : 1];
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int64_t c = xcd_chunk(blockIdx.x, F.nchunks);
   if (c >= F.nchunks) return;  // whole workgroup, before any barrier
   const int cnt = F.ccnt[c];
   const bool staged = cnt <= ECAP && cnt <= F.ecap;
+  if (St::COMPACT && tid < NGP * NPE) hex8_dN_node(tid / NPE, tid % NPE, &sdN[3 * tid]);
+  __syncthreads();
+  if (staged) {
+    const int32_t* ct = F.ctab + c * ECAP;
+    constexpr int BS = 64 * CW;
+    // Hex8: one lane per (element slot, Gauss point) forms g of the 8 nodes from the record's inverse
+    // Jacobian; its loads are issued before the verbatim copy, so both share one memory latency
+    static_assert(!St::COMPACT || ECAP * NGP <= BS, "one (slot, Gauss point) per lane");
+    const bool jl = St::COMPACT && tid < cnt * NGP;
+    double Jr[9];
+    if (St::COMPACT) {
+      const double* J = F.GS + (size_t)ct[jl ? tid / NGP : 0] * St::TS + St::G0 + (tid % NGP) * 9;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Jr[i] = jl ? J[i] : 0.0;
+    }
+    // verbatim part: four 16-byte loads in flight per lane; indices past the end re-read the last group
+    const int tot = cnt * CP2;
+    auto ld = [&](int t) {
+      t = t < tot ? t : tot - 1;
+      const int k = t / CP2;
+      return reinterpret_cast<const double2*>(F.GS + (size_t)ct[k] * St::TS + CP0)[t - k * CP2];
+    };
+    auto dst = [&](int t) {
+      const int k = t / CP2;
+      return k * (SLP / 2) + GX / 2 + (t - k * CP2);
+    };
+    for (int t0 = tid; t0 < tot; t0 += 4 * BS) {
+      const double2 r0 = ld(t0), r1 = ld(t0 + BS), r2 = ld(t0 + 2 * BS), r3 = ld(t0 + 3 * BS);
+      sMem[dst(t0)] = r0;
+      if (t0 + BS < tot) sMem[dst(t0 + BS)] = r1;
+      if (t0 + 2 * BS < tot) sMem[dst(t0 + 2 * BS)] = r2;
+      if (t0 + 3 * BS < tot) sMem[dst(t0 + 3 * BS)] = r3;
+    }
+    if (jl) {
+      const int k = tid / NGP, gp = tid % NGP;
+      double* o = reinterpret_cast<double*>(sMem) + k * SLP + gp * NPE * 3;
+#pragma unroll
+      for (int a = 0; a < NPE; ++a) {
+        double g[3];
+        grad_jinv(Jr, &sdN[3 * (gp * NPE + a)], g);
+        o[3 * a] = g[0]; o[3 * a + 1] = g[1]; o[3 * a + 2] = g[2];
+      }
+    }
+  }
+  __syncthreads();
   const int64_t w = (int64_t)F.cwave[c] + wid;
   const bool wv = wid < F.cnw[c];
   const int2 it = wv ? F.fitem[w * 64 + lane] : make_int2(-1, -1);
   const bool live = it.x >= 0;
   const int blk = live ? (it.y >> 4) : -1;
-  const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-  const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
-  const int slot = live && staged ? F.fslot[w * 64 + lane] : 0;
   double v[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) v[q] = 0.0;
-  const int32_t* ct = F.ctab + c * ECAP;
-  const int tot = cnt * SLq;
-  for (int ph = 0; ph < NPH; ++ph) {
+  if (live) {
+    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+    const double* rec = F.GS + (size_t)e * St::TS;
+    // one Gauss point at a time: LDS latency needs no unrolling to hide, and registers are the limit
     if (staged) {
-      if (ph > 0) __syncthreads();  // every item is done with the previous phase
-      // element k's phase: its G, B, S segments; four 16-byte loads in flight per lane, indices past
-      // the end re-read the last group
-      auto ld = [&](int t) {
-        t = t < tot ? t : tot - 1;
-        const int k = t / SLq, r = t - k * SLq;
-        const double* src = F.GS + (size_t)ct[k] * St::TS;
-        const int off = r < GL / 2 ? St::G0 + ph * GL + 2 * r
-                        : r < (GL + BL) / 2 ? St::B0 + ph * BL + 2 * r - GL
-                                            : St::S0 + ph * SN + 2 * r - GL - BL;
-        return *reinterpret_cast<const double2*>(src + off);
-      };
-      auto dst = [&](int t) { return t + (t / SLq) * (SLP / 2 - SLq); };
-      constexpr int BS = 64 * CW;
-      for (int t0 = tid; t0 < tot; t0 += 4 * BS) {
-        const double2 r0 = ld(t0), r1 = ld(t0 + BS), r2 = ld(t0 + 2 * BS), r3 = ld(t0 + 3 * BS);
-        sMem[dst(t0)] = r0;
-        if (t0 + BS < tot) sMem[dst(t0 + BS)] = r1;
-        if (t0 + 2 * BS < tot) sMem[dst(t0 + 2 * BS)] = r2;
-        if (t0 + 3 * BS < tot) sMem[dst(t0 + 3 * BS)] = r3;
-      }
-      __syncthreads();
-      // one Gauss point at a time: LDS latency needs no unrolling to hide, and registers are the limit
-      const double* sl = reinterpret_cast<const double*>(sMem) + slot * SLP;
-      if (live) item_gps<NPE, NGP, MIXED, NV, NGp, 1>(sl, sl + GL, sl + GL + BL, ph * NGp, a, b, v);
-    } else if (live) {
-      item_gps<NPE, NGP, MIXED, NV, NGp, 1>(rec + St::G0 + ph * GL, rec + St::B0 + ph * BL, rec + St::S0 + ph * SN,
-                                            ph * NGp, a, b, v);
+      const double* sl = reinterpret_cast<const double*>(sMem) + F.fslot[w * 64 + lane] * SLP;
+      item_gps<NPE, NGP, MIXED, NV, NGP, 1>(sl, sl + GX + (St::B0 - CP0), sl + GX + (St::S0 - CP0), 0, a, b, v);
+    } else {
+      item_gps<NPE, NGP, MIXED, NV, NGP, 1, St::COMPACT>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
     }
+    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   }
-  if (live && b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   __syncthreads();  // the record copies become the run-sum area
   double* const part = reinterpret_cast<double*>(sMem) + wid * 64 * NT;
   sCol[tid] = blk;
@@ -1225,7 +1260,7 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
     hipLaunchKernelGGL((k_stage<8, 8, false, 8, MPF_STAGE_LDS_HEX>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A,
                        nElem, p->GS.ptr);
     if (flat && chunked)
-      hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS, MPF_CHUNK_PHASES>),
+      hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS>),
                          dim3(chunk_grid), dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
     else if (flat)
       hipLaunchKernelGGL((k_flat<8, 8, false>), dim3(flat_grid), dim3(64), 0, s, F);
