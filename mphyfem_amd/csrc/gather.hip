@@ -58,12 +58,18 @@ struct Stage {
 };
 
 // g, b g of node c at Gauss point gp from a staging record, either layout
+// (dN: Hex8 dN/dxi table [NGP][NPE][3], e.g. in LDS; null: computed here)
 template <int NPE, int NGP, bool MIXED>
-__device__ __forceinline__ void record_g(const double* rec, int gp, int c, double g[3], double bg[3]) {
+__device__ __forceinline__ void record_g(const double* rec, int gp, int c, double g[3], double bg[3],
+                                         const double* dN = nullptr) {
   using St = Stage<NPE, NGP, MIXED>;
   if (St::COMPACT) {
     double d[3];
-    hex8_dN_node(gp, c, d);
+    if (dN) {
+      d[0] = dN[3 * (gp * NPE + c)]; d[1] = dN[3 * (gp * NPE + c) + 1]; d[2] = dN[3 * (gp * NPE + c) + 2];
+    } else {
+      hex8_dN_node(gp, c, d);
+    }
     grad_jinv(rec + St::G0 + gp * 9, d, g);
     bsym_times(rec + St::B0 + gp * 6, g, bg);
   } else {
@@ -75,14 +81,16 @@ __device__ __forceinline__ void record_g(const double* rec, int gp, int c, doubl
 
 // 3x3 tangent block (a, b) of one element from its staging record, either layout
 template <int NPE, int NGP, bool MIXED>
-__device__ __forceinline__ void record_block(const double* rec, int a, int b, double Kb[9]) {
+__device__ __forceinline__ void record_block(const double* rec, int a, int b, double Kb[9],
+                                             const double* dN = nullptr) {
   using St = Stage<NPE, NGP, MIXED>;
 #pragma unroll
   for (int t = 0; t < 9; ++t) Kb[t] = 0.0;
+#pragma unroll 2
   for (int gp = 0; gp < NGP; ++gp) {
     double ga[3], ba[3], gb[3], bb[3];
-    record_g<NPE, NGP, MIXED>(rec, gp, a, ga, ba);
-    record_g<NPE, NGP, MIXED>(rec, gp, b, gb, bb);
+    record_g<NPE, NGP, MIXED>(rec, gp, a, ga, ba, dN);
+    record_g<NPE, NGP, MIXED>(rec, gp, b, gb, bb, dN);
     const double* S = rec + St::S0 + gp * St::SS;
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], Kb);
   }
@@ -672,6 +680,17 @@ __device__ __forceinline__ void run_heads(const FlatArgs& F, int64_t item, int l
 #pragma unroll
     for (int q = 0; q < NT; ++q) acc[q] += part[t2 * NT + q];
   const int flags = MIXED ? F.btab[8 * (size_t)blk + 3] : (F.btab[4 * (size_t)blk + 1] & 0x1ff);
+  if ((flags >> 8) & 1) {
+    // self block K_nn: its (i, k) and (k, i) entries (and n's u-p and p-u couplings) agree to
+    // rounding; their mean makes the assembled matrix exactly symmetric
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = i + 1; k < 3; ++k) acc[3 * i + k] = acc[3 * k + i] = 0.5 * (acc[3 * i + k] + acc[3 * k + i]);
+    if (MIXED)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc[9 + i] = acc[12 + i] = 0.5 * (acc[9 + i] + acc[12 + i]);
+  }
   if ((flags >> 8) & 1) {  // self column: n's right-hand side, its elements' residuals in element order
     const int n = F.fnode[item];
     const int mn = (flags >> 3) & 7;
@@ -954,9 +973,13 @@ struct LiftArgs {
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
   using St = Stage<NPE, NGP, MIXED>;
+  __shared__ double sdN[St::COMPACT ? NGP * NPE * 3 : 1];
   const int q = blockIdx.x;
   if (q >= L.nnear) return;
   const int n = L.list[q], lane = threadIdx.x;
+  static_assert(!St::COMPACT || NGP * NPE <= 64, "one table entry per lane");
+  if (St::COMPACT && lane < NGP * NPE) hex8_dN_node(lane / NPE, lane % NPE, &sdN[3 * lane]);
+  __syncthreads();
   const int info = L.nodeinfo[n];
   const int mn = info & 7;
   const bool isPn = MIXED && ((info >> 3) & 1);
@@ -975,7 +998,7 @@ __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
     for (int k = 0; k < 3; ++k) db[k] = ((mb >> k) & 1) ? 0.0 : L.dispApplied[3 * (size_t)vb + k];
     const double* rec = L.GS + (size_t)e * St::TS;
     double Kb[9];
-    record_block<NPE, NGP, MIXED>(rec, a, b, Kb);
+    record_block<NPE, NGP, MIXED>(rec, a, b, Kb, St::COMPACT ? sdN : nullptr);
 #pragma unroll
     for (int i = 0; i < 3; ++i) acc[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
     if (MIXED && isPn && a < 4) {

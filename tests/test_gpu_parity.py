@@ -266,6 +266,37 @@ def test_hex_chunk_staging_vs_oracle(mpf, orc, gpu, cap):
         P.set_chunk_stage_cap(33)
 
 
+@pytest.mark.parametrize("etype", ["hex8", "tet10p1"])
+def test_gather_matrix_exactly_symmetric(mpf, gpu, etype):
+    """The gather sums each pair of column blocks once and writes it to both places, so the
+    assembled matrix (u-u, u-p, p-u and p-p blocks alike) equals its transpose bit for bit."""
+    import scipy.sparse as sp
+
+    if etype == "hex8":
+        n = 6
+        X, conn = cube_hex(n)
+        P = mpf.Problem(mpf.HEX8, X, conn, mpf.matl_from_E_nu(1e3, 0.3, tangent=1))
+        apply_bcs(P, hex_stretch_bcs(n))
+    else:
+        n = 4
+        X, conn = mpf.mesh_tet10(n, jitter=0.1, seed=2)
+        P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
+        apply_bcs(P, tet_compress_bcs(n))
+    P.prepare_matrix_pattern()
+    u = random_state(len(X), 0.004, seed=51)
+    P.set_state(disp=u, disp_prev=0.5 * u)
+    P.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    vals, _ = P.system()
+    pat = P.pattern()
+    nd = len(pat.rowptr) - 1
+    A = sp.csr_matrix((vals, pat.colidx, pat.rowptr), shape=(nd, nd))
+    D = (A - A.T).tocsr()
+    D.eliminate_zeros()
+    assert D.nnz == 0
+    assert abs(A).max() > 0
+
+
 def test_gather_matches_scatter_bitwise_reproducible(mpf, gpu):
     """The gather path writes each CSR value once in a fixed order: repeated assemblies are
     bit-identical, and both algorithms agree to rounding."""
