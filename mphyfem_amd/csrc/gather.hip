@@ -609,18 +609,9 @@ struct FlatArgs {
 // m's pressure column), acc[12..14] = Kpu(n, m), acc[15] = Kpp(n, m); m's rows receive
 // Kup(m, n) = Kpu(n, m)^T, Kpu(m, n) = Kup(n, m)^T, Kpp(m, n) = Kpp(n, m).
 template <bool MIXED>
-__device__ __forceinline__ void write_block(const FlatArgs& F, size_t blk, const double* acc) {
+__device__ __forceinline__ void write_block(const FlatArgs& F, const int* bt, const double* acc) {
   constexpr int BT = MIXED ? 4 : 2;
   double* __restrict__ V = F.vals;
-  int bt[2 * BT];
-  if (MIXED) {
-    const int4* p = reinterpret_cast<const int4*>(F.btab) + 2 * blk;
-    const int4 x = p[0], y = p[1];
-    bt[0] = x.x; bt[1] = x.y; bt[2] = x.z; bt[3] = x.w; bt[4] = y.x; bt[5] = y.y; bt[6] = y.z; bt[7] = y.w;
-  } else {
-    const int4 x = reinterpret_cast<const int4*>(F.btab)[blk];
-    bt[0] = x.x; bt[1] = x.y; bt[2] = x.z; bt[3] = x.w;
-  }
   {
     const int masks = MIXED ? bt[3] : (bt[1] & 0x1ff);
     const int La = MIXED ? (bt[2] & 0xffff) : (bt[1] >> 9);
@@ -669,9 +660,31 @@ __device__ __forceinline__ void write_block(const FlatArgs& F, size_t blk, const
 // run (its head) sums the run in lane order, which is element order, and writes the block and its
 // mirror; the self column's head also writes n's right-hand-side rows.  Every CSR value is written
 // exactly once.
+// An item lane's column-block table entry (its block's CSR location and its mirror's), loaded
+// before the lane's block is computed so that the run head has it at hand.
+template <bool MIXED>
+struct BlockEntry {
+  int v[MIXED ? 8 : 4];
+  __device__ __forceinline__ void load(const int32_t* btab, int blk) {
+    const int4* p = reinterpret_cast<const int4*>(btab) + (MIXED ? 2 : 1) * (size_t)blk;
+    const int4 x = p[0];
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    if (MIXED) {
+      const int4 y = p[MIXED ? 1 : 0];
+      v[MIXED ? 4 : 0] = y.x; v[MIXED ? 5 : 1] = y.y; v[MIXED ? 6 : 2] = y.z; v[MIXED ? 7 : 3] = y.w;
+    }
+  }
+  __device__ __forceinline__ int flags() const { return MIXED ? v[3] : (v[1] & 0x1ff); }
+};
+
+// Column runs of one wavefront's items, after every item lane has stored its values at part[t * NT]
+// and its block index (nbptr[n] + j, unique per node and column) at col[t]: the first lane of each
+// run (its head) sums the run in lane order, which is element order, and writes the block and its
+// mirror (bt: the head's table entry); the self column's head also writes n's right-hand-side rows
+// (n: the head's node).  Every CSR value is written exactly once.
 template <bool MIXED, int NV, int NT>
-__device__ __forceinline__ void run_heads(const FlatArgs& F, int64_t item, int lane, int blk, const double v[NT],
-                                          const int32_t* col, const double* part) {
+__device__ __forceinline__ void run_heads(const FlatArgs& F, int n, int lane, int blk, const double v[NT],
+                                          const int32_t* col, const double* part, const BlockEntry<MIXED>& bt) {
   if (lane > 0 && col[lane - 1] == blk) return;
   double acc[NT];
 #pragma unroll
@@ -679,7 +692,7 @@ __device__ __forceinline__ void run_heads(const FlatArgs& F, int64_t item, int l
   for (int t2 = lane + 1; t2 < 64 && col[t2] == blk; ++t2)
 #pragma unroll
     for (int q = 0; q < NT; ++q) acc[q] += part[t2 * NT + q];
-  const int flags = MIXED ? F.btab[8 * (size_t)blk + 3] : (F.btab[4 * (size_t)blk + 1] & 0x1ff);
+  const int flags = bt.flags();
   if ((flags >> 8) & 1) {
     // self block K_nn: its (i, k) and (k, i) entries (and n's u-p and p-u couplings) agree to
     // rounding; their mean makes the assembled matrix exactly symmetric
@@ -690,16 +703,14 @@ __device__ __forceinline__ void run_heads(const FlatArgs& F, int64_t item, int l
     if (MIXED)
 #pragma unroll
       for (int i = 0; i < 3; ++i) acc[9 + i] = acc[12 + i] = 0.5 * (acc[9 + i] + acc[12 + i]);
-  }
-  if ((flags >> 8) & 1) {  // self column: n's right-hand side, its elements' residuals in element order
-    const int n = F.fnode[item];
+    // n's right-hand side, its elements' residuals in element order
     const int mn = (flags >> 3) & 7;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       if ((mn >> i) & 1) F.rhs[F.nda[3 * (size_t)n + i]] = acc[NV + i];
     if (MIXED && ((flags >> 7) & 1)) F.rhs[F.dispDOF + F.idp[n]] = acc[NT - 1];
   }
-  write_block<MIXED>(F, (size_t)blk, acc);
+  write_block<MIXED>(F, bt.v, acc);
 }
 
 // Matrix kernel, one wavefront per 64 laid-out items, every item reading its element's staging record
@@ -717,21 +728,25 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   if (w >= F.nwaves) return;
   const int2 it = F.fitem[w * 64 + lane];
   const bool live = it.x >= 0;
-  const int blk = live ? (it.y >> 4) : -1;
+  const int blk = live ? (it.y >> 4) : 0;
+  const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+  const bool self = live && a == b;  // the element's own item of n carries its residual rows
   double v[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) v[q] = 0.0;
-  if (live) {
-    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-    const double* rec = F.GS + (size_t)e * St::TS;
-    item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
-    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);  // the element's own item of n
-  }
-  sCol[lane] = blk;
+  const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
+  if (live) item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
+  if (self) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
+  // the run head's table entry and node, loaded before the barrier (registers are the limit
+  // during the block computation)
+  BlockEntry<MIXED> bt;
+  bt.load(F.btab, blk);
+  const int n = self ? F.fnode[w * 64 + lane] : 0;
+  sCol[lane] = live ? blk : -1;
 #pragma unroll
   for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, w * 64 + lane, lane, blk, v, sCol, sPart);
+  if (live) run_heads<MIXED, NV, NT>(F, n, lane, blk, v, sCol, sPart, bt);
 }
 
 // Chunked matrix kernel: one workgroup per chunk (the wavefronts of one layout group of 8 nodes in
@@ -780,6 +795,25 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   if (c >= F.nchunks) return;  // whole workgroup, before any barrier
   const int cnt = F.ccnt[c];
   const bool staged = cnt <= ECAP && cnt <= F.ecap;
+  // per-lane loads that need nothing from LDS, issued before the staging so that their latency
+  // overlaps it: the item, its slot, its block's table entry, and (self items) the node and the
+  // element's residual rows
+  const int64_t w = (int64_t)F.cwave[c] + wid;
+  const bool wv = wid < F.cnw[c];
+  const int2 it = wv ? F.fitem[w * 64 + lane] : make_int2(-1, -1);
+  const bool live = it.x >= 0;
+  const int blk = live ? (it.y >> 4) : 0;
+  const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
+  const bool self = live && a == b;
+  const int slot = live && staged ? F.fslot[w * 64 + lane] : 0;
+  BlockEntry<MIXED> bt;
+  bt.load(F.btab, blk);
+  const int n = self ? F.fnode[w * 64 + lane] : 0;
+  const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
+  double v[NT];
+#pragma unroll
+  for (int q = 0; q < NT; ++q) v[q] = 0.0;
+  if (self) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   if (St::COMPACT && tid < NGP * NPE) hex8_dN_node(tid / NPE, tid % NPE, &sdN[3 * tid]);
   __syncthreads();
   if (staged) {
@@ -825,33 +859,22 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
     }
   }
   __syncthreads();
-  const int64_t w = (int64_t)F.cwave[c] + wid;
-  const bool wv = wid < F.cnw[c];
-  const int2 it = wv ? F.fitem[w * 64 + lane] : make_int2(-1, -1);
-  const bool live = it.x >= 0;
-  const int blk = live ? (it.y >> 4) : -1;
-  double v[NT];
-#pragma unroll
-  for (int q = 0; q < NT; ++q) v[q] = 0.0;
   if (live) {
-    const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-    const double* rec = F.GS + (size_t)e * St::TS;
     // one Gauss point at a time: LDS latency needs no unrolling to hide, and registers are the limit
     if (staged) {
-      const double* sl = reinterpret_cast<const double*>(sMem) + F.fslot[w * 64 + lane] * SLP;
+      const double* sl = reinterpret_cast<const double*>(sMem) + slot * SLP;
       item_gps<NPE, NGP, MIXED, NV, NGP, 1>(sl, sl + GX + (St::B0 - CP0), sl + GX + (St::S0 - CP0), 0, a, b, v);
     } else {
       item_gps<NPE, NGP, MIXED, NV, NGP, 1, St::COMPACT>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
     }
-    if (b == a) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   }
   __syncthreads();  // the record copies become the run-sum area
   double* const part = reinterpret_cast<double*>(sMem) + wid * 64 * NT;
-  sCol[tid] = blk;
+  sCol[tid] = live ? blk : -1;
 #pragma unroll
   for (int q = 0; q < NT; ++q) part[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, w * 64 + lane, lane, blk, v, sCol + wid * 64, part);
+  if (live) run_heads<MIXED, NV, NT>(F, n, lane, blk, v, sCol + wid * 64, part, bt);
 }
 
 // Chunk tables (once per pattern), one workgroup of 512 lanes per chunk of at most 8 wavefronts: the
