@@ -25,7 +25,8 @@ namespace mpf {
 
 namespace {
 
-constexpr int kNBCap = 512;  // candidate capacity per node (adjacent elements x npe)
+constexpr int kNBCap = 512;      // candidate capacity per node (adjacent elements x npe), first pass
+constexpr int kNBCapBig = 8192;  // second pass, for the nodes above kNBCap (LDS 32 KiB)
 
 __device__ __forceinline__ uint64_t color_priority(int64_t e) {
   uint64_t z = (uint64_t)e + 0x9E3779B97F4A7C15ull;
@@ -92,25 +93,28 @@ __global__ void k_node_fill(const int32_t* __restrict__ conn, int64_t nent, int3
 
 // One wavefront per node: gather the nodes of the adjacent elements into LDS, bitonic-sort,
 // unique.  COUNT pass writes |NB(n)|, the row length L(n) and the overflow flag; FILL pass
-// writes NB(n) at nbptr[n].
-template <bool FILL>
+// writes NB(n) at nbptr[n].  Nodes with more than kNBCap candidates are left to a second launch
+// with CAP = kNBCapBig (which skips the others); beyond that the overflow flag is set.
+template <bool FILL, int CAP = kNBCap>
 __global__ void __launch_bounds__(64) k_neighbours(const int32_t* __restrict__ conn, int32_t npe, int32_t nNode,
                                                      const int32_t* __restrict__ nptr, const int32_t* __restrict__ nel,
                                                      const int32_t* __restrict__ nda, const int32_t* __restrict__ isP,
                                                      int32_t* __restrict__ nbcount, int32_t* __restrict__ rowlen_node,
                                                      const int32_t* __restrict__ nbptr, int32_t* __restrict__ nb,
                                                      int32_t* __restrict__ overflow) {
-  __shared__ int32_t B[kNBCap];  // one wavefront per block: the barriers below are uniform
+  __shared__ int32_t B[CAP];  // one wavefront per block: the barriers below are uniform
+  constexpr int LOWER = CAP == kNBCap ? 0 : kNBCap;  // this launch's nodes: LOWER < candidates <= CAP
   const int lane = threadIdx.x;
   const int n = blockIdx.x;
-  const bool valid = n < nNode;
+  if (n >= nNode) return;  // whole block
   int ncand = 0;
-  if (valid) {
+  {
     const int e0 = nptr[n], e1 = nptr[n + 1];
     ncand = (e1 - e0) * npe;
-    if (ncand > kNBCap) {
-      if (lane == 0) atomicOr(overflow, 1);
-      ncand = 0;
+    if (LOWER > 0 && ncand <= LOWER) return;  // the first launch's node (whole block)
+    if (ncand > CAP) {
+      if (CAP == kNBCapBig && lane == 0) atomicOr(overflow, 1);
+      return;                    // the second launch's node, or too many (whole block)
     }
     for (int t = lane; t < ncand; t += 64) {
       const int e = nel[e0 + t / npe];
@@ -137,7 +141,7 @@ __global__ void __launch_bounds__(64) k_neighbours(const int32_t* __restrict__ c
   }
   // unique + compaction with a running wave prefix
   int total = 0, ulen = 0, plen = 0;
-  const int base = valid && FILL ? nbptr[n] : 0;
+  const int base = FILL ? nbptr[n] : 0;
   for (int c = 0; c < ncand; c += 64) {
     const int i = c + lane;
     bool keep = false;
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(64) k_neighbours(const int32_t* __restrict__ c
       ulen += __shfl_xor(ulen, off, 64);
       plen += __shfl_xor(plen, off, 64);
     }
-    if (valid && lane == 0) {
+    if (lane == 0) {
       nbcount[n] = total;
       rowlen_node[n] = ulen + plen;
     }
@@ -482,6 +486,9 @@ mpf_status build_pattern(mpf_problem* p) {
   hipLaunchKernelGGL(k_neighbours<false>, dim3(gn), dim3(64), 0, s, p->conn.ptr, p->npe, p->nNode, P.nptr.ptr,
                      P.nel.ptr, P.nda.ptr, mixed ? isP.ptr : nullptr, nbcount.ptr, rowlen_node.ptr, nullptr, nullptr,
                      overflow.ptr);
+  hipLaunchKernelGGL((k_neighbours<false, kNBCapBig>), dim3(gn), dim3(64), 0, s, p->conn.ptr, p->npe, p->nNode,
+                     P.nptr.ptr, P.nel.ptr, P.nda.ptr, mixed ? isP.ptr : nullptr, nbcount.ptr, rowlen_node.ptr, nullptr,
+                     nullptr, overflow.ptr);
   MPF_TRY(P.nbptr.alloc(p->nNode + 1));
   int64_t nbtot = 0;
   MPF_TRY(exscan(nbcount.ptr, P.nbptr.ptr, p->nNode + 1, &nbtot, s));
@@ -489,11 +496,14 @@ mpf_status build_pattern(mpf_problem* p) {
   hipLaunchKernelGGL(k_neighbours<true>, dim3(gn), dim3(64), 0, s, p->conn.ptr, p->npe, p->nNode, P.nptr.ptr,
                      P.nel.ptr, P.nda.ptr, mixed ? isP.ptr : nullptr, nullptr, nullptr, P.nbptr.ptr, P.nb.ptr,
                      overflow.ptr);
+  hipLaunchKernelGGL((k_neighbours<true, kNBCapBig>), dim3(gn), dim3(64), 0, s, p->conn.ptr, p->npe, p->nNode,
+                     P.nptr.ptr, P.nel.ptr, P.nda.ptr, mixed ? isP.ptr : nullptr, nullptr, nullptr, P.nbptr.ptr,
+                     P.nb.ptr, overflow.ptr);
   int32_t ovf = 0;
   MPF_HIP(hipMemcpyAsync(&ovf, overflow.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   MPF_HIP(hipStreamSynchronize(s));
   if (ovf) {
-    std::fprintf(stderr, "libmpf: a node touches more than %d element-node entries\n", kNBCap);
+    std::fprintf(stderr, "libmpf: a node touches more than %d element-node entries\n", kNBCapBig);
     return MPF_ERR_ARG;
   }
 

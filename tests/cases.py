@@ -37,3 +37,52 @@ def normwise_rel(a, b) -> float:
     b = np.asarray(b, dtype=np.float64)
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+def tet10_fan(subdiv: int = 1, radius: float = 1.0):
+    """Tet10 mesh of 20 * 4^subdiv tetrahedra sharing one central vertex: the faces of a subdivided
+    icosahedron joined to its centre (80 tets around one node for subdiv = 1).  Mid-edge nodes at
+    the edge midpoints, Tet10 local order 4:(0,1) 5:(1,2) 6:(0,2) 7:(0,3) 8:(2,3) 9:(1,3).
+    Returns X, conn (int32) and the indices of the nodes on the outer sphere."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, dtype=float) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        mids = {}
+
+        def mid(i, j):
+            k = (min(i, j), max(i, j))
+            if k not in mids:
+                m = V[i] + V[j]
+                V.append(m / np.linalg.norm(m))
+                mids[k] = len(V) - 1
+            return mids[k]
+        F2 = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            F2 += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = F2
+    X = [np.zeros(3)] + [radius * v for v in V]  # node 0 is the centre
+    outer = list(range(1, len(X)))
+    edge_node = {}
+
+    def enode(i, j):
+        k = (min(i, j), max(i, j))
+        if k not in edge_node:
+            X.append(0.5 * (X[i] + X[j]))
+            edge_node[k] = len(X) - 1
+            if i != 0 and j != 0:
+                outer.append(len(X) - 1)
+        return edge_node[k]
+    conn = []
+    for a, b, c in F:
+        v = [0, a + 1, b + 1, c + 1]
+        if np.linalg.det(np.array([X[v[1]] - X[v[0]], X[v[2]] - X[v[0]], X[v[3]] - X[v[0]]])) < 0:
+            v[2], v[3] = v[3], v[2]
+        e = [(0, 1), (1, 2), (0, 2), (0, 3), (2, 3), (1, 3)]
+        conn.append(v + [enode(v[p], v[q]) for p, q in e])
+    return np.array(X), np.array(conn, dtype=np.int32), np.array(outer, dtype=np.int32)

@@ -11,7 +11,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, random_state, tet_compress_bcs
+from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, random_state, tet10_fan, tet_compress_bcs
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -264,6 +264,33 @@ def test_hex_chunk_staging_vs_oracle(mpf, orc, gpu, cap):
         assert normwise_rel(rhs, orhs) < TOL_K
     with pytest.raises(mpf.MpfError):
         P.set_chunk_stage_cap(33)
+
+
+def test_high_valence_node_falls_back_to_scatter(mpf, orc, gpu):
+    """A node in 80 elements exceeds the gather tables (64 elements, 128 coupled nodes per node):
+    the problem switches itself to the coloured scatter, and K and R still match the oracle."""
+    X, conn, outer = tet10_fan(1)
+    bcs = [(outer, 0, None), (outer, 1, None), (outer, 2, -0.02)]
+    P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
+    O = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    u = random_state(len(X), 0.003, seed=61)
+    p = np.random.default_rng(62).uniform(-0.2, 0.2, len(X))
+    for Q in (P, O):
+        apply_bcs(Q, bcs)
+        Q.prepare_matrix_pattern()
+        Q.set_state(disp=u, disp_prev=0.5 * u, pres=p, pres_prev=p)
+        Q.set_boundary_conditions(1.0)
+    assert P.assembly_mode() == mpf.ASSEMBLY_COLORED_SCATTER
+    pat = P.pattern()
+    rowptr, colidx, _, _ = O.pattern()
+    assert np.array_equal(pat.rowptr, rowptr) and np.array_equal(pat.colidx, colidx)
+    for first in (True, False):
+        P.calc_stiffness_and_residual(first, 1.0)
+        vals, rhs = P.system()
+        ov, orhs, st = O.assemble(first, 1.0)
+        assert st == 0
+        assert normwise_rel(vals, ov) < TOL_K
+        assert normwise_rel(rhs, orhs) < TOL_K
 
 
 @pytest.mark.parametrize("etype", ["hex8", "tet10p1"])
