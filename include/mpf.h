@@ -167,7 +167,13 @@ mpf_status mpf_rhs_norm(mpf_problem* p, double* norm);
 
 /* Krylov solve of the assembled system (SolverPetsc::factoriseAndSolve, SolverPetsc.cpp:321;   */
 /* KSPSolve with zero initial guess, :290-292).                                                  */
-enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1 };
+/* MPF_KSP_BAND: direct solve by banded LDL^T on the GPU after a bandwidth-reducing renumbering    */
+/* (reverse Cuthill-McKee or slices across an axis); displacement systems only; replaces the      */
+/* plain method's direct solve (SolverEigen::solve, Eigen::SimplicialLDLT, SolverEigen.cpp:191-228);*/
+/* pc, restart, maxit, rtol are ignored;                                                          */
+/* a zero or non-finite pivot gives MPF_ERR_DIVERGED.  The half-bandwidth after renumbering must   */
+/* stay below about 490 dofs (MPF_ERR_ARG).                                                      */
+enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1, MPF_KSP_BAND = 2 };
 enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2 };
 typedef struct {
   int32_t ksp;       /* MPF_KSP_* */
@@ -240,6 +246,8 @@ typedef struct {
   int32_t chunk;          /* CG iterations per graph launch (0: keep) */
   int32_t tile;           /* dofs per tile (0: keep the autotuned / default value) */
   int32_t lanes_per_row;  /* SpMV lanes per row, 4, 8 or 16 (0: keep) */
+  int32_t solver;         /* 0: Jacobi-PCG per mesh; 1: banded LDL^T direct solve per mesh (MPF_KSP_BAND, */
+                          /*    one workgroup per mesh; exact Newton corrections, cg_* ignored)        */
 } mpf_batch_opts;
 
 typedef struct {

@@ -22,7 +22,7 @@ LIB_PATH = _PKG / "libmpf.so"
 
 HEX8 = 8
 TET10P1 = 10
-KSP_CG, KSP_GMRES = 0, 1
+KSP_CG, KSP_GMRES, KSP_BAND = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
 ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER, ASSEMBLY_GATHER_DIRECT = 0, 1, 2
 
@@ -72,7 +72,10 @@ class NewtonReport(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("nsteps", C.c_int32), ("dt", C.c_double), ("itmax", C.c_int32), ("rtol", C.c_double),
                 ("cg_maxit", C.c_int32), ("cg_rtol", C.c_double), ("use_graph", C.c_int32), ("chunk", C.c_int32),
-                ("tile", C.c_int32), ("lanes_per_row", C.c_int32)]
+                ("tile", C.c_int32), ("lanes_per_row", C.c_int32), ("solver", C.c_int32)]
+
+
+BATCH_SOLVER_PCG, BATCH_SOLVER_BAND = 0, 1
 
 
 class BatchReport(C.Structure):
@@ -557,9 +560,10 @@ class Problem:
         return t.value, l.value, ms.value
 
     def batch_newton_solve(self, nsteps=1, dt=1.0, itmax=20, rtol=1e-9, cg_maxit=5000, cg_rtol=1e-10,
-                           use_graph=True, chunk=0, tile=0, lanes_per_row=0):
-        """Newton per mesh over nsteps load steps; returns (iters[nsteps][nmesh], relres[nmesh], BatchReport)."""
-        o = BatchOpts(nsteps, dt, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row)
+                           use_graph=True, chunk=0, tile=0, lanes_per_row=0, solver=BATCH_SOLVER_PCG):
+        """Newton per mesh over nsteps load steps; returns (iters[nsteps][nmesh], relres[nmesh], BatchReport).
+        solver: BATCH_SOLVER_PCG (Jacobi-PCG, inexact Newton) or BATCH_SOLVER_BAND (banded LDL^T per mesh)."""
+        o = BatchOpts(nsteps, dt, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row, solver)
         iters = np.zeros((nsteps, self.nmesh), dtype=np.int32)
         rel = np.zeros(self.nmesh)
         rep = BatchReport()
@@ -568,10 +572,10 @@ class Problem:
         return iters, rel, rep
 
     def batch_newton_step(self, lam, guess=None, rep=None, itmax=20, rtol=1e-9, cg_maxit=5000, cg_rtol=1e-10,
-                          use_graph=True, chunk=0, tile=0, lanes_per_row=0):
+                          use_graph=True, chunk=0, tile=0, lanes_per_row=0, solver=BATCH_SOLVER_PCG):
         """One load step of every mesh; guess: device tensor [nNode*3] (float64) or None.
         Returns (iters[nmesh], relres[nmesh], rep) with rep accumulated across calls."""
-        o = BatchOpts(1, 1.0, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row)
+        o = BatchOpts(1, 1.0, itmax, rtol, cg_maxit, cg_rtol, int(use_graph), chunk, tile, lanes_per_row, solver)
         iters = np.zeros(self.nmesh, dtype=np.int32)
         rel = np.zeros(self.nmesh)
         rep = BatchReport() if rep is None else rep

@@ -1,0 +1,414 @@
+// band.hip -- direct solve of the assembled displacement system by banded LDL^T on the GPU.
+//
+// The reference's plain method solves every Newton correction directly (full Newton with a direct
+// solve: SolverEigen::solve factors K by Eigen::SimplicialLDLT, SolverEigen.cpp:191-228).  Here the free dofs of each independent
+// diagonal block of K (the whole problem, or each mesh of a batch) are renumbered node by node
+// (reverse Cuthill-McKee or slices across an axis, whichever gives the smaller half-bandwidth),
+// which makes the block banded; the block is then factored as
+// L D L^T without pivoting (K is the symmetric tangent of the consistent mode; a zero or
+// non-finite pivot is reported) and solved, one workgroup per block:
+//
+//   factorisation, right-looking in panels of W columns: one wavefront factors the panel in LDS
+//   (columns scaled by their pivot, the rest of the panel updated column by column), then the
+//   workgroup applies the panel's rank-W update to the (b x b) trailing triangle in the band;
+//   solves, also in blocks of W rows: the W x W triangle by one wavefront in LDS, the coupling to
+//   the next b rows by the workgroup (forward), or per row of the block by one wavefront with a
+//   butterfly sum (backward).
+//
+// Every sum is taken in a fixed order, so the solution is bitwise reproducible.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "mpf_internal.hpp"
+
+namespace mpf {
+
+namespace {
+
+constexpr int kW = 16;      // panel width (columns per panel, rows per solve block)
+constexpr int kBS = 1024;   // threads per workgroup: 16 wavefronts, one per row of a backward block
+static_assert(kBS / 64 == kW, "one wavefront per panel row in the backward solve");
+
+struct BandArgs {
+  const int32_t* bstart;  // first position of each block [nblk + 1]
+  const int32_t* bw;      // half-bandwidth of each block
+  const int64_t* boff;    // band storage offset of each block
+  double* band;           // per block: column-major lower band, entry (r, c) at c * (b + 1) + (r - c)
+  double* y;              // permuted right-hand side, then the solution
+  int32_t* bad;           // set on a zero or non-finite pivot
+};
+
+__global__ void k_band_zero(double* __restrict__ a, int64_t n, int32_t* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+  if (i == 0) *bad = 0;
+}
+
+// Lower band of every block from the CSR rows (one lane per row), and the permuted right-hand side.
+__global__ void k_band_fill(int32_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                            const double* __restrict__ vals, const double* __restrict__ rhs,
+                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ pblk, BandArgs A) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int i = iperm[r];
+  const int blk = pblk[i];
+  const int s = A.bstart[blk], b = A.bw[blk];
+  double* L = A.band + A.boff[blk];
+  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+    const int j = iperm[colidx[k]];
+    if (j <= i) L[(size_t)(j - s) * (b + 1) + (i - j)] = vals[k];
+  }
+  A.y[i] = rhs[r];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Factor and solve one block per workgroup.  Dynamic LDS: the panel, (kW + bmax) x kW doubles,
+// column-major with row stride PR = kW + bmax, then kW doubles for a solve block.
+__global__ void __launch_bounds__(kBS) k_band_ldlt(BandArgs A, int32_t bmax) {
+  extern __shared__ double sP[];
+  const int PR = kW + bmax;
+  double* sy = sP + (size_t)PR * kW;
+  const int blk = blockIdx.x;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
+  double* L = A.band + A.boff[blk];
+  double* y = A.y + s0;
+  const int B1 = b + 1;
+
+  // ---- factorisation ----
+  for (int j0 = 0; j0 < n; j0 += kW) {
+    const int w = min(kW, n - j0);
+    for (int t = tid; t < w * B1; t += kBS) {  // panel (r, c), r - c <= b
+      const int c = t / B1, k = t - c * B1, r = c + k;
+      sP[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
+    }
+    __syncthreads();
+    if (wid == 0) {
+      for (int c = 0; c < w; ++c) {
+        const double d = sP[c * PR + c];
+        if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
+        const double inv = 1.0 / d;
+        for (int r = c + 1 + lane; r <= c + b; r += 64) sP[c * PR + r] *= inv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int cc = c + 1; cc < w; ++cc) {
+          const double f = d * sP[c * PR + cc];
+          for (int r = cc + lane; r <= c + b; r += 64) sP[cc * PR + r] -= sP[c * PR + r] * f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < w * B1; t += kBS) {
+      const int c = t / B1, k = t - c * B1, r = c + k;
+      if (j0 + r < n) L[(size_t)(j0 + c) * B1 + k] = sP[c * PR + r];
+    }
+    // trailing update of columns w .. w + b - 1 (panel-relative), rows c .. w - 1 + b:
+    // A(r, c) -= sum_k L(r, k) d_k L(c, k) over the panel's columns k with r - k <= b
+    for (int t = tid; t < b * b; t += kBS) {
+      const int cc = t / b, rr = t - cc * b;
+      if (rr < cc) continue;
+      const int c = w + cc, r = w + rr;
+      if (j0 + r >= n) continue;
+      double sum = 0.0;
+      for (int k = 0; k < w; ++k) {
+        if (r - k > b) continue;
+        sum += sP[k * PR + r] * sP[k * PR + k] * sP[k * PR + c];
+      }
+      L[(size_t)(j0 + c) * B1 + (r - c)] -= sum;
+    }
+    __syncthreads();
+  }
+
+  // ---- forward: L z = y (unit lower) ----
+  for (int j0 = 0; j0 < n; j0 += kW) {
+    const int w = min(kW, n - j0);
+    if (tid < w) sy[tid] = y[j0 + tid];
+    __syncthreads();
+    if (wid == 0) {
+      for (int c = 0; c < w; ++c) {
+        const int r = c + 1 + lane;
+        if (r < w && r - c <= b) sy[r] -= L[(size_t)(j0 + c) * B1 + (r - c)] * sy[c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+    __syncthreads();
+    if (tid < w) y[j0 + tid] = sy[tid];
+    for (int r = j0 + w + tid; r < min(n, j0 + w + b); r += kBS) {
+      double acc = y[r];
+      for (int k = 0; k < w; ++k) {
+        const int dist = r - (j0 + k);
+        if (dist <= b) acc -= L[(size_t)(j0 + k) * B1 + dist] * sy[k];
+      }
+      y[r] = acc;
+    }
+    __syncthreads();
+  }
+  // ---- D ----
+  for (int i = tid; i < n; i += kBS) y[i] /= L[(size_t)i * B1];
+  __syncthreads();
+  // ---- backward: L^T x = z ----
+  const int nb = (n + kW - 1) / kW;
+  for (int q = nb - 1; q >= 0; --q) {
+    const int j0 = q * kW, w = min(kW, n - j0);
+    if (wid < w) {  // row j0 + wid: minus the rows already solved beyond the block
+      const int k = j0 + wid;
+      double part = 0.0;
+      for (int r = j0 + w + lane; r <= min(n - 1, k + b); r += 64) part += L[(size_t)k * B1 + (r - k)] * y[r];
+      part = wave_sum(part);
+      if (lane == 0) sy[wid] = y[k] - part;
+    }
+    __syncthreads();
+    if (wid == 0) {
+      for (int c = w - 1; c >= 0; --c) {
+        const int r = lane;
+        if (r < c && c - r <= b) sy[r] -= L[(size_t)(j0 + r) * B1 + (c - r)] * sy[c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+    __syncthreads();
+    if (tid < w) y[j0 + tid] = sy[tid];
+    __syncthreads();
+  }
+}
+
+// du (CSR order) from the permuted solution; blocks whose flag is 0 (e.g. converged meshes) get 0.
+__global__ void k_band_scatter(int32_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ pblk,
+                               const double* __restrict__ y, const int32_t* __restrict__ active, int32_t astride,
+                               double* __restrict__ du) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool on = active == nullptr || active[(size_t)pblk[i] * astride] != 0;
+  du[perm[i]] = on ? y[i] : 0.0;
+}
+
+}  // namespace
+
+struct BandWork {
+  uint64_t key = 0;       // problem uid and block layout it was built for
+  int32_t nblk = 0, n = 0, bmax = 0;
+  int64_t nband = 0;
+  std::vector<int32_t> bw_h;
+  DevBuf<int32_t> perm, iperm, pblk, bstart, bw;
+  DevBuf<int64_t> boff;
+  DevBuf<double> band, y;
+  DevBuf<int32_t> bad;
+};
+
+void band_free(mpf_problem* p) {
+  delete p->band;
+  p->band = nullptr;
+}
+
+// Node order of each node range [node_off[m], node_off[m + 1]) expanded to free dofs (a node's dofs
+// in order), and the half-bandwidth of each block: of four candidate orders the one with the
+// smallest half-bandwidth — reverse Cuthill-McKee on the coupled-node graph (best on elongated or
+// irregular meshes) and the nodes sorted by position with each axis in turn slowest (slices
+// across that axis; best on block-shaped structured meshes).
+mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
+  const PatternBufs& P = p->pat;
+  if (!p->prepared) return MPF_ERR_STATE;
+  if (P.presDOF > 0) return MPF_ERR_ARG;  // displacement systems only (no pivoting)
+  const int nblk = (int)node_off.size() - 1;
+  const uint64_t key = p->uid * 1000003ull + (uint64_t)nblk * 7919ull + (uint64_t)node_off.back();
+  if (p->band && p->band->key == key) return MPF_OK;
+  if (!p->band) p->band = new BandWork();
+  BandWork& W = *p->band;
+  hipStream_t s = p->stream;
+  const int nNode = p->nNode;
+  std::vector<int32_t> nbptr(nNode + 1), nda(3 * (size_t)nNode);
+  std::vector<double> X(3 * (size_t)nNode);
+  MPF_HIP(hipMemcpyAsync(nbptr.data(), P.nbptr.ptr, nbptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipMemcpyAsync(nda.data(), P.nda.ptr, nda.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipMemcpyAsync(X.data(), p->X.ptr, X.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> nb(nbptr[nNode] > 0 ? nbptr[nNode] : 1);
+  MPF_HIP(hipMemcpy(nb.data(), P.nb.ptr, (size_t)nbptr[nNode] * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int n = P.dispDOF;
+  std::vector<int32_t> perm(n), iperm(n), pblk(n), bstart(nblk + 1, 0), bw(nblk, 0);
+  std::vector<int64_t> boff(nblk + 1, 0);
+  std::vector<int32_t> first(nNode, -1), last(nNode, -1), mark(nNode, 0), level;
+  auto deg = [&](int v) { return nbptr[v + 1] - nbptr[v]; };
+  auto nfree = [&](int v) {
+    return (nda[3 * (size_t)v] >= 0) + (nda[3 * (size_t)v + 1] >= 0) + (nda[3 * (size_t)v + 2] >= 0);
+  };
+  // half-bandwidth of a node order of [a, z)
+  auto bandwidth = [&](const std::vector<int32_t>& order, int a, int z) {
+    int q = 0;
+    for (int v : order) {
+      const int f = nfree(v);
+      first[v] = f ? q : -1;
+      last[v] = f ? q + f - 1 : -1;
+      q += f;
+    }
+    int b = 0;
+    for (int u = a; u < z; ++u) {
+      if (last[u] < 0) continue;
+      for (int k = nbptr[u]; k < nbptr[u + 1]; ++k) {
+        const int v = nb[k];
+        if (first[v] >= 0 && first[v] <= last[u]) b = std::max(b, last[u] - first[v]);
+      }
+    }
+    return b;
+  };
+  int pos = 0;
+  for (int m = 0; m < nblk; ++m) {
+    const int a = node_off[m], z = node_off[m + 1];
+    bstart[m] = pos;
+    // reverse Cuthill-McKee: per connected component, breadth-first order from a pseudo-peripheral
+    // start (the last node reached from the previous start, twice), neighbours in increasing
+    // degree; then reversed
+    std::vector<int32_t> best;
+    {
+      std::vector<int32_t> order;
+      for (int v0 = a; v0 < z; ++v0) {
+        if (mark[v0]) continue;
+        int start = v0;
+        for (int sweep = 0; sweep < 3; ++sweep) {
+          level.assign(1, start);
+          mark[start] = 2;
+          for (size_t h = 0; h < level.size(); ++h) {
+            const int u = level[h];
+            const size_t from = level.size();
+            for (int k = nbptr[u]; k < nbptr[u + 1]; ++k) {
+              const int v = nb[k];
+              if (v >= a && v < z && mark[v] == 0) { mark[v] = 2; level.push_back(v); }
+            }
+            std::stable_sort(level.begin() + from, level.end(), [&](int x, int y) { return deg(x) < deg(y); });
+          }
+          if (sweep < 2) {
+            for (int v : level) mark[v] = 0;
+            start = level.back();
+          }
+        }
+        for (int v : level) mark[v] = 1;
+        order.insert(order.end(), level.begin(), level.end());
+      }
+      std::reverse(order.begin(), order.end());
+      best = order;
+    }
+    int bbest = bandwidth(best, a, z);
+    // slice index of every node along each axis: the nodes sorted by that coordinate, a new slice
+    // wherever the gap to the previous node exceeds half the largest gap (on a jittered structured
+    // grid, exactly its grid lines)
+    std::vector<int64_t> qk(3 * (size_t)(z - a), 0);
+    for (int d = 0; d < 3; ++d) {
+      std::vector<int32_t> srt(z - a);
+      for (int v = a; v < z; ++v) srt[v - a] = v;
+      std::stable_sort(srt.begin(), srt.end(),
+                       [&](int x, int y) { return X[3 * (size_t)x + d] < X[3 * (size_t)y + d]; });
+      double gmax = 0.0;
+      for (size_t t = 1; t < srt.size(); ++t)
+        gmax = std::max(gmax, X[3 * (size_t)srt[t] + d] - X[3 * (size_t)srt[t - 1] + d]);
+      int64_t sl = 0;
+      for (size_t t = 0; t < srt.size(); ++t) {
+        if (t > 0 && X[3 * (size_t)srt[t] + d] - X[3 * (size_t)srt[t - 1] + d] > 0.5 * gmax) ++sl;
+        qk[3 * (size_t)(srt[t] - a) + d] = sl;
+      }
+    }
+    for (int ax = 0; ax < 3; ++ax) {  // slices across axis ax
+      std::vector<int32_t> order(z - a);
+      for (int v = a; v < z; ++v) order[v - a] = v;
+      const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+      std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int64_t* px = &qk[3 * (size_t)(x - a)];
+        const int64_t* py = &qk[3 * (size_t)(y - a)];
+        if (px[ax] != py[ax]) return px[ax] < py[ax];
+        if (px[a1] != py[a1]) return px[a1] < py[a1];
+        return px[a2] < py[a2];
+      });
+      const int bb = bandwidth(order, a, z);
+      if (bb < bbest) { bbest = bb; best = order; }
+    }
+    for (int v : best) {
+      for (int d = 0; d < 3; ++d) {
+        const int r = nda[3 * (size_t)v + d];
+        if (r < 0) continue;
+        perm[pos] = r;
+        iperm[r] = pos;
+        pblk[pos] = m;
+        ++pos;
+      }
+    }
+    bw[m] = bbest;
+    boff[m + 1] = boff[m] + (int64_t)(pos - bstart[m]) * (bbest + 1);
+  }
+  bstart[nblk] = pos;
+  if (pos != n) return MPF_ERR_STATE;  // every free dof belongs to one block
+  W.nblk = nblk;
+  W.n = n;
+  W.bmax = nblk > 0 ? *std::max_element(bw.begin(), bw.end()) : 0;
+  W.nband = boff[nblk];
+  W.bw_h = bw;
+  const int lds = (int)(((size_t)(kW + W.bmax) * kW + kW) * sizeof(double));
+  if (lds > 64 * 1024) return MPF_ERR_ARG;  // half-bandwidth beyond the panel's LDS (about 490 dofs)
+  MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
+  MPF_TRY(W.iperm.upload(iperm.data(), iperm.size(), s));
+  MPF_TRY(W.pblk.upload(pblk.data(), pblk.size(), s));
+  MPF_TRY(W.bstart.upload(bstart.data(), bstart.size(), s));
+  MPF_TRY(W.bw.upload(bw.data(), bw.size(), s));
+  MPF_TRY(W.boff.upload(boff.data(), boff.size(), s));
+  MPF_TRY(W.band.alloc(W.nband > 0 ? (size_t)W.nband : 1));
+  MPF_TRY(W.y.alloc(n > 0 ? (size_t)n : 1));
+  MPF_TRY(W.bad.alloc(1));
+  MPF_HIP(hipStreamSynchronize(s));
+  W.key = key;
+  return MPF_OK;
+}
+
+// Enqueue: band from the assembled K, factor and solve every block, du = K^-1 rhs per block (0 for
+// blocks whose active[m * astride] is 0; active may be null).  Allocates nothing (graph-capturable).
+mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t astride) {
+  BandWork* W = p->band;
+  if (!W) return MPF_ERR_STATE;
+  hipStream_t s = p->stream;
+  const PatternBufs& P = p->pat;
+  BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
+  hipLaunchKernelGGL(k_band_zero, dim3(grid_for((size_t)W->nband + 1, 256)), dim3(256), 0, s, W->band.ptr, W->nband,
+                     W->bad.ptr);
+  hipLaunchKernelGGL(k_band_fill, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, P.rowptr.ptr, P.colidx.ptr,
+                     p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A);
+  const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
+  hipLaunchKernelGGL(k_band_ldlt, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax);
+  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
+                     W->y.ptr, active, astride, p->du.ptr);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+// Single problem (mpf_solve with MPF_KSP_BAND): one block holding every node.
+mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep) {
+  const auto t0 = std::chrono::steady_clock::now();
+  MPF_TRY(p->du.alloc(p->pat.dispDOF > 0 ? p->pat.dispDOF : 1));
+  const std::vector<int32_t> off{0, p->nNode};
+  MPF_TRY(band_prepare(p, off));
+  hipStream_t s = p->stream;
+  MPF_TRY(band_enqueue_solve(p, nullptr, 0));
+  int32_t bad = 0;
+  MPF_HIP(hipMemcpyAsync(&bad, p->band->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+  rep->iterations = 1;
+  rep->converged = bad ? 0 : 1;
+  rep->relres = 0.0;
+  rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return bad ? MPF_ERR_DIVERGED : MPF_OK;
+}
+
+int32_t* band_bad_flag(mpf_problem* p) { return p->band ? p->band->bad.ptr : nullptr; }
+
+}  // namespace mpf

@@ -50,13 +50,17 @@ struct BatchWork {
   DevBuf<int32_t> cnt;  // [4]: CG-active meshes, Newton-active meshes, diverged meshes, negative Jacobian
   int32_t* cnt_h = nullptr;  // pinned host mirror
   hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr, g_r0 = nullptr;
+  hipGraphExec_t g_band = nullptr;  // direct solve of every mesh (banded LDL^T)
   double g_lam = 0.0;
+  int32_t g_solver = -1;            // linear solver the graphs were captured for
   ~BatchWork() { release_graphs(); if (cnt_h) (void)hipHostFree(cnt_h); }
   void release_graphs() {
     for (auto& g : g_asm) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     for (auto& g : g_upd) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (g_chunk) { (void)hipGraphExecDestroy(g_chunk); g_chunk = nullptr; }
     if (g_r0) { (void)hipGraphExecDestroy(g_r0); g_r0 = nullptr; }
+    if (g_band) { (void)hipGraphExecDestroy(g_band); g_band = nullptr; }
+    g_solver = -1;
   }
 };
 
@@ -383,8 +387,9 @@ mpf_status enqueue_cg_iter(mpf_problem* p, BatchWork& B, int32_t maxit) {
   return MPF_OK;
 }
 
-// assembly + residual norms + Newton test + CG start
-mpf_status enqueue_newton_head(mpf_problem* p, BatchWork& B, bool first, double lam, double rtol, double eta_min) {
+// assembly + residual norms + Newton test (+ CG start)
+mpf_status enqueue_newton_head(mpf_problem* p, BatchWork& B, bool first, double lam, double rtol, double eta_min,
+                               bool cg_start = true) {
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   const int64_t n = P.dispDOF;
@@ -394,6 +399,10 @@ mpf_status enqueue_newton_head(mpf_problem* p, BatchWork& B, bool first, double 
   hipLaunchKernelGGL(k_b_dot, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, p->rhs.ptr, p->rhs.ptr, B.part.ptr, 0);
   hipLaunchKernelGGL(k_b_newton, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr,
                      (int32_t)first, rtol, eta_min, B.cnt.ptr);
+  if (!cg_start) {
+    MPF_HIP(hipGetLastError());
+    return MPF_OK;
+  }
   hipLaunchKernelGGL(k_b_diag, dim3(grid_for(n, 256)), dim3(256), 0, s, n, P.rowptr.ptr, P.colidx.ptr, p->vals.ptr,
                      B.dinv.ptr);
   hipLaunchKernelGGL(k_b_start, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, p->rhs.ptr, B.dinv.ptr,
@@ -577,9 +586,10 @@ mpf_status batch_autotune(mpf_problem* p, int32_t* T_out, int32_t* LPR_out, doub
 
 static mpf_status batch_configure(mpf_problem* p, const mpf_batch_opts* o, mpf_batch_report* rep) {
   if (!p->batch || !p->prepared || !o || !rep) return MPF_ERR_STATE;
-  if (o->itmax <= 0 || o->cg_maxit <= 0) return MPF_ERR_ARG;
+  if (o->itmax <= 0 || o->cg_maxit <= 0 || o->solver < 0 || o->solver > 1) return MPF_ERR_ARG;
   MPF_TRY(batch_prepare(p));
   BatchWork& B = *p->batch;
+  if (o->solver == 1) MPF_TRY(band_prepare(p, B.node_off));
   if (o->chunk > 0 && o->chunk != B.chunk) {
     B.chunk = o->chunk;
     B.release_graphs();
@@ -626,23 +636,31 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
   BatchWork& B = *p->batch;
   hipStream_t s = p->stream;
   const bool G = o->use_graph != 0;
+  const bool band = o->solver == 1;  // exact corrections by banded LDL^T per mesh, else Jacobi-PCG
   const size_t n3 = (size_t)p->nNode * 3;
   auto t_all = std::chrono::steady_clock::now();
   // saveSolution, setBoundaryConditions
   MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
   MPF_TRY(mpf_set_bc(p, lam));
   MPF_TRY(reset_mesh_flags(p, B));
-  if (G && (B.g_lam != lam || !B.g_asm[0])) {
+  auto head = [&](int f) { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol, !band); };
+  auto direct = [&] { return band_enqueue_solve(p, B.mf.ptr + kNA, kMF); };
+  if (G && (B.g_lam != lam || !B.g_asm[0] || B.g_solver != o->solver)) {
     for (int f = 0; f < 2; ++f) {
-      MPF_TRY(capture(p, &B.g_asm[f], [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+      MPF_TRY(capture(p, &B.g_asm[f], [&] { return head(f); }));
       MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
     }
-    MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
-      for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
-      return MPF_OK;
-    }));
+    if (band) {
+      MPF_TRY(capture(p, &B.g_band, direct));
+    } else {
+      MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
+        for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
+        return MPF_OK;
+      }));
+    }
     MPF_TRY(capture(p, &B.g_r0, [&] { return enqueue_r0(p, B, lam); }));
     B.g_lam = lam;
+    B.g_solver = o->solver;
   }
   if (guess) {
     auto ta = std::chrono::steady_clock::now();
@@ -661,7 +679,7 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
   for (int iter = 1; iter <= o->itmax + 1; ++iter) {
     const int f = (iter == 1 && !guess) ? 1 : 0;
     auto ta = std::chrono::steady_clock::now();
-    MPF_TRY(launch(p, B.g_asm[f], G, [&] { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol); }));
+    MPF_TRY(launch(p, B.g_asm[f], G, [&] { return head(f); }));
     ++rep->graph_launches;
     MPF_TRY(read_counts(p, B));
     rep->seconds_assembly += secs(ta);
@@ -684,8 +702,16 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
     if (iter > o->itmax) break;
     ++rep->newton_iters;
     auto ts = std::chrono::steady_clock::now();
+    if (band) {
+      MPF_TRY(launch(p, B.g_band, G, direct));
+      ++rep->graph_launches;
+      int32_t pbad = 0;
+      MPF_HIP(hipMemcpyAsync(&pbad, band_bad_flag(p), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      MPF_HIP(hipStreamSynchronize(s));
+      if (pbad) { result = MPF_ERR_DIVERGED; break; }
+    }
     int its = 0;
-    while (B.cnt_h[0] > 0 && its < o->cg_maxit) {
+    while (!band && B.cnt_h[0] > 0 && its < o->cg_maxit) {
       MPF_TRY(launch(p, B.g_chunk, G, [&]() -> mpf_status {
         for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
         return MPF_OK;

@@ -632,6 +632,7 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
 }
 
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep) {
+  if (o->ksp == MPF_KSP_BAND) return band_solve(p, rep);
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
   if (o->ksp == MPF_KSP_GMRES && o->restart > kMaxRestart) return MPF_ERR_ARG;

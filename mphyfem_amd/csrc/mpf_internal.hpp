@@ -151,6 +151,7 @@ struct PatternBufs {
 
 struct KrylovWork;  // krylov.hip
 struct BatchWork;   // batch.hip
+struct BandWork;    // band.hip
 
 }  // namespace mpf
 
@@ -188,6 +189,7 @@ struct mpf_problem {
   mpf::DevBuf<double> GS;
   mpf::DevBuf<double> scratch;  // reductions (norms)
   mpf::BatchWork* batch = nullptr;  // independent meshes solved together (batch.hip)
+  mpf::BandWork* band = nullptr;    // banded LDL^T direct solver (band.hip)
 };
 
 namespace mpf {
@@ -208,6 +210,12 @@ mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* 
 mpf_status batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
                              int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep);
 mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);  // krylov.hip
+// band.hip
+mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off);
+mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t astride);
+mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep);
+int32_t* band_bad_flag(mpf_problem* p);
+void band_free(mpf_problem* p);
 void krylov_free(mpf_problem* p);
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);
 }  // namespace mpf

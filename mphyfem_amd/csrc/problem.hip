@@ -485,6 +485,7 @@ mpf_status mpf_problem_destroy(mpf_problem* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   krylov_free(p);
   batch_free(p);
+  band_free(p);
   hostX_store().erase(p);
   hipStream_t s = p->stream;
   delete p;
@@ -676,8 +677,8 @@ mpf_status mpf_rhs_norm(mpf_problem* p, double* norm) {
 mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep) {
   if (!p || !o || !rep) return MPF_ERR_ARG;
   if (!p->assembled) return MPF_ERR_STATE;  // SolverPetsc::factoriseAndSolve returns -1 (SolverPetsc.cpp:321-323)
-  if (o->ksp != MPF_KSP_CG && o->ksp != MPF_KSP_GMRES) return MPF_ERR_ARG;
-  if (o->pc < MPF_PC_NONE || o->pc > MPF_PC_BJACOBI3) return MPF_ERR_ARG;
+  if (o->ksp != MPF_KSP_CG && o->ksp != MPF_KSP_GMRES && o->ksp != MPF_KSP_BAND) return MPF_ERR_ARG;
+  if (o->ksp != MPF_KSP_BAND && (o->pc < MPF_PC_NONE || o->pc > MPF_PC_BJACOBI3)) return MPF_ERR_ARG;
   std::memset(rep, 0, sizeof(*rep));
   MPF_TRY(krylov_solve(p, o, rep));
   p->solved = true;

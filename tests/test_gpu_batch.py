@@ -49,6 +49,28 @@ def test_batched_newton_matches_each_mesh_alone(mpf, orc, gpu):
         assert normwise_rel(u[3 * lo:3 * hi], uo) < 1e-9, m
 
 
+def test_batched_band_newton_matches_each_mesh_alone(mpf, orc, gpu):
+    """Batched Newton with exact corrections (banded LDL^T per mesh, one workgroup each): every
+    mesh against the oracle's Newton with its direct solve; graph replay equal to direct launches."""
+    B, P = _batch(mpf)
+    iters, rel, rep = P.batch_newton_solve(nsteps=2, dt=0.5, itmax=25, rtol=1e-11, solver=mpf.BATCH_SOLVER_BAND)
+    assert rep.steps_done == 2 and np.all(rel < 1e-11), (rel, iters)
+    assert rep.cg_iters_total == 0
+    u, _ = P.state()
+    for m in range(len(EDGES)):
+        O = _oracle_mesh(orc, mpf, B, m)
+        st, oit, orel = O.newton(2, 0.5, solver=0, rtol=1e-11, itmax=25)
+        assert st == 0
+        uo, _ = O.state()
+        lo, hi = B.node_off[m], B.node_off[m + 1]
+        assert normwise_rel(u[3 * lo:3 * hi], uo) < 1e-9, m
+    _, Pe = _batch(mpf)
+    ie, re, _ = Pe.batch_newton_solve(nsteps=2, dt=0.5, itmax=25, rtol=1e-11, solver=mpf.BATCH_SOLVER_BAND,
+                                      use_graph=False)
+    ue, _ = Pe.state()
+    assert np.array_equal(iters, ie) and np.array_equal(u, ue)
+
+
 def test_batched_graph_replay_equals_direct_launches(mpf, gpu):
     _, Pg = _batch(mpf)
     _, Pe = _batch(mpf)

@@ -365,6 +365,39 @@ def test_krylov_vs_oracle(mpf, orc, gpu):
         assert np.linalg.norm(O.spmv(vals, du) - rhs) / np.linalg.norm(rhs) < 2e-11
 
 
+def test_band_direct_solve_vs_oracle(mpf, orc, gpu):
+    """Banded LDL^T on the GPU (reverse Cuthill-McKee renumbering) against the oracle's banded
+    LDL^T on the same assembled system, and bitwise repeatable."""
+    n = 8
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
+    X, conn, P = _hex_problem(mpf, n, m)
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    for Q in (P, O):
+        Q.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    vals, rhs = P.system()
+    x_ref, st = O.direct_solve(vals, rhs)
+    assert st == 0
+    r = P.factorise_and_solve(ksp=mpf.KSP_BAND)
+    assert r.converged
+    du = P.solution()
+    assert normwise_rel(du, x_ref) < 1e-10
+    assert np.linalg.norm(O.spmv(vals, du) - rhs) / np.linalg.norm(rhs) < 1e-12
+    P.factorise_and_solve(ksp=mpf.KSP_BAND)
+    assert np.array_equal(P.solution(), du)
+
+
+def test_band_rejects_mixed_system(mpf, gpu):
+    X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=2)
+    P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
+    apply_bcs(P, tet_compress_bcs(2))
+    P.prepare_matrix_pattern()
+    P.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    with pytest.raises(mpf.MpfError):
+        P.factorise_and_solve(ksp=mpf.KSP_BAND)
+
+
 def test_gmres_mixed_vs_oracle(mpf, orc, gpu):
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
@@ -402,6 +435,22 @@ def test_newton_cfg1_vs_oracle(mpf, orc, gpu):
     uo, _ = O.state()
     assert normwise_rel(ud, uo) < 1e-9
     assert abs(ud[3 * M_face1(mpf, n)].mean() - 0.1) < 1e-12
+
+
+def test_newton_cfg1_direct_vs_oracle(mpf, orc, gpu):
+    """The plain method as specified: full Newton with a direct solve (banded LDL^T on the GPU)."""
+    n = 10
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=mpf.TANGENT_CONSISTENT)
+    X, conn, P = _hex_problem(mpf, n, m)
+    opts = P.newton_opts(nsteps=10, dt=0.1, itmax=25, rtol=1e-9, ksp=mpf.KSP_BAND)
+    iters, rep = P.solve_with_newton_raphson(opts)
+    assert rep.steps_done == 10 and np.all(iters <= 6), iters
+    O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    st, oit, rel = O.newton(10, 0.1, solver=0, rtol=1e-9, itmax=25)
+    assert st == 0
+    ud, _ = P.state()
+    uo, _ = O.state()
+    assert normwise_rel(ud, uo) < 1e-9
 
 
 def M_face1(mpf, n):
