@@ -107,6 +107,11 @@ def read_traffic(workload: str):
 SOLVER = dict(itmax=25, rtol=1e-9, cg_maxit=20000, cg_rtol=1e-10, chunk=32)
 
 
+def solver_opts(M, a) -> dict:
+    """Batched Newton options: Jacobi-PCG (inexact Newton) or the banded LDL^T direct solve per mesh."""
+    return dict(SOLVER, solver=M.BATCH_SOLVER_BAND if a.solver == "band" else M.BATCH_SOLVER_PCG)
+
+
 def bench_batch(M, a) -> int:
     """One load step (lambda = 1) of every block of the batched workload, converged per block to
     ||R||/||R0|| < 1e-9, after the tile autotuner."""
@@ -122,9 +127,10 @@ def bench_batch(M, a) -> int:
     T, L, ms_it = P.batch_autotune()
     log(f"autotune: tile {T}, {L} lanes per row, {ms_it:.3f} ms per CG iteration")
     torch.cuda.synchronize()
-    iters, rel, rep = P.batch_newton_solve(nsteps=1, dt=1.0, **SOLVER)
+    iters, rel, rep = P.batch_newton_solve(nsteps=1, dt=1.0, **solver_opts(M, a))
     log(f"newton: {rep.newton_iters} batch iterations, max CG {rep.cg_iters_max}, {rep.seconds:.3f} s")
-    out = {"metric": "s per converged load step (256 Hex8 blocks, batched Newton + Jacobi-PCG)",
+    lin = "banded LDL^T" if a.solver == "band" else "Jacobi-PCG"
+    out = {"metric": f"s per converged load step ({a.nmesh} Hex8 blocks, batched Newton + {lin})",
            "value": round(rep.seconds, 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(rep.seconds * 1e3, 2), "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -145,13 +151,14 @@ def bench_learned(M, a) -> int:
     from mphyfem_amd import learned
 
     r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs,
-                           residual_epochs=a.residual_epochs, **SOLVER)
+                           residual_epochs=a.residual_epochs, **solver_opts(M, a))
     out = {"metric": "s per converged load step, learned vs plain Newton start (held-out cantilevers)",
            "value": round(r["learned"]["seconds"] / max(r["steps"] - 1, 1), 4), "unit": "s",
            "n_gpus": 1, "steps": r["steps"], "warmup": 0, "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 solve, bf16/fp32 network", "data": "synthetic",
            "config": {"workload": "Hex8 cantilevers H=8 W=4 L=round(U[4,12]*8), mu log-U[1,100], nu=0.3, "
-                                  "tip shear ramped over 20 steps, seed 4"},
+                                  "tip shear ramped over 20 steps, seed 4",
+                      "linear_solver": "banded LDL^T per beam" if a.solver == "band" else "Jacobi-PCG per beam"},
            "learned": r}
     print(json.dumps(out), flush=True)
     return 0
@@ -216,6 +223,9 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--residual-epochs", type=int, default=0,
                     help="learned: extra epochs on the linearised start residual")
+    ap.add_argument("--solver", choices=["pcg", "band"], default=None,
+                    help="batch/learned: per-mesh linear solve, Jacobi-PCG (inexact Newton; batch default) or "
+                         "banded LDL^T (exact Newton corrections, the reference's plain method; learned default)")
     ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
     ap.add_argument("--krylov-maxit", type=int, default=300)
     ap.add_argument("--no-krylov", action="store_true")
@@ -242,6 +252,8 @@ def main(argv=None) -> int:
         return 1
     torch.cuda.init()
     wl = a.workload
+    if a.solver is None:
+        a.solver = "band" if wl == "learned" else "pcg"
     if wl == "batch":
         return bench_batch(M, a)
     if wl == "learned":

@@ -7,6 +7,7 @@ torch only holds device memory for the trajectories.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -80,10 +81,14 @@ class StepLog:
     newton: list = field(default_factory=list)     # per step: batch Newton iterations (slowest mesh)
 
 
+def _progress(msg: str):
+    print(f"[learned] {msg}", file=sys.stderr, flush=True)
+
+
 def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTEPS, store=None, **solver):
     """Ramp the load over nsteps with Newton started from: 'plain' the previous converged solution,
     'extrap' the linear extrapolation, 'learned' the network's guess.  store: optional device tensor
-    [nsteps + 1, nNode * 3] receiving u_0 .. u_nsteps."""
+    [nsteps + 1, nNode * 3] receiving u_0 .. u_nsteps.  One progress line per load step on stderr."""
     import torch
 
     n3 = 3 * P.nNode
@@ -113,6 +118,8 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
         log.iters.append(iters.copy())
         log.newton.append(rep.newton_iters)
         log.start_rel.append(rep.start_relres_mean)
+        _progress(f"{mode} step {k + 1}/{nsteps}: {rep.newton_iters} batch Newton iterations, "
+                  f"{iters.mean():.2f} assemblies per mesh, {log.seconds[-1]:.3f} s")
         hist[0].copy_(hist[1])
         P.copy_state_device(hist[1])
         if store is not None:
@@ -145,7 +152,7 @@ def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, resid
         losses.append(tot / (nsteps - 1))
         if log_every and (ep % log_every == 0 or ep == total - 1 or ep == epochs):
             kind = "start residual^2" if res else "displacement"
-            print(f"[learned] epoch {ep}: {kind} loss {losses[-1]:.4e}", flush=True)
+            _progress(f"epoch {ep}: {kind} loss {losses[-1]:.4e}")
     return losses
 
 
