@@ -1,19 +1,18 @@
 // band.hip -- direct solve of the assembled displacement system by banded LDL^T on the GPU.
 //
 // The reference's plain method solves every Newton correction directly (full Newton with a direct
-// solve: SolverEigen::solve factors K by Eigen::SimplicialLDLT, SolverEigen.cpp:191-228).  Here the free dofs of each independent
-// diagonal block of K (the whole problem, or each mesh of a batch) are renumbered node by node
-// (reverse Cuthill-McKee or slices across an axis, whichever gives the smaller half-bandwidth),
-// which makes the block banded; the block is then factored as
-// L D L^T without pivoting (K is the symmetric tangent of the consistent mode; a zero or
-// non-finite pivot is reported) and solved, one workgroup per block:
+// solve: SolverEigen::solve factors K by Eigen::SimplicialLDLT, SolverEigen.cpp:191-228).  Here the
+// free dofs of each independent diagonal block of K (the whole problem, or each mesh of a batch) are
+// renumbered node by node (reverse Cuthill-McKee or slices across an axis, whichever gives the
+// smaller half-bandwidth b), which makes the block banded; the block is then factored as L D L^T
+// without pivoting (K is the symmetric tangent of the consistent mode; a zero or non-finite pivot is
+// reported) and solved, one workgroup per block and kernel:
 //
-//   factorisation, right-looking in panels of W columns: one wavefront factors the panel in LDS
-//   (columns scaled by their pivot, the rest of the panel updated column by column), then the
-//   workgroup applies the panel's rank-W update to the (b x b) trailing triangle in the band;
-//   solves, also in blocks of W rows: the W x W triangle by one wavefront in LDS, the coupling to
-//   the next b rows by the workgroup (forward), or per row of the block by one wavefront with a
-//   butterfly sum (backward).
+//   k_band_factor: right-looking in panels of kW columns -- the kW x kW diagonal block in one
+//   wavefront's registers, the panel's other rows one thread each, the rank-kW update of the b x b
+//   trailing triangle on the matrix cores (FP64 MFMA);
+//   k_band_solve: forward and backward substitution in blocks of kW rows, the unit triangle in one
+//   wavefront's registers, the coupling to the next b rows by the whole workgroup.
 //
 // Every sum is taken in a fixed order, so the solution is bitwise reproducible.
 
@@ -30,6 +29,10 @@ namespace {
 
 constexpr int kW = 16;      // panel width (columns per panel, rows per solve block)
 constexpr int kBS = 1024;   // threads per workgroup: 16 wavefronts, one per row of a backward block
+#ifndef MPF_BAND_TILES
+#define MPF_BAND_TILES 2
+#endif
+constexpr int kNH = MPF_BAND_TILES;  // trailing-update tiles per wavefront and round (old values loaded first)
 static_assert(kBS / 64 == kW, "one wavefront per panel row in the backward solve");
 
 struct BandArgs {
@@ -70,20 +73,43 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Factor and solve one block per workgroup.  Dynamic LDS: the panel, (kW + bmax) x kW doubles,
-// column-major with row stride PR = kW + bmax, then kW doubles for a solve block.
-__global__ void __launch_bounds__(kBS) k_band_ldlt(BandArgs A, int32_t bmax) {
+// v from lane l of the wavefront (l uniform)
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// Factor one block per workgroup (blocks whose active[blk * astride] is 0 are skipped; active may
+// be null).  Dynamic LDS: the panel, (kW + bmax) x kW doubles, column-major with row stride
+// PR = kW + bmax, then kW pivots.
+//
+// Panel step at column j0 (w = min(kW, n - j0) columns, panel-relative rows 0 .. w - 1 + b):
+//   (1) the w x w diagonal block is factored by wavefront 0 in registers (lane i holds row i;
+//       the pivot column is read across lanes);
+//   (2) every row r >= w of the panel solves u_c = A(r, c) - sum_{k<c} u_k L(c, k),
+//       L(r, c) = u_c / d_c on its own (one thread per row);
+//   (3) the trailing b x b window takes A(r, c') -= sum_k L(r, k) d_k L(c', k) for r >= c',
+//       in 16 x 16 tiles of the lower triangle on the matrix cores (v_mfma_f64_16x16x4f64, k = 16
+//       as 4 MFMAs), kNH tiles per wavefront at a time with their old values loaded first.
+__global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
+                                                     int32_t astride) {
   extern __shared__ double sP[];
-  const int PR = kW + bmax;
-  double* sy = sP + (size_t)PR * kW;
   const int blk = blockIdx.x;
+  if (active && !active[(size_t)blk * astride]) return;
+  const int PR = kW + bmax;
+  double* sd = sP + (size_t)PR * kW;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
   double* L = A.band + A.boff[blk];
-  double* y = A.y + s0;
   const int B1 = b + 1;
+  const int T = (b + 15) >> 4;  // trailing tiles per side
+  const int NT = T * (T + 1) / 2;
+  constexpr int NW = kBS / 64;
 
-  // ---- factorisation ----
   for (int j0 = 0; j0 < n; j0 += kW) {
     const int w = min(kW, n - j0);
     for (int t = tid; t < w * B1; t += kBS) {  // panel (r, c), r - c <= b
@@ -91,62 +117,133 @@ __global__ void __launch_bounds__(kBS) k_band_ldlt(BandArgs A, int32_t bmax) {
       sP[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
     }
     __syncthreads();
-    if (wid == 0) {
-      for (int c = 0; c < w; ++c) {
-        const double d = sP[c * PR + c];
-        if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
-        const double inv = 1.0 / d;
-        for (int r = c + 1 + lane; r <= c + b; r += 64) sP[c * PR + r] *= inv;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (int cc = c + 1; cc < w; ++cc) {
-          const double f = d * sP[c * PR + cc];
-          for (int r = cc + lane; r <= c + b; r += 64) sP[cc * PR + r] -= sP[c * PR + r] * f;
+    if (wid == 0) {  // (1) diagonal block
+      double a[kW];
+#pragma unroll
+      for (int c = 0; c < kW; ++c) a[c] = (lane < w && c < w && c <= lane && lane - c <= b) ? sP[c * PR + lane] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kW; ++c) {
+        if (c < w) {
+          const double d = lane_value(a[c], c);
+          if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
+          const double inv = 1.0 / d;
+#pragma unroll
+          for (int cc = c + 1; cc < kW; ++cc) {
+            const double v = lane_value(a[c], cc);  // A(cc, c)
+            if (lane >= cc) a[cc] -= a[c] * v * inv;
+          }
+          if (lane > c) a[c] *= inv;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       }
+#pragma unroll
+      for (int c = 0; c < kW; ++c) {
+        if (lane < w && c <= lane && lane - c <= b) sP[c * PR + lane] = a[c];  // L(lane, c), d on the diagonal
+        if (lane == c && c < w) sd[c] = a[c];
+      }
+    }
+    __syncthreads();
+    for (int r = w + tid; r < w + b; r += kBS) {  // (2) rows below the diagonal block
+      if (j0 + r >= n) break;
+      double u[kW];  // u_c = L(r, c) d_c
+#pragma unroll
+      for (int c = 0; c < kW; ++c) {
+        u[c] = 0.0;
+        if (c < w) {
+          double v = r - c <= b ? sP[c * PR + r] : 0.0;
+#pragma unroll
+          for (int k = 0; k < c; ++k)
+            if (c - k <= b) v -= u[k] * sP[k * PR + c];
+          u[c] = v;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kW; ++c)
+        if (c < w && r - c <= b) sP[c * PR + r] = u[c] / sd[c];
     }
     __syncthreads();
     for (int t = tid; t < w * B1; t += kBS) {
       const int c = t / B1, k = t - c * B1, r = c + k;
       if (j0 + r < n) L[(size_t)(j0 + c) * B1 + k] = sP[c * PR + r];
     }
-    // trailing update of columns w .. w + b - 1 (panel-relative), rows c .. w - 1 + b:
-    // A(r, c) -= sum_k L(r, k) d_k L(c, k) over the panel's columns k with r - k <= b
-    for (int t = tid; t < b * b; t += kBS) {
-      const int cc = t / b, rr = t - cc * b;
-      if (rr < cc) continue;
-      const int c = w + cc, r = w + rr;
-      if (j0 + r >= n) continue;
-      double sum = 0.0;
-      for (int k = 0; k < w; ++k) {
-        if (r - k > b) continue;
-        sum += sP[k * PR + r] * sP[k * PR + k] * sP[k * PR + c];
+    // (3) trailing update, tiles (ti >= tj) of panel-relative rows / columns w + 16 t ..
+    const int i16 = lane & 15, k4 = lane >> 4;
+    for (int q0 = wid; q0 < NT; q0 += kNH * NW) {
+      int R0[kNH], C0[kNH];
+      double old[kNH][4];
+#pragma unroll
+      for (int h = 0; h < kNH; ++h) {
+        const int q = q0 + h * NW;
+        int ti = 0;
+        while ((ti + 1) * (ti + 2) / 2 <= q) ++ti;
+        R0[h] = w + 16 * ti;
+        C0[h] = w + 16 * (q - ti * (ti + 1) / 2);
+        const int c = C0[h] + i16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {  // lane holds D(4 v + lane / 16, lane % 16) in acc[v]
+          const int r = R0[h] + 4 * v + k4;
+          old[h][v] = (q < NT && r >= c && r < w + b && j0 + r < n) ? L[(size_t)(j0 + c) * B1 + (r - c)] : 0.0;
+        }
       }
-      L[(size_t)(j0 + c) * B1 + (r - c)] -= sum;
+#pragma unroll
+      for (int h = 0; h < kNH; ++h) {
+        if (q0 + h * NW >= NT) break;
+        double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < kW / 4; ++s) {
+          const int k = 4 * s + k4;
+          const int ra = R0[h] + i16, cb = C0[h] + i16;
+          const double av = (k < w && ra < w + b && ra - k <= b) ? sP[k * PR + ra] : 0.0;
+          const double bv = (k < w && cb < w + b && cb - k <= b) ? sP[k * PR + cb] * sd[k] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+        const int c = C0[h] + i16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = R0[h] + 4 * v + k4;
+          if (r >= c && r < w + b && j0 + r < n) L[(size_t)(j0 + c) * B1 + (r - c)] = old[h][v] - acc[v];
+        }
+      }
     }
     __syncthreads();
   }
+}
+
+// Solve L D L^T x = y in place for one block per workgroup (skipped like the factorisation), in
+// blocks of kW rows: the kW x kW unit triangle by wavefront 0 with the block's L in registers and
+// the solved values read across lanes; the coupling to the next b rows by the workgroup (forward)
+// or per row of the block by one wavefront with a butterfly sum (backward).
+__global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
+  __shared__ double sy[kW];
+  const int blk = blockIdx.x;
+  if (active && !active[(size_t)blk * astride]) return;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
+  const double* L = A.band + A.boff[blk];
+  double* y = A.y + s0;
+  const int B1 = b + 1;
 
   // ---- forward: L z = y (unit lower) ----
   for (int j0 = 0; j0 < n; j0 += kW) {
     const int w = min(kW, n - j0);
-    if (tid < w) sy[tid] = y[j0 + tid];
-    __syncthreads();
     if (wid == 0) {
-      for (int c = 0; c < w; ++c) {
-        const int r = c + 1 + lane;
-        if (r < w && r - c <= b) sy[r] -= L[(size_t)(j0 + c) * B1 + (r - c)] * sy[c];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      double l[kW];  // lane r: L(r, c), c < r
+#pragma unroll
+      for (int c = 0; c < kW; ++c)
+        l[c] = (lane < w && c < lane && lane - c <= b) ? L[(size_t)(j0 + c) * B1 + (lane - c)] : 0.0;
+      double z = lane < w ? y[j0 + lane] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kW; ++c) {
+        if (c < w) {
+          const double zc = lane_value(z, c);
+          if (lane > c) z -= l[c] * zc;
+        }
+      }
+      if (lane < w) {
+        sy[lane] = z;
+        y[j0 + lane] = z;
       }
     }
     __syncthreads();
-    if (tid < w) y[j0 + tid] = sy[tid];
     for (int r = j0 + w + tid; r < min(n, j0 + w + b); r += kBS) {
       double acc = y[r];
       for (int k = 0; k < w; ++k) {
@@ -173,21 +270,24 @@ __global__ void __launch_bounds__(kBS) k_band_ldlt(BandArgs A, int32_t bmax) {
     }
     __syncthreads();
     if (wid == 0) {
-      for (int c = w - 1; c >= 0; --c) {
-        const int r = lane;
-        if (r < c && c - r <= b) sy[r] -= L[(size_t)(j0 + r) * B1 + (c - r)] * sy[c];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      double l[kW];  // lane r: L(c, r), c > r
+#pragma unroll
+      for (int c = 0; c < kW; ++c)
+        l[c] = (lane < w && c < w && c > lane && c - lane <= b) ? L[(size_t)(j0 + lane) * B1 + (c - lane)] : 0.0;
+      double x = lane < w ? sy[lane] : 0.0;
+#pragma unroll
+      for (int c = kW - 1; c >= 0; --c) {
+        if (c < w) {
+          const double xc = lane_value(x, c);
+          if (lane < c) x -= l[c] * xc;
+        }
       }
+      if (lane < w) y[j0 + lane] = x;
     }
-    __syncthreads();
-    if (tid < w) y[j0 + tid] = sy[tid];
     __syncthreads();
   }
 }
 
-// du (CSR order) from the permuted solution; blocks whose flag is 0 (e.g. converged meshes) get 0.
 __global__ void k_band_scatter(int32_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ pblk,
                                const double* __restrict__ y, const int32_t* __restrict__ active, int32_t astride,
                                double* __restrict__ du) {
@@ -384,7 +484,8 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   hipLaunchKernelGGL(k_band_fill, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, P.rowptr.ptr, P.colidx.ptr,
                      p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A);
   const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
-  hipLaunchKernelGGL(k_band_ldlt, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax);
+  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride);
+  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
                      W->y.ptr, active, astride, p->du.ptr);
   MPF_HIP(hipGetLastError());
