@@ -190,6 +190,10 @@ typedef struct {
 } mpf_krylov_report;
 /* Solves K du = rhs; du stays on the device (mpf_get_solution copies it). */
 mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);
+/* Layout of the banded direct solver after its first use (MPF_KSP_BAND or the batched solver 1):  */
+/* independent blocks, largest half-bandwidth, free dofs, and the factorisation's algorithmic      */
+/* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
+mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);
 /* factoriseSolveAndUpdate's update (femSolidmechanics2.cpp:1124-1217): on the first iteration   */
 /* prescribed dofs take dispApplied, free dofs += du, pressure dofs += du[dispDOF + k].          */

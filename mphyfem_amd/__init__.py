@@ -144,6 +144,7 @@ SIGNATURES = [
     ("mpf_set_assembly_mode", C.c_int, [_P, C.c_int32]),
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
+    ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                             C.c_uint64, _DP, _I32P]),
@@ -536,6 +537,12 @@ class Problem:
     def set_chunk_stage_cap(self, cap: int):
         """Hex8 gather: most distinct elements a chunk of 8 wavefronts stages in LDS (0..32)."""
         _check(lib().mpf_set_chunk_stage_cap(self._h, cap), "mpf_set_chunk_stage_cap")
+
+    def band_stats(self):
+        """(blocks, largest half-bandwidth, free dofs, factorisation flops) of the banded direct solver."""
+        nb, bm, nd, fl = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
+        return nb.value, bm.value, nd.value, fl.value
 
     def assembly_mode(self) -> int:
         v = C.c_int32()

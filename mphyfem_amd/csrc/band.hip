@@ -303,7 +303,7 @@ struct BandWork {
   uint64_t key = 0;       // problem uid and block layout it was built for
   int32_t nblk = 0, n = 0, bmax = 0;
   int64_t nband = 0;
-  std::vector<int32_t> bw_h;
+  std::vector<int32_t> bw_h, bstart_h;
   DevBuf<int32_t> perm, iperm, pblk, bstart, bw;
   DevBuf<int64_t> boff;
   DevBuf<double> band, y;
@@ -455,6 +455,7 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   W.bmax = nblk > 0 ? *std::max_element(bw.begin(), bw.end()) : 0;
   W.nband = boff[nblk];
   W.bw_h = bw;
+  W.bstart_h = bstart;
   const int lds = (int)(((size_t)(kW + W.bmax) * kW + kW) * sizeof(double));
   if (lds > 64 * 1024) return MPF_ERR_ARG;  // half-bandwidth beyond the panel's LDS (about 490 dofs)
   MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
@@ -513,3 +514,17 @@ mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep) {
 int32_t* band_bad_flag(mpf_problem* p) { return p->band ? p->band->bad.ptr : nullptr; }
 
 }  // namespace mpf
+
+extern "C" mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof,
+                                     double* flops) {
+  if (!p || !p->band || p->band->key == 0) return MPF_ERR_STATE;
+  const mpf::BandWork& W = *p->band;
+  double f = 0.0;  // per column of each block: b (b + 1) / 2 multiply-adds in the trailing update
+  for (int m = 0; m < W.nblk; ++m)
+    f += (double)(W.bstart_h[m + 1] - W.bstart_h[m]) * (double)W.bw_h[m] * (double)(W.bw_h[m] + 1);
+  if (nblk) *nblk = W.nblk;
+  if (bmax) *bmax = W.bmax;
+  if (ndof) *ndof = W.n;
+  if (flops) *flops = f;
+  return MPF_OK;
+}

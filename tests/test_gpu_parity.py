@@ -366,7 +366,7 @@ def test_krylov_vs_oracle(mpf, orc, gpu):
 
 
 def test_band_direct_solve_vs_oracle(mpf, orc, gpu):
-    """Banded LDL^T on the GPU (reverse Cuthill-McKee renumbering) against the oracle's banded
+    """Banded LDL^T on the GPU (bandwidth-reducing renumbering) against the oracle's banded
     LDL^T on the same assembled system, and bitwise repeatable."""
     n = 8
     m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
