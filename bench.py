@@ -152,7 +152,8 @@ def bench_learned(M, a) -> int:
 
     r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs,
                            residual_epochs=a.residual_epochs, **solver_opts(M, a))
-    out = {"metric": "s per converged load step, learned vs plain Newton start (held-out cantilevers)",
+    out = {"metric": "s per converged load step, learned vs plain Newton start (held-out cantilevers; learned = "
+                     "network correction on the tangent-secant predictor, guarded)",
            "value": round(r["learned"]["seconds"] / max(r["steps"] - 1, 1), 4), "unit": "s",
            "n_gpus": 1, "steps": r["steps"], "warmup": 0, "higher_is_better": False, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 solve, bf16/fp32 network", "data": "synthetic",

@@ -279,6 +279,13 @@ mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32
 /* rep accumulates (zero it before the first step).                                             */
 mpf_status mpf_batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
                                  int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep);
+/* Tangent increment for the next load step (batched solver 1 only, after a converged step):       */
+/* du_nodes (device, [nNode*3]) = K^-1 R(u; lam) on the free dofs, 0 on prescribed dofs, where R  */
+/* is the residual at the current state with the load and prescribed values of lam and K the      */
+/* factors each mesh's last direct solve left.  No counterpart in the reference: it feeds the     */
+/* tangent-secant start point u_k + 2 du - (u_k - u_{k-1}) of the learned guess.  MPF_ERR_STATE    */
+/* before any direct solve.                                                                        */
+mpf_status mpf_batch_tangent_increment(mpf_problem* p, double lam, double* du_nodes);
 
 /* Device pointers of the problem's displacement [nNode*3] and pressure [nNode] (valid until      */
 /* destroy; ordered on the problem's stream).                                                     */

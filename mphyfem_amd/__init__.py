@@ -123,6 +123,7 @@ SIGNATURES = [
     ("mpf_set_batch", C.c_int, [_P, C.c_int32, _I32P, _I32P, _DP, _DP]),
     ("mpf_batch_autotune", C.c_int, [_P, _I32P, _I32P, _DP]),
     ("mpf_batch_newton_solve", C.c_int, [_P, C.POINTER(BatchOpts), _I32P, _DP, C.POINTER(BatchReport)]),
+    ("mpf_batch_tangent_increment", C.c_int, [_P, C.c_double, _P]),
     ("mpf_batch_newton_step", C.c_int, [_P, C.c_double, C.POINTER(BatchOpts), _P, _I32P, _DP,
                                         C.POINTER(BatchReport)]),
     ("mpf_state_device_ptrs", C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
@@ -595,6 +596,15 @@ class Problem:
         st = lib().mpf_batch_newton_step(self._h, lam, C.byref(o), gp, _ip(iters), _dp(rel), C.byref(rep))
         _check(st, "mpf_batch_newton_step")
         return iters, rel, rep
+
+    def batch_tangent_increment(self, lam, out):
+        """out (device float64 tensor [nNode*3]) = K^-1 R(u; lam) with the factors of each mesh's last
+        direct solve (batched solver BATCH_SOLVER_BAND): the tangent increment to load factor lam."""
+        import torch
+        assert out.dtype == torch.float64 and out.is_contiguous() and out.numel() == 3 * self.nNode
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_batch_tangent_increment(self._h, lam, C.c_void_p(out.data_ptr())),
+               "mpf_batch_tangent_increment")
 
     def state_device_ptr(self) -> int:
         """Device address of the displacement [nNode*3] (float64), ordered on the problem's stream."""

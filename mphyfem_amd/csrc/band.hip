@@ -44,20 +44,31 @@ struct BandArgs {
   int32_t* bad;           // set on a zero or non-finite pivot
 };
 
-__global__ void k_band_zero(double* __restrict__ a, int64_t n, int32_t* __restrict__ bad) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0.0;
-  if (i == 0) *bad = 0;
+// Zero the band column of every position of an active block (one wavefront per column, 4 per
+// workgroup; blocks whose active[blk * astride] is 0 keep their factors), and the pivot flag.
+__global__ void __launch_bounds__(256) k_band_zero(int32_t n, const int32_t* __restrict__ pblk, BandArgs A,
+                                                   const int32_t* __restrict__ active, int32_t astride) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *A.bad = 0;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int blk = pblk[i];
+  if (active && !active[(size_t)blk * astride]) return;
+  const int b = A.bw[blk];
+  double* col = A.band + A.boff[blk] + (size_t)(i - A.bstart[blk]) * (b + 1);
+  for (int k = lane; k <= b; k += 64) col[k] = 0.0;
 }
 
-// Lower band of every block from the CSR rows (one lane per row), and the permuted right-hand side.
+// Lower band of every active block from the CSR rows (one lane per row), and the permuted
+// right-hand side.
 __global__ void k_band_fill(int32_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                             const double* __restrict__ vals, const double* __restrict__ rhs,
-                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ pblk, BandArgs A) {
+                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ pblk, BandArgs A,
+                            const int32_t* __restrict__ active, int32_t astride) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const int i = iperm[r];
   const int blk = pblk[i];
+  if (active && !active[(size_t)blk * astride]) return;
   const int s = A.bstart[blk], b = A.bw[blk];
   double* L = A.band + A.boff[blk];
   for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
@@ -297,6 +308,13 @@ __global__ void k_band_scatter(int32_t n, const int32_t* __restrict__ perm, cons
   du[perm[i]] = on ? y[i] : 0.0;
 }
 
+// permuted right-hand side only (a solve with the factors already in the band)
+__global__ void k_band_rhs(int32_t nrows, const int32_t* __restrict__ iperm, const double* __restrict__ rhs,
+                           double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrows) y[iperm[r]] = rhs[r];
+}
+
 }  // namespace
 
 struct BandWork {
@@ -308,6 +326,7 @@ struct BandWork {
   DevBuf<int64_t> boff;
   DevBuf<double> band, y;
   DevBuf<int32_t> bad;
+  bool factored = false;  // a factorisation has been enqueued since the layout was built
 };
 
 void band_free(mpf_problem* p) {
@@ -329,6 +348,7 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   if (p->band && p->band->key == key) return MPF_OK;
   if (!p->band) p->band = new BandWork();
   BandWork& W = *p->band;
+  W.factored = false;
   hipStream_t s = p->stream;
   const int nNode = p->nNode;
   std::vector<int32_t> nbptr(nNode + 1), nda(3 * (size_t)nNode);
@@ -480,15 +500,32 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
-  hipLaunchKernelGGL(k_band_zero, dim3(grid_for((size_t)W->nband + 1, 256)), dim3(256), 0, s, W->band.ptr, W->nband,
-                     W->bad.ptr);
+  hipLaunchKernelGGL(k_band_zero, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, W->pblk.ptr, A, active,
+                     astride);
   hipLaunchKernelGGL(k_band_fill, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, P.rowptr.ptr, P.colidx.ptr,
-                     p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A);
+                     p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
   const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
   hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
                      W->y.ptr, active, astride, p->du.ptr);
+  MPF_HIP(hipGetLastError());
+  W->factored = true;
+  return MPF_OK;
+}
+
+// Enqueue: du = K^-1 rhs for every block with the factors left by the last solve of that block
+// (each block's own last factorisation; blocks skipped since keep theirs).  MPF_ERR_STATE before any.
+mpf_status band_enqueue_resolve(mpf_problem* p) {
+  BandWork* W = p->band;
+  if (!W || !W->factored) return MPF_ERR_STATE;
+  hipStream_t s = p->stream;
+  BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
+  hipLaunchKernelGGL(k_band_rhs, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->iperm.ptr, p->rhs.ptr,
+                     W->y.ptr);
+  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, nullptr, 0);
+  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
+                     W->y.ptr, nullptr, 0, p->du.ptr);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

@@ -725,6 +725,7 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
     ++rep->graph_launches;
     rep->seconds_solve += secs(ts);
   }
+  p->assembled = true;  // vals / rhs hold the step's last assembly
   std::vector<int32_t> mfh((size_t)B.nmesh * kMF);
   std::vector<double> msh((size_t)B.nmesh * kMS);
   MPF_HIP(hipMemcpyAsync(mfh.data(), B.mf.ptr, mfh.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -746,6 +747,35 @@ mpf_status batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o
                              int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep) {
   MPF_TRY(batch_configure(p, o, rep));
   return batch_step(p, lam, o, guess, mesh_iters, mesh_relres, rep);
+}
+
+__global__ void k_b_free_to_nodes(int64_t n3, const int32_t* __restrict__ nda, const double* __restrict__ du,
+                                  double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n3) return;
+  const int32_t eq = nda[j];
+  out[j] = eq >= 0 ? du[eq] : 0.0;
+}
+
+// Tangent increment for the next load step: out = K^-1 R(u; lam) on the free dofs (0 on prescribed
+// dofs), R the residual at the current state with the load and prescribed values of lam (the lift
+// of the first Newton iteration), K the factors each mesh's last banded direct solve left -- the
+// tangent at that mesh's last Newton iterate.  At a converged u_k this is h u'(lam_k) to first order.
+mpf_status batch_tangent_increment(mpf_problem* p, double lam, double* out) {
+  if (!p || !out) return MPF_ERR_ARG;
+  if (!p->batch || !p->prepared) return MPF_ERR_STATE;
+  hipStream_t s = p->stream;
+  MPF_TRY(mpf_set_bc(p, lam));
+  hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, s, p->bad.ptr, 1);
+  MPF_TRY(enqueue_assembly(p, true, lam, 0));
+  MPF_TRY(band_enqueue_resolve(p));
+  const int64_t n3 = (int64_t)p->nNode * 3;
+  hipLaunchKernelGGL(k_b_free_to_nodes, dim3(grid_for(n3, 256)), dim3(256), 0, s, n3, p->pat.nda.ptr, p->du.ptr, out);
+  MPF_HIP(hipGetLastError());
+  int32_t bad = 0;
+  MPF_HIP(hipMemcpyAsync(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+  return bad ? MPF_ERR_NEG_JACOBIAN : MPF_OK;
 }
 
 mpf_status batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
@@ -788,6 +818,10 @@ mpf_status mpf_batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opt
                                  int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep) {
   if (!p) return MPF_ERR_ARG;
   return mpf::batch_newton_step(p, lam, o, guess, mesh_iters, mesh_relres, rep);
+}
+
+mpf_status mpf_batch_tangent_increment(mpf_problem* p, double lam, double* du_nodes) {
+  return mpf::batch_tangent_increment(p, lam, du_nodes);
 }
 
 }  // extern "C"

@@ -213,6 +213,7 @@ mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
 // band.hip
 mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off);
 mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t astride);
+mpf_status band_enqueue_resolve(mpf_problem* p);
 mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep);
 int32_t* band_bad_flag(mpf_problem* p);
 void band_free(mpf_problem* p);

@@ -1,6 +1,19 @@
 """Learned Newton initial guess: cantilever workload, trajectories from the plain batched solver,
 training of the message-passing network (BF16 MFMA kernels, csrc/mpnn.hip) and the comparison of
-plain Newton, linear extrapolation and the learned guess on held-out beams.
+plain Newton, linear extrapolation, the tangent-secant predictor and the learned guess on held-out
+beams.
+
+Start points of load step k+1 (equal load increments h):
+  plain     u_k (the reference's Newton start);
+  extrap    u_k + (u_k - u_{k-1});
+  tangent   u_k + 2 d_k - (u_k - u_{k-1}), d_k = K^-1 R(u_k; lam_{k+1}) with the factors of step k's
+            last direct solve (= h u'(lam_k) to first order): the second-order Taylor predictor
+            with u'' from the secant (batched direct solver only);
+  learned   the network's correction on top of the tangent predictor (of the extrapolation with the
+            iterative solver), kept per mesh only where it lowers the start residual.
+The network sees the base predictor through a virtual previous state v_k with u_k + (u_k - v_k)
+equal to the base (v_k = u_{k-1} for the extrapolation, 2 u_k - u_{k-1} - 2 d_k for the tangent
+predictor), so its features, loss and guess kernels are the same for either base.
 
 Everything numerical runs in libmpf (the batched Newton/PCG solver and the network's kernels);
 torch only holds device memory for the trajectories.
@@ -13,8 +26,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import HEX8, TANGENT_CONSISTENT, HexBatch, Mpnn, Problem, SplitMix64, face_nodes, hex_face_quads, matl, \
-    mesh_hex
+from . import BATCH_SOLVER_BAND, BATCH_SOLVER_PCG, HEX8, TANGENT_CONSISTENT, HexBatch, Mpnn, Problem, SplitMix64, \
+    face_nodes, hex_face_quads, matl, mesh_hex
 
 H_ELEM, W_ELEM = 8, 4      # beam height (z) and width (y) in elements, element size 1/8
 NSTEPS = 20                # tip load ramped over 20 load steps, lambda = k / 20
@@ -85,10 +98,19 @@ def _progress(msg: str):
     print(f"[learned] {msg}", file=sys.stderr, flush=True)
 
 
-def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTEPS, store=None, **solver):
+def uses_tangent(solver: dict) -> bool:
+    """The tangent predictor needs the factors of the batched direct solver."""
+    return solver.get("solver", BATCH_SOLVER_PCG) == BATCH_SOLVER_BAND
+
+
+def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTEPS, store=None, vstore=None,
+              **solver):
     """Ramp the load over nsteps with Newton started from: 'plain' the previous converged solution,
-    'extrap' the linear extrapolation, 'learned' the network's guess.  store: optional device tensor
-    [nsteps + 1, nNode * 3] receiving u_0 .. u_nsteps.  One progress line per load step on stderr."""
+    'extrap' the linear extrapolation, 'tangent' the tangent-secant predictor, 'learned' the network's
+    guess on the base predictor (tangent with the direct solver, else extrapolation).  store:
+    optional device tensor [nsteps + 1, nNode * 3] receiving u_0 .. u_nsteps; vstore: likewise the
+    virtual previous states v_k of the tangent predictor (k >= 1; v_0 = 0).  One progress line per
+    load step on stderr."""
     import torch
 
     n3 = 3 * P.nNode
@@ -96,22 +118,37 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
     dev = torch.device("cuda")
     hist = [torch.zeros(n3, dtype=torch.float64, device=dev) for _ in range(2)]  # u_{k-1}, u_k
     guess = torch.zeros(n3, dtype=torch.float64, device=dev)
+    tang = uses_tangent(solver)
+    if mode == "tangent" and not tang:
+        raise ValueError("the tangent predictor needs solver=BATCH_SOLVER_BAND")
+    dtan = torch.zeros(n3, dtype=torch.float64, device=dev) if tang else None
+    vprev = torch.zeros(n3, dtype=torch.float64, device=dev)
     if store is not None:
         store[0].zero_()
+    if vstore is not None:
+        vstore[0].zero_()
+        vstore[1].zero_()
     log = StepLog()
     for k in range(nsteps):
         lam = (k + 1) / nsteps
         g = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()  # the guess is part of the step's wall time
+        want_v = k >= 1 and tang and (mode in ("tangent", "learned") or vstore is not None)
+        if want_v:  # v_k = 2 u_k - u_{k-1} - 2 d_k
+            P.batch_tangent_increment(lam, dtan)
+            torch.add(hist[1], hist[1], out=vprev).sub_(hist[0]).sub_(dtan, alpha=2.0)
+            if vstore is not None:
+                vstore[k].copy_(vprev)
         if mode != "plain" and k >= 1:
             rho = 1.0  # equal load increments
-            net.features(P, hist[1], hist[0], rho, lam, k / (k + 1.0))
+            prev = vprev if (tang and mode in ("tangent", "learned")) else hist[0]
+            net.features(P, hist[1], prev, rho, lam, k / (k + 1.0))
             if mode == "learned":
                 net.forward(P)
-                log.kept.append(net.guarded_guess(P, lam, hist[1], hist[0], rho, guess))
+                log.kept.append(net.guarded_guess(P, lam, hist[1], prev, rho, guess))
             else:
-                net.guess(P, hist[1], hist[0], rho, False, guess)
+                net.guess(P, hist[1], prev, rho, False, guess)
             g = guess
         iters, rel, rep = P.batch_newton_step(lam, guess=g, **solver)
         log.seconds.append(time.perf_counter() - t0)
@@ -128,9 +165,10 @@ def run_steps(P: Problem, mode: str, net: Mpnn | None = None, nsteps: int = NSTE
 
 
 def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, residual_epochs: int = 0,
-          log_every: int = 10):
-    """Full-batch steps over the stored trajectories u_0..u_N: sample k predicts u_{k+1} from u_k,
-    u_{k-1} (k = 1 .. N-1).  The first `epochs` minimise the displacement error; the following
+          log_every: int = 10, vtraj=None):
+    """Full-batch steps over the stored trajectories u_0..u_N: sample k predicts u_{k+1} from u_k and
+    the previous state u_{k-1} (k = 1 .. N-1), or the virtual previous state vtraj[k] of the tangent
+    predictor when given.  The first `epochs` minimise the displacement error; the following
     `residual_epochs` the linearised start residual ||K e|| / ||dlam f|| (what Newton's convergence
     depends on; one tangent assembly per sample).  Returns the mean loss per epoch."""
     nsteps = traj.shape[0] - 1
@@ -141,12 +179,13 @@ def train(net: Mpnn, P: Problem, traj, epochs: int = 60, lr: float = 2e-3, resid
         res = ep >= epochs
         for k in range(1, nsteps):
             lam = (k + 1) / nsteps
-            net.features(P, traj[k], traj[k - 1], 1.0, lam, k / (k + 1.0))
+            prev = traj[k - 1] if vtraj is None else vtraj[k]
+            net.features(P, traj[k], prev, 1.0, lam, k / (k + 1.0))
             net.forward(P)
             if res:
-                tot += net.residual_loss(P, lam, 1.0 / nsteps, traj[k + 1], traj[k], traj[k - 1], 1.0)
+                tot += net.residual_loss(P, lam, 1.0 / nsteps, traj[k + 1], traj[k], prev, 1.0)
             else:
-                tot += net.loss(P, traj[k + 1], traj[k], traj[k - 1], 1.0)
+                tot += net.loss(P, traj[k + 1], traj[k], prev, 1.0)
             net.backward(P)
             net.adam(lr=lr * (0.5 ** ((ep - (epochs if res else 0)) / max((residual_epochs if res else epochs) / 3, 1))))
         losses.append(tot / (nsteps - 1))
@@ -184,25 +223,29 @@ def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs:
     if autotune:
         out["tile"], out["lanes_per_row"], _ = Ptr.batch_autotune()
     traj = torch.zeros((nsteps + 1, 3 * Ptr.nNode), dtype=torch.float64, device="cuda")
-    log_tr = run_steps(Ptr, "plain", None, nsteps, store=traj, **solver)
+    tang = uses_tangent(solver)
+    vtraj = torch.zeros_like(traj) if tang else None
+    log_tr = run_steps(Ptr, "plain", None, nsteps, store=traj, vstore=vtraj, **solver)
     out["train_trajectories_seconds"] = round(time.perf_counter() - t0, 2)
     out["train_plain"] = summarize(log_tr)
     net = Mpnn(seed=seed)
     t1 = time.perf_counter()
-    losses = train(net, Ptr, traj, epochs=epochs, lr=lr, residual_epochs=residual_epochs)
+    losses = train(net, Ptr, traj, epochs=epochs, lr=lr, residual_epochs=residual_epochs, vtraj=vtraj)
     out["train_seconds"] = round(time.perf_counter() - t1, 2)
     out["loss_first"], out["loss_last"] = losses[0], losses[epochs - 1]
     out["residual_epochs"] = residual_epochs
     if residual_epochs:
         out["start_residual_loss_first"], out["start_residual_loss_last"] = losses[epochs], losses[-1]
-    del traj, Ptr
+    out["base_predictor"] = "tangent" if tang else "extrap"
+    del traj, vtraj, Ptr
     _, Pte = cantilever_problem(beams, test_idx)
     if autotune:
         Pte.batch_autotune()
-    for mode in ("plain", "extrap", "learned"):
+    modes = ("plain", "extrap", "tangent", "learned") if tang else ("plain", "extrap", "learned")
+    for mode in modes:
         out[mode] = summarize(run_steps(Pte, mode, net, nsteps, **solver))
-    out["speedup_learned_vs_plain"] = round(out["plain"]["seconds"] / max(out["learned"]["seconds"], 1e-12), 3)
-    out["speedup_extrap_vs_plain"] = round(out["plain"]["seconds"] / max(out["extrap"]["seconds"], 1e-12), 3)
+    for mode in modes[1:]:
+        out[f"speedup_{mode}_vs_plain"] = round(out["plain"]["seconds"] / max(out[mode]["seconds"], 1e-12), 3)
     return out
 
 

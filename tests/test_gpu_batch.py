@@ -71,6 +71,44 @@ def test_batched_band_newton_matches_each_mesh_alone(mpf, orc, gpu):
     assert np.array_equal(iters, ie) and np.array_equal(u, ue)
 
 
+def test_tangent_increment_solves_the_tangent_system(mpf, gpu):
+    """d = K^-1 R(u_1; lam_2) with the factors each mesh's last direct solve left, per mesh against
+    the tangent assembled at the converged u_1 (the factored tangent is at the iterate before, one
+    Newton correction of relative size ~1e-6 away, hence the 1e-4 bound; factors that were lost
+    give O(1)); zero on prescribed dofs; bitwise repeatable; an error before any direct solve."""
+    import torch
+
+    load = np.array([1.0, 0.02, 1.0, 2.0])  # meshes that converge after different iteration counts
+    B, P = mpf.batch_problem(EDGES, E, NU, V0 * load, V1 * load)
+    P.prepare_matrix_pattern()
+    d = torch.zeros(3 * P.nNode, dtype=torch.float64, device="cuda")
+    with pytest.raises(mpf.MpfError):
+        P.batch_tangent_increment(1.0, d)
+    iters, rel, rep = P.batch_newton_step(0.5, itmax=25, rtol=1e-11, solver=mpf.BATCH_SOLVER_BAND)
+    assert np.all(rel < 1e-11)
+    print("assemblies per mesh", iters)
+    assert iters.min() < iters.max()  # some mesh's factors were left from an earlier iteration
+    P.batch_tangent_increment(1.0, d)
+    vals, rhs = P.system()  # K at the converged u_1 (the step's last assembly), R(u_1; 1.0)
+    pat = P.pattern()
+    nda = pat.NodeDofArray.ravel()
+    dh = d.cpu().numpy()
+    free = nda >= 0
+    assert np.all(dh[~free] == 0.0)
+    x = np.zeros(len(rhs))
+    x[nda[free]] = dh[free]
+    rows = np.repeat(np.arange(len(rhs)), np.diff(pat.rowptr))
+    Kx = np.bincount(rows, weights=vals * x[pat.colidx], minlength=len(rhs))
+    for m in range(len(EDGES)):
+        eq = nda[3 * B.node_off[m]:3 * B.node_off[m + 1]]
+        eq = eq[eq >= 0]
+        r = rhs[eq]
+        assert np.linalg.norm(Kx[eq] - r) / np.linalg.norm(r) < 1e-4, m
+    d2 = torch.zeros_like(d)
+    P.batch_tangent_increment(1.0, d2)
+    assert torch.equal(d, d2)
+
+
 def test_batched_graph_replay_equals_direct_launches(mpf, gpu):
     _, Pg = _batch(mpf)
     _, Pe = _batch(mpf)
