@@ -33,6 +33,10 @@ sys.path.insert(0, str(ROOT))
 # + nK*(8 read + 8 write CSR value + 4 CSR-slot index) + nR*16 (+ nP*12 pressure gather/slot).
 BYTES_PER_ELEM = {"tet10": 24232, "hex8": 12320}
 HBM_PEAK_GBS = 8000.0
+# algorithmic FP64 work of one element's tangent + residual (per Gauss point: K_e 64*72 + Gc 8*162 +
+# stress 8*18 + material ~900 + geometry ~700 for Hex8, x 8 points; Tet10/P1 with its u-p blocks, x 4)
+FLOPS_PER_ELEM = {"tet10": 4.6e4, "hex8": 6.1e4}
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg: str) -> None:
@@ -70,9 +74,22 @@ def cpu_baseline(M, workload: str, budget_s: float):
         done += len(conn)
     dt = time.perf_counter() - t0
     shape = f"{n}^3x6 = {len(conn)}-element Tet10/P1" if workload == "tet10" else f"{n}^3 = {len(conn)}-element Hex8"
-    return {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++, g++ -O3) assembling {done} elements: repeated full sweeps of a "
-                      f"{shape} mesh of the same generator, material and boundary conditions, {dt:.1f} s"}
+    out = {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
+           "sample": f"oracle (serial C++, g++ -O3) assembling {done} elements: repeated full sweeps of a "
+                     f"{shape} mesh of the same generator, material and boundary conditions, {dt:.1f} s"}
+    # all cores: the same element loop colour by colour, OpenMP threads inside a colour
+    nthr = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    color, ncol = Q.color()
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s / 2:
+        Q.assemble_colored(color, nthr, vals, rhs)
+        done += len(conn)
+    dt = time.perf_counter() - t0
+    out["all_cores"] = {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": nthr,
+                        "sample": f"the same oracle loop over the {ncol} element colours with {nthr} OpenMP threads "
+                                  f"inside a colour, {done} elements in {dt:.1f} s"}
+    return out
 
 
 def build_problem_host(M, n, workload):
@@ -341,10 +358,19 @@ def main(argv=None) -> int:
         kname = "k_stage + k_flat + k_rhs + k_lift (row-owner gather, 4 launches"
     else:
         kname = ("tet10_kernel<4,false>" if wl == "tet10" else "hex8_kernel<8,false>") + f" ({ncolors} colour launches"
+    # secondaries: the unique bytes an assembly must move (CSR values written once, mesh and state
+    # read once, residual written once) and the FP64 rate of the element work
+    npe = conn.shape[1]
+    unique = nnz * 8 + nElem * npe * 4 + len(X) * 48 + (dispDOF + presDOF) * 8 + presDOF * 8
+    flops = FLOPS_PER_ELEM[wl] * nElem
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(wl) if a.assembly == "gather" else None,
                 "kernel": f"{kname} per assembly)", "assembly": a.assembly,
-                "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe}
+                "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe,
+                "unique_bytes": unique,
+                "unique_frac": round(unique / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "fp64_tflops": round(flops / (ms_kern * 1e-3) / 1e12, 3),
+                "fp64_frac": round(flops / (ms_kern * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
     if wl == "tet10":
         metric = "Melem/s assembly (Tet10/P1 tangent+residual into FP64 CSR)"
         desc = f"Tet10/P1 mixed Neo-Hookean, {n}^3x6 Kuhn tets ({nElem} elements), one tangent+residual " \

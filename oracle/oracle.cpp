@@ -562,82 +562,95 @@ double norm2(const double* v, int n) {
 // over elements [first, first+count); lift with applyDirichletBCs (ElementBase.cpp:567-594) /
 // applyDirichletBCs2field (ElementBase.cpp:717-782); scatter as
 // SolverPetsc::assembleMatrixAndVector (SolverPetsc.cpp:357-378) / ...2field (:434-502).
-int assemble_range(orc_problem* p, int first, int count, bool firstIter, double* vals, double* rhs) {
+// Per-element scratch of the assembly loop.
+struct AsmWork {
+  std::vector<double> Ke, Re, Kup, Kpu, Kpp, Fp;
+  explicit AsmWork(int nsize) : Ke(nsize * nsize), Re(nsize), Kup(nsize * 4), Kpu(4 * nsize), Kpp(16), Fp(4) {}
+};
+
+// One element of calcStiffnessAndResidual's loop (femSolidmechanics2.cpp:1061-1111): element
+// matrices, the first-iteration lift, and the scatter into the CSR values and rhs.
+static int assemble_elem(orc_problem* p, int e, bool firstIter, double* vals, double* rhs, AsmWork& W) {
   const int npe = p->npe, nsize = 3 * npe;
-  std::vector<double> Ke(nsize * nsize), Re(nsize), Kup(nsize * 4), Kpu(4 * nsize), Kpp(16), Fp(4);
+  std::vector<double>&Ke = W.Ke, &Re = W.Re, &Kup = W.Kup, &Kpu = W.Kpu, &Kpp = W.Kpp, &Fp = W.Fp;
   int fav[30], favP[4];
   double Xe[10][3], ue[10][3], upe[10][3], pe[4];
-  for (int e = first; e < first + count; ++e) {
-    const int* cn = &p->conn[(size_t)e * npe];
-    for (int a = 0; a < npe; ++a)
-      for (int d = 0; d < 3; ++d) {
-        Xe[a][d] = p->X[3 * cn[a] + d];
-        ue[a][d] = p->disp[3 * cn[a] + d];
-        upe[a][d] = p->dispPrev[3 * cn[a] + d];
-      }
-    elem_forassy(p, e, fav);
-    if (!p->mixed) {
-      if (elem_disp(npe, Xe, ue, p->mu, p->Kinv, p->Utype, Ke.data(), Re.data(), p->consistent)) return 1;
-      if (firstIter) {
-        for (int ii = 0; ii < nsize; ++ii) {
-          if (fav[ii] != -1) continue;
-          const double fact = p->dispApplied[3 * cn[ii / 3] + ii % 3];
-          for (int jj = 0; jj < nsize; ++jj)
-            if (fav[jj] != -1) Re[jj] -= Ke[jj * nsize + ii] * fact;
-        }
-      }
+  const int* cn = &p->conn[(size_t)e * npe];
+  for (int a = 0; a < npe; ++a)
+    for (int d = 0; d < 3; ++d) {
+      Xe[a][d] = p->X[3 * cn[a] + d];
+      ue[a][d] = p->disp[3 * cn[a] + d];
+      upe[a][d] = p->dispPrev[3 * cn[a] + d];
+    }
+  elem_forassy(p, e, fav);
+  if (!p->mixed) {
+    if (elem_disp(npe, Xe, ue, p->mu, p->Kinv, p->Utype, Ke.data(), Re.data(), p->consistent)) return 1;
+    if (firstIter) {
       for (int ii = 0; ii < nsize; ++ii) {
-        const int r = fav[ii];
-        if (r < 0) continue;  // VEC_IGNORE_NEGATIVE_INDICES (SolverPetsc.cpp:52)
-        rhs[r] += Re[ii];
-        for (int jj = 0; jj < nsize; ++jj) {
-          const int c = fav[jj];
-          if (c < 0) continue;
-          csr_add(p, vals, r, c, Ke[ii * nsize + jj]);
-        }
-      }
-    } else {
-      for (int q = 0; q < 4; ++q) { pe[q] = p->pres[cn[q]]; favP[q] = p->ID_pres[cn[q]]; }
-      if (elem_tet10p1(Xe, ue, upe, pe, p->mu, p->Kinv, p->Utype, Ke.data(), Kup.data(), Kpu.data(), Kpp.data(),
-                       Re.data(), Fp.data()))
-        return 1;
-      if (firstIter) {
-        for (int ii = 0; ii < nsize; ++ii) {
-          if (fav[ii] != -1) continue;
-          const double fact = p->dispApplied[3 * cn[ii / 3] + ii % 3];
-          for (int jj = 0; jj < nsize; ++jj)
-            if (fav[jj] != -1) Re[jj] -= Ke[jj * nsize + ii] * fact;
-          for (int jj = 0; jj < 4; ++jj)
-            if (favP[jj] != -1) Fp[jj] -= Kpu[jj * nsize + ii] * fact;
-        }
-        // pressure dofs are never prescribed on this path (every corner node is a pressure dof)
-      }
-      const int off = p->dispDOF;
-      for (int ii = 0; ii < nsize; ++ii) {
-        const int r = fav[ii];
-        if (r == -1) continue;
-        rhs[r] += Re[ii];
-        for (int jj = 0; jj < nsize; ++jj) {
-          const int c = fav[jj];
-          if (c != -1) csr_add(p, vals, r, c, Ke[ii * nsize + jj]);
-        }
-        for (int jj = 0; jj < 4; ++jj)
-          if (favP[jj] != -1) csr_add(p, vals, r, favP[jj] + off, Kup[ii * 4 + jj]);
-      }
-      for (int ii = 0; ii < 4; ++ii) {
-        const int r0 = favP[ii];
-        if (r0 == -1) continue;
-        const int r = r0 + off;
-        rhs[r] += Fp[ii];
-        for (int jj = 0; jj < nsize; ++jj) {
-          const int c = fav[jj];
-          if (c != -1) csr_add(p, vals, r, c, Kpu[ii * nsize + jj]);
-        }
-        for (int jj = 0; jj < 4; ++jj)
-          if (favP[jj] != -1) csr_add(p, vals, r, favP[jj] + off, Kpp[ii * 4 + jj]);
+        if (fav[ii] != -1) continue;
+        const double fact = p->dispApplied[3 * cn[ii / 3] + ii % 3];
+        for (int jj = 0; jj < nsize; ++jj)
+          if (fav[jj] != -1) Re[jj] -= Ke[jj * nsize + ii] * fact;
       }
     }
+    for (int ii = 0; ii < nsize; ++ii) {
+      const int r = fav[ii];
+      if (r < 0) continue;  // VEC_IGNORE_NEGATIVE_INDICES (SolverPetsc.cpp:52)
+      rhs[r] += Re[ii];
+      for (int jj = 0; jj < nsize; ++jj) {
+        const int c = fav[jj];
+        if (c < 0) continue;
+        csr_add(p, vals, r, c, Ke[ii * nsize + jj]);
+      }
+    }
+  } else {
+    for (int q = 0; q < 4; ++q) { pe[q] = p->pres[cn[q]]; favP[q] = p->ID_pres[cn[q]]; }
+    if (elem_tet10p1(Xe, ue, upe, pe, p->mu, p->Kinv, p->Utype, Ke.data(), Kup.data(), Kpu.data(), Kpp.data(),
+                     Re.data(), Fp.data()))
+      return 1;
+    if (firstIter) {
+      for (int ii = 0; ii < nsize; ++ii) {
+        if (fav[ii] != -1) continue;
+        const double fact = p->dispApplied[3 * cn[ii / 3] + ii % 3];
+        for (int jj = 0; jj < nsize; ++jj)
+          if (fav[jj] != -1) Re[jj] -= Ke[jj * nsize + ii] * fact;
+        for (int jj = 0; jj < 4; ++jj)
+          if (favP[jj] != -1) Fp[jj] -= Kpu[jj * nsize + ii] * fact;
+      }
+      // pressure dofs are never prescribed on this path (every corner node is a pressure dof)
+    }
+    const int off = p->dispDOF;
+    for (int ii = 0; ii < nsize; ++ii) {
+      const int r = fav[ii];
+      if (r == -1) continue;
+      rhs[r] += Re[ii];
+      for (int jj = 0; jj < nsize; ++jj) {
+        const int c = fav[jj];
+        if (c != -1) csr_add(p, vals, r, c, Ke[ii * nsize + jj]);
+      }
+      for (int jj = 0; jj < 4; ++jj)
+        if (favP[jj] != -1) csr_add(p, vals, r, favP[jj] + off, Kup[ii * 4 + jj]);
+    }
+    for (int ii = 0; ii < 4; ++ii) {
+      const int r0 = favP[ii];
+      if (r0 == -1) continue;
+      const int r = r0 + off;
+      rhs[r] += Fp[ii];
+      for (int jj = 0; jj < nsize; ++jj) {
+        const int c = fav[jj];
+        if (c != -1) csr_add(p, vals, r, c, Kpu[ii * nsize + jj]);
+      }
+      for (int jj = 0; jj < 4; ++jj)
+        if (favP[jj] != -1) csr_add(p, vals, r, favP[jj] + off, Kpp[ii * 4 + jj]);
+    }
   }
+  return 0;
+}
+
+int assemble_range(orc_problem* p, int first, int count, bool firstIter, double* vals, double* rhs) {
+  AsmWork W(3 * p->npe);
+  for (int e = first; e < first + count; ++e)
+    if (assemble_elem(p, e, firstIter, vals, rhs, W)) return 1;
   return 0;
 }
 
@@ -1141,6 +1154,27 @@ int orc_assemble(orc_problem* p, int first_iter, double lam, double* vals, doubl
 
 int orc_assemble_range(orc_problem* p, int first, int count, double* vals, double* rhs) {
   return assemble_range(p, first, count, false, vals, rhs);
+}
+
+// The same element loop over all elements, colour by colour with OpenMP threads inside a colour
+// (elements of one colour share no node, so their scatters touch disjoint rows).  CPU-baseline
+// timing only: the CSR sums come in another order than the serial loop's.
+int orc_assemble_colored(orc_problem* p, const int32_t* color, int nthreads, double* vals, double* rhs) {
+  int ncol = 0;
+  for (int e = 0; e < p->nElem; ++e) ncol = std::max(ncol, color[e] + 1);
+  std::vector<std::vector<int>> lists(ncol);
+  for (int e = 0; e < p->nElem; ++e) lists[color[e]].push_back(e);
+  int bad = 0;
+#pragma omp parallel num_threads(nthreads) reduction(| : bad)
+  {
+    AsmWork W(3 * p->npe);
+    for (int c = 0; c < ncol; ++c) {
+      const std::vector<int>& L = lists[c];
+#pragma omp for schedule(dynamic, 16)
+      for (int i = 0; i < (int)L.size(); ++i) bad |= assemble_elem(p, L[i], false, vals, rhs, W);
+    }
+  }
+  return bad;
 }
 
 int orc_pcg(const orc_problem* p, const double* vals, const double* b, double* x, int precond, double rtol, int maxit,

@@ -115,6 +115,8 @@ int  orc_newton(orc_problem* p, int nsteps, double dt, int solver, int precond, 
 /* One element-assembly sweep only (timing helper for the CPU baseline): assembles elements
  * [first, first+count) into vals/rhs.  Returns 0 or 1. */
 int  orc_assemble_range(orc_problem* p, int first, int count, double* vals, double* rhs);
+/* all elements, colour by colour, OpenMP threads within a colour (CPU-baseline timing only) */
+int  orc_assemble_colored(orc_problem* p, const int32_t* color, int nthreads, double* vals, double* rhs);
 
 /* Colour priority shared by oracle and product: SplitMix64 finaliser of (e + golden). */
 uint64_t orc_color_priority(int64_t e);

@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
             "orc_set_bc": (None, [C.c_void_p, C.c_double]),
             "orc_assemble": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, _D]),
             "orc_assemble_range": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _D, _D]),
+            "orc_assemble_colored": (C.c_int, [C.c_void_p, _I, C.c_int, _D, _D]),
             "orc_pcg": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_double, C.c_int, _D]),
             "orc_gmres": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int, C.c_int, C.c_double, C.c_int, _D]),
             "orc_direct_solve": (C.c_int, [C.c_void_p, _D, _D, _D]),
@@ -213,6 +214,11 @@ class Problem:
 
     def assemble_range(self, first, count, vals, rhs):
         return lib().orc_assemble_range(self.h, first, count, _d(vals), _d(rhs))
+
+    def assemble_colored(self, color, nthreads, vals, rhs):
+        """All elements colour by colour on nthreads OpenMP threads (timing; sums in another order)."""
+        return lib().orc_assemble_colored(self.h, _i(np.ascontiguousarray(color, dtype=np.int32)), nthreads,
+                                          _d(vals), _d(rhs))
 
     def pcg(self, vals, b, precond=2, rtol=1e-10, maxit=100000):
         x = np.zeros_like(b)

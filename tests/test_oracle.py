@@ -280,6 +280,28 @@ def test_coloring(orc, mpf):
         assert len(nodes) == len(np.unique(nodes))
 
 
+@pytest.mark.parametrize("etype", [8, 10])
+def test_colored_parallel_assembly_matches_serial(orc, mpf, etype):
+    """The oracle's colour-parallel loop (the all-cores CPU baseline) assembles the serial loop's
+    K and R up to summation order."""
+    if etype == 8:
+        X, conn = cube_hex(4, jitter=0.1, seed=3)
+        P = orc.Problem(8, X, conn, 50.0, 1e-2, 3)
+    else:
+        X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=2)
+        P = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    P.prepare_matrix_pattern()
+    rng = np.random.default_rng(5)
+    P.set_state(disp=0.01 * rng.standard_normal(3 * len(X)))
+    d, pp, nnz = P.sizes()
+    v1, r1 = np.zeros(nnz), np.zeros(d + pp)
+    v2, r2 = np.zeros(nnz), np.zeros(d + pp)
+    assert P.assemble_range(0, len(conn), v1, r1) == 0
+    color, _ = P.color()
+    assert P.assemble_colored(color, 4, v2, r2) == 0
+    assert normwise_rel(v2, v1) < 1e-13 and normwise_rel(r2, r1) < 1e-13
+
+
 def test_solvers_vs_scipy(orc):
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
