@@ -26,8 +26,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import BATCH_SOLVER_BAND, BATCH_SOLVER_PCG, HEX8, TANGENT_CONSISTENT, HexBatch, Mpnn, Problem, SplitMix64, \
-    face_nodes, hex_face_quads, matl, mesh_hex
+from . import BATCH_SOLVER_BAND, BATCH_SOLVER_PCG, ERR_DIVERGED, ERR_NEG_JACOBIAN, ERR_NOT_CONVERGED, HEX8, \
+    TANGENT_CONSISTENT, HexBatch, MpfError, Mpnn, Problem, SplitMix64, face_nodes, hex_face_quads, matl, mesh_hex
 
 H_ELEM, W_ELEM = 8, 4      # beam height (z) and width (y) in elements, element size 1/8
 NSTEPS = 20                # tip load ramped over 20 load steps, lambda = k / 20
@@ -275,6 +275,29 @@ def block_newton_opts(P: Problem):
                          lin_rtol=1e-6, maxit=100000, forcing=False)
 
 
+_CUT_STATUS = (ERR_NEG_JACOBIAN, ERR_NOT_CONVERGED, ERR_DIVERGED)
+
+
+def newton_with_cut(P: Problem, lam0: float, lam1: float, opts, guess=None, depth: int = 0, max_depth: int = 6):
+    """Newton from the converged state at lam0 to lam1, started from guess (or plainly).  On a negative
+    Jacobian, divergence or no convergence the state is restored and the increment halved, both
+    halves started plainly (MyTime::cut, MyTime.cpp:26-106; reset, SolutionDataSolid.cpp:157-167),
+    down to 1/64 of the increment.  Returns (assemblies and Krylov iterations of the converged
+    solves, number of cuts)."""
+    ustart, _ = P.state()
+    try:
+        it, rep = P.newton_step(lam1, opts, guess)
+        return it, rep.krylov_iters, 0
+    except MpfError as e:
+        if e.status not in _CUT_STATUS or depth >= max_depth:
+            raise
+    P.set_state(ustart)
+    mid = 0.5 * (lam0 + lam1)
+    a1, k1, c1 = newton_with_cut(P, lam0, mid, opts, None, depth + 1, max_depth)
+    a2, k2, c2 = newton_with_cut(P, mid, lam1, opts, None, depth + 1, max_depth)
+    return a1 + a2, k1 + k2, 1 + c1 + c2
+
+
 def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, store=None, log=print):
     """Load steps 1..nsteps of the block from 'plain', 'extrap' or 'learned' start points."""
     import torch
@@ -302,39 +325,46 @@ def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, sto
             else:
                 net.guess(P, hist[1], hist[0], 1.0, False, guess)
             g = guess.cpu().numpy()
-        it, rep = P.newton_step(lam, opts, g)
+        it, kry, cuts = newton_with_cut(P, k / BLOCK_STEPS, lam, opts, g)
         dt = time.perf_counter() - t0
         hist[0].copy_(hist[1])
         P.copy_state_device(hist[1])
         if store is not None:
             store[k + 1].copy_(hist[1])
-        steps.append({"step": k + 1, "assemblies": it, "krylov_iterations": rep.krylov_iters, "seconds": round(dt, 3),
-                      "network_correction_kept": kept})
+        steps.append({"step": k + 1, "assemblies": it, "krylov_iterations": kry, "seconds": round(dt, 3),
+                      "cuts": cuts, "network_correction_kept": kept})
         if log:
-            log(f"block {mode} step {k + 1}: {it} assemblies, {rep.krylov_iters} PCG iterations, {dt:.2f} s")
+            log(f"block {mode} step {k + 1}: {it} assemblies, {kry} PCG iterations, {cuts} cuts, {dt:.2f} s")
     return steps
 
 
 def block_experiment(n_train: int = 48, n_eval: int = 128, eval_steps: int = 3, epochs: int = 40,
-                     lr: float = 2e-3, log=print) -> dict:
+                     lr: float = 2e-3, log=print, modes=("plain", "extrap", "learned")) -> dict:
     """Train the network on the trajectory of an n_train^3 block (all 15 steps, plain Newton), then run
-    the first eval_steps steps of the n_eval^3 block from plain, extrapolated and learned start points."""
+    the first eval_steps steps of the n_eval^3 block from the start points in modes (plain,
+    extrapolated, learned; the network is trained only when 'learned' is among them)."""
     import torch
 
     out = {"train_block": n_train, "eval_block": n_eval, "eval_steps": eval_steps, "epochs": epochs}
+    net = Mpnn(seed=5)
+    if "learned" not in modes:
+        Pe = compression_block(n_eval)
+        for mode in modes:
+            out[mode] = block_steps(Pe, mode, eval_steps, net, log=log)
+        Pe.close()
+        return out
     Pt = compression_block(n_train)
     traj = torch.zeros((BLOCK_STEPS + 1, 3 * Pt.nNode), dtype=torch.float64, device="cuda")
     t0 = time.perf_counter()
     block_steps(Pt, "plain", BLOCK_STEPS, store=traj, log=None)
     out["train_trajectory_seconds"] = round(time.perf_counter() - t0, 2)
-    net = Mpnn(seed=5)
     t1 = time.perf_counter()
     losses = train(net, Pt, traj, epochs=epochs, lr=lr, log_every=0)
     out["train_seconds"] = round(time.perf_counter() - t1, 2)
     out["loss_first"], out["loss_last"] = losses[0], losses[-1]
     del traj, Pt
     Pe = compression_block(n_eval)
-    for mode in ("plain", "extrap", "learned"):
+    for mode in modes:
         out[mode] = block_steps(Pe, mode, eval_steps, net, log=log)
     Pe.close()
     return out

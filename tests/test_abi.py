@@ -115,3 +115,46 @@ def test_no_device_is_an_error_not_a_fallback(mpf):
     with pytest.raises(mpf.MpfError) as ei:
         mpf.Problem(mpf.HEX8, X, conn, mpf.matl(1.0, 1e-2))
     assert ei.value.status == mpf.ERR_NO_DEVICE
+
+
+class _CutProblem:
+    """Stand-in for Problem.newton_step: a negative Jacobian whenever the load increment exceeds
+    max_inc, and a record of the load factors reached."""
+
+    def __init__(self, max_inc):
+        self.max_inc, self.lam, self.u, self.calls = max_inc, 0.0, np.zeros(3), []
+
+    def state(self):
+        return self.u.copy(), None
+
+    def set_state(self, u):
+        self.u = np.array(u)
+
+    def newton_step(self, lam, opts, guess):
+        import types
+
+        import mphyfem_amd as M
+        self.calls.append(lam)
+        if lam - self.lam > self.max_inc + 1e-12:
+            self.u = self.u + 1.0  # a failed attempt leaves a spoiled state behind
+            raise M.NegativeJacobian(M.ERR_NEG_JACOBIAN, "newton_step")
+        self.lam = lam
+        self.u = np.full(3, lam)
+        return 3, types.SimpleNamespace(krylov_iters=10)
+
+
+def test_newton_step_cut_reaches_the_target_by_halving():
+    from mphyfem_amd.learned import newton_with_cut
+    P = _CutProblem(max_inc=0.3)
+    it, kry, cuts = newton_with_cut(P, 0.0, 1.0, None)
+    assert P.lam == 1.0 and np.all(P.u == 1.0)  # restored after each failure, then sub-stepped
+    assert cuts == 3 and it == 4 * 3 and kry == 4 * 10  # 1 -> 0.5 -> 0.25 x2, then 0.25 x2
+    assert P.calls == [1.0, 0.5, 0.25, 0.5, 1.0, 0.75, 1.0]
+
+
+def test_newton_step_cut_gives_up_below_the_smallest_increment():
+    import mphyfem_amd as M
+    from mphyfem_amd.learned import newton_with_cut
+    P = _CutProblem(max_inc=0.001)
+    with pytest.raises(M.NegativeJacobian):
+        newton_with_cut(P, 0.0, 1.0, None)

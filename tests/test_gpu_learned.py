@@ -13,3 +13,17 @@ def test_learned_guess_saves_newton_iterations(mpf, gpu):
                            cg_maxit=20000, cg_rtol=1e-10, chunk=16)
     assert r["loss_last"] < r["loss_first"]
     assert r["learned"]["assemblies_per_mesh_step"] < r["plain"]["assemblies_per_mesh_step"], r
+
+
+def test_tangent_predictor_and_learned_guess_with_direct_solves(mpf, gpu):
+    """With exact Newton corrections (banded direct solve per beam) the tangent-secant predictor
+    saves Newton iterations over the extrapolation, which saves them over the plain start; the
+    guarded learned guess is built on the tangent predictor and is never worse than the
+    extrapolation."""
+    from mphyfem_amd import learned
+    r = learned.experiment(nbeams=12, ntrain=8, ntest=4, epochs=12, nsteps=8, autotune=False, itmax=25, rtol=1e-9,
+                           solver=mpf.BATCH_SOLVER_BAND)
+    assert r["base_predictor"] == "tangent"
+    a = {m: r[m]["assemblies_per_mesh_step"] for m in ("plain", "extrap", "tangent", "learned")}
+    assert a["tangent"] < a["extrap"] < a["plain"], a
+    assert a["learned"] <= a["extrap"], a
