@@ -397,9 +397,14 @@ def main(argv=None) -> int:
             r = P.factorise_and_solve(ksp=M.KSP_CG, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit)
             name = "pcg"
             pc = "block-Jacobi (3x3 node blocks)"
+        ms_it = r.seconds / max(r.iterations, 1) * 1e3
         out[name] = {"pc": pc, "iterations": r.iterations, "relres": r.relres, "converged": bool(r.converged),
-                     "seconds": round(r.seconds, 4),
-                     "ms_per_iteration": round(r.seconds / max(r.iterations, 1) * 1e3, 4)}
+                     "seconds": round(r.seconds, 4), "ms_per_iteration": round(ms_it, 4)}
+        if name == "pcg":  # CSR values + int32 columns once, ~6 vector streams of n doubles per iteration
+            bpi = 12 * nnz + 144 * (dispDOF + presDOF)
+            out[name]["roofline"] = {"bound": "hbm", "bytes_per_iteration": bpi,
+                                     "achieved": round(bpi / (ms_it * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": round(bpi / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"{name} {r.iterations} its relres {r.relres:.2e} {r.seconds:.2f} s")
     if not a.no_cpu:
         log("cpu baseline")

@@ -18,7 +18,7 @@
 namespace mpf {
 
 constexpr int kRB = 256;        // threads per block for vector kernels
-constexpr int kMaxParts = 1024; // partial sums per reduction
+constexpr int kMaxParts = kMaxPartials;  // partial sums per reduction (4096 workgroups: a full GPU of waves)
 constexpr int kMaxRestart = 200;  // GMRES(m) upper bound (Hessenberg column scratch on the device)
 
 struct KrylovWork {
@@ -34,6 +34,7 @@ struct KrylovWork {
   DevBuf<double> blk_inv;
   int64_t nblk = 0;
   int pc_kind = -1;
+  bool grp_spmv = false;  // every node block's rows share one column list (checked once)
 };
 
 namespace {
@@ -82,6 +83,7 @@ __global__ void __launch_bounds__(kRB) k_spmv(int64_t n, const int32_t* __restri
     double s = 0.0;
     if (r < n) {
       const int b = rowptr[r], e = rowptr[r + 1];
+#pragma unroll 4
       for (int k = b + l; k < e; k += LPR) s += vals[k] * x[colidx[k]];
     }
 #pragma unroll
@@ -95,6 +97,79 @@ __global__ void __launch_bounds__(kRB) k_spmv(int64_t n, const int32_t* __restri
     const double t = block_sum(dot, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
   }
+}
+
+// y = A x over row groups: block k holds rows bs[k] .. bs[k] + bm[k] - 1 (a node's free dofs, or one
+// pressure row), which share one column list, so each column index and x value is read once for
+// up to three rows.  Entries go to the LPR lanes as in k_spmv (same per-row summation order);
+// part[b] = sum over the block's rows of x_r y_r (when part != null).
+template <int LPR>
+__global__ void __launch_bounds__(kRB) k_spmv_grp(int64_t nblk, const int32_t* __restrict__ bs,
+                                                  const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                  double* __restrict__ part, const double* __restrict__ sc) {
+  if (sc && sc[S_DONE] != 0.0) return;
+  __shared__ double sh[kRB / 64];
+  const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  constexpr int GPB = kRB / LPR;
+  double dot = 0.0;
+  for (int64_t k0 = (int64_t)blockIdx.x * GPB; k0 < nblk; k0 += (int64_t)gridDim.x * GPB) {
+    const int64_t k = k0 + g;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int st = 0, m = 0;
+    if (k < nblk) {
+      st = bs[k];
+      m = bm[k];
+      const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+      const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
+#pragma unroll 4
+      for (int j = l; j < len; j += LPR) {
+        const double xv = x[colidx[b0 + j]];
+        s0 += vals[b0 + j] * xv;
+        if (m > 1) s1 += vals[b1 + j] * xv;
+        if (m > 2) s2 += vals[b2 + j] * xv;
+      }
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPR);
+      s1 += __shfl_xor(s1, off, LPR);
+      s2 += __shfl_xor(s2, off, LPR);
+    }
+    if (k < nblk && l == 0) {
+      y[st] = s0;
+      if (m > 1) y[st + 1] = s1;
+      if (m > 2) y[st + 2] = s2;
+      if (part) {
+        dot += x[st] * s0;
+        if (m > 1) dot += x[st + 1] * s1;
+        if (m > 2) dot += x[st + 2] * s2;
+      }
+    }
+  }
+  if (part) {
+    const double t = block_sum(dot, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+// bad[0] = 1 if some block's rows do not share one column list (one thread per block)
+__global__ void k_check_groups(int64_t nblk, const int32_t* __restrict__ bs, const int32_t* __restrict__ bm,
+                               const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                               int32_t* __restrict__ bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  const int st = bs[k], m = bm[k];
+  const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+  bool same = m >= 1 && m <= 3;
+  for (int i = 1; same && i < m; ++i) {
+    const int bi = rowptr[st + i];
+    if (rowptr[st + i + 1] - bi != len) { same = false; break; }
+    for (int j = 0; j < len; ++j)
+      if (colidx[bi + j] != colidx[b0 + j]) { same = false; break; }
+  }
+  if (!same) bad[0] = 1;
 }
 
 // out[j] = sum_i part[j*stride + i], i < np (fixed order)
@@ -412,6 +487,29 @@ mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals
   return MPF_OK;
 }
 
+// y = K x for the Krylov solvers: the row-group kernel when the node blocks share column lists,
+// else the per-row kernel.  part: partial x.y per workgroup (np of them) or null.
+static mpf_status spmv_k(const mpf_problem* p, const KrylovWork& W, const double* x, double* y, double* part,
+                         unsigned np, const double* sc, hipStream_t s) {
+  const int64_t n = W.n, nnz = p->pat.nnz;
+  if (!W.grp_spmv)
+    return spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, nnz, x, y, part, np, sc, s);
+  const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
+  const int32_t *bs = W.blk_start.ptr, *bm = W.blk_size.ptr, *rp = p->pat.rowptr.ptr, *ci = p->pat.colidx.ptr;
+  if (avg > 48) {
+    const unsigned g = part ? np : grid_for(W.nblk, kRB / 32);
+    hipLaunchKernelGGL(k_spmv_grp<32>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+  } else if (avg > 12) {
+    const unsigned g = part ? np : grid_for(W.nblk, kRB / 16);
+    hipLaunchKernelGGL(k_spmv_grp<16>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+  } else {
+    const unsigned g = part ? np : grid_for(W.nblk, kRB / 4);
+    hipLaunchKernelGGL(k_spmv_grp<4>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+  }
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch) {
   const unsigned np = nparts_for(n);
   hipLaunchKernelGGL(k_dot_part, dim3(np), dim3(kRB), 0, s, n, x, x, scratch);
@@ -474,6 +572,18 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
     }
     MPF_TRY(W.blk_inv.alloc((size_t)W.nblk * 9));
     W.pc_kind = kind;
+    W.grp_spmv = false;
+    if (kind == 2 && W.nblk > 0) {
+      DevBuf<int32_t> bad;
+      MPF_TRY(bad.alloc(1));
+      int32_t hbad = 0;
+      MPF_HIP(hipMemcpyAsync(bad.ptr, &hbad, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_check_groups, dim3(grid_for(W.nblk, B)), dim3(B), 0, s, W.nblk, W.blk_start.ptr,
+                         W.blk_size.ptr, p->pat.rowptr.ptr, p->pat.colidx.ptr, bad.ptr);
+      MPF_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      MPF_HIP(hipStreamSynchronize(s));
+      W.grp_spmv = hbad == 0;
+    }
   }
   hipLaunchKernelGGL(k_block_setup, dim3(grid_for(W.nblk, 256)), dim3(256), 0, s, W.nblk, W.blk_start.ptr,
                      W.blk_size.ptr, p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, W.blk_inv.ptr,
@@ -509,8 +619,7 @@ static mpf_status pcg(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_repor
   for (;;) {
     for (int c = 0; c < chunk; ++c, ++it) {
       const int par = it & 1;
-      MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, p->pat.nnz, W.p.ptr, W.q.ptr, W.part.ptr, nps,
-                   sc, s));
+      MPF_TRY(spmv_k(p, W, W.p.ptr, W.q.ptr, W.part.ptr, nps, sc, s));
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, (int)nps, 1, sc + S_PQ);
       hipLaunchKernelGGL(k_cg_update, dim3(np), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
                          W.blk_inv.ptr, x, W.r.ptr, W.z.ptr, W.p.ptr, W.q.ptr, W.part.ptr, (int)np, sc, par);
@@ -560,7 +669,7 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
       MPF_HIP(hipMemcpyAsync(W.r.ptr, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
       rn = bn;
     } else {
-      MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, p->pat.nnz, x, W.w.ptr, nullptr, 0, nullptr, s));
+      MPF_TRY(spmv_k(p, W, x, W.w.ptr, nullptr, 0, nullptr, s));
       hipLaunchKernelGGL(k_sub, dim3(nps), dim3(kRB), 0, s, n, b, W.w.ptr, W.r.ptr);
       MPF_TRY(dev_norm2(W.r.ptr, n, &rn, s, W.part.ptr));
     }
@@ -574,8 +683,7 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
       // w = A M^-1 v_j
       hipLaunchKernelGGL(k_pc_apply, dim3(npb), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
                          W.blk_inv.ptr, V + (size_t)j * n, W.z.ptr);
-      MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, p->pat.nnz, W.z.ptr, W.w.ptr, nullptr, 0,
-                   nullptr, s));
+      MPF_TRY(spmv_k(p, W, W.z.ptr, W.w.ptr, nullptr, 0, nullptr, s));
       // CGS2
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_multidot, dim3(nps), dim3(kRB), 0, s, n, j + 1, V, W.w.ptr, W.part.ptr, (int)nps);
@@ -620,7 +728,7 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
     if (rn / bn < o->rtol) break;
   }
   // true residual
-  MPF_TRY(spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, p->pat.nnz, x, W.w.ptr, nullptr, 0, nullptr, s));
+  MPF_TRY(spmv_k(p, W, x, W.w.ptr, nullptr, 0, nullptr, s));
   hipLaunchKernelGGL(k_sub, dim3(nps), dim3(kRB), 0, s, n, b, W.w.ptr, W.r.ptr);
   double tr = 0.0;
   MPF_TRY(dev_norm2(W.r.ptr, n, &tr, s, W.part.ptr));

@@ -218,5 +218,8 @@ mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep);
 int32_t* band_bad_flag(mpf_problem* p);
 void band_free(mpf_problem* p);
 void krylov_free(mpf_problem* p);
+// Most partial sums of one device reduction (one per workgroup of the vector kernels).
+constexpr int kMaxPartials = 4096;
+// ||x||_2 to the host; scratch: kMaxPartials + 1 doubles of device memory.
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);
 }  // namespace mpf

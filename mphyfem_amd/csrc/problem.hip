@@ -292,7 +292,7 @@ mpf_status do_set_bc(mpf_problem* p, double lam) {
 }
 
 mpf_status rhs_norm(mpf_problem* p, double* out) {
-  if (p->scratch.n < 4096) MPF_TRY(p->scratch.alloc(4096));
+  if (p->scratch.n < (size_t)kMaxPartials + 1) MPF_TRY(p->scratch.alloc((size_t)kMaxPartials + 1));
   const int64_t n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
   return dev_norm2(p->rhs.ptr, n, out, p->stream, p->scratch.ptr);
 }
