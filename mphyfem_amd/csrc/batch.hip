@@ -41,6 +41,10 @@ struct BatchWork {
   int32_t T = 1024, LPR = 8, chunk = 16, ntiles = 0;
   bool tuned = false;
   DevBuf<int32_t> tlo, thi, tmesh, mtile;  // tile ranges and mesh, mesh -> first tile [nmesh + 1]
+  // row groups: the free rows of one node, [gs[g], gs[g + 1]); tile t holds groups [tg[t], tg[t + 1])
+  std::vector<int32_t> gs_h;
+  DevBuf<int32_t> gs, tg;
+  bool grp_spmv = false;  // every group's rows share one column list (checked once)
   // CG vectors (x is the problem's du)
   DevBuf<double> dinv, r, z, pv, q;
   DevBuf<double> part;  // [ntiles][2]
@@ -143,6 +147,67 @@ __global__ void __launch_bounds__(kB) k_b_spmv(const int32_t* __restrict__ tlo, 
   }
   dot = block_sum(dot, sh);
   if (threadIdx.x == 0) part[2 * b] = dot;
+}
+
+// q = A p over the tile's row groups (a node's free rows share one column list, so each column
+// index and p value is read once for up to three rows); same per-row summation order as k_b_spmv.
+template <int LPR>
+__global__ void __launch_bounds__(kB) k_b_spmv_grp(const int32_t* __restrict__ tg, const int32_t* __restrict__ gs,
+                                                   const int32_t* __restrict__ tmesh, const int32_t* __restrict__ mf,
+                                                   const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                                   const double* __restrict__ vals, const double* __restrict__ x,
+                                                   double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[kB / 64];
+  const int b = blockIdx.x;
+  if (!mf[tmesh[b] * kMF + kCGA]) {
+    if (threadIdx.x == 0) part[2 * b] = 0.0;
+    return;
+  }
+  const int gi = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  double dot = 0.0;
+  for (int g = tg[b] + gi; g < tg[b + 1]; g += kB / LPR) {
+    const int st = gs[g], m = gs[g + 1] - st;
+    const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+    const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll 4
+    for (int j = l; j < len; j += LPR) {
+      const double xv = x[colidx[b0 + j]];
+      s0 += vals[b0 + j] * xv;
+      if (m > 1) s1 += vals[b1 + j] * xv;
+      if (m > 2) s2 += vals[b2 + j] * xv;
+    }
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPR);
+      s1 += __shfl_xor(s1, off, LPR);
+      s2 += __shfl_xor(s2, off, LPR);
+    }
+    if (l == 0) {
+      y[st] = s0;
+      dot += x[st] * s0;
+      if (m > 1) { y[st + 1] = s1; dot += x[st + 1] * s1; }
+      if (m > 2) { y[st + 2] = s2; dot += x[st + 2] * s2; }
+    }
+  }
+  dot = block_sum(dot, sh);
+  if (threadIdx.x == 0) part[2 * b] = dot;
+}
+
+// bad[0] = 1 if some group's rows do not share one column list, or a group has more than 3 rows
+__global__ void k_b_check_groups(int32_t ng, const int32_t* __restrict__ gs, const int32_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ colidx, int32_t* __restrict__ bad) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int st = gs[g], m = gs[g + 1] - st;
+  const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+  bool same = m >= 1 && m <= 3;
+  for (int i = 1; same && i < m; ++i) {
+    const int bi = rowptr[st + i];
+    if (rowptr[st + i + 1] - bi != len) { same = false; break; }
+    for (int j = 0; j < len; ++j)
+      if (colidx[bi + j] != colidx[b0 + j]) { same = false; break; }
+  }
+  if (!same) bad[0] = 1;
 }
 
 // CG start: x = 0, r = b, z = D^-1 r, p = z; partials r.r and r.z (meshes not Newton-active: r = 0).
@@ -337,17 +402,27 @@ __global__ void k_b_zero_i32(int32_t* a, int32_t n) {
   if (i < n) a[i] = 0;
 }
 
+// Tiles of about T dofs inside each mesh, cut only where a node's rows begin (so every row group
+// lies in one tile).
 mpf_status build_tiles(BatchWork& B, int T, hipStream_t s) {
-  std::vector<int32_t> lo, hi, tm, mt(B.nmesh + 1, 0);
+  std::vector<int32_t> lo, hi, tm, tgh, mt(B.nmesh + 1, 0);
+  const std::vector<int32_t>& gs = B.gs_h;
+  const int32_t ng = (int32_t)gs.size() - 1;
+  int32_t g = 0;
   for (int m = 0; m < B.nmesh; ++m) {
     mt[m] = (int32_t)lo.size();
-    for (int32_t a = B.dof_off[m]; a < B.dof_off[m + 1]; a += T) {
+    while (g < ng && gs[g] < B.dof_off[m + 1]) {  // groups of mesh m, in dof order
+      const int32_t a = gs[g];
       lo.push_back(a);
-      hi.push_back(std::min(a + T, B.dof_off[m + 1]));
+      tgh.push_back(g);
       tm.push_back(m);
+      while (g < ng && gs[g] < B.dof_off[m + 1] && gs[g] - a < T) ++g;
+      hi.push_back(gs[g] < B.dof_off[m + 1] ? gs[g] : B.dof_off[m + 1]);
     }
   }
+  tgh.push_back(g);
   mt[B.nmesh] = (int32_t)lo.size();
+  MPF_TRY(B.tg.upload(tgh.data(), tgh.size(), s));
   B.T = T;
   B.ntiles = (int32_t)lo.size();
   MPF_TRY(B.tlo.upload(lo.data(), lo.size(), s));
@@ -364,18 +439,34 @@ mpf_status enqueue_cg_iter(mpf_problem* p, BatchWork& B, int32_t maxit) {
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
-  switch (B.LPR) {
-    case 4:
-      hipLaunchKernelGGL(k_b_spmv<4>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
-                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
-      break;
-    case 16:
-      hipLaunchKernelGGL(k_b_spmv<16>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
-                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
-      break;
-    default:
-      hipLaunchKernelGGL(k_b_spmv<8>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
-                         P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+  if (B.grp_spmv) {
+    switch (B.LPR) {
+      case 4:
+        hipLaunchKernelGGL(k_b_spmv_grp<4>, gt, bt, 0, s, B.tg.ptr, B.gs.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+        break;
+      case 16:
+        hipLaunchKernelGGL(k_b_spmv_grp<16>, gt, bt, 0, s, B.tg.ptr, B.gs.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+        break;
+      default:
+        hipLaunchKernelGGL(k_b_spmv_grp<8>, gt, bt, 0, s, B.tg.ptr, B.gs.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+    }
+  } else {
+    switch (B.LPR) {
+      case 4:
+        hipLaunchKernelGGL(k_b_spmv<4>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+        break;
+      case 16:
+        hipLaunchKernelGGL(k_b_spmv<16>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+        break;
+      default:
+        hipLaunchKernelGGL(k_b_spmv<8>, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, P.rowptr.ptr,
+                           P.colidx.ptr, p->vals.ptr, B.pv.ptr, B.q.ptr, B.part.ptr);
+    }
   }
   hipLaunchKernelGGL(k_b_alpha, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr);
   hipLaunchKernelGGL(k_b_xrz, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, B.tmesh.ptr, B.mf.ptr, B.ms.ptr, B.dinv.ptr,
@@ -504,18 +595,34 @@ static mpf_status batch_prepare(mpf_problem* p) {
   MPF_TRY(p->pat.nda.download(nda.data(), nda.size(), p->stream));
   MPF_HIP(hipStreamSynchronize(p->stream));
   B.dof_off.assign(B.nmesh + 1, 0);
+  B.gs_h.clear();
   int32_t c = 0;
   for (int m = 0; m < B.nmesh; ++m) {
     B.dof_off[m] = c;
-    for (int32_t v = B.node_off[m]; v < B.node_off[m + 1]; ++v)
+    for (int32_t v = B.node_off[m]; v < B.node_off[m + 1]; ++v) {
+      const int32_t c0 = c;
       for (int d = 0; d < 3; ++d)
         if (nda[3 * (size_t)v + d] >= 0) {
           if (nda[3 * (size_t)v + d] != c) return MPF_ERR_STATE;  // free dofs must be node-major
           ++c;
         }
+      if (c > c0) B.gs_h.push_back(c0);
+    }
   }
   B.dof_off[B.nmesh] = c;
+  B.gs_h.push_back(c);
   if (c == 0) return MPF_ERR_ARG;  // nothing to solve
+  {  // row groups share their column lists? (the node-to-node pattern makes them; checked once)
+    const int32_t ng = (int32_t)B.gs_h.size() - 1;
+    MPF_TRY(B.gs.upload(B.gs_h.data(), B.gs_h.size(), p->stream));
+    hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, p->stream, p->bad.ptr, 1);
+    hipLaunchKernelGGL(k_b_check_groups, dim3(grid_for(ng, 256)), dim3(256), 0, p->stream, ng, B.gs.ptr,
+                       p->pat.rowptr.ptr, p->pat.colidx.ptr, p->bad.ptr);
+    int32_t bad = 1;
+    MPF_HIP(hipMemcpyAsync(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+    MPF_HIP(hipStreamSynchronize(p->stream));
+    B.grp_spmv = bad == 0;
+  }
   const size_t n = c;
   MPF_TRY(B.dinv.alloc(n));
   MPF_TRY(B.r.alloc(n));
