@@ -258,6 +258,25 @@ def bench_full(M, a) -> int:
     return 0
 
 
+def measured_copy_gbs(reps: int = 5) -> float:
+    """Device-to-device copy bandwidth (read + write bytes per second) of 1 GiB, HIP events."""
+    import torch
+
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    return 2.0 * n * 8 / (ms * 1e-3) / 1e9
+
+
 def copy_calibration() -> int:
     """Three device-to-device copies of 1 GiB: what FETCH_SIZE / WRITE_SIZE read for known traffic."""
     import torch
@@ -375,6 +394,9 @@ def main(argv=None) -> int:
                 "unique_frac": round(unique / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "fp64_tflops": round(flops / (ms_kern * 1e-3) / 1e12, 3),
                 "fp64_frac": round(flops / (ms_kern * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
+    copy_gbs = measured_copy_gbs()
+    roofline["measured_copy_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
     if wl == "tet10":
         metric = "Melem/s assembly (Tet10/P1 tangent+residual into FP64 CSR)"
         desc = f"Tet10/P1 mixed Neo-Hookean, {n}^3x6 Kuhn tets ({nElem} elements), one tangent+residual " \
