@@ -107,7 +107,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 //       in 16 x 16 tiles of the lower triangle on the matrix cores (v_mfma_f64_16x16x4f64, k = 16
 //       as 4 MFMAs), kNH tiles per wavefront at a time with their old values loaded first.
 __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
-                                                     int32_t astride) {
+                                                     int32_t astride, int32_t* __restrict__ factored) {
   extern __shared__ double sP[];
   const int blk = blockIdx.x;
   if (active && !active[(size_t)blk * astride]) return;
@@ -217,6 +217,7 @@ __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, c
     }
     __syncthreads();
   }
+  if (tid == 0) factored[blk] = 1;  // the band now holds this block's factors
 }
 
 // Solve L D L^T x = y in place for one block per workgroup (skipped like the factorisation), in
@@ -326,6 +327,7 @@ struct BandWork {
   DevBuf<int64_t> boff;
   DevBuf<double> band, y;
   DevBuf<int32_t> bad;
+  DevBuf<int32_t> bok;    // per block: 1 once the band holds its factors
   bool factored = false;  // a factorisation has been enqueued since the layout was built
 };
 
@@ -487,6 +489,10 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   MPF_TRY(W.band.alloc(W.nband > 0 ? (size_t)W.nband : 1));
   MPF_TRY(W.y.alloc(n > 0 ? (size_t)n : 1));
   MPF_TRY(W.bad.alloc(1));
+  {
+    const std::vector<int32_t> zero(std::max(nblk, 1), 0);
+    MPF_TRY(W.bok.upload(zero.data(), zero.size(), s));
+  }
   MPF_HIP(hipStreamSynchronize(s));
   W.key = key;
   return MPF_OK;
@@ -505,7 +511,7 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   hipLaunchKernelGGL(k_band_fill, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, P.rowptr.ptr, P.colidx.ptr,
                      p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
   const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
-  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride);
+  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
                      W->y.ptr, active, astride, p->du.ptr);
@@ -515,7 +521,8 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
 }
 
 // Enqueue: du = K^-1 rhs for every block with the factors left by the last solve of that block
-// (each block's own last factorisation; blocks skipped since keep theirs).  MPF_ERR_STATE before any.
+// (each block's own last factorisation; blocks skipped since keep theirs); du = 0 on a block that
+// was never factored (a mesh converged at its start).  MPF_ERR_STATE before any factorisation.
 mpf_status band_enqueue_resolve(mpf_problem* p) {
   BandWork* W = p->band;
   if (!W || !W->factored) return MPF_ERR_STATE;
@@ -523,9 +530,9 @@ mpf_status band_enqueue_resolve(mpf_problem* p) {
   BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
   hipLaunchKernelGGL(k_band_rhs, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->iperm.ptr, p->rhs.ptr,
                      W->y.ptr);
-  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, nullptr, 0);
+  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, W->bok.ptr, 1);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
-                     W->y.ptr, nullptr, 0, p->du.ptr);
+                     W->y.ptr, W->bok.ptr, 1, p->du.ptr);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

@@ -109,6 +109,24 @@ def test_tangent_increment_solves_the_tangent_system(mpf, gpu):
     assert torch.equal(d, d2)
 
 
+def test_tangent_increment_of_a_mesh_never_factored_is_zero(mpf, gpu):
+    """A mesh with no load converges at its start and is never factored: its tangent increment is
+    zero (not a solve with an empty band), the others' are finite."""
+    import torch
+
+    load = np.array([1.0, 0.0, 1.0, 1.0])
+    B, P = mpf.batch_problem(EDGES, E, NU, V0 * load, V1 * load)
+    P.prepare_matrix_pattern()
+    iters, rel, rep = P.batch_newton_step(0.5, itmax=25, rtol=1e-11, solver=mpf.BATCH_SOLVER_BAND)
+    assert iters[1] == 1 and np.all(iters[[0, 2, 3]] > 1), iters
+    d = torch.zeros(3 * P.nNode, dtype=torch.float64, device="cuda")
+    P.batch_tangent_increment(1.0, d)
+    dh = d.cpu().numpy()
+    assert np.all(np.isfinite(dh))
+    assert np.all(dh[3 * B.node_off[1]:3 * B.node_off[2]] == 0.0)
+    assert np.abs(dh[3 * B.node_off[0]:3 * B.node_off[1]]).max() > 0.0
+
+
 def test_batched_graph_replay_equals_direct_launches(mpf, gpu):
     _, Pg = _batch(mpf)
     _, Pe = _batch(mpf)
