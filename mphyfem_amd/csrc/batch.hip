@@ -547,6 +547,12 @@ mpf_status batch_node_offsets(const mpf_problem* p, std::vector<int32_t>& off) {
   return MPF_OK;
 }
 
+mpf_status batch_elem_offsets(const mpf_problem* p, std::vector<int32_t>& off) {
+  if (!p->batch || p->batch->nmesh <= 0) return MPF_ERR_STATE;
+  off = p->batch->elem_off;
+  return MPF_OK;
+}
+
 void batch_free(mpf_problem* p) {
   delete p->batch;
   p->batch = nullptr;

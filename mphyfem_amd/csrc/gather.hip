@@ -187,6 +187,27 @@ __global__ void k_morton(int32_t nNode, const double* __restrict__ X, double x0,
   id[i] = i;
 }
 
+// Batched meshes (node ranges [moff[m], moff[m + 1])): mesh index above a Morton code of the node on
+// its own mesh's lattice (origin mq[4m..4m+2], inverse spacing mq[4m+3], 15 bits per axis), so the
+// order runs mesh by mesh even where the meshes overlap in space.
+__global__ void k_morton_mesh(int32_t nNode, int32_t nmesh, const int32_t* __restrict__ moff,
+                              const double* __restrict__ mq, const double* __restrict__ X,
+                              uint64_t* __restrict__ code, int32_t* __restrict__ id) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nNode) return;
+  int lo = 0, hi = nmesh - 1;  // last mesh whose first node is <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (moff[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const double* q = mq + 4 * lo;
+  uint64_t c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c[d] = (uint64_t)fmin(fmax((X[3 * (size_t)i + d] - q[d]) * q[3] + 0.5, 0.0), 32767.0);
+  code[i] = ((uint64_t)lo << 45) | spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+  id[i] = i;
+}
+
 // XCD-aware chunk order: consecutive workgroups land on different XCDs (round robin), so each XCD is
 // given a contiguous 1/8 of the work.  Work is laid out in Morton order of the nodes, so that share is a
 // compact region of the mesh whose elements' staging records stay in that XCD's L2.
@@ -1113,8 +1134,42 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
     if (!(h > 0.0)) h = 1.0;
     h = std::max(h, emax / 2097151.0);
     const double sc[3] = {1.0 / h, 1.0 / h, 1.0 / h};
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, p->X.ptr, p->xmin[0], p->xmin[1],
-                       p->xmin[2], sc[0], sc[1], sc[2], code.ptr, ids.ptr);
+    std::vector<int32_t> moff, eoff;
+    const bool multi = p->batch && batch_node_offsets(p, moff) == MPF_OK && batch_elem_offsets(p, eoff) == MPF_OK &&
+                       moff.size() > 2 && moff.size() < (1u << 18);
+    DevBuf<int32_t> moff_d;
+    DevBuf<double> mq_d;
+    if (multi) {  // each mesh on its own lattice, h_m = (box volume / elements)^(1/3) of that mesh
+      const int nm = (int)moff.size() - 1;
+      std::vector<double> Xh((size_t)nNode * 3), mq(4 * (size_t)nm);
+      MPF_TRY(p->X.download(Xh.data(), Xh.size(), s));
+      MPF_HIP(hipStreamSynchronize(s));
+      for (int m = 0; m < nm; ++m) {
+        double lo3[3] = {1e300, 1e300, 1e300}, hi3[3] = {-1e300, -1e300, -1e300};
+        for (int32_t v = moff[m]; v < moff[m + 1]; ++v)
+          for (int d = 0; d < 3; ++d) {
+            lo3[d] = std::min(lo3[d], Xh[3 * (size_t)v + d]);
+            hi3[d] = std::max(hi3[d], Xh[3 * (size_t)v + d]);
+          }
+        const int32_t ne = eoff[m + 1] - eoff[m];
+        double e[3], hm = 0.0;
+        for (int d = 0; d < 3; ++d) e[d] = moff[m + 1] > moff[m] ? hi3[d] - lo3[d] : 0.0;
+        const double em = std::max(e[0], std::max(e[1], e[2]));
+        if (e[0] > 0.0 && e[1] > 0.0 && e[2] > 0.0 && ne > 0) hm = std::cbrt(e[0] * e[1] * e[2] / (double)ne);
+        if (!(hm > 0.0)) hm = ne > 0 ? em / std::cbrt((double)ne) : 1.0;
+        if (!(hm > 0.0)) hm = 1.0;
+        hm = std::max(hm, em / 32767.0);
+        for (int d = 0; d < 3; ++d) mq[4 * (size_t)m + d] = moff[m + 1] > moff[m] ? lo3[d] : 0.0;
+        mq[4 * (size_t)m + 3] = 1.0 / hm;
+      }
+      MPF_TRY(moff_d.upload(moff.data(), moff.size(), s));
+      MPF_TRY(mq_d.upload(mq.data(), mq.size(), s));
+      hipLaunchKernelGGL(k_morton_mesh, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, nm, moff_d.ptr, mq_d.ptr,
+                         p->X.ptr, code.ptr, ids.ptr);
+    } else {
+      hipLaunchKernelGGL(k_morton, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, p->X.ptr, p->xmin[0],
+                         p->xmin[1], p->xmin[2], sc[0], sc[1], sc[2], code.ptr, ids.ptr);
+    }
     size_t tmp = 0;
     MPF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, code.ptr, code2.ptr, ids.ptr, P.g_order.ptr, nNode, 0, 63,
                                                s));

@@ -202,6 +202,7 @@ mpf_status build_pattern(mpf_problem* p);          // pattern.hip
 ElemArgs batch_elem_args(mpf_problem* p, ElemArgs a);
 void batch_free(mpf_problem* p);
 mpf_status batch_node_offsets(const mpf_problem* p, std::vector<int32_t>& off);
+mpf_status batch_elem_offsets(const mpf_problem* p, std::vector<int32_t>& off);
 mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                      const double* mu, const double* Kinv);
 mpf_status batch_autotune(mpf_problem* p, int32_t* T, int32_t* LPR, double* ms_per_iter);
