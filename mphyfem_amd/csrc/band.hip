@@ -237,6 +237,15 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
   // ---- forward: L z = y (unit lower) ----
   for (int j0 = 0; j0 < n; j0 += kW) {
     const int w = min(kW, n - j0);
+    // the coupling rows' band entries and right-hand sides do not depend on this block's z: load
+    // them before the block is solved (b < kBS, so one row per thread)
+    const int r = j0 + w + tid;
+    const bool crow = r < min(n, j0 + w + b);
+    double lc[kW];
+#pragma unroll
+    for (int k = 0; k < kW; ++k)
+      lc[k] = (crow && k < w && r - (j0 + k) <= b) ? L[(size_t)(j0 + k) * B1 + (r - (j0 + k))] : 0.0;
+    const double yr = crow ? y[r] : 0.0;
     if (wid == 0) {
       double l[kW];  // lane r: L(r, c), c < r
 #pragma unroll
@@ -256,12 +265,10 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
       }
     }
     __syncthreads();
-    for (int r = j0 + w + tid; r < min(n, j0 + w + b); r += kBS) {
-      double acc = y[r];
-      for (int k = 0; k < w; ++k) {
-        const int dist = r - (j0 + k);
-        if (dist <= b) acc -= L[(size_t)(j0 + k) * B1 + dist] * sy[k];
-      }
+    if (crow) {
+      double acc = yr;
+      for (int k = 0; k < w; ++k)
+        if (r - (j0 + k) <= b) acc -= lc[k] * sy[k];
       y[r] = acc;
     }
     __syncthreads();
@@ -273,6 +280,12 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
   const int nb = (n + kW - 1) / kW;
   for (int q = nb - 1; q >= 0; --q) {
     const int j0 = q * kW, w = min(kW, n - j0);
+    double lt[kW];  // wavefront 0, lane r: L(c, r), c > r, loaded before the row sums
+    if (wid == 0) {
+#pragma unroll
+      for (int c = 0; c < kW; ++c)
+        lt[c] = (lane < w && c < w && c > lane && c - lane <= b) ? L[(size_t)(j0 + lane) * B1 + (c - lane)] : 0.0;
+    }
     if (wid < w) {  // row j0 + wid: minus the rows already solved beyond the block
       const int k = j0 + wid;
       double part = 0.0;
@@ -282,16 +295,12 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
     }
     __syncthreads();
     if (wid == 0) {
-      double l[kW];  // lane r: L(c, r), c > r
-#pragma unroll
-      for (int c = 0; c < kW; ++c)
-        l[c] = (lane < w && c < w && c > lane && c - lane <= b) ? L[(size_t)(j0 + lane) * B1 + (c - lane)] : 0.0;
       double x = lane < w ? sy[lane] : 0.0;
 #pragma unroll
       for (int c = kW - 1; c >= 0; --c) {
         if (c < w) {
           const double xc = lane_value(x, c);
-          if (lane < c) x -= l[c] * xc;
+          if (lane < c) x -= lt[c] * xc;
         }
       }
       if (lane < w) y[j0 + lane] = x;
@@ -480,6 +489,7 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   W.bstart_h = bstart;
   const int lds = (int)(((size_t)(kW + W.bmax) * kW + kW) * sizeof(double));
   if (lds > 64 * 1024) return MPF_ERR_ARG;  // half-bandwidth beyond the panel's LDS (about 490 dofs)
+  if (W.bmax >= kBS) return MPF_ERR_ARG;     // the solve gives each coupling row one thread
   MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
   MPF_TRY(W.iperm.upload(iperm.data(), iperm.size(), s));
   MPF_TRY(W.pblk.upload(pblk.data(), pblk.size(), s));
