@@ -58,24 +58,25 @@ __global__ void __launch_bounds__(256) k_band_zero(int32_t n, const int32_t* __r
   for (int k = lane; k <= b; k += 64) col[k] = 0.0;
 }
 
-// Lower band of every active block from the CSR rows (one lane per row), and the permuted
-// right-hand side.
-__global__ void k_band_fill(int32_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-                            const double* __restrict__ vals, const double* __restrict__ rhs,
-                            const int32_t* __restrict__ iperm, const int32_t* __restrict__ pblk, BandArgs A,
-                            const int32_t* __restrict__ active, int32_t astride) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// Lower band of every active block from the CSR rows (one wavefront per row, lanes over its
+// entries, 4 rows per workgroup), and the permuted right-hand side.
+__global__ void __launch_bounds__(256) k_band_fill(int32_t nrows, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ colidx,
+                                                   const double* __restrict__ vals, const double* __restrict__ rhs,
+                                                   const int32_t* __restrict__ iperm, const int32_t* __restrict__ pblk,
+                                                   BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= nrows) return;
   const int i = iperm[r];
   const int blk = pblk[i];
   if (active && !active[(size_t)blk * astride]) return;
   const int s = A.bstart[blk], b = A.bw[blk];
   double* L = A.band + A.boff[blk];
-  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+  for (int k = rowptr[r] + lane; k < rowptr[r + 1]; k += 64) {
     const int j = iperm[colidx[k]];
     if (j <= i) L[(size_t)(j - s) * (b + 1) + (i - j)] = vals[k];
   }
-  A.y[i] = rhs[r];
+  if (lane == 0) A.y[i] = rhs[r];
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -518,8 +519,8 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
   hipLaunchKernelGGL(k_band_zero, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, W->pblk.ptr, A, active,
                      astride);
-  hipLaunchKernelGGL(k_band_fill, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, P.rowptr.ptr, P.colidx.ptr,
-                     p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
+  hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, P.rowptr.ptr,
+                     P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
   const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
   hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
