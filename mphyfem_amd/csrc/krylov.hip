@@ -24,6 +24,7 @@ constexpr int kMaxRestart = 200;  // GMRES(m) upper bound (Hessenberg column scr
 struct KrylovWork {
   int64_t n = 0;
   DevBuf<double> r, z, p, q, w;
+  DevBuf<double> r2;          // PCG residual, the other half of the pair (alternates by iteration)
   DevBuf<double> part;        // partial sums [kMaxParts * 64]
   DevBuf<double> sc;          // device scalars
   DevBuf<double> V;           // GMRES basis [(m+1) * n]
@@ -31,6 +32,7 @@ struct KrylovWork {
   // preconditioner blocks
   DevBuf<int32_t> blk_start;
   DevBuf<int32_t> blk_size;
+  DevBuf<int32_t> row_blk;    // block of each row
   DevBuf<double> blk_inv;
   int64_t nblk = 0;
   int pc_kind = -1;
@@ -227,28 +229,49 @@ __global__ void __launch_bounds__(kRB) k_cg_start(int64_t nblk, const int32_t* _
   if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[np + blockIdx.x] = t2; }
 }
 
-// x += alpha p ; r -= alpha q ; z = M r ; partials of r.z and r.r.   alpha = rz[par] / pq
-__global__ void __launch_bounds__(kRB) k_cg_update(int64_t nblk, const int32_t* __restrict__ bs,
-                                                   const int32_t* __restrict__ bm, const double* __restrict__ binv,
-                                                   double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ z, const double* __restrict__ p,
-                                                   const double* __restrict__ q, double* __restrict__ part, int np,
-                                                   const double* __restrict__ sc, int par) {
+// One thread per row: x += alpha p ; r_out = r_in - alpha q ; z = M r_out with the row's 3x3 block
+// (its block's r_in and q are read, only the row itself is written, so rows of one block never
+// race); partials of r.z and r.r.   alpha = rz[par] / pq
+__global__ void __launch_bounds__(kRB) k_cg_update_rows(int64_t n, const int32_t* __restrict__ rblk,
+                                                        const int32_t* __restrict__ bs, const int32_t* __restrict__ bm,
+                                                        const double* __restrict__ binv, double* __restrict__ x,
+                                                        const double* __restrict__ rin, double* __restrict__ rout,
+                                                        double* __restrict__ z, const double* __restrict__ p,
+                                                        const double* __restrict__ q, double* __restrict__ part, int np,
+                                                        const double* __restrict__ sc, int par) {
   if (sc[S_DONE] != 0.0) return;
   __shared__ double sh[kRB / 64];
   const double alpha = sc[S_RZ0 + par] / sc[S_PQ];
   double rz = 0.0, rr = 0.0;
-  for (int64_t k = (int64_t)blockIdx.x * kRB + threadIdx.x; k < nblk; k += (int64_t)gridDim.x * kRB) {
-    const int s = bs[k], m = bm[k];
-    for (int i = 0; i < m; ++i) {
-      x[s + i] += alpha * p[s + i];
-      r[s + i] -= alpha * q[s + i];
+  for (int64_t i = (int64_t)blockIdx.x * kRB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRB) {
+    x[i] += alpha * p[i];
+    const int k = rblk[i], s0 = bs[k], m = bm[k], li = (int)(i - s0);
+    if (li < 0 || li >= m || m > 3) {  // not reached: every row lies in its block
+      rout[i] = rin[i] - alpha * q[i];
+      z[i] = 0.0;
+      continue;
     }
-    apply_block(s, m, binv + 9 * k, r, z, rz, rr);
+    double rv[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < m; ++j) rv[j] = rin[s0 + j] - alpha * q[s0 + j];
+    const double* Bi = binv + 9 * (size_t)k + 3 * li;
+    double zi = 0.0;
+    for (int j = 0; j < m; ++j) zi += Bi[j] * rv[j];
+    rout[i] = rv[li];
+    z[i] = zi;
+    rz += rv[li] * zi;
+    rr += rv[li] * rv[li];
   }
   const double t1 = block_sum(rz, sh);
   const double t2 = block_sum(rr, sh);
   if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[np + blockIdx.x] = t2; }
+}
+
+// row -> block of the preconditioner's block list
+__global__ void k_row_blk(int64_t nblk, const int32_t* __restrict__ bs, const int32_t* __restrict__ bm,
+                          int32_t* __restrict__ rblk) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  for (int j = 0; j < bm[k]; ++j) rblk[bs[k] + j] = (int32_t)k;
 }
 
 // reduce the two partial arrays into rz[1-par], rr; convergence test
@@ -529,6 +552,7 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
   if (W.n != n) {
     W.n = n;
     MPF_TRY(W.r.alloc(n)); MPF_TRY(W.z.alloc(n)); MPF_TRY(W.p.alloc(n)); MPF_TRY(W.q.alloc(n)); MPF_TRY(W.w.alloc(n));
+    MPF_TRY(W.r2.alloc(n));
     MPF_TRY(W.sc.alloc(S_N + kMaxRestart + 2));
     W.pc_kind = -1;
   }
@@ -571,6 +595,10 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
       MPF_HIP(hipStreamSynchronize(s));
     }
     MPF_TRY(W.blk_inv.alloc((size_t)W.nblk * 9));
+    MPF_TRY(W.row_blk.alloc(n > 0 ? (size_t)n : 1));
+    MPF_HIP(hipMemsetAsync(W.row_blk.ptr, 0, (n > 0 ? (size_t)n : 1) * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_row_blk, dim3(grid_for(W.nblk, B)), dim3(B), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
+                       W.row_blk.ptr);
     W.pc_kind = kind;
     W.grp_spmv = false;
     if (kind == 2 && W.nblk > 0) {
@@ -621,9 +649,11 @@ static mpf_status pcg(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_repor
       const int par = it & 1;
       MPF_TRY(spmv_k(p, W, W.p.ptr, W.q.ptr, W.part.ptr, nps, sc, s));
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, (int)nps, 1, sc + S_PQ);
-      hipLaunchKernelGGL(k_cg_update, dim3(np), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
-                         W.blk_inv.ptr, x, W.r.ptr, W.z.ptr, W.p.ptr, W.q.ptr, W.part.ptr, (int)np, sc, par);
-      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)np, sc, par, it);
+      double* rb[2] = {W.r.ptr, W.r2.ptr};
+      hipLaunchKernelGGL(k_cg_update_rows, dim3(nps), dim3(kRB), 0, s, n, W.row_blk.ptr, W.blk_start.ptr,
+                         W.blk_size.ptr, W.blk_inv.ptr, x, rb[par], rb[1 - par], W.z.ptr, W.p.ptr, W.q.ptr, W.part.ptr,
+                         (int)nps, sc, par);
+      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, sc, par, it);
       hipLaunchKernelGGL(k_cg_p, dim3(nps), dim3(kRB), 0, s, n, W.z.ptr, W.p.ptr, sc, par);
     }
     MPF_HIP(hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, s));
