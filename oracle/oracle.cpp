@@ -657,7 +657,7 @@ int assemble_range(orc_problem* p, int first, int count, bool firstIter, double*
 // ---------------------------------------------------------------------------
 // CPU Krylov solvers and a banded direct solver (stand-ins for PETSc KSP,
 // SolverPetsc.cpp:250-330; the reference's KSP iterates are not reproducible
-// without PETSc, solution parity is judged by the residual).
+// without PETSc, solution parity is checked through the residual).
 // ---------------------------------------------------------------------------
 
 void spmv(const orc_problem* p, const double* vals, const double* x, double* y) {
