@@ -385,6 +385,9 @@ def test_band_direct_solve_vs_oracle(mpf, orc, gpu):
     assert np.linalg.norm(O.spmv(vals, du) - rhs) / np.linalg.norm(rhs) < 1e-12
     P.factorise_and_solve(ksp=mpf.KSP_BAND)
     assert np.array_equal(P.solution(), du)
+    nblk, bmax, ndof, flops = P.band_stats()  # one block: the whole displacement system
+    assert nblk == 1 and ndof == len(rhs) and 0 < bmax < ndof
+    assert flops == float(ndof) * bmax * (bmax + 1)
 
 
 def test_band_rejects_mixed_system(mpf, gpu):
