@@ -278,23 +278,28 @@ def block_newton_opts(P: Problem):
 _CUT_STATUS = (ERR_NEG_JACOBIAN, ERR_NOT_CONVERGED, ERR_DIVERGED)
 
 
-def newton_with_cut(P: Problem, lam0: float, lam1: float, opts, guess=None, depth: int = 0, max_depth: int = 6):
+def newton_with_cut(P: Problem, lam0: float, lam1: float, opts, guess=None, depth: int = 0, max_depth: int = 6,
+                    log=None):
     """Newton from the converged state at lam0 to lam1, started from guess (or plainly).  On a negative
     Jacobian, divergence or no convergence the state is restored and the increment halved, both
     halves started plainly (MyTime::cut, MyTime.cpp:26-106; reset, SolutionDataSolid.cpp:157-167),
     down to 1/64 of the increment.  Returns (assemblies and Krylov iterations of the converged
-    solves, number of cuts)."""
+    solves, number of cuts).  log: called after every failed attempt and converged sub-step."""
     ustart, _ = P.state()
     try:
         it, rep = P.newton_step(lam1, opts, guess)
+        if log and depth > 0:
+            log(f"  sub-step to lambda {lam1:.6f} converged: {it} assemblies, {rep.krylov_iters} PCG iterations")
         return it, rep.krylov_iters, 0
     except MpfError as e:
         if e.status not in _CUT_STATUS or depth >= max_depth:
             raise
+        if log:
+            log(f"  Newton to lambda {lam1:.6f} failed ({e}); cutting the increment in half")
     P.set_state(ustart)
     mid = 0.5 * (lam0 + lam1)
-    a1, k1, c1 = newton_with_cut(P, lam0, mid, opts, None, depth + 1, max_depth)
-    a2, k2, c2 = newton_with_cut(P, mid, lam1, opts, None, depth + 1, max_depth)
+    a1, k1, c1 = newton_with_cut(P, lam0, mid, opts, None, depth + 1, max_depth, log)
+    a2, k2, c2 = newton_with_cut(P, mid, lam1, opts, None, depth + 1, max_depth, log)
     return a1 + a2, k1 + k2, 1 + c1 + c2
 
 
@@ -325,7 +330,7 @@ def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, sto
             else:
                 net.guess(P, hist[1], hist[0], 1.0, False, guess)
             g = guess.cpu().numpy()
-        it, kry, cuts = newton_with_cut(P, k / BLOCK_STEPS, lam, opts, g)
+        it, kry, cuts = newton_with_cut(P, k / BLOCK_STEPS, lam, opts, g, log=log)
         dt = time.perf_counter() - t0
         hist[0].copy_(hist[1])
         P.copy_state_device(hist[1])
