@@ -372,10 +372,13 @@ int elem_disp(int npe, const double (*X)[3], const double (*u)[3], double mu, do
 
 // ElementBase::calcStiffnessAndResidualMixed3D (ElementBase2.cpp:278-552) for Tet10/P1
 // (Elem_Solid_3D_Mixed1.cpp:34-76: nGP = 4, nlbfP = 4, pressure nodes = first 4 nodes).
-int elem_tet10p1(const double (*X)[3], const double (*u)[3], const double (*up)[3], const double* presn,
-                 double mu, double Kinv, int Utype, double* Kuu, double* Kup, double* Kpu, double* Kpp,
-                 double* Fu, double* Fp) {
-  const int npe = 10, nsize = 30, nP = 4;
+// ElementBase::calcStiffnessAndResidualMixed3D (ElementBase2.cpp:278-552) for npe displacement nodes
+// and nP pressure functions: nP = 4 the P1 corner functions of a Tet10 (Elem_Solid_3D_Mixed1), nP = 1
+// the element-constant pressure of Elem_Solid_3D_Mixed0 (Np = 1, pres = presDOF[0], :389-393).
+static int elem_mixed(int npe, int nP, const double (*X)[3], const double (*u)[3], const double (*up)[3],
+                      const double* presn, double mu, double Kinv, int Utype, double* Kuu, double* Kup, double* Kpu,
+                      double* Kpp, double* Fu, double* Fp) {
+  const int nsize = 3 * npe;
   std::fill(Kuu, Kuu + nsize * nsize, 0.0);
   std::fill(Kup, Kup + nsize * nP, 0.0);
   std::fill(Kpu, Kpu + nP * nsize, 0.0);
@@ -402,7 +405,8 @@ int elem_tet10p1(const double (*X)[3], const double (*u)[3], const double (*up)[
     if (detF < 0.0) return 1;
     basis3d(npe, xi[gp], xcur, N, dx, dy, dz, Jac);
     const double dvol = w[gp] * Jac;
-    basis_tetra(4, xi[gp][0], xi[gp][1], xi[gp][2], Np, nullptr, nullptr, nullptr);
+    if (nP == 1) Np[0] = 1.0;
+    else basis_tetra(4, xi[gp][0], xi[gp][1], xi[gp][2], Np, nullptr, nullptr, nullptr);
     double pres = 0.0;
     for (int q = 0; q < nP; ++q) pres += (Np[q] * presn[q]);
     neohooke(true, Utype, Kinv, mu, F, pres, stre, C);
@@ -455,6 +459,31 @@ int elem_tet10p1(const double (*X)[3], const double (*u)[3], const double (*up)[
       bb5 *= thetahat * af;
       for (int jj = 0; jj < nP; ++jj) Kpp[ii * nP + jj] -= bb5 * Np[jj];
     }
+  }
+  return 0;
+}
+
+int elem_tet10p1(const double (*X)[3], const double (*u)[3], const double (*up)[3], const double* presn,
+                 double mu, double Kinv, int Utype, double* Kuu, double* Kup, double* Kpu, double* Kpp,
+                 double* Fu, double* Fp) {
+  return elem_mixed(10, 4, X, u, up, presn, mu, Kinv, Utype, Kuu, Kup, Kpu, Kpp, Fu, Fp);
+}
+
+// Elem_Solid_3D_Mixed0::calcStiffnessAndResidual (Elem_Solid_3D_Mixed0.cpp:93-104): the Hex8 mixed
+// element with one element pressure, condensed:  K = Kuu - Kup (Kpu / Kpp),  F = Fu - Kup (Fp / Kpp).
+// The uncondensed Kuu, Fu and Kup, Kpu, Kpp, Fp are returned too (solveForPressure, :112-199:
+// p += (Fp - Kpu du_e) / Kpp).
+int elem_hex8p0(const double (*X)[3], const double (*u)[3], const double (*up)[3], double pe, double mu,
+                double Kinv, int Utype, double* Kc, double* Fc, double* Kuu, double* Fu, double* Kup, double* Kpu,
+                double* Kpp, double* Fp) {
+  const int nsize = 24;
+  if (elem_mixed(8, 1, X, u, up, &pe, mu, Kinv, Utype, Kuu, Kup, Kpu, Kpp, Fu, Fp)) return 1;
+  double kpu_s[24];
+  for (int j = 0; j < nsize; ++j) kpu_s[j] = Kpu[j] / Kpp[0];
+  const double fp_s = Fp[0] / Kpp[0];
+  for (int i = 0; i < nsize; ++i) {
+    for (int j = 0; j < nsize; ++j) Kc[i * nsize + j] = Kuu[i * nsize + j] - Kup[i] * kpu_s[j];
+    Fc[i] = Fu[i] - Kup[i] * fp_s;
   }
   return 0;
 }
@@ -935,6 +964,13 @@ int orc_elem_disp_consistent(int npe, const double* X, const double* u, double m
 }
 
 void orc_set_tangent(orc_problem* p, int consistent) { p->consistent = consistent != 0; }
+
+int orc_elem_hex8p0(const double* X, const double* u, const double* uprev, double pe, double mu, double Kinv,
+                    int Utype, double* Kc, double* Fc, double* Kuu, double* Fu, double* Kup, double* Kpu, double* Kpp,
+                    double* Fp) {
+  return elem_hex8p0((const double(*)[3])X, (const double(*)[3])u, (const double(*)[3])uprev, pe, mu, Kinv, Utype,
+                     Kc, Fc, Kuu, Fu, Kup, Kpu, Kpp, Fp);
+}
 
 int orc_elem_tet10p1(const double* X, const double* u, const double* uprev, const double* pres, double mu,
                      double Kinv, int Utype, double* Kuu, double* Kup, double* Kpu, double* Kpp, double* Fu,

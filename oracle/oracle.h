@@ -55,6 +55,11 @@ int  orc_elem_disp_consistent(int npe, const double* X, const double* u, double 
 /* ElementBase::calcStiffnessAndResidualMixed3D (ElementBase2.cpp:278-552), Tet10/P1.
  * X,u,uprev: [10][3]; pres: [4] (corner values).  Kuu [30][30], Kup [30][4],
  * Kpu [4][30], Kpp [4][4], Fu [30], Fp [4]. */
+/* Hex8 Q1/P0 (Elem_Solid_3D_Mixed0): condensed Kc[24x24], Fc[24]; uncondensed Kuu, Fu, Kup[24],
+   Kpu[24], Kpp[1], Fp[1] */
+int  orc_elem_hex8p0(const double* X, const double* u, const double* uprev, double pe, double mu, double Kinv,
+                     int Utype, double* Kc, double* Fc, double* Kuu, double* Fu, double* Kup, double* Kpu,
+                     double* Kpp, double* Fp);
 int  orc_elem_tet10p1(const double* X, const double* u, const double* uprev, const double* pres,
                       double mu, double Kinv, int Utype,
                       double* Kuu, double* Kup, double* Kpu, double* Kpp, double* Fu, double* Fp);

@@ -38,6 +38,8 @@ def lib() -> C.CDLL:
             "orc_elem_disp": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, _D]),
             "orc_elem_disp_consistent": (C.c_int, [C.c_int, _D, _D, C.c_double, C.c_double, C.c_int, _D, _D]),
             "orc_set_tangent": (None, [C.c_void_p, C.c_int]),
+            "orc_elem_hex8p0": (C.c_int, [_D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          _D, _D, _D, _D, _D, _D, _D, _D]),
             "orc_elem_tet10p1": (C.c_int, [_D, _D, _D, _D, C.c_double, C.c_double, C.c_int,
                                            _D, _D, _D, _D, _D, _D]),
             "orc_problem_create": (C.c_void_p, [C.c_int, C.c_int, C.c_int, _D, _I, C.c_double, C.c_double, C.c_int]),
@@ -133,6 +135,16 @@ def elem_disp_consistent(X, u, mu, Kinv, Utype=3):
     Re = np.zeros(3 * npe)
     st = lib().orc_elem_disp_consistent(npe, _d(X), _d(u), mu, Kinv, Utype, _d(Ke), _d(Re))
     return Ke, Re, st
+
+
+def elem_hex8p0(X, u, uprev, pe, mu, Kinv, Utype=3):
+    """Hex8 Q1/P0: (Kc, Fc, Kuu, Fu, Kup, Kpu, Kpp, Fp, status)."""
+    Kc, Kuu = np.zeros((24, 24)), np.zeros((24, 24))
+    Fc, Fu, Kup, Kpu = np.zeros(24), np.zeros(24), np.zeros(24), np.zeros(24)
+    Kpp, Fp = np.zeros(1), np.zeros(1)
+    st = lib().orc_elem_hex8p0(_d(f64(X)), _d(f64(u)), _d(f64(uprev)), float(pe), mu, Kinv, Utype, _d(Kc), _d(Fc),
+                               _d(Kuu), _d(Fu), _d(Kup), _d(Kpu), _d(Kpp), _d(Fp))
+    return Kc, Fc, Kuu, Fu, Kup, Kpu, float(Kpp[0]), float(Fp[0]), st
 
 
 def elem_tet10p1(X, u, uprev, pres, mu, Kinv, Utype=3):

@@ -339,3 +339,70 @@ def test_newton_consistent_quadratic_and_tangent_independent_solution(orc):
             assert np.all(iters <= 5)
         sols.append(P.state()[0])
     assert normwise_rel(sols[0], sols[1]) < 1e-9
+
+
+def _hex8_element(jitter, seed):
+    X, conn = cube_hex(1, jitter=jitter, seed=seed)
+    return X[conn[0]]
+
+
+def test_hex8p0_element_closed_forms(orc):
+    """Hex8 Q1/P0 in its reference state (unit cube, V0 = 1, F = I, so the stress is p I):
+    Kpp = -V0/K, Fp = p V0/K, Kpu = Kup^T = int grad N dV with zero sum per direction, Fu = -p Kup."""
+    Xe = _hex8_element(0.0, 0)
+    u = np.zeros(24)
+    pe, mu, Kinv = 0.7, 50.0, 1e-2
+    Kc, Fc, Kuu, Fu, Kup, Kpu, Kpp, Fp, st = orc.elem_hex8p0(Xe, u, u, pe, mu, Kinv)
+    assert st == 0
+    assert abs(Kpp - (-Kinv)) < 1e-15 and abs(Fp - Kinv * pe) < 1e-15
+    assert np.array_equal(Kpu, Kup)
+    assert np.abs(Kup.reshape(8, 3).sum(axis=0)).max() < 1e-14
+    assert normwise_rel(Fu, -pe * Kup) < 1e-13
+
+
+def _hex8p0_state(orc):
+    rng = np.random.default_rng(11)
+    Xe = _hex8_element(0.0, 3) + 0.06 * rng.standard_normal((8, 3))  # a general hexahedron
+    u = 0.03 * rng.standard_normal(24)
+    up = 0.01 * rng.standard_normal(24)
+    return Xe, u, up, 0.4, 50.0, 1e-2
+
+
+def test_hex8p0_element_fd_tangent(orc):
+    """[[Kuu, Kup], [Kpu, Kpp]] against central differences of -[Fu; Fp] in (u, p)."""
+    Xe, u, up, pe, mu, Kinv = _hex8p0_state(orc)
+    _, _, Kuu, Fu, Kup, Kpu, Kpp, Fp, st = orc.elem_hex8p0(Xe, u, up, pe, mu, Kinv)
+    assert st == 0
+    K = np.zeros((25, 25))
+    K[:24, :24], K[:24, 24], K[24, :24], K[24, 24] = Kuu, Kup, Kpu, Kpp
+
+    def resid(z):
+        r = orc.elem_hex8p0(Xe, z[:24], up, z[24], mu, Kinv)
+        return np.concatenate([r[3], [r[7]]])
+
+    z0 = np.concatenate([u, [pe]])
+    h = 1e-6
+    Kfd = np.zeros((25, 25))
+    for j in range(25):
+        e = np.zeros(25)
+        e[j] = h
+        Kfd[:, j] = -(resid(z0 + e) - resid(z0 - e)) / (2 * h)
+    assert normwise_rel(Kfd, K) < 1e-6
+
+
+def test_hex8p0_condensation_identity(orc):
+    """Solving the condensed system gives the displacement of the full mixed system, and the
+    pressure recovery (Fp - Kpu du) / Kpp its pressure increment."""
+    Xe, u, up, pe, mu, Kinv = _hex8p0_state(orc)
+    Kc, Fc, Kuu, Fu, Kup, Kpu, Kpp, Fp, st = orc.elem_hex8p0(Xe, u, up, pe, mu, Kinv)
+    assert st == 0
+    K = np.zeros((25, 25))
+    K[:24, :24], K[:24, 24], K[24, :24], K[24, 24] = Kuu, Kup, Kpu, Kpp
+    keep = np.arange(3, 24)  # node 0 held: the element alone is singular in its rigid modes
+    full = np.concatenate([keep, [24]])
+    sol = np.linalg.solve(K[np.ix_(full, full)], np.concatenate([Fu, [Fp]])[full])
+    du = np.linalg.solve(Kc[np.ix_(keep, keep)], Fc[keep])
+    assert normwise_rel(du, sol[:-1]) < 1e-10
+    dfull = np.zeros(24)
+    dfull[keep] = du
+    assert abs((Fp - Kpu @ dfull) / Kpp - sol[-1]) <= 1e-10 * max(1.0, abs(sol[-1]))
