@@ -87,6 +87,17 @@ mpf_status mpf_elem_hex8_nh_batch(const double* X, const double* u, const int32_
                                   const mpf_matl* m, int32_t first, int32_t count,
                                   double* Ke, double* Re, int32_t* bad, mpf_stream_t stream);
 
+/* Hex8 Q1/P0 mixed element with static condensation (Elem_Solid_3D_Mixed0::calcStiffnessAndResidual, */
+/* Elem_Solid_3D_Mixed0.cpp:93-104, over ElementBase::calcStiffnessAndResidualMixed3D with Np = 1).   */
+/* pres_elem: [nElem] element pressures; u_prev as for Tet10/P1.  Kc [count][24][24] = Kuu - Kup     */
+/* (Kpu / Kpp), Fc [count][24] = Fu - Kup (Fp / Kpp), and for the pressure update after a solve       */
+/* (solveForPressure, :112-199: p += (Fp - Kpu du_e) / Kpp, Kpu = Kup^T) Kup [count][24], Kpp and Fp  */
+/* [count].  bad as for the displacement element.                                                   */
+mpf_status mpf_elem_hex8p0_nh_batch(const double* X, const double* u, const double* u_prev,
+                                    const double* pres_elem, const int32_t* conn, const mpf_matl* m,
+                                    int32_t first, int32_t count, double* Kc, double* Fc, double* Kup,
+                                    double* Kpp, double* Fp, int32_t* bad, mpf_stream_t stream);
+
 /* Tet10/P1 mixed element: ElementBase::calcStiffnessAndResidualMixed3D via                     */
 /* Elem_Solid_3D_Mixed1::calcStiffnessAndResidualMixed (ElementBase2.cpp:278-552;               */
 /* Elem_Solid_3D_Mixed1.cpp:89-97; ElementBase.h:171).  u_prev: previous converged displacement  */

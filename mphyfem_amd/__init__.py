@@ -97,6 +97,8 @@ SIGNATURES = [
     ("mpf_device_count", C.c_int32, []),
     ("mpf_neohooke_stress_tangent_batch", C.c_int, [_P, _P, C.POINTER(Matl), C.c_int32, _P, _P, _P]),
     ("mpf_elem_hex8_nh_batch", C.c_int, [_P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    ("mpf_elem_hex8p0_nh_batch", C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32,
+                                            _P, _P, _P, _P, _P, _P, _P]),
     ("mpf_elem_tet10p1_nh_batch", C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32,
                                              _P, _P, _P, _P, _P, _P, _P, _P]),
     ("mpf_problem_create", C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32, _DP, _I32P,
@@ -354,6 +356,26 @@ def elem_hex8_nh_batch(X, u, conn, m: Matl, first: int = 0, count: int | None = 
                                       Ke.data_ptr(), Re.data_ptr(), bad.data_ptr(), _stream())
     _check(st, "mpf_elem_hex8_nh_batch")
     return Ke, Re, int(bad.item())
+
+
+def elem_hex8p0_nh_batch(X, u, u_prev, pres_elem, conn, m: Matl, first: int = 0, count: int | None = None):
+    """Elem_Solid_3D_Mixed0 (Hex8 Q1/P0), condensed, for a batch of elements:
+    (Kc [n,24,24], Fc [n,24], Kup [n,24], Kpp [n], Fp [n], bad)."""
+    import torch
+    count = conn.shape[0] - first if count is None else count
+    dev = X.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    Kc = torch.empty((count, 24, 24), **f64)
+    Fc = torch.empty((count, 24), **f64)
+    Kup = torch.empty((count, 24), **f64)
+    Kpp = torch.empty((count,), **f64)
+    Fp = torch.empty((count,), **f64)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    st = lib().mpf_elem_hex8p0_nh_batch(X.data_ptr(), u.data_ptr(), u_prev.data_ptr(), pres_elem.data_ptr(),
+                                        conn.data_ptr(), C.byref(m), first, count, Kc.data_ptr(), Fc.data_ptr(),
+                                        Kup.data_ptr(), Kpp.data_ptr(), Fp.data_ptr(), bad.data_ptr(), _stream())
+    _check(st, "mpf_elem_hex8p0_nh_batch")
+    return Kc, Fc, Kup, Kpp, Fp, int(bad.item())
 
 
 def elem_tet10p1_nh_batch(X, u, u_prev, pres, conn, m: Matl, first: int = 0, count: int | None = None):

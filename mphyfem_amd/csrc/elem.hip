@@ -176,6 +176,104 @@ __global__ void __launch_bounds__(64) hex8_kernel(ElemArgs A, int32_t first, int
 }
 
 // ============================================================================================
+// Hex8 Q1/P0 mixed element, condensed (Elem_Solid_3D_Mixed0.cpp:93-104), dense outputs
+// ============================================================================================
+struct Hex8P0Smem {
+  double X[8][8][3];
+  double U[8][8][3];
+  double UP[8][8][3];
+  double Pe[8];              // element pressures
+  double G[8][8][8][6];      // [k][gp][a]: g_a (3), (b g)_a (3)
+  double S[8][8][8];         // [k][gp]: s1..s4, dv, dv0, thetahat, J - Jhat - thetahat p
+  double R[8][8][3];         // [k][a]: Fu
+  double Kup[8][24];         // [k]: int grad N dv
+  double Kpp[8], Fp[8];
+  double dN[8][3][8];
+};
+
+// One wavefront per 8 elements: phase 1 lane = (element, Gauss point); phase 2 lane = (a, b):
+// Kc(a,b) = Kuu(a,b) - Kup_a (Kpu_b / Kpp) with Kpu = Kup^T, Fc_a = Fu_a - Kup_a (Fp / Kpp).
+__global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t first, int32_t count,
+                                                          double* __restrict__ Kc, double* __restrict__ Fc,
+                                                          double* __restrict__ Kup, double* __restrict__ Kpp,
+                                                          double* __restrict__ Fp) {
+  __shared__ Hex8P0Smem sm;
+  const int lane = threadIdx.x;
+  const int base_idx = blockIdx.x * 8;
+  if (lane < 8) {
+    double d[3][8];
+    hex8_dN(lane, d);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int a = 0; a < 8; ++a) sm.dN[lane][r][a] = d[r][a];
+  }
+  {  // lane = (k, a): node data; idle slots of a partial block repeat the block's first element
+    const int k = lane >> 3, a = lane & 7;
+    const int idx = (base_idx + k < count) ? base_idx + k : base_idx;
+    const int e = first + idx;
+    const size_t nd = (size_t)A.conn[(size_t)e * 8 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sm.X[k][a][c] = A.X[3 * nd + c];
+      sm.U[k][a][c] = A.u[3 * nd + c];
+      sm.UP[k][a][c] = A.uprev[3 * nd + c];
+    }
+    if (a == 0) sm.Pe[k] = A.pres[e];
+  }
+  __syncthreads();
+  {
+    const int k = lane >> 3, gp = lane & 7;
+    const bool act = base_idx + k < count;
+    bool negJ = false;
+    double mu, K;
+    elem_material(A, first + (act ? base_idx + k : base_idx), mu, K);
+    element_point<8, 8, true>(sm.X[k], sm.U[k], sm.UP[k], &sm.Pe[k], sm.dN[gp], gp, 1.0, mu, K, A.Utype, false, true,
+                              &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
+    if (act && negJ) flag_bad(A.bad, first + base_idx + k);
+  }
+  __syncthreads();
+  {  // Kup (ElementBase2.cpp:500-505 with Np = 1), Kpp and Fp (:510-539): lane = (k, a)
+    const int k = lane >> 3, a = lane & 7;
+    double u3[3] = {0.0, 0.0, 0.0}, kpp = 0.0, fp = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+      const double dv = sm.S[k][gp][4], dv0 = sm.S[k][gp][5];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u3[i] += dv * sm.G[k][gp][a][i];
+      fp -= dv0 * sm.S[k][gp][7];
+      kpp -= (dv0 * sm.S[k][gp][6]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sm.Kup[k][3 * a + i] = u3[i];
+    if (a == 0) { sm.Kpp[k] = kpp; sm.Fp[k] = fp; }
+  }
+  __syncthreads();
+  const int a = lane >> 3, b = lane & 7;
+  for (int k = 0; k < 8; ++k) {
+    const int idx = base_idx + k;
+    if (idx >= count) break;  // uniform across the wave
+    double Kb[9];
+    tangent_block<8, false>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 48, &sm.S[k][0][0], 8, Kb);
+    const double kpp = sm.Kpp[k];
+    double* Ko = Kc + (size_t)idx * 576;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Ko[(3 * a + i) * 24 + 3 * b + j] = Kb[3 * i + j] - sm.Kup[k][3 * a + i] * (sm.Kup[k][3 * b + j] / kpp);
+    if (b == 0) {
+      const double fps = sm.Fp[k] / kpp;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Fc[(size_t)idx * 24 + 3 * a + i] = sm.R[k][a][i] - sm.Kup[k][3 * a + i] * fps;
+        Kup[(size_t)idx * 24 + 3 * a + i] = sm.Kup[k][3 * a + i];
+      }
+      if (a == 0) { Kpp[idx] = kpp; Fp[idx] = sm.Fp[k]; }
+    }
+  }
+}
+
+// ============================================================================================
 // Tet10 / P1 mixed element
 // ============================================================================================
 template <int EPW>
@@ -452,6 +550,15 @@ mpf_status launch_hex8_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStre
   if (sc.count <= 0) return MPF_OK;
   hipLaunchKernelGGL((hex8_kernel<kHexEPW, false>), dim3(grid_for(sc.count, kHexEPW)), dim3(64), 0, s, a, 0,
                      sc.count, nullptr, nullptr, sc);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status launch_hex8p0_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kc, double* Fc, double* Kup,
+                               double* Kpp, double* Fp, hipStream_t s) {
+  if (count <= 0) return MPF_OK;
+  hipLaunchKernelGGL(hex8p0_dense_kernel, dim3(grid_for(count, 8)), dim3(64), 0, s, a, first, count, Kc, Fc, Kup, Kpp,
+                     Fp);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

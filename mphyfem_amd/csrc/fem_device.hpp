@@ -274,10 +274,14 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   const double dv = wgp * inv3(B, Bi);
   double pres = 0.0;
   if (MIXED) {
-    double Np[4];
-    tet_Np(gp, Np);
+    if constexpr (NPE == 8) {
+      pres = Pn[0];  // Q1/P0: one pressure per element, Np = 1 (ElementBase2.cpp:389-393)
+    } else {
+      double Np[4];
+      tet_Np(gp, Np);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) pres += Np[q] * Pn[q];
+      for (int q = 0; q < 4; ++q) pres += Np[q] * Pn[q];
+    }
   }
   NHPoint P;
   neohooke_point(F, J, mu, K, Utype, MIXED, pres, dv, consistent, P);

@@ -104,6 +104,8 @@ struct ScatterArgs {
 // Kernel launchers (elem.hip).
 mpf_status launch_hex8_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,
                              hipStream_t s);
+mpf_status launch_hex8p0_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kc, double* Fc, double* Kup,
+                               double* Kpp, double* Fp, hipStream_t s);
 mpf_status launch_tet10_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kuu, double* Kup,
                               double* Kpu, double* Kpp, double* Fu, double* Fp, hipStream_t s);
 mpf_status launch_hex8_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);

@@ -408,6 +408,19 @@ mpf_status mpf_elem_hex8_nh_batch(const double* X, const double* u, const int32_
   return launch_hex8_dense(a, first, count, Ke, Re, (hipStream_t)stream);
 }
 
+mpf_status mpf_elem_hex8p0_nh_batch(const double* X, const double* u, const double* u_prev, const double* pres_elem,
+                                    const int32_t* conn, const mpf_matl* m, int32_t first, int32_t count, double* Kc,
+                                    double* Fc, double* Kup, double* Kpp, double* Fp, int32_t* bad,
+                                    mpf_stream_t stream) {
+  if (!X || !u || !u_prev || !pres_elem || !conn || !m || !Kc || !Fc || !Kup || !Kpp || !Fp || first < 0 ||
+      count < 0 || m->Kinv <= 0.0)
+    return MPF_ERR_ARG;
+  ElemArgs a{};
+  a.X = X; a.u = u; a.uprev = u_prev; a.pres = pres_elem; a.conn = conn;
+  a.mu = m->mu; a.Kinv = m->Kinv; a.Utype = m->Utype; a.bad = bad;
+  return launch_hex8p0_dense(a, first, count, Kc, Fc, Kup, Kpp, Fp, (hipStream_t)stream);
+}
+
 mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const double* u_prev, const double* pres,
                                      const int32_t* conn, const mpf_matl* m, int32_t first, int32_t count, double* Kuu,
                                      double* Kup, double* Kpu, double* Kpp, double* Fu, double* Fp, int32_t* bad,

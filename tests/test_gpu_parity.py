@@ -62,6 +62,29 @@ def test_hex8_batch_vs_oracle(mpf, orc, gpu, tangent):
     assert np.array_equal(Ke2.cpu().numpy(), Ke[13:18])
 
 
+def test_hex8p0_batch_vs_oracle(mpf, orc, gpu):
+    """Hex8 Q1/P0, condensed, every element of a jittered block against the oracle (which is pinned
+    by finite differences, closed forms and the condensation identity): Kc, Fc, Kup, Kpp, Fp within
+    1e-12."""
+    import torch
+    n = 3
+    X, conn = mpf.mesh_hex(n, n, n, jitter=0.1, seed=5)
+    u = random_state(len(X), 0.02, seed=7)
+    up = 0.5 * u
+    pe = np.random.default_rng(8).uniform(-0.5, 0.5, len(conn))
+    mu, Kinv = 50.0, 1e-2
+    out = mpf.elem_hex8p0_nh_batch(_t(gpu, X), _t(gpu, u), _t(gpu, up), _t(gpu, pe), _t(gpu, conn, torch.int32),
+                                   mpf.matl(mu, Kinv, 3))
+    Kc, Fc, Kup, Kpp, Fp, bad = [o.cpu().numpy() if hasattr(o, "cpu") else o for o in out]
+    assert bad == 0
+    ref = [orc.elem_hex8p0(X[c], u.reshape(-1, 3)[c], up.reshape(-1, 3)[c], pe[e], mu, Kinv, 3)
+           for e, c in enumerate(conn)]
+    assert all(o[8] == 0 for o in ref)
+    for got, k in ((Kc, 0), (Fc, 1), (Kup, 4), (Kpp, 6), (Fp, 7)):
+        want = np.array([o[k] for o in ref])
+        assert normwise_rel(got, want) < 1e-12, (k, normwise_rel(got, want))
+
+
 def test_tet10p1_golden_and_batch(mpf, orc, gpu):
     import torch
     g = np.load(GOLD / "tet10p1_element.npz")
