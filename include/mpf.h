@@ -47,7 +47,8 @@ typedef enum {
 } mpf_status;
 
 /* Element type ids (npElem). */
-enum { MPF_HEX8 = 8, MPF_TET10P1 = 10 };
+/* MPF_HEX8P0: Hex8 Q1/P0, one pressure per element condensed out of the system (Elem_Solid_3D_Mixed0). */
+enum { MPF_HEX8 = 8, MPF_HEX8P0 = 9, MPF_TET10P1 = 10 };
 
 /* Neo-Hookean parameters: Matl_NeoHookean (Matl_NeoHookean.cpp:45-53) reads mu = data_deviatoric[0],
  * Utype and Kinv (MaterialBase.cpp:91-115); K = 1/Kinv. */
@@ -114,7 +115,11 @@ mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const dou
 /* Problem: the femSolidmechanics-shaped driver state on one device (femSolidmechanics.h).       */
 typedef struct mpf_problem mpf_problem;
 
-/* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8 or MPF_TET10P1. */
+/* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8, MPF_HEX8P0 or   */
+/* MPF_TET10P1.  MPF_HEX8P0 assembles the condensed element matrices by the coloured scatter only  */
+/* (mpf_set_assembly_mode rejects the gather modes), recovers the element pressures in            */
+/* mpf_update (solveForPressure, Elem_Solid_3D_Mixed0.cpp:112-199), and has no batched path       */
+/* (mpf_set_batch rejects it).                                                                     */
 mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNode, int32_t nElem,
                               const double* X, const int32_t* conn, const mpf_matl* m);
 mpf_status mpf_problem_destroy(mpf_problem* p);
@@ -156,7 +161,8 @@ mpf_status mpf_get_sizes(const mpf_problem* p, int32_t* dispDOF, int32_t* presDO
 mpf_status mpf_get_pattern(const mpf_problem* p, int32_t* rowptr, int32_t* colidx,
                            int32_t* NodeDofArray, int32_t* ID_pres, int32_t* color);
 
-/* Solution state (host copies).  disp/disp_prev [nNode*3]; pres/pres_prev [nNode].             */
+/* Solution state (host copies).  disp/disp_prev [nNode*3]; pres/pres_prev [nNode], or [nElem]  */
+/* for MPF_HEX8P0.                                                                                */
 mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_prev,
                          const double* pres, const double* pres_prev);
 mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres);

@@ -21,6 +21,7 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libmpf.so"
 
 HEX8 = 8
+HEX8P0 = 9  # Hex8 Q1/P0: one element pressure, condensed
 TET10P1 = 10
 KSP_CG, KSP_GMRES, KSP_BAND = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
@@ -493,8 +494,9 @@ class Problem:
         _check(lib().mpf_set_state(self._h, *[_dp(x) for x in a]), "mpf_set_state")
 
     def state(self):
+        """(disp [nNode*3], pres [nNode], or [nElem] for HEX8P0)."""
         disp = np.zeros(self.nNode * 3)
-        pres = np.zeros(self.nNode)
+        pres = np.zeros(self.nElem if self.elem_type == HEX8P0 else self.nNode)
         _check(lib().mpf_get_state(self._h, _dp(disp), _dp(pres)), "mpf_get_state")
         return disp, pres
 

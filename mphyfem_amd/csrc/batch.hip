@@ -912,7 +912,7 @@ extern "C" {
 
 mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                          const double* mu, const double* Kinv) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 and Tet10/P1
   return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
 }
 

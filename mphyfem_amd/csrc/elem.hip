@@ -273,6 +273,77 @@ __global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t fi
   }
 }
 
+// Coloured scatter of the condensed Q1/P0 element matrices (Kc, Fc by element id, written by
+// hex8p0_dense_kernel over all elements): one wavefront per element of the colour, lane = (a, b),
+// with the first-iteration Dirichlet lift Fc_a -= sum_b Kc_ab dubar_b (ElementBase.cpp:567-594).
+__global__ void __launch_bounds__(64) hex8p0_scatter_kernel(const int32_t* __restrict__ conn, ScatterArgs sc,
+                                                            const double* __restrict__ Kc,
+                                                            const double* __restrict__ Fc) {
+  const int lane = threadIdx.x;
+  const int idx = blockIdx.x;  // grid = sc.count
+  const int e = sc.elems[idx];
+  const int a = lane >> 3, b = lane & 7;
+  const int na = conn[(size_t)e * 8 + a], nb = conn[(size_t)e * 8 + b];
+  const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
+  const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
+  const double* Ke = Kc + (size_t)e * 576;
+  double Kb[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) Kb[3 * i + kk] = Ke[(3 * a + i) * 24 + 3 * b + kk];
+  double c[3] = {0.0, 0.0, 0.0};
+  if (sc.dispApplied) {
+    double db[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double v = Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
+      v += shfl_xor_d(v, 1);
+      v += shfl_xor_d(v, 2);
+      v += shfl_xor_d(v, 4);
+      c[i] = v;
+    }
+  }
+  int addr[9];
+  const int s0 = (sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
+  block_slots(s0, ma, mb, La, addr);
+  rmw_add<9>(sc.vals, addr, Kb);
+  if (b == 0) {
+    int ra[3];
+    double rv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ra[i] = ((ma >> i) & 1) ? sc.nda[3 * (size_t)na + i] : -1;
+      rv[i] = Fc[(size_t)e * 24 + 3 * a + i] - c[i];
+    }
+    rmw_add<3>(sc.rhs, ra, rv);
+  }
+}
+
+// solveForPressure (Elem_Solid_3D_Mixed0.cpp:112-199): p_e += (Fp_e - Kpu_e . du_e) / Kpp_e, Kpu = Kup^T, where
+// du_e is the element's displacement increment: the solved correction on free dofs and, in the first
+// iteration, the applied Dirichlet increment on prescribed ones (femSolidmechanics2.cpp:1137-1186).
+__global__ void k_hex8p0_pressure(int32_t nElem, const int32_t* __restrict__ conn, const int32_t* __restrict__ nda,
+                                  const double* __restrict__ du, const double* __restrict__ dispApplied,
+                                  const double* __restrict__ Kup, const double* __restrict__ Kpp,
+                                  const double* __restrict__ Fp, double* __restrict__ pres) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nElem) return;
+  double s = 0.0;
+  for (int a = 0; a < 8; ++a) {
+    const size_t n = (size_t)conn[(size_t)e * 8 + a];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int d = nda[3 * n + i];
+      const double inc = d >= 0 ? du[d] : (dispApplied ? dispApplied[3 * n + i] : 0.0);
+      s += Kup[(size_t)e * 24 + 3 * a + i] * inc;
+    }
+  }
+  pres[e] += (Fp[e] - s) / Kpp[e];
+}
+
 // ============================================================================================
 // Tet10 / P1 mixed element
 // ============================================================================================
@@ -559,6 +630,24 @@ mpf_status launch_hex8p0_dense(const ElemArgs& a, int32_t first, int32_t count, 
   if (count <= 0) return MPF_OK;
   hipLaunchKernelGGL(hex8p0_dense_kernel, dim3(grid_for(count, 8)), dim3(64), 0, s, a, first, count, Kc, Fc, Kup, Kpp,
                      Fp);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status launch_hex8p0_scatter(const int32_t* conn, const ScatterArgs& sc, const double* Kc, const double* Fc,
+                                 hipStream_t s) {
+  if (sc.count <= 0) return MPF_OK;
+  hipLaunchKernelGGL(hex8p0_scatter_kernel, dim3(sc.count), dim3(64), 0, s, conn, sc, Kc, Fc);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status launch_hex8p0_pressure(int32_t nElem, const int32_t* conn, const int32_t* nda, const double* du,
+                                  const double* dispApplied, const double* Kup, const double* Kpp, const double* Fp,
+                                  double* pres, hipStream_t s) {
+  if (nElem <= 0) return MPF_OK;
+  hipLaunchKernelGGL(k_hex8p0_pressure, dim3(grid_for(nElem, 256)), dim3(256), 0, s, nElem, conn, nda, du,
+                     dispApplied, Kup, Kpp, Fp, pres);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

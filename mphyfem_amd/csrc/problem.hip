@@ -226,10 +226,13 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
   sc.rhs = p->rhs.ptr;
   sc.dispApplied = first_iter ? p->dispApplied.ptr : nullptr;
   sc.scatterK = scatterK;
+  if (p->p0)  // condensed element matrices of every element, then scattered colour by colour
+    MPF_TRY(launch_hex8p0_dense(A, 0, p->nElem, p->q_Kc.ptr, p->q_Fc.ptr, p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, s));
   for (int c = 0; c < P.ncolors; ++c) {
     sc.elems = P.perm.ptr + P.color_off[c];
     sc.count = P.color_off[c + 1] - P.color_off[c];
-    if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
+    if (p->p0) MPF_TRY(launch_hex8p0_scatter(p->conn.ptr, sc, p->q_Kc.ptr, p->q_Fc.ptr, s));
+    else if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
     else MPF_TRY(launch_tet10_scatter(A, sc, s));
   }
   MPF_HIP(hipGetLastError());
@@ -278,6 +281,11 @@ mpf_status do_update(mpf_problem* p, bool first_iter) {
   if (P.presDOF > 0)
     hipLaunchKernelGGL(k_update_pres, dim3(grid_for(P.presDOF, 256)), dim3(256), 0, s, P.presDOF, P.assyP.ptr,
                        p->du.ptr, P.dispDOF, p->pres.ptr);
+  if (p->p0) {  // from the element matrices of the assembly this correction solved
+    if (!p->assembled) return MPF_ERR_STATE;
+    MPF_TRY(launch_hex8p0_pressure(p->nElem, p->conn.ptr, P.nda.ptr, p->du.ptr, first_iter ? p->dispApplied.ptr : nullptr,
+                                   p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, p->pres.ptr, s));
+  }
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
@@ -309,7 +317,7 @@ mpf_status newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, con
   const size_t n3 = (size_t)p->nNode * 3;
   // saveSolution: dispPrev = disp, presPrev = pres (SolutionDataSolid.cpp:170-184)
   MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  MPF_HIP(hipMemcpyAsync(p->presPrev.ptr, p->pres.ptr, p->nNode * sizeof(double), hipMemcpyDeviceToDevice, s));
+  MPF_HIP(hipMemcpyAsync(p->presPrev.ptr, p->pres.ptr, p->npres * sizeof(double), hipMemcpyDeviceToDevice, s));
   MPF_TRY(do_set_bc(p, lam));
   double R0 = -1.0;
   int iter = 1;
@@ -437,21 +445,25 @@ mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const dou
 mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNode, int32_t nElem, const double* X,
                               const int32_t* conn, const mpf_matl* m) {
   if (!out || !X || !conn || !m || nNode <= 0 || nElem <= 0) return MPF_ERR_ARG;
-  if (elem_type != MPF_HEX8 && elem_type != MPF_TET10P1) return MPF_ERR_ARG;
+  if (elem_type != MPF_HEX8 && elem_type != MPF_HEX8P0 && elem_type != MPF_TET10P1) return MPF_ERR_ARG;
   if (m->Kinv <= 0.0) return MPF_ERR_ARG;
-  const int npe = elem_type == MPF_HEX8 ? 8 : 10;
+  const bool p0 = elem_type == MPF_HEX8P0;
+  const int npe = elem_type == MPF_TET10P1 ? 10 : 8;
   for (int64_t i = 0; i < (int64_t)nElem * npe; ++i)
     if (conn[i] < 0 || conn[i] >= nNode) return MPF_ERR_ARG;
   MPF_TRY(check_device());
   static std::atomic<uint64_t> next_uid{1};
   auto* p = new mpf_problem();
   p->uid = next_uid++;
-  p->etype = elem_type;
+  p->etype = p0 ? MPF_HEX8 : elem_type;  // Q1/P0 shares the Hex8 numbering and pattern (no pressure dofs)
+  p->p0 = p0;
   p->npe = npe;
   p->nNode = nNode;
   p->nElem = nElem;
+  p->npres = p0 ? nElem : nNode;
   p->m = *m;
-  p->m.mixed = (elem_type == MPF_TET10P1) ? 1 : 0;
+  p->m.mixed = (elem_type == MPF_HEX8) ? 0 : 1;
+  if (p0) p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;  // condensed element matrices go through the slot map
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     delete p;
     return MPF_ERR_HIP;
@@ -463,8 +475,8 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   if (st == MPF_OK) st = p->disp.alloc(n3);
   if (st == MPF_OK) st = p->dispPrev.alloc(n3);
   if (st == MPF_OK) st = p->dispApplied.alloc(n3);
-  if (st == MPF_OK) st = p->pres.alloc(nNode);
-  if (st == MPF_OK) st = p->presPrev.alloc(nNode);
+  if (st == MPF_OK) st = p->pres.alloc((size_t)p->npres);
+  if (st == MPF_OK) st = p->presPrev.alloc((size_t)p->npres);
   if (st == MPF_OK) st = p->bad.alloc(2);
   if (st != MPF_OK) {
     mpf_problem_destroy(p);
@@ -473,8 +485,8 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n3)), dim3(256), 0, p->stream, p->disp.ptr, (int64_t)n3, 0.0);
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n3)), dim3(256), 0, p->stream, p->dispPrev.ptr, (int64_t)n3, 0.0);
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n3)), dim3(256), 0, p->stream, p->dispApplied.ptr, (int64_t)n3, 0.0);
-  hipLaunchKernelGGL(k_fill_d, dim3(g256(nNode)), dim3(256), 0, p->stream, p->pres.ptr, (int64_t)nNode, 0.0);
-  hipLaunchKernelGGL(k_fill_d, dim3(g256(nNode)), dim3(256), 0, p->stream, p->presPrev.ptr, (int64_t)nNode, 0.0);
+  hipLaunchKernelGGL(k_fill_d, dim3(g256(p->npres)), dim3(256), 0, p->stream, p->pres.ptr, p->npres, 0.0);
+  hipLaunchKernelGGL(k_fill_d, dim3(g256(p->npres)), dim3(256), 0, p->stream, p->presPrev.ptr, p->npres, 0.0);
   hostX_store()[p].assign(X, X + n3);
   for (int c = 0; c < 3; ++c) {
     p->xmin[c] = nNode > 0 ? X[c] : 0.0;
@@ -548,6 +560,14 @@ mpf_status mpf_prepare_pattern(mpf_problem* p) {
   MPF_TRY(p->rhs.alloc(n > 0 ? n : 1));
   MPF_TRY(p->du.alloc(n > 0 ? n : 1));
   if (p->asm_mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
+  if (p->p0) {
+    const size_t ne = (size_t)p->nElem;
+    MPF_TRY(p->q_Kc.alloc(ne * 576));
+    MPF_TRY(p->q_Fc.alloc(ne * 24));
+    MPF_TRY(p->q_Kup.alloc(ne * 24));
+    MPF_TRY(p->q_Kpp.alloc(ne));
+    MPF_TRY(p->q_Fp.alloc(ne));
+  }
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->du.ptr, n, 0.0);
   if (!p->trac.empty()) {
     std::vector<double> F;
@@ -595,8 +615,8 @@ mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_
   const size_t n3 = (size_t)p->nNode * 3;
   if (disp) MPF_HIP(hipMemcpyAsync(p->disp.ptr, disp, n3 * sizeof(double), hipMemcpyHostToDevice, s));
   if (disp_prev) MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, disp_prev, n3 * sizeof(double), hipMemcpyHostToDevice, s));
-  if (pres) MPF_HIP(hipMemcpyAsync(p->pres.ptr, pres, p->nNode * sizeof(double), hipMemcpyHostToDevice, s));
-  if (pres_prev) MPF_HIP(hipMemcpyAsync(p->presPrev.ptr, pres_prev, p->nNode * sizeof(double), hipMemcpyHostToDevice, s));
+  if (pres) MPF_HIP(hipMemcpyAsync(p->pres.ptr, pres, p->npres * sizeof(double), hipMemcpyHostToDevice, s));
+  if (pres_prev) MPF_HIP(hipMemcpyAsync(p->presPrev.ptr, pres_prev, p->npres * sizeof(double), hipMemcpyHostToDevice, s));
   MPF_HIP(hipStreamSynchronize(s));
   return MPF_OK;
 }
@@ -620,7 +640,7 @@ mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
   if (!p) return MPF_ERR_ARG;
   hipStream_t s = p->stream;
   if (disp) MPF_TRY(p->disp.download(disp, (size_t)p->nNode * 3, s));
-  if (pres) MPF_TRY(p->pres.download(pres, p->nNode, s));
+  if (pres) MPF_TRY(p->pres.download(pres, (size_t)p->npres, s));
   MPF_HIP(hipStreamSynchronize(s));
   return MPF_OK;
 }
@@ -629,6 +649,7 @@ mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
   if (!p || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER &&
              mode != MPF_ASSEMBLY_GATHER_DIRECT))
     return MPF_ERR_ARG;
+  if (p->p0 && mode != MPF_ASSEMBLY_COLORED_SCATTER) return MPF_ERR_ARG;  // Q1/P0: coloured scatter only
   p->asm_mode = mode;
   if (p->prepared && mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
   return MPF_OK;
@@ -757,7 +778,7 @@ mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* i
     } else if (st == MPF_ERR_NEG_JACOBIAN || st == MPF_ERR_NOT_CONVERGED || st == MPF_ERR_DIVERGED) {
       // reset (SolutionDataSolid.cpp:157-167) and cut
       MPF_HIP(hipMemcpyAsync(p->disp.ptr, p->dispPrev.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-      MPF_HIP(hipMemcpyAsync(p->pres.ptr, p->presPrev.ptr, p->nNode * sizeof(double), hipMemcpyDeviceToDevice, s));
+      MPF_HIP(hipMemcpyAsync(p->pres.ptr, p->presPrev.ptr, p->npres * sizeof(double), hipMemcpyDeviceToDevice, s));
       cur -= dt;
       dt *= 0.5;
       if (dt < stack[0] * 0.01) {

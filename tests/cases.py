@@ -86,3 +86,65 @@ def tet10_fan(subdiv: int = 1, radius: float = 1.0):
         e = [(0, 1), (1, 2), (0, 2), (0, 3), (2, 3), (1, 3)]
         conn.append(v + [enode(v[p], v[q]) for p, q in e])
     return np.array(X), np.array(conn, dtype=np.int32), np.array(outer, dtype=np.int32)
+
+
+def q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps, dt, rtol=1e-10, itmax=20, Utype=3):
+    """CPU Newton driver for Hex8 Q1/P0 on a small mesh (dense global system), built on the oracle's
+    condensed element (orc.elem_hex8p0) and following the reference's order of operations:
+    saveSolution, setBoundaryConditions (dispApplied = value * lambda - disp on prescribed dofs,
+    femSolidmechanics1.cpp:864-890), calcStiffnessAndResidual with the first-iteration lift
+    (ElementBase.cpp:567-594), ||R|| / ||R0|| < rtol checked after each assembly, then the update
+    (femSolidmechanics2.cpp:1137-1186) and solveForPressure per element
+    (Elem_Solid_3D_Mixed0.cpp:112-199).  Returns (disp [nNode*3], element pressures [nElem],
+    Newton iterations per step)."""
+    X = np.asarray(X, dtype=np.float64)
+    conn = np.asarray(conn)
+    nn, ne = len(X), len(conn)
+    spec = {}
+    for nodes, dof, val in bcs:
+        vals = np.broadcast_to(0.0 if val is None else val, np.shape(nodes))
+        for nd, v in zip(np.asarray(nodes).tolist(), vals.tolist()):
+            for c in (range(3) if dof < 0 else (dof,)):
+                spec[3 * nd + c] = 0.0 if dof < 0 else v
+    presc = np.array(sorted(spec), dtype=np.int64)
+    free = np.setdiff1d(np.arange(3 * nn), presc)
+    pval = np.array([spec[d] for d in presc.tolist()])
+    u, p = np.zeros(3 * nn), np.zeros(ne)
+    iters = []
+    for step in range(1, nsteps + 1):
+        lam = step * dt
+        uprev = u.copy()
+        applied = pval * lam - u[presc]
+        R0 = -1.0
+        for it in range(1, itmax + 1):
+            first = it == 1
+            K, F = np.zeros((3 * nn, 3 * nn)), np.zeros(3 * nn)
+            Kup, Kpp, Fp = np.zeros((ne, 24)), np.zeros(ne), np.zeros(ne)
+            for e, c in enumerate(conn):
+                Kc, Fc, _, _, kup, kpu, kpp, fp, st = orc.elem_hex8p0(X[c], u.reshape(-1, 3)[c],
+                                                                     uprev.reshape(-1, 3)[c], p[e], mu, Kinv, Utype)
+                assert st == 0
+                d = (3 * np.asarray(c)[:, None] + np.arange(3)).ravel()
+                K[np.ix_(d, d)] += Kc
+                F[d] += Fc
+                Kup[e], Kpp[e], Fp[e] = kpu, kpp, fp
+            R = F[free]
+            if first:
+                R = R - K[np.ix_(free, presc)] @ applied
+            rn = np.linalg.norm(R)
+            if R0 < 0:
+                R0 = rn
+            if R0 == 0.0 or rn / R0 < rtol:
+                break
+            inc = np.zeros(3 * nn)
+            inc[free] = np.linalg.solve(K[np.ix_(free, free)], R)
+            if first:
+                inc[presc] = applied
+            u += inc
+            for e, c in enumerate(conn):
+                d = (3 * np.asarray(c)[:, None] + np.arange(3)).ravel()
+                p[e] += (Fp[e] - Kup[e] @ inc[d]) / Kpp[e]
+        else:
+            raise RuntimeError(f"Q1/P0 Newton did not converge at step {step}")
+        iters.append(it)
+    return u, p, iters

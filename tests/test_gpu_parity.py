@@ -11,7 +11,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, random_state, tet10_fan, tet_compress_bcs
+from tests.cases import (apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, q1p0_newton, random_state, tet10_fan,
+                         tet_compress_bcs)
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -83,6 +84,37 @@ def test_hex8p0_batch_vs_oracle(mpf, orc, gpu):
     for got, k in ((Kc, 0), (Fc, 1), (Kup, 4), (Kpp, 6), (Fp, 7)):
         want = np.array([o[k] for o in ref])
         assert normwise_rel(got, want) < 1e-12, (k, normwise_rel(got, want))
+
+
+@pytest.mark.parametrize("ksp", ["band", "cg"])
+def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp):
+    """Hex8 Q1/P0 problem on the GPU (condensed element matrices scattered colour by colour, element
+    pressures recovered after each correction) against the CPU Q1/P0 driver on the oracle's element:
+    a near-incompressible jittered block stretched 10% in two load steps, converged displacements
+    and element pressures within 1e-9."""
+    n, mu, Kinv = 3, 1.0, 1e-3
+    X, conn = cube_hex(n, jitter=0.1, seed=3)
+    bcs = hex_stretch_bcs(n)
+    m = mpf.matl(mu, Kinv, 3)
+    P = mpf.Problem(mpf.HEX8P0, X, conn, m)
+    apply_bcs(P, bcs)
+    P.prepare_matrix_pattern()
+    assert P.assembly_mode() == mpf.ASSEMBLY_COLORED_SCATTER
+    with pytest.raises(mpf.MpfError):
+        P.set_assembly_mode(mpf.ASSEMBLY_GATHER)
+    if ksp == "band":
+        opts = P.newton_opts(nsteps=2, dt=0.5, itmax=20, rtol=1e-10, ksp=mpf.KSP_BAND)
+    else:
+        opts = P.newton_opts(nsteps=2, dt=0.5, itmax=20, rtol=1e-10, ksp=mpf.KSP_CG, pc=mpf.PC_BJACOBI3,
+                             lin_rtol=1e-13, maxit=50000)
+    iters, rep = P.solve_with_newton_raphson(opts)
+    assert rep.steps_done == 2, iters
+    ud, pd = P.state()
+    assert pd.shape == (len(conn),)
+    uo, po, oit = q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps=2, dt=0.5, rtol=1e-10)
+    assert normwise_rel(ud, uo) < 1e-9, normwise_rel(ud, uo)
+    assert normwise_rel(pd, po) < 1e-9, normwise_rel(pd, po)
+    assert np.abs(po).max() > 1e-3
 
 
 def test_tet10p1_golden_and_batch(mpf, orc, gpu):

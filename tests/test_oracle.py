@@ -27,7 +27,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel
+from tests.cases import apply_bcs, cube_hex, hex_stretch_bcs, normwise_rel, q1p0_newton
 
 GOLD = Path(__file__).resolve().parent / "golden"
 
@@ -406,3 +406,28 @@ def test_hex8p0_condensation_identity(orc):
     dfull = np.zeros(24)
     dfull[keep] = du
     assert abs((Fp - Kpu @ dfull) / Kpp - sol[-1]) <= 1e-10 * max(1.0, abs(sol[-1]))
+
+
+def test_q1p0_newton_driver_reaches_mixed_equilibrium(orc):
+    """The CPU Q1/P0 driver (condensed solve, element pressure recovery after each correction) ends
+    where the uncondensed mixed equations hold: Fu = 0 on the free dofs and Fp = 0 in every element
+    (a wrong recovery leaves Fp nonzero while the condensed residual still vanishes), the prescribed
+    face sits at its value, and Newton converges quadratically."""
+    n, mu, Kinv = 2, 1.0, 1e-3
+    X, conn = cube_hex(n, jitter=0.1, seed=3)
+    bcs = hex_stretch_bcs(n)
+    u, p, iters = q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps=2, dt=0.5, rtol=1e-11)
+    assert max(iters) <= 6, iters
+    assert np.allclose(u[3 * bcs[1][0]], 0.1, atol=1e-14)
+    fixed = set((3 * np.asarray(bcs[0][0])[:, None] + np.arange(3)).ravel().tolist()) | set((3 * bcs[1][0]).tolist())
+    Fu_all = np.zeros_like(u)
+    fp_max = 0.0
+    for e, c in enumerate(conn):
+        _, _, _, Fu, _, _, Kpp, Fp, st = orc.elem_hex8p0(X[c], u.reshape(-1, 3)[c], u.reshape(-1, 3)[c], p[e], mu, Kinv)
+        assert st == 0
+        Fu_all[(3 * np.asarray(c)[:, None] + np.arange(3)).ravel()] += Fu
+        fp_max = max(fp_max, abs(Fp / Kpp))
+    free = np.array([d for d in range(len(u)) if d not in fixed])
+    assert np.linalg.norm(Fu_all[free]) < 1e-9 * mu
+    assert fp_max < 1e-9 * max(1.0, np.abs(p).max())
+    assert np.abs(p).max() > 1e-3  # a stretched near-incompressible block carries pressure
