@@ -31,11 +31,13 @@ sys.path.insert(0, str(ROOT))
 
 # Algorithmic bytes per element (element-scatter model): npe*(4 conn + 24 X + 24 u)
 # + nK*(8 read + 8 write CSR value + 4 CSR-slot index) + nR*16 (+ nP*12 pressure gather/slot).
-BYTES_PER_ELEM = {"tet10": 24232, "hex8": 12320}
+# Hex8 Q1/P0 adds the element pressure and u_prev of its nodes (8 + 192) and writes Kup, Kpp, Fp (208).
+BYTES_PER_ELEM = {"tet10": 24232, "hex8": 12320, "hex8p0": 12728}
 HBM_PEAK_GBS = 8000.0
 # algorithmic FP64 work of one element's tangent + residual (per Gauss point: K_e 64*72 + Gc 8*162 +
 # stress 8*18 + material ~900 + geometry ~700 for Hex8, x 8 points; Tet10/P1 with its u-p blocks, x 4)
-FLOPS_PER_ELEM = {"tet10": 4.6e4, "hex8": 6.1e4}
+# Q1/P0: the Hex8 work plus Kup/Kpp/Fp (8 points x 30) and the condensation (576 x 3)
+FLOPS_PER_ELEM = {"tet10": 4.6e4, "hex8": 6.1e4, "hex8p0": 6.3e4}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -47,6 +49,8 @@ def build_problem(M, n: int, workload: str):
     X, conn, _, bcs = build_problem_host(M, n, workload)
     if workload == "hex8":
         P = M.Problem(M.HEX8, X, conn, M.matl_from_E_nu(1e3, 0.49, tangent=M.TANGENT_CONSISTENT))
+    elif workload == "hex8p0":
+        P = M.Problem(M.HEX8P0, X, conn, M.matl_from_E_nu(1e3, 0.49))
     else:
         P = M.Problem(M.TET10P1, X, conn, M.matl(1.0, 1e-3, 3, mixed=True))
     for nodes, dof, val in bcs:
@@ -92,8 +96,37 @@ def cpu_baseline(M, workload: str, budget_s: float):
     return out
 
 
+def cpu_baseline_hex8p0(M, budget_s: float):
+    """Oracle Q1/P0 element routine (serial C++, -O3, one core) over a bounded sample of the same mesh
+    and material: element computation and condensation only (no scatter), one ctypes call each."""
+    import ctypes as C
+    from oracle import oracle as O
+    n = 32
+    X, conn, _, _ = build_problem_host(M, n, "hex8p0")
+    mu, Kinv, Ut = material_host(M, "hex8p0")
+    Xe = np.ascontiguousarray(X[conn])      # [nElem][8][3]
+    Ue = np.ascontiguousarray(0.01 * Xe)   # a small deformation
+    D = C.POINTER(C.c_double)
+    px = [C.cast(C.c_void_p(Xe.ctypes.data + 192 * e), D) for e in range(len(conn))]
+    pu = [C.cast(C.c_void_p(Ue.ctypes.data + 192 * e), D) for e in range(len(conn))]
+    outs = [np.zeros(k) for k in (576, 24, 576, 24, 24, 24, 1, 1)]
+    po = [o.ctypes.data_as(D) for o in outs]
+    f = O.lib().orc_elem_hex8p0
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        for e in range(len(conn)):
+            f(px[e], pu[e], pu[e], 0.0, mu, Kinv, Ut, *po)
+        done += len(conn)
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": f"oracle Q1/P0 element routine (serial C++, g++ -O3, one ctypes call per element, no "
+                      f"scatter) over {done} elements: repeated sweeps of the {n}^3 = {len(conn)}-element block "
+                      f"of the same generator and material, {dt:.1f} s"}
+
+
 def build_problem_host(M, n, workload):
-    if workload == "hex8":
+    if workload in ("hex8", "hex8p0"):
         X, conn = M.mesh_hex(n, n, n, jitter=0.0, seed=5)
         bcs = [(M.face_nodes(M.HEX8, n, n, n, 4), -1, None), (M.face_nodes(M.HEX8, n, n, n, 5), 2, -0.3)]
     else:
@@ -103,7 +136,7 @@ def build_problem_host(M, n, workload):
 
 
 def material_host(M, workload):
-    if workload == "hex8":
+    if workload in ("hex8", "hex8p0"):
         m = M.matl_from_E_nu(1e3, 0.49)
         return m.mu, m.Kinv, 3
     return 1.0, 1e-3, 3
@@ -296,7 +329,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tet10", "hex8", "batch", "learned", "direct", "full"], default="tet10",
+    ap.add_argument("--workload", choices=["tet10", "hex8", "hex8p0", "batch", "learned", "direct", "full"],
+                    default="tet10",
                     help="tet10/hex8: assembly; batch: 256 independent Hex8 blocks, batched Newton + Jacobi-PCG; "
                          "learned: cantilever beams, plain vs learned Newton initial guess; "
                          "direct: the 200 held-out cantilevers, plain full Newton with the banded direct solve; "
@@ -351,6 +385,8 @@ def main(argv=None) -> int:
     if wl == "full":
         return bench_full(M, a)
     n = a.n or (60 if wl == "tet10" else 128)
+    if wl == "hex8p0":
+        a.assembly = "scatter"  # Q1/P0 assembles through the coloured scatter only
     log(f"workload {wl}, n = {n}")
     X, conn, P, _ = build_problem(M, n, wl)
     nElem = len(conn)
@@ -380,7 +416,8 @@ def main(argv=None) -> int:
     if a.assembly != "scatter":
         kname = "k_stage + k_flat + k_rhs + k_lift (row-owner gather, 4 launches"
     else:
-        kname = ("tet10_kernel<4,false>" if wl == "tet10" else "hex8_kernel<8,false>") + f" ({ncolors} colour launches"
+        kname = {"tet10": "tet10_kernel<4,false>", "hex8": "hex8_kernel<8,false>",
+                 "hex8p0": "hex8p0_kernel<true>"}[wl] + f" ({ncolors} colour launches"
     # secondaries: the unique bytes an assembly must move (CSR values written once, mesh and state
     # read once, residual written once) and the FP64 rate of the element work
     npe = conn.shape[1]
@@ -401,6 +438,10 @@ def main(argv=None) -> int:
         metric = "Melem/s assembly (Tet10/P1 tangent+residual into FP64 CSR)"
         desc = f"Tet10/P1 mixed Neo-Hookean, {n}^3x6 Kuhn tets ({nElem} elements), one tangent+residual " \
                f"assembly at 5% compression"
+    elif wl == "hex8p0":
+        metric = "Melem/s assembly (Hex8 Q1/P0 condensed tangent+residual into FP64 CSR)"
+        desc = f"Hex8 Q1/P0 mixed Neo-Hookean nu=0.49, {n}^3 block ({nElem} elements), one condensed " \
+               f"tangent+residual assembly"
     else:
         metric = "Melem/s assembly (Hex8 tangent+residual into FP64 CSR)"
         desc = f"Hex8 Neo-Hookean nu=0.49, {n}^3 block ({nElem} elements), one tangent+residual assembly"
@@ -430,7 +471,8 @@ def main(argv=None) -> int:
         log(f"{name} {r.iterations} its relres {r.relres:.2e} {r.seconds:.2f} s")
     if not a.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(M, wl, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline_hex8p0(M, a.cpu_seconds) if wl == "hex8p0" else \
+            cpu_baseline(M, wl, a.cpu_seconds)
     P.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -193,13 +193,19 @@ struct Hex8P0Smem {
 
 // One wavefront per 8 elements: phase 1 lane = (element, Gauss point); phase 2 lane = (a, b):
 // Kc(a,b) = Kuu(a,b) - Kup_a (Kpu_b / Kpp) with Kpu = Kup^T, Fc_a = Fu_a - Kup_a (Fp / Kpp).
-__global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t first, int32_t count,
-                                                          double* __restrict__ Kc, double* __restrict__ Fc,
-                                                          double* __restrict__ Kup, double* __restrict__ Kpp,
-                                                          double* __restrict__ Fp) {
+// SCATTER = false: dense Kc, Fc, Kup, Kpp, Fp of elements first .. first + count - 1 (output index idx).
+// SCATTER = true: elements sc.elems[0 .. count) of one colour; each lane adds its condensed 3x3 block
+// into CSR through the slot map, with the first-iteration lift Fc_a -= sum_b Kc_ab dubar_b
+// (ElementBase.cpp:567-594), and Kup, Kpp, Fp are kept by element id for the pressure recovery.
+template <bool SCATTER>
+__global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, int32_t count,
+                                                    double* __restrict__ Kc, double* __restrict__ Fc,
+                                                    double* __restrict__ Kup, double* __restrict__ Kpp,
+                                                    double* __restrict__ Fp, ScatterArgs sc) {
   __shared__ Hex8P0Smem sm;
   const int lane = threadIdx.x;
   const int base_idx = blockIdx.x * 8;
+  auto elem_of = [&](int idx) { return SCATTER ? sc.elems[idx] : first + idx; };
   if (lane < 8) {
     double d[3][8];
     hex8_dN(lane, d);
@@ -211,7 +217,7 @@ __global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t fi
   {  // lane = (k, a): node data; idle slots of a partial block repeat the block's first element
     const int k = lane >> 3, a = lane & 7;
     const int idx = (base_idx + k < count) ? base_idx + k : base_idx;
-    const int e = first + idx;
+    const int e = elem_of(idx);
     const size_t nd = (size_t)A.conn[(size_t)e * 8 + a];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -227,10 +233,10 @@ __global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t fi
     const bool act = base_idx + k < count;
     bool negJ = false;
     double mu, K;
-    elem_material(A, first + (act ? base_idx + k : base_idx), mu, K);
+    elem_material(A, elem_of(act ? base_idx + k : base_idx), mu, K);
     element_point<8, 8, true>(sm.X[k], sm.U[k], sm.UP[k], &sm.Pe[k], sm.dN[gp], gp, 1.0, mu, K, A.Utype, false, true,
                               &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
-    if (act && negJ) flag_bad(A.bad, first + base_idx + k);
+    if (act && negJ) flag_bad(A.bad, elem_of(base_idx + k));
   }
   __syncthreads();
   {  // Kup (ElementBase2.cpp:500-505 with Np = 1), Kpp and Fp (:510-539): lane = (k, a)
@@ -255,70 +261,63 @@ __global__ void __launch_bounds__(64) hex8p0_dense_kernel(ElemArgs A, int32_t fi
     double Kb[9];
     tangent_block<8, false>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 48, &sm.S[k][0][0], 8, Kb);
     const double kpp = sm.Kpp[k];
-    double* Ko = Kc + (size_t)idx * 576;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        Ko[(3 * a + i) * 24 + 3 * b + j] = Kb[3 * i + j] - sm.Kup[k][3 * a + i] * (sm.Kup[k][3 * b + j] / kpp);
-    if (b == 0) {
-      const double fps = sm.Fp[k] / kpp;
+      for (int j = 0; j < 3; ++j) Kb[3 * i + j] = Kb[3 * i + j] - sm.Kup[k][3 * a + i] * (sm.Kup[k][3 * b + j] / kpp);
+    const double fps = sm.Fp[k] / kpp;
+    double fc[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        Fc[(size_t)idx * 24 + 3 * a + i] = sm.R[k][a][i] - sm.Kup[k][3 * a + i] * fps;
-        Kup[(size_t)idx * 24 + 3 * a + i] = sm.Kup[k][3 * a + i];
+    for (int i = 0; i < 3; ++i) fc[i] = sm.R[k][a][i] - sm.Kup[k][3 * a + i] * fps;
+    if (!SCATTER) {
+      double* Ko = Kc + (size_t)idx * 576;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Ko[(3 * a + i) * 24 + 3 * b + j] = Kb[3 * i + j];
+      if (b == 0)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Fc[(size_t)idx * 24 + 3 * a + i] = fc[i];
+    } else {
+      const int e = sc.elems[idx];
+      const int na = A.conn[(size_t)e * 8 + a], nb = A.conn[(size_t)e * 8 + b];
+      const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
+      const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
+      double c[3] = {0.0, 0.0, 0.0};
+      if (sc.dispApplied) {
+        double db[3];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double v = Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
+          v += shfl_xor_d(v, 1);
+          v += shfl_xor_d(v, 2);
+          v += shfl_xor_d(v, 4);
+          c[i] = v;
+        }
       }
-      if (a == 0) { Kpp[idx] = kpp; Fp[idx] = sm.Fp[k]; }
+      int addr[9];
+      const int s0 = (sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
+      block_slots(s0, ma, mb, La, addr);
+      rmw_add<9>(sc.vals, addr, Kb);
+      if (b == 0) {
+        int ra[3];
+        double rv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          ra[i] = ((ma >> i) & 1) ? sc.nda[3 * (size_t)na + i] : -1;
+          rv[i] = fc[i] - c[i];
+        }
+        rmw_add<3>(sc.rhs, ra, rv);
+      }
     }
-  }
-}
-
-// Coloured scatter of the condensed Q1/P0 element matrices (Kc, Fc by element id, written by
-// hex8p0_dense_kernel over all elements): one wavefront per element of the colour, lane = (a, b),
-// with the first-iteration Dirichlet lift Fc_a -= sum_b Kc_ab dubar_b (ElementBase.cpp:567-594).
-__global__ void __launch_bounds__(64) hex8p0_scatter_kernel(const int32_t* __restrict__ conn, ScatterArgs sc,
-                                                            const double* __restrict__ Kc,
-                                                            const double* __restrict__ Fc) {
-  const int lane = threadIdx.x;
-  const int idx = blockIdx.x;  // grid = sc.count
-  const int e = sc.elems[idx];
-  const int a = lane >> 3, b = lane & 7;
-  const int na = conn[(size_t)e * 8 + a], nb = conn[(size_t)e * 8 + b];
-  const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
-  const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
-  const double* Ke = Kc + (size_t)e * 576;
-  double Kb[9];
+    if (b == 0) {
+      const size_t o = SCATTER ? (size_t)sc.elems[idx] : (size_t)idx;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk) Kb[3 * i + kk] = Ke[(3 * a + i) * 24 + 3 * b + kk];
-  double c[3] = {0.0, 0.0, 0.0};
-  if (sc.dispApplied) {
-    double db[3];
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double v = Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
-      v += shfl_xor_d(v, 1);
-      v += shfl_xor_d(v, 2);
-      v += shfl_xor_d(v, 4);
-      c[i] = v;
+      for (int i = 0; i < 3; ++i) Kup[o * 24 + 3 * a + i] = sm.Kup[k][3 * a + i];
+      if (a == 0) { Kpp[o] = kpp; Fp[o] = sm.Fp[k]; }
     }
-  }
-  int addr[9];
-  const int s0 = (sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
-  block_slots(s0, ma, mb, La, addr);
-  rmw_add<9>(sc.vals, addr, Kb);
-  if (b == 0) {
-    int ra[3];
-    double rv[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      ra[i] = ((ma >> i) & 1) ? sc.nda[3 * (size_t)na + i] : -1;
-      rv[i] = Fc[(size_t)e * 24 + 3 * a + i] - c[i];
-    }
-    rmw_add<3>(sc.rhs, ra, rv);
   }
 }
 
@@ -628,16 +627,18 @@ mpf_status launch_hex8_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStre
 mpf_status launch_hex8p0_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kc, double* Fc, double* Kup,
                                double* Kpp, double* Fp, hipStream_t s) {
   if (count <= 0) return MPF_OK;
-  hipLaunchKernelGGL(hex8p0_dense_kernel, dim3(grid_for(count, 8)), dim3(64), 0, s, a, first, count, Kc, Fc, Kup, Kpp,
-                     Fp);
+  ScatterArgs sc{};
+  hipLaunchKernelGGL(hex8p0_kernel<false>, dim3(grid_for(count, 8)), dim3(64), 0, s, a, first, count, Kc, Fc, Kup,
+                     Kpp, Fp, sc);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
 
-mpf_status launch_hex8p0_scatter(const int32_t* conn, const ScatterArgs& sc, const double* Kc, const double* Fc,
+mpf_status launch_hex8p0_scatter(const ElemArgs& a, const ScatterArgs& sc, double* Kup, double* Kpp, double* Fp,
                                  hipStream_t s) {
   if (sc.count <= 0) return MPF_OK;
-  hipLaunchKernelGGL(hex8p0_scatter_kernel, dim3(sc.count), dim3(64), 0, s, conn, sc, Kc, Fc);
+  hipLaunchKernelGGL(hex8p0_kernel<true>, dim3(grid_for(sc.count, 8)), dim3(64), 0, s, a, 0, sc.count, nullptr,
+                     nullptr, Kup, Kpp, Fp, sc);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

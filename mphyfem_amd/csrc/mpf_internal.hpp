@@ -109,7 +109,7 @@ mpf_status launch_hex8p0_dense(const ElemArgs& a, int32_t first, int32_t count, 
 mpf_status launch_tet10_dense(const ElemArgs& a, int32_t first, int32_t count, double* Kuu, double* Kup,
                               double* Kpu, double* Kpp, double* Fu, double* Fp, hipStream_t s);
 mpf_status launch_hex8_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
-mpf_status launch_hex8p0_scatter(const int32_t* conn, const ScatterArgs& sc, const double* Kc, const double* Fc,
+mpf_status launch_hex8p0_scatter(const ElemArgs& a, const ScatterArgs& sc, double* Kup, double* Kpp, double* Fp,
                                  hipStream_t s);
 mpf_status launch_hex8p0_pressure(int32_t nElem, const int32_t* conn, const int32_t* nda, const double* du,
                                   const double* dispApplied, const double* Kup, const double* Kpp, const double* Fp,
@@ -183,9 +183,9 @@ struct mpf_problem {
   mpf::PatternBufs pat;
   bool prepared = false;
   // Hex8 Q1/P0 (elem_type MPF_HEX8P0; etype is MPF_HEX8 for the pattern): one pressure per element,
-  // condensed element outputs of the last assembly (Kc [nElem][24][24], Fc, Kup [nElem][24], Kpp, Fp)
+  // and the last assembly's Kup [nElem][24], Kpp [nElem], Fp [nElem] for the pressure recovery
   bool p0 = false;
-  mpf::DevBuf<double> q_Kc, q_Fc, q_Kup, q_Kpp, q_Fp;
+  mpf::DevBuf<double> q_Kup, q_Kpp, q_Fp;
   // state (pres, presPrev: [npres] = nNode, or nElem for Q1/P0)
   int64_t npres = 0;
   mpf::DevBuf<double> disp, dispPrev, pres, presPrev, dispApplied;

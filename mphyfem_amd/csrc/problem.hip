@@ -226,12 +226,10 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
   sc.rhs = p->rhs.ptr;
   sc.dispApplied = first_iter ? p->dispApplied.ptr : nullptr;
   sc.scatterK = scatterK;
-  if (p->p0)  // condensed element matrices of every element, then scattered colour by colour
-    MPF_TRY(launch_hex8p0_dense(A, 0, p->nElem, p->q_Kc.ptr, p->q_Fc.ptr, p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, s));
   for (int c = 0; c < P.ncolors; ++c) {
     sc.elems = P.perm.ptr + P.color_off[c];
     sc.count = P.color_off[c + 1] - P.color_off[c];
-    if (p->p0) MPF_TRY(launch_hex8p0_scatter(p->conn.ptr, sc, p->q_Kc.ptr, p->q_Fc.ptr, s));
+    if (p->p0) MPF_TRY(launch_hex8p0_scatter(A, sc, p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, s));
     else if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
     else MPF_TRY(launch_tet10_scatter(A, sc, s));
   }
@@ -562,8 +560,6 @@ mpf_status mpf_prepare_pattern(mpf_problem* p) {
   if (p->asm_mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
   if (p->p0) {
     const size_t ne = (size_t)p->nElem;
-    MPF_TRY(p->q_Kc.alloc(ne * 576));
-    MPF_TRY(p->q_Fc.alloc(ne * 24));
     MPF_TRY(p->q_Kup.alloc(ne * 24));
     MPF_TRY(p->q_Kpp.alloc(ne));
     MPF_TRY(p->q_Fp.alloc(ne));
