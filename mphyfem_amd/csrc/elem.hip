@@ -255,10 +255,8 @@ __global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, i
   }
   __syncthreads();
   const int a = lane >> 3, b = lane & 7;
-  for (int k = 0; k < 8; ++k) {
-    const int idx = base_idx + k;
-    if (idx >= count) break;  // uniform across the wave
-    double Kb[9];
+  // condensed 3x3 block (a, b) and condensed residual of node a for element slot k
+  auto condensed = [&](int k, double Kb[9], double fc[3]) {
     tangent_block<8, false>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 48, &sm.S[k][0][0], 8, Kb);
     const double kpp = sm.Kpp[k];
 #pragma unroll
@@ -266,10 +264,22 @@ __global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, i
 #pragma unroll
       for (int j = 0; j < 3; ++j) Kb[3 * i + j] = Kb[3 * i + j] - sm.Kup[k][3 * a + i] * (sm.Kup[k][3 * b + j] / kpp);
     const double fps = sm.Fp[k] / kpp;
-    double fc[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) fc[i] = sm.R[k][a][i] - sm.Kup[k][3 * a + i] * fps;
-    if (!SCATTER) {
+  };
+  auto keep_coupling = [&](int k, size_t o) {  // Kup, Kpp, Fp at output index o
+    if (b == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Kup[o * 24 + 3 * a + i] = sm.Kup[k][3 * a + i];
+      if (a == 0) { Kpp[o] = sm.Kpp[k]; Fp[o] = sm.Fp[k]; }
+    }
+  };
+  if (!SCATTER) {
+    for (int k = 0; k < 8; ++k) {
+      const int idx = base_idx + k;
+      if (idx >= count) break;  // uniform across the wave
+      double Kb[9], fc[3];
+      condensed(k, Kb, fc);
       double* Ko = Kc + (size_t)idx * 576;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
@@ -278,11 +288,28 @@ __global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, i
       if (b == 0)
 #pragma unroll
         for (int i = 0; i < 3; ++i) Fc[(size_t)idx * 24 + 3 * a + i] = fc[i];
-    } else {
-      const int e = sc.elems[idx];
+      keep_coupling(k, (size_t)idx);
+    }
+    return;
+  }
+  // scatter: two elements per batch (18 independent CSR loads per lane); elements of one colour
+  // share no node, so the 18 slots of a lane are distinct
+  for (int k0 = 0; k0 < 8; k0 += 2) {
+    if (base_idx + k0 >= count) break;  // uniform across the wave
+    int addr[18];
+    double val[18];
+    int ra[6];
+    double rv[6];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + h;
+      const bool live = base_idx + k < count;
+      double Kb[9], fc[3];
+      condensed(k, Kb, fc);
+      const int e = sc.elems[live ? base_idx + k : base_idx + k0];
       const int na = A.conn[(size_t)e * 8 + a], nb = A.conn[(size_t)e * 8 + b];
       const int infa = sc.nodeinfo[na], infb = sc.nodeinfo[nb];
-      const int ma = infa & 7, mb = infb & 7, La = infa >> 4;
+      const int ma = live ? (infa & 7) : 0, mb = infb & 7, La = infa >> 4;
       double c[3] = {0.0, 0.0, 0.0};
       if (sc.dispApplied) {
         double db[3];
@@ -297,27 +324,19 @@ __global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, i
           c[i] = v;
         }
       }
-      int addr[9];
       const int s0 = (sc.scatterK && ma && mb) ? sc.slot[(size_t)e * kHex8Slots + lane] : -1;
-      block_slots(s0, ma, mb, La, addr);
-      rmw_add<9>(sc.vals, addr, Kb);
-      if (b == 0) {
-        int ra[3];
-        double rv[3];
+      block_slots(s0, ma, mb, La, addr + 9 * h);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          ra[i] = ((ma >> i) & 1) ? sc.nda[3 * (size_t)na + i] : -1;
-          rv[i] = fc[i] - c[i];
-        }
-        rmw_add<3>(sc.rhs, ra, rv);
+      for (int t = 0; t < 9; ++t) val[9 * h + t] = Kb[t];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        ra[3 * h + i] = (b == 0 && ((ma >> i) & 1)) ? sc.nda[3 * (size_t)na + i] : -1;
+        rv[3 * h + i] = fc[i] - c[i];
       }
+      if (live) keep_coupling(k, (size_t)e);
     }
-    if (b == 0) {
-      const size_t o = SCATTER ? (size_t)sc.elems[idx] : (size_t)idx;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) Kup[o * 24 + 3 * a + i] = sm.Kup[k][3 * a + i];
-      if (a == 0) { Kpp[o] = kpp; Fp[o] = sm.Fp[k]; }
-    }
+    rmw_add<18>(sc.vals, addr, val);
+    rmw_add<6>(sc.rhs, ra, rv);
   }
 }
 
