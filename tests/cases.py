@@ -88,15 +88,16 @@ def tet10_fan(subdiv: int = 1, radius: float = 1.0):
     return np.array(X), np.array(conn, dtype=np.int32), np.array(outer, dtype=np.int32)
 
 
-def q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps, dt, rtol=1e-10, itmax=20, Utype=3):
+def q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps, dt, rtol=1e-10, itmax=20, Utype=3, fext=None):
     """CPU Newton driver for Hex8 Q1/P0 on a small mesh (dense global system), built on the oracle's
     condensed element (orc.elem_hex8p0) and following the reference's order of operations:
     saveSolution, setBoundaryConditions (dispApplied = value * lambda - disp on prescribed dofs,
     femSolidmechanics1.cpp:864-890), calcStiffnessAndResidual with the first-iteration lift
     (ElementBase.cpp:567-594), ||R|| / ||R0|| < rtol checked after each assembly, then the update
     (femSolidmechanics2.cpp:1137-1186) and solveForPressure per element
-    (Elem_Solid_3D_Mixed0.cpp:112-199).  Returns (disp [nNode*3], element pressures [nElem],
-    Newton iterations per step)."""
+    (Elem_Solid_3D_Mixed0.cpp:112-199).  fext: external force on every dof for lambda = 1, added as
+    lambda * fext (femSolidmechanics2.cpp:785-789).  Returns (disp [nNode*3], element pressures
+    [nElem], Newton iterations per step)."""
     X = np.asarray(X, dtype=np.float64)
     conn = np.asarray(conn)
     nn, ne = len(X), len(conn)
@@ -128,6 +129,8 @@ def q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps, dt, rtol=1e-10, itmax=20, U
                 K[np.ix_(d, d)] += Kc
                 F[d] += Fc
                 Kup[e], Kpp[e], Fp[e] = kpu, kpp, fp
+            if fext is not None:
+                F += lam * fext
             R = F[free]
             if first:
                 R = R - K[np.ix_(free, presc)] @ applied
@@ -148,3 +151,22 @@ def q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps, dt, rtol=1e-10, itmax=20, U
             raise RuntimeError(f"Q1/P0 Newton did not converge at step {step}")
         iters.append(it)
     return u, p, iters
+
+
+def oracle_fext(orc, X, conn, bcs, faces, v0, v1=0.0):
+    """External force of a Quad4 dead-load traction patch for lambda = 1 on every dof (0 on prescribed
+    ones), from the oracle's Hex8 problem assembled at the stress-free state, where the internal force
+    vanishes and the right-hand side is the external force alone."""
+    m = M.matl_from_E_nu(1e3, 0.3)
+    O = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
+    for nodes, dof, val in bcs:
+        O.add_dirichlet(nodes, dof, val)
+    O.add_traction(faces, v0, v1)
+    O.prepare_matrix_pattern()
+    _, rhs, st = O.assemble(False, 1.0)
+    assert st == 0
+    _, _, nda, _ = O.pattern()
+    f = np.zeros(3 * len(X))
+    d = nda.ravel()
+    f[d >= 0] = rhs[d[d >= 0]]
+    return f

@@ -86,6 +86,30 @@ def test_hex8p0_batch_vs_oracle(mpf, orc, gpu):
         assert normwise_rel(got, want) < 1e-12, (k, normwise_rel(got, want))
 
 
+def test_hex8p0_newton_traction_vs_oracle(mpf, orc, gpu):
+    """Hex8 Q1/P0 under a normal dead-load traction on the top face as well as the end stretch: the
+    external force enters the condensed system like the displacement element's (lambda * Fext added
+    to the right-hand side); converged displacements and element pressures within 1e-9."""
+    from tests.cases import oracle_fext
+    n, mu, Kinv = 3, 1.0, 1e-3
+    X, conn = cube_hex(n, jitter=0.1, seed=4)
+    bcs = hex_stretch_bcs(n)
+    faces, v0 = mpf.hex_face_quads(n, n, n, 5), -0.05
+    P = mpf.Problem(mpf.HEX8P0, X, conn, mpf.matl(mu, Kinv, 3))
+    apply_bcs(P, bcs)
+    P.add_traction(faces, v0)
+    P.prepare_matrix_pattern()
+    opts = P.newton_opts(nsteps=2, dt=0.5, itmax=20, rtol=1e-10, ksp=mpf.KSP_BAND)
+    iters, rep = P.solve_with_newton_raphson(opts)
+    assert rep.steps_done == 2, iters
+    ud, pd = P.state()
+    fext = oracle_fext(orc, X, conn, bcs, faces, v0)
+    assert np.abs(fext).max() > 0
+    uo, po, _ = q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps=2, dt=0.5, rtol=1e-10, fext=fext)
+    assert normwise_rel(ud, uo) < 1e-9, normwise_rel(ud, uo)
+    assert normwise_rel(pd, po) < 1e-9, normwise_rel(pd, po)
+
+
 @pytest.mark.parametrize("ksp", ["band", "cg"])
 def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp):
     """Hex8 Q1/P0 problem on the GPU (condensed element matrices scattered colour by colour, element
