@@ -270,8 +270,11 @@ def bench_full(M, a) -> int:
 
     n = a.n or 128
     modes = tuple(m for m in a.full_modes.split(",") if m)
+    p0 = a.full_elem == "hex8p0"
+    if p0:
+        modes = ("plain",)
     r = learned.block_experiment(n_train=48, n_eval=n, eval_steps=a.full_steps, epochs=a.epochs, log=log,
-                                 modes=modes)
+                                 modes=modes, elem_type=M.HEX8P0 if p0 else M.HEX8)
 
     def per_step(mode):
         later = r[mode][1:]  # step 1 has no history: every mode starts it plainly
@@ -279,10 +282,11 @@ def bench_full(M, a) -> int:
 
     t = {m: per_step(m) for m in modes}
     first = modes[0]
-    out = {"metric": "s per converged load step (128^3 Hex8, nu=0.49, block-Jacobi PCG Newton)",
+    elem = "Hex8 Q1/P0" if p0 else "Hex8"
+    out = {"metric": f"s per converged load step ({n}^3 {elem}, nu=0.49, block-Jacobi PCG Newton)",
            "value": round(t[first], 3), "unit": "s", "n_gpus": 1, "steps": a.full_steps, "warmup": 0,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"{n}^3 Hex8 block, E=1e3, nu=0.49, bottom fixed, top UZ=-0.3 lambda, 15 steps; "
+           "config": {"workload": f"{n}^3 {elem} block, E=1e3, nu=0.49, bottom fixed, top UZ=-0.3 lambda, 15 steps; "
                                   f"first {a.full_steps} measured, start points {','.join(modes)} (value: {first})"},
            "seconds_per_step": {m: round(v, 3) for m, v in t.items()},
            "speedup_vs_plain": {m: round(t["plain"] / v, 3) for m, v in t.items() if "plain" in t and m != "plain" and v},
@@ -336,6 +340,8 @@ def main(argv=None) -> int:
                          "direct: the 200 held-out cantilevers, plain full Newton with the banded direct solve; "
                          "full: 128^3 Hex8 nu=0.49 Newton load steps, plain vs extrapolated start")
     ap.add_argument("--full-steps", type=int, default=2, help="full: load steps measured (of 15)")
+    ap.add_argument("--full-elem", choices=["hex8", "hex8p0"], default="hex8",
+                    help="full: displacement Hex8, or Hex8 Q1/P0 (plain start only)")
     ap.add_argument("--full-modes", default="plain,extrap,learned",
                     help="full: comma-separated start points to run (plain, extrap, learned)")
     ap.add_argument("--nmesh", type=int, default=256, help="batch: number of blocks")

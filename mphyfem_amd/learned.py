@@ -255,12 +255,15 @@ def experiment(nbeams: int = 2000, ntrain: int = 1800, ntest: int = 200, epochs:
 BLOCK_STEPS = 15
 
 
-def compression_block(n: int) -> Problem:
-    """n^3 Hex8 unit block, E = 1e3, nu = 0.49, bottom fixed, top UZ = -0.3 lambda (15 load steps)."""
-    from . import TANGENT_CONSISTENT as TC, matl_from_E_nu
+def compression_block(n: int, elem_type: int = HEX8) -> Problem:
+    """n^3 Hex8 unit block, E = 1e3, nu = 0.49, bottom fixed, top UZ = -0.3 lambda (15 load steps);
+    elem_type HEX8 (displacement) or HEX8P0 (Q1/P0, one condensed pressure per element)."""
+    from . import HEX8P0, TANGENT_CONSISTENT as TC, matl_from_E_nu
 
+    if elem_type not in (HEX8, HEX8P0):
+        raise ValueError("compression_block: elem_type HEX8 or HEX8P0")
     X, conn = mesh_hex(n, n, n, jitter=0.0, seed=5)
-    P = Problem(HEX8, X, conn, matl_from_E_nu(1e3, 0.49, tangent=TC))
+    P = Problem(elem_type, X, conn, matl_from_E_nu(1e3, 0.49, tangent=TC))
     P.add_dirichlet(face_nodes(HEX8, n, n, n, 4), -1)
     P.add_dirichlet(face_nodes(HEX8, n, n, n, 5), 2, -0.3)
     P.prepare_matrix_pattern()
@@ -284,8 +287,9 @@ def newton_with_cut(P: Problem, lam0: float, lam1: float, opts, guess=None, dept
     Jacobian, divergence or no convergence the state is restored and the increment halved, both
     halves started plainly (MyTime::cut, MyTime.cpp:26-106; reset, SolutionDataSolid.cpp:157-167),
     down to 1/64 of the increment.  Returns (assemblies and Krylov iterations of the converged
-    solves, number of cuts).  log: called after every failed attempt and converged sub-step."""
-    ustart, _ = P.state()
+    solves, number of cuts).  log: called after every failed attempt and converged sub-step.
+    The restored state holds the pressures too (nodal for Tet10/P1, per element for Q1/P0)."""
+    ustart, pstart = P.state()
     try:
         it, rep = P.newton_step(lam1, opts, guess)
         if log and depth > 0:
@@ -296,7 +300,7 @@ def newton_with_cut(P: Problem, lam0: float, lam1: float, opts, guess=None, dept
             raise
         if log:
             log(f"  Newton to lambda {lam1:.6f} failed ({e}); cutting the increment in half")
-    P.set_state(ustart)
+    P.set_state(ustart, pres=pstart)
     mid = 0.5 * (lam0 + lam1)
     a1, k1, c1 = newton_with_cut(P, lam0, mid, opts, None, depth + 1, max_depth, log)
     a2, k2, c2 = newton_with_cut(P, mid, lam1, opts, None, depth + 1, max_depth, log)
@@ -344,16 +348,19 @@ def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, sto
 
 
 def block_experiment(n_train: int = 48, n_eval: int = 128, eval_steps: int = 3, epochs: int = 40,
-                     lr: float = 2e-3, log=print, modes=("plain", "extrap", "learned")) -> dict:
+                     lr: float = 2e-3, log=print, modes=("plain", "extrap", "learned"), elem_type: int = HEX8) -> dict:
     """Train the network on the trajectory of an n_train^3 block (all 15 steps, plain Newton), then run
     the first eval_steps steps of the n_eval^3 block from the start points in modes (plain,
     extrapolated, learned; the network is trained only when 'learned' is among them)."""
     import torch
 
-    out = {"train_block": n_train, "eval_block": n_eval, "eval_steps": eval_steps, "epochs": epochs}
+    out = {"train_block": n_train, "eval_block": n_eval, "eval_steps": eval_steps, "epochs": epochs,
+           "element": "Hex8 Q1/P0" if elem_type != HEX8 else "Hex8"}
     net = Mpnn(seed=5)
+    if elem_type != HEX8 and modes != ("plain",):
+        raise ValueError("block_experiment: the Q1/P0 block runs from the plain start only")
     if "learned" not in modes:
-        Pe = compression_block(n_eval)
+        Pe = compression_block(n_eval, elem_type)
         for mode in modes:
             out[mode] = block_steps(Pe, mode, eval_steps, net, log=log)
         Pe.close()

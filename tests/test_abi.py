@@ -122,13 +122,15 @@ class _CutProblem:
     max_inc, and a record of the load factors reached."""
 
     def __init__(self, max_inc):
-        self.max_inc, self.lam, self.u, self.calls = max_inc, 0.0, np.zeros(3), []
+        self.max_inc, self.lam, self.u, self.p, self.calls = max_inc, 0.0, np.zeros(3), np.zeros(2), []
 
     def state(self):
-        return self.u.copy(), None
+        return self.u.copy(), self.p.copy()
 
-    def set_state(self, u):
+    def set_state(self, u, pres=None):
         self.u = np.array(u)
+        if pres is not None:
+            self.p = np.array(pres)
 
     def newton_step(self, lam, opts, guess):
         import types
@@ -137,7 +139,9 @@ class _CutProblem:
         self.calls.append(lam)
         if lam - self.lam > self.max_inc + 1e-12:
             self.u = self.u + 1.0  # a failed attempt leaves a spoiled state behind
+            self.p = self.p - 5.0
             raise M.NegativeJacobian(M.ERR_NEG_JACOBIAN, "newton_step")
+        self.p = self.p + (lam - self.lam)  # pressures accumulate over converged increments
         self.lam = lam
         self.u = np.full(3, lam)
         return 3, types.SimpleNamespace(krylov_iters=10)
@@ -148,6 +152,7 @@ def test_newton_step_cut_reaches_the_target_by_halving():
     P = _CutProblem(max_inc=0.3)
     it, kry, cuts = newton_with_cut(P, 0.0, 1.0, None)
     assert P.lam == 1.0 and np.all(P.u == 1.0)  # restored after each failure, then sub-stepped
+    assert np.allclose(P.p, 1.0)  # the pressure state is restored with the displacements
     assert cuts == 3 and it == 4 * 3 and kry == 4 * 10  # 1 -> 0.5 -> 0.25 x2, then 0.25 x2
     assert P.calls == [1.0, 0.5, 0.25, 0.5, 1.0, 0.75, 1.0]
 
