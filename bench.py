@@ -277,7 +277,7 @@ def bench_full(M, a) -> int:
                                  modes=modes, elem_type=M.HEX8P0 if p0 else M.HEX8)
 
     def per_step(mode):
-        later = r[mode][1:]  # step 1 has no history: every mode starts it plainly
+        later = [s for s in r[mode][1:] if "failed" not in s]  # step 1: every mode starts plainly
         return sum(s["seconds"] for s in later) / max(len(later), 1)
 
     t = {m: per_step(m) for m in modes}
@@ -289,6 +289,7 @@ def bench_full(M, a) -> int:
            "config": {"workload": f"{n}^3 {elem} block, E=1e3, nu=0.49, bottom fixed, top UZ=-0.3 lambda, 15 steps; "
                                   f"first {a.full_steps} measured, start points {','.join(modes)} (value: {first})"},
            "seconds_per_step": {m: round(v, 3) for m, v in t.items()},
+           "steps_converged": {m: sum(1 for s in r[m] if "failed" not in s) for m in modes},
            "speedup_vs_plain": {m: round(t["plain"] / v, 3) for m, v in t.items() if "plain" in t and m != "plain" and v},
            "full": r}
     print(json.dumps(out), flush=True)

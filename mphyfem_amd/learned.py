@@ -334,7 +334,16 @@ def block_steps(P: Problem, mode: str, nsteps: int, net: Mpnn | None = None, sto
             else:
                 net.guess(P, hist[1], hist[0], 1.0, False, guess)
             g = guess.cpu().numpy()
-        it, kry, cuts = newton_with_cut(P, k / BLOCK_STEPS, lam, opts, g, log=log)
+        try:
+            it, kry, cuts = newton_with_cut(P, k / BLOCK_STEPS, lam, opts, g, log=log)
+        except MpfError as e:
+            if e.status not in _CUT_STATUS:
+                raise
+            # not reached at the smallest increment: recorded, and the load path ends here
+            steps.append({"step": k + 1, "failed": str(e), "seconds": round(time.perf_counter() - t0, 3)})
+            if log:
+                log(f"block {mode} step {k + 1}: not reached ({e})")
+            break
         dt = time.perf_counter() - t0
         hist[0].copy_(hist[1])
         P.copy_state_device(hist[1])
