@@ -110,6 +110,34 @@ def test_hex8p0_newton_traction_vs_oracle(mpf, orc, gpu):
     assert normwise_rel(pd, po) < 1e-9, normwise_rel(pd, po)
 
 
+def test_hex8p0_relieves_locking_on_gpu(mpf, orc, gpu):
+    """Clamped 4^3 block under a small top load at nu = 0.4999: the GPU displacement Hex8 locks (over
+    10x less deflection) where the GPU Q1/P0 does not, and the Q1/P0 displacements match the CPU
+    Q1/P0 driver within 1e-9."""
+    from tests.cases import oracle_fext
+    n, nu = 4, 0.4999
+    X, conn = mpf.mesh_hex(n, n, n, jitter=0.0, seed=5)
+    bot, top = mpf.face_nodes(mpf.HEX8, n, n, n, 4), mpf.face_nodes(mpf.HEX8, n, n, n, 5)
+    faces = mpf.hex_face_quads(n, n, n, 5)
+    uz = {}
+    for name, et in (("hex8", mpf.HEX8), ("q1p0", mpf.HEX8P0)):
+        P = mpf.Problem(et, X, conn, mpf.matl_from_E_nu(1.0, nu, tangent=mpf.TANGENT_CONSISTENT))
+        P.add_dirichlet(bot, -1)
+        P.add_traction(faces, -1e-3)
+        P.prepare_matrix_pattern()
+        _, rep = P.solve_with_newton_raphson(P.newton_opts(nsteps=1, dt=1.0, itmax=30, rtol=1e-8,
+                                                           ksp=mpf.KSP_BAND))
+        assert rep.steps_done == 1
+        uz[name] = P.state()[0]
+        P.close()
+    assert uz["q1p0"][3 * top + 2].mean() / uz["hex8"][3 * top + 2].mean() > 10.0
+    m = mpf.matl_from_E_nu(1.0, nu)
+    bcs = [(bot, -1, None)]
+    uo, _, _ = q1p0_newton(orc, X, conn, bcs, m.mu, m.Kinv, 1, 1.0, rtol=1e-8,
+                           fext=oracle_fext(orc, X, conn, bcs, faces, -1e-3))
+    assert normwise_rel(uz["q1p0"], uo) < 1e-9, normwise_rel(uz["q1p0"], uo)
+
+
 @pytest.mark.parametrize("ksp", ["band", "cg"])
 def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp):
     """Hex8 Q1/P0 problem on the GPU (condensed element matrices scattered colour by colour, element
