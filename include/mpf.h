@@ -214,6 +214,8 @@ mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, in
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);
 /* factoriseSolveAndUpdate's update (femSolidmechanics2.cpp:1124-1217): on the first iteration   */
 /* prescribed dofs take dispApplied, free dofs += du, pressure dofs += du[dispDOF + k].          */
+/* MPF_HEX8P0: then every element pressure += (Fp - Kpu . du_e) / Kpp from the last assembly    */
+/* (solveForPressure, Elem_Solid_3D_Mixed0.cpp:112-199); MPF_ERR_STATE before any assembly.      */
 mpf_status mpf_update(mpf_problem* p, int32_t first_iter);
 
 /* Newton-Raphson load stepping (solveWithNewtonRaphson, femSolidmechanics2.cpp:730-851).        */
