@@ -268,6 +268,7 @@ mpf_status assemble_sync(mpf_problem* p, bool first_iter, double lam, int scatte
 }
 
 mpf_status do_update(mpf_problem* p, bool first_iter) {
+  if (p->p0 && !p->assembled) return MPF_ERR_STATE;  // the pressure recovery reads the last assembly
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   if (first_iter && !p->bc_dof_h.empty())
@@ -280,7 +281,6 @@ mpf_status do_update(mpf_problem* p, bool first_iter) {
     hipLaunchKernelGGL(k_update_pres, dim3(grid_for(P.presDOF, 256)), dim3(256), 0, s, P.presDOF, P.assyP.ptr,
                        p->du.ptr, P.dispDOF, p->pres.ptr);
   if (p->p0) {  // from the element matrices of the assembly this correction solved
-    if (!p->assembled) return MPF_ERR_STATE;
     MPF_TRY(launch_hex8p0_pressure(p->nElem, p->conn.ptr, P.nda.ptr, p->du.ptr, first_iter ? p->dispApplied.ptr : nullptr,
                                    p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, p->pres.ptr, s));
   }
