@@ -431,7 +431,7 @@ def main(argv=None) -> int:
     unique = nnz * 8 + nElem * npe * 4 + len(X) * 48 + (dispDOF + presDOF) * 8 + presDOF * 8
     flops = FLOPS_PER_ELEM[wl] * nElem
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(wl) if a.assembly == "gather" else None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(wl) if a.assembly == "gather" or wl == "hex8p0" else None,
                 "kernel": f"{kname} per assembly)", "assembly": a.assembly,
                 "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe,
                 "unique_bytes": unique,
