@@ -63,7 +63,7 @@ def cpu_baseline(M, workload: str, budget_s: float):
     from oracle import oracle as O
     n = 16 if workload == "tet10" else 32
     X, conn, _, bcs = build_problem_host(M, n, workload)
-    Q = O.Problem(10 if workload == "tet10" else 8, X, conn, *material_host(M, workload))
+    Q = O.Problem({"tet10": 10, "hex8": 8, "hex8p0": 9}[workload], X, conn, *material_host(M, workload))
     for nodes, dof, val in bcs:
         Q.add_dirichlet(nodes, dof, val)
     Q.prepare_matrix_pattern()
@@ -77,7 +77,8 @@ def cpu_baseline(M, workload: str, budget_s: float):
         Q.assemble_range(0, len(conn), vals, rhs)
         done += len(conn)
     dt = time.perf_counter() - t0
-    shape = f"{n}^3x6 = {len(conn)}-element Tet10/P1" if workload == "tet10" else f"{n}^3 = {len(conn)}-element Hex8"
+    shape = {"tet10": f"{n}^3x6 = {len(conn)}-element Tet10/P1", "hex8": f"{n}^3 = {len(conn)}-element Hex8",
+             "hex8p0": f"{n}^3 = {len(conn)}-element Hex8 Q1/P0"}[workload]
     out = {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
            "sample": f"oracle (serial C++, g++ -O3) assembling {done} elements: repeated full sweeps of a "
                      f"{shape} mesh of the same generator, material and boundary conditions, {dt:.1f} s"}
@@ -94,35 +95,6 @@ def cpu_baseline(M, workload: str, budget_s: float):
                         "sample": f"the same oracle loop over the {ncol} element colours with {nthr} OpenMP threads "
                                   f"inside a colour, {done} elements in {dt:.1f} s"}
     return out
-
-
-def cpu_baseline_hex8p0(M, budget_s: float):
-    """Oracle Q1/P0 element routine (serial C++, -O3, one core) over a bounded sample of the same mesh
-    and material: element computation and condensation only (no scatter), one ctypes call each."""
-    import ctypes as C
-    from oracle import oracle as O
-    n = 32
-    X, conn, _, _ = build_problem_host(M, n, "hex8p0")
-    mu, Kinv, Ut = material_host(M, "hex8p0")
-    Xe = np.ascontiguousarray(X[conn])      # [nElem][8][3]
-    Ue = np.ascontiguousarray(0.01 * Xe)   # a small deformation
-    D = C.POINTER(C.c_double)
-    px = [C.cast(C.c_void_p(Xe.ctypes.data + 192 * e), D) for e in range(len(conn))]
-    pu = [C.cast(C.c_void_p(Ue.ctypes.data + 192 * e), D) for e in range(len(conn))]
-    outs = [np.zeros(k) for k in (576, 24, 576, 24, 24, 24, 1, 1)]
-    po = [o.ctypes.data_as(D) for o in outs]
-    f = O.lib().orc_elem_hex8p0
-    done = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        for e in range(len(conn)):
-            f(px[e], pu[e], pu[e], 0.0, mu, Kinv, Ut, *po)
-        done += len(conn)
-    dt = time.perf_counter() - t0
-    return {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": f"oracle Q1/P0 element routine (serial C++, g++ -O3, one ctypes call per element, no "
-                      f"scatter) over {done} elements: repeated sweeps of the {n}^3 = {len(conn)}-element block "
-                      f"of the same generator and material, {dt:.1f} s"}
 
 
 def build_problem_host(M, n, workload):
@@ -478,8 +450,7 @@ def main(argv=None) -> int:
         log(f"{name} {r.iterations} its relres {r.relres:.2e} {r.seconds:.2f} s")
     if not a.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline_hex8p0(M, a.cpu_seconds) if wl == "hex8p0" else \
-            cpu_baseline(M, wl, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(M, wl, a.cpu_seconds)
     P.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -509,6 +509,8 @@ struct orc_problem {
   std::vector<int> conn;       // [nElem][npe]
   double mu, Kinv; int Utype;
   bool mixed;
+  bool p0 = false;          // Hex8 Q1/P0 (elem_type 9): pressure per element, condensed out of the system
+  std::vector<double> qKpu, qKpp, qFp;  // Q1/P0: the last assembly's Kpu [nElem][24], Kpp, Fp per element
   bool consistent = false;  // tangent mode (see neohooke)
   std::vector<DirPatch> dir;
   std::vector<TracPatch> trac;
@@ -593,8 +595,9 @@ double norm2(const double* v, int n) {
 // SolverPetsc::assembleMatrixAndVector (SolverPetsc.cpp:357-378) / ...2field (:434-502).
 // Per-element scratch of the assembly loop.
 struct AsmWork {
-  std::vector<double> Ke, Re, Kup, Kpu, Kpp, Fp;
-  explicit AsmWork(int nsize) : Ke(nsize * nsize), Re(nsize), Kup(nsize * 4), Kpu(4 * nsize), Kpp(16), Fp(4) {}
+  std::vector<double> Ke, Re, Kup, Kpu, Kpp, Fp, Kuu, Fu;
+  explicit AsmWork(int nsize)
+      : Ke(nsize * nsize), Re(nsize), Kup(nsize * 4), Kpu(4 * nsize), Kpp(16), Fp(4), Kuu(nsize * nsize), Fu(nsize) {}
 };
 
 // One element of calcStiffnessAndResidual's loop (femSolidmechanics2.cpp:1061-1111): element
@@ -613,7 +616,16 @@ static int assemble_elem(orc_problem* p, int e, bool firstIter, double* vals, do
     }
   elem_forassy(p, e, fav);
   if (!p->mixed) {
-    if (elem_disp(npe, Xe, ue, p->mu, p->Kinv, p->Utype, Ke.data(), Re.data(), p->consistent)) return 1;
+    if (p->p0) {  // condensed Q1/P0 element; Kpu, Kpp, Fp kept for solveForPressure
+      if (elem_hex8p0(Xe, ue, upe, p->pres[e], p->mu, p->Kinv, p->Utype, Ke.data(), Re.data(), W.Kuu.data(),
+                      W.Fu.data(), Kup.data(), Kpu.data(), Kpp.data(), Fp.data()))
+        return 1;
+      std::copy(Kpu.begin(), Kpu.begin() + 24, p->qKpu.begin() + (size_t)e * 24);
+      p->qKpp[e] = Kpp[0];
+      p->qFp[e] = Fp[0];
+    } else if (elem_disp(npe, Xe, ue, p->mu, p->Kinv, p->Utype, Ke.data(), Re.data(), p->consistent)) {
+      return 1;
+    }
     if (firstIter) {
       for (int ii = 0; ii < nsize; ++ii) {
         if (fav[ii] != -1) continue;
@@ -982,9 +994,15 @@ int orc_elem_tet10p1(const double* X, const double* u, const double* uprev, cons
 orc_problem* orc_problem_create(int elem_type, int nNode, int nElem, const double* X, const int32_t* conn,
                                 double mu, double Kinv, int Utype) {
   auto* p = new orc_problem();
-  p->etype = elem_type;
+  p->p0 = (elem_type == 9);
+  p->etype = p->p0 ? 8 : elem_type;  // Q1/P0 shares the Hex8 numbering and pattern
   p->npe = (elem_type == 10) ? 10 : 8;
   p->mixed = (elem_type == 10);
+  if (p->p0) {
+    p->qKpu.assign((size_t)nElem * 24, 0.0);
+    p->qKpp.assign(nElem, 1.0);
+    p->qFp.assign(nElem, 0.0);
+  }
   p->nNode = nNode; p->nElem = nElem;
   p->X.assign(X, X + (size_t)nNode * 3);
   p->conn.assign(conn, conn + (size_t)nElem * p->npe);
@@ -1153,15 +1171,16 @@ int orc_color(const orc_problem* p, int32_t* color) {
 void orc_set_state(orc_problem* p, const double* disp, const double* dispPrev, const double* pres,
                    const double* presPrev) {
   const size_t n3 = (size_t)p->nNode * 3;
+  const int np = p->p0 ? p->nElem : p->nNode;  // pressures per element (Q1/P0) or per node
   if (disp) std::copy(disp, disp + n3, p->disp.begin());
   if (dispPrev) std::copy(dispPrev, dispPrev + n3, p->dispPrev.begin());
-  if (pres) std::copy(pres, pres + p->nNode, p->pres.begin());
-  if (presPrev) std::copy(presPrev, presPrev + p->nNode, p->presPrev.begin());
+  if (pres) std::copy(pres, pres + np, p->pres.begin());
+  if (presPrev) std::copy(presPrev, presPrev + np, p->presPrev.begin());
 }
 
 void orc_get_state(const orc_problem* p, double* disp, double* pres) {
   if (disp) std::copy(p->disp.begin(), p->disp.end(), disp);
-  if (pres) std::copy(p->pres.begin(), p->pres.begin() + p->nNode, pres);
+  if (pres) std::copy(p->pres.begin(), p->pres.begin() + (p->p0 ? p->nElem : p->nNode), pres);
 }
 
 // setBoundaryConditions (femSolidmechanics1.cpp:813-915): dispApplied = value*lambda - disp.
@@ -1265,6 +1284,20 @@ int orc_newton(orc_problem* p, int nsteps, double dt, int solver, int precond, d
         for (int d : p->dofs_specified) p->disp[d] += p->dispApplied[d];
       for (int i = 0; i < p->dispDOF; ++i) p->disp[p->assyForSoln[i]] += du[i];
       for (int i = 0; i < p->presDOF; ++i) p->pres[p->assyForSolnPres[i]] += du[p->dispDOF + i];
+      if (p->p0) {
+        // solveForPressure (Elem_Solid_3D_Mixed0.cpp:112-199) with dispIncr: the solved correction on
+        // free dofs and, in the first iteration, dispApplied on prescribed ones (femSolidmechanics2.cpp:1137-1186)
+        std::vector<double> inc((size_t)p->nNode * 3, 0.0);
+        if (first)
+          for (int d : p->dofs_specified) inc[d] = p->dispApplied[d];
+        for (int i = 0; i < p->dispDOF; ++i) inc[p->assyForSoln[i]] = du[i];
+        for (int e = 0; e < p->nElem; ++e) {
+          double v = 0.0;
+          for (int a = 0; a < 8; ++a)
+            for (int d = 0; d < 3; ++d) v += p->qKpu[(size_t)e * 24 + 3 * a + d] * inc[3 * p->conn[(size_t)e * 8 + a] + d];
+          p->pres[e] += (p->qFp[e] - v) / p->qKpp[e];
+        }
+      }
     }
     if (iters_out) iters_out[step] = iter;
     if (!conv) return 5;

@@ -158,6 +158,7 @@ class Problem:
     """CPU restatement of femSolidmechanics' numbering, pattern, assembly and Newton driver."""
 
     def __init__(self, elem_type, X, conn, mu, Kinv, Utype=3):
+        self.elem_type = elem_type
         self.X = f64(X)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32)
         self.nNode, self.nElem = self.X.shape[0], self.conn.shape[0]
@@ -209,8 +210,9 @@ class Problem:
         lib().orc_set_state(self.h, *[_d(x) for x in a])
 
     def state(self):
+        """(disp [nNode*3], pres [nNode], or [nElem] for elem_type 9, Hex8 Q1/P0)."""
         disp = np.zeros(self.nNode * 3)
-        pres = np.zeros(self.nNode)
+        pres = np.zeros(self.nElem if self.elem_type == 9 else self.nNode)
         lib().orc_get_state(self.h, _d(disp), _d(pres))
         return disp, pres
 

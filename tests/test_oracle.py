@@ -470,3 +470,20 @@ def test_q1p0_relieves_volumetric_locking(orc, mpf):
     p4 = _clamped_block_deflection(orc, mpf, 0.499, "q1p0")
     assert abs(p5 / p4 - 1.0) < 0.01
     assert h5 / h3 < 0.1 and p5 / p3 > 0.8
+
+
+def test_q1p0_oracle_problem_matches_driver(orc):
+    """The C oracle's Q1/P0 problem (elem_type 9: condensed scatter, per-element pressure recovery in
+    its Newton update) reproduces the pinned CPU Q1/P0 driver."""
+    n = 3
+    X, conn = cube_hex(n, jitter=0.1, seed=3)
+    bcs = hex_stretch_bcs(n)
+    Q = orc.Problem(9, X, conn, 1.0, 1e-3, 3)
+    apply_bcs(Q, bcs)
+    Q.prepare_matrix_pattern()
+    st, it, _ = Q.newton(2, 0.5, solver=0, rtol=1e-10, itmax=20)
+    assert st == 0
+    uq, pq = Q.state()
+    u, p, its = q1p0_newton(orc, X, conn, bcs, 1.0, 1e-3, 2, 0.5, rtol=1e-10)
+    assert list(it) == its
+    assert normwise_rel(uq, u) < 1e-12 and normwise_rel(pq, p) < 1e-12
