@@ -1,8 +1,8 @@
 """CPU pins of the oracle (oracle/oracle.cpp), the restatement of the reference path.
 
-The reference has no tests and no golden vectors (SURVEY section 4 of the build notes lists what
-it does have), so each oracle function is pinned by mathematics that a plausible mistake (dropped
-term, wrong sign or index, transposed operand) would fail:
+The reference ships no tests and no golden vectors for this path, so each oracle function is
+pinned by mathematics that a plausible mistake (dropped term, wrong sign or index, transposed
+operand) would fail:
 
   quadrature          exactness on polynomials of the design degree; volumes 8 and 1/6
   basis functions     partition of unity, zero-sum derivatives, Kronecker property at the nodes
@@ -19,6 +19,11 @@ term, wrong sign or index, transposed operand) would fail:
   solvers             scipy sparse direct solve
   Newton              quadratic convergence with the consistent tangent, and the same converged
                       displacement from both tangents where the reference tangent converges
+  Hex8 Q1/P0 element  FD of the uncondensed 25x25 mixed tangent, reference-state closed forms,
+                      the condensation identity
+  Q1/P0 driver        ends at the uncondensed mixed equilibrium (Fu = 0, Fp = 0 per element),
+                      relieves volumetric locking as nu -> 1/2; the oracle's Q1/P0 problem
+                      reproduces it
 """
 from __future__ import annotations
 
