@@ -1,6 +1,6 @@
 """CPU restatement of the learned-guess network (mphyfem_amd/csrc/mpnn.hip), numpy.
 
-TEST INFRASTRUCTURE ONLY: tests/ use it as the checker for the GPU forward and backward kernels;
+TEST INFRASTRUCTURE ONLY: tests/ compare the GPU forward and backward kernels against it;
 the product never imports it.  No counterpart exists in the reference (the learned initial guess
 has none).
 

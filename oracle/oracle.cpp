@@ -1,5 +1,5 @@
 // oracle.cpp -- CPU FP64 restatement of chennachaos/mphyfem's femSolidmechanics
-// assembly + Newton-Krylov path.  TEST INFRASTRUCTURE ONLY (the checker): only
+// assembly + Newton-Krylov path.  TEST INFRASTRUCTURE ONLY (the test oracle): only
 // tests/, the smoke entry point and bench.py's CPU-baseline leg load it.
 //
 // The code follows the reference's loops and operation order where they fix

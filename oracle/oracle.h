@@ -2,7 +2,7 @@
  * oracle.h -- CPU FP64 restatement of the femSolidmechanics assembly and
  * Newton-Krylov path of chennachaos/mphyfem.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle: tests/, the smoke
  * entry point and the CPU-baseline leg of bench.py may load it; the product
  * library (mphyfem_amd/libmpf.so) never links, loads or calls it.
  *
