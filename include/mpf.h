@@ -48,7 +48,9 @@ typedef enum {
 
 /* Element type ids (npElem). */
 /* MPF_HEX8P0: Hex8 Q1/P0, one pressure per element condensed out of the system (Elem_Solid_3D_Mixed0). */
-enum { MPF_HEX8 = 8, MPF_HEX8P0 = 9, MPF_TET10P1 = 10 };
+/* MPF_TET10: Tet10 displacement element, Elem_Solid_3D_Disp with npElem = 10 and the 4-point rule      */
+/* (Elem_Solid_3D_Disp.cpp:47-51), same node order as MPF_TET10P1.                                      */
+enum { MPF_HEX8 = 8, MPF_HEX8P0 = 9, MPF_TET10P1 = 10, MPF_TET10 = 11 };
 
 /* Neo-Hookean parameters: Matl_NeoHookean (Matl_NeoHookean.cpp:45-53) reads mu = data_deviatoric[0],
  * Utype and Kinv (MaterialBase.cpp:91-115); K = 1/Kinv. */
@@ -88,6 +90,13 @@ mpf_status mpf_elem_hex8_nh_batch(const double* X, const double* u, const int32_
                                   const mpf_matl* m, int32_t first, int32_t count,
                                   double* Ke, double* Re, int32_t* bad, mpf_stream_t stream);
 
+/* Tet10 displacement element: Elem_Solid_3D_Disp::calcStiffnessAndResidual with npElem = 10, nGP = 4 */
+/* (Elem_Solid_3D_Disp.cpp:47-51, 196-374).  conn: [nElem][10]; Ke [count][30][30] row-major,         */
+/* Re [count][30]; bad as for the Hex8 element.                                                       */
+mpf_status mpf_elem_tet10_nh_batch(const double* X, const double* u, const int32_t* conn,
+                                   const mpf_matl* m, int32_t first, int32_t count,
+                                   double* Ke, double* Re, int32_t* bad, mpf_stream_t stream);
+
 /* Hex8 Q1/P0 mixed element with static condensation (Elem_Solid_3D_Mixed0::calcStiffnessAndResidual, */
 /* Elem_Solid_3D_Mixed0.cpp:93-104, over ElementBase::calcStiffnessAndResidualMixed3D with Np = 1).   */
 /* pres_elem: [nElem] element pressures; u_prev as for Tet10/P1.  Kc [count][24][24] = Kuu - Kup     */
@@ -115,8 +124,8 @@ mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const dou
 /* Problem: the femSolidmechanics-shaped driver state on one device (femSolidmechanics.h).       */
 typedef struct mpf_problem mpf_problem;
 
-/* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8, MPF_HEX8P0 or   */
-/* MPF_TET10P1.  MPF_HEX8P0 assembles the condensed element matrices by the coloured scatter only  */
+/* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8, MPF_HEX8P0,      */
+/* MPF_TET10P1 or MPF_TET10.  MPF_HEX8P0 assembles the condensed element matrices by the coloured scatter only  */
 /* (mpf_set_assembly_mode rejects the gather modes), recovers the element pressures in            */
 /* mpf_update (solveForPressure, Elem_Solid_3D_Mixed0.cpp:112-199), and has no batched path       */
 /* (mpf_set_batch rejects it).                                                                     */

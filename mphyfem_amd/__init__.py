@@ -23,6 +23,7 @@ LIB_PATH = _PKG / "libmpf.so"
 HEX8 = 8
 HEX8P0 = 9  # Hex8 Q1/P0: one element pressure, condensed
 TET10P1 = 10
+TET10 = 11  # Tet10 displacement element (Elem_Solid_3D_Disp with npElem = 10)
 KSP_CG, KSP_GMRES, KSP_BAND = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
 ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER, ASSEMBLY_GATHER_DIRECT = 0, 1, 2
@@ -98,6 +99,7 @@ SIGNATURES = [
     ("mpf_device_count", C.c_int32, []),
     ("mpf_neohooke_stress_tangent_batch", C.c_int, [_P, _P, C.POINTER(Matl), C.c_int32, _P, _P, _P]),
     ("mpf_elem_hex8_nh_batch", C.c_int, [_P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32, _P, _P, _P, _P]),
+    ("mpf_elem_tet10_nh_batch", C.c_int, [_P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32, _P, _P, _P, _P]),
     ("mpf_elem_hex8p0_nh_batch", C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32,
                                             _P, _P, _P, _P, _P, _P, _P]),
     ("mpf_elem_tet10p1_nh_batch", C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Matl), C.c_int32, C.c_int32,
@@ -379,6 +381,20 @@ def elem_hex8p0_nh_batch(X, u, u_prev, pres_elem, conn, m: Matl, first: int = 0,
     return Kc, Fc, Kup, Kpp, Fp, int(bad.item())
 
 
+def elem_tet10_nh_batch(X, u, conn, m: Matl, first: int = 0, count: int | None = None):
+    """Elem_Solid_3D_Disp::calcStiffnessAndResidual (Tet10, 4 Gauss points) for a batch of elements on the
+    device: (Ke [count, 30, 30], Re [count, 30], bad)."""
+    import torch
+    count = conn.shape[0] - first if count is None else count
+    Ke = torch.empty((count, 30, 30), dtype=torch.float64, device=X.device)
+    Re = torch.empty((count, 30), dtype=torch.float64, device=X.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=X.device)
+    st = lib().mpf_elem_tet10_nh_batch(X.data_ptr(), u.data_ptr(), conn.data_ptr(), C.byref(m), first, count,
+                                       Ke.data_ptr(), Re.data_ptr(), bad.data_ptr(), _stream())
+    _check(st, "mpf_elem_tet10_nh_batch")
+    return Ke, Re, int(bad.item())
+
+
 def elem_tet10p1_nh_batch(X, u, u_prev, pres, conn, m: Matl, first: int = 0, count: int | None = None):
     """ElementBase::calcStiffnessAndResidualMixed3D (Tet10/P1) for a batch of elements."""
     import torch
@@ -487,16 +503,28 @@ class Problem:
                "mpf_get_pattern")
         return Pattern(rowptr, colidx, nda, idp, color, d, p, nc)
 
+    def _npres(self) -> int:
+        return self.nElem if self.elem_type == HEX8P0 else self.nNode
+
     def set_state(self, disp=None, disp_prev=None, pres=None, pres_prev=None):
-        def c(a):
-            return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
-        a = [c(disp), c(disp_prev), c(pres), c(pres_prev)]
+        """disp/disp_prev: [nNode*3]; pres/pres_prev: [nNode], or [nElem] for HEX8P0 (the C call copies
+        exactly that many doubles, so shorter arrays are refused here)."""
+        def c(a, n, name):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != n:
+                raise ValueError(f"{name} has {a.size} entries, expected {n}")
+            return a
+        n3, npres = 3 * self.nNode, self._npres()
+        a = [c(disp, n3, "disp"), c(disp_prev, n3, "disp_prev"), c(pres, npres, "pres"),
+             c(pres_prev, npres, "pres_prev")]
         _check(lib().mpf_set_state(self._h, *[_dp(x) for x in a]), "mpf_set_state")
 
     def state(self):
         """(disp [nNode*3], pres [nNode], or [nElem] for HEX8P0)."""
         disp = np.zeros(self.nNode * 3)
-        pres = np.zeros(self.nElem if self.elem_type == HEX8P0 else self.nNode)
+        pres = np.zeros(self._npres())
         _check(lib().mpf_get_state(self._h, _dp(disp), _dp(pres)), "mpf_get_state")
         return disp, pres
 
@@ -550,7 +578,9 @@ class Problem:
     def newton_step(self, lam: float, opts: NewtonOpts, guess=None):
         it = C.c_int32()
         rep = NewtonReport()
-        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64).reshape(-1)
+        if g is not None and g.size != 3 * self.nNode:
+            raise ValueError(f"guess has {g.size} entries, expected {3 * self.nNode}")
         st = lib().mpf_newton_step(self._h, lam, C.byref(opts), _dp(g), C.byref(it), C.byref(rep))
         _check(st, "mpf_newton_step")
         return it.value, rep

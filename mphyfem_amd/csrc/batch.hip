@@ -55,7 +55,9 @@ struct BatchWork {
   int32_t* cnt_h = nullptr;  // pinned host mirror
   hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr, g_r0 = nullptr;
   hipGraphExec_t g_band = nullptr;  // direct solve of every mesh (banded LDL^T)
-  double g_lam = 0.0;
+  // the graphs hold these as kernel arguments: any change re-captures them
+  double g_lam = 0.0, g_rtol = -1.0, g_cg_rtol = -1.0;
+  int32_t g_cg_maxit = -1;
   int32_t g_solver = -1;            // linear solver the graphs were captured for
   ~BatchWork() { release_graphs(); if (cnt_h) (void)hipHostFree(cnt_h); }
   void release_graphs() {
@@ -561,6 +563,9 @@ void batch_free(mpf_problem* p) {
 mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                      const double* mu, const double* Kinv) {
   if (p->etype != MPF_HEX8 || nmesh <= 0 || !node_off || !elem_off || !mu || !Kinv) return MPF_ERR_ARG;
+  // the layout, tiles, per-mesh flags, band key and captured graphs are built for one batch: a
+  // second set_batch would leave them sized for the first
+  if (p->batch) return MPF_ERR_STATE;
   if (node_off[0] != 0 || elem_off[0] != 0 || node_off[nmesh] != p->nNode || elem_off[nmesh] != p->nElem)
     return MPF_ERR_ARG;
   for (int m = 0; m < nmesh; ++m)
@@ -582,7 +587,7 @@ mpf_status batch_set(mpf_problem* p, int32_t nmesh, const int32_t* node_off, con
       }
     }
   }
-  if (!p->batch) p->batch = new BatchWork();
+  p->batch = new BatchWork();
   BatchWork& B = *p->batch;
   B.nmesh = nmesh;
   B.node_off.assign(node_off, node_off + nmesh + 1);
@@ -758,7 +763,8 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
   MPF_TRY(reset_mesh_flags(p, B));
   auto head = [&](int f) { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol, !band); };
   auto direct = [&] { return band_enqueue_solve(p, B.mf.ptr + kNA, kMF); };
-  if (G && (B.g_lam != lam || !B.g_asm[0] || B.g_solver != o->solver)) {
+  if (G && (B.g_lam != lam || !B.g_asm[0] || B.g_solver != o->solver || B.g_rtol != o->rtol ||
+            B.g_cg_rtol != o->cg_rtol || B.g_cg_maxit != o->cg_maxit)) {
     for (int f = 0; f < 2; ++f) {
       MPF_TRY(capture(p, &B.g_asm[f], [&] { return head(f); }));
       MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
@@ -773,6 +779,9 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
     }
     MPF_TRY(capture(p, &B.g_r0, [&] { return enqueue_r0(p, B, lam); }));
     B.g_lam = lam;
+    B.g_rtol = o->rtol;
+    B.g_cg_rtol = o->cg_rtol;
+    B.g_cg_maxit = o->cg_maxit;
     B.g_solver = o->solver;
   }
   if (guess) {
@@ -912,7 +921,7 @@ extern "C" {
 
 mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                          const double* mu, const double* Kinv) {
-  if (!p || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 and Tet10/P1
+  if (!p || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 only
   return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
 }
 

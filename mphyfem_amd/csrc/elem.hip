@@ -363,7 +363,9 @@ __global__ void k_hex8p0_pressure(int32_t nElem, const int32_t* __restrict__ con
 }
 
 // ============================================================================================
-// Tet10 / P1 mixed element
+// Tet10 elements: the P1 mixed element (MIXED) and the displacement element (Elem_Solid_3D_Disp
+// with npElem = 10, Elem_Solid_3D_Disp.cpp:47-51), which is the same kernel without the pressure
+// field, the u-p couplings and the pressure rows
 // ============================================================================================
 template <int EPW>
 struct Tet10Smem {
@@ -379,12 +381,16 @@ struct Tet10Smem {
   double dN[4][3][10];      // [gp][r][a]: dN_a/dxi_r at the Gauss points
 };
 
-template <int EPW>
+template <int EPW, bool HAS_S5>
 __device__ __forceinline__ void tet10_uu_block(const Tet10Smem<EPW>& sm, int k, int a, int b, double Kb[9]) {
-  tangent_block<4, false>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 60, &sm.S[k][0][0], 8, Kb);
+  tangent_block<4, HAS_S5>(&sm.G[k][0][a][0], &sm.G[k][0][b][0], 60, &sm.S[k][0][0], 8, Kb);
 }
 
-template <int EPW, bool DENSE>
+// Slots per element of the coloured scatter's slot map.
+template <bool MIXED>
+constexpr int tet10_slots() { return MIXED ? kTet10Slots : 100; }
+
+template <int EPW, bool DENSE, bool MIXED = true>
 __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, int32_t count, double* __restrict__ Kuu,
                                                    double* __restrict__ Kup, double* __restrict__ Kpu,
                                                    double* __restrict__ Kpp, double* __restrict__ Fu,
@@ -412,9 +418,9 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
     for (int c = 0; c < 3; ++c) {
       sm.X[k][a][c] = A.X[3 * nd + c];
       sm.U[k][a][c] = A.u[3 * nd + c];
-      sm.UP[k][a][c] = A.uprev[3 * nd + c];
+      if (MIXED) sm.UP[k][a][c] = A.uprev[3 * nd + c];
     }
-    if (a < 4) sm.Pn[k][a] = A.pres[nd];
+    if (MIXED && a < 4) sm.Pn[k][a] = A.pres[nd];
   }
   __syncthreads();
   {
@@ -425,8 +431,9 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
     bool negJ = false;
     double mu, K;
     elem_material(A, DENSE ? first + (act ? base_idx + k : base_idx) : sc.elems[act ? base_idx + k : base_idx], mu, K);
-    element_point<10, 4, true>(sm.X[k], sm.U[k], sm.UP[k], sm.Pn[k], sm.dN[gp], gp, wq, mu, K, A.Utype,
-                               false, writer, &sm.G[k][gp][0][0], sm.S[k][gp], &sm.R[k][0][0], negJ);
+    element_point<10, 4, MIXED>(sm.X[k], sm.U[k], MIXED ? sm.UP[k] : nullptr, MIXED ? sm.Pn[k] : nullptr, sm.dN[gp],
+                                gp, wq, mu, K, A.Utype, !MIXED && A.consistent != 0, writer, &sm.G[k][gp][0][0],
+                                sm.S[k][gp], &sm.R[k][0][0], negJ);
     if (act && negJ) flag_bad(A.bad, DENSE ? first + base_idx + k : sc.elems[base_idx + k]);
   }
   __syncthreads();
@@ -445,8 +452,8 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
       const bool has2 = item2 < 100;
       const int a2 = has2 ? item2 / 10 : 0, b2 = has2 ? item2 % 10 : 0;
       double Kb[18];
-      tet10_uu_block(sm, k, a1, b1, Kb);
-      tet10_uu_block(sm, k, a2, b2, Kb + 9);
+      tet10_uu_block<EPW, !MIXED>(sm, k, a1, b1, Kb);
+      tet10_uu_block<EPW, !MIXED>(sm, k, a2, b2, Kb + 9);
       if (DENSE) {
         double* Ko = Kuu + (size_t)idx * 900;
 #pragma unroll
@@ -460,20 +467,24 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
         int addr[18];
         const int na1 = A.conn[(size_t)e * 10 + a1], nb1 = A.conn[(size_t)e * 10 + b1];
         const int i1a = sc.nodeinfo[na1], i1b = sc.nodeinfo[nb1];
-        const int s1 = (sc.scatterK && (i1a & 7) && (i1b & 7)) ? sc.slot[(size_t)e * kTet10Slots + a1 * 10 + b1] : -1;
+        const int s1 =
+            (sc.scatterK && (i1a & 7) && (i1b & 7)) ? sc.slot[(size_t)e * tet10_slots<MIXED>() + a1 * 10 + b1] : -1;
         block_slots(s1, i1a & 7, i1b & 7, i1a >> 4, addr);
         int s2 = -1, i2a = 0, i2b = 0;
         if (has2) {
           const int na2 = A.conn[(size_t)e * 10 + a2], nb2 = A.conn[(size_t)e * 10 + b2];
           i2a = sc.nodeinfo[na2];
           i2b = sc.nodeinfo[nb2];
-          s2 = (sc.scatterK && (i2a & 7) && (i2b & 7)) ? sc.slot[(size_t)e * kTet10Slots + a2 * 10 + b2] : -1;
+          s2 = (sc.scatterK && (i2a & 7) && (i2b & 7)) ? sc.slot[(size_t)e * tet10_slots<MIXED>() + a2 * 10 + b2]
+                                                       : -1;
         }
         block_slots(s2, i2a & 7, i2b & 7, i2a >> 4, addr + 9);
         rmw_add<18>(V, addr, Kb);
       }
     }
-    if (lane >= 36 && lane < 46) {
+    if (!MIXED) {
+      // displacement element: no pressure field
+    } else if (lane >= 36 && lane < 46) {
       // Kup(3a+i, q) += (dv g_a,i) Np_q ;  Kpu(q, 3a+j) += (dv Np_q) g_a,j   (ElementBase2.cpp:500-505, 524-533)
       const int a = lane - 36;
       double up[3][4], pu[4][3];
@@ -590,7 +601,7 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
 #pragma unroll
           for (int kk = 0; kk < 3; ++kk) db[kk] = ((mb >> kk) & 1) ? 0.0 : sc.dispApplied[3 * (size_t)nb + kk];
           double Kb[9];
-          tet10_uu_block(sm, k, a, b, Kb);
+          tet10_uu_block<EPW, !MIXED>(sm, k, a, b, Kb);
 #pragma unroll
           for (int i = 0; i < 3; ++i) c[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
         }
@@ -603,7 +614,7 @@ __global__ void __launch_bounds__(64) tet10_kernel(ElemArgs A, int32_t first, in
         rv[i] = sm.R[k][a][i] - c[i];
       }
       rmw_add<3>(sc.rhs, ra, rv);
-    } else if (lane < 14) {
+    } else if (MIXED && lane < 14) {
       const int q = lane - 10;
       const int nq = A.conn[(size_t)e * 10 + q];
       double c = 0.0;
@@ -678,6 +689,24 @@ mpf_status launch_tet10_dense(const ElemArgs& a, int32_t first, int32_t count, d
   ScatterArgs sc{};
   hipLaunchKernelGGL((tet10_kernel<kTetEPW, true>), dim3(grid_for(count, kTetEPW)), dim3(64), 0, s, a, first, count,
                      Kuu, Kup, Kpu, Kpp, Fu, Fp, sc);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status launch_tet10d_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,
+                               hipStream_t s) {
+  if (count <= 0) return MPF_OK;
+  ScatterArgs sc{};
+  hipLaunchKernelGGL((tet10_kernel<kTetEPW, true, false>), dim3(grid_for(count, kTetEPW)), dim3(64), 0, s, a, first,
+                     count, Ke, nullptr, nullptr, nullptr, Re, nullptr, sc);
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status launch_tet10d_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s) {
+  if (sc.count <= 0) return MPF_OK;
+  hipLaunchKernelGGL((tet10_kernel<kTetEPW, false, false>), dim3(grid_for(sc.count, kTetEPW)), dim3(64), 0, s, a, 0,
+                     sc.count, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, sc);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

@@ -36,16 +36,17 @@ constexpr int kMaxAdj = 64;   // elements around one node
 constexpr int kMaxNB = 128;   // nodes coupled to one node (column blocks of its rows)
 
 // Staging record layout (doubles) per element:
-//   Tet10/P1:  G [NGP][NPE][6]  g_a (3), (b g)_a (3)
+//   Tet10, Tet10/P1:  G [NGP][NPE][6]  g_a (3), (b g)_a (3)
 //   Hex8:      J [NGP][9]       the inverse Jacobian (g_a = grad_jinv(J, dN_a/dxi));
 //              B [NGP][6]       b00 b11 b22 b01 b02 b12   (g and b g formed on use: 184 doubles per
 //                               element instead of 448)
-//   S  [NGP][SS]       Hex8: s1..s5 (SS = 5); Tet10/P1: s1..s4, dv, dv0, thetahat, resip (SS = 8)
+//   S  [NGP][SS]       displacement elements: s1..s5 (SS = 5); Tet10/P1: s1..s4, dv, dv0, thetahat, resip
+//                      (SS = 8)
 //   R  [NPE][3]        element residual (no lift)
 // The first SL doubles (G or J, B, S) are what the matrix kernel reads for a block.
 template <int NPE, int NGP, bool MIXED>
 struct Stage {
-  static constexpr bool COMPACT = !MIXED;
+  static constexpr bool COMPACT = NPE == 8 && !MIXED;
   static constexpr int GW = COMPACT ? 3 : 6;      // doubles per node and Gauss point of g data
   static constexpr int SS = MIXED ? 8 : 5;
   static constexpr int G0 = 0;
@@ -251,14 +252,13 @@ __device__ __forceinline__ bool has_rows(int info) {
 //                             column block of n's rows the (s, b) block lands in), s0 = first element of
 //                             column j's run
 //   cnt[n]                  = number of those items (0 for a node without rows)
-template <int NPE>
+template <int NPE, bool MIXED>
 __global__ void __launch_bounds__(64) k_node_items(int32_t nNode, const int32_t* __restrict__ nptr,
                                                    const int32_t* __restrict__ nel, const int32_t* __restrict__ nbptr,
                                                    const int32_t* __restrict__ nb, const int32_t* __restrict__ conn,
                                                    const int32_t* __restrict__ nodeinfo, int32_t* __restrict__ nelA,
                                                    int32_t* __restrict__ keys, int64_t* __restrict__ cnt,
                                                    int32_t* __restrict__ overflow) {
-  constexpr bool MIXED = NPE == 10;
   static_assert(kMaxAdj * NPE <= kItemCap, "item capacity");
   __shared__ int32_t sE[kMaxAdj], sNB[kMaxNB], sK[kItemCap];
   const int lane = threadIdx.x;
@@ -1072,7 +1072,9 @@ __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
 }  // namespace
 
 size_t stage_doubles_per_elem(int etype) {
-  return etype == MPF_HEX8 ? (size_t)Stage<8, 8, false>::TS : (size_t)Stage<10, 4, true>::TS;
+  if (etype == MPF_HEX8) return (size_t)Stage<8, 8, false>::TS;
+  if (etype == MPF_TET10) return (size_t)Stage<10, 4, false>::TS;
+  return (size_t)Stage<10, 4, true>::TS;
 }
 
 static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
@@ -1189,11 +1191,14 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
   MPF_TRY(ovf.alloc(1));
   MPF_HIP(hipMemsetAsync(ovf.ptr, 0, sizeof(int32_t), s));
   if (mixed)
-    hipLaunchKernelGGL(k_node_items<10>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
-                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
+    hipLaunchKernelGGL((k_node_items<10, true>), dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
+                       P.nbptr.ptr, P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
+  else if (npe == 10)
+    hipLaunchKernelGGL((k_node_items<10, false>), dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
+                       P.nbptr.ptr, P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
   else
-    hipLaunchKernelGGL(k_node_items<8>, dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr, P.nbptr.ptr,
-                       P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
+    hipLaunchKernelGGL((k_node_items<8, false>), dim3(nNode), dim3(64), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
+                       P.nbptr.ptr, P.nb.ptr, p->conn.ptr, P.nodeinfo.ptr, P.g_nelA.ptr, keys.ptr, cnt.ptr, ovf.ptr);
   MPF_HIP(hipGetLastError());
   int32_t o = 0;
   MPF_HIP(hipMemcpyAsync(&o, ovf.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1207,7 +1212,7 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
   MPF_TRY(glen.alloc(ngroups + 1));
   MPF_TRY(goff.alloc(ngroups + 1));
   const unsigned ggrid = (unsigned)((ngroups + 255) / 256);
-  if (mixed)
+  if (npe == 10)
     hipLaunchKernelGGL(k_group_len<10>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
                        keys.ptr, cnt.ptr, glen.ptr);
   else
@@ -1234,7 +1239,7 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
   MPF_TRY(P.g_fnode.alloc(total > 0 ? (size_t)total : 64));
   MPF_HIP(hipMemsetAsync(P.g_fitem.ptr, 0xff, (total > 0 ? (size_t)total : 64) * sizeof(int2), s));
   MPF_HIP(hipMemsetAsync(P.g_fnode.ptr, 0xff, (total > 0 ? (size_t)total : 64) * sizeof(int32_t), s));
-  if (mixed)
+  if (npe == 10)
     hipLaunchKernelGGL(k_group_fill<10>, dim3(ggrid), dim3(256), 0, s, ngroups, nNode, P.g_order.ptr, P.nptr.ptr,
                        P.nbptr.ptr, keys.ptr, cnt.ptr, P.g_nelA.ptr, goff.ptr, P.g_fitem.ptr, P.g_fnode.ptr);
   else
@@ -1244,7 +1249,7 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
   P.g_nwaves = total / 64;
 
   // chunk tables of the chunked matrix kernel (Hex8): each group's wavefronts in chunks of <= CW
-  if (!mixed && P.g_nwaves > 0) {
+  if (p->etype == MPF_HEX8 && P.g_nwaves > 0) {
     constexpr int CW = MPF_CHUNK_WAVES, ECAP = MPF_CHUNK_ELEMS;
     static_assert(ECAP == 32, "mpf_set_chunk_stage_cap documents a cap of 32");
     static_assert(CW >= 1 && CW <= 8, "k_chunk_table covers 8 wavefronts");
@@ -1288,7 +1293,7 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
                        P.g_btab.ptr);
   MPF_HIP(hipGetLastError());
   MPF_TRY(P.g_near.alloc(nNode));
-  if (mixed)
+  if (npe == 10)
     hipLaunchKernelGGL(k_near<10>, dim3(grid_for(nNode, 256)), dim3(256), 0, s, nNode, P.nptr.ptr, P.nel.ptr,
                        p->conn.ptr, P.nodeinfo.ptr, P.g_near.ptr);
   else
@@ -1357,7 +1362,13 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   const unsigned flat_grid = (unsigned)(((F.nwaves + 7) / 8) * 8);
   const unsigned chunk_grid = (unsigned)(((F.nchunks + 7) / 8) * 8);
   const bool chunked = p->asm_mode != MPF_ASSEMBLY_GATHER_DIRECT && F.nchunks > 0;
-  if (!mixed) {
+  if (p->etype == MPF_TET10) {
+    hipLaunchKernelGGL((k_stage<10, 4, false, 16, MPF_STAGE_LDS_TET>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A,
+                       nElem, p->GS.ptr);
+    if (flat) hipLaunchKernelGGL((k_flat<10, 4, false>), dim3(flat_grid), dim3(64), 0, s, F);
+    hipLaunchKernelGGL((k_rhs<10, 4, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
+    if (lift) hipLaunchKernelGGL((k_lift<10, 4, false>), dim3(L.nnear), dim3(64), 0, s, L);
+  } else if (!mixed) {
     hipLaunchKernelGGL((k_stage<8, 8, false, 8, MPF_STAGE_LDS_HEX>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A,
                        nElem, p->GS.ptr);
     if (flat && chunked)

@@ -134,10 +134,10 @@ void mpf_mesh_tet10(int32_t n, double L, double jitter, uint64_t seed, double* X
 }
 
 // Nodes of a face of a structured block.  elem_type MPF_HEX8: (nx+1)(ny+1)(nz+1) grid;
-// MPF_TET10P1: (2n+1)^3 grid with nx = ny = nz = n.
+// MPF_TET10P1, MPF_TET10: (2n+1)^3 grid with nx = ny = nz = n.
 int32_t mpf_mesh_face_nodes(int32_t elem_type, int32_t nx, int32_t ny, int32_t nz, int32_t face, int32_t* nodes) {
   int64_t mx = nx + 1, my = ny + 1, mz = nz + 1;
-  if (elem_type == MPF_TET10P1) { mx = 2 * (int64_t)nx + 1; my = 2 * (int64_t)ny + 1; mz = 2 * (int64_t)nz + 1; }
+  if (elem_type == MPF_TET10P1 || elem_type == MPF_TET10) { mx = 2 * (int64_t)nx + 1; my = 2 * (int64_t)ny + 1; mz = 2 * (int64_t)nz + 1; }
   const int axis = face / 2;
   const bool hi = face % 2;
   int32_t cnt = 0;

@@ -115,10 +115,15 @@ mpf_status launch_hex8p0_pressure(int32_t nElem, const int32_t* conn, const int3
                                   const double* dispApplied, const double* Kup, const double* Kpp, const double* Fp,
                                   double* pres, hipStream_t s);
 mpf_status launch_tet10_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
+// Tet10 displacement element (dense operator, coloured scatter with a 100-slot map per element)
+mpf_status launch_tet10d_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,
+                               hipStream_t s);
+mpf_status launch_tet10d_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
 
 // Slot-map sizes per element.
 constexpr int kHex8Slots = 64;                    // (a,b) node pairs
 constexpr int kTet10Slots = 100 + 40 + 40 + 16;   // uu (a,b), up (a,q), pu (q,b), pp (q1,q2)
+constexpr int kTet10DSlots = 100;                 // Tet10 displacement element: uu (a,b)
 
 // ---------------------------------------------------------------------------------------------
 // Problem state (the femSolidmechanics-shaped driver state on one device).

@@ -279,15 +279,17 @@ __device__ __forceinline__ int slot_of(const int32_t* __restrict__ rowptr, const
 }
 
 // Hex8 slot map: slot[e][a*8+b] = CSR position of (first free row of a, first free col of b).
-__global__ void k_slots_hex8(const int32_t* __restrict__ conn, int32_t nElem, const int32_t* __restrict__ nda,
+// Displacement-element slot map (NPE^2 per element): [a*NPE+b] = (first free row of a, first free col of b)
+template <int NPE>
+__global__ void k_slots_disp(const int32_t* __restrict__ conn, int32_t nElem, const int32_t* __restrict__ nda,
                              const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                              int32_t* __restrict__ slot) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nElem * 64) return;
-  const int64_t e = t / 64;
-  const int a = (int)(t % 64) / 8, b = (int)(t % 8);
-  const int ra = first_free(nda, conn[e * 8 + a]);
-  const int cb = first_free(nda, conn[e * 8 + b]);
+  if (t >= (int64_t)nElem * NPE * NPE) return;
+  const int64_t e = t / (NPE * NPE);
+  const int a = (int)(t % (NPE * NPE)) / NPE, b = (int)(t % NPE);
+  const int ra = first_free(nda, conn[e * NPE + a]);
+  const int cb = first_free(nda, conn[e * NPE + b]);
   slot[t] = slot_of(rowptr, colidx, ra, cb);
 }
 
@@ -523,10 +525,14 @@ mpf_status build_pattern(mpf_problem* p) {
                      mixed ? P.idp.ptr : nullptr, P.dispDOF, P.rowptr.ptr, P.colidx.ptr);
 
   // ---- per-element slot map ----
-  if (!mixed) {
+  if (p->etype == MPF_HEX8) {
     MPF_TRY(P.slot.alloc((size_t)p->nElem * kHex8Slots));
-    hipLaunchKernelGGL(k_slots_hex8, dim3(grid_for((int64_t)p->nElem * 64, B)), dim3(B), 0, s, p->conn.ptr, p->nElem,
-                       P.nda.ptr, P.rowptr.ptr, P.colidx.ptr, P.slot.ptr);
+    hipLaunchKernelGGL(k_slots_disp<8>, dim3(grid_for((int64_t)p->nElem * kHex8Slots, B)), dim3(B), 0, s,
+                       p->conn.ptr, p->nElem, P.nda.ptr, P.rowptr.ptr, P.colidx.ptr, P.slot.ptr);
+  } else if (p->etype == MPF_TET10) {
+    MPF_TRY(P.slot.alloc((size_t)p->nElem * kTet10DSlots));
+    hipLaunchKernelGGL(k_slots_disp<10>, dim3(grid_for((int64_t)p->nElem * kTet10DSlots, B)), dim3(B), 0, s,
+                       p->conn.ptr, p->nElem, P.nda.ptr, P.rowptr.ptr, P.colidx.ptr, P.slot.ptr);
   } else {
     MPF_TRY(P.slot.alloc((size_t)p->nElem * kTet10Slots));
     hipLaunchKernelGGL(k_slots_tet10, dim3(grid_for((int64_t)p->nElem * kTet10Slots, B)), dim3(B), 0, s,

@@ -231,6 +231,7 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
     sc.count = P.color_off[c + 1] - P.color_off[c];
     if (p->p0) MPF_TRY(launch_hex8p0_scatter(A, sc, p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, s));
     else if (p->etype == MPF_HEX8) MPF_TRY(launch_hex8_scatter(A, sc, s));
+    else if (p->etype == MPF_TET10) MPF_TRY(launch_tet10d_scatter(A, sc, s));
     else MPF_TRY(launch_tet10_scatter(A, sc, s));
   }
   MPF_HIP(hipGetLastError());
@@ -414,6 +415,16 @@ mpf_status mpf_elem_hex8_nh_batch(const double* X, const double* u, const int32_
   return launch_hex8_dense(a, first, count, Ke, Re, (hipStream_t)stream);
 }
 
+mpf_status mpf_elem_tet10_nh_batch(const double* X, const double* u, const int32_t* conn, const mpf_matl* m,
+                                   int32_t first, int32_t count, double* Ke, double* Re, int32_t* bad,
+                                   mpf_stream_t stream) {
+  if (!X || !u || !conn || !m || !Ke || !Re || first < 0 || count < 0 || m->Kinv <= 0.0) return MPF_ERR_ARG;
+  ElemArgs a{};
+  a.X = X; a.u = u; a.conn = conn; a.mu = m->mu; a.Kinv = m->Kinv; a.Utype = m->Utype; a.bad = bad;
+  a.consistent = m->tangent == MPF_TANGENT_CONSISTENT;
+  return launch_tet10d_dense(a, first, count, Ke, Re, (hipStream_t)stream);
+}
+
 mpf_status mpf_elem_hex8p0_nh_batch(const double* X, const double* u, const double* u_prev, const double* pres_elem,
                                     const int32_t* conn, const mpf_matl* m, int32_t first, int32_t count, double* Kc,
                                     double* Fc, double* Kup, double* Kpp, double* Fp, int32_t* bad,
@@ -443,10 +454,11 @@ mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const dou
 mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNode, int32_t nElem, const double* X,
                               const int32_t* conn, const mpf_matl* m) {
   if (!out || !X || !conn || !m || nNode <= 0 || nElem <= 0) return MPF_ERR_ARG;
-  if (elem_type != MPF_HEX8 && elem_type != MPF_HEX8P0 && elem_type != MPF_TET10P1) return MPF_ERR_ARG;
+  if (elem_type != MPF_HEX8 && elem_type != MPF_HEX8P0 && elem_type != MPF_TET10P1 && elem_type != MPF_TET10)
+    return MPF_ERR_ARG;
   if (m->Kinv <= 0.0) return MPF_ERR_ARG;
   const bool p0 = elem_type == MPF_HEX8P0;
-  const int npe = elem_type == MPF_TET10P1 ? 10 : 8;
+  const int npe = (elem_type == MPF_TET10P1 || elem_type == MPF_TET10) ? 10 : 8;
   for (int64_t i = 0; i < (int64_t)nElem * npe; ++i)
     if (conn[i] < 0 || conn[i] >= nNode) return MPF_ERR_ARG;
   MPF_TRY(check_device());
@@ -460,7 +472,7 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   p->nElem = nElem;
   p->npres = p0 ? nElem : nNode;
   p->m = *m;
-  p->m.mixed = (elem_type == MPF_HEX8) ? 0 : 1;
+  p->m.mixed = (elem_type == MPF_HEX8 || elem_type == MPF_TET10) ? 0 : 1;
   if (p0) p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;  // condensed element matrices go through the slot map
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     delete p;

@@ -996,7 +996,8 @@ orc_problem* orc_problem_create(int elem_type, int nNode, int nElem, const doubl
   auto* p = new orc_problem();
   p->p0 = (elem_type == 9);
   p->etype = p->p0 ? 8 : elem_type;  // Q1/P0 shares the Hex8 numbering and pattern
-  p->npe = (elem_type == 10) ? 10 : 8;
+  // 11: Tet10 displacement element (Elem_Solid_3D_Disp with npElem = 10, Elem_Solid_3D_Disp.cpp:47-51)
+  p->npe = (elem_type == 10 || elem_type == 11) ? 10 : 8;
   p->mixed = (elem_type == 10);
   if (p->p0) {
     p->qKpu.assign((size_t)nElem * 24, 0.0);

@@ -67,7 +67,8 @@ int  orc_elem_tet10p1(const double* X, const double* u, const double* uprev, con
 /* ---- problem: numbering, pattern, colouring, assembly, Newton ---- */
 typedef struct orc_problem orc_problem;
 
-/* elem_type 8 = Hex8 displacement, 10 = Tet10/P1 mixed.  X [nNode][3], conn [nElem][npe]. */
+/* elem_type 8 = Hex8 displacement, 9 = Hex8 Q1/P0, 10 = Tet10/P1 mixed, 11 = Tet10 displacement.
+ * X [nNode][3], conn [nElem][npe]. */
 orc_problem* orc_problem_create(int elem_type, int nNode, int nElem, const double* X, const int32_t* conn,
                                 double mu, double Kinv, int Utype);
 void orc_problem_destroy(orc_problem* p);

@@ -9,6 +9,9 @@ come from the mathematics, computed without the oracle:
     leaves out (utilitiesmaterialHyperelastic.cpp:280-285), so K_reference = K_consistent - K_volgeo.
   * tet10p1_element.npz -- the same for a Tet10/P1 mixed element: Fu, Fp from sigma(p) and
     Fp = -int Np (J - 1 - p/K) dV0, and the full 34x34 tangent by central differences.
+  * tet10_element.npz -- a jittered, deformed Tet10 displacement element (4-point rule): R_e from
+    the closed-form Cauchy stress (checked against the energy FD), K_consistent = -dR/du by central
+    differences and K_volgeo, as for the Hex8 element.
   * patterns.npz -- CSR rows of tiny meshes by brute-force dense coupling (every (i, j) pair of
     free dofs sharing an element, plus the u-p couplings), for the reference's numbering.
 Run:  python tests/golden/make_golden.py
@@ -124,6 +127,35 @@ def tet10_N_dN(xi):
     return N, dN, l
 
 
+def tet10_residual(X, u, mu, K, sigma=cauchy_closed):
+    """Tet10 displacement element: R_a,i = -sum_gp w det(dx/dxi) sum_m g_a,m sigma_i,m."""
+    R = np.zeros((10, 3))
+    x = X + u
+    for xi in TET_GP:
+        _, d, _ = tet10_N_dN(xi)
+        gX = np.linalg.solve(d @ X, d)
+        F = np.eye(3) + u.T @ gX.T
+        Bc = d @ x
+        g = np.linalg.solve(Bc, d)
+        dv = TET_W * np.linalg.det(Bc)
+        R -= dv * (g.T @ sigma(F, mu, K))
+    return R.ravel()
+
+
+def tet10_volgeo(X, u, mu, K):
+    Kg = np.zeros((30, 30))
+    x = X + u
+    for xi in TET_GP:
+        _, d, _ = tet10_N_dN(xi)
+        gX = np.linalg.solve(d @ X, d)
+        J = np.linalg.det(np.eye(3) + u.T @ gX.T)
+        Bc = d @ x
+        g = np.linalg.solve(Bc, d)
+        dv = TET_W * np.linalg.det(Bc)
+        Kg += K * (J - 1.0) * dv * np.kron(g.T @ g, np.eye(3))
+    return Kg
+
+
 def tet10p1_residual(X, u, up, p, mu, K):
     Fu = np.zeros((10, 3))
     Fp = np.zeros(4)
@@ -209,6 +241,18 @@ def main():
     z = np.concatenate([ut.ravel(), pt])
     K34 = fd_jac(lambda zz: tet10p1_residual(Xt, zz[:30].reshape(10, 3), upt, zz[30:], muT, KT), z, 1e-7)
     np.savez(HERE / "tet10p1_element.npz", X=Xt, u=ut, uprev=upt, pres=pt, mu=muT, Kinv=1.0 / KT, R=R34, K=K34)
+    # ---- Tet10 displacement element (own generator: the fixtures above keep their draws) ----
+    rng2 = np.random.default_rng(20261203)
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.25
+    Xd = np.vstack([V, [(V[a] + V[b]) / 2 for a, b in TET_EDGES]]) + rng2.uniform(-0.005, 0.005, (10, 3))
+    ud = rng2.uniform(-0.007, 0.007, (10, 3))
+    muD, KD = 384.6153846153846, 1.0 / 1.2e-3
+    Rd = tet10_residual(Xd, ud, muD, KD)
+    Rd_energy = tet10_residual(Xd, ud, muD, KD, sigma=cauchy_fd)
+    assert np.abs(Rd - Rd_energy).max() < 1e-6 * np.abs(Rd).max(), "closed-form stress disagrees with energy FD"
+    Kd = fd_jac(lambda z: tet10_residual(Xd, z.reshape(10, 3), muD, KD), ud.ravel(), 1e-7)
+    Kgd = tet10_volgeo(Xd, ud, muD, KD)
+    np.savez(HERE / "tet10_element.npz", X=Xd, u=ud, mu=muD, Kinv=1.2e-3, Re=Rd, Ke_consistent=Kd, Ke_volgeo=Kgd)
     # ---- patterns ----
     out = {}
     for (nx, ny, nz) in [(1, 1, 1), (2, 2, 2), (3, 2, 1)]:

@@ -153,3 +153,33 @@ def test_batch_rejects_shared_nodes(mpf, gpu):
     bad_off = np.array([0, 20, B.node_off[-1]], dtype=np.int32)  # splits the first mesh's nodes
     with pytest.raises(mpf.MpfError):
         P.set_batch(bad_off, B.elem_off, [1.0, 1.0], [1e-3, 1e-3])
+
+
+def test_set_batch_twice_is_refused(mpf, gpu):
+    """The batch layout, tiles, per-mesh flags and captured graphs are built for one batch: a second
+    set_batch on the same problem is refused before anything is reallocated, and the problem still
+    solves the first batch."""
+    B, P = _batch(mpf)
+    mu = E / (2.0 * (1.0 + NU))
+    Kinv = 3.0 * (1.0 - 2.0 * NU) / E
+    with pytest.raises(mpf.MpfError):
+        P.set_batch(B.node_off, B.elem_off, mu, Kinv)
+    _, rel, rep = P.batch_newton_solve(nsteps=1, dt=1.0, rtol=1e-9)
+    assert rep.steps_done == 1 and np.all(rel < 1e-9)
+
+
+def test_graph_recaptured_when_tolerances_change(mpf, gpu):
+    """Two load steps at the same load factor with different Newton and CG tolerances: the graph
+    replay (tolerances are kernel arguments frozen at capture) matches direct launches bitwise."""
+    runs = []
+    for use_graph in (True, False):
+        _, P = _batch(mpf)
+        out = []
+        for rtol, cg_rtol, cg_maxit in ((1e-6, 1e-8, 3000), (1e-11, 1e-13, 20000)):
+            it, rel, _ = P.batch_newton_step(0.5, itmax=25, rtol=rtol, cg_rtol=cg_rtol, cg_maxit=cg_maxit,
+                                             use_graph=use_graph, chunk=8)
+            out.append((it.copy(), rel.copy(), P.state()[0]))
+        runs.append(out)
+    for (ig, rg, ug), (ie, re_, ue) in zip(*runs):
+        assert np.array_equal(ig, ie) and np.array_equal(rg, re_) and np.array_equal(ug, ue)
+    assert np.all(runs[0][1][1] < 1e-11)

@@ -13,6 +13,8 @@ operand) would fail:
   Hex8 element        golden fixture (independent numpy R_e; K = -dR/du by FD; K_ref = K_FD - K_volgeo),
                       rigid modes at F = I, symmetry, homogeneous-deformation patch test
   Tet10/P1 element    golden fixture (independent numpy residual; full 34x34 tangent by FD)
+  Tet10 element       golden fixture (independent numpy residual; K = -dR/du by FD, both tangent
+                      modes), rigid modes, symmetry
   numbering/pattern   brute-force dense coupling on tiny meshes (bit-exact), nnz of the
                       CPU-runnable 10^3 workload = 231 601, symmetric and sorted rows
   colouring           validity (no shared node within a colour), determinism, greedy order
@@ -24,6 +26,12 @@ operand) would fail:
   Q1/P0 driver        ends at the uncondensed mixed equilibrium (Fu = 0, Fp = 0 per element),
                       relieves volumetric locking as nu -> 1/2; the oracle's Q1/P0 problem
                       reproduces it
+  Quad4 traction      closed forms on a flat face: v0 A / 4 per node along n = dx/deta x dx/dxi (the
+                      order of the face's nodes fixes its sign), the unnormalised tangent term
+                      v1 (h/2) A / 4 per node, zero net force and moment on a closed surface
+  Dirichlet lift      the first Newton iterate after the lift is the linearised solution: its error
+                      to the converged displacement is O(delta^2) for a prescribed stretch delta,
+                      while without the lift it stays O(delta)
 """
 from __future__ import annotations
 
@@ -210,20 +218,45 @@ def test_pattern_bruteforce(orc, key):
     assert np.array_equal(nda, g[key + "_nda"])
 
 
-def test_pattern_tet10_bruteforce(orc, mpf):
+@pytest.mark.parametrize("etype", [10, 11], ids=["tet10p1", "tet10"])
+def test_pattern_tet10_bruteforce(orc, mpf, etype):
     from tests.golden.make_golden import brute_pattern
     X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=2)
     bot = mpf.face_nodes(mpf.TET10P1, 2, 2, 2, 4)
     top = mpf.face_nodes(mpf.TET10P1, 2, 2, 2, 5)
-    P = orc.Problem(10, X, conn, 1.0, 1e-3, 3)
+    P = orc.Problem(etype, X, conn, 1.0, 1e-3, 3)
     P.add_dirichlet(bot, -1)
     P.add_dirichlet(top, 2, -0.05)
     P.prepare_matrix_pattern()
     rowptr, colidx, nda, idp = P.pattern()
     fixed = {3 * int(v) + d for v in bot for d in range(3)} | {3 * int(v) + 2 for v in top}
-    rp, ci, nd, ip = brute_pattern(len(X), conn, fixed, True)
+    rp, ci, nd, ip = brute_pattern(len(X), conn, fixed, etype == 10)
     assert np.array_equal(rowptr, rp) and np.array_equal(colidx, ci)
-    assert np.array_equal(nda, nd) and np.array_equal(idp, ip)
+    assert np.array_equal(nda, nd)
+    if etype == 10:
+        assert np.array_equal(idp, ip)
+    else:
+        assert P.sizes()[1] == 0  # no pressure dofs
+
+
+def test_tet10_element_golden(orc):
+    """Tet10 displacement element (npElem 10, 4-point rule): R_e against the independent numpy
+    residual, K_e against its FD Jacobian in both tangent modes, symmetry, rigid-body modes."""
+    g = np.load(GOLD / "tet10_element.npz")
+    X, u, mu, Kinv = g["X"], g["u"], float(g["mu"]), float(g["Kinv"])
+    Ke, Re, st = orc.elem_disp(X, u, mu, Kinv, 3)
+    assert st == 0 and Ke.shape == (30, 30)
+    assert normwise_rel(Re, g["Re"]) < 1e-12
+    assert normwise_rel(Ke, g["Ke_consistent"] - g["Ke_volgeo"]) < 1e-7
+    Kc, Rc, _ = orc.elem_disp_consistent(X, u, mu, Kinv, 3)
+    assert normwise_rel(Kc, g["Ke_consistent"]) < 1e-7
+    assert np.array_equal(Rc, Re) and normwise_rel(Ke, Ke.T) < 1e-14
+    assert normwise_rel(Kc, Ke) > 1e-4
+    K0, R0, _ = orc.elem_disp(X, np.zeros((10, 3)), 100.0, 1e-2, 3)
+    assert np.abs(R0).max() < 1e-12
+    modes = [np.tile(np.eye(3)[d], 10) for d in range(3)] + [np.cross(w, X).ravel() for w in np.eye(3)]
+    for r in modes:
+        assert np.abs(K0 @ r).max() < 1e-10 * np.abs(K0).max()
 
 
 def test_pattern_cfg1_counts(orc):
@@ -423,7 +456,7 @@ def test_q1p0_newton_driver_reaches_mixed_equilibrium(orc):
     bcs = hex_stretch_bcs(n)
     u, p, iters = q1p0_newton(orc, X, conn, bcs, mu, Kinv, nsteps=2, dt=0.5, rtol=1e-11)
     assert max(iters) <= 6, iters
-    assert np.allclose(u[3 * bcs[1][0]], 0.1, atol=1e-14)
+    assert np.allclose(u[3 * bcs[1][0]], 0.1, atol=1e-13)
     fixed = set((3 * np.asarray(bcs[0][0])[:, None] + np.arange(3)).ravel().tolist()) | set((3 * bcs[1][0]).tolist())
     Fu_all = np.zeros_like(u)
     fp_max = 0.0
@@ -492,3 +525,118 @@ def test_q1p0_oracle_problem_matches_driver(orc):
     u, p, its = q1p0_newton(orc, X, conn, bcs, 1.0, 1e-3, 2, 0.5, rtol=1e-10)
     assert list(it) == its
     assert normwise_rel(uq, u) < 1e-12 and normwise_rel(pq, p) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------------
+# Quad4 dead-load traction (femSolidmechanics1.cpp:1026-1070; BasisFunctionsLagrange.cpp:1688-1732)
+# and the first-iteration Dirichlet lift (ElementBase.cpp:567-594), pinned by closed forms.
+# ---------------------------------------------------------------------------------------------
+def _cube_element(h):
+    """One Hex8 element [0,h]^3 in the reference's node order."""
+    X = h * np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    return X, np.arange(8, dtype=np.int32)[None, :]
+
+
+def test_traction_flat_face_normal_load_closed_form(orc):
+    """A flat h x h face under v0 alone: every node receives v0 A / 4 along n = dx/deta x dx/dxi.
+    For the top face listed 4,5,6,7 (counter-clockwise seen from +z) dx/dxi = +x h/2 and
+    dx/deta = +y h/2, so n = y x x = -z; listed 4,7,6,5 it is +z.  A normal taken as du x dv
+    instead, or left unnormalised, fails one of the two."""
+    from tests.cases import oracle_fext
+    h, v0 = 0.7, 3.5
+    X, conn = _cube_element(h)
+    A = h * h
+    for quad, nz in (([4, 5, 6, 7], -1.0), ([4, 7, 6, 5], 1.0)):
+        f = oracle_fext(orc, X, conn, [], np.array([quad], dtype=np.int32), v0, 0.0).reshape(-1, 3)
+        # the bottom nodes carry only the rounding-level internal force of the stress-free state
+        assert np.abs(f[:4]).max() < 1e-13
+        np.testing.assert_allclose(f[4:], np.tile([0.0, 0.0, v0 * A / 4 * nz], (4, 1)), rtol=0, atol=1e-13)
+
+
+def test_traction_tangent_term_is_unnormalised(orc):
+    """The v1 term uses tangent1 = dx/dxi without normalising it: on an h x h face it adds
+    v1 (h/2) A e_xi in total, v1 (h/2) A / 4 per node; a unit tangent would give v1 A."""
+    from tests.cases import oracle_fext
+    h, v1 = 0.6, -2.0
+    X, conn = _cube_element(h)
+    A = h * h
+    f = oracle_fext(orc, X, conn, [], np.array([[4, 5, 6, 7]], dtype=np.int32), 0.0, v1).reshape(-1, 3)
+    np.testing.assert_allclose(f[4:], np.tile([v1 * (h / 2) * A / 4, 0.0, 0.0], (4, 1)), rtol=0, atol=1e-13)
+    # dx/dxi runs along +y when the same face is listed 4,7,6,5
+    f = oracle_fext(orc, X, conn, [], np.array([[4, 7, 6, 5]], dtype=np.int32), 0.0, v1).reshape(-1, 3)
+    np.testing.assert_allclose(f[4:], np.tile([0.0, v1 * (h / 2) * A / 4, 0.0], (4, 1)), rtol=0, atol=1e-13)
+    # the two terms superpose
+    f0 = oracle_fext(orc, X, conn, [], np.array([[4, 5, 6, 7]], dtype=np.int32), 1.25, 0.0)
+    f1 = oracle_fext(orc, X, conn, [], np.array([[4, 5, 6, 7]], dtype=np.int32), 0.0, v1)
+    f01 = oracle_fext(orc, X, conn, [], np.array([[4, 5, 6, 7]], dtype=np.int32), 1.25, v1)
+    np.testing.assert_allclose(f01, f0 + f1, rtol=0, atol=1e-13)
+
+
+def test_traction_closed_surface_zero_force_and_moment(mpf, orc):
+    """Uniform normal traction on all six faces of a jittered block (boundary faces planar, their
+    quads distorted): with outward normals every face's force is v0 times its area vector, and over
+    the closed surface the net force and the net moment about the origin vanish (2x2 Gauss points
+    integrate x times the bilinear surface Jacobian exactly on planar quads)."""
+    from tests.cases import oracle_fext
+    nx, ny, nz = 3, 2, 4
+    L = (1.2, 0.8, 1.6)
+    X, conn = mpf.mesh_hex(nx, ny, nz, L=L, jitter=0.15, seed=7)
+    v0 = 2.5
+    total = np.zeros((len(X), 3))
+    area = {0: L[1] * L[2], 1: L[1] * L[2], 2: L[0] * L[2], 3: L[0] * L[2], 4: L[0] * L[1], 5: L[0] * L[1]}
+    for face in range(6):
+        q = mpf.hex_face_quads(nx, ny, nz, face)
+        f = oracle_fext(orc, X, conn, [], q, v0, 0.0).reshape(-1, 3)
+        n = np.zeros(3)
+        n[face // 2] = 1.0 if face % 2 else -1.0
+        np.testing.assert_allclose(f.sum(0), v0 * area[face] * n, rtol=0, atol=1e-13)
+        total += f
+    assert np.abs(total.sum(0)).max() < 1e-13
+    moment = np.cross(X, total).sum(0)
+    assert np.abs(moment).max() < 1e-13
+
+
+def _first_iterate_error(orc, mpf, delta, lift=True):
+    """Relative distance between the first Newton iterate of one load step (from u = 0, with the
+    step's prescribed increment lifted into the right-hand side when lift) and the converged u."""
+    from tests.cases import apply_bcs
+    n = 3
+    X, conn = cube_hex(n, jitter=0.05, seed=11)
+    m = mpf.matl_from_E_nu(1e3, 0.3)
+    bcs = [(mpf.face_nodes(mpf.HEX8, n, n, n, 0), -1, None), (mpf.face_nodes(mpf.HEX8, n, n, n, 1), 0, delta)]
+    O = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
+    O.set_tangent(True)
+    apply_bcs(O, bcs)
+    O.prepare_matrix_pattern()
+    O.set_boundary_conditions(1.0)
+    vals, rhs, st = O.assemble(lift, 1.0)
+    assert st == 0
+    du, st = O.direct_solve(vals, rhs)
+    assert st == 0
+    _, _, nda, _ = O.pattern()
+    d = nda.ravel()
+    u1 = np.zeros(3 * len(X))
+    u1[d >= 0] = du[d[d >= 0]]
+    right = mpf.face_nodes(mpf.HEX8, n, n, n, 1)
+    u1[3 * right] = delta  # prescribed dofs take their increment in the first update
+    C = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
+    C.set_tangent(True)
+    apply_bcs(C, bcs)
+    C.prepare_matrix_pattern()
+    st, _, _ = C.newton(1, 1.0, solver=0, rtol=1e-13, itmax=30)
+    assert st == 0
+    uc = C.state()[0]
+    return np.linalg.norm(u1 - uc) / np.linalg.norm(uc)
+
+
+def test_dirichlet_lift_first_iterate_is_second_order(orc, mpf):
+    """The lift makes the first Newton iterate the linearised solution: its distance to the converged
+    displacement is O(delta^2) for a prescribed stretch delta, so halving delta quarters it.  Without
+    the lift (or with its sign flipped) the free dofs do not move in the first iteration and the
+    error stays O(1) relative, i.e. O(delta) absolute."""
+    e = [_first_iterate_error(orc, mpf, d) for d in (0.02, 0.01, 0.005)]
+    assert e[0] < 0.05
+    r1, r2 = e[0] / e[1], e[1] / e[2]
+    assert 1.8 < r1 < 2.2 and 1.8 < r2 < 2.2, e  # relative error O(delta): absolute O(delta^2)
+    e_nolift = [_first_iterate_error(orc, mpf, d, lift=False) for d in (0.02, 0.01)]
+    assert min(e_nolift) > 0.2 and abs(e_nolift[0] / e_nolift[1] - 1.0) < 0.1, e_nolift
