@@ -169,13 +169,16 @@ def test_set_batch_twice_is_refused(mpf, gpu):
 
 
 def test_graph_recaptured_when_tolerances_change(mpf, gpu):
-    """Two load steps at the same load factor with different Newton and CG tolerances: the graph
-    replay (tolerances are kernel arguments frozen at capture) matches direct launches bitwise."""
+    """Two load steps at the same load factor with different Newton and CG tolerances, each from the
+    undeformed state (so both start from the same residual): the graph replay (tolerances are
+    kernel arguments frozen at capture) matches direct launches bitwise."""
     runs = []
     for use_graph in (True, False):
         _, P = _batch(mpf)
+        zero = np.zeros(3 * P.nNode)
         out = []
         for rtol, cg_rtol, cg_maxit in ((1e-6, 1e-8, 3000), (1e-11, 1e-13, 20000)):
+            P.set_state(disp=zero, disp_prev=zero)
             it, rel, _ = P.batch_newton_step(0.5, itmax=25, rtol=rtol, cg_rtol=cg_rtol, cg_maxit=cg_maxit,
                                              use_graph=use_graph, chunk=8)
             out.append((it.copy(), rel.copy(), P.state()[0]))
