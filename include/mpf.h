@@ -11,8 +11,8 @@
  *                                                              Elem_Solid_3D_Mixed1.cpp:66-70)
  *   - sigma[9] and C[81] use the column-major index map (0,0),(1,0),(2,0),(0,1),...
  *                                                             (utilitiesmaterialHyperelastic.cpp:191)
- *   - dense matrices crossing this ABI are ROW-major (Eigen's default is column-major:
- *     the C++ wrapper in mpf_host.hpp transposes).
+ *   - dense matrices crossing this ABI are ROW-major (Eigen's default is column-major: the
+ *     C++ layer in mpf_host.hpp copies element by element through operator()(i, j)).
  *   - global numbering: free displacement dofs node-major skipping prescribed dofs, then
  *     pressure dofs at offset dispDOF in increasing node id; CSR rows hold sorted unique
  *     columns                                                 (femSolidmechanics2.cpp:121-478)
@@ -363,6 +363,50 @@ mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const
 /* assemblies; the problem's state is restored).  n_learned: meshes that keep the correction.          */
 mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, const double* uk, const double* ukm1,
                                   double rho, double* guess, int32_t* n_learned);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Linear system over a caller-supplied CSR pattern, filled from caller-computed element         */
+/* matrices: the SolverPetsc operator (SolverPetsc.h:16-96) for a driver that keeps its own      */
+/* element loop.  The handle is an mpf_problem without a mesh; mpf_solve (CG / GMRES),            */
+/* mpf_get_solution, mpf_get_system, mpf_rhs_norm, mpf_system_device_ptrs and                      */
+/* mpf_problem_destroy apply to it, the mesh calls return MPF_ERR_ARG.                             */
+/* SolverPetsc::initialise (SolverPetsc.cpp:24-114) with the pattern prepareMatrixPattern inserts */
+/* (femSolidmechanics2.cpp:294-446).  Host arrays: rowptr [n+1], colidx [nnz] sorted unique per   */
+/* row, n = n_disp + n_pres (pressure rows after the displacement rows, femSolidmechanics2.cpp:175);*/
+/* nda: optional NodeDofArray [nNode][3] (-1 = prescribed; a node's free dofs consecutive) for      */
+/* MPF_PC_BJACOBI3, else pass nNode = 0 and nda = NULL.  Values and rhs start at zero.              */
+mpf_status mpf_system_create(mpf_problem** out, int32_t n_disp, int32_t n_pres, const int32_t* rowptr,
+                             const int32_t* colidx, int32_t nNode, const int32_t* nda);
+/* SolverPetsc::zeroMtx (SolverPetsc.cpp:145-163): values and rhs to zero. */
+mpf_status mpf_system_zero(mpf_problem* p);
+/* Dof map of a batch of `count` element matrices of nrow x ncol (device arrays rows [count][nrow],   */
+/* cols [count][ncol]; -1 = skip the row / column, as assembleMatrixAndVector2field's "!= -1" tests, */
+/* SolverPetsc.cpp:434-502; r_off / c_off are added to the indices that are not -1).  Builds once   */
+/* the sorted CSR-slot lists the adds reuse; *map_id names it.  An entry outside the pattern is      */
+/* MPF_ERR_ARG (PETSc would insert it; the pattern here is fixed).  count*nrow*ncol < 2^31.          */
+mpf_status mpf_system_map_elements(mpf_problem* p, int32_t count, int32_t nrow, int32_t ncol, const int32_t* rows,
+                                   const int32_t* cols, int32_t r_off, int32_t c_off, int32_t* map_id);
+/* SolverPetsc::assembleMatrixAndVector(r1, c1, row, col, K, F) (SolverPetsc.cpp:357-378) for every  */
+/* element of a map: K [count][nrow][ncol] row-major and F [count][nrow], device; either may be NULL. */
+/* Every CSR value and rhs entry receives its contributions in element order, as ADD_VALUES one     */
+/* element after another: equal bit for bit to the sequential host loop.  Marks the system           */
+/* assembled (currentStatus = ASSEMBLY_OK, femSolidmechanics2.cpp:1115).                             */
+mpf_status mpf_system_add_elements(mpf_problem* p, int32_t map_id, const double* K, const double* F);
+/* rhs[i] += f[i] for all rows (device f [n]): Fext added to the residual (femSolidmechanics2.cpp:785-789). */
+mpf_status mpf_system_add_rhs(mpf_problem* p, const double* f);
+/* ElementBase::applyDirichletBCs (ElementBase.cpp:567-594) for `count` element matrices of       */
+/* nsize x nsize (device): F_e(jj) -= K_e(jj, ii) * dispApplied[gdof(ii)] for every ii with        */
+/* forAssy(ii) == -1 and jj with forAssy(jj) != -1, ii in increasing order.  forAssy, gdof         */
+/* [count][nsize] (forAssyVec, globalDOFnums), K row-major, F [count][nsize] updated in place.    */
+/* The mixed form (applyDirichletBCs2field, :717-782) is this call on the combined u-p matrix.    */
+mpf_status mpf_elem_apply_dirichlet_batch(int32_t count, int32_t nsize, const int32_t* forAssy, const int32_t* gdof,
+                                          const double* dispApplied, const double* K, double* F,
+                                          mpf_stream_t stream);
+/* Device memory for callers without the HIP headers. */
+mpf_status mpf_malloc(void** ptr, size_t bytes);
+mpf_status mpf_free(void* ptr);
+mpf_status mpf_memcpy_htod(void* dst, const void* src, size_t bytes);
+mpf_status mpf_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Timing helpers: average device time (ms) of one assembly (element kernels + scatter, no       */

@@ -158,6 +158,17 @@ SIGNATURES = [
     ("mpf_mesh_tet10", None, [C.c_int32, C.c_double, C.c_double, C.c_uint64, _DP, _I32P]),
     ("mpf_mesh_face_nodes", C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32P]),
     ("mpf_mesh_hex_face_quads", C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32P]),
+    ("mpf_system_create", C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, _I32P, _I32P, C.c_int32, _I32P]),
+    ("mpf_system_zero", C.c_int, [_P]),
+    ("mpf_system_map_elements", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int32, C.c_int32,
+                                          _I32P]),
+    ("mpf_system_add_elements", C.c_int, [_P, C.c_int32, _P, _P]),
+    ("mpf_system_add_rhs", C.c_int, [_P, _P]),
+    ("mpf_elem_apply_dirichlet_batch", C.c_int, [C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P]),
+    ("mpf_malloc", C.c_int, [C.POINTER(_P), C.c_size_t]),
+    ("mpf_free", C.c_int, [_P]),
+    ("mpf_memcpy_htod", C.c_int, [_P, _P, C.c_size_t]),
+    ("mpf_memcpy_dtoh", C.c_int, [_P, _P, C.c_size_t]),
 ]
 
 _lib = None
@@ -679,6 +690,95 @@ class Problem:
         _check(lib().mpf_time_assembly(self._h, reps, int(first_iter), lam, C.byref(a), C.byref(s)),
                "mpf_time_assembly")
         return a.value, s.value
+
+
+# ---------------------------------------------------------------------------------------------
+# SolverPetsc-shaped linear system over a caller-supplied pattern (csrc/system.hip)
+# ---------------------------------------------------------------------------------------------
+class System:
+    """SolverPetsc's assemble / solve calls (SolverPetsc.h:16-96) on the device, for a caller that
+    keeps its own pattern and element loop.  Element matrices arrive as device tensors
+    K [count, nrow, ncol] and F [count, nrow] with their dof lists rows [count, nrow], cols
+    [count, ncol] (-1 = skip); map_elements builds the CSR-slot map once, add_elements sums in element
+    order (bitwise the sequential ADD_VALUES loop)."""
+
+    def __init__(self, dispDOF: int, presDOF: int, rowptr, colidx, NodeDofArray=None):
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+        if rowptr.size != dispDOF + presDOF + 1 or colidx.size != int(rowptr[-1]):
+            raise ValueError("rowptr / colidx sizes do not match the dof counts")
+        self.n, self.nnz = dispDOF + presDOF, int(rowptr[-1])
+        nda = None if NodeDofArray is None else np.ascontiguousarray(NodeDofArray, dtype=np.int32).reshape(-1, 3)
+        self._h = C.c_void_p()
+        st = lib().mpf_system_create(C.byref(self._h), dispDOF, presDOF, _ip(rowptr), _ip(colidx),
+                                     0 if nda is None else len(nda), None if nda is None else _ip(nda))
+        _check(st, "mpf_system_create")
+
+    def close(self):
+        if self._h:
+            lib().mpf_problem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def zero_mtx(self):
+        _check(lib().mpf_system_zero(self._h), "mpf_system_zero")
+
+    def map_elements(self, rows, cols, r_off: int = 0, c_off: int = 0) -> int:
+        """rows / cols: contiguous device int32 tensors [count, nrow] / [count, ncol]."""
+        import torch
+        count, nrow = rows.shape
+        if cols.shape[0] != count or rows.dtype != torch.int32 or cols.dtype != torch.int32:
+            raise ValueError("rows / cols: int32 [count, n] with the same count")
+        torch.cuda.current_stream().synchronize()
+        mid = C.c_int32()
+        _check(lib().mpf_system_map_elements(self._h, count, nrow, cols.shape[1], _dptr(rows.contiguous()),
+                                             _dptr(cols.contiguous()), r_off, c_off, C.byref(mid)),
+               "mpf_system_map_elements")
+        return mid.value
+
+    def add_elements(self, map_id: int, K=None, F=None):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_system_add_elements(self._h, map_id, None if K is None else _dptr(K),
+                                             None if F is None else _dptr(F)), "mpf_system_add_elements")
+
+    def add_rhs(self, f):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        _check(lib().mpf_system_add_rhs(self._h, _dptr(f)), "mpf_system_add_rhs")
+
+    def system(self):
+        vals, rhs = np.zeros(self.nnz), np.zeros(self.n)
+        _check(lib().mpf_get_system(self._h, _dp(vals), _dp(rhs)), "mpf_get_system")
+        return vals, rhs
+
+    def factorise_and_solve(self, ksp=KSP_CG, pc=PC_JACOBI, rtol=1e-10, maxit=10000, restart=50):
+        o = KrylovOpts(ksp, pc, restart, maxit, rtol)
+        r = KrylovReport()
+        st = lib().mpf_solve(self._h, C.byref(o), C.byref(r))
+        if st not in (OK, ERR_NOT_CONVERGED):
+            _check(st, "mpf_solve")
+        return r
+
+    def solution(self):
+        x = np.zeros(self.n)
+        _check(lib().mpf_get_solution(self._h, _dp(x)), "mpf_get_solution")
+        return x
+
+
+def apply_dirichlet_batch(forAssy, gdof, dispApplied, K, F):
+    """ElementBase::applyDirichletBCs for a batch of element matrices (device tensors, F in place)."""
+    import torch
+    count, nsize = forAssy.shape
+    torch.cuda.current_stream().synchronize()
+    _check(lib().mpf_elem_apply_dirichlet_batch(count, nsize, _dptr(forAssy), _dptr(gdof), _dptr(dispApplied),
+                                                _dptr(K), _dptr(F), None), "mpf_elem_apply_dirichlet_batch")
+    torch.cuda.synchronize()
 
 
 # ---------------------------------------------------------------------------------------------

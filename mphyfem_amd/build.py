@@ -23,6 +23,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libmpf.so"
+HOST_CALLER_SRC = ROOT / "tests" / "host_caller.cpp"
+HOST_CALLER = ROOT / "tests" / "host_caller"
 ORACLE_SRC = ROOT / "oracle" / "oracle.cpp"
 ORACLE_LIB = ROOT / "oracle" / "liboracle.so"
 
@@ -62,6 +64,16 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_host_caller() -> Path:
+    """The compiled C++ caller of include/mpf_host.hpp (a test of the boundary), linked to libmpf.so."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = HOST_CALLER.with_name(HOST_CALLER.name + ".tmp")
+    _run([gxx, "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "include"), str(HOST_CALLER_SRC), "-o", str(tmp),
+          "-L", str(PKG), "-lmpf", "-Wl,-rpath,$ORIGIN/../mphyfem_amd", "-Wl,-rpath-link,/opt/rocm/lib"])
+    os.replace(tmp, HOST_CALLER)
+    return HOST_CALLER
+
+
 def build_oracle() -> Path:
     gxx = shutil.which("g++") or "g++"
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
@@ -79,6 +91,7 @@ def main(argv: list[str] | None = None) -> int:
     a = ap.parse_args(argv)
     if not a.oracle:
         print("built", build_lib(a.j))
+        print("built", build_host_caller())
     if not a.lib:
         print("built", build_oracle())
     return 0

@@ -921,24 +921,24 @@ extern "C" {
 
 mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                          const double* mu, const double* Kinv) {
-  if (!p || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 only
+  if (!p || p->etype == 0 || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 only
   return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
 }
 
 mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   return mpf::batch_autotune(p, tile, lanes_per_row, ms_per_cg_iter);
 }
 
 mpf_status mpf_batch_newton_solve(mpf_problem* p, const mpf_batch_opts* o, int32_t* mesh_iters, double* mesh_relres,
                                   mpf_batch_report* rep) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   return mpf::batch_newton_solve(p, o, mesh_iters, mesh_relres, rep);
 }
 
 mpf_status mpf_batch_newton_step(mpf_problem* p, double lam, const mpf_batch_opts* o, const double* guess,
                                  int32_t* mesh_iters, double* mesh_relres, mpf_batch_report* rep) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   return mpf::batch_newton_step(p, lam, o, guess, mesh_iters, mesh_relres, rep);
 }
 

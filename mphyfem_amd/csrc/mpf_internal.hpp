@@ -164,6 +164,7 @@ struct PatternBufs {
 struct KrylovWork;  // krylov.hip
 struct BatchWork;   // batch.hip
 struct BandWork;    // band.hip
+struct SystemWork;  // system.hip
 
 }  // namespace mpf
 
@@ -173,6 +174,7 @@ struct mpf_problem {
   mpf_matl m{};
   hipStream_t stream = nullptr;
   mpf::DevBuf<double> X;
+  std::vector<double> X_h;  // host copy of X (traction integration at prepare)
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};  // bounding box of the nodes
   mpf::DevBuf<int32_t> conn;
   // Dirichlet data (host lists, merged last-wins per dof like setBoundaryConditions)
@@ -207,9 +209,11 @@ struct mpf_problem {
   mpf::DevBuf<double> scratch;  // reductions (norms)
   mpf::BatchWork* batch = nullptr;  // independent meshes solved together (batch.hip)
   mpf::BandWork* band = nullptr;    // banded LDL^T direct solver (band.hip)
+  mpf::SystemWork* sys = nullptr;   // element maps of a caller-pattern system (etype 0, system.hip)
 };
 
 namespace mpf {
+uint64_t next_problem_uid();                        // problem.hip: unique per process
 size_t stage_doubles_per_elem(int etype);          // gather.hip
 mpf_status ensure_gather_tables(mpf_problem* p);   // gather.hip (falls back to the scatter when unsupported)
 mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const double* dispApplied, int writeK,
@@ -236,6 +240,7 @@ mpf_status band_solve(mpf_problem* p, mpf_krylov_report* rep);
 int32_t* band_bad_flag(mpf_problem* p);
 void band_free(mpf_problem* p);
 void krylov_free(mpf_problem* p);
+void system_free(mpf_problem* p);                  // system.hip
 // Most partial sums of one device reduction (one per workgroup of the vector kernels).
 constexpr int kMaxPartials = 4096;
 // ||x||_2 to the host; scratch: kMaxPartials + 1 doubles of device memory.

@@ -771,7 +771,7 @@ mpf_status mpf_mpnn_zero_grad(mpf_mpnn* net) {
 // Forward on a problem's node graph: x [nNode][32] bf16 (device; may be the network's own buffer
 // filled by mpf_mpnn_features when null), y [nNode][32] fp32 (device, optional copy-out).
 mpf_status mpf_mpnn_forward(mpf_mpnn* net, mpf_problem* p, const uint16_t* x, float* y) {
-  if (!net || !p) return MPF_ERR_ARG;
+  if (!net || !p || p->etype == 0) return MPF_ERR_ARG;
   const int32_t *nbptr, *nb;
   MPF_TRY(graph_of(p, &nbptr, &nb));
   const int64_t n = p->nNode;
@@ -786,7 +786,7 @@ mpf_status mpf_mpnn_forward(mpf_mpnn* net, mpf_problem* p, const uint16_t* x, fl
 // Backward of the last forward for dL/dy = dy [nNode][32] fp32 (device; NULL: the one left by
 // mpf_mpnn_loss); gradients accumulate into the network.
 mpf_status mpf_mpnn_backward(mpf_mpnn* net, mpf_problem* p, const float* dy) {
-  if (!net || !p) return MPF_ERR_ARG;
+  if (!net || !p || p->etype == 0) return MPF_ERR_ARG;
   const int32_t *nbptr, *nb;
   MPF_TRY(graph_of(p, &nbptr, &nb));
   const int64_t n = p->nNode;
@@ -801,7 +801,7 @@ mpf_status mpf_mpnn_backward(mpf_mpnn* net, mpf_problem* p, const float* dy) {
 // per-mesh scales used by mpf_mpnn_loss and mpf_mpnn_guess.  rho = (lam_{k+1} - lam_k)/(lam_k - lam_{k-1}).
 mpf_status mpf_mpnn_features(mpf_mpnn* net, mpf_problem* p, const double* uk, const double* ukm1, double rho,
                              double lam_next, double lam_ratio) {
-  if (!net || !p || !uk || !ukm1) return MPF_ERR_ARG;
+  if (!net || !p || p->etype == 0 || !uk || !ukm1) return MPF_ERR_ARG;
   MPF_TRY(ensure_ctx(net, p));
   const int64_t n = p->nNode;
   MPF_TRY(ensure_cap(net, n));

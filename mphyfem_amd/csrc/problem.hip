@@ -184,6 +184,11 @@ mpf_status check_device() {
 
 }  // namespace
 
+uint64_t mpf::next_problem_uid() {
+  static std::atomic<uint64_t> next_uid{1};
+  return next_uid++;
+}
+
 // Per-problem host logic shared with the batched solver (batch.hip).
 ElemArgs elem_args(mpf_problem* p) {
   ElemArgs a{};
@@ -201,12 +206,6 @@ ElemArgs elem_args(mpf_problem* p) {
 }
 
 
-
-// host copy of X for traction integration
-std::map<const mpf_problem*, std::vector<double>>& hostX_store() {
-  static std::map<const mpf_problem*, std::vector<double>> m;
-  return m;
-}
 
 // Enqueue the element kernels of one assembly (no zero fill, no external forces).
 mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK) {
@@ -462,9 +461,8 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   for (int64_t i = 0; i < (int64_t)nElem * npe; ++i)
     if (conn[i] < 0 || conn[i] >= nNode) return MPF_ERR_ARG;
   MPF_TRY(check_device());
-  static std::atomic<uint64_t> next_uid{1};
   auto* p = new mpf_problem();
-  p->uid = next_uid++;
+  p->uid = mpf::next_problem_uid();
   p->etype = p0 ? MPF_HEX8 : elem_type;  // Q1/P0 shares the Hex8 numbering and pattern (no pressure dofs)
   p->p0 = p0;
   p->npe = npe;
@@ -497,7 +495,7 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n3)), dim3(256), 0, p->stream, p->dispApplied.ptr, (int64_t)n3, 0.0);
   hipLaunchKernelGGL(k_fill_d, dim3(g256(p->npres)), dim3(256), 0, p->stream, p->pres.ptr, p->npres, 0.0);
   hipLaunchKernelGGL(k_fill_d, dim3(g256(p->npres)), dim3(256), 0, p->stream, p->presPrev.ptr, p->npres, 0.0);
-  hostX_store()[p].assign(X, X + n3);
+  p->X_h.assign(X, X + n3);
   for (int c = 0; c < 3; ++c) {
     p->xmin[c] = nNode > 0 ? X[c] : 0.0;
     p->xmax[c] = p->xmin[c];
@@ -521,7 +519,7 @@ mpf_status mpf_problem_destroy(mpf_problem* p) {
   krylov_free(p);
   batch_free(p);
   band_free(p);
-  hostX_store().erase(p);
+  system_free(p);
   hipStream_t s = p->stream;
   delete p;
   if (s) (void)hipStreamDestroy(s);
@@ -529,7 +527,7 @@ mpf_status mpf_problem_destroy(mpf_problem* p) {
 }
 
 mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, int32_t dof, const double* value) {
-  if (!p || n < 0 || (n > 0 && !nodes) || dof > 2) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || n < 0 || (n > 0 && !nodes) || dof > 2) return MPF_ERR_ARG;
   if (p->prepared) return MPF_ERR_STATE;
   // merge last-wins per dof (setBoundaryConditions loops the patches in order)
   std::map<int32_t, double> cur;
@@ -546,7 +544,7 @@ mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, in
 }
 
 mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1) {
-  if (!p || nf < 0 || (nf > 0 && !faces)) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || nf < 0 || (nf > 0 && !faces)) return MPF_ERR_ARG;
   if (p->prepared) return MPF_ERR_STATE;
   for (int64_t i = 0; i < (int64_t)nf * 4; ++i)
     if (faces[i] < 0 || faces[i] >= p->nNode) return MPF_ERR_ARG;
@@ -579,7 +577,7 @@ mpf_status mpf_prepare_pattern(mpf_problem* p) {
   hipLaunchKernelGGL(k_fill_d, dim3(g256(n)), dim3(256), 0, s, p->du.ptr, n, 0.0);
   if (!p->trac.empty()) {
     std::vector<double> F;
-    traction_unit(hostX_store()[p], p->trac, p->nNode, F);
+    traction_unit(p->X_h, p->trac, p->nNode, F);
     MPF_TRY(p->fext_unit.upload(F.data(), F.size(), s));
     p->has_fext = true;
   }
@@ -618,7 +616,7 @@ mpf_status mpf_get_pattern(const mpf_problem* p, int32_t* rowptr, int32_t* colid
 
 mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_prev, const double* pres,
                          const double* pres_prev) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   hipStream_t s = p->stream;
   const size_t n3 = (size_t)p->nNode * 3;
   if (disp) MPF_HIP(hipMemcpyAsync(p->disp.ptr, disp, n3 * sizeof(double), hipMemcpyHostToDevice, s));
@@ -630,14 +628,14 @@ mpf_status mpf_set_state(mpf_problem* p, const double* disp, const double* disp_
 }
 
 mpf_status mpf_state_device_ptrs(mpf_problem* p, double** disp, double** pres) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   if (disp) *disp = p->disp.ptr;
   if (pres) *pres = p->pres.ptr;
   return MPF_OK;
 }
 
 mpf_status mpf_copy_state_device(mpf_problem* p, double* disp_dst) {
-  if (!p || !disp_dst) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || !disp_dst) return MPF_ERR_ARG;
   MPF_HIP(hipMemcpyAsync(disp_dst, p->disp.ptr, (size_t)p->nNode * 3 * sizeof(double), hipMemcpyDeviceToDevice,
                          p->stream));
   MPF_HIP(hipStreamSynchronize(p->stream));
@@ -645,7 +643,7 @@ mpf_status mpf_copy_state_device(mpf_problem* p, double* disp_dst) {
 }
 
 mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   hipStream_t s = p->stream;
   if (disp) MPF_TRY(p->disp.download(disp, (size_t)p->nNode * 3, s));
   if (pres) MPF_TRY(p->pres.download(pres, (size_t)p->npres, s));
@@ -654,7 +652,7 @@ mpf_status mpf_get_state(const mpf_problem* p, double* disp, double* pres) {
 }
 
 mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
-  if (!p || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER &&
+  if (!p || p->etype == 0 || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER &&
              mode != MPF_ASSEMBLY_GATHER_DIRECT))
     return MPF_ERR_ARG;
   if (p->p0 && mode != MPF_ASSEMBLY_COLORED_SCATTER) return MPF_ERR_ARG;  // Q1/P0: coloured scatter only
@@ -664,7 +662,7 @@ mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
 }
 
 mpf_status mpf_set_chunk_stage_cap(mpf_problem* p, int32_t cap) {
-  if (!p || cap < 0 || cap > 32) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || cap < 0 || cap > 32) return MPF_ERR_ARG;
   p->chunk_cap = cap;
   return MPF_OK;
 }
@@ -676,7 +674,7 @@ mpf_status mpf_get_assembly_mode(const mpf_problem* p, int32_t* mode) {
 }
 
 mpf_status mpf_set_bc(mpf_problem* p, double lam) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   MPF_TRY(do_set_bc(p, lam));
   MPF_HIP(hipStreamSynchronize(p->stream));
@@ -684,7 +682,7 @@ mpf_status mpf_set_bc(mpf_problem* p, double lam) {
 }
 
 mpf_status mpf_assemble(mpf_problem* p, int32_t first_iter, double lam) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   return assemble_sync(p, first_iter != 0, lam, 1);
 }
@@ -721,6 +719,9 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
   if (!p->assembled) return MPF_ERR_STATE;  // SolverPetsc::factoriseAndSolve returns -1 (SolverPetsc.cpp:321-323)
   if (o->ksp != MPF_KSP_CG && o->ksp != MPF_KSP_GMRES && o->ksp != MPF_KSP_BAND) return MPF_ERR_ARG;
   if (o->ksp != MPF_KSP_BAND && (o->pc < MPF_PC_NONE || o->pc > MPF_PC_BJACOBI3)) return MPF_ERR_ARG;
+  if (p->etype == 0 && o->ksp == MPF_KSP_BAND) return MPF_ERR_ARG;  // the band ordering needs the mesh
+  if (p->etype == 0 && p->nNode == 0 && o->pc == MPF_PC_BJACOBI3 && p->pat.dispDOF > 0)
+    return MPF_ERR_ARG;  // node blocks need the caller's NodeDofArray
   std::memset(rep, 0, sizeof(*rep));
   MPF_TRY(krylov_solve(p, o, rep));
   p->solved = true;
@@ -736,7 +737,7 @@ mpf_status mpf_get_solution(const mpf_problem* p, double* du) {
 }
 
 mpf_status mpf_update(mpf_problem* p, int32_t first_iter) {
-  if (!p) return MPF_ERR_ARG;
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   if (!p->solved) return MPF_ERR_STATE;
   MPF_TRY(do_update(p, first_iter != 0));
   MPF_HIP(hipStreamSynchronize(p->stream));
@@ -745,7 +746,7 @@ mpf_status mpf_update(mpf_problem* p, int32_t first_iter) {
 
 mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, const double* guess, int32_t* iters,
                            mpf_newton_report* rep) {
-  if (!p || !o || !rep) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || !o || !rep) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   std::memset(rep, 0, sizeof(*rep));
   const auto t0 = std::chrono::steady_clock::now();
@@ -760,7 +761,7 @@ mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o,
 // MyTime (MyTime.cpp:26-106): update cur += dt; on failure cut (cur -= dt, dt *= 0.5, refuse below
 // 1% of the initial dt) and reset the state; on success pop the step-size stack.
 mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* iters, mpf_newton_report* rep) {
-  if (!p || !o || !rep || o->nsteps < 0 || o->dt <= 0) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || !o || !rep || o->nsteps < 0 || o->dt <= 0) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   std::memset(rep, 0, sizeof(*rep));
   const auto t0 = std::chrono::steady_clock::now();
@@ -807,7 +808,7 @@ mpf_status mpf_newton_solve(mpf_problem* p, const mpf_newton_opts* o, int32_t* i
 
 mpf_status mpf_time_assembly(mpf_problem* p, int32_t reps, int32_t first_iter, double lam, double* ms_per_assembly,
                              double* ms_scatter_kernels) {
-  if (!p || reps <= 0) return MPF_ERR_ARG;
+  if (!p || p->etype == 0 || reps <= 0) return MPF_ERR_ARG;
   if (!p->prepared) return MPF_ERR_STATE;
   hipStream_t s = p->stream;
   hipEvent_t e0, e1;

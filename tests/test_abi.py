@@ -163,3 +163,16 @@ def test_newton_step_cut_gives_up_below_the_smallest_increment():
     P = _CutProblem(max_inc=0.001)
     with pytest.raises(M.NegativeJacobian):
         newton_with_cut(P, 0.0, 1.0, None)
+
+
+def test_host_layer_header_compiles_and_caller_links(mpf):
+    """include/mpf_host.hpp is self-contained C++17 over mpf.h (no HIP headers), and the compiled
+    caller links against libmpf.so and loads without touching a device."""
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++", "-I",
+                        str(ROOT / "include"), "-"], input='#include "mpf_host.hpp"\nint main() { return 0; }\n',
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    caller = ROOT / "tests" / "host_caller"
+    assert caller.exists(), "run `python -m mphyfem_amd.build`"
+    r = subprocess.run([str(caller), "--link-check"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "Negative Jacobian in the element" in r.stdout, (r.stdout, r.stderr)
