@@ -140,6 +140,10 @@ mpf_status mpf_add_dirichlet(mpf_problem* p, int32_t n, const int32_t* nodes, in
 /* Quad4 dead-load traction patch t = (v0*normal + v1*tangent1) * lambda                        */
 /* (femSolidmechanics1.cpp:922-1119; BasisFunctionsLagrange.cpp:1688-1732).  faces [nf][4].     */
 mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1);
+/* The same dead load on Tria6 faces [nf][6] (corners, then the mid-edge nodes of (0,1), (1,2),   */
+/* (2,0)), the boundary faces of a Tet10 mesh: LagrangeBasisFunsFace with npElem 6 and the 3-point */
+/* triangle rule (BasisFunctionsLagrange.cpp:364-418, 1688-1732; QuadratureUtil.cpp:223-236).      */
+mpf_status mpf_add_traction_tri6(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1);
 
 /* Assembly algorithm used by mpf_assemble (same K and R either way):                           */
 /*   MPF_ASSEMBLY_GATHER (default): per-element Gauss-point staging, then the (element, node)     */
@@ -363,6 +367,69 @@ mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const
 /* assemblies; the problem's state is restored).  n_learned: meshes that keep the correction.          */
 mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, const double* uk, const double* ukm1,
                                   double rho, double* guess, int32_t* n_learned);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Input decks: the reference's block-structured config file (femBase::readInput and its read*     */
+/* blocks, femBase.cpp:370-1200), the Gmsh 2.2 ASCII mesh its Files block names (readInputGMSH,    */
+/* femBase.cpp:121-364; looked up beside the config file, ".msh" appended) and petsc_options.dat    */
+/* beside it (-ksp_type, -pc_type, -ksp_rtol, -ksp_max_it, -ksp_gmres_restart; preonly / lu ->     */
+/* MPF_KSP_BAND; without the file the code default CG + ILU of SolverPetsc.cpp:117-140 becomes CG  */
+/* + 3x3 node-block Jacobi).  Supported: 3D, one domain of Hex8 (Gmsh 5) or Tet10 (Gmsh 11),        */
+/* Matl_NeoHookean, ELEM_SOLID_3D_DISP / _MIXED0 / _MIXED1, fixed and specified patches with        */
+/* constant values, traction patches of Quad4 (Gmsh 3) or Tria6 (Gmsh 9) faces, time functions,    */
+/* solvertype newton, all time-varying loads on one time function; anything else: MPF_ERR_ARG     */
+/* with a message on stderr.                                                                       */
+typedef struct mpf_deck mpf_deck;
+typedef struct {
+  int32_t ndim, nNode, nElem, npe, gmsh_type;
+  int32_t elem_type;            /* MPF_* once mpf_deck_setup has run, else 0 */
+  int32_t npatch, nbc, ntrac, nmat, ntf;
+  int32_t arclength;            /* solvertype arclength */
+  double dt, dt_min, dt_max, final_time;  /* timeStep (1 to 3 values, MyTime::set), finalTime */
+  int32_t max_steps, max_iters;
+  double tol;                   /* absolute ||R|| tolerance (converged, femSolidmechanics1.cpp:616-622) */
+  mpf_krylov_opts lin;          /* linear solver from petsc_options.dat or the default */
+  int32_t lin_from_file;
+  int32_t pad_;
+} mpf_deck_info;
+typedef struct {
+  int32_t steps_done, newton_iters, krylov_iters, cuts, converged, pad_;
+  double final_time, final_lam, final_norm, seconds;
+} mpf_deck_report;
+mpf_status mpf_deck_load(const char* config_path, mpf_deck** out);
+mpf_status mpf_deck_destroy(mpf_deck* d);
+mpf_status mpf_deck_get_info(const mpf_deck* d, mpf_deck_info* info);
+/* Boundary patch `index` (0 .. npatch-1): its physical name, dimension and sorted unique nodes   */
+/* (BoundaryPatch::processData, BoundaryPatch.cpp:45-53); nodes may be NULL to ask the count.      */
+mpf_status mpf_deck_patch(const mpf_deck* d, int32_t index, char* name, int32_t name_len, int32_t* dim,
+                          int32_t* nnodes, int32_t* nodes);
+/* Builds the problem (prepareMatrixPattern with the deck's patches); *p stays owned by the deck. */
+mpf_status mpf_deck_setup(mpf_deck* d, mpf_problem** p);
+/* solveWithNewtonRaphson over the deck's time steps (femSolidmechanics2.cpp:730-851, MyTime cut / */
+/* stck, load factor = the loads' time function at the current time, absolute tolerance); with     */
+/* vtu_prefix a .vtu per converged step (postProcess).  MPF_ERR_NOT_CONVERGED when the step size   */
+/* falls below 1% of the first (MyTime::cut).                                                      */
+mpf_status mpf_deck_run(mpf_deck* d, const char* vtu_prefix, mpf_deck_report* rep);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Post-processing of the current state.  Per Gauss point MPF_STRESS_NCOMP values: the Cauchy     */
+/* stress (9, the column-major 9-index map), von Mises and tr(sigma)/3 (the pressure field itself  */
+/* for the mixed elements) -- Elem_Solid_3D_Disp::projectStress (Elem_Solid_3D_Disp.cpp:480-557).  */
+/* out: host [nElem][nGP][11] (nGP 8 for Hex8, 4 for Tet10).                                      */
+#define MPF_STRESS_NCOMP 11
+mpf_status mpf_stress_gauss(mpf_problem* p, double* out);
+/* Element values = mean over the Gauss points (Elem_Solid_3D_Disp.cpp:416-421); nodal values =   */
+/* mean of the element values over the elements at the node, in increasing element id            */
+/* (femSolidmechanics::projectStresses, femSolidmechanics3.cpp:918-960).  elem: host [nElem][11],  */
+/* nodal: host [nNode][11]; either may be NULL.                                                   */
+mpf_status mpf_stress_project(mpf_problem* p, double* elem, double* nodal);
+/* VTK XML unstructured grid of the current state (femSolidmechanics::postProcess,               */
+/* femSolidmechanics3.cpp:267-776): points at X + u, point data "disp", "velocity" (zero: static), */
+/* "pressure" (Tet10/P1: vertex values, mid-edge nodes the mean of their edge's vertices),        */
+/* "stress", "vonMises" (nodal means); cell data "ElemID", "MatlID", "pressure" (Q1/P0),          */
+/* "stress", "vonMises" (element means); field data "TIME".  Hex8 as VTK_HEXAHEDRON, Tet10 as       */
+/* VTK_QUADRATIC_TETRA with the reference's node map.  binary: raw appended data, else ASCII.      */
+mpf_status mpf_write_vtu(mpf_problem* p, const char* path, double time, int32_t binary);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Linear system over a caller-supplied CSR pattern, filled from caller-computed element         */

@@ -93,6 +93,21 @@ _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
 
+class DeckInfo(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("nNode", C.c_int32), ("nElem", C.c_int32), ("npe", C.c_int32),
+                ("gmsh_type", C.c_int32), ("elem_type", C.c_int32), ("npatch", C.c_int32), ("nbc", C.c_int32),
+                ("ntrac", C.c_int32), ("nmat", C.c_int32), ("ntf", C.c_int32), ("arclength", C.c_int32),
+                ("dt", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double), ("final_time", C.c_double),
+                ("max_steps", C.c_int32), ("max_iters", C.c_int32), ("tol", C.c_double), ("lin", KrylovOpts),
+                ("lin_from_file", C.c_int32), ("pad_", C.c_int32)]
+
+
+class DeckReport(C.Structure):
+    _fields_ = [("steps_done", C.c_int32), ("newton_iters", C.c_int32), ("krylov_iters", C.c_int32),
+                ("cuts", C.c_int32), ("converged", C.c_int32), ("pad_", C.c_int32), ("final_time", C.c_double),
+                ("final_lam", C.c_double), ("final_norm", C.c_double), ("seconds", C.c_double)]
+
+
 # (name, restype, argtypes) for every symbol of include/mpf.h
 SIGNATURES = [
     ("mpf_status_string", C.c_char_p, [C.c_int32]),
@@ -121,6 +136,16 @@ SIGNATURES = [
     ("mpf_rhs_norm", C.c_int, [_P, _DP]),
     ("mpf_solve", C.c_int, [_P, C.POINTER(KrylovOpts), C.POINTER(KrylovReport)]),
     ("mpf_get_solution", C.c_int, [_P, _DP]),
+    ("mpf_add_traction_tri6", C.c_int, [_P, C.c_int32, _I32P, C.c_double, C.c_double]),
+    ("mpf_deck_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    ("mpf_deck_destroy", C.c_int, [_P]),
+    ("mpf_deck_get_info", C.c_int, [_P, C.POINTER(DeckInfo)]),
+    ("mpf_deck_patch", C.c_int, [_P, C.c_int32, C.c_char_p, C.c_int32, _I32P, _I32P, _I32P]),
+    ("mpf_deck_setup", C.c_int, [_P, C.POINTER(_P)]),
+    ("mpf_deck_run", C.c_int, [_P, C.c_char_p, C.POINTER(DeckReport)]),
+    ("mpf_stress_gauss", C.c_int, [_P, _DP]),
+    ("mpf_stress_project", C.c_int, [_P, _DP, _DP]),
+    ("mpf_write_vtu", C.c_int, [_P, C.c_char_p, C.c_double, C.c_int32]),
     ("mpf_update", C.c_int, [_P, C.c_int32]),
     ("mpf_newton_solve", C.c_int, [_P, C.POINTER(NewtonOpts), _I32P, C.POINTER(NewtonReport)]),
     ("mpf_newton_step", C.c_int, [_P, C.c_double, C.POINTER(NewtonOpts), _DP, _I32P, C.POINTER(NewtonReport)]),
@@ -491,8 +516,12 @@ class Problem:
         _check(lib().mpf_add_dirichlet(self._h, len(nodes), _ip(nodes), dof, _dp(val)), "mpf_add_dirichlet")
 
     def add_traction(self, faces, v0: float, v1: float = 0.0):
+        """faces [nf, 4] (Quad4) or [nf, 6] (Tria6: corners, then the mid-edge nodes of (0,1), (1,2), (2,0))."""
         faces = np.ascontiguousarray(faces, dtype=np.int32)
-        _check(lib().mpf_add_traction(self._h, faces.shape[0], _ip(faces), v0, v1), "mpf_add_traction")
+        if faces.ndim != 2 or faces.shape[1] not in (4, 6):
+            raise ValueError("faces must be [nf, 4] or [nf, 6]")
+        f = lib().mpf_add_traction_tri6 if faces.shape[1] == 6 else lib().mpf_add_traction
+        _check(f(self._h, faces.shape[0], _ip(faces), v0, v1), "mpf_add_traction")
 
     def prepare_matrix_pattern(self):
         _check(lib().mpf_prepare_pattern(self._h), "mpf_prepare_pattern")
@@ -564,6 +593,22 @@ class Problem:
         if st not in (OK, ERR_NOT_CONVERGED):
             _check(st, "mpf_solve")
         return r
+
+    def stress_gauss(self):
+        """[nElem, nGP, 11]: Cauchy stress (9, column-major map), von Mises, tr(sigma)/3."""
+        ngp = 8 if self.elem_type in (HEX8, HEX8P0) else 4
+        out = np.zeros((self.nElem, ngp, 11))
+        _check(lib().mpf_stress_gauss(self._h, _dp(out)), "mpf_stress_gauss")
+        return out
+
+    def stress_project(self):
+        """(element means [nElem, 11], nodal means over the adjacent elements [nNode, 11])."""
+        ev, nv = np.zeros((self.nElem, 11)), np.zeros((self.nNode, 11))
+        _check(lib().mpf_stress_project(self._h, _dp(ev), _dp(nv)), "mpf_stress_project")
+        return ev, nv
+
+    def write_vtu(self, path, time: float = 0.0, binary: bool = True):
+        _check(lib().mpf_write_vtu(self._h, str(path).encode(), time, int(binary)), "mpf_write_vtu")
 
     def solution(self):
         d, p, _, _ = self.sizes()
@@ -690,6 +735,71 @@ class Problem:
         _check(lib().mpf_time_assembly(self._h, reps, int(first_iter), lam, C.byref(a), C.byref(s)),
                "mpf_time_assembly")
         return a.value, s.value
+
+
+class _ProblemView(Problem):
+    """A Problem whose handle another object owns (a Deck's problem): the calls work, close() leaves
+    the handle alone."""
+
+    def __init__(self, h, elem_type: int, nNode: int, nElem: int):
+        self._h = h
+        self.elem_type, self.nNode, self.nElem = elem_type, nNode, nElem
+        self.X = self.conn = None
+
+    def close(self):
+        self._h = C.c_void_p()
+
+
+# ---------------------------------------------------------------------------------------------
+# input decks (csrc/deck.cpp): config + Gmsh 2.2 mesh + petsc_options.dat
+# ---------------------------------------------------------------------------------------------
+class Deck:
+    """A reference input deck (config file, the Gmsh mesh it names, petsc_options.dat beside it)."""
+
+    def __init__(self, config_path):
+        self._h = C.c_void_p()
+        _check(lib().mpf_deck_load(str(config_path).encode(), C.byref(self._h)), "mpf_deck_load")
+
+    def close(self):
+        if self._h:
+            lib().mpf_deck_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> DeckInfo:
+        i = DeckInfo()
+        _check(lib().mpf_deck_get_info(self._h, C.byref(i)), "mpf_deck_get_info")
+        return i
+
+    def patches(self):
+        """{name: (dim, sorted unique node ids)}."""
+        out = {}
+        for k in range(self.info().npatch):
+            name = C.create_string_buffer(256)
+            dim, nn = C.c_int32(), C.c_int32()
+            _check(lib().mpf_deck_patch(self._h, k, name, 256, C.byref(dim), C.byref(nn), None), "mpf_deck_patch")
+            nodes = np.zeros(nn.value, dtype=np.int32)
+            _check(lib().mpf_deck_patch(self._h, k, None, 0, None, None, _ip(nodes)), "mpf_deck_patch")
+            out[name.value.decode()] = (dim.value, nodes)
+        return out
+
+    def setup(self):
+        """Builds the problem; returns a Problem view of it (owned by the deck)."""
+        h = C.c_void_p()
+        _check(lib().mpf_deck_setup(self._h, C.byref(h)), "mpf_deck_setup")
+        i = self.info()
+        return _ProblemView(h, i.elem_type, i.nNode, i.nElem)
+
+    def run(self, vtu_prefix=None) -> DeckReport:
+        r = DeckReport()
+        st = lib().mpf_deck_run(self._h, None if vtu_prefix is None else str(vtu_prefix).encode(), C.byref(r))
+        _check(st, "mpf_deck_run")
+        return r
 
 
 # ---------------------------------------------------------------------------------------------

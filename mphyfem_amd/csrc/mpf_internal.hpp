@@ -183,7 +183,7 @@ struct mpf_problem {
   mpf::DevBuf<int32_t> bc_dof;
   mpf::DevBuf<double> bc_val;
   // traction patches (host) and the unit-load external force vector (device)
-  struct Trac { std::vector<int32_t> faces; double v0, v1; };
+  struct Trac { std::vector<int32_t> faces; double v0, v1; int npf = 4; };  // Quad4 (4) or Tria6 (6) faces
   std::vector<Trac> trac;
   mpf::DevBuf<double> fext_unit;  // [nNode*3] for lambda = 1
   bool has_fext = false;

@@ -140,37 +140,57 @@ inline unsigned g256(int64_t n) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
-// External force vector for lambda = 1 from the Quad4 traction patches, on the host (one-time per
-// problem; the per-step cost is the scaled add in k_add_fext).  femSolidmechanics1.cpp:1026-1082.
+// External force vector for lambda = 1 from the traction patches, on the host (one-time per
+// problem; the per-step cost is the scaled add in k_add_fext).  femSolidmechanics1.cpp:1026-1082
+// with LagrangeBasisFunsFace (BasisFunctionsLagrange.cpp:1688-1732): Quad4 faces with the 2x2 rule,
+// Tria6 faces (LagrangeBasisFunsTria npElem 6, :364-418) with the 3-point rule (QuadratureUtil.cpp:
+// 562-570); t = v0 n + v1 dx/dxi1 with n = dx/dxi2 x dx/dxi1 normalised.
 void traction_unit(const std::vector<double>& X, const std::vector<mpf_problem::Trac>& tr, int nNode,
                    std::vector<double>& F) {
   F.assign((size_t)nNode * 3, 0.0);
   const double g = 0.577350269189626;
-  const double gp[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+  const double gq[4][2] = {{-g, -g}, {g, -g}, {g, g}, {-g, g}};
+  const double gt[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {4.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 4.0 / 6.0}};
+  const double wt = 0.5 * (1.0 / 3.0);
   for (const auto& t : tr) {
-    const int nf = (int)t.faces.size() / 4;
+    const int npf = t.npf;
+    const int nf = (int)t.faces.size() / npf;
+    const int ngp = npf == 4 ? 4 : 3;
     for (int f = 0; f < nf; ++f) {
-      const int* nn = &t.faces[4 * f];
-      double Fl[12] = {0};
-      for (int q = 0; q < 4; ++q) {
-        const double Nu[2] = {0.5 * (1 - gp[q][0]), 0.5 * (1 + gp[q][0])}, dNu[2] = {-0.5, 0.5};
-        const double Nv[2] = {0.5 * (1 - gp[q][1]), 0.5 * (1 + gp[q][1])}, dNv[2] = {-0.5, 0.5};
-        const double N[4] = {Nv[0] * Nu[0], Nv[0] * Nu[1], Nv[1] * Nu[1], Nv[1] * Nu[0]};
-        const double d1[4] = {Nv[0] * dNu[0], Nv[0] * dNu[1], Nv[1] * dNu[1], Nv[1] * dNu[0]};
-        const double d2[4] = {dNv[0] * Nu[0], dNv[0] * Nu[1], dNv[1] * Nu[1], dNv[1] * Nu[0]};
+      const int* nn = &t.faces[(size_t)npf * f];
+      double Fl[18] = {0};
+      for (int q = 0; q < ngp; ++q) {
+        double N[6], d1[6], d2[6], w;
+        if (npf == 4) {
+          const double Nu[2] = {0.5 * (1 - gq[q][0]), 0.5 * (1 + gq[q][0])}, dNu[2] = {-0.5, 0.5};
+          const double Nv[2] = {0.5 * (1 - gq[q][1]), 0.5 * (1 + gq[q][1])}, dNv[2] = {-0.5, 0.5};
+          N[0] = Nv[0] * Nu[0]; N[1] = Nv[0] * Nu[1]; N[2] = Nv[1] * Nu[1]; N[3] = Nv[1] * Nu[0];
+          d1[0] = Nv[0] * dNu[0]; d1[1] = Nv[0] * dNu[1]; d1[2] = Nv[1] * dNu[1]; d1[3] = Nv[1] * dNu[0];
+          d2[0] = dNv[0] * Nu[0]; d2[1] = dNv[0] * Nu[1]; d2[2] = dNv[1] * Nu[1]; d2[3] = dNv[1] * Nu[0];
+          w = 1.0;
+        } else {
+          const double x1 = gt[q][0], x2 = gt[q][1], x3 = 1.0 - x1 - x2;
+          N[0] = x3 * (2.0 * x3 - 1.0); N[1] = x1 * (2.0 * x1 - 1.0); N[2] = x2 * (2.0 * x2 - 1.0);
+          N[3] = 4.0 * x1 * x3; N[4] = 4.0 * x1 * x2; N[5] = 4.0 * x2 * x3;
+          d1[0] = -4.0 * x3 + 1.0; d1[1] = 4.0 * x1 - 1.0; d1[2] = 0.0;
+          d1[3] = 4.0 * (x3 - x1); d1[4] = 4.0 * x2; d1[5] = -4.0 * x2;
+          d2[0] = -4.0 * x3 + 1.0; d2[1] = 0.0; d2[2] = 4.0 * x2 - 1.0;
+          d2[3] = -4.0 * x1; d2[4] = 4.0 * x1; d2[5] = 4.0 * (x3 - x2);
+          w = wt;
+        }
         double du[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < npf; ++a)
           for (int d = 0; d < 3; ++d) { du[d] += X[3 * nn[a] + d] * d1[a]; dv[d] += X[3 * nn[a] + d] * d2[a]; }
         double n1[3] = {dv[1] * du[2] - dv[2] * du[1], dv[2] * du[0] - dv[0] * du[2], dv[0] * du[1] - dv[1] * du[0]};
         const double Jac = std::sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
         for (int d = 0; d < 3; ++d) n1[d] /= Jac;
-        const double dvol = 1.0 * Jac;
+        const double dvol = w * Jac;
         for (int d = 0; d < 3; ++d) {
           const double tr_d = (t.v0 * n1[d] + t.v1 * du[d]) * dvol;
-          for (int a = 0; a < 4; ++a) Fl[3 * a + d] += N[a] * tr_d;
+          for (int a = 0; a < npf; ++a) Fl[3 * a + d] += N[a] * tr_d;
         }
       }
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < npf; ++a)
         for (int d = 0; d < 3; ++d) F[3 * nn[a] + d] += Fl[3 * a + d];
     }
   }
@@ -552,6 +572,20 @@ mpf_status mpf_add_traction(mpf_problem* p, int32_t nf, const int32_t* faces, do
   t.faces.assign(faces, faces + (size_t)nf * 4);
   t.v0 = v0;
   t.v1 = v1;
+  p->trac.push_back(t);
+  return MPF_OK;
+}
+
+mpf_status mpf_add_traction_tri6(mpf_problem* p, int32_t nf, const int32_t* faces, double v0, double v1) {
+  if (!p || p->etype == 0 || nf < 0 || (nf > 0 && !faces)) return MPF_ERR_ARG;
+  if (p->prepared) return MPF_ERR_STATE;
+  for (int64_t i = 0; i < (int64_t)nf * 6; ++i)
+    if (faces[i] < 0 || faces[i] >= p->nNode) return MPF_ERR_ARG;
+  mpf_problem::Trac t;
+  t.faces.assign(faces, faces + (size_t)nf * 6);
+  t.v0 = v0;
+  t.v1 = v1;
+  t.npf = 6;
   p->trac.push_back(t);
   return MPF_OK;
 }

@@ -252,6 +252,7 @@ mpf_status mpf_system_create(mpf_problem** out, int32_t n_disp, int32_t n_pres, 
   if (st == MPF_OK && nda) st = P.nda.upload(nda, 3 * (size_t)nNode, p->stream);
   if (st == MPF_OK) st = p->vals.alloc(nnz > 0 ? (size_t)nnz : 1);
   if (st == MPF_OK) st = p->rhs.alloc((size_t)n);
+  if (st == MPF_OK) st = p->du.alloc((size_t)n);
   if (st == MPF_OK) st = p->scratch.alloc((size_t)kMaxPartials + 1);
   if (st == MPF_OK && hipStreamSynchronize(p->stream) != hipSuccess) st = MPF_ERR_HIP;
   if (st != MPF_OK) {

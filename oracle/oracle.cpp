@@ -499,7 +499,7 @@ void find_unique(std::vector<int>& v) {
 }
 
 struct DirPatch { std::vector<int> nodes; int dof; std::vector<double> value; };
-struct TracPatch { std::vector<int> faces; double v0, v1; };
+struct TracPatch { std::vector<int> faces; double v0, v1; int npf = 4; };  // npf: 4 (Quad4) or 6 (Tria6)
 
 }  // namespace
 
@@ -548,36 +548,68 @@ inline void csr_add(const orc_problem* p, double* vals, int r, int c, double v) 
 
 // Quad4 face dead load (femSolidmechanics1.cpp:1026-1082; LagrangeBasisFunsFace
 // BasisFunctionsLagrange.cpp:1688-1732): n = (dx/deta x dx/dxi)/|.|, tangent1 = dx/dxi.
+// LagrangeBasisFunsTria, npElem = 6 (BasisFunctionsLagrange.cpp:364-418): xi3 = 1 - xi1 - xi2,
+// nodes 0..2 at the corners, 3 on (0,1), 4 on (1,2), 5 on (2,0)
+void basis_tria6(double xi1, double xi2, double N[6], double d1[6], double d2[6]) {
+  const double xi3 = 1.0 - xi1 - xi2;
+  N[0] = xi3 * (2.0 * xi3 - 1.0); N[1] = xi1 * (2.0 * xi1 - 1.0); N[2] = xi2 * (2.0 * xi2 - 1.0);
+  N[3] = 4.0 * xi1 * xi3; N[4] = 4.0 * xi1 * xi2; N[5] = 4.0 * xi2 * xi3;
+  d1[0] = -4.0 * xi3 + 1.0; d1[1] = 4.0 * xi1 - 1.0; d1[2] = 0.0;
+  d1[3] = 4.0 * (xi3 - xi1); d1[4] = 4.0 * xi2; d1[5] = -4.0 * xi2;
+  d2[0] = -4.0 * xi3 + 1.0; d2[1] = 0.0; d2[2] = 4.0 * xi2 - 1.0;
+  d2[3] = -4.0 * xi1; d2[4] = 4.0 * xi1; d2[5] = 4.0 * (xi3 - xi2);
+}
+
+// getGaussPoints2D for npElem = 6: the 3-point triangle rule (QuadratureUtil.cpp:223-236, 562-570)
+int gauss_tria3(double xi[][2], double w[]) {
+  const double a1 = 1.0 / 3.0, fact = 0.5;
+  xi[0][0] = 1.0 / 6.0; xi[0][1] = 1.0 / 6.0; w[0] = fact * a1;
+  xi[1][0] = 4.0 / 6.0; xi[1][1] = 1.0 / 6.0; w[1] = fact * a1;
+  xi[2][0] = 1.0 / 6.0; xi[2][1] = 4.0 / 6.0; w[2] = fact * a1;
+  return 3;
+}
+
+// addExternalForces, 3D branch (femSolidmechanics1.cpp:1026-1082) with LagrangeBasisFunsFace
+// (BasisFunctionsLagrange.cpp:1688-1732) for Quad4 and Tria6 faces.
 void external_forces(const orc_problem* p, double lam, std::vector<double>& Fext) {
   Fext.assign((size_t)p->nNode * 3, 0.0);
-  double gxi[4][2], gw[4];
-  gauss_quad4(gxi, gw);
+  double gq[4][2], wq[4], gt[3][2], wt[3];
+  gauss_quad4(gq, wq);
+  gauss_tria3(gt, wt);
   for (const auto& tp : p->trac) {
     const double sv0 = tp.v0 * lam, sv1 = tp.v1 * lam;
-    const int nf = (int)tp.faces.size() / 4;
+    const int npf = tp.npf;
+    const int nf = (int)tp.faces.size() / npf;
+    const int ngp = npf == 4 ? 4 : 3;
     for (int f = 0; f < nf; ++f) {
-      const int* nn = &tp.faces[4 * f];
-      double xN[4], yN[4], zN[4];
-      for (int a = 0; a < 4; ++a) { xN[a] = p->X[3 * nn[a]]; yN[a] = p->X[3 * nn[a] + 1]; zN[a] = p->X[3 * nn[a] + 2]; }
-      double Fl[12] = {0};
-      for (int gp = 0; gp < 4; ++gp) {
-        double N[4], d1[4], d2[4];
-        basis_quad4(gxi[gp][0], gxi[gp][1], N, d1, d2);
+      const int* nn = &tp.faces[(size_t)npf * f];
+      double xN[6], yN[6], zN[6];
+      for (int a = 0; a < npf; ++a) { xN[a] = p->X[3 * nn[a]]; yN[a] = p->X[3 * nn[a] + 1]; zN[a] = p->X[3 * nn[a] + 2]; }
+      double Fl[18] = {0};
+      for (int gp = 0; gp < ngp; ++gp) {
+        double N[6], d1[6], d2[6], w;
+        if (npf == 4) {
+          basis_quad4(gq[gp][0], gq[gp][1], N, d1, d2);
+          w = wq[gp];
+        } else {
+          basis_tria6(gt[gp][0], gt[gp][1], N, d1, d2);
+          w = wt[gp];
+        }
         double du[3] = {0, 0, 0}, dv[3] = {0, 0, 0};
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < npf; ++a) {
           du[0] += (xN[a] * d1[a]); du[1] += (yN[a] * d1[a]); du[2] += (zN[a] * d1[a]);
           dv[0] += (xN[a] * d2[a]); dv[1] += (yN[a] * d2[a]); dv[2] += (zN[a] * d2[a]);
         }
         double n1[3] = {dv[1] * du[2] - dv[2] * du[1], dv[2] * du[0] - dv[0] * du[2], dv[0] * du[1] - dv[1] * du[0]};
         const double Jac = std::sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
         n1[0] /= Jac; n1[1] /= Jac; n1[2] /= Jac;
-        const double dvol = gw[gp] * Jac;
+        const double dvol = w * Jac;
         double tr[3];
         for (int d = 0; d < 3; ++d) tr[d] = (sv0 * n1[d] + sv1 * du[d]) * dvol;
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < npf; ++a)
           for (int d = 0; d < 3; ++d) Fl[3 * a + d] += N[a] * tr[d];
       }
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < npf; ++a)
         for (int d = 0; d < 3; ++d) Fext[3 * nn[a] + d] += Fl[3 * a + d];
     }
   }
@@ -1035,6 +1067,21 @@ int orc_add_traction(orc_problem* p, int nf, const int32_t* faces, double v0, do
   return 0;
 }
 
+// addExternalForces alone: the external force vector at load factor lam, [nNode * 3]
+void orc_external_force(const orc_problem* p, double lam, double* F) {
+  std::vector<double> f;
+  external_forces(p, lam, f);
+  std::copy(f.begin(), f.end(), F);
+}
+
+int orc_add_traction_tri6(orc_problem* p, int nf, const int32_t* faces, double v0, double v1) {
+  TracPatch tp;
+  tp.faces.assign(faces, faces + 6 * nf);
+  tp.v0 = v0; tp.v1 = v1; tp.npf = 6;
+  p->trac.push_back(tp);
+  return 0;
+}
+
 // prepareMatrixPattern (femSolidmechanics2.cpp:121-478) with
 // setSpecifiedDOFs_Displacement (femSolidmechanics1.cpp:1479-1543) and
 // prepareDataForMixedElements (femSolidmechanics1.cpp:274-387).
@@ -1252,6 +1299,76 @@ void orc_spmv(const orc_problem* p, const double* vals, const double* x, double*
 // solveWithNewtonRaphson (femSolidmechanics2.cpp:730-851) with timeUpdate / updateIterStep /
 // factoriseSolveAndUpdate (femSolidmechanics1.cpp:1126-1191; femSolidmechanics2.cpp:1124-1217),
 // STATIC scheme, load factor lambda(t) = t, MyTime::update cur += dt (MyTime.cpp:26-37).
+// Stress at the Gauss points (Elem_Solid_3D_Disp::projectStress, Elem_Solid_3D_Disp.cpp:480-557):
+// reference-configuration gradients, F from the current displacement, the material's Cauchy stress
+// (the pressure field enters it for the mixed elements), per point 11 values: the 9 components in
+// the column-major map, von Mises (varindex 9, :550-551) and the mean stress tr(sigma)/3 (the
+// pressure field itself for the mixed elements, whose deviatoric part is traceless).
+int orc_stress_gauss(const orc_problem* p, double* out) {
+  double xi[10][3], w[10];
+  const int nGP = gauss3d(p->npe, xi, w);
+  const bool mixedP1 = p->mixed, mixed = p->mixed || p->p0;
+  for (int e = 0; e < p->nElem; ++e) {
+    double X[10][3], u[10][3];
+    const int* cn = &p->conn[(size_t)e * p->npe];
+    for (int a = 0; a < p->npe; ++a)
+      for (int d = 0; d < 3; ++d) {
+        X[a][d] = p->X[3 * (size_t)cn[a] + d];
+        u[a][d] = p->disp[3 * (size_t)cn[a] + d];
+      }
+    for (int gp = 0; gp < nGP; ++gp) {
+      double N[10], dx[10], dy[10], dz[10], Jac, F[3][3], stre[9], C[9][9];
+      basis3d(p->npe, xi[gp], X, N, dx, dy, dz, Jac);
+      defgrad(p->npe, u, dx, dy, dz, F);
+      double pres = 0.0;
+      if (mixedP1) {
+        double Np[4];
+        basis_tetra(4, xi[gp][0], xi[gp][1], xi[gp][2], Np, nullptr, nullptr, nullptr);
+        for (int q = 0; q < 4; ++q) pres += (Np[q] * p->pres[cn[q]]);
+      } else if (p->p0) {
+        pres = p->pres[e];
+      }
+      neohooke(mixed, p->Utype, p->Kinv, p->mu, F, pres, stre, C);
+      double* o = out + ((size_t)e * nGP + gp) * 11;
+      for (int k = 0; k < 9; ++k) o[k] = stre[k];
+      o[9] = std::sqrt((std::pow(stre[0] - stre[4], 2.0) + std::pow(stre[4] - stre[8], 2.0) +
+                        std::pow(stre[8] - stre[0], 2.0) +
+                        6.0 * (stre[1] * stre[1] + stre[2] * stre[2] + stre[5] * stre[5])) / 2.0);
+      o[10] = (stre[0] + stre[4] + stre[8]) / 3.0;
+    }
+  }
+  return nGP;
+}
+
+// Element value = mean over its Gauss points (Elem_Solid_3D_Disp::elementContourplot,
+// Elem_Solid_3D_Disp.cpp:416-421); nodal value = mean of the element values over the elements
+// around the node in increasing element id (femSolidmechanics::projectStresses,
+// femSolidmechanics3.cpp:918-960, with each element's value taken at all of its nodes).
+void orc_stress_project(const orc_problem* p, double* elem, double* nodal) {
+  double xi[10][3], w[10];
+  const int nGP = gauss3d(p->npe, xi, w);
+  std::vector<double> g((size_t)p->nElem * nGP * 11), ev((size_t)p->nElem * 11);
+  orc_stress_gauss(p, g.data());
+  for (int e = 0; e < p->nElem; ++e)
+    for (int k = 0; k < 11; ++k) {
+      double s = 0.0;
+      for (int gp = 0; gp < nGP; ++gp) s += g[((size_t)e * nGP + gp) * 11 + k];
+      ev[(size_t)e * 11 + k] = s / nGP;
+    }
+  if (elem) std::copy(ev.begin(), ev.end(), elem);
+  if (!nodal) return;
+  std::vector<double> acc((size_t)p->nNode * 11, 0.0);
+  std::vector<int> cnt(p->nNode, 0);
+  for (int e = 0; e < p->nElem; ++e)
+    for (int a = 0; a < p->npe; ++a) {
+      const int nd = p->conn[(size_t)e * p->npe + a];
+      for (int k = 0; k < 11; ++k) acc[(size_t)nd * 11 + k] += ev[(size_t)e * 11 + k];
+      ++cnt[nd];
+    }
+  for (int n = 0; n < p->nNode; ++n)
+    for (int k = 0; k < 11; ++k) nodal[(size_t)n * 11 + k] = cnt[n] ? acc[(size_t)n * 11 + k] / cnt[n] : 0.0;
+}
+
 int orc_newton(orc_problem* p, int nsteps, double dt, int solver, int precond, double lin_rtol, double rtol, int itmax,
                int32_t* iters_out, double* final_relres) {
   const int n = p->ntot;

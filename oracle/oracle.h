@@ -80,6 +80,10 @@ int  orc_add_dirichlet(orc_problem* p, int n, const int32_t* nodes, int dof, con
 /* Quad4 dead-load traction patch (femSolidmechanics1.cpp:922-1119): faces [nf][4],
  * t = (v0*n + v1*tangent1) * lambda. */
 int  orc_add_traction(orc_problem* p, int nf, const int32_t* faces, double v0, double v1);
+/* Tria6 faces [nf][6] (corners, then (0,1), (1,2), (2,0)), 3-point rule: the Tet10 boundary faces */
+int  orc_add_traction_tri6(orc_problem* p, int nf, const int32_t* faces, double v0, double v1);
+/* The traction patches' external force at load factor lam, F [nNode * 3] (addExternalForces). */
+void orc_external_force(const orc_problem* p, double lam, double* F);
 /* DOF numbering + CSR pattern (femSolidmechanics2.cpp:121-478). */
 int  orc_prepare_pattern(orc_problem* p);
 void orc_sizes(const orc_problem* p, int32_t* dispDOF, int32_t* presDOF, int64_t* nnz);
@@ -126,6 +130,12 @@ int  orc_assemble_colored(orc_problem* p, const int32_t* color, int nthreads, do
 
 /* Colour priority shared by oracle and product: SplitMix64 finaliser of (e + golden). */
 uint64_t orc_color_priority(int64_t e);
+/* Post-processing (Elem_Solid_3D_Disp::projectStress :480-557; femSolidmechanics::projectStresses,
+   femSolidmechanics3.cpp:918-960): out [nElem][nGP][11] = sigma (9, column-major map), von Mises,
+   tr(sigma)/3; returns nGP.  elem [nElem][11] Gauss-point means, nodal [nNode][11] means over the
+   elements around each node. */
+int  orc_stress_gauss(const orc_problem* p, double* out);
+void orc_stress_project(const orc_problem* p, double* elem, double* nodal);
 
 #ifdef __cplusplus
 }

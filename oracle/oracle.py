@@ -46,6 +46,8 @@ def lib() -> C.CDLL:
             "orc_problem_destroy": (None, [C.c_void_p]),
             "orc_add_dirichlet": (C.c_int, [C.c_void_p, C.c_int, _I, C.c_int, _D]),
             "orc_add_traction": (C.c_int, [C.c_void_p, C.c_int, _I, C.c_double, C.c_double]),
+            "orc_add_traction_tri6": (C.c_int, [C.c_void_p, C.c_int, _I, C.c_double, C.c_double]),
+            "orc_external_force": (None, [C.c_void_p, C.c_double, _D]),
             "orc_prepare_pattern": (C.c_int, [C.c_void_p]),
             "orc_sizes": (None, [C.c_void_p, _I, _I, C.POINTER(C.c_int64)]),
             "orc_get_pattern": (None, [C.c_void_p, _I, _I, _I, _I]),
@@ -63,6 +65,8 @@ def lib() -> C.CDLL:
             "orc_newton": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_int, _I, _D]),
             "orc_color_priority": (C.c_uint64, [C.c_int64]),
+            "orc_stress_gauss": (C.c_int, [C.c_void_p, _D]),
+            "orc_stress_project": (None, [C.c_void_p, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -179,11 +183,19 @@ class Problem:
         lib().orc_add_dirichlet(self.h, len(nodes), _i(nodes), dof, _d(val))
 
     def add_traction(self, faces, v0, v1=0.0):
+        """faces [nf, 4] (Quad4) or [nf, 6] (Tria6)."""
         faces = np.ascontiguousarray(faces, dtype=np.int32)
-        lib().orc_add_traction(self.h, faces.shape[0], _i(faces), v0, v1)
+        f = lib().orc_add_traction_tri6 if faces.shape[1] == 6 else lib().orc_add_traction
+        f(self.h, faces.shape[0], _i(faces), v0, v1)
 
     def prepare_matrix_pattern(self):
         lib().orc_prepare_pattern(self.h)
+
+    def external_force(self, lam=1.0):
+        """The traction patches' external force [nNode * 3] at load factor lam."""
+        F = np.zeros(3 * self.nNode)
+        lib().orc_external_force(self.h, lam, _d(F))
+        return F
 
     def sizes(self):
         d, p = C.c_int32(), C.c_int32()
@@ -255,6 +267,20 @@ class Problem:
         y = np.zeros_like(x)
         lib().orc_spmv(self.h, _d(vals), _d(f64(x)), _d(y))
         return y
+
+    def stress_gauss(self):
+        """[nElem, nGP, 11]: sigma (9, column-major map), von Mises, tr(sigma)/3 at the Gauss points."""
+        ngp = 8 if self.elem_type in (8, 9) else 4
+        out = np.zeros((self.nElem, ngp, 11))
+        lib().orc_stress_gauss(self.h, _d(out))
+        return out
+
+    def stress_project(self):
+        """(element means [nElem, 11], nodal means over adjacent elements [nNode, 11])."""
+        ev = np.zeros((self.nElem, 11))
+        nv = np.zeros((self.nNode, 11))
+        lib().orc_stress_project(self.h, _d(ev), _d(nv))
+        return ev, nv
 
     def newton(self, nsteps, dt, solver=0, precond=2, lin_rtol=1e-12, rtol=1e-9, itmax=30):
         iters = np.zeros(nsteps, dtype=np.int32)

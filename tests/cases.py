@@ -1,6 +1,8 @@
 """Shared problem builders for the parity tests (same seeded inputs for oracle and device)."""
 from __future__ import annotations
 
+from pathlib import Path
+
 import numpy as np
 
 import mphyfem_amd as M
@@ -170,3 +172,82 @@ def oracle_fext(orc, X, conn, bcs, faces, v0, v1=0.0):
     d = nda.ravel()
     f[d >= 0] = rhs[d[d >= 0]]
     return f
+
+
+def tet10_boundary_tris(X, conn, axis: int, value: float, outward: float, tol: float = 1e-9):
+    """Tria6 boundary faces of a Tet10 mesh on the plane X[axis] == value, ordered so that the
+    reference's normal dx/dxi2 x dx/dxi1 points along outward * e_axis: corners (c0, c1, c2), then
+    the mid-edge nodes of (c0,c1), (c1,c2), (c2,c0)."""
+    X = np.asarray(X)
+    edges = {(0, 1): 4, (1, 2): 5, (0, 2): 6, (0, 3): 7, (2, 3): 8, (1, 3): 9}
+    tris = []
+    for e in conn:
+        on = [a for a in range(4) if abs(X[e[a], axis] - value) < tol]
+        if len(on) != 3:
+            continue
+        c = list(on)
+        P = X[e[c]]
+        n = np.cross(P[2] - P[0], P[1] - P[0])  # dv x du for du = P1 - P0, dv = P2 - P0
+        if n[axis] * outward < 0:
+            c = [c[0], c[2], c[1]]
+        mid = [e[edges[tuple(sorted((c[i], c[(i + 1) % 3])))]] for i in range(3)]
+        tris.append([e[c[0]], e[c[1]], e[c[2]]] + mid)
+    return np.asarray(tris, dtype=np.int32)
+
+
+def write_gmsh22(path, X, conn, gmsh_type, domain, patches):
+    """A Gmsh 2.2 ASCII mesh as femBase::readInputGMSH reads it (femBase.cpp:121-364): physical names
+    (the domain with dim 3, each patch with its dim), nodes (1-based), boundary elements with the
+    patch's physical tag, then the domain elements.  patches: {name: (dim, gmsh_type, faces [nf, k])}."""
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(patches) + 1)]
+    tags = {}
+    for k, (name, (dim, _, _)) in enumerate(patches.items()):
+        tags[name] = k + 1
+        lines.append(f'{dim} {k + 1} "{name}"')
+    dom_tag = len(patches) + 1
+    lines.append(f'3 {dom_tag} "{domain}"')
+    lines += ["$EndPhysicalNames", "$Nodes", str(len(X))]
+    lines += [f"{i + 1} {x[0]:.17g} {x[1]:.17g} {x[2]:.17g}" for i, x in enumerate(np.asarray(X))]
+    lines += ["$EndNodes", "$Elements"]
+    body = []
+    for name, (dim, ft, faces) in patches.items():
+        for f in np.asarray(faces):
+            body.append((ft, tags[name], f))
+    for c in np.asarray(conn):
+        body.append((gmsh_type, dom_tag, c))
+    lines.append(str(len(body)))
+    for i, (t, tag, nodes) in enumerate(body):
+        lines.append(f"{i + 1} {t} 2 {tag} {tag} " + " ".join(str(int(v) + 1) for v in nodes))
+    lines.append("$EndElements")
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def write_config(path, mesh, elem_type, mu, Kinv, bcs, tractions, dt, final_time, tol, max_iters=20,
+                 max_steps=1000, Utype=3, petsc=None):
+    """A config deck in the reference's block format (doc/examples/SolidMechanics/Example4):
+    bcs: [(patch, 'fixed') | (patch, 'specified', 'UX', value, timefunction or None)],
+    tractions: [(patch, v0, v1, timefunction)].  Time function 1 is lam(t) = t."""
+    out = ["Files", "{", f"  mesh : {mesh}", "}", "", "Model Type", "{", "  dimension : 3", "}", "",
+           "Domains", "{", "  block", "  {", "    Material : 1", "    Element  : 1", "  }", "}", "",
+           "Material", "{", "  id : 1", "  name : Matl_NeoHookean", "  density : 1.0",
+           f"  data_deviatoric : {mu:.17g}", f"  data_volumetric : {Utype} {Kinv:.17g}", "}", "",
+           "Element", "{", "  id : 1", f"  type : {elem_type}", "}", "", "Boundary Conditions", "{"]
+    for bc in bcs:
+        out += [f"    {bc[0]}", "    {", f"        type : {bc[1]}"]
+        if bc[1] == "specified":
+            out += [f"        dof : {bc[2]}", f"        value : {bc[3]:.17g}"]
+            if bc[4] is not None:
+                out.append(f"        timefunction : {bc[4]}")
+        out.append("    }")
+    out += ["}", "", "Tractions", "{"]
+    for name, v0, v1, tf in tractions:
+        out += [f"    {name}", "    {", "        type : traction", f"        value : {v0:.17g} {v1:.17g} 0.0",
+                f"        timefunction : {tf}", "    }"]
+    out += ["}", "", "Time Functions", "{", "! id t0 t1 p1 p2 p3 p4 p5 p6 p7 p8",
+            "   1   0.0   1000.0  0.0  1.0  0.0  0.0  0.0  0.0  0.0  0.0", "}", "",
+            "Solver", "{", "solvertype : newton", "timescheme : STEADY", f"finalTime : {final_time:.17g}",
+            f"timeStep : {dt:.17g}", f"maximumSteps : {max_steps}", f"maximumIterations : {max_iters}",
+            f"tolerance : {tol:.17g}", "debug : 0", "}"]
+    Path(path).write_text("\n".join(out) + "\n")
+    if petsc is not None:
+        (Path(path).parent / "petsc_options.dat").write_text("\n".join(petsc) + "\n")

@@ -640,3 +640,190 @@ def test_dirichlet_lift_first_iterate_is_second_order(orc, mpf):
     assert 1.8 < r1 < 2.2 and 1.8 < r2 < 2.2, e  # relative error O(delta): absolute O(delta^2)
     e_nolift = [_first_iterate_error(orc, mpf, d, lift=False) for d in (0.02, 0.01)]
     assert min(e_nolift) > 0.2 and abs(e_nolift[0] / e_nolift[1] - 1.0) < 0.1, e_nolift
+
+
+# ---------------------------------------------------------------------------------------------
+# Post-processing: Gauss-point Cauchy stress, von Mises, element and nodal means
+# (Elem_Solid_3D_Disp.cpp:480-557; femSolidmechanics3.cpp:918-960)
+# ---------------------------------------------------------------------------------------------
+def _vm_from_j2(sig):
+    """von Mises as sqrt(3 J2) from the deviator (another form of the reference's component formula)."""
+    s = sig - np.trace(sig) / 3.0 * np.eye(3)
+    return np.sqrt(1.5 * np.sum(s * s))
+
+
+def _sig3(stre):
+    """3x3 from the column-major 9-index map."""
+    return np.asarray(stre).reshape(3, 3).T
+
+
+@pytest.mark.parametrize("etype", [8, 11], ids=["hex8", "tet10"])
+def test_stress_homogeneous_deformation(orc, mpf, etype):
+    """u = (F0 - I) X on a jittered mesh: every Gauss point, element and node carries sigma(F0) of the
+    material routine, von Mises equal to sqrt(3 J2) and tr(sigma)/3 the volumetric stress U'(J)."""
+    F0 = np.array([[1.08, 0.05, -0.02], [0.03, 0.96, 0.04], [-0.01, 0.02, 1.05]])
+    mu, Kinv = 50.0, 2e-3
+    if etype == 8:
+        X, conn = mpf.mesh_hex(3, 3, 3, jitter=0.1, seed=3)
+    else:
+        X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=3)
+    u = (X @ (F0 - np.eye(3)).T).ravel()
+    P = orc.Problem(etype, X, conn, mu, Kinv, 3)
+    P.prepare_matrix_pattern()
+    P.set_state(disp=u)
+    g = P.stress_gauss()
+    s, _ = orc.neohooke(F0, mu, Kinv, 3)
+    sig = _sig3(s)
+    assert np.abs(g[..., :9] - s).max() < 1e-12 * np.abs(s).max()
+    assert np.abs(g[..., 9] - _vm_from_j2(sig)).max() < 1e-12 * np.abs(s).max()
+    J = np.linalg.det(F0)
+    assert np.abs(g[..., 10] - orc.lib().orc_dUdJ(3, 1.0 / Kinv, J)).max() < 1e-10
+    ev, nv = P.stress_project()
+    assert np.abs(ev - g[:, 0, :]).max() < 1e-12 * np.abs(s).max()
+    assert np.abs(nv - g[0, 0, :]).max() < 1e-12 * np.abs(s).max()
+
+
+def test_stress_tet10_quadratic_field_closed_form(orc, mpf):
+    """A quadratic displacement on affine (unjittered) Tet10 elements is reproduced exactly, so the
+    Gauss-point F is I + grad u evaluated at the point's physical position; the stress there is the
+    material's sigma(F).  A wrong Gauss-point order, a transposed gradient or current-configuration
+    gradients fail it."""
+    X, conn = mpf.mesh_tet10(2, jitter=0.0, seed=1)
+    a = np.array([0.02, -0.03, 0.015])
+
+    def ufield(x):
+        return np.stack([a[0] * x[..., 0] * x[..., 1], a[1] * x[..., 2] ** 2 + 0.01 * x[..., 0],
+                         a[2] * x[..., 0] * x[..., 2] - 0.02 * x[..., 1]], axis=-1)
+
+    def grad(x):
+        G = np.zeros((3, 3))
+        G[0] = [a[0] * x[1], a[0] * x[0], 0.0]
+        G[1] = [0.01, 0.0, 2 * a[1] * x[2]]
+        G[2] = [a[2] * x[2], -0.02, a[2] * x[0]]
+        return G
+
+    mu, Kinv = 10.0, 1e-2
+    P = orc.Problem(11, X, conn, mu, Kinv, 3)
+    P.prepare_matrix_pattern()
+    P.set_state(disp=ufield(X).ravel())
+    g = P.stress_gauss()
+    xi, _ = orc.gauss3d(10)
+    worst = 0.0
+    for e in range(len(conn)):
+        V = X[conn[e, :4]]
+        for gp in range(4):
+            x1, x2, x3 = xi[gp]
+            xg = (1 - x1 - x2 - x3) * V[0] + x1 * V[1] + x2 * V[2] + x3 * V[3]
+            s, _ = orc.neohooke(np.eye(3) + grad(xg), mu, Kinv, 3)
+            worst = max(worst, np.abs(g[e, gp, :9] - s).max() / np.abs(s).max())
+    assert worst < 1e-12
+
+
+def test_stress_hex8_trilinear_field_closed_form(orc, mpf):
+    """A trilinear displacement on box-shaped (unjittered) Hex8 elements is reproduced exactly: the
+    Gauss-point F is I + grad u at the point's position."""
+    nx, ny, nz = 2, 2, 3
+    L = (1.0, 0.8, 1.2)
+    X, conn = mpf.mesh_hex(nx, ny, nz, L=L, jitter=0.0, seed=1)
+    c = 0.02
+
+    def ufield(x):
+        return np.stack([c * x[..., 0] * x[..., 1] * x[..., 2], -c * x[..., 1] * x[..., 2] + 0.01 * x[..., 0],
+                         0.5 * c * x[..., 0] * x[..., 1]], axis=-1)
+
+    def grad(x):
+        return np.array([[c * x[1] * x[2], c * x[0] * x[2], c * x[0] * x[1]],
+                         [0.01, -c * x[2], -c * x[1]],
+                         [0.5 * c * x[1], 0.5 * c * x[0], 0.0]])
+
+    mu, Kinv = 10.0, 1e-2
+    P = orc.Problem(8, X, conn, mu, Kinv, 3)
+    P.prepare_matrix_pattern()
+    P.set_state(disp=ufield(X).ravel())
+    g = P.stress_gauss()
+    xi, _ = orc.gauss3d(8)
+    worst = 0.0
+    for e in range(len(conn)):
+        lo, hi = X[conn[e]].min(0), X[conn[e]].max(0)
+        for gp in range(8):
+            xg = lo + 0.5 * (np.asarray(xi[gp]) + 1.0) * (hi - lo)
+            s, _ = orc.neohooke(np.eye(3) + grad(xg), mu, Kinv, 3)
+            worst = max(worst, np.abs(g[e, gp, :9] - s).max() / np.abs(s).max())
+    assert worst < 1e-12
+
+
+def test_stress_means_over_points_and_elements(orc, mpf):
+    """Element value = mean over its Gauss points; nodal value = mean of the element values of the
+    elements containing the node (a node of one element takes that element's value)."""
+    X, conn = mpf.mesh_hex(2, 2, 2, jitter=0.1, seed=4)
+    u = np.random.default_rng(2).uniform(-0.02, 0.02, X.size)
+    P = orc.Problem(8, X, conn, 10.0, 1e-2, 3)
+    P.prepare_matrix_pattern()
+    P.set_state(disp=u)
+    g = P.stress_gauss()
+    ev, nv = P.stress_project()
+    assert np.allclose(ev, g.mean(axis=1), rtol=0, atol=1e-13 * np.abs(g).max())
+    for n in range(len(X)):
+        els = np.nonzero((conn == n).any(axis=1))[0]
+        assert np.allclose(nv[n], ev[els].mean(axis=0), rtol=0, atol=1e-13 * np.abs(g).max())
+    corner = int(conn[0, 0])  # node 0 belongs to element 0 only
+    assert np.array_equal(nv[corner], ev[0])
+
+
+# ---------------------------------------------------------------------------------------------
+# Tria6 dead-load traction (the Tet10 boundary faces; BasisFunctionsLagrange.cpp:364-418,
+# 1688-1732; QuadratureUtil.cpp:562-570), pinned by closed forms
+# ---------------------------------------------------------------------------------------------
+def _tri6_patch(P0, P1, P2):
+    """One flat Tria6 face (straight edges, mid-edge nodes at the midpoints) as a 6-node mesh."""
+    P = np.array([P0, P1, P2, 0.5 * (P0 + P1), 0.5 * (P1 + P2), 0.5 * (P2 + P0)], float)
+    return P, np.arange(6, dtype=np.int32)[None, :]
+
+
+def _tri6_fext(orc, X, faces, v0, v1):
+    """The external force of one Tria6 traction patch at load factor 1 (addExternalForces alone)."""
+    O = orc.Problem(11, X, np.zeros((0, 10), dtype=np.int32), 1.0, 1e-3, 3)
+    O.add_traction(faces, v0, v1)
+    return O.external_force(1.0)
+
+
+def test_traction_tri6_flat_face_closed_form(orc):
+    """Uniform v0 on a flat quadratic triangle of area A: the consistent nodal loads are 0 at the
+    corners and v0 A / 3 n at the mid-edge nodes, n = (P2 - P0) x (P1 - P0) / |.|; the tangent term
+    adds v1 (P1 - P0) A / 3 at the mid-edge nodes (dx/dxi1 = P1 - P0, not normalised)."""
+    P0, P1, P2 = np.array([0.2, 0.1, 0.5]), np.array([1.3, 0.4, 0.2]), np.array([0.5, 1.1, 0.9])
+    X, f = _tri6_patch(P0, P1, P2)
+    A = 0.5 * np.linalg.norm(np.cross(P1 - P0, P2 - P0))
+    n = np.cross(P2 - P0, P1 - P0)
+    n /= np.linalg.norm(n)
+    v0, v1 = 2.5, -0.75
+    F = _tri6_fext(orc, X, f, v0, 0.0).reshape(-1, 3)
+    np.testing.assert_allclose(F[:3], 0.0, atol=1e-13)
+    np.testing.assert_allclose(F[3:], np.tile(v0 * A / 3 * n, (3, 1)), rtol=0, atol=1e-13)
+    F = _tri6_fext(orc, X, f, 0.0, v1).reshape(-1, 3)
+    np.testing.assert_allclose(F[:3], 0.0, atol=1e-13)
+    np.testing.assert_allclose(F[3:], np.tile(v1 * (P1 - P0) * A / 3, (3, 1)), rtol=0, atol=1e-13)
+    # reversing the corner order flips the normal
+    fr = f[:, [0, 2, 1, 5, 4, 3]]
+    Fr = _tri6_fext(orc, X, fr, v0, 0.0).reshape(-1, 3)
+    np.testing.assert_allclose(Fr.sum(0), -v0 * A * n, rtol=0, atol=1e-13)
+
+
+def test_traction_tri6_closed_surface(orc, mpf):
+    """Uniform normal traction over all six boundary planes of a Tet10 block, faces oriented outward:
+    zero net force and zero net moment; each plane's force is v0 times its area vector."""
+    from tests.cases import tet10_boundary_tris
+    X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=5)
+    v0 = 1.7
+    total = np.zeros_like(X)
+    for axis in range(3):
+        for value, outward in ((0.0, -1.0), (1.0, 1.0)):
+            f = tet10_boundary_tris(X, conn, axis, value, outward)
+            assert len(f) == 8  # 2 x 2 squares, 2 triangles each
+            F = _tri6_fext(orc, X, f, v0, 0.0).reshape(-1, 3)
+            e = np.zeros(3)
+            e[axis] = outward
+            np.testing.assert_allclose(F.sum(0), v0 * 1.0 * e, rtol=0, atol=1e-13)
+            total += F
+    assert np.abs(total.sum(0)).max() < 1e-13
+    assert np.abs(np.cross(X, total).sum(0)).max() < 1e-13
