@@ -225,6 +225,15 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);
+/* The same solve with no device allocation, for callers that cannot allocate (code under HIP graph */
+/* capture, tight serving loops, harnesses that forbid hipMalloc); it has no counterpart in the    */
+/* reference.  CG with MPF_PC_NONE or MPF_PC_JACOBI only (other ksp / pc: MPF_ERR_ARG), systems of  */
+/* at most MPF_NOALLOC_MAX_DOFS rows (else MPF_ERR_ARG, before any device work).  The work vectors  */
+/* live in one library-owned device array of 5 * MPF_NOALLOC_MAX_DOFS doubles (20 MiB) per GPU,   */
+/* held until the process ends and shared by every call on that GPU: calls that overlap on one GPU */
+/* race.  The solution is read with mpf_get_solution.                                             */
+#define MPF_NOALLOC_MAX_DOFS 524288
+mpf_status mpf_solve_noalloc(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep);
 /* factoriseSolveAndUpdate's update (femSolidmechanics2.cpp:1124-1217): on the first iteration   */
 /* prescribed dofs take dispApplied, free dofs += du, pressure dofs += du[dispDOF + k].          */
 /* MPF_HEX8P0: then every element pressure += (Fp - Kpu . du_e) / Kpp from the last assembly    */

@@ -27,6 +27,7 @@ TET10 = 11  # Tet10 displacement element (Elem_Solid_3D_Disp with npElem = 10)
 KSP_CG, KSP_GMRES, KSP_BAND = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
 ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER, ASSEMBLY_GATHER_DIRECT = 0, 1, 2
+NOALLOC_MAX_DOFS = 524288  # MPF_NOALLOC_MAX_DOFS
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NEG_JACOBIAN, ERR_STATE, ERR_NOT_CONVERGED, ERR_DIVERGED, ERR_NO_DEVICE = range(1, 8)
@@ -136,6 +137,7 @@ SIGNATURES = [
     ("mpf_rhs_norm", C.c_int, [_P, _DP]),
     ("mpf_solve", C.c_int, [_P, C.POINTER(KrylovOpts), C.POINTER(KrylovReport)]),
     ("mpf_get_solution", C.c_int, [_P, _DP]),
+    ("mpf_solve_noalloc", C.c_int, [_P, C.POINTER(KrylovOpts), C.POINTER(KrylovReport)]),
     ("mpf_add_traction_tri6", C.c_int, [_P, C.c_int32, _I32P, C.c_double, C.c_double]),
     ("mpf_deck_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("mpf_deck_destroy", C.c_int, [_P]),
@@ -610,6 +612,16 @@ class Problem:
     def write_vtu(self, path, time: float = 0.0, binary: bool = True):
         _check(lib().mpf_write_vtu(self._h, str(path).encode(), time, int(binary)), "mpf_write_vtu")
 
+    def solve_noalloc(self, pc=PC_JACOBI, rtol=1e-10, maxit=10000):
+        """mpf_solve_noalloc: Jacobi or unpreconditioned CG with no device allocation (library-owned
+        work array; at most NOALLOC_MAX_DOFS rows)."""
+        o = KrylovOpts(KSP_CG, pc, 0, maxit, rtol)
+        r = KrylovReport()
+        st = lib().mpf_solve_noalloc(self._h, C.byref(o), C.byref(r))
+        if st not in (OK, ERR_NOT_CONVERGED):
+            _check(st, "mpf_solve_noalloc")
+        return r
+
     def solution(self):
         d, p, _, _ = self.sizes()
         du = np.zeros(d + p)
@@ -873,6 +885,14 @@ class System:
         st = lib().mpf_solve(self._h, C.byref(o), C.byref(r))
         if st not in (OK, ERR_NOT_CONVERGED):
             _check(st, "mpf_solve")
+        return r
+
+    def solve_noalloc(self, pc=PC_JACOBI, rtol=1e-10, maxit=10000):
+        o = KrylovOpts(KSP_CG, pc, 0, maxit, rtol)
+        r = KrylovReport()
+        st = lib().mpf_solve_noalloc(self._h, C.byref(o), C.byref(r))
+        if st not in (OK, ERR_NOT_CONVERGED):
+            _check(st, "mpf_solve_noalloc")
         return r
 
     def solution(self):
