@@ -378,6 +378,33 @@ mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, con
                                   double rho, double* guess, int32_t* n_learned);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Arc-length continuation on the device: femSolidmechanics::solveWithArclength                   */
+/* (femSolidmechanics2.cpp:855-1039) with SolverPetsc::solveArclengthSystem (SolverPetsc.cpp:      */
+/* 671-757, psi = 1): each Newton iteration solves K du' = R and K du1 = Fext (one factorisation    */
+/* with MPF_KSP_BAND) and picks the load increment dl from the linearised arc-length sphere.  The   */
+/* load is the problem's traction patches (Fext at load factor 1); the first step is load-controlled */
+/* at lam = dt; the arc length is then the first step's and adapts as the reference's (doubles     */
+/* after two converged steps, 0.5 / 0.25 after a failure, within [ds1 / 1024, ds1]).  Displacement */
+/* elements with homogeneous prescribed values only (MPF_ERR_ARG otherwise: the reference's loop   */
+/* leaves pressures and prescribed increments untouched).  The state moves along the path.          */
+typedef struct {
+  double dt;            /* first load factor, and the time added per attempt (myTime.dt) */
+  double final_time;    /* attempts continue while the time is <= final_time */
+  int32_t max_steps;    /* stepsMax */
+  int32_t max_iters;    /* iterationsMax (Newton runs iter < max_iters, as the reference's loop) */
+  double tol;           /* absolute ||R|| */
+  mpf_krylov_opts lin;
+} mpf_arclength_opts;
+typedef struct {
+  int32_t steps_done, attempts, newton_iters, krylov_iters;
+  double final_lam, final_ds, seconds;
+} mpf_arclength_report;
+/* lam_out / ds_out [max_steps] (may be NULL): per converged step its load factor and arc length   */
+/* (0 for the first step).                                                                         */
+mpf_status mpf_arclength_solve(mpf_problem* p, const mpf_arclength_opts* o, double* lam_out, double* ds_out,
+                               mpf_arclength_report* rep);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Input decks: the reference's block-structured config file (femBase::readInput and its read*     */
 /* blocks, femBase.cpp:370-1200), the Gmsh 2.2 ASCII mesh its Files block names (readInputGMSH,    */
 /* femBase.cpp:121-364; looked up beside the config file, ".msh" appended) and petsc_options.dat    */

@@ -94,6 +94,17 @@ _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
 
+class ArclengthOpts(C.Structure):
+    _fields_ = [("dt", C.c_double), ("final_time", C.c_double), ("max_steps", C.c_int32), ("max_iters", C.c_int32),
+                ("tol", C.c_double), ("lin", KrylovOpts)]
+
+
+class ArclengthReport(C.Structure):
+    _fields_ = [("steps_done", C.c_int32), ("attempts", C.c_int32), ("newton_iters", C.c_int32),
+                ("krylov_iters", C.c_int32), ("final_lam", C.c_double), ("final_ds", C.c_double),
+                ("seconds", C.c_double)]
+
+
 class DeckInfo(C.Structure):
     _fields_ = [("ndim", C.c_int32), ("nNode", C.c_int32), ("nElem", C.c_int32), ("npe", C.c_int32),
                 ("gmsh_type", C.c_int32), ("elem_type", C.c_int32), ("npatch", C.c_int32), ("nbc", C.c_int32),
@@ -139,6 +150,7 @@ SIGNATURES = [
     ("mpf_get_solution", C.c_int, [_P, _DP]),
     ("mpf_solve_noalloc", C.c_int, [_P, C.POINTER(KrylovOpts), C.POINTER(KrylovReport)]),
     ("mpf_add_traction_tri6", C.c_int, [_P, C.c_int32, _I32P, C.c_double, C.c_double]),
+    ("mpf_arclength_solve", C.c_int, [_P, C.POINTER(ArclengthOpts), _DP, _DP, C.POINTER(ArclengthReport)]),
     ("mpf_deck_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("mpf_deck_destroy", C.c_int, [_P]),
     ("mpf_deck_get_info", C.c_int, [_P, C.POINTER(DeckInfo)]),
@@ -595,6 +607,17 @@ class Problem:
         if st not in (OK, ERR_NOT_CONVERGED):
             _check(st, "mpf_solve")
         return r
+
+    def solve_with_arclength(self, dt, final_time, max_steps, max_iters=20, tol=1e-9, ksp=KSP_BAND, pc=PC_BJACOBI3,
+                             lin_rtol=1e-12, maxit=20000):
+        """Arc-length continuation (femSolidmechanics::solveWithArclength); returns (load factors, arc
+        lengths, report) of the converged steps."""
+        o = ArclengthOpts(dt, final_time, max_steps, max_iters, tol, KrylovOpts(ksp, pc, 50, maxit, lin_rtol))
+        lam = np.zeros(max(max_steps, 1))
+        ds = np.zeros(max(max_steps, 1))
+        r = ArclengthReport()
+        _check(lib().mpf_arclength_solve(self._h, C.byref(o), _dp(lam), _dp(ds), C.byref(r)), "mpf_arclength_solve")
+        return lam[:r.steps_done], ds[:r.steps_done], r
 
     def stress_gauss(self):
         """[nElem, nGP, 11]: Cauchy stress (9, column-major map), von Mises, tr(sigma)/3."""

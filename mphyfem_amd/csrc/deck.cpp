@@ -479,7 +479,8 @@ const Patch* find_patch(const mpf_deck* d, const std::string& name) {
 // validate what the path supports and build the problem (prepareMatrixPattern)
 mpf_status build(mpf_deck* d) {
   if (d->ndim != 3) return fail(d, "only 3D decks run on this path"), MPF_ERR_ARG;
-  if (d->solvertype != "newton") return fail(d, "solvertype '" + d->solvertype + "': use newton here"), MPF_ERR_ARG;
+  if (d->solvertype != "newton" && d->solvertype != "arclength")
+    return fail(d, "solvertype '" + d->solvertype + "'"), MPF_ERR_ARG;
   if (d->body_force_nonzero) return fail(d, "a nonzero body force is not supported"), MPF_ERR_ARG;
   if (d->nodal_forces) return fail(d, "Nodal Forces are not supported"), MPF_ERR_ARG;
   if (d->domains.size() != 1) return fail(d, "exactly one domain is supported"), MPF_ERR_ARG;
@@ -526,6 +527,11 @@ mpf_status build(mpf_deck* d) {
   }
   d->load_tf = tfn == -2 ? -1 : tfn;
   if (d->load_tf >= (int)d->tfs.size()) return fail(d, "undefined time function"), MPF_ERR_ARG;
+  // arc-length: the reference forms Fext once, at time 0 (femSolidmechanics2.cpp:868), and the
+  // continuation scales it; here Fext is the tractions at load factor 1, so their time function
+  // must be 1 at time 0
+  if (d->solvertype == "arclength" && d->load_tf >= 0 && d->tfs[d->load_tf].value(0.0) != 1.0)
+    return fail(d, "arclength: the loads' time function must be 1 at time 0"), MPF_ERR_ARG;
   // the linear solver: petsc_options.dat, else the reference's code default CG + ILU
   // (SolverPetsc.cpp:117-140), here CG + 3x3 node-block Jacobi; an indefinite mixed system runs
   // GMRES, and the direct solve (displacement systems only) becomes GMRES at 1e-10 for it
@@ -671,6 +677,29 @@ mpf_status mpf_deck_run(mpf_deck* d, const char* vtu_prefix, mpf_deck_report* re
   std::memset(rep, 0, sizeof(*rep));
   MPF_TRY(mpf_deck_setup(d, nullptr));
   mpf_problem* p = d->p;
+  if (d->solvertype == "arclength") {  // solveWithArclength (femSolidmechanics2.cpp:855-1039)
+    mpf_arclength_opts ao{};
+    ao.dt = d->dt;
+    ao.final_time = d->finalTime;
+    ao.max_steps = d->maxSteps < (1 << 20) ? d->maxSteps : (1 << 20);
+    ao.max_iters = d->maxIters;
+    ao.tol = d->tol;
+    ao.lin = d->lin;
+    mpf_arclength_report ar{};
+    MPF_TRY(mpf_arclength_solve(p, &ao, nullptr, nullptr, &ar));
+    rep->steps_done = ar.steps_done;
+    rep->newton_iters = ar.newton_iters;
+    rep->krylov_iters = ar.krylov_iters;
+    rep->final_lam = ar.final_lam;
+    rep->converged = ar.steps_done > 0 ? 1 : 0;
+    rep->seconds = ar.seconds;
+    if (vtu_prefix) {
+      char fn[4096];
+      std::snprintf(fn, sizeof fn, "%s-%06d.vtu", vtu_prefix, ar.steps_done);
+      MPF_TRY(mpf_write_vtu(p, fn, ar.final_lam, 1));
+    }
+    return MPF_OK;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const double EPS = 1e-10;  // MathBasic.h:17
   const size_t n3 = 3 * (size_t)d->nNode;

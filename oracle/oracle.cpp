@@ -1369,6 +1369,103 @@ void orc_stress_project(const orc_problem* p, double* elem, double* nodal) {
     for (int k = 0; k < 11; ++k) nodal[(size_t)n * 11 + k] = cnt[n] ? acc[(size_t)n * 11 + k] / cnt[n] : 0.0;
 }
 
+// Arc-length continuation: femSolidmechanics::solveWithArclength (femSolidmechanics2.cpp:855-1039)
+// with SolverPetsc::solveArclengthSystem (SolverPetsc.cpp:671-757), timeUpdate's saveSolution
+// (SolutionDataSolid.cpp:170-184: dispPrev2 = dispPrev, dispPrev = disp at the start of every attempt,
+// so the displacement predictor reduces to the last state and only the load factor is extrapolated)
+// and the direct solve.  Fext is the traction patches' force at load factor 1; prescribed dofs stay
+// at zero (the loop never adds prescribed increments).  Per converged step: lam_out, the arc length
+// in force at that step ds_out (0 for the first), the displacement u_out [n3] (may be NULL); the
+// state after the last step stays in p->disp.
+// Returns the number of converged steps, or -1 when a solve fails.
+int orc_arclength(orc_problem* p, double dt, double timeFinal, int stepsMax, int iterationsMax, double conv_tol,
+                  double* lam_out, double* ds_out, double* u_out, int32_t* attempts_out) {
+  const double EPS = 1e-10;
+  const int n = p->ntot, n3 = 3 * p->nNode;
+  std::vector<double> Fext;
+  external_forces(p, 1.0, Fext);
+  double FtF = 0.0;
+  for (int i = 0; i < n3; ++i) FtF += Fext[i] * Fext[i];
+  std::vector<double> Fshort(n, 0.0), vals(p->colidx.size()), rhs(n), du1(n), du2(n), Dus(n);
+  for (int i = 0; i < p->dispDOF; ++i) Fshort[i] = Fext[p->assyForSoln[i]];
+  std::vector<double> dispPrev = p->disp, dispPrev2 = p->disp, DuFull(n3, 0.0);
+  double loadFactor = dt, loadFactorPrev = 0.0, loadFactorPrev2 = 0.0;
+  double arclenIncr = 0.0, arclenIncrPrev = 0.0, arclenIncrMax = 0.0, arclenIncrMin = 0.0;
+  bool convergedFlag = false, convergedFlagPrev = false;
+  double cur = 0.0;
+  int stepsCompleted = 1, attempts = 0;
+  std::fill(p->dispApplied.begin(), p->dispApplied.end(), 0.0);
+  while (cur <= timeFinal - EPS && stepsCompleted <= stepsMax) {
+    cur += dt;  // MyTime::update
+    ++attempts;
+    dispPrev2 = dispPrev;  // saveSolution
+    dispPrev = p->disp;
+    convergedFlagPrev = convergedFlag;
+    convergedFlag = false;
+    if (stepsCompleted > 1) {
+      const double DsFactor = arclenIncr / arclenIncrPrev;
+      for (int i = 0; i < n3; ++i) p->disp[i] = (1.0 + DsFactor) * dispPrev[i] - DsFactor * dispPrev2[i];
+      loadFactor = (1.0 + DsFactor) * loadFactorPrev - DsFactor * loadFactorPrev2;
+    }
+    for (int i = 0; i < n3; ++i) DuFull[i] = p->disp[i] - dispPrev[i];
+    double Dl = loadFactor - loadFactorPrev, dl = 0.0;
+    for (int iter = 1; iter < iterationsMax; ++iter) {
+      std::fill(vals.begin(), vals.end(), 0.0);
+      std::fill(rhs.begin(), rhs.end(), 0.0);
+      if (assemble_range(p, 0, p->nElem, iter == 1, vals.data(), rhs.data())) break;  // negative Jacobian
+      for (int i = 0; i < p->dispDOF; ++i) rhs[i] += loadFactor * Fext[p->assyForSoln[i]];
+      if (norm2(rhs.data(), n) < conv_tol) { convergedFlag = true; break; }
+      // solveArclengthSystem (psi = 1)
+      for (int i = 0; i < p->dispDOF; ++i) Dus[i] = DuFull[p->assyForSoln[i]];
+      double A = 0.0, b = 1.0;
+      const bool later = stepsCompleted > 1;
+      if (later) {
+        double dd = 0.0;
+        for (int i = 0; i < n; ++i) dd += Dus[i] * Dus[i];
+        A = dd + Dl * Dl * FtF - arclenIncr * arclenIncr;
+        b = 2.0 * Dl * FtF;
+      }
+      if (direct_solve(p, vals.data(), rhs.data(), du2.data())) return -1;
+      for (int i = 0; i < n; ++i) du2[i] = -du2[i];
+      if (direct_solve(p, vals.data(), Fshort.data(), du1.data())) return -1;
+      double adu2 = 0.0, adu1 = 0.0;
+      if (later)
+        for (int i = 0; i < n; ++i) { adu2 += 2.0 * Dus[i] * du2[i]; adu1 += 2.0 * Dus[i] * du1[i]; }
+      dl = (adu2 - A) / (b + adu1);
+      for (int i = 0; i < p->dispDOF; ++i) {
+        const double v = -du2[i] + dl * du1[i];
+        p->disp[p->assyForSoln[i]] += v;
+        DuFull[p->assyForSoln[i]] += v;
+      }
+      loadFactor += dl;
+      Dl += dl;
+    }
+    if (convergedFlag) {
+      if (stepsCompleted == 1) {
+        double dd = 0.0;
+        for (int i = 0; i < n3; ++i) dd += DuFull[i] * DuFull[i];
+        arclenIncr = std::sqrt(dd + loadFactor * loadFactor * FtF);
+        arclenIncrMax = arclenIncr;
+        arclenIncrMin = arclenIncr / 1024.0;
+      }
+      if (lam_out) lam_out[stepsCompleted - 1] = loadFactor;
+      if (ds_out) ds_out[stepsCompleted - 1] = stepsCompleted > 1 ? arclenIncr : 0.0;
+      if (u_out) std::copy(p->disp.begin(), p->disp.end(), u_out + (size_t)(stepsCompleted - 1) * n3);
+      loadFactorPrev2 = loadFactorPrev;
+      loadFactorPrev = loadFactor;
+      dispPrev2 = dispPrev;
+      dispPrev = p->disp;
+      arclenIncrPrev = arclenIncr;
+      if (convergedFlagPrev) arclenIncr = std::min(std::max(2.0 * arclenIncr, arclenIncrMin), arclenIncrMax);
+      ++stepsCompleted;
+    } else {
+      arclenIncr = std::max(arclenIncr * (convergedFlagPrev ? 0.5 : 0.25), arclenIncrMin);
+    }
+  }
+  if (attempts_out) *attempts_out = attempts;
+  return stepsCompleted - 1;
+}
+
 int orc_newton(orc_problem* p, int nsteps, double dt, int solver, int precond, double lin_rtol, double rtol, int itmax,
                int32_t* iters_out, double* final_relres) {
   const int n = p->ntot;

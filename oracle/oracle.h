@@ -135,6 +135,10 @@ uint64_t orc_color_priority(int64_t e);
    tr(sigma)/3; returns nGP.  elem [nElem][11] Gauss-point means, nodal [nNode][11] means over the
    elements around each node. */
 int  orc_stress_gauss(const orc_problem* p, double* out);
+/* Arc-length continuation (femSolidmechanics2.cpp:855-1039; SolverPetsc.cpp:671-757), direct solves:
+   per converged step the load factor and the arc length used; returns the converged steps. */
+int  orc_arclength(orc_problem* p, double dt, double timeFinal, int stepsMax, int iterationsMax, double conv_tol,
+                   double* lam_out, double* ds_out, double* u_out, int32_t* attempts_out);
 void orc_stress_project(const orc_problem* p, double* elem, double* nodal);
 
 #ifdef __cplusplus

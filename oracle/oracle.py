@@ -66,6 +66,8 @@ def lib() -> C.CDLL:
                                      C.c_int, _I, _D]),
             "orc_color_priority": (C.c_uint64, [C.c_int64]),
             "orc_stress_gauss": (C.c_int, [C.c_void_p, _D]),
+            "orc_arclength": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, _D, _D, _D,
+                                        _I]),
             "orc_stress_project": (None, [C.c_void_p, _D, _D]),
         }
         for name, (res, args) in sig.items():
@@ -267,6 +269,18 @@ class Problem:
         y = np.zeros_like(x)
         lib().orc_spmv(self.h, _d(vals), _d(f64(x)), _d(y))
         return y
+
+    def arclength(self, dt, final_time, max_steps, max_iters, tol):
+        """(load factors, arc lengths, displacements [k, nNode*3], attempts) of the k converged steps;
+        the state is the last step's."""
+        lam = np.zeros(max_steps)
+        ds = np.zeros(max_steps)
+        us = np.zeros((max_steps, 3 * self.nNode))
+        att = np.zeros(1, dtype=np.int32)
+        k = lib().orc_arclength(self.h, dt, final_time, max_steps, max_iters, tol, _d(lam), _d(ds), _d(us), _i(att))
+        if k < 0:
+            raise RuntimeError("orc_arclength: a solve failed")
+        return lam[:k], ds[:k], us[:k], int(att[0])
 
     def stress_gauss(self):
         """[nElem, nGP, 11]: sigma (9, column-major map), von Mises, tr(sigma)/3 at the Gauss points."""

@@ -223,10 +223,11 @@ def write_gmsh22(path, X, conn, gmsh_type, domain, patches):
 
 
 def write_config(path, mesh, elem_type, mu, Kinv, bcs, tractions, dt, final_time, tol, max_iters=20,
-                 max_steps=1000, Utype=3, petsc=None):
+                 max_steps=1000, Utype=3, petsc=None, solvertype="newton",
+                 timefunction="   1   0.0   1000.0  0.0  1.0  0.0  0.0  0.0  0.0  0.0  0.0"):
     """A config deck in the reference's block format (doc/examples/SolidMechanics/Example4):
     bcs: [(patch, 'fixed') | (patch, 'specified', 'UX', value, timefunction or None)],
-    tractions: [(patch, v0, v1, timefunction)].  Time function 1 is lam(t) = t."""
+    tractions: [(patch, v0, v1, timefunction)].  Time function 1 is lam(t) = t unless given."""
     out = ["Files", "{", f"  mesh : {mesh}", "}", "", "Model Type", "{", "  dimension : 3", "}", "",
            "Domains", "{", "  block", "  {", "    Material : 1", "    Element  : 1", "  }", "}", "",
            "Material", "{", "  id : 1", "  name : Matl_NeoHookean", "  density : 1.0",
@@ -244,8 +245,8 @@ def write_config(path, mesh, elem_type, mu, Kinv, bcs, tractions, dt, final_time
         out += [f"    {name}", "    {", "        type : traction", f"        value : {v0:.17g} {v1:.17g} 0.0",
                 f"        timefunction : {tf}", "    }"]
     out += ["}", "", "Time Functions", "{", "! id t0 t1 p1 p2 p3 p4 p5 p6 p7 p8",
-            "   1   0.0   1000.0  0.0  1.0  0.0  0.0  0.0  0.0  0.0  0.0", "}", "",
-            "Solver", "{", "solvertype : newton", "timescheme : STEADY", f"finalTime : {final_time:.17g}",
+            timefunction, "}", "",
+            "Solver", "{", f"solvertype : {solvertype}", "timescheme : STEADY", f"finalTime : {final_time:.17g}",
             f"timeStep : {dt:.17g}", f"maximumSteps : {max_steps}", f"maximumIterations : {max_iters}",
             f"tolerance : {tol:.17g}", "debug : 0", "}"]
     Path(path).write_text("\n".join(out) + "\n")

@@ -827,3 +827,48 @@ def test_traction_tri6_closed_surface(orc, mpf):
             total += F
     assert np.abs(total.sum(0)).max() < 1e-13
     assert np.abs(np.cross(X, total).sum(0)).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------------------------
+# Arc-length continuation (femSolidmechanics2.cpp:855-1039; SolverPetsc.cpp:671-757)
+# ---------------------------------------------------------------------------------------------
+def arclength_beam(orc, mpf):
+    """Cantilever 3 x 0.5 x 0.5 (6 x 2 x 2 Hex8, x = 0 clamped) under a dead shear traction on the tip
+    face; E = 1e4, nu = 0.3; the load factor's scale makes the path bend visibly within 8 steps."""
+    nx, ny, nz = 6, 2, 2
+    X, conn = mpf.mesh_hex(nx, ny, nz, L=(3.0, 0.5, 0.5), jitter=0.05, seed=3)
+    m = mpf.matl_from_E_nu(1e4, 0.3)
+    P = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
+    P.set_tangent(True)
+    fixed = mpf.face_nodes(mpf.HEX8, nx, ny, nz, 0)
+    quads = mpf.hex_face_quads(nx, ny, nz, 1)
+    P.add_dirichlet(fixed, -1)
+    P.add_traction(quads, 0.0, 400.0)
+    P.prepare_matrix_pattern()
+    return X, conn, m, fixed, quads, P
+
+
+def test_arclength_constraint_and_equilibrium(orc, mpf):
+    """Every converged step lies on the equilibrium path and on its arc-length sphere:
+    ||u_k - u_(k-1)||^2 + (lam_k - lam_(k-1))^2 F.F = ds_k^2 (psi = 1); the first step is load controlled
+    at lam = dt; the last state equals a plain Newton solve at the same load factor.  A dropped
+    F.F factor or a wrong sign in the load update fails the sphere; a wrong residual fails Newton."""
+    X, conn, m, fixed, quads, P = arclength_beam(orc, mpf)
+    F = P.external_force(1.0)
+    FtF = F @ F
+    lam, ds, us, att = P.arclength(0.1, 1.0, 8, 30, 1e-10)
+    assert len(lam) == 8 and att == 8 and lam[0] == 0.1
+    assert np.all(np.diff(lam) > 0.09) and lam[-1] > 0.8
+    for k in range(1, len(lam)):
+        du = us[k] - us[k - 1]
+        lhs = du @ du + (lam[k] - lam[k - 1]) ** 2 * FtF
+        assert abs(lhs - ds[k] ** 2) < 1e-8 * ds[k] ** 2, (k, lhs, ds[k] ** 2)
+    Q = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
+    Q.set_tangent(True)
+    Q.add_dirichlet(fixed, -1)
+    Q.add_traction(quads, 0.0, 400.0 * lam[-1])
+    Q.prepare_matrix_pattern()
+    st, _, _ = Q.newton(8, 0.125, solver=0, rtol=1e-10, itmax=30)
+    assert st == 0
+    uq, _ = Q.state()
+    assert normwise_rel(us[-1], uq) < 1e-9
