@@ -8,6 +8,11 @@ tangent + residual assembly (zeroMtx, every colour of the atomic-free element sc
 lift) into the CSR matrix, inputs resident in HBM.  After the timed steps the block-Jacobi GMRES
 solve of that system is run once and reported beside it.
 
+Beside the headline line the default run also reports "hex8" (the 128^3 Hex8 block's assembly, with
+its roofline) and "load_step" (seconds per converged load step on the 200 held-out cantilevers:
+plain, extrapolated, tangent-secant and learned starts, with training in the run); --no-secondary
+skips both.
+
 --workload hex8: the 128^3 Hex8 block (2 097 152 elements, nu = 0.49, bottom fixed, top UZ = -0.3),
 block-Jacobi PCG beside it.
 
@@ -94,7 +99,25 @@ def cpu_baseline(M, workload: str, budget_s: float):
     out["all_cores"] = {"value": round(done / dt / 1e6, 5), "unit": "Melem/s", "cores": nthr,
                         "sample": f"the same oracle loop over the {ncol} element colours with {nthr} OpenMP threads "
                                   f"inside a colour, {done} elements in {dt:.1f} s"}
+    out["host"] = host_cpu()
     return out
+
+
+def host_cpu() -> dict:
+    """nproc, the cores this process may use, and the CPU model (as lscpu's "Model name")."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "model": model}
 
 
 def build_problem_host(M, n, workload):
@@ -301,69 +324,10 @@ def copy_calibration() -> int:
     return 0
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tet10", "hex8", "hex8p0", "batch", "learned", "direct", "full"],
-                    default="tet10",
-                    help="tet10/hex8: assembly; batch: 256 independent Hex8 blocks, batched Newton + Jacobi-PCG; "
-                         "learned: cantilever beams, plain vs learned Newton initial guess; "
-                         "direct: the 200 held-out cantilevers, plain full Newton with the banded direct solve; "
-                         "full: 128^3 Hex8 nu=0.49 Newton load steps, plain vs extrapolated start")
-    ap.add_argument("--full-steps", type=int, default=2, help="full: load steps measured (of 15)")
-    ap.add_argument("--full-elem", choices=["hex8", "hex8p0"], default="hex8",
-                    help="full: displacement Hex8, or Hex8 Q1/P0 (plain start only)")
-    ap.add_argument("--full-modes", default="plain,extrap,learned",
-                    help="full: comma-separated start points to run (plain, extrap, learned)")
-    ap.add_argument("--nmesh", type=int, default=256, help="batch: number of blocks")
-    ap.add_argument("--beams", type=int, default=2000, help="learned: beams (the last --test are held out)")
-    ap.add_argument("--train", type=int, default=1800)
-    ap.add_argument("--test", type=int, default=200)
-    ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--residual-epochs", type=int, default=0,
-                    help="learned: extra epochs on the linearised start residual")
-    ap.add_argument("--solver", choices=["pcg", "band"], default=None,
-                    help="batch/learned: per-mesh linear solve, Jacobi-PCG (inexact Newton; batch default) or "
-                         "banded LDL^T (exact Newton corrections, the reference's plain method; learned default)")
-    ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
-    ap.add_argument("--krylov-maxit", type=int, default=300)
-    ap.add_argument("--no-krylov", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--assembly", choices=["gather", "direct", "scatter"], default="gather",
-                    help="row-owner gather assembly (default; Hex8 stages chunks of element records in LDS), "
-                         "the gather with every item reading the records directly, or coloured element scatter")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 1 GiB device-to-device 3 times (reference run for the HBM byte counters)")
-    a = ap.parse_args(argv)
-    if a.copy_calibration:
-        return copy_calibration()
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this path runs on one GPU only (no multi-GPU code); run with --gpus 1", file=sys.stderr)
-        return 2
-
+def assembly_bench(M, a, wl: str, n: int):
+    """The timed assembly steps of one workload: (bench line dict, problem, host data)."""
     import torch
 
-    import mphyfem_amd as M
-
-    if not torch.cuda.is_available() or M.device_count() == 0:
-        print("bench.py: no HIP device", file=sys.stderr)
-        return 1
-    torch.cuda.init()
-    wl = a.workload
-    if a.solver is None:
-        a.solver = "band" if wl == "learned" else "pcg"
-    if wl == "batch":
-        return bench_batch(M, a)
-    if wl == "learned":
-        return bench_learned(M, a)
-    if wl == "direct":
-        return bench_direct(M, a)
-    if wl == "full":
-        return bench_full(M, a)
-    n = a.n or (60 if wl == "tet10" else 128)
     if wl == "hex8p0":
         a.assembly = "scatter"  # Q1/P0 assembles through the coloured scatter only
     log(f"workload {wl}, n = {n}")
@@ -430,6 +394,77 @@ def main(argv=None) -> int:
            "config": {"workload": desc, "elements": nElem, "dofs": dispDOF + presDOF, "nnz": nnz,
                       "colours": ncolors, "pattern_seconds": round(t_pattern, 3)},
            "roofline": roofline}
+    return out, P, {"X": X, "conn": conn, "dispDOF": dispDOF, "presDOF": presDOF, "nnz": nnz}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["tet10", "hex8", "hex8p0", "batch", "learned", "direct", "full"],
+                    default="tet10",
+                    help="tet10/hex8: assembly; batch: 256 independent Hex8 blocks, batched Newton + Jacobi-PCG; "
+                         "learned: cantilever beams, plain vs learned Newton initial guess; "
+                         "direct: the 200 held-out cantilevers, plain full Newton with the banded direct solve; "
+                         "full: 128^3 Hex8 nu=0.49 Newton load steps, plain vs extrapolated start")
+    ap.add_argument("--full-steps", type=int, default=2, help="full: load steps measured (of 15)")
+    ap.add_argument("--full-elem", choices=["hex8", "hex8p0"], default="hex8",
+                    help="full: displacement Hex8, or Hex8 Q1/P0 (plain start only)")
+    ap.add_argument("--full-modes", default="plain,extrap,learned",
+                    help="full: comma-separated start points to run (plain, extrap, learned)")
+    ap.add_argument("--nmesh", type=int, default=256, help="batch: number of blocks")
+    ap.add_argument("--beams", type=int, default=2000, help="learned: beams (the last --test are held out)")
+    ap.add_argument("--train", type=int, default=1800)
+    ap.add_argument("--test", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--residual-epochs", type=int, default=0,
+                    help="learned: extra epochs on the linearised start residual")
+    ap.add_argument("--solver", choices=["pcg", "band"], default=None,
+                    help="batch/learned: per-mesh linear solve, Jacobi-PCG (inexact Newton; batch default) or "
+                         "banded LDL^T (exact Newton corrections, the reference's plain method; learned default)")
+    ap.add_argument("--n", type=int, default=0, help="cubes per edge (default 60 for tet10, 128 for hex8)")
+    ap.add_argument("--krylov-maxit", type=int, default=300)
+    ap.add_argument("--no-krylov", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="tet10: skip the Hex8 assembly and load-step objects beside the headline line")
+    ap.add_argument("--assembly", choices=["gather", "direct", "scatter"], default="gather",
+                    help="row-owner gather assembly (default; Hex8 stages chunks of element records in LDS), "
+                         "the gather with every item reading the records directly, or coloured element scatter")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device 3 times (reference run for the HBM byte counters)")
+    a = ap.parse_args(argv)
+    if a.copy_calibration:
+        return copy_calibration()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this path runs on one GPU only (no multi-GPU code); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import mphyfem_amd as M
+
+    if not torch.cuda.is_available() or M.device_count() == 0:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    torch.cuda.init()
+    wl = a.workload
+    if a.solver is None:
+        a.solver = "band" if wl == "learned" else "pcg"
+    if wl == "batch":
+        return bench_batch(M, a)
+    if wl == "learned":
+        return bench_learned(M, a)
+    if wl == "direct":
+        return bench_direct(M, a)
+    if wl == "full":
+        return bench_full(M, a)
+    n = a.n or (60 if wl == "tet10" else 128)
+    out, P, info = assembly_bench(M, a, wl, n)
+    X, conn = info["X"], info["conn"]
+    dispDOF, presDOF, nnz = info["dispDOF"], info["presDOF"], info["nnz"]
     if not a.no_krylov:
         if wl == "tet10":
             r = P.factorise_and_solve(ksp=M.KSP_GMRES, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit, restart=50)
@@ -452,8 +487,49 @@ def main(argv=None) -> int:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(M, wl, a.cpu_seconds)
     P.close()
+    if wl == "tet10" and not a.no_secondary:
+        # the full-scale Hex8 block's assembly beside the headline one
+        log("hex8 assembly, 128^3 block")
+        h, HP, _ = assembly_bench(M, a, "hex8", 128)
+        HP.close()
+        out["hex8"] = {k: h[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "roofline")}
+        # the second headline metric: seconds per converged load step, learned vs plain
+        log("load step: learned vs plain on the held-out cantilevers")
+        out["load_step"] = load_step_summary(M, a)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def load_step_summary(M, a) -> dict:
+    """Seconds per converged load step (||R|| / ||R0|| < 1e-9, R0 at the plain start) on the held-out
+    cantilevers: plain (previous converged state), linear extrapolation, the tangent-secant
+    predictor, and the learned guess (the network's correction on that predictor, guarded); full
+    Newton with the banded direct solve per beam; the network trained on the plain solver's own
+    trajectories of the training beams in the same run."""
+    from mphyfem_amd import learned
+
+    r = learned.experiment(nbeams=a.beams, ntrain=a.train, ntest=a.test, epochs=a.epochs,
+                           residual_epochs=a.residual_epochs, **dict(SOLVER, solver=M.BATCH_SOLVER_BAND))
+    nst = max(r["steps"] - 1, 1)
+    modes = [m for m in ("plain", "extrap", "tangent", "learned") if m in r]
+    out = {"metric": "s per converged load step, learned vs plain Newton start (||R||/||R0|| < 1e-9)",
+           "unit": "s", "higher_is_better": False,
+           "config": {"workload": f"{a.beams} Hex8 cantilevers H=8 W=4 L=round(U[4,12]*8), mu log-U[1,100], "
+                                  f"nu=0.3, tip shear ramped over {r['steps']} steps, seed 4; {a.train} trained on, "
+                                  f"{a.test} held out; steps 2-{r['steps']} timed",
+                      "linear_solver": "banded LDL^T per beam (full Newton)", "epochs": a.epochs}}
+    for m in modes:
+        out[m] = {"s_per_step": round(r[m]["seconds"] / nst, 4),
+                  "assemblies_per_mesh_step": r[m]["assemblies_per_mesh_step"],
+                  "batch_newton_iters": r[m]["batch_newton_iters"]}
+    out["value"] = out["learned"]["s_per_step"]
+    out["speedup_vs_plain"] = {m: round(r["plain"]["seconds"] / max(r[m]["seconds"], 1e-12), 3) for m in modes}
+    if "tangent" in r:
+        out["learned_vs_tangent"] = round(r["tangent"]["seconds"] / max(r["learned"]["seconds"], 1e-12), 3)
+    out["meshes_keeping_correction_per_step"] = r["learned"]["meshes_keeping_correction_per_step"]
+    out["train_seconds"] = r.get("train_seconds")
+    out["loss_first_last"] = [r.get("loss_first"), r.get("loss_last")]
+    return out
 
 
 if __name__ == "__main__":

@@ -364,10 +364,14 @@ mpf_status newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o, con
       if (iters) *iters = iter;
       return st;
     }
+    if (std::isnan(rn) || std::isnan(R0)) {  // a NaN residual never passes the convergence test
+      if (iters) *iters = iter;
+      return MPF_ERR_DIVERGED;
+    }
     if (R0 < 0) R0 = rn;
     lastrel = R0 > 0 ? rn / R0 : 0.0;
     if (R0 == 0.0 || lastrel < o->rtol) { conv = true; break; }
-    if (std::isnan(rn) || (iter > 3 && rnPrev > 0 && rn / rnPrev > 1.0e7)) {
+    if (iter > 3 && rnPrev > 0 && rn / rnPrev > 1.0e7) {
       if (iters) *iters = iter;
       return MPF_ERR_DIVERGED;
     }

@@ -920,7 +920,7 @@ int direct_solve(const orc_problem* p, const double* vals, const double* b, doub
     double dj = L[(size_t)j * W + bw];
     const int k0 = std::max(0, j - bw);
     for (int k = k0; k < j; ++k) { const double l = L[(size_t)j * W + bw + k - j]; dj -= l * l * D[k]; }
-    if (!(dj > 0.0)) return 1;
+    if (dj == 0.0 || !std::isfinite(dj)) return 1;  // LDL^T without pivoting: negative pivots are fine
     D[j] = dj;
     const int iend = std::min(n - 1, j + bw);
     for (int i = j + 1; i <= iend; ++i) {

@@ -80,15 +80,19 @@ def test_deck_tet10p1_example4_vs_oracle(mpf, orc, gpu, tmp_path):
 
 def test_deck_arclength_vs_oracle(mpf, orc, gpu, tmp_path):
     """solvertype arclength: the deck's cantilever follows the oracle's arc-length path (the loads'
-    time function is 1, so Fext formed at time 0 is the full traction)."""
+    time function is 1, so Fext formed at time 0 is the full traction).  Decks run the reference's
+    element tangent, so the oracle does too: the same steps, cuts and load factors."""
     from tests.test_oracle import arclength_beam
-    X, conn, m, fixed, quads, O = arclength_beam(orc, mpf)
-    lam_o, _, us_o, _ = O.arclength(0.1, 1.0, 8, 30, 1e-10)
+    X, conn, m, fixed, quads, O = arclength_beam(orc, mpf, consistent=False)
+    # 6 converged steps (the reference tangent's Newton misses some attempts): the run ends on a
+    # converged step, so the final states compare
+    lam_o, _, us_o, _ = O.arclength(0.1, 1.0, 6, 30, 1e-10)
+    assert len(lam_o) == 6
     nx, ny, nz = 6, 2, 2
     patches = {"clamp": (2, 3, mpf.hex_face_quads(nx, ny, nz, 0)), "tip": (2, 3, quads)}
     write_gmsh22(tmp_path / "beam.msh", X, conn, 5, "block", patches)
     write_config(tmp_path / "config", "beam", "ELEM_SOLID_3D_DISP", m.mu, m.Kinv, [("clamp", "fixed")],
-                 [("tip", 0.0, 400.0, 1)], 0.1, 1.0, 1e-10, max_iters=30, max_steps=8, solvertype="arclength",
+                 [("tip", 0.0, 400.0, 1)], 0.1, 1.0, 1e-10, max_iters=30, max_steps=6, solvertype="arclength",
                  timefunction="   1   0.0   1000.0  1.0  0.0  0.0  0.0  0.0  0.0  0.0  0.0",
                  petsc=["-ksp_type preonly", "-pc_type lu"])
     D = mpf.Deck(tmp_path / "config")
@@ -96,7 +100,5 @@ def test_deck_arclength_vs_oracle(mpf, orc, gpu, tmp_path):
     rep = D.run()
     assert rep.steps_done == len(lam_o) and abs(rep.final_lam - lam_o[-1]) < 1e-9 * lam_o[-1]
     u, _ = P.state()
-    # the deck runs the reference's element tangent; the oracle's path used the consistent one: the
-    # converged states on the path are the same within the Newton tolerance
-    assert normwise_rel(u, us_o[-1]) < 1e-8
+    assert normwise_rel(u, us_o[-1]) < 1e-9
     D.close()

@@ -832,14 +832,14 @@ def test_traction_tri6_closed_surface(orc, mpf):
 # ---------------------------------------------------------------------------------------------
 # Arc-length continuation (femSolidmechanics2.cpp:855-1039; SolverPetsc.cpp:671-757)
 # ---------------------------------------------------------------------------------------------
-def arclength_beam(orc, mpf):
+def arclength_beam(orc, mpf, consistent=True):
     """Cantilever 3 x 0.5 x 0.5 (6 x 2 x 2 Hex8, x = 0 clamped) under a dead shear traction on the tip
     face; E = 1e4, nu = 0.3; the load factor's scale makes the path bend visibly within 8 steps."""
     nx, ny, nz = 6, 2, 2
     X, conn = mpf.mesh_hex(nx, ny, nz, L=(3.0, 0.5, 0.5), jitter=0.05, seed=3)
     m = mpf.matl_from_E_nu(1e4, 0.3)
     P = orc.Problem(8, X, conn, m.mu, m.Kinv, 3)
-    P.set_tangent(True)
+    P.set_tangent(consistent)
     fixed = mpf.face_nodes(mpf.HEX8, nx, ny, nz, 0)
     quads = mpf.hex_face_quads(nx, ny, nz, 1)
     P.add_dirichlet(fixed, -1)
