@@ -203,8 +203,19 @@ mpf_status mpf_rhs_norm(mpf_problem* p, double* norm);
 /* pc, restart, maxit, rtol are ignored;                                                          */
 /* a zero or non-finite pivot gives MPF_ERR_DIVERGED.  The half-bandwidth after renumbering must   */
 /* stay below about 490 dofs (MPF_ERR_ARG).                                                      */
+/* MPF_PC_PMG (MPF_KSP_GMRES only; MPF_TET10 and MPF_TET10P1 problems, else MPF_ERR_ARG): flexible */
+/* GMRES(m) around a two-level p-multigrid preconditioner.  Displacement rows: one V-cycle with     */
+/* damped 3x3 node-block Jacobi smoothing on the quadratic dofs and a coarse correction on the      */
+/* vertex dofs, whose matrix is the Galerkin product P^T Kuu P (P: the Tet10 -> Tet4 interpolation, */
+/* a mid-edge node taking the mean of its two vertices) solved by block-Jacobi CG to a loose        */
+/* tolerance.  Pressure rows (MPF_TET10P1): block upper-triangular, the pressure block being        */
+/* diag(Kpp) (1 + 1 / (mu Kinv)) (the pressure mass matrix scaled to the Schur complement of a      */
+/* nearly incompressible material; Kinv = 0: the node-block Schur diagonal).  It stands where       */
+/* petsc_options select a multigrid or field-split preconditioner (-pc_type mg / gamg / fieldsplit)*/
+/* for the reference's KSPSolve (SolverPetsc.cpp:117-140, 290); the code defaults there are CG and  */
+/* ILU.  mpf_set_pmg changes its parameters, mpf_pmg_stats reports its last use.                    */
 enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1, MPF_KSP_BAND = 2 };
-enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2 };
+enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2, MPF_PC_PMG = 3 };
 typedef struct {
   int32_t ksp;       /* MPF_KSP_* */
   int32_t pc;        /* MPF_PC_* */
@@ -224,6 +235,27 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* independent blocks, largest half-bandwidth, free dofs, and the factorisation's algorithmic      */
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
+/* Two-level preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before   */
+/* and after the coarse correction (1..8, default 2), the ratio beta / alpha of its interval        */
+/* (1 < ratio < 1e6, default 10; beta = 1.1 x the power-iteration estimate of the largest           */
+/* eigenvalue of the node-block-Jacobi-preconditioned Kuu), the coarse CG's relative tolerance      */
+/* (0 < coarse_rtol < 1, default 0.1) and iteration limit (default 1000).  No counterpart in the    */
+/* reference.                                                                                        */
+mpf_status mpf_set_pmg(mpf_problem* p, int32_t degree, double ratio, double coarse_rtol, int32_t coarse_maxit);
+/* Its coarse level and its work since the last solve began: coarse dofs and nonzeros, V-cycles   */
+/* applied, coarse CG iterations in total, the set-up time of the last solve (transfer tables on  */
+/* first use; the Galerkin product, the block inverses and the eigenvalue estimate every solve)    */
+/* and that estimate.  No counterpart in the                                                       */
+/* reference (reporting only).                                                                     */
+typedef struct {
+  int64_t coarse_dofs;
+  int64_t coarse_nnz;
+  int64_t vcycles;
+  int64_t coarse_iterations;
+  double setup_seconds;
+  double lambda_max;    /* the smoother's eigenvalue estimate */
+} mpf_pmg_report;
+mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep);
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);
 /* The same solve with no device allocation, for callers that cannot allocate (code under HIP graph */
 /* capture, tight serving loops, harnesses that forbid hipMalloc); it has no counterpart in the    */

@@ -25,7 +25,7 @@ HEX8P0 = 9  # Hex8 Q1/P0: one element pressure, condensed
 TET10P1 = 10
 TET10 = 11  # Tet10 displacement element (Elem_Solid_3D_Disp with npElem = 10)
 KSP_CG, KSP_GMRES, KSP_BAND = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_BJACOBI3 = 0, 1, 2
+PC_NONE, PC_JACOBI, PC_BJACOBI3, PC_PMG = 0, 1, 2, 3
 ASSEMBLY_GATHER, ASSEMBLY_COLORED_SCATTER, ASSEMBLY_GATHER_DIRECT = 0, 1, 2
 NOALLOC_MAX_DOFS = 524288  # MPF_NOALLOC_MAX_DOFS
 
@@ -54,6 +54,11 @@ class Matl(C.Structure):
 class KrylovOpts(C.Structure):
     _fields_ = [("ksp", C.c_int32), ("pc", C.c_int32), ("restart", C.c_int32), ("maxit", C.c_int32),
                 ("rtol", C.c_double)]
+
+
+class PmgReport(C.Structure):
+    _fields_ = [("coarse_dofs", C.c_int64), ("coarse_nnz", C.c_int64), ("vcycles", C.c_int64),
+                ("coarse_iterations", C.c_int64), ("setup_seconds", C.c_double), ("lambda_max", C.c_double)]
 
 
 class KrylovReport(C.Structure):
@@ -190,6 +195,8 @@ SIGNATURES = [
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
     ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
+    ("mpf_set_pmg", C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_int32]),
+    ("mpf_pmg_stats", C.c_int, [_P, C.POINTER(PmgReport)]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
                             C.c_uint64, _DP, _I32P]),
@@ -689,6 +696,17 @@ class Problem:
         nb, bm, nd, fl = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
         _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
         return nb.value, bm.value, nd.value, fl.value
+
+    def set_pmg(self, degree=2, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000):
+        """Parameters of the two-level preconditioner PC_PMG (Chebyshev smoothing degree and interval
+        ratio, the coarse CG's relative tolerance and iteration limit)."""
+        _check(lib().mpf_set_pmg(self._h, degree, ratio, coarse_rtol, coarse_maxit), "mpf_set_pmg")
+
+    def pmg_stats(self) -> PmgReport:
+        """Coarse level size and the V-cycles / coarse CG iterations of the last PC_PMG solve."""
+        r = PmgReport()
+        _check(lib().mpf_pmg_stats(self._h, C.byref(r)), "mpf_pmg_stats")
+        return r
 
     def assembly_mode(self) -> int:
         v = C.c_int32()

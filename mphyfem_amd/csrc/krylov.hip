@@ -1,5 +1,6 @@
 // krylov.hip -- FP64 CSR SpMV, deterministic reductions, Jacobi / node-block-Jacobi
-// preconditioners, PCG and GMRES(m) on gfx950.
+// preconditioners, PCG, GMRES(m), and flexible GMRES(m) around the two-level preconditioner of
+// pmg.hip, on gfx950.
 //
 // Stands in for PETSc's KSPSolve behind SolverPetsc::factoriseAndSolve (SolverPetsc.cpp:250-330):
 // zero initial guess (VecZeroEntries(solnVec), :290), stop on ||b - A x|| / ||b|| < rtol.
@@ -29,6 +30,8 @@ struct KrylovWork {
   DevBuf<double> sc;          // device scalars
   DevBuf<double> V;           // GMRES basis [(m+1) * n]
   int vcap = 0;
+  DevBuf<double> Z;           // flexible GMRES: preconditioned basis [m * n]
+  int zcap = 0;
   // preconditioner blocks
   DevBuf<int32_t> blk_start;
   DevBuf<int32_t> blk_size;
@@ -43,6 +46,7 @@ namespace {
 
 // scalar slots
 enum { S_RZ0 = 0, S_RZ1 = 1, S_PQ = 2, S_RR = 3, S_BB = 4, S_DONE = 5, S_ITS = 6, S_TOL2 = 7, S_MAXIT = 8, S_N = 16 };
+static_assert(S_N == kCgScalars, "CG scalar slots");
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -510,27 +514,52 @@ mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals
   return MPF_OK;
 }
 
-// y = K x for the Krylov solvers: the row-group kernel when the node blocks share column lists,
-// else the per-row kernel.  part: partial x.y per workgroup (np of them) or null.
-static mpf_status spmv_k(const mpf_problem* p, const KrylovWork& W, const double* x, double* y, double* part,
-                         unsigned np, const double* sc, hipStream_t s) {
-  const int64_t n = W.n, nnz = p->pat.nnz;
-  if (!W.grp_spmv)
-    return spmv(p->pat.rowptr.ptr, p->pat.colidx.ptr, p->vals.ptr, n, nnz, x, y, part, np, sc, s);
+// y = A x: the row-group kernel when the blocks share column lists, else the per-row kernel.
+// part: partial x.y per workgroup (np of them) or null.
+static mpf_status spmv_view(const CsrView& A, const BlockPcView& B, const double* x, double* y, double* part,
+                            unsigned np, const double* sc, hipStream_t s) {
+  const int64_t n = A.n, nnz = A.nnz;
+  if (!B.grp) return spmv(A.rowptr, A.colidx, A.vals, n, nnz, x, y, part, np, sc, s);
   const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
-  const int32_t *bs = W.blk_start.ptr, *bm = W.blk_size.ptr, *rp = p->pat.rowptr.ptr, *ci = p->pat.colidx.ptr;
+  const int32_t *bs = B.bs, *bm = B.bm, *rp = A.rowptr, *ci = A.colidx;
   if (avg > 48) {
-    const unsigned g = part ? np : grid_for(W.nblk, kRB / 32);
-    hipLaunchKernelGGL(k_spmv_grp<32>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+    const unsigned g = part ? np : grid_for(B.nblk, kRB / 32);
+    hipLaunchKernelGGL(k_spmv_grp<32>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
   } else if (avg > 12) {
-    const unsigned g = part ? np : grid_for(W.nblk, kRB / 16);
-    hipLaunchKernelGGL(k_spmv_grp<16>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+    const unsigned g = part ? np : grid_for(B.nblk, kRB / 16);
+    hipLaunchKernelGGL(k_spmv_grp<16>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
   } else {
-    const unsigned g = part ? np : grid_for(W.nblk, kRB / 4);
-    hipLaunchKernelGGL(k_spmv_grp<4>, dim3(g), dim3(kRB), 0, s, W.nblk, bs, bm, rp, ci, p->vals.ptr, x, y, part, sc);
+    const unsigned g = part ? np : grid_for(B.nblk, kRB / 4);
+    hipLaunchKernelGGL(k_spmv_grp<4>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
   }
   MPF_HIP(hipGetLastError());
   return MPF_OK;
+}
+
+static CsrView csr_of(const mpf_problem* p) {
+  CsrView A;
+  A.n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
+  A.nnz = p->pat.nnz;
+  A.rowptr = p->pat.rowptr.ptr;
+  A.colidx = p->pat.colidx.ptr;
+  A.vals = p->vals.ptr;
+  return A;
+}
+
+static BlockPcView blocks_of(const KrylovWork& W) {
+  BlockPcView B;
+  B.nblk = W.nblk;
+  B.bs = W.blk_start.ptr;
+  B.bm = W.blk_size.ptr;
+  B.rblk = W.row_blk.ptr;
+  B.binv = W.blk_inv.ptr;
+  B.grp = W.grp_spmv;
+  return B;
+}
+
+static mpf_status spmv_k(const mpf_problem* p, const KrylovWork& W, const double* x, double* y, double* part,
+                         unsigned np, const double* sc, hipStream_t s) {
+  return spmv_view(csr_of(p), blocks_of(W), x, y, part, np, sc, s);
 }
 
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch) {
@@ -541,6 +570,36 @@ mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s
   MPF_HIP(hipMemcpyAsync(&h, scratch + kMaxParts, sizeof(double), hipMemcpyDeviceToHost, s));
   MPF_HIP(hipStreamSynchronize(s));
   *out_host = std::sqrt(h);
+  return MPF_OK;
+}
+
+mpf_status block_pc_structure(const CsrView& A, int64_t nblk, const int32_t* bs, const int32_t* bm, int32_t* rblk,
+                              bool* grp, hipStream_t s) {
+  const int B = 256;
+  if (A.n > 0) MPF_HIP(hipMemsetAsync(rblk, 0, (size_t)A.n * sizeof(int32_t), s));
+  if (nblk > 0) hipLaunchKernelGGL(k_row_blk, dim3(grid_for(nblk, B)), dim3(B), 0, s, nblk, bs, bm, rblk);
+  *grp = false;
+  if (nblk > 0) {
+    DevBuf<int32_t> bad;
+    MPF_TRY(bad.alloc(1));
+    int32_t hbad = 0;
+    MPF_HIP(hipMemcpyAsync(bad.ptr, &hbad, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_check_groups, dim3(grid_for(nblk, B)), dim3(B), 0, s, nblk, bs, bm, A.rowptr, A.colidx,
+                       bad.ptr);
+    MPF_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    *grp = hbad == 0;
+  }
+  MPF_HIP(hipGetLastError());
+  return MPF_OK;
+}
+
+mpf_status block_pc_values(const CsrView& A, int64_t nblk, const int32_t* bs, const int32_t* bm, double* binv,
+                           hipStream_t s) {
+  if (nblk > 0)
+    hipLaunchKernelGGL(k_block_setup, dim3(grid_for(nblk, 256)), dim3(256), 0, s, nblk, bs, bm, A.rowptr, A.colidx,
+                       A.vals, binv, 0);
+  MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
 
@@ -563,8 +622,12 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
     MPF_TRY(W.V.alloc((size_t)(restart + 1) * n));
     W.vcap = restart + 1;
   }
+  if (pc == MPF_PC_PMG && W.zcap < restart) {
+    MPF_TRY(W.Z.alloc((size_t)restart * n));
+    W.zcap = restart;
+  }
   // block list
-  const int kind = (pc == MPF_PC_BJACOBI3) ? 2 : 1;
+  const int kind = (pc == MPF_PC_BJACOBI3 || pc == MPF_PC_PMG) ? 2 : 1;
   if (W.pc_kind != kind) {
     const int B = 256;
     if (kind == 1) {
@@ -623,38 +686,38 @@ static mpf_status setup_work(mpf_problem* p, int pc, int restart) {
                        W.blk_size.ptr, nub, W.blk_inv.ptr);
   }
   MPF_HIP(hipGetLastError());
+  if (pc == MPF_PC_PMG) {
+    if (!W.grp_spmv) return MPF_ERR_ARG;  // node rows that do not share a column list: not an element pattern
+    MPF_TRY(pmg_setup(p, blocks_of(W), W.nblk - p->pat.presDOF));
+  }
   return MPF_OK;
 }
 
-static mpf_status pcg(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep) {
-  KrylovWork& W = *p->kw;
-  hipStream_t s = p->stream;
-  const int64_t n = W.n;
-  const unsigned np = nparts_for(W.nblk);
+mpf_status pcg_csr(const CsrView& A, const BlockPcView& B, const CgScratch& w, const double* b, double* x,
+                   double rtol, int maxit, hipStream_t s, mpf_krylov_report* rep) {
+  const int64_t n = A.n;
+  const unsigned np = nparts_for(B.nblk);
   const unsigned nps = nparts_for(n);
-  double* sc = W.sc.ptr;
-  double* x = p->du.ptr;
-  const double* b = p->rhs.ptr;
-  hipLaunchKernelGGL(k_cg_start, dim3(np), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr, W.blk_inv.ptr, b,
-                     x, W.r.ptr, W.z.ptr, W.p.ptr, W.part.ptr, (int)np);
+  double* sc = w.sc;
+  hipLaunchKernelGGL(k_cg_start, dim3(np), dim3(kRB), 0, s, B.nblk, B.bs, B.bm, B.binv, b, x, w.r, w.z, w.p, w.part,
+                     (int)np);
   // rz -> sc[S_RZ0], bb -> sc[S_RR]
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)np, (int)np, 1, sc + S_RZ0);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr + np, (int)np, (int)np, 1, sc + S_RR);
-  hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(64), 0, s, sc, o->rtol, o->maxit);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, w.part, (int)np, (int)np, 1, sc + S_RZ0);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, w.part + np, (int)np, (int)np, 1, sc + S_RR);
+  hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(64), 0, s, sc, rtol, maxit);
   int it = 0;
   int chunk = 8;
   double hs[S_N];
   for (;;) {
     for (int c = 0; c < chunk; ++c, ++it) {
       const int par = it & 1;
-      MPF_TRY(spmv_k(p, W, W.p.ptr, W.q.ptr, W.part.ptr, nps, sc, s));
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, (int)nps, 1, sc + S_PQ);
-      double* rb[2] = {W.r.ptr, W.r2.ptr};
-      hipLaunchKernelGGL(k_cg_update_rows, dim3(nps), dim3(kRB), 0, s, n, W.row_blk.ptr, W.blk_start.ptr,
-                         W.blk_size.ptr, W.blk_inv.ptr, x, rb[par], rb[1 - par], W.z.ptr, W.p.ptr, W.q.ptr, W.part.ptr,
-                         (int)nps, sc, par);
-      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, sc, par, it);
-      hipLaunchKernelGGL(k_cg_p, dim3(nps), dim3(kRB), 0, s, n, W.z.ptr, W.p.ptr, sc, par);
+      MPF_TRY(spmv_view(A, B, w.p, w.q, w.part, nps, sc, s));
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, w.part, (int)nps, (int)nps, 1, sc + S_PQ);
+      double* rb[2] = {w.r, w.r2};
+      hipLaunchKernelGGL(k_cg_update_rows, dim3(nps), dim3(kRB), 0, s, n, B.rblk, B.bs, B.bm, B.binv, x, rb[par],
+                         rb[1 - par], w.z, w.p, w.q, w.part, (int)nps, sc, par);
+      hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kRB), 0, s, w.part, (int)nps, sc, par, it);
+      hipLaunchKernelGGL(k_cg_p, dim3(nps), dim3(kRB), 0, s, n, w.z, w.p, sc, par);
     }
     MPF_HIP(hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, s));
     MPF_HIP(hipStreamSynchronize(s));
@@ -669,13 +732,23 @@ static mpf_status pcg(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_repor
   return MPF_OK;
 }
 
+static mpf_status pcg(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep) {
+  KrylovWork& W = *p->kw;
+  const CgScratch w{W.r.ptr, W.r2.ptr, W.z.ptr, W.p.ptr, W.q.ptr, W.part.ptr, W.sc.ptr};
+  return pcg_csr(csr_of(p), blocks_of(W), w, p->rhs.ptr, p->du.ptr, o->rtol, o->maxit, p->stream, rep);
+}
+
 // Right-preconditioned GMRES(m), classical Gram-Schmidt with one re-orthogonalisation;
-// Hessenberg least squares (Givens) on the host.
+// Hessenberg least squares (Givens) on the host.  With MPF_PC_PMG the preconditioner changes from
+// one application to the next (its coarse solve is an inner CG to a loose tolerance), so the
+// preconditioned vectors z_j are kept and the update is x += Z y (flexible GMRES).
 static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report* rep) {
   KrylovWork& W = *p->kw;
   hipStream_t s = p->stream;
   const int64_t n = W.n;
   const int m = o->restart > 0 ? o->restart : 30;
+  const bool flex = o->pc == MPF_PC_PMG;
+  const int64_t nub = W.nblk - p->pat.presDOF;
   const unsigned nps = nparts_for(n);
   const unsigned npb = nparts_for(W.nblk);
   double* x = p->du.ptr;
@@ -711,9 +784,15 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
     int j = 0;
     for (; j < m && it < o->maxit; ++j) {
       // w = A M^-1 v_j
-      hipLaunchKernelGGL(k_pc_apply, dim3(npb), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
-                         W.blk_inv.ptr, V + (size_t)j * n, W.z.ptr);
-      MPF_TRY(spmv_k(p, W, W.z.ptr, W.w.ptr, nullptr, 0, nullptr, s));
+      if (flex) {
+        double* zj = W.Z.ptr + (size_t)j * n;
+        MPF_TRY(pmg_apply(p, blocks_of(W), nub, V + (size_t)j * n, zj));
+        MPF_TRY(spmv_k(p, W, zj, W.w.ptr, nullptr, 0, nullptr, s));
+      } else {
+        hipLaunchKernelGGL(k_pc_apply, dim3(npb), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
+                           W.blk_inv.ptr, V + (size_t)j * n, W.z.ptr);
+        MPF_TRY(spmv_k(p, W, W.z.ptr, W.w.ptr, nullptr, 0, nullptr, s));
+      }
       // CGS2
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_multidot, dim3(nps), dim3(kRB), 0, s, n, j + 1, V, W.w.ptr, W.part.ptr, (int)nps);
@@ -749,11 +828,15 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
       for (int k = i + 1; k < j; ++k) sacc -= H[(size_t)i * m + k] * y[k];
       y[i] = sacc / H[(size_t)i * m + i];
     }
-    // x += M^-1 (V y)
+    // x += M^-1 (V y)   (flexible: x += Z y)
     MPF_HIP(hipMemcpyAsync(d_h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_combine, dim3(nps), dim3(kRB), 0, s, n, j, V, d_h, W.w.ptr);
-    hipLaunchKernelGGL(k_pc_apply, dim3(npb), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr, W.blk_inv.ptr,
-                       W.w.ptr, W.z.ptr);
+    if (flex) {
+      hipLaunchKernelGGL(k_combine, dim3(nps), dim3(kRB), 0, s, n, j, W.Z.ptr, d_h, W.z.ptr);
+    } else {
+      hipLaunchKernelGGL(k_combine, dim3(nps), dim3(kRB), 0, s, n, j, V, d_h, W.w.ptr);
+      hipLaunchKernelGGL(k_pc_apply, dim3(npb), dim3(kRB), 0, s, W.nblk, W.blk_start.ptr, W.blk_size.ptr,
+                         W.blk_inv.ptr, W.w.ptr, W.z.ptr);
+    }
     hipLaunchKernelGGL(k_axpy, dim3(nps), dim3(kRB), 0, s, n, 1.0, W.z.ptr, x);
     if (rn / bn < o->rtol) break;
   }
@@ -774,6 +857,9 @@ mpf_status krylov_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
   if (o->ksp == MPF_KSP_GMRES && o->restart > kMaxRestart) return MPF_ERR_ARG;
+  // the two-level preconditioner needs the Tet10 mesh and a flexible outer method
+  if (o->pc == MPF_PC_PMG && (o->ksp != MPF_KSP_GMRES || (p->etype != MPF_TET10 && p->etype != MPF_TET10P1)))
+    return MPF_ERR_ARG;
   MPF_TRY(p->du.alloc(n > 0 ? n : 1));
   MPF_TRY(setup_work(p, o->pc, o->ksp == MPF_KSP_GMRES ? (o->restart > 0 ? o->restart : 30) : 0));
   mpf_status st;

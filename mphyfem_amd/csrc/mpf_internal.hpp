@@ -162,6 +162,7 @@ struct PatternBufs {
 };
 
 struct KrylovWork;  // krylov.hip
+struct PmgWork;     // pmg.hip
 struct BatchWork;   // batch.hip
 struct BandWork;    // band.hip
 struct SystemWork;  // system.hip
@@ -210,6 +211,7 @@ struct mpf_problem {
   mpf::BatchWork* batch = nullptr;  // independent meshes solved together (batch.hip)
   mpf::BandWork* band = nullptr;    // banded LDL^T direct solver (band.hip)
   mpf::SystemWork* sys = nullptr;   // element maps of a caller-pattern system (etype 0, system.hip)
+  mpf::PmgWork* pmg = nullptr;      // two-level p-multigrid preconditioner (pmg.hip)
 };
 
 namespace mpf {
@@ -243,6 +245,42 @@ void krylov_free(mpf_problem* p);
 void system_free(mpf_problem* p);                  // system.hip
 // Most partial sums of one device reduction (one per workgroup of the vector kernels).
 constexpr int kMaxPartials = 4096;
+// krylov.hip: one CSR matrix and a block-diagonal preconditioner over runs of consecutive rows
+// (blocks of 1..3 rows; binv holds each block's inverse as a 3x3 row-major array).
+struct CsrView {
+  int64_t n = 0, nnz = 0;
+  const int32_t* rowptr = nullptr;
+  const int32_t* colidx = nullptr;
+  const double* vals = nullptr;
+};
+struct BlockPcView {
+  int64_t nblk = 0;
+  const int32_t* bs = nullptr;    // first row of each block
+  const int32_t* bm = nullptr;    // its number of rows
+  const int32_t* rblk = nullptr;  // block of each row
+  const double* binv = nullptr;   // [nblk][9]
+  bool grp = false;               // every block's rows share one column list (row-group SpMV)
+};
+constexpr int kCgScalars = 16;    // device scalars of one CG run
+struct CgScratch {                // n doubles each; part: 2 * kMaxPartials; sc: kCgScalars
+  double *r, *r2, *z, *p, *q, *part, *sc;
+};
+// Block-preconditioned CG from x = 0 to ||b - A x|| <= rtol ||b|| or maxit iterations.
+mpf_status pcg_csr(const CsrView& A, const BlockPcView& B, const CgScratch& w, const double* b, double* x,
+                   double rtol, int maxit, hipStream_t s, mpf_krylov_report* rep);
+// Block structure set-up: rblk from (bs, bm); *grp = 1 when every block's rows share one column list.
+mpf_status block_pc_structure(const CsrView& A, int64_t nblk, const int32_t* bs, const int32_t* bm,
+                              int32_t* rblk, bool* grp, hipStream_t s);
+// Block inverses from the current values of A.
+mpf_status block_pc_values(const CsrView& A, int64_t nblk, const int32_t* bs, const int32_t* bm, double* binv,
+                           hipStream_t s);
+// pmg.hip: the two-level preconditioner of Tet10 and Tet10/P1 systems.  pmg_setup builds the
+// transfer operators once per pattern and the coarse matrix from the current values; fine: the
+// 3x3 node blocks of the displacement rows (first nub blocks of the block-Jacobi list) and, for a
+// mixed system, the pressure rows' Schur-diagonal blocks after them.  pmg_apply: z = M^-1 v.
+mpf_status pmg_setup(mpf_problem* p, const BlockPcView& fine, int64_t nub);
+mpf_status pmg_apply(mpf_problem* p, const BlockPcView& fine, int64_t nub, const double* v, double* z);
+void pmg_free(mpf_problem* p);
 // ||x||_2 to the host; scratch: kMaxPartials + 1 doubles of device memory.
 mpf_status dev_norm2(const double* x, int64_t n, double* out_host, hipStream_t s, double* scratch);
 }  // namespace mpf

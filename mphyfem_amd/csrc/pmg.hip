@@ -1,0 +1,673 @@
+// pmg.hip -- two-level p-multigrid preconditioner for Tet10 and Tet10/P1 systems on gfx950.
+//
+// The reference solves the assembled system with KSPSolve behind SolverPetsc::factoriseAndSolve
+// (SolverPetsc.cpp:250-330), the Krylov method and preconditioner coming from petsc_options
+// (code defaults CG + ILU, :117-140).  For the nearly incompressible Tet10/P1 system a node-block
+// Jacobi GMRES stalls; this preconditioner, used inside flexible GMRES (krylov.hip), reaches
+// ||b - A x|| / ||b|| < 1e-8 with iteration counts that grow slowly with the mesh.
+//
+// Levels: the fine level is the Tet10 displacement dofs, the coarse level the vertex dofs (the
+// linear Tet4 space inside the quadratic one), reached by the interpolation P: a vertex keeps its
+// value, the mid-edge node of edge (a, b) takes (u_a + u_b) / 2, with the element's edge table
+// 4:(0,1) 5:(1,2) 6:(0,2) 7:(0,3) 8:(2,3) 9:(1,3) (BasisFunctionsLagrange.cpp:870-956).  Only free
+// dofs take part: a fixed vertex dof has no coarse unknown, so P has no entry towards it.
+//
+// Application z = M^-1 v (block upper-triangular in the displacement / pressure split):
+//   z_p = S^-1 v_p                 S = diag(Kpp) (1 + 1 / (mu Kinv)): the pressure mass matrix
+//                                  (Kpp = -M / K) scaled to the Schur complement -M (1/K + 1/mu)
+//   z_u = V(v_u - Kup z_p)         V: one V-cycle on Kuu
+// V-cycle with B the 3x3 node blocks of Kuu and Kc = P^T Kuu P:
+//   x = Cheb(t, 0);  r = t - Kuu x;  x += P CG(Kc, P^T r);  x = Cheb(t, x)
+// Cheb: the Chebyshev iteration of the given degree for Kuu x = t preconditioned by B, over the
+// interval [beta / ratio, beta] of B^-1 Kuu's spectrum (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1), beta = 1.1 lambda, lambda the Rayleigh-quotient estimate of the
+// largest eigenvalue after 20 power steps from the ones vector (the factors of PETSc's
+// KSPChebyshev default for multigrid smoothing).  Damped Jacobi would need the damping matched to
+// lambda, which ranges from about 2 (Kuu of Tet10/P1) to above 4 (nearly incompressible Tet10).
+// The coarse CG (node-block Jacobi, krylov.hip) stops at a loose relative tolerance, which makes
+// M change from one application to the next; the outer method is therefore flexible GMRES.
+//
+// Device layout: P as two (coarse index, weight) slots per fine displacement row; P^T as CSR over
+// the coarse rows (fine rows ascending); Kc in CSR over the vertex-adjacency pattern (vertices
+// that share an element), rows of one vertex consecutive, so the coarse CG uses the row-group
+// SpMV.  Kc is formed on the device from the assembled CSR each solve, one thread per coarse row,
+// in a fixed order (bitwise reproducible).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "mpf_internal.hpp"
+
+namespace mpf {
+
+struct PmgWork {
+  // parameters
+  int degree = 2;          // Chebyshev degree before and after the coarse correction
+  double ratio = 10.0;     // beta / alpha of the smoothing interval
+  double crtol = 0.1;
+  int cmaxit = 1000;
+  // tables (built once per pattern)
+  bool built = false;
+  const int32_t* rowptr_seen = nullptr;
+  int64_t d = 0, pn = 0, nc = 0, nnzc = 0, ncb = 0;
+  DevBuf<int32_t> pidx;               // [2 d] coarse parents of each fine displacement row (-1: none)
+  DevBuf<double> pw;                  // [2 d] their weights
+  DevBuf<int32_t> rptr, rfine;        // P^T: [nc + 1], [d_entries]
+  DevBuf<double> rw;
+  DevBuf<int32_t> crow, ccol;         // Kc pattern
+  DevBuf<double> cval;
+  DevBuf<int32_t> cbs, cbm, crblk;    // coarse node blocks
+  DevBuf<double> cbinv;
+  bool cgrp = false;
+  DevBuf<double> sinv;                // [pn]
+  DevBuf<double> t, xa, xb, dd, res;  // [d]
+  double lambda = 0.0;                // estimate of the largest eigenvalue of B^-1 Kuu
+  DevBuf<double> rc, xc;              // [nc]
+  DevBuf<double> cr, cr2, cz, cp, cq, cpart, csc;  // coarse CG
+  DevBuf<int32_t> bad;
+  // statistics of the current solve
+  int64_t vcycles = 0, coarse_its = 0;
+  double setup_seconds = 0.0;
+};
+
+namespace {
+
+constexpr int kT = 256;
+
+// Kc(I, J) = sum over f in row I of P^T of w_fI sum over g with Kuu(f, g) != 0 of Kuu(f, g) w_gJ,
+// one thread per coarse row, visited in (f, g, parent slot) order.  bad[0] = 1 if some (I, J) is
+// not in the pattern (a mesh that is not conforming).
+__global__ void __launch_bounds__(kT) k_galerkin(int64_t nc, const int32_t* __restrict__ rptr,
+                                                 const int32_t* __restrict__ rfine, const double* __restrict__ rw,
+                                                 const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                                 int32_t d, const int32_t* __restrict__ pidx,
+                                                 const double* __restrict__ pw, const int32_t* __restrict__ crow,
+                                                 const int32_t* __restrict__ ccol, double* __restrict__ cval,
+                                                 int32_t* __restrict__ bad) {
+  const int64_t I = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (I >= nc) return;
+  const int cb = crow[I], ce = crow[I + 1];
+  for (int k = cb; k < ce; ++k) cval[k] = 0.0;
+  for (int e = rptr[I]; e < rptr[I + 1]; ++e) {
+    const int f = rfine[e];
+    const double wf = rw[e];
+    for (int k = rowptr[f]; k < rowptr[f + 1]; ++k) {
+      const int g = colidx[k];
+      if (g >= d) break;  // columns ascend: the pressure columns close the row
+      const double kv = wf * vals[k];
+      for (int t = 0; t < 2; ++t) {
+        const int J = pidx[2 * (size_t)g + t];
+        if (J < 0) continue;
+        int lo = cb, hi = ce;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (ccol[mid] < J) lo = mid + 1; else hi = mid;
+        }
+        if (lo < ce && ccol[lo] == J) cval[lo] += kv * pw[2 * (size_t)g + t];
+        else bad[0] = 1;
+      }
+    }
+  }
+}
+
+// Pressure block: 1 / (Kpp_qq scale), or the node-block Schur diagonal's inverse (binv_p[9 q])
+// when scale is 0 or the diagonal is zero.
+__global__ void __launch_bounds__(kT) k_sinv(int64_t pn, int32_t d, const int32_t* __restrict__ rowptr,
+                                             const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                             double scale, const double* __restrict__ binv_p,
+                                             double* __restrict__ sinv) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q >= pn) return;
+  const int r = d + (int)q;
+  int lo = rowptr[r], hi = rowptr[r + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (colidx[mid] < r) lo = mid + 1; else hi = mid;
+  }
+  const double dg = (lo < rowptr[r + 1] && colidx[lo] == r) ? vals[lo] : 0.0;
+  sinv[q] = (scale > 0.0 && dg != 0.0) ? 1.0 / (dg * scale) : binv_p[9 * q];
+}
+
+// z_p = S^-1 v_p
+__global__ void __launch_bounds__(kT) k_pres(int64_t pn, const double* __restrict__ vp, const double* __restrict__ sinv,
+                                             double* __restrict__ zp) {
+  const int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (q < pn) zp[q] = vp[q] * sinv[q];
+}
+
+// t_r = v_r - sum over the row's pressure columns of K(r, c) z_p(c - d), displacement rows r
+__global__ void __launch_bounds__(kT) k_kup(int32_t d, const int32_t* __restrict__ rowptr,
+                                            const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                            const double* __restrict__ zp, const double* __restrict__ v,
+                                            double* __restrict__ t) {
+  const int r = blockIdx.x * kT + threadIdx.x;
+  if (r >= d) return;
+  double s = v[r];
+  for (int k = rowptr[r + 1] - 1; k >= rowptr[r]; --k) {
+    const int c = colidx[k];
+    if (c < d) break;
+    s -= vals[k] * zp[c - d];
+  }
+  t[r] = s;
+}
+
+// x = dd = c2 B^-1 t over the node blocks (the first Chebyshev step from x = 0)
+__global__ void __launch_bounds__(kT) k_jac0(int64_t nub, const int32_t* __restrict__ bs,
+                                             const int32_t* __restrict__ bm, const double* __restrict__ binv,
+                                             double c2, const double* __restrict__ t, double* __restrict__ x,
+                                             double* __restrict__ dd) {
+  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (k >= nub) return;
+  const int s = bs[k], m = bm[k];
+  double r[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < m; ++i) r[i] = t[s + i];
+  const double* B = binv + 9 * k;
+  for (int i = 0; i < m; ++i) {
+    double z = 0.0;
+    for (int j = 0; j < m; ++j) z += B[3 * i + j] * r[j];
+    x[s + i] = c2 * z;
+    dd[s + i] = c2 * z;
+  }
+}
+
+// One node-block sweep over the displacement rows, LPR lanes per block (the block's rows share
+// one column list, so each column index and x value is read once for its up to three rows):
+//   MODE 1 (Chebyshev step): dd = c1 dd + c2 B^-1 (t - Kuu x_in);  x_out = x_in + dd
+//   MODE 2 (residual):       x_out = t - Kuu x_in
+template <int LPR, int MODE>
+__global__ void __launch_bounds__(kT) k_sweep(int64_t nub, const int32_t* __restrict__ bs,
+                                              const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                              int32_t d, const double* __restrict__ binv, double c1, double c2,
+                                              const double* __restrict__ t, const double* __restrict__ xin,
+                                              double* __restrict__ xout, double* __restrict__ dd) {
+  const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  constexpr int GPB = kT / LPR;
+  for (int64_t k0 = (int64_t)blockIdx.x * GPB; k0 < nub; k0 += (int64_t)gridDim.x * GPB) {
+    const int64_t k = k0 + g;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int st = 0, m = 0;
+    if (k < nub) {
+      st = bs[k];
+      m = bm[k];
+      const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+      const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
+#pragma unroll 4
+      for (int j = l; j < len; j += LPR) {
+        const int c = colidx[b0 + j];
+        if (c < d) {
+          const double xv = xin[c];
+          s0 += vals[b0 + j] * xv;
+          if (m > 1) s1 += vals[b1 + j] * xv;
+          if (m > 2) s2 += vals[b2 + j] * xv;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPR);
+      s1 += __shfl_xor(s1, off, LPR);
+      s2 += __shfl_xor(s2, off, LPR);
+    }
+    if (k < nub && l == 0) {
+      const double r[3] = {t[st] - s0, m > 1 ? t[st + 1] - s1 : 0.0, m > 2 ? t[st + 2] - s2 : 0.0};
+      if (MODE == 2) {
+        for (int i = 0; i < m; ++i) xout[st + i] = r[i];
+      } else {
+        const double* B = binv + 9 * k;
+        for (int i = 0; i < m; ++i) {
+          double z = 0.0;
+          for (int j = 0; j < m; ++j) z += B[3 * i + j] * r[j];
+          const double di = (c1 != 0.0 ? c1 * dd[st + i] : 0.0) + c2 * z;
+          dd[st + i] = di;
+          xout[st + i] = xin[st + i] + di;
+        }
+      }
+    }
+  }
+}
+
+// One power step for B^-1 Kuu with x = c xin: y = B^-1 Kuu x, and per workgroup the partial sums
+// of s.y, x.s and y.y (s = Kuu x) into part[0 / 1 / 2][blockIdx.x].
+template <int LPR>
+__global__ void __launch_bounds__(kT) k_power(int64_t nub, const int32_t* __restrict__ bs,
+                                              const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                              int32_t d, const double* __restrict__ binv, double c,
+                                              const double* __restrict__ xin, double* __restrict__ y,
+                                              double* __restrict__ part) {
+  __shared__ double sh[3][kT / 64];
+  const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  constexpr int GPB = kT / LPR;
+  double a_sy = 0.0, a_xs = 0.0, a_yy = 0.0;
+  for (int64_t k0 = (int64_t)blockIdx.x * GPB; k0 < nub; k0 += (int64_t)gridDim.x * GPB) {
+    const int64_t k = k0 + g;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int st = 0, m = 0;
+    if (k < nub) {
+      st = bs[k];
+      m = bm[k];
+      const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+      const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
+      for (int j = l; j < len; j += LPR) {
+        const int col = colidx[b0 + j];
+        if (col < d) {
+          const double xv = c * xin[col];
+          s0 += vals[b0 + j] * xv;
+          if (m > 1) s1 += vals[b1 + j] * xv;
+          if (m > 2) s2 += vals[b2 + j] * xv;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPR);
+      s1 += __shfl_xor(s1, off, LPR);
+      s2 += __shfl_xor(s2, off, LPR);
+    }
+    if (k < nub && l == 0) {
+      const double sv[3] = {s0, s1, s2};
+      const double* B = binv + 9 * k;
+      for (int i = 0; i < m; ++i) {
+        double z = 0.0;
+        for (int j = 0; j < m; ++j) z += B[3 * i + j] * sv[j];
+        y[st + i] = z;
+        a_sy += sv[i] * z;
+        a_xs += c * xin[st + i] * sv[i];
+        a_yy += z * z;
+      }
+    }
+  }
+  double v[3] = {a_sy, a_xs, a_yy};
+  for (int q = 0; q < 3; ++q) {
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+    if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int w = 0; w < kT / 64; ++w) t += sh[threadIdx.x][w];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kT) k_ones(int64_t n, double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) x[i] = 1.0;
+}
+
+// rc = P^T r
+__global__ void __launch_bounds__(kT) k_restrict(int64_t nc, const int32_t* __restrict__ rptr,
+                                                 const int32_t* __restrict__ rfine, const double* __restrict__ rw,
+                                                 const double* __restrict__ r, double* __restrict__ rc) {
+  const int64_t I = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int e = rptr[I]; e < rptr[I + 1]; ++e) s += rw[e] * r[rfine[e]];
+  rc[I] = s;
+}
+
+// x += P xc
+__global__ void __launch_bounds__(kT) k_prolong(int32_t d, const int32_t* __restrict__ pidx,
+                                                const double* __restrict__ pw, const double* __restrict__ xc,
+                                                double* __restrict__ x) {
+  const int f = blockIdx.x * kT + threadIdx.x;
+  if (f >= d) return;
+  const int j0 = pidx[2 * (size_t)f], j1 = pidx[2 * (size_t)f + 1];
+  double s = x[f];
+  if (j0 >= 0) s += pw[2 * (size_t)f] * xc[j0];
+  if (j1 >= 0) s += pw[2 * (size_t)f + 1] * xc[j1];
+  x[f] = s;
+}
+
+template <int MODE>
+void launch_sweep(int lpr, int64_t nub, const BlockPcView& F, const CsrView& A, int32_t d, double c1, double c2,
+                  const double* t, const double* xin, double* xout, double* dd, hipStream_t s) {
+  if (lpr == 32)
+    hipLaunchKernelGGL((k_sweep<32, MODE>), dim3(grid_for(nub, kT / 32)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
+                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+  else if (lpr == 16)
+    hipLaunchKernelGGL((k_sweep<16, MODE>), dim3(grid_for(nub, kT / 16)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
+                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+  else
+    hipLaunchKernelGGL((k_sweep<4, MODE>), dim3(grid_for(nub, kT / 4)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
+                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+}
+
+CsrView fine_csr(const mpf_problem* p);
+
+// Largest eigenvalue of B^-1 Kuu: Rayleigh quotient (s.y) / (x.s) of 20 power steps from the ones
+// vector (partial sums added on the host in workgroup order).
+mpf_status estimate_lambda(mpf_problem* p, PmgWork& W, const BlockPcView& F, int64_t nub, int lpr, double* out) {
+  hipStream_t s = p->stream;
+  const CsrView A = fine_csr(p);
+  const int32_t d = (int32_t)W.d;
+  const unsigned G = std::min<unsigned>(grid_for(nub, kT / lpr), kMaxPartials);
+  hipLaunchKernelGGL(k_ones, dim3(grid_for(d, kT)), dim3(kT), 0, s, (int64_t)d, W.xa.ptr);
+  double* x = W.xa.ptr;
+  double* y = W.xb.ptr;
+  double c = 1.0, lam = 0.0;
+  std::vector<double> h(3 * (size_t)G);
+  for (int it = 0; it < 20; ++it) {
+    if (lpr == 32)
+      hipLaunchKernelGGL(k_power<32>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+                         c, x, y, W.cpart.ptr);
+    else if (lpr == 16)
+      hipLaunchKernelGGL(k_power<16>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+                         c, x, y, W.cpart.ptr);
+    else
+      hipLaunchKernelGGL(k_power<4>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+                         c, x, y, W.cpart.ptr);
+    MPF_HIP(hipGetLastError());
+    MPF_HIP(hipMemcpyAsync(h.data(), W.cpart.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipStreamSynchronize(s));
+    double sy = 0.0, xs = 0.0, yy = 0.0;
+    for (unsigned b = 0; b < G; ++b) {
+      sy += h[b];
+      xs += h[G + b];
+      yy += h[2 * (size_t)G + b];
+    }
+    if (!(xs > 0.0) || !(yy > 0.0)) break;
+    lam = sy / xs;
+    c = 1.0 / std::sqrt(yy);
+    std::swap(x, y);
+  }
+  if (!(lam > 0.0) || !std::isfinite(lam)) return MPF_ERR_DIVERGED;
+  *out = lam;
+  return MPF_OK;
+}
+
+CsrView fine_csr(const mpf_problem* p) {
+  CsrView A;
+  A.n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
+  A.nnz = p->pat.nnz;
+  A.rowptr = p->pat.rowptr.ptr;
+  A.colidx = p->pat.colidx.ptr;
+  A.vals = p->vals.ptr;
+  return A;
+}
+
+// Transfer tables and the coarse pattern from the mesh (host, once per pattern).
+mpf_status build_tables(mpf_problem* p, PmgWork& W) {
+  hipStream_t s = p->stream;
+  const int32_t nN = p->nNode, nE = p->nElem, npe = p->npe;
+  const int32_t d = p->pat.dispDOF;
+  if (npe != 10) return MPF_ERR_ARG;
+  std::vector<int32_t> conn((size_t)nE * npe), nda((size_t)nN * 3);
+  MPF_TRY(p->conn.download(conn.data(), conn.size(), s));
+  MPF_TRY(p->pat.nda.download(nda.data(), nda.size(), s));
+  MPF_HIP(hipStreamSynchronize(s));
+  static const int kEdge[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {2, 3}, {1, 3}};
+  std::vector<char> isv(nN, 0);
+  std::vector<int32_t> par((size_t)nN * 2, -1);
+  for (int32_t e = 0; e < nE; ++e)
+    for (int a = 0; a < 4; ++a) isv[conn[(size_t)e * npe + a]] = 1;
+  for (int32_t e = 0; e < nE; ++e)
+    for (int q = 0; q < 6; ++q) {
+      const int32_t mnode = conn[(size_t)e * npe + 4 + q];
+      if (par[2 * (size_t)mnode] < 0) {
+        par[2 * (size_t)mnode] = conn[(size_t)e * npe + kEdge[q][0]];
+        par[2 * (size_t)mnode + 1] = conn[(size_t)e * npe + kEdge[q][1]];
+      }
+    }
+  // coarse numbering: free vertex dofs, vertex-major in node order
+  std::vector<int32_t> cnum((size_t)nN * 3, -1), cbs, cbm;
+  int64_t nc = 0;
+  for (int32_t v = 0; v < nN; ++v) {
+    if (!isv[v]) continue;
+    const int64_t st = nc;
+    for (int a = 0; a < 3; ++a)
+      if (nda[3 * (size_t)v + a] >= 0) cnum[3 * (size_t)v + a] = (int32_t)nc++;
+    if (nc > st) {
+      cbs.push_back((int32_t)st);
+      cbm.push_back((int32_t)(nc - st));
+    }
+  }
+  if (nc == 0 || nc > INT32_MAX / 2) return MPF_ERR_ARG;
+  // P: two (coarse dof, weight) slots per fine displacement row
+  std::vector<int32_t> pidx((size_t)d * 2, -1);
+  std::vector<double> pw((size_t)d * 2, 0.0);
+  for (int32_t nd = 0; nd < nN; ++nd)
+    for (int a = 0; a < 3; ++a) {
+      const int32_t f = nda[3 * (size_t)nd + a];
+      if (f < 0 || f >= d) continue;
+      if (isv[nd]) {
+        pidx[2 * (size_t)f] = cnum[3 * (size_t)nd + a];
+        pw[2 * (size_t)f] = 1.0;
+      } else if (par[2 * (size_t)nd] >= 0) {
+        for (int t = 0; t < 2; ++t) {
+          const int32_t c = cnum[3 * (size_t)par[2 * (size_t)nd + t] + a];
+          if (c >= 0) {
+            pidx[2 * (size_t)f + t] = c;
+            pw[2 * (size_t)f + t] = 0.5;
+          }
+        }
+      }
+    }
+  // P^T as CSR over the coarse rows (fine rows ascending within a row)
+  std::vector<int32_t> rptr(nc + 1, 0);
+  for (size_t k = 0; k < pidx.size(); ++k)
+    if (pidx[k] >= 0) ++rptr[pidx[k] + 1];
+  for (int64_t i = 0; i < nc; ++i) rptr[i + 1] += rptr[i];
+  std::vector<int32_t> rfine(rptr[nc]), fill(rptr.begin(), rptr.end() - 1);
+  std::vector<double> rw(rptr[nc]);
+  for (int32_t f = 0; f < d; ++f)
+    for (int t = 0; t < 2; ++t) {
+      const int32_t c = pidx[2 * (size_t)f + t];
+      if (c < 0) continue;
+      rfine[fill[c]] = f;
+      rw[fill[c]++] = pw[2 * (size_t)f + t];
+    }
+  // coarse pattern: vertices that share an element
+  std::vector<int32_t> vptr(nN + 1, 0);
+  for (int32_t e = 0; e < nE; ++e)
+    for (int a = 0; a < 4; ++a) ++vptr[conn[(size_t)e * npe + a] + 1];
+  for (int32_t i = 0; i < nN; ++i) vptr[i + 1] += vptr[i];
+  std::vector<int32_t> vel(vptr[nN]), vfill(vptr.begin(), vptr.end() - 1);
+  for (int32_t e = 0; e < nE; ++e)
+    for (int a = 0; a < 4; ++a) vel[vfill[conn[(size_t)e * npe + a]]++] = e;
+  std::vector<int32_t> crow(1, 0), ccol, nbr;
+  crow.reserve(nc + 1);
+  for (int32_t v = 0; v < nN; ++v) {
+    if (!isv[v]) continue;
+    int nfree = 0;
+    for (int a = 0; a < 3; ++a) nfree += cnum[3 * (size_t)v + a] >= 0;
+    if (!nfree) continue;
+    nbr.clear();
+    for (int32_t k = vptr[v]; k < vptr[v + 1]; ++k)
+      for (int a = 0; a < 4; ++a) nbr.push_back(conn[(size_t)vel[k] * npe + a]);
+    std::sort(nbr.begin(), nbr.end());
+    nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
+    const size_t row0 = ccol.size();
+    for (int32_t w : nbr)
+      for (int a = 0; a < 3; ++a)
+        if (cnum[3 * (size_t)w + a] >= 0) ccol.push_back(cnum[3 * (size_t)w + a]);
+    const size_t len = ccol.size() - row0;
+    for (int i = 0; i < nfree; ++i) {
+      if (i > 0) ccol.insert(ccol.end(), ccol.begin() + row0, ccol.begin() + row0 + len);
+      crow.push_back((int32_t)ccol.size());
+    }
+    if (ccol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
+  }
+  W.d = d;
+  W.pn = p->pat.presDOF;
+  W.nc = nc;
+  W.nnzc = (int64_t)ccol.size();
+  W.ncb = (int64_t)cbs.size();
+  MPF_TRY(W.pidx.upload(pidx.data(), pidx.size(), s));
+  MPF_TRY(W.pw.upload(pw.data(), pw.size(), s));
+  MPF_TRY(W.rptr.upload(rptr.data(), rptr.size(), s));
+  MPF_TRY(W.rfine.upload(rfine.data(), rfine.size(), s));
+  MPF_TRY(W.rw.upload(rw.data(), rw.size(), s));
+  MPF_TRY(W.crow.upload(crow.data(), crow.size(), s));
+  MPF_TRY(W.ccol.upload(ccol.data(), ccol.size(), s));
+  MPF_TRY(W.cbs.upload(cbs.data(), cbs.size(), s));
+  MPF_TRY(W.cbm.upload(cbm.data(), cbm.size(), s));
+  MPF_TRY(W.cval.alloc(W.nnzc));
+  MPF_TRY(W.crblk.alloc(nc));
+  MPF_TRY(W.cbinv.alloc(9 * (size_t)W.ncb));
+  MPF_TRY(W.sinv.alloc(W.pn > 0 ? W.pn : 1));
+  for (DevBuf<double>* b : {&W.t, &W.xa, &W.xb, &W.dd, &W.res}) MPF_TRY(b->alloc(d > 0 ? d : 1));
+  for (DevBuf<double>* b : {&W.rc, &W.xc, &W.cr, &W.cr2, &W.cz, &W.cp, &W.cq}) MPF_TRY(b->alloc(nc));
+  MPF_TRY(W.cpart.alloc(3 * (size_t)kMaxPartials));  // coarse CG: 2 per workgroup; power steps: 3
+  MPF_TRY(W.csc.alloc(kCgScalars));
+  MPF_TRY(W.bad.alloc(1));
+  CsrView C;
+  C.n = nc;
+  C.nnz = W.nnzc;
+  C.rowptr = W.crow.ptr;
+  C.colidx = W.ccol.ptr;
+  MPF_TRY(block_pc_structure(C, W.ncb, W.cbs.ptr, W.cbm.ptr, W.crblk.ptr, &W.cgrp, s));  // synchronises
+  W.rowptr_seen = p->pat.rowptr.ptr;
+  W.built = true;
+  return MPF_OK;
+}
+
+CsrView coarse_csr(const PmgWork& W) {
+  CsrView C;
+  C.n = W.nc;
+  C.nnz = W.nnzc;
+  C.rowptr = W.crow.ptr;
+  C.colidx = W.ccol.ptr;
+  C.vals = W.cval.ptr;
+  return C;
+}
+
+int lanes_for(const mpf_problem* p) {
+  const int64_t n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
+  const double avg = n > 0 ? (double)p->pat.nnz / (double)n : 0.0;
+  return avg > 48 ? 32 : (avg > 12 ? 16 : 4);
+}
+
+}  // namespace
+
+mpf_status pmg_setup(mpf_problem* p, const BlockPcView& F, int64_t nub) {
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = p->stream;
+  if (!p->pmg) p->pmg = new PmgWork();
+  PmgWork& W = *p->pmg;
+  if (!W.built || W.rowptr_seen != p->pat.rowptr.ptr || W.d != p->pat.dispDOF || W.pn != p->pat.presDOF)
+    MPF_TRY(build_tables(p, W));
+  W.vcycles = 0;
+  W.coarse_its = 0;
+  const CsrView A = fine_csr(p);
+  int32_t hbad = 0;
+  MPF_HIP(hipMemcpyAsync(W.bad.ptr, &hbad, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_galerkin, dim3(grid_for(W.nc, kT)), dim3(kT), 0, s, W.nc, W.rptr.ptr, W.rfine.ptr, W.rw.ptr,
+                     A.rowptr, A.colidx, A.vals, (int32_t)W.d, W.pidx.ptr, W.pw.ptr, W.crow.ptr, W.ccol.ptr,
+                     W.cval.ptr, W.bad.ptr);
+  MPF_TRY(block_pc_values(coarse_csr(W), W.ncb, W.cbs.ptr, W.cbm.ptr, W.cbinv.ptr, s));
+  MPF_TRY(estimate_lambda(p, W, F, nub, lanes_for(p), &W.lambda));
+  if (W.pn > 0) {
+    const double scale = (p->m.Kinv > 0.0 && p->m.mu > 0.0) ? 1.0 + 1.0 / (p->m.mu * p->m.Kinv) : 0.0;
+    hipLaunchKernelGGL(k_sinv, dim3(grid_for(W.pn, kT)), dim3(kT), 0, s, W.pn, (int32_t)W.d, A.rowptr, A.colidx,
+                       A.vals, scale, F.binv + 9 * nub, W.sinv.ptr);
+  }
+  MPF_HIP(hipGetLastError());
+  MPF_HIP(hipMemcpyAsync(&hbad, W.bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MPF_HIP(hipStreamSynchronize(s));
+  if (hbad) return MPF_ERR_ARG;
+  W.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return MPF_OK;
+}
+
+mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const double* v, double* z) {
+  if (!p->pmg || !p->pmg->built) return MPF_ERR_STATE;
+  PmgWork& W = *p->pmg;
+  hipStream_t s = p->stream;
+  const int32_t d = (int32_t)W.d;
+  const CsrView A = fine_csr(p);
+  const int lpr = lanes_for(p);
+  const double* t = v;
+  if (W.pn > 0) {
+    hipLaunchKernelGGL(k_pres, dim3(grid_for(W.pn, kT)), dim3(kT), 0, s, W.pn, v + d, W.sinv.ptr, z + d);
+    hipLaunchKernelGGL(k_kup, dim3(grid_for(d, kT)), dim3(kT), 0, s, d, A.rowptr, A.colidx, A.vals, z + d, v,
+                       W.t.ptr);
+    t = W.t.ptr;
+  }
+  // Chebyshev coefficients (three-term recurrence of the iteration's corrections)
+  const double beta = 1.1 * W.lambda, alpha = beta / W.ratio;
+  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha), sigma = theta / delta;
+  double c1[8], c2[8];
+  double rho_old = 1.0 / sigma;
+  c1[0] = 0.0;
+  c2[0] = 1.0 / theta;
+  for (int k = 1; k < W.degree; ++k) {
+    const double rho = 1.0 / (2.0 * sigma - rho_old);
+    c1[k] = rho * rho_old;
+    c2[k] = 2.0 * rho / delta;
+    rho_old = rho;
+  }
+  // pre-smoothing from x = 0
+  double* cur = W.xa.ptr;
+  double* oth = W.xb.ptr;
+  hipLaunchKernelGGL(k_jac0, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, c2[0], t, cur,
+                     W.dd.ptr);
+  for (int k = 1; k < W.degree; ++k) {
+    launch_sweep<1>(lpr, nub, F, A, d, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
+    std::swap(cur, oth);
+  }
+  // coarse correction
+  launch_sweep<2>(lpr, nub, F, A, d, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
+  hipLaunchKernelGGL(k_restrict, dim3(grid_for(W.nc, kT)), dim3(kT), 0, s, W.nc, W.rptr.ptr, W.rfine.ptr, W.rw.ptr,
+                     W.res.ptr, W.rc.ptr);
+  BlockPcView C;
+  C.nblk = W.ncb;
+  C.bs = W.cbs.ptr;
+  C.bm = W.cbm.ptr;
+  C.rblk = W.crblk.ptr;
+  C.binv = W.cbinv.ptr;
+  C.grp = W.cgrp;
+  const CgScratch cw{W.cr.ptr, W.cr2.ptr, W.cz.ptr, W.cp.ptr, W.cq.ptr, W.cpart.ptr, W.csc.ptr};
+  mpf_krylov_report cr{};
+  MPF_TRY(pcg_csr(coarse_csr(W), C, cw, W.rc.ptr, W.xc.ptr, W.crtol, W.cmaxit, s, &cr));
+  W.coarse_its += cr.iterations;
+  hipLaunchKernelGGL(k_prolong, dim3(grid_for(d, kT)), dim3(kT), 0, s, d, W.pidx.ptr, W.pw.ptr, W.xc.ptr, cur);
+  // post-smoothing from the corrected x, the last step into z
+  for (int k = 0; k < W.degree; ++k) {
+    double* out = (k == W.degree - 1) ? z : oth;
+    launch_sweep<1>(lpr, nub, F, A, d, c1[k], c2[k], t, cur, out, W.dd.ptr, s);
+    oth = cur;
+    cur = out;
+  }
+  MPF_HIP(hipGetLastError());
+  ++W.vcycles;
+  return MPF_OK;
+}
+
+void pmg_free(mpf_problem* p) {
+  delete p->pmg;
+  p->pmg = nullptr;
+}
+
+}  // namespace mpf
+
+using namespace mpf;
+
+extern "C" mpf_status mpf_set_pmg(mpf_problem* p, int32_t degree, double ratio, double coarse_rtol,
+                                  int32_t coarse_maxit) {
+  if (!p || degree < 1 || degree > 8 || !(ratio > 1.0 && ratio < 1e6) || !(coarse_rtol > 0.0 && coarse_rtol < 1.0) ||
+      coarse_maxit < 1)
+    return MPF_ERR_ARG;
+  if (!p->pmg) p->pmg = new PmgWork();
+  p->pmg->degree = degree;
+  p->pmg->ratio = ratio;
+  p->pmg->crtol = coarse_rtol;
+  p->pmg->cmaxit = coarse_maxit;
+  return MPF_OK;
+}
+
+extern "C" mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep) {
+  if (!p || !rep) return MPF_ERR_ARG;
+  if (!p->pmg || !p->pmg->built) return MPF_ERR_STATE;
+  rep->coarse_dofs = p->pmg->nc;
+  rep->coarse_nnz = p->pmg->nnzc;
+  rep->vcycles = p->pmg->vcycles;
+  rep->coarse_iterations = p->pmg->coarse_its;
+  rep->setup_seconds = p->pmg->setup_seconds;
+  rep->lambda_max = p->pmg->lambda;
+  return MPF_OK;
+}

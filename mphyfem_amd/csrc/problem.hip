@@ -541,6 +541,7 @@ mpf_status mpf_problem_destroy(mpf_problem* p) {
   if (!p) return MPF_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   krylov_free(p);
+  pmg_free(p);
   batch_free(p);
   band_free(p);
   system_free(p);
@@ -756,7 +757,7 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
   if (!p || !o || !rep) return MPF_ERR_ARG;
   if (!p->assembled) return MPF_ERR_STATE;  // SolverPetsc::factoriseAndSolve returns -1 (SolverPetsc.cpp:321-323)
   if (o->ksp != MPF_KSP_CG && o->ksp != MPF_KSP_GMRES && o->ksp != MPF_KSP_BAND) return MPF_ERR_ARG;
-  if (o->ksp != MPF_KSP_BAND && (o->pc < MPF_PC_NONE || o->pc > MPF_PC_BJACOBI3)) return MPF_ERR_ARG;
+  if (o->ksp != MPF_KSP_BAND && (o->pc < MPF_PC_NONE || o->pc > MPF_PC_PMG)) return MPF_ERR_ARG;
   if (p->etype == 0 && o->ksp == MPF_KSP_BAND) return MPF_ERR_ARG;  // the band ordering needs the mesh
   if (p->etype == 0 && p->nNode == 0 && o->pc == MPF_PC_BJACOBI3 && p->pat.dispDOF > 0)
     return MPF_ERR_ARG;  // node blocks need the caller's NodeDofArray
