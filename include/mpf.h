@@ -236,7 +236,7 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
 /* Two-level preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before   */
-/* and after the coarse correction (1..8, default 2), the ratio beta / alpha of its interval        */
+/* and after the coarse correction (1..8, default 3), the ratio beta / alpha of its interval        */
 /* (1 < ratio < 1e6, default 10; beta = 1.1 x the power-iteration estimate of the largest           */
 /* eigenvalue of the node-block-Jacobi-preconditioned Kuu), the coarse CG's relative tolerance      */
 /* (0 < coarse_rtol < 1, default 0.1) and iteration limit (default 1000).  No counterpart in the    */
@@ -305,7 +305,8 @@ mpf_status mpf_newton_step(mpf_problem* p, double lam, const mpf_newton_opts* o,
 
 /* ------------------------------------------------------------------------------------------ */
 /* Batched independent meshes.  No counterpart in the reference: many femSolidmechanics problems  */
-/* (Hex8, one Neo-Hookean material each) solved together.  The problem's mesh is the union of     */
+/* (Hex8 or Hex8 Q1/P0, one Neo-Hookean material each) solved together.  The problem's mesh is the */
+/* union of                                                                                         */
 /* nmesh meshes: mesh m owns nodes [node_off[m], node_off[m+1]) and elements [elem_off[m],        */
 /* elem_off[m+1]) (no element may reference another mesh's node), with material mu[m], Kinv[m]   */
 /* (Utype and tangent mode from the problem's mpf_matl).  Call after mpf_problem_create.         */
@@ -334,13 +335,25 @@ typedef struct {
   int64_t cg_iters_max;     /* CG iterations of the slowest mesh, summed over solves */
   int32_t graph_launches;
   int32_t tile, lanes_per_row, chunk;
-  double seconds, seconds_assembly, seconds_solve;
+  double seconds;
+  double seconds_assembly;   /* iteration graphs (update, assembly, test and the first K CG its) */
+  double seconds_solve;      /* CG continued past K, or the direct solves' checks               */
   double start_relres_mean, start_relres_max;  /* ||R_m|| / ||R0_m|| at the start point (last step) */
+  int32_t iteration_graphs;  /* one per Newton iteration: update + assembly + test + K CG iterations */
+  int32_t chunk_graphs;      /* CG continued past K (graph_launches = these two + start-point ones)  */
+  double seconds_capture;    /* graph capture and instantiation (first use of each K)               */
 } mpf_batch_report;
 
-/* Jacobi-PCG tile shape by measurement: every (tile, lanes_per_row) candidate is timed over a     */
-/* few CG iterations of the batch's own system; the fastest is kept for mpf_batch_newton_solve.   */
+/* Element batch and Jacobi-PCG tile shape by measurement, on the batch's own system: first the     */
+/* assembly (the row-owner gather whose chunk kernel stages at most 8, 16, 24 or 32 distinct       */
+/* elements per chunk in LDS, the gather reading each record directly, or the coloured scatter;    */
+/* Q1/P0: the scatter), each timed over three assemblies, then every (tile, lanes_per_row)         */
+/* candidate over a few CG iterations; the fastest are kept for mpf_batch_newton_solve.  The       */
+/* gather variants give bitwise the same system.                                                   */
 mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter);
+/* The autotuner's assembly pick: MPF_ASSEMBLY_* mode, staged-element cap, ms per assembly          */
+/* (MPF_ERR_STATE before mpf_batch_autotune).                                                      */
+mpf_status mpf_batch_assembly_choice(const mpf_problem* p, int32_t* mode, int32_t* stage_cap, double* ms_per_assembly);
 /* Newton-Raphson over nsteps load steps for every mesh at once (femSolidmechanics2.cpp:730-851  */
 /* per mesh) with Jacobi-PCG per mesh.  mesh_iters [nsteps][nmesh]: assemblies to convergence;    */
 /* mesh_relres [nmesh]: final ||R_m|| / ||R0_m||.  Either may be NULL.                           */

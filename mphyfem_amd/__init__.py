@@ -91,7 +91,8 @@ class BatchReport(C.Structure):
                 ("cg_iters_max", C.c_int64), ("graph_launches", C.c_int32), ("tile", C.c_int32),
                 ("lanes_per_row", C.c_int32), ("chunk", C.c_int32), ("seconds", C.c_double),
                 ("seconds_assembly", C.c_double), ("seconds_solve", C.c_double),
-                ("start_relres_mean", C.c_double), ("start_relres_max", C.c_double)]
+                ("start_relres_mean", C.c_double), ("start_relres_max", C.c_double),
+                ("iteration_graphs", C.c_int32), ("chunk_graphs", C.c_int32), ("seconds_capture", C.c_double)]
 
 
 _P = C.c_void_p
@@ -196,6 +197,7 @@ SIGNATURES = [
     ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
     ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
     ("mpf_set_pmg", C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_int32]),
+    ("mpf_batch_assembly_choice", C.c_int, [_P, _I32P, _I32P, _DP]),
     ("mpf_pmg_stats", C.c_int, [_P, C.POINTER(PmgReport)]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -380,10 +382,11 @@ def batch_workload(nmesh: int = 256, nmin: int = 8, nmax: int = 32, seed: int = 
     return np.array(edges), np.array(E), np.array(nu), np.array(v0), np.array(v1)
 
 
-def batch_problem(edges, E, nu, v0, v1, tangent: int = TANGENT_CONSISTENT, Utype: int = 3):
-    """Problem holding the batch: bottom faces fixed, top faces under their dead traction."""
+def batch_problem(edges, E, nu, v0, v1, tangent: int = TANGENT_CONSISTENT, Utype: int = 3, elem_type: int = HEX8):
+    """Problem holding the batch: bottom faces fixed, top faces under their dead traction (HEX8, or
+    HEX8P0 for Q1/P0 meshes)."""
     B = mesh_batch_hex(edges)
-    P = Problem(HEX8, B.X, B.conn, matl_from_E_nu(float(E[0]), float(nu[0]), Utype, False, tangent))
+    P = Problem(elem_type, B.X, B.conn, matl_from_E_nu(float(E[0]), float(nu[0]), Utype, elem_type == HEX8P0, tangent))
     E, nu = np.asarray(E, dtype=np.float64), np.asarray(nu, dtype=np.float64)
     P.set_batch(B.node_off, B.elem_off, E / (2.0 * (1.0 + nu)), 3.0 * (1.0 - 2.0 * nu) / E)
     for m in range(len(edges)):
@@ -697,7 +700,7 @@ class Problem:
         _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
         return nb.value, bm.value, nd.value, fl.value
 
-    def set_pmg(self, degree=2, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000):
+    def set_pmg(self, degree=3, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000):
         """Parameters of the two-level preconditioner PC_PMG (Chebyshev smoothing degree and interval
         ratio, the coarse CG's relative tolerance and iteration limit)."""
         _check(lib().mpf_set_pmg(self._h, degree, ratio, coarse_rtol, coarse_maxit), "mpf_set_pmg")
@@ -729,6 +732,13 @@ class Problem:
         t, l, ms = C.c_int32(), C.c_int32(), C.c_double()
         _check(lib().mpf_batch_autotune(self._h, C.byref(t), C.byref(l), C.byref(ms)), "mpf_batch_autotune")
         return t.value, l.value, ms.value
+
+    def batch_assembly_choice(self):
+        """(ASSEMBLY_* mode, staged-element cap, ms per assembly) picked by batch_autotune."""
+        m, c, ms = C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().mpf_batch_assembly_choice(self._h, C.byref(m), C.byref(c), C.byref(ms)),
+               "mpf_batch_assembly_choice")
+        return m.value, c.value, ms.value
 
     def batch_newton_solve(self, nsteps=1, dt=1.0, itmax=20, rtol=1e-9, cg_maxit=5000, cg_rtol=1e-10,
                            use_graph=True, chunk=0, tile=0, lanes_per_row=0, solver=BATCH_SOLVER_PCG):

@@ -13,9 +13,13 @@
 // ranges because free dofs are numbered node-major).  One workgroup per tile; per-tile partial dot
 // products are reduced per mesh in tile order, so every reduction is deterministic.
 //
-// One Newton iteration of the whole batch is three graph launches: assembly + residual norms +
-// Newton test + CG start; CG chunks of `chunk` iterations (repeated until every mesh's CG stopped);
-// the update.  The graphs are captured once per load step (the load factor is a kernel argument).
+// One Newton iteration of the whole batch is one graph launch and one 16-byte read-back: the update
+// of the previous iteration's solve, the assembly, residual norms, Newton test and CG start, then K
+// CG iterations (or the banded direct solve of every mesh).  K follows the slowest mesh's CG count
+// in the previous solve, rounded up to a power of two; the iterations after every mesh's CG has
+// stopped cost only their launch (each tile checks its mesh's flag first), and a solve that needs
+// more than K goes on with graphs of `chunk` iterations.  The load factor is read from device
+// memory, so the graphs are captured once (per K) and replayed across load steps.
 
 #include <algorithm>
 #include <chrono>
@@ -26,8 +30,10 @@
 #include "mpf_internal.hpp"
 
 // problem.hip
-mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK);
-mpf_status do_update(mpf_problem* p, bool first_iter);
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK,
+                            const double* lam_dev = nullptr);
+mpf_status do_update(mpf_problem* p, bool first_iter, const int32_t* emat = nullptr, const int32_t* mflag = nullptr,
+                     int32_t stride = 0);
 
 namespace mpf {
 
@@ -40,6 +46,9 @@ struct BatchWork {
   // tiles
   int32_t T = 1024, LPR = 8, chunk = 16, ntiles = 0;
   bool tuned = false;
+  // element batch picked by the autotuner: assembly algorithm, chunk kernel's staged-element cap
+  int32_t asm_mode = -1, asm_cap = -1;
+  double asm_ms = 0.0;
   DevBuf<int32_t> tlo, thi, tmesh, mtile;  // tile ranges and mesh, mesh -> first tile [nmesh + 1]
   // row groups: the free rows of one node, [gs[g], gs[g + 1]); tile t holds groups [tg[t], tg[t + 1])
   std::vector<int32_t> gs_h;
@@ -51,24 +60,33 @@ struct BatchWork {
   // per-mesh scalars: see the k_b_* kernels for the slots
   DevBuf<double> ms;    // [nmesh][kMS]
   DevBuf<int32_t> mf;   // [nmesh][kMF]
-  DevBuf<int32_t> cnt;  // [4]: CG-active meshes, Newton-active meshes, diverged meshes, negative Jacobian
+  DevBuf<int32_t> cnt;  // [4]: CG-active meshes, Newton-active meshes, diverged meshes, most CG
+                        // iterations of one mesh in the current solve
   int32_t* cnt_h = nullptr;  // pinned host mirror
-  hipGraphExec_t g_asm[2] = {nullptr, nullptr}, g_upd[2] = {nullptr, nullptr}, g_chunk = nullptr, g_r0 = nullptr;
-  hipGraphExec_t g_band = nullptr;  // direct solve of every mesh (banded LDL^T)
+  DevBuf<double> lam_dev;    // load factor read by the captured assembly
+  // Newton-iteration graphs [variant][K index]: variant 0 / 1: head of the first iteration (plain /
+  // guessed start); 2 / 3: update of a first / later solve, then the head.  K index 0: no solve
+  // (the last test); k > 0: kKs[k] CG iterations, or the direct solve.
+  static constexpr int kNV = 4, kNK = 9;
+  hipGraphExec_t g_it[kNV][kNK] = {};
+  hipGraphExec_t g_chunk = nullptr, g_r0 = nullptr;
   // the graphs hold these as kernel arguments: any change re-captures them
-  double g_lam = 0.0, g_rtol = -1.0, g_cg_rtol = -1.0;
+  double g_rtol = -1.0, g_cg_rtol = -1.0;
   int32_t g_cg_maxit = -1;
   int32_t g_solver = -1;            // linear solver the graphs were captured for
+  int32_t k_pred = 4;               // K index of the next first-iteration solve (256 iterations)
   ~BatchWork() { release_graphs(); if (cnt_h) (void)hipHostFree(cnt_h); }
   void release_graphs() {
-    for (auto& g : g_asm) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-    for (auto& g : g_upd) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& row : g_it)
+      for (auto& g : row) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     if (g_chunk) { (void)hipGraphExecDestroy(g_chunk); g_chunk = nullptr; }
     if (g_r0) { (void)hipGraphExecDestroy(g_r0); g_r0 = nullptr; }
-    if (g_band) { (void)hipGraphExecDestroy(g_band); g_band = nullptr; }
     g_solver = -1;
   }
 };
+
+// CG iterations per Newton-iteration graph, by K index
+constexpr int kKs[BatchWork::kNK] = {0, 32, 64, 128, 256, 512, 1024, 2048, 4096};
 
 // per-mesh double slots
 //   kETA forcing term of the last solve, kRNP residual norm before the last solve
@@ -298,6 +316,19 @@ __device__ __forceinline__ void count_into(int32_t v, int32_t* out) {
   __syncthreads();
 }
 
+__device__ __forceinline__ void max_into(int32_t v, int32_t* out) {
+  __shared__ int32_t sm[kB / 64];
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = max(s, sm[w]);
+    *out = s;
+  }
+  __syncthreads();
+}
+
 // after the residual norm partials (comp 0): Newton test per mesh.  first: R0 = ||R||.
 __global__ void __launch_bounds__(kB) k_b_newton(int32_t nmesh, const int32_t* __restrict__ mtile,
                                                  const double* __restrict__ part, double* __restrict__ ms,
@@ -359,6 +390,7 @@ __global__ void __launch_bounds__(kB) k_b_start_scalars(int32_t nmesh, const int
     act += F[kCGA];
   }
   count_into(act, cnt);
+  if (threadIdx.x == 0) cnt[3] = 0;
 }
 
 __global__ void __launch_bounds__(kB) k_b_alpha(int32_t nmesh, const int32_t* __restrict__ mtile,
@@ -375,7 +407,7 @@ __global__ void __launch_bounds__(kB) k_b_alpha(int32_t nmesh, const int32_t* __
 __global__ void __launch_bounds__(kB) k_b_beta(int32_t nmesh, const int32_t* __restrict__ mtile,
                                                const double* __restrict__ part, double* __restrict__ ms,
                                                int32_t* __restrict__ mf, int32_t maxit, int32_t* __restrict__ cnt) {
-  int32_t act = 0;
+  int32_t act = 0, most = 0;
   for (int m = threadIdx.x; m < nmesh; m += blockDim.x) {
     double* S = ms + m * kMS;
     int32_t* F = mf + m * kMF;
@@ -388,8 +420,14 @@ __global__ void __launch_bounds__(kB) k_b_beta(int32_t nmesh, const int32_t* __r
       if (rr <= S[kTOL2] * S[kRR0] || F[kCGIT] >= maxit || rr != rr) F[kCGA] = 0;
     }
     act += F[kCGA];
+    most = max(most, F[kNA] ? F[kCGIT] : 0);
   }
   count_into(act, cnt);
+  max_into(most, cnt + 3);
+}
+
+__global__ void k_b_set_scalar(double* __restrict__ dst, double v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
 }
 
 // prescribed dofs at their values for load factor lam (the start point of a guessed step)
@@ -488,7 +526,7 @@ mpf_status enqueue_newton_head(mpf_problem* p, BatchWork& B, bool first, double 
   const int64_t n = P.dispDOF;
   const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
   hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, s, p->bad.ptr, 1);
-  MPF_TRY(enqueue_assembly(p, first, lam, 1));
+  MPF_TRY(enqueue_assembly(p, first, lam, 1, B.lam_dev.ptr));
   hipLaunchKernelGGL(k_b_dot, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, p->rhs.ptr, p->rhs.ptr, B.part.ptr, 0);
   hipLaunchKernelGGL(k_b_newton, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr,
                      (int32_t)first, rtol, eta_min, B.cnt.ptr);
@@ -643,6 +681,8 @@ static mpf_status batch_prepare(mpf_problem* p) {
   MPF_TRY(B.ms.alloc((size_t)B.nmesh * kMS));
   MPF_TRY(B.mf.alloc((size_t)B.nmesh * kMF));
   MPF_TRY(B.cnt.alloc(4));
+  MPF_TRY(B.lam_dev.alloc(1));
+  hipLaunchKernelGGL(k_b_set_scalar, dim3(1), dim3(1), 0, p->stream, B.lam_dev.ptr, 1.0);  // autotuner's system
   if (!B.cnt_h) MPF_HIP(hipHostMalloc((void**)&B.cnt_h, 4 * sizeof(int32_t), hipHostMallocDefault));
   return build_tiles(B, B.T, p->stream);
 }
@@ -667,13 +707,48 @@ mpf_status batch_autotune(mpf_problem* p, int32_t* T_out, int32_t* LPR_out, doub
     MPF_HIP(hipMemcpyAsync(B.mf.ptr, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
     MPF_HIP(hipStreamSynchronize(s));
   }
+  hipEvent_t e0, e1;
+  MPF_HIP(hipEventCreate(&e0));
+  MPF_HIP(hipEventCreate(&e1));
+  // element batch: the row-owner gather with its chunk kernel staging at most cap distinct elements
+  // per chunk in LDS (cap 8 .. 32), the gather reading every record directly, or the coloured scatter
+  // (Q1/P0: the coloured scatter only), each timed over three first-iteration assemblies
+  {
+    struct Cand { int32_t mode, cap; };
+    std::vector<Cand> cands;
+    if (p->p0) {
+      cands.push_back({MPF_ASSEMBLY_COLORED_SCATTER, p->chunk_cap});
+    } else {
+      for (int32_t cap : {8, 16, 24, 32}) cands.push_back({MPF_ASSEMBLY_GATHER, cap});
+      cands.push_back({MPF_ASSEMBLY_GATHER_DIRECT, p->chunk_cap});
+      cands.push_back({MPF_ASSEMBLY_COLORED_SCATTER, p->chunk_cap});
+    }
+    double abest = 1e300;
+    Cand pick = cands[0];
+    for (const Cand& c : cands) {
+      p->asm_mode = c.mode;
+      p->chunk_cap = c.cap;
+      if (c.mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
+      MPF_TRY(enqueue_assembly(p, true, 1.0, 1));  // warm-up
+      MPF_HIP(hipEventRecord(e0, s));
+      const int reps = 3;
+      for (int k = 0; k < reps; ++k) MPF_TRY(enqueue_assembly(p, true, 1.0, 1));
+      MPF_HIP(hipEventRecord(e1, s));
+      MPF_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MPF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < abest) { abest = ms / reps; pick = c; }
+    }
+    p->asm_mode = pick.mode;
+    p->chunk_cap = pick.cap;
+    B.asm_mode = pick.mode;
+    B.asm_cap = pick.cap;
+    B.asm_ms = abest;
+  }
   const int Ts[] = {512, 1024, 2048, 4096};
   const int Ls[] = {4, 8, 16};
   double best = 1e300;
   int bT = B.T, bL = B.LPR;
-  hipEvent_t e0, e1;
-  MPF_HIP(hipEventCreate(&e0));
-  MPF_HIP(hipEventCreate(&e1));
   for (int T : Ts) {
     MPF_TRY(build_tiles(B, T, s));
     MPF_TRY(enqueue_newton_head(p, B, true, 1.0, 0.0, 1e-300));
@@ -731,7 +806,7 @@ static mpf_status enqueue_r0(mpf_problem* p, BatchWork& B, double lam) {
   hipStream_t s = p->stream;
   const dim3 gt(std::max(B.ntiles, 1)), bt(kB);
   hipLaunchKernelGGL(k_b_zero_i32, dim3(1), dim3(64), 0, s, p->bad.ptr, 1);
-  MPF_TRY(enqueue_assembly(p, true, lam, 0));
+  MPF_TRY(enqueue_assembly(p, true, lam, 0, B.lam_dev.ptr));
   hipLaunchKernelGGL(k_b_dot, gt, bt, 0, s, B.tlo.ptr, B.thi.ptr, p->rhs.ptr, p->rhs.ptr, B.part.ptr, 0);
   hipLaunchKernelGGL(k_b_newton, dim3(1), bt, 0, s, B.nmesh, B.mtile.ptr, B.part.ptr, B.ms.ptr, B.mf.ptr, 1, 0.0,
                      1.0, B.cnt.ptr);
@@ -761,29 +836,45 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
   MPF_HIP(hipMemcpyAsync(p->dispPrev.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
   MPF_TRY(mpf_set_bc(p, lam));
   MPF_TRY(reset_mesh_flags(p, B));
-  auto head = [&](int f) { return enqueue_newton_head(p, B, f == 1, lam, o->rtol, o->cg_rtol, !band); };
-  auto direct = [&] { return band_enqueue_solve(p, B.mf.ptr + kNA, kMF); };
-  if (G && (B.g_lam != lam || !B.g_asm[0] || B.g_solver != o->solver || B.g_rtol != o->rtol ||
-            B.g_cg_rtol != o->cg_rtol || B.g_cg_maxit != o->cg_maxit)) {
-    for (int f = 0; f < 2; ++f) {
-      MPF_TRY(capture(p, &B.g_asm[f], [&] { return head(f); }));
-      MPF_TRY(capture(p, &B.g_upd[f], [&] { return do_update(p, f == 1); }));
+  hipLaunchKernelGGL(k_b_set_scalar, dim3(1), dim3(1), 0, s, B.lam_dev.ptr, lam);
+  MPF_HIP(hipGetLastError());
+  // one Newton iteration: [update of the previous solve] + head + (K CG iterations | direct solve)
+  auto iter_work = [&](int v, int kidx) -> mpf_status {
+    if (v >= 2) MPF_TRY(do_update(p, v == 2, p->p0 ? B.emat.ptr : nullptr, B.mf.ptr + kNA, kMF));
+    MPF_TRY(enqueue_newton_head(p, B, v == 0, lam, o->rtol, o->cg_rtol, !band));
+    if (kidx > 0) {
+      if (band) MPF_TRY(band_enqueue_solve(p, B.mf.ptr + kNA, kMF));
+      else
+        for (int k = 0; k < kKs[kidx]; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
     }
-    if (band) {
-      MPF_TRY(capture(p, &B.g_band, direct));
-    } else {
-      MPF_TRY(capture(p, &B.g_chunk, [&]() -> mpf_status {
-        for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
-        return MPF_OK;
-      }));
-    }
-    MPF_TRY(capture(p, &B.g_r0, [&] { return enqueue_r0(p, B, lam); }));
-    B.g_lam = lam;
+    return MPF_OK;
+  };
+  auto chunk_work = [&]() -> mpf_status {
+    for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
+    return MPF_OK;
+  };
+  if (G && (B.g_solver != o->solver || B.g_rtol != o->rtol || B.g_cg_rtol != o->cg_rtol ||
+            B.g_cg_maxit != o->cg_maxit)) {
+    B.release_graphs();
     B.g_rtol = o->rtol;
     B.g_cg_rtol = o->cg_rtol;
     B.g_cg_maxit = o->cg_maxit;
     B.g_solver = o->solver;
   }
+  // graph of (variant, K index), captured on first use
+  auto iter_graph = [&](int v, int kidx) -> mpf_status {
+    if (!G || B.g_it[v][kidx]) return MPF_OK;
+    auto tc = std::chrono::steady_clock::now();
+    MPF_TRY(capture(p, &B.g_it[v][kidx], [&] { return iter_work(v, kidx); }));
+    rep->seconds_capture += secs(tc);
+    return MPF_OK;
+  };
+  if (G && !B.g_chunk && !band) {
+    auto tc = std::chrono::steady_clock::now();
+    MPF_TRY(capture(p, &B.g_chunk, chunk_work));
+    rep->seconds_capture += secs(tc);
+  }
+  if (G && guess && !B.g_r0) MPF_TRY(capture(p, &B.g_r0, [&] { return enqueue_r0(p, B, lam); }));
   if (guess) {
     auto ta = std::chrono::steady_clock::now();
     MPF_TRY(launch(p, B.g_r0, G, [&] { return enqueue_r0(p, B, lam); }));
@@ -798,15 +889,22 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
   }
   mpf_status result = MPF_OK;
   bool step_conv = false;
+  int kpred = band ? 1 : std::max(1, std::min(B.k_pred, BatchWork::kNK - 1));
+  int prev_first = -1;  // first-iteration flag of the solve whose update is pending
   for (int iter = 1; iter <= o->itmax + 1; ++iter) {
     const int f = (iter == 1 && !guess) ? 1 : 0;
+    const int v = iter == 1 ? (f ? 0 : 1) : (prev_first ? 2 : 3);
+    const int kidx = iter > o->itmax ? 0 : kpred;
     auto ta = std::chrono::steady_clock::now();
-    MPF_TRY(launch(p, B.g_asm[f], G, [&] { return head(f); }));
+    MPF_TRY(iter_graph(v, kidx));
+    MPF_TRY(launch(p, B.g_it[v][kidx], G, [&] { return iter_work(v, kidx); }));
+    p->assembled = true;  // the next update (Q1/P0: pressure recovery) reads this assembly
     ++rep->graph_launches;
+    ++rep->iteration_graphs;
+    int32_t bad = 0;
+    MPF_HIP(hipMemcpyAsync(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     MPF_TRY(read_counts(p, B));
     rep->seconds_assembly += secs(ta);
-    int32_t bad = 0;
-    MPF_HIP(hipMemcpy(&bad, p->bad.ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (bad) { result = MPF_ERR_NEG_JACOBIAN; break; }
     if (B.cnt_h[2] > 0) { result = MPF_ERR_DIVERGED; break; }
     if (iter == 1) {  // residual at the start point, relative to the plain-start R0
@@ -825,26 +923,28 @@ static mpf_status batch_step(mpf_problem* p, double lam, const mpf_batch_opts* o
     ++rep->newton_iters;
     auto ts = std::chrono::steady_clock::now();
     if (band) {
-      MPF_TRY(launch(p, B.g_band, G, direct));
-      ++rep->graph_launches;
       int32_t pbad = 0;
       MPF_HIP(hipMemcpyAsync(&pbad, band_bad_flag(p), sizeof(int32_t), hipMemcpyDeviceToHost, s));
       MPF_HIP(hipStreamSynchronize(s));
       if (pbad) { result = MPF_ERR_DIVERGED; break; }
+    } else {
+      int its = kKs[kidx];
+      while (B.cnt_h[0] > 0 && its < o->cg_maxit) {
+        MPF_TRY(launch(p, B.g_chunk, G, chunk_work));
+        ++rep->graph_launches;
+        ++rep->chunk_graphs;
+        its += B.chunk;
+        MPF_TRY(read_counts(p, B));
+      }
+      const int most = B.cnt_h[3];
+      rep->cg_iters_max += most;
+      // next solve's K: the smallest listed count >= 1.25 x this solve's slowest mesh
+      int k = 1;
+      while (k < BatchWork::kNK - 1 && kKs[k] < most + most / 4) ++k;
+      if (iter == 1) B.k_pred = k;
+      kpred = k;
     }
-    int its = 0;
-    while (!band && B.cnt_h[0] > 0 && its < o->cg_maxit) {
-      MPF_TRY(launch(p, B.g_chunk, G, [&]() -> mpf_status {
-        for (int k = 0; k < B.chunk; ++k) MPF_TRY(enqueue_cg_iter(p, B, o->cg_maxit));
-        return MPF_OK;
-      }));
-      ++rep->graph_launches;
-      its += B.chunk;
-      MPF_TRY(read_counts(p, B));
-    }
-    rep->cg_iters_max += its;
-    MPF_TRY(launch(p, B.g_upd[f], G, [&] { return do_update(p, f == 1); }));
-    ++rep->graph_launches;
+    prev_first = f;
     rep->seconds_solve += secs(ts);
   }
   p->assembled = true;  // vals / rhs hold the step's last assembly
@@ -921,8 +1021,17 @@ extern "C" {
 
 mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off, const int32_t* elem_off,
                          const double* mu, const double* Kinv) {
-  if (!p || p->etype == 0 || p->p0) return MPF_ERR_ARG;  // the batched solver runs the displacement Hex8 only
+  if (!p || p->etype == 0) return MPF_ERR_ARG;
   return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
+}
+
+mpf_status mpf_batch_assembly_choice(const mpf_problem* p, int32_t* mode, int32_t* stage_cap, double* ms_per_assembly) {
+  if (!p || !p->batch) return MPF_ERR_ARG;
+  if (p->batch->asm_mode < 0) return MPF_ERR_STATE;
+  if (mode) *mode = p->batch->asm_mode;
+  if (stage_cap) *stage_cap = p->batch->asm_cap;
+  if (ms_per_assembly) *ms_per_assembly = p->batch->asm_ms;
+  return MPF_OK;
 }
 
 mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter) {

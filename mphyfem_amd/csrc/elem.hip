@@ -343,12 +343,16 @@ __global__ void __launch_bounds__(64) hex8p0_kernel(ElemArgs A, int32_t first, i
 // solveForPressure (Elem_Solid_3D_Mixed0.cpp:112-199): p_e += (Fp_e - Kpu_e . du_e) / Kpp_e, Kpu = Kup^T, where
 // du_e is the element's displacement increment: the solved correction on free dofs and, in the first
 // iteration, the applied Dirichlet increment on prescribed ones (femSolidmechanics2.cpp:1137-1186).
+// emat (when given): element -> mesh of a batch; elements of meshes whose flag mflag[mesh * stride]
+// is 0 (no solve this iteration) keep their pressure.
 __global__ void k_hex8p0_pressure(int32_t nElem, const int32_t* __restrict__ conn, const int32_t* __restrict__ nda,
                                   const double* __restrict__ du, const double* __restrict__ dispApplied,
                                   const double* __restrict__ Kup, const double* __restrict__ Kpp,
-                                  const double* __restrict__ Fp, double* __restrict__ pres) {
+                                  const double* __restrict__ Fp, double* __restrict__ pres,
+                                  const int32_t* __restrict__ emat, const int32_t* __restrict__ mflag, int32_t stride) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nElem) return;
+  if (emat && !mflag[(size_t)emat[e] * stride]) return;
   double s = 0.0;
   for (int a = 0; a < 8; ++a) {
     const size_t n = (size_t)conn[(size_t)e * 8 + a];
@@ -675,10 +679,11 @@ mpf_status launch_hex8p0_scatter(const ElemArgs& a, const ScatterArgs& sc, doubl
 
 mpf_status launch_hex8p0_pressure(int32_t nElem, const int32_t* conn, const int32_t* nda, const double* du,
                                   const double* dispApplied, const double* Kup, const double* Kpp, const double* Fp,
-                                  double* pres, hipStream_t s) {
+                                  double* pres, hipStream_t s, const int32_t* emat, const int32_t* mflag,
+                                  int32_t stride) {
   if (nElem <= 0) return MPF_OK;
   hipLaunchKernelGGL(k_hex8p0_pressure, dim3(grid_for(nElem, 256)), dim3(256), 0, s, nElem, conn, nda, du,
-                     dispApplied, Kup, Kpp, Fp, pres);
+                     dispApplied, Kup, Kpp, Fp, pres, emat, mflag, stride);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }

@@ -113,7 +113,8 @@ mpf_status launch_hex8p0_scatter(const ElemArgs& a, const ScatterArgs& sc, doubl
                                  hipStream_t s);
 mpf_status launch_hex8p0_pressure(int32_t nElem, const int32_t* conn, const int32_t* nda, const double* du,
                                   const double* dispApplied, const double* Kup, const double* Kpp, const double* Fp,
-                                  double* pres, hipStream_t s);
+                                  double* pres, hipStream_t s, const int32_t* emat = nullptr,
+                                  const int32_t* mflag = nullptr, int32_t stride = 0);
 mpf_status launch_tet10_scatter(const ElemArgs& a, const ScatterArgs& sc, hipStream_t s);
 // Tet10 displacement element (dense operator, coloured scatter with a 100-slot map per element)
 mpf_status launch_tet10d_dense(const ElemArgs& a, int32_t first, int32_t count, double* Ke, double* Re,

@@ -25,7 +25,8 @@
 
 #include "mpf_internal.hpp"
 
-mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK);  // problem.hip
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK,
+                            const double* lam_dev = nullptr);  // problem.hip
 
 namespace mpf {
 mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals, int64_t n, int64_t nnz,

@@ -44,7 +44,7 @@ namespace mpf {
 
 struct PmgWork {
   // parameters
-  int degree = 2;          // Chebyshev degree before and after the coarse correction
+  int degree = 3;          // Chebyshev degree before and after the coarse correction
   double ratio = 10.0;     // beta / alpha of the smoothing interval
   double crtol = 0.1;
   int cmaxit = 1000;

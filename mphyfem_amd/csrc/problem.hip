@@ -45,10 +45,13 @@ __global__ void k_set_bc(int32_t nbc, const int32_t* __restrict__ dof, const dou
 }
 
 // rhs[i] += lam * Fext[assyForSoln[i]]  (femSolidmechanics2.cpp:785-789)
+// lam_dev (when given) holds the load factor in device memory, so that a captured graph reads the
+// current one at replay
 __global__ void k_add_fext(int32_t nfree, const int32_t* __restrict__ assy, const double* __restrict__ fext, double lam,
-                           double* __restrict__ rhs) {
+                           const double* __restrict__ lam_dev, double* __restrict__ rhs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nfree) rhs[i] += lam * fext[assy[i]];
+  const double l = lam_dev ? *lam_dev : lam;
+  if (i < nfree) rhs[i] += l * fext[assy[i]];
 }
 
 // factoriseSolveAndUpdate (femSolidmechanics2.cpp:1137-1196)
@@ -259,7 +262,7 @@ mpf_status enqueue_element_kernels(mpf_problem* p, bool first_iter, int scatterK
 
 // Enqueue one full assembly (no synchronisation).  scatterK = 0 assembles the residual only.
 // zeroMtx (SolverPetsc.cpp:145-163) is only needed by the scatter, which accumulates.
-mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK) {
+mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int scatterK, const double* lam_dev = nullptr) {
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   const int64_t n = (int64_t)P.dispDOF + P.presDOF;
@@ -270,7 +273,7 @@ mpf_status enqueue_assembly(mpf_problem* p, bool first_iter, double lam, int sca
   MPF_TRY(enqueue_element_kernels(p, first_iter, scatterK));
   if (p->has_fext && P.dispDOF > 0)
     hipLaunchKernelGGL(k_add_fext, dim3(grid_for(P.dispDOF, 256)), dim3(256), 0, s, P.dispDOF, P.assy.ptr,
-                       p->fext_unit.ptr, lam, p->rhs.ptr);
+                       p->fext_unit.ptr, lam, lam_dev, p->rhs.ptr);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
@@ -287,7 +290,10 @@ mpf_status assemble_sync(mpf_problem* p, bool first_iter, double lam, int scatte
   return MPF_OK;
 }
 
-mpf_status do_update(mpf_problem* p, bool first_iter) {
+// emat / mflag / stride (batched meshes): Q1/P0 elements of meshes without a solve this iteration
+// keep their pressures (their du is 0, but Fp / Kpp would still move them)
+mpf_status do_update(mpf_problem* p, bool first_iter, const int32_t* emat = nullptr, const int32_t* mflag = nullptr,
+                     int32_t stride = 0) {
   if (p->p0 && !p->assembled) return MPF_ERR_STATE;  // the pressure recovery reads the last assembly
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
@@ -302,7 +308,7 @@ mpf_status do_update(mpf_problem* p, bool first_iter) {
                        p->du.ptr, P.dispDOF, p->pres.ptr);
   if (p->p0) {  // from the element matrices of the assembly this correction solved
     MPF_TRY(launch_hex8p0_pressure(p->nElem, p->conn.ptr, P.nda.ptr, p->du.ptr, first_iter ? p->dispApplied.ptr : nullptr,
-                                   p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, p->pres.ptr, s));
+                                   p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr, p->pres.ptr, s, emat, mflag, stride));
   }
   MPF_HIP(hipGetLastError());
   return MPF_OK;

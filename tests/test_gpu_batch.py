@@ -128,23 +128,87 @@ def test_tangent_increment_of_a_mesh_never_factored_is_zero(mpf, gpu):
 
 
 def test_batched_graph_replay_equals_direct_launches(mpf, gpu):
+    """Two load steps: the graphs, captured once and replayed at both load factors (read from device
+    memory), against the same kernels launched directly; one graph launch per Newton iteration."""
     _, Pg = _batch(mpf)
     _, Pe = _batch(mpf)
-    ig, rg, _ = Pg.batch_newton_solve(nsteps=1, dt=1.0, rtol=1e-10, use_graph=True, chunk=8)
-    ie, re, _ = Pe.batch_newton_solve(nsteps=1, dt=1.0, rtol=1e-10, use_graph=False, chunk=8)
+    ig, rg, rep = Pg.batch_newton_solve(nsteps=2, dt=0.5, rtol=1e-10, use_graph=True, chunk=8)
+    ie, re, repe = Pe.batch_newton_solve(nsteps=2, dt=0.5, rtol=1e-10, use_graph=False, chunk=8)
     ug, _ = Pg.state()
     ue, _ = Pe.state()
     assert np.array_equal(ig, ie)
     assert np.array_equal(ug, ue)  # same kernels, same order: bitwise
+    print("newton iterations", rep.newton_iters, "iteration graphs", rep.iteration_graphs, "chunk graphs",
+          rep.chunk_graphs, "capture s", rep.seconds_capture, flush=True)
+    # one per Newton iteration plus each step's closing test
+    assert rep.iteration_graphs == rep.newton_iters + rep.steps_done
+    assert rep.graph_launches == rep.iteration_graphs + rep.chunk_graphs
+    assert (repe.iteration_graphs, repe.chunk_graphs) == (rep.iteration_graphs, rep.chunk_graphs)
 
 
 def test_batch_autotune_choice_is_used(mpf, gpu):
     _, P = _batch(mpf)
     T, L, ms = P.batch_autotune()
     assert T in (512, 1024, 2048, 4096) and L in (4, 8, 16) and ms > 0.0
+    mode, cap, ams = P.batch_assembly_choice()
+    print("autotune: tile", T, "lanes", L, "assembly mode", mode, "cap", cap, "ms", ams, flush=True)
+    assert mode in (mpf.ASSEMBLY_GATHER, mpf.ASSEMBLY_GATHER_DIRECT, mpf.ASSEMBLY_COLORED_SCATTER) and ams > 0.0
+    assert P.assembly_mode() == mode
     _, rel, rep = P.batch_newton_solve(nsteps=1, dt=1.0, rtol=1e-9)
     assert (rep.tile, rep.lanes_per_row) == (T, L)
     assert np.all(rel < 1e-9)
+
+
+def test_gather_stage_caps_give_the_same_system(mpf, gpu):
+    """The element-batch candidates of the autotuner that share the row-owner gather (chunk kernel
+    staging 8..32 distinct elements, or direct record reads) assemble bitwise the same K and R."""
+    _, P = _batch(mpf)
+    P.set_boundary_conditions(1.0)
+    ref = None
+    for mode, cap in ((mpf.ASSEMBLY_GATHER, 32), (mpf.ASSEMBLY_GATHER, 8), (mpf.ASSEMBLY_GATHER, 16),
+                      (mpf.ASSEMBLY_GATHER_DIRECT, 32)):
+        P.set_assembly_mode(mode)
+        P.set_chunk_stage_cap(cap)
+        P.calc_stiffness_and_residual(True, 1.0)
+        vals, rhs = P.system()
+        if ref is None:
+            ref = (vals.copy(), rhs.copy())
+        assert np.array_equal(vals, ref[0]) and np.array_equal(rhs, ref[1]), (mode, cap)
+
+
+def test_batched_q1p0_matches_each_mesh_alone(mpf, gpu):
+    """Hex8 Q1/P0 meshes in one batch (Jacobi-PCG and banded direct solves) against each mesh solved
+    alone by the device Newton with the same element, loads and tolerances: displacements and element
+    pressures within 1e-9; pressures of meshes that converged early are left alone."""
+    E_ = np.array([1.0e3, 3.0e2, 5.0e3])
+    NU_ = np.array([0.45, 0.49, 0.3])
+    edges = [2, 3, 2]
+    V0_, V1_ = -0.02 * E_, 0.005 * E_
+    for solver in (mpf.BATCH_SOLVER_PCG, mpf.BATCH_SOLVER_BAND):
+        B, P = mpf.batch_problem(edges, E_, NU_, V0_, V1_, elem_type=mpf.HEX8P0)
+        P.prepare_matrix_pattern()
+        iters, rel, rep = P.batch_newton_solve(nsteps=2, dt=0.5, itmax=25, rtol=1e-11, cg_maxit=20000,
+                                               cg_rtol=1e-13, solver=solver)
+        assert rep.steps_done == 2 and np.all(rel < 1e-11), (rel, iters)
+        u, pr = P.state()
+        for m in range(len(edges)):
+            lo, hi = B.node_off[m], B.node_off[m + 1]
+            e0, e1 = B.elem_off[m], B.elem_off[m + 1]
+            S = mpf.Problem(mpf.HEX8P0, B.X[lo:hi], B.conn[e0:e1] - lo,
+                            mpf.matl_from_E_nu(float(E_[m]), float(NU_[m]), 3, True, mpf.TANGENT_CONSISTENT))
+            S.add_dirichlet(B.bottom[m] - lo, -1)
+            S.add_traction(B.top[m] - lo, float(V0_[m]), float(V1_[m]))
+            S.prepare_matrix_pattern()
+            _, srep = S.solve_with_newton_raphson(S.newton_opts(nsteps=2, dt=0.5, itmax=25, rtol=1e-11,
+                                                                ksp=mpf.KSP_BAND))
+            assert srep.steps_done == 2
+            us, ps = S.state()
+            print("Q1/P0 batch", solver, "mesh", m, normwise_rel(u[3 * lo:3 * hi], us), normwise_rel(pr[e0:e1], ps),
+                  flush=True)
+            assert normwise_rel(u[3 * lo:3 * hi], us) < 1e-9, m
+            assert normwise_rel(pr[e0:e1], ps) < 1e-9, m
+            S.close()
+        P.close()
 
 
 def test_batch_rejects_shared_nodes(mpf, gpu):
