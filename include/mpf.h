@@ -416,9 +416,10 @@ mpf_status mpf_mpnn_adam(mpf_mpnn* net, float lr, float beta1, float beta2, floa
 /* Newton start point u_k + rho (u_k - u_{k-1}) (+ the network's correction when use_net), device. */
 mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const double* ukm1, double rho,
                           int32_t use_net, double* guess);
-/* Guarded start point for load factor lam: per mesh, the network's correction (last forward) is kept  */
-/* only where the residual with it is below the residual at the plain extrapolation (two residual-only */
-/* assemblies; the problem's state is restored).  n_learned: meshes that keep the correction.          */
+/* Guarded start point for load factor lam: per mesh, the network's correction (last forward) enters  */
+/* with the weight w in [0, 1] that minimises the residual linearised between the base predictor      */
+/* (w = 0) and the whole correction (w = 1), both assembled (two residual-only assemblies; the        */
+/* problem's state is restored).  n_learned: meshes with w > 0.                                        */
 mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, const double* uk, const double* ukm1,
                                   double rho, double* guess, int32_t* n_learned);
 

@@ -1043,8 +1043,8 @@ class Mpnn:
                "mpf_mpnn_guess")
 
     def guarded_guess(self, P: Problem, lam: float, uk, ukm1, rho: float, out) -> int:
-        """Per mesh: the network's start point where it lowers the residual, else the extrapolation.
-        Returns the number of meshes that keep the network's correction."""
+        """Per mesh: the base predictor plus the network's correction scaled by the weight in [0, 1]
+        that minimises the linearised start residual.  Returns the number of meshes with a weight > 0."""
         import torch
         torch.cuda.current_stream().synchronize()
         k = C.c_int32()

@@ -77,6 +77,7 @@ struct mpf_mpnn {
   mpf::DevBuf<double> wres;           // per mesh: 1 / ||f_m||^2 over free dofs (unit external load)
   mpf::DevBuf<double> ebuf, rbuf, gbuf, msq;  // residual-loss work vectors
   mpf::DevBuf<double> usave, gbuf2;           // guarded guess: saved state, second candidate
+  mpf::DevBuf<double> rnet, mdot;             // guarded guess: residual with the correction, per-mesh dots
   mpf::DevBuf<float> wmesh;                   // guarded guess: per-mesh weight of the correction
 };
 
@@ -457,6 +458,22 @@ __global__ void __launch_bounds__(256) k_mesh_sq(const int32_t* __restrict__ dof
   const int m = blockIdx.x;
   double v = 0.0;
   for (int64_t i = doff[m] + threadIdx.x; i < doff[m + 1]; i += 256) v += r[i] * r[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (threadIdx.x < s2) sh[threadIdx.x] += sh[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = sh[0];
+}
+
+// per mesh: sum of a b over its free-dof range
+__global__ void __launch_bounds__(256) k_mesh_dot(const int32_t* __restrict__ doff, const double* __restrict__ a,
+                                                  const double* __restrict__ b, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int m = blockIdx.x;
+  double v = 0.0;
+  for (int64_t i = doff[m] + threadIdx.x; i < doff[m + 1]; i += 256) v += a[i] * b[i];
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int s2 = 128; s2 > 0; s2 >>= 1) {
@@ -887,9 +904,12 @@ mpf_status mpf_mpnn_guess(mpf_mpnn* net, mpf_problem* p, const double* uk, const
   return MPF_OK;
 }
 
-// Guarded learned start: per mesh, the network's correction is kept only where the residual at load
-// factor lam (prescribed dofs at their values) is smaller with it than at the plain extrapolation.
-// Two residual-only assemblies; the problem's displacement is restored.  n_learned: meshes kept.
+// Guarded learned start: per mesh m the network's correction c_m enters with the weight w_m in [0, 1]
+// that minimises the linearised residual at load factor lam (prescribed dofs at their values),
+//   R(base + w c) ~ R_b + w (R_n - R_b),   w = (R_b.R_b - R_n.R_b) / |R_n - R_b|^2, clipped to [0, 1],
+// from the residuals R_n with the whole correction and R_b at the base predictor: two residual-only
+// assemblies, exact at w = 0 and w = 1, so the model never prefers a weight whose end points are
+// both worse than the base.  The problem's displacement is restored.  n_learned: meshes with w > 0.
 mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, const double* uk, const double* ukm1,
                                   double rho, double* guess, int32_t* n_learned) {
   if (!net || !p || !uk || !ukm1 || !guess || net->ctx != p->uid) return MPF_ERR_ARG;
@@ -897,6 +917,9 @@ mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, con
   const size_t n3 = (size_t)n * 3;
   if (net->usave.n < n3) MPF_TRY(net->usave.alloc(n3));
   if (net->gbuf2.n < n3) MPF_TRY(net->gbuf2.alloc(n3));
+  const int64_t nfree = p->pat.dispDOF;
+  if (net->rnet.n < (size_t)std::max<int64_t>(nfree, 1)) MPF_TRY(net->rnet.alloc(std::max<int64_t>(nfree, 1)));
+  if (net->mdot.n < (size_t)net->nmesh) MPF_TRY(net->mdot.alloc(net->nmesh));
   MPF_HIP(hipStreamSynchronize(net->stream));
   hipStream_t s = p->stream;
   MPF_HIP(hipMemcpyAsync(net->usave.ptr, p->disp.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -912,14 +935,26 @@ mpf_status mpf_mpnn_guarded_guess(mpf_mpnn* net, mpf_problem* p, double lam, con
                          p->disp.ptr);
     MPF_TRY(enqueue_assembly(p, false, lam, 0));
     hipLaunchKernelGGL(k_mesh_sq, dim3(net->nmesh), dim3(256), 0, s, net->doff_d.ptr, p->rhs.ptr, net->msq.ptr);
+    if (c == 0 && nfree > 0)
+      MPF_HIP(hipMemcpyAsync(net->rnet.ptr, p->rhs.ptr, nfree * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (c == 1)
+      hipLaunchKernelGGL(k_mesh_dot, dim3(net->nmesh), dim3(256), 0, s, net->doff_d.ptr, net->rnet.ptr, p->rhs.ptr,
+                         net->mdot.ptr);
     sq[c].resize(net->nmesh);
     MPF_HIP(hipMemcpyAsync(sq[c].data(), net->msq.ptr, net->nmesh * sizeof(double), hipMemcpyDeviceToHost, s));
     MPF_HIP(hipStreamSynchronize(s));
   }
+  std::vector<double> dnb(net->nmesh);
+  MPF_HIP(hipMemcpy(dnb.data(), net->mdot.ptr, net->nmesh * sizeof(double), hipMemcpyDeviceToHost));
   std::vector<float> w(net->nmesh);
   int32_t kept = 0;
   for (int m = 0; m < net->nmesh; ++m) {
-    w[m] = sq[0][m] < sq[1][m] ? 1.f : 0.f;
+    const double a = sq[0][m], b = sq[1][m], cr = dnb[m];  // |R_n|^2, |R_b|^2, R_n.R_b
+    const double den = a + b - 2.0 * cr;                   // |R_n - R_b|^2
+    double wm = den > 0.0 ? (b - cr) / den : 0.0;
+    if (!(wm > 0.0)) wm = 0.0;
+    if (wm > 1.0) wm = 1.0;
+    w[m] = (float)wm;
     kept += w[m] != 0.f;
   }
   if (net->wmesh.n < (size_t)net->nmesh) MPF_TRY(net->wmesh.alloc(net->nmesh));
