@@ -63,6 +63,8 @@ struct PmgWork {
   bool cgrp = false;
   DevBuf<double> sinv;                // [pn]
   DevBuf<double> t, xa, xb, dd, res;  // [d]
+  DevBuf<int32_t> ulen;               // [nub]: displacement columns of each node block's rows
+  bool ulen_ready = false;
   double lambda = 0.0;                // estimate of the largest eigenvalue of B^-1 Kuu
   DevBuf<double> rc, xc;              // [nc]
   DevBuf<double> cr, cr2, cz, cp, cq, cpart, csc;  // coarse CG
@@ -138,32 +140,47 @@ __global__ void __launch_bounds__(kT) k_pres(int64_t pn, const double* __restric
   if (q < pn) zp[q] = vp[q] * sinv[q];
 }
 
-// t_r = v_r - sum over the row's pressure columns of K(r, c) z_p(c - d), displacement rows r
-__global__ void __launch_bounds__(kT) k_kup(int32_t d, const int32_t* __restrict__ rowptr,
-                                            const int32_t* __restrict__ colidx, const double* __restrict__ vals,
-                                            const double* __restrict__ zp, const double* __restrict__ v,
-                                            double* __restrict__ t) {
-  const int r = blockIdx.x * kT + threadIdx.x;
-  if (r >= d) return;
-  double s = v[r];
-  for (int k = rowptr[r + 1] - 1; k >= rowptr[r]; --k) {
-    const int c = colidx[k];
-    if (c < d) break;
-    s -= vals[k] * zp[c - d];
+// ulen[k]: the number of displacement columns (< d) of node block k's rows, which share one column
+// list with the pressure columns (>= d) after them
+__global__ void __launch_bounds__(kT) k_ulen(int64_t nub, const int32_t* __restrict__ bs,
+                                             const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                             int32_t d, int32_t* __restrict__ ulen) {
+  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (k >= nub) return;
+  const int r = bs[k], b0 = rowptr[r];
+  int lo = b0, hi = rowptr[r + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (colidx[mid] < d) lo = mid + 1; else hi = mid;
   }
-  t[r] = s;
+  ulen[k] = lo - b0;
 }
 
-// x = dd = c2 B^-1 t over the node blocks (the first Chebyshev step from x = 0)
+// First Chebyshev step from x = 0 over the node blocks: t = v - Kup z_p on the block's rows (zp
+// non-null: the pressure columns, shared by the block's rows, read once), x = dd = c2 B^-1 t.
 __global__ void __launch_bounds__(kT) k_jac0(int64_t nub, const int32_t* __restrict__ bs,
                                              const int32_t* __restrict__ bm, const double* __restrict__ binv,
-                                             double c2, const double* __restrict__ t, double* __restrict__ x,
-                                             double* __restrict__ dd) {
+                                             const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                             const double* __restrict__ vals, const int32_t* __restrict__ ulen,
+                                             int32_t d, const double* __restrict__ zp, double c2,
+                                             const double* __restrict__ v, double* __restrict__ t,
+                                             double* __restrict__ x, double* __restrict__ dd) {
   const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (k >= nub) return;
   const int s = bs[k], m = bm[k];
   double r[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < m; ++i) r[i] = t[s + i];
+  for (int i = 0; i < m; ++i) r[i] = v[s + i];
+  if (zp) {
+    const int b0 = rowptr[s], len = rowptr[s + 1] - b0;
+    const int b1 = m > 1 ? rowptr[s + 1] : b0, b2 = m > 2 ? rowptr[s + 2] : b0;
+    for (int j = ulen[k]; j < len; ++j) {
+      const double zv = zp[colidx[b0 + j] - d];
+      r[0] -= vals[b0 + j] * zv;
+      if (m > 1) r[1] -= vals[b1 + j] * zv;
+      if (m > 2) r[2] -= vals[b2 + j] * zv;
+    }
+    for (int i = 0; i < m; ++i) t[s + i] = r[i];
+  }
   const double* B = binv + 9 * k;
   for (int i = 0; i < m; ++i) {
     double z = 0.0;
@@ -181,50 +198,56 @@ template <int LPR, int MODE>
 __global__ void __launch_bounds__(kT) k_sweep(int64_t nub, const int32_t* __restrict__ bs,
                                               const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ colidx, const double* __restrict__ vals,
-                                              int32_t d, const double* __restrict__ binv, double c1, double c2,
-                                              const double* __restrict__ t, const double* __restrict__ xin,
-                                              double* __restrict__ xout, double* __restrict__ dd) {
+                                              const int32_t* __restrict__ ulen, const double* __restrict__ binv,
+                                              double c1, double c2, const double* __restrict__ t,
+                                              const double* __restrict__ xin, double* __restrict__ xout,
+                                              double* __restrict__ dd) {
   const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
   constexpr int GPB = kT / LPR;
   for (int64_t k0 = (int64_t)blockIdx.x * GPB; k0 < nub; k0 += (int64_t)gridDim.x * GPB) {
     const int64_t k = k0 + g;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     int st = 0, m = 0;
+    // lane l < m finishes row l of the block: its operands are loaded before the row sums so that
+    // their latency overlaps the sums
+    double tr[3] = {0.0, 0.0, 0.0}, Bl[3] = {0.0, 0.0, 0.0}, xo = 0.0, dq = 0.0;
     if (k < nub) {
       st = bs[k];
       m = bm[k];
-      const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+      if (l < m) {
+        for (int j = 0; j < m; ++j) tr[j] = t[st + j];
+        if (MODE == 1) {
+          for (int j = 0; j < m; ++j) Bl[j] = binv[9 * k + 3 * l + j];
+          xo = xin[st + l];
+          if (c1 != 0.0) dq = dd[st + l];
+        }
+      }
+      const int b0 = rowptr[st], len = ulen[k];
       const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
 #pragma unroll 4
       for (int j = l; j < len; j += LPR) {
-        const int c = colidx[b0 + j];
-        if (c < d) {
-          const double xv = xin[c];
-          s0 += vals[b0 + j] * xv;
-          if (m > 1) s1 += vals[b1 + j] * xv;
-          if (m > 2) s2 += vals[b2 + j] * xv;
-        }
+        const double xv = xin[colidx[b0 + j]];
+        s0 += vals[b0 + j] * xv;
+        if (m > 1) s1 += vals[b1 + j] * xv;
+        if (m > 2) s2 += vals[b2 + j] * xv;
       }
     }
 #pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) {
+    for (int off = LPR / 2; off > 0; off >>= 1) {  // every lane of the group ends with the sums
       s0 += __shfl_xor(s0, off, LPR);
       s1 += __shfl_xor(s1, off, LPR);
       s2 += __shfl_xor(s2, off, LPR);
     }
-    if (k < nub && l == 0) {
-      const double r[3] = {t[st] - s0, m > 1 ? t[st + 1] - s1 : 0.0, m > 2 ? t[st + 2] - s2 : 0.0};
+    if (k < nub && l < m) {
+      const double r[3] = {tr[0] - s0, m > 1 ? tr[1] - s1 : 0.0, m > 2 ? tr[2] - s2 : 0.0};
       if (MODE == 2) {
-        for (int i = 0; i < m; ++i) xout[st + i] = r[i];
+        xout[st + l] = l == 0 ? r[0] : (l == 1 ? r[1] : r[2]);
       } else {
-        const double* B = binv + 9 * k;
-        for (int i = 0; i < m; ++i) {
-          double z = 0.0;
-          for (int j = 0; j < m; ++j) z += B[3 * i + j] * r[j];
-          const double di = (c1 != 0.0 ? c1 * dd[st + i] : 0.0) + c2 * z;
-          dd[st + i] = di;
-          xout[st + i] = xin[st + i] + di;
-        }
+        double z = 0.0;
+        for (int j = 0; j < m; ++j) z += Bl[j] * r[j];
+        const double di = (c1 != 0.0 ? c1 * dq : 0.0) + c2 * z;
+        dd[st + l] = di;
+        xout[st + l] = xo + di;
       }
     }
   }
@@ -236,8 +259,8 @@ template <int LPR>
 __global__ void __launch_bounds__(kT) k_power(int64_t nub, const int32_t* __restrict__ bs,
                                               const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ colidx, const double* __restrict__ vals,
-                                              int32_t d, const double* __restrict__ binv, double c,
-                                              const double* __restrict__ xin, double* __restrict__ y,
+                                              const int32_t* __restrict__ ulen, const double* __restrict__ binv,
+                                              double c, const double* __restrict__ xin, double* __restrict__ y,
                                               double* __restrict__ part) {
   __shared__ double sh[3][kT / 64];
   const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
@@ -250,16 +273,13 @@ __global__ void __launch_bounds__(kT) k_power(int64_t nub, const int32_t* __rest
     if (k < nub) {
       st = bs[k];
       m = bm[k];
-      const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+      const int b0 = rowptr[st], len = ulen[k];
       const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
       for (int j = l; j < len; j += LPR) {
-        const int col = colidx[b0 + j];
-        if (col < d) {
-          const double xv = c * xin[col];
-          s0 += vals[b0 + j] * xv;
-          if (m > 1) s1 += vals[b1 + j] * xv;
-          if (m > 2) s2 += vals[b2 + j] * xv;
-        }
+        const double xv = c * xin[colidx[b0 + j]];
+        s0 += vals[b0 + j] * xv;
+        if (m > 1) s1 += vals[b1 + j] * xv;
+        if (m > 2) s2 += vals[b2 + j] * xv;
       }
     }
 #pragma unroll
@@ -324,17 +344,17 @@ __global__ void __launch_bounds__(kT) k_prolong(int32_t d, const int32_t* __rest
 }
 
 template <int MODE>
-void launch_sweep(int lpr, int64_t nub, const BlockPcView& F, const CsrView& A, int32_t d, double c1, double c2,
-                  const double* t, const double* xin, double* xout, double* dd, hipStream_t s) {
+void launch_sweep(int lpr, int64_t nub, const BlockPcView& F, const CsrView& A, const int32_t* ulen, double c1,
+                  double c2, const double* t, const double* xin, double* xout, double* dd, hipStream_t s) {
   if (lpr == 32)
     hipLaunchKernelGGL((k_sweep<32, MODE>), dim3(grid_for(nub, kT / 32)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
   else if (lpr == 16)
     hipLaunchKernelGGL((k_sweep<16, MODE>), dim3(grid_for(nub, kT / 16)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
   else
     hipLaunchKernelGGL((k_sweep<4, MODE>), dim3(grid_for(nub, kT / 4)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, d, F.binv, c1, c2, t, xin, xout, dd);
+                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
 }
 
 CsrView fine_csr(const mpf_problem* p);
@@ -353,13 +373,13 @@ mpf_status estimate_lambda(mpf_problem* p, PmgWork& W, const BlockPcView& F, int
   std::vector<double> h(3 * (size_t)G);
   for (int it = 0; it < 20; ++it) {
     if (lpr == 32)
-      hipLaunchKernelGGL(k_power<32>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+      hipLaunchKernelGGL(k_power<32>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
                          c, x, y, W.cpart.ptr);
     else if (lpr == 16)
-      hipLaunchKernelGGL(k_power<16>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+      hipLaunchKernelGGL(k_power<16>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
                          c, x, y, W.cpart.ptr);
     else
-      hipLaunchKernelGGL(k_power<4>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, d, F.binv,
+      hipLaunchKernelGGL(k_power<4>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
                          c, x, y, W.cpart.ptr);
     MPF_HIP(hipGetLastError());
     MPF_HIP(hipMemcpyAsync(h.data(), W.cpart.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -523,6 +543,7 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
   MPF_TRY(block_pc_structure(C, W.ncb, W.cbs.ptr, W.cbm.ptr, W.crblk.ptr, &W.cgrp, s));  // synchronises
   W.rowptr_seen = p->pat.rowptr.ptr;
   W.built = true;
+  W.ulen_ready = false;
   return MPF_OK;
 }
 
@@ -560,6 +581,12 @@ mpf_status pmg_setup(mpf_problem* p, const BlockPcView& F, int64_t nub) {
                      A.rowptr, A.colidx, A.vals, (int32_t)W.d, W.pidx.ptr, W.pw.ptr, W.crow.ptr, W.ccol.ptr,
                      W.cval.ptr, W.bad.ptr);
   MPF_TRY(block_pc_values(coarse_csr(W), W.ncb, W.cbs.ptr, W.cbm.ptr, W.cbinv.ptr, s));
+  if (!W.ulen_ready || W.ulen.n != (size_t)nub) {
+    MPF_TRY(W.ulen.alloc(nub > 0 ? nub : 1));
+    hipLaunchKernelGGL(k_ulen, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, A.rowptr, A.colidx, (int32_t)W.d,
+                       W.ulen.ptr);
+    W.ulen_ready = true;
+  }
   MPF_TRY(estimate_lambda(p, W, F, nub, lanes_for(p), &W.lambda));
   if (W.pn > 0) {
     const double scale = (p->m.Kinv > 0.0 && p->m.mu > 0.0) ? 1.0 + 1.0 / (p->m.mu * p->m.Kinv) : 0.0;
@@ -581,11 +608,9 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   const int32_t d = (int32_t)W.d;
   const CsrView A = fine_csr(p);
   const int lpr = lanes_for(p);
-  const double* t = v;
+  const double* t = v;  // right-hand side of the V-cycle: v_u - Kup z_p (written by k_jac0)
   if (W.pn > 0) {
     hipLaunchKernelGGL(k_pres, dim3(grid_for(W.pn, kT)), dim3(kT), 0, s, W.pn, v + d, W.sinv.ptr, z + d);
-    hipLaunchKernelGGL(k_kup, dim3(grid_for(d, kT)), dim3(kT), 0, s, d, A.rowptr, A.colidx, A.vals, z + d, v,
-                       W.t.ptr);
     t = W.t.ptr;
   }
   // Chebyshev coefficients (three-term recurrence of the iteration's corrections)
@@ -604,14 +629,14 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   // pre-smoothing from x = 0
   double* cur = W.xa.ptr;
   double* oth = W.xb.ptr;
-  hipLaunchKernelGGL(k_jac0, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, c2[0], t, cur,
-                     W.dd.ptr);
+  hipLaunchKernelGGL(k_jac0, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, A.rowptr, A.colidx,
+                     A.vals, W.ulen.ptr, d, W.pn > 0 ? z + d : nullptr, c2[0], v, W.t.ptr, cur, W.dd.ptr);
   for (int k = 1; k < W.degree; ++k) {
-    launch_sweep<1>(lpr, nub, F, A, d, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
+    launch_sweep<1>(lpr, nub, F, A, W.ulen.ptr, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
     std::swap(cur, oth);
   }
   // coarse correction
-  launch_sweep<2>(lpr, nub, F, A, d, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
+  launch_sweep<2>(lpr, nub, F, A, W.ulen.ptr, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
   hipLaunchKernelGGL(k_restrict, dim3(grid_for(W.nc, kT)), dim3(kT), 0, s, W.nc, W.rptr.ptr, W.rfine.ptr, W.rw.ptr,
                      W.res.ptr, W.rc.ptr);
   BlockPcView C;
@@ -629,7 +654,7 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   // post-smoothing from the corrected x, the last step into z
   for (int k = 0; k < W.degree; ++k) {
     double* out = (k == W.degree - 1) ? z : oth;
-    launch_sweep<1>(lpr, nub, F, A, d, c1[k], c2[k], t, cur, out, W.dd.ptr, s);
+    launch_sweep<1>(lpr, nub, F, A, W.ulen.ptr, c1[k], c2[k], t, cur, out, W.dd.ptr, s);
     oth = cur;
     cur = out;
   }
