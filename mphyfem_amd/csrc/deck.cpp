@@ -331,6 +331,8 @@ void read_petsc_options(mpf_deck* d, const std::string& path) {
   else d->lin.ksp = MPF_KSP_CG;
   if (pc == "none") d->lin.pc = MPF_PC_NONE;
   else if (pc == "jacobi") d->lin.pc = MPF_PC_JACOBI;
+  else if (pc == "mg" || pc == "gamg" || pc == "ml" || pc == "hypre" || pc == "fieldsplit")
+    d->lin.pc = MPF_PC_PMG;  // multigrid / field split: the two-level p-multigrid (Tet10 meshes)
   else d->lin.pc = MPF_PC_BJACOBI3;  // bjacobi / ilu / sor / asm: the 3x3 node-block Jacobi
 }
 
@@ -536,6 +538,10 @@ mpf_status build(mpf_deck* d) {
   // (SolverPetsc.cpp:117-140), here CG + 3x3 node-block Jacobi; an indefinite mixed system runs
   // GMRES, and the direct solve (displacement systems only) becomes GMRES at 1e-10 for it
   const bool mixed = d->etype == MPF_TET10P1;
+  if (d->lin.pc == MPF_PC_PMG) {  // needs the Tet10 mesh, and changes between applications
+    if (d->etype != MPF_TET10P1 && d->etype != MPF_TET10) d->lin.pc = MPF_PC_BJACOBI3;
+    else if (d->lin.ksp != MPF_KSP_BAND) d->lin.ksp = MPF_KSP_GMRES;
+  }
   if (mixed && (d->lin.ksp == MPF_KSP_CG || d->lin.ksp == MPF_KSP_BAND)) {
     if (d->lin.ksp == MPF_KSP_BAND) d->lin.rtol = 1e-10;
     d->lin.ksp = MPF_KSP_GMRES;

@@ -79,3 +79,18 @@ def test_generated_3d_deck_reads_back(mpf, tmp_path, kind):
     for name, f in faces.items():
         assert np.array_equal(P[name][1], np.unique(f))
     D.close()
+
+
+@pytest.mark.parametrize("pc, want", [("gamg", "PC_PMG"), ("mg", "PC_PMG"), ("fieldsplit", "PC_PMG"),
+                                      ("ilu", "PC_BJACOBI3"), ("jacobi", "PC_JACOBI"), ("none", "PC_NONE")])
+def test_petsc_pc_type_mapping(mpf, tmp_path, pc, want):
+    """-pc_type in petsc_options.dat: multigrid and field-split preconditioners map to the two-level
+    p-multigrid (which mpf_deck_setup keeps for Tet10 meshes only, inside flexible GMRES), the
+    incomplete factorisations and block methods to the 3x3 node-block Jacobi."""
+    X, conn = mpf.mesh_hex(2, 2, 2, jitter=0.0, seed=1)
+    write_gmsh22(tmp_path / "blk.msh", X, conn, 5, "block", {"left": (2, 3, mpf.hex_face_quads(2, 2, 2, 0))})
+    write_config(tmp_path / "config", "blk", "ELEM_SOLID_3D_DISP", 1.0, 1e-3, [("left", "fixed")], [], 1.0, 1.0,
+                 1e-9, petsc=["-ksp_type gmres", f"-pc_type {pc}"])
+    D = mpf.Deck(tmp_path / "config")
+    assert D.info().lin.pc == getattr(mpf, want) and D.info().lin.ksp == mpf.KSP_GMRES
+    D.close()

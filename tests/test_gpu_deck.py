@@ -45,7 +45,9 @@ def test_deck_hex8_stretch_vs_oracle(mpf, orc, gpu, tmp_path):
     D.close()
 
 
-def test_deck_tet10p1_example4_vs_oracle(mpf, orc, gpu, tmp_path):
+@pytest.mark.parametrize("petsc", [None, ["-ksp_type gmres", "-pc_type gamg", "-ksp_rtol 1e-12"]],
+                         ids=["default", "gamg"])
+def test_deck_tet10p1_example4_vs_oracle(mpf, orc, gpu, tmp_path, petsc):
     X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=4)
     sym = {f"sym{'XYZ'[a]}": (2, 9, tet10_boundary_tris(X, conn, a, 0.0, -1.0)) for a in range(3)}
     top = tet10_boundary_tris(X, conn, 2, 1.0, 1.0)
@@ -57,9 +59,13 @@ def test_deck_tet10p1_example4_vs_oracle(mpf, orc, gpu, tmp_path):
            ("symZ", "specified", "UZ", 0.0, None), ("pressure", "specified", "UX", 0.0, None),
            ("pressure", "specified", "UY", 0.0, None)]
     write_config(tmp_path / "config", "block3d", "ELEM_SOLID_3D_MIXED1", mu, Kinv, bcs, [("pressure", p0, 0.0, 1)],
-                 0.2, 1.0, 1e-10, max_iters=20)
+                 0.2, 1.0, 1e-10, max_iters=20, petsc=petsc)
     D = mpf.Deck(tmp_path / "config")
+    if petsc:  # a multigrid preconditioner maps to the two-level p-multigrid inside flexible GMRES
+        assert D.info().lin.pc == mpf.PC_PMG
     P = D.setup()
+    if petsc:
+        assert (D.info().lin.ksp, D.info().lin.pc) == (mpf.KSP_GMRES, mpf.PC_PMG)
     rep = D.run()
     assert rep.converged and rep.steps_done == 5 and rep.final_norm < 1e-10
     u, p = P.state()
