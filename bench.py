@@ -346,8 +346,6 @@ def assembly_bench(M, a, wl: str, n: int):
     """The timed assembly steps of one workload: (bench line dict, problem, host data)."""
     import torch
 
-    if wl == "hex8p0":
-        a.assembly = "scatter"  # Q1/P0 assembles through the coloured scatter only
     log(f"workload {wl}, n = {n}")
     X, conn, P, _ = build_problem(M, n, wl)
     nElem = len(conn)
@@ -385,7 +383,8 @@ def assembly_bench(M, a, wl: str, n: int):
     unique = nnz * 8 + nElem * npe * 4 + len(X) * 48 + (dispDOF + presDOF) * 8 + presDOF * 8
     flops = FLOPS_PER_ELEM[wl] * nElem
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(wl) if a.assembly == "gather" or wl == "hex8p0" else None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(wl if a.assembly == "gather" else wl + "_" + a.assembly),
                 "kernel": f"{kname} per assembly)", "assembly": a.assembly,
                 "kernel_ms_per_assembly": round(ms_kern, 4), "bytes_per_elem": bpe,
                 "unique_bytes": unique,

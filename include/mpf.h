@@ -125,10 +125,11 @@ mpf_status mpf_elem_tet10p1_nh_batch(const double* X, const double* u, const dou
 typedef struct mpf_problem mpf_problem;
 
 /* Creates a problem from host arrays (copied to the device).  elem_type MPF_HEX8, MPF_HEX8P0,      */
-/* MPF_TET10P1 or MPF_TET10.  MPF_HEX8P0 assembles the condensed element matrices by the coloured scatter only  */
-/* (mpf_set_assembly_mode rejects the gather modes), recovers the element pressures in            */
-/* mpf_update (solveForPressure, Elem_Solid_3D_Mixed0.cpp:112-199), and has no batched path       */
-/* (mpf_set_batch rejects it).                                                                     */
+/* MPF_TET10P1 or MPF_TET10.  MPF_HEX8P0 assembles the statically condensed system (displacement    */
+/* rows only; the gather's staging records carry each element's Kup and Kpp, so every block adds    */
+/* -Kup_a Kup_b^T / Kpp, and its residual is condensed) in any assembly mode, recovers the element  */
+/* pressures in mpf_update (solveForPressure, Elem_Solid_3D_Mixed0.cpp:112-199), and batches like   */
+/* MPF_HEX8.                                                                                         */
 mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNode, int32_t nElem,
                               const double* X, const int32_t* conn, const mpf_matl* m);
 mpf_status mpf_problem_destroy(mpf_problem* p);
@@ -346,8 +347,8 @@ typedef struct {
 
 /* Element batch and Jacobi-PCG tile shape by measurement, on the batch's own system: first the     */
 /* assembly (the row-owner gather whose chunk kernel stages at most 8, 16, 24 or 32 distinct       */
-/* elements per chunk in LDS, the gather reading each record directly, or the coloured scatter;    */
-/* Q1/P0: the scatter), each timed over three assemblies, then every (tile, lanes_per_row)         */
+/* elements per chunk in LDS, the gather reading each record directly, or the coloured scatter),   */
+/* each timed over three assemblies, then every (tile, lanes_per_row)                              */
 /* candidate over a few CG iterations; the fastest are kept for mpf_batch_newton_solve.  The       */
 /* gather variants give bitwise the same system.                                                   */
 mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_row, double* ms_per_cg_iter);

@@ -711,18 +711,14 @@ mpf_status batch_autotune(mpf_problem* p, int32_t* T_out, int32_t* LPR_out, doub
   MPF_HIP(hipEventCreate(&e0));
   MPF_HIP(hipEventCreate(&e1));
   // element batch: the row-owner gather with its chunk kernel staging at most cap distinct elements
-  // per chunk in LDS (cap 8 .. 32), the gather reading every record directly, or the coloured scatter
-  // (Q1/P0: the coloured scatter only), each timed over three first-iteration assemblies
+  // per chunk in LDS (cap 8 .. 32), the gather reading every record directly, or the coloured scatter,
+  // each timed over three first-iteration assemblies
   {
     struct Cand { int32_t mode, cap; };
     std::vector<Cand> cands;
-    if (p->p0) {
-      cands.push_back({MPF_ASSEMBLY_COLORED_SCATTER, p->chunk_cap});
-    } else {
-      for (int32_t cap : {8, 16, 24, 32}) cands.push_back({MPF_ASSEMBLY_GATHER, cap});
-      cands.push_back({MPF_ASSEMBLY_GATHER_DIRECT, p->chunk_cap});
-      cands.push_back({MPF_ASSEMBLY_COLORED_SCATTER, p->chunk_cap});
-    }
+    for (int32_t cap : {8, 16, 24, 32}) cands.push_back({MPF_ASSEMBLY_GATHER, cap});
+    cands.push_back({MPF_ASSEMBLY_GATHER_DIRECT, p->chunk_cap});
+    cands.push_back({MPF_ASSEMBLY_COLORED_SCATTER, p->chunk_cap});
     double abest = 1e300;
     Cand pick = cands[0];
     for (const Cand& c : cands) {

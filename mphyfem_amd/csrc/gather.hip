@@ -40,23 +40,40 @@ constexpr int kMaxNB = 128;   // nodes coupled to one node (column blocks of its
 //   Hex8:      J [NGP][9]       the inverse Jacobian (g_a = grad_jinv(J, dN_a/dxi));
 //              B [NGP][6]       b00 b11 b22 b01 b02 b12   (g and b g formed on use: 184 doubles per
 //                               element instead of 448)
-//   S  [NGP][SS]       displacement elements: s1..s5 (SS = 5); Tet10/P1: s1..s4, dv, dv0, thetahat, resip
-//                      (SS = 8)
-//   R  [NPE][3]        element residual (no lift)
-// The first SL doubles (G or J, B, S) are what the matrix kernel reads for a block.
+//   S  [NGP][SS]       displacement elements: s1..s5 (SS = 5); mixed elements: s1..s4, dv, dv0,
+//                      thetahat, resip (SS = 8)
+//   W  [26]            Hex8 Q1/P0 only: Kup [NPE][3], Kpp, 0 (the static condensation's coupling)
+//   R  [NPE][3]        element residual (no lift; Q1/P0: condensed, Fu - Kup Fp / Kpp)
+// The first SL doubles (G or J, B, S, W) are what the matrix kernel reads for a block.
+// MIXED with NPE = 8 is the Hex8 Q1/P0 element (one pressure per element, condensed; the system has
+// displacement rows only), with NPE = 10 the Tet10/P1 element (pressure rows on the corner nodes).
 template <int NPE, int NGP, bool MIXED>
 struct Stage {
-  static constexpr bool COMPACT = NPE == 8 && !MIXED;
+  static constexpr bool P0 = MIXED && NPE == 8;
+  static constexpr bool P1 = MIXED && NPE != 8;
+  static constexpr bool COMPACT = NPE == 8;
   static constexpr int GW = COMPACT ? 3 : 6;      // doubles per node and Gauss point of g data
   static constexpr int SS = MIXED ? 8 : 5;
   static constexpr int G0 = 0;
   static constexpr int B0 = COMPACT ? NGP * 9 : NGP * NPE * GW;
   static constexpr int S0 = B0 + (COMPACT ? NGP * 6 : 0);
-  static constexpr int R0 = S0 + NGP * SS;
+  static constexpr int W0 = S0 + NGP * SS;
+  static constexpr int WN = P0 ? 26 : 0;
+  static constexpr int R0 = W0 + WN;
   static constexpr int SL = R0;
   static constexpr int TS = ((R0 + NPE * 3) + 7) / 8 * 8;
-  static_assert(B0 % 2 == 0 && S0 % 2 == 0 && SL % 2 == 0, "16-byte groups");
+  static_assert(B0 % 2 == 0 && S0 % 2 == 0 && W0 % 2 == 0 && SL % 2 == 0, "16-byte groups");
 };
+
+// Hex8 Q1/P0: the static condensation's part of block (a, b), -Kup_a Kup_b^T / Kpp
+// (Elem_Solid_3D_Mixed0.cpp:93-104), from the record's W
+__device__ __forceinline__ void add_condensed(const double* W, int a, int b, double Kb[9]) {
+  const double kpp = W[24];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Kb[3 * i + j] = Kb[3 * i + j] - W[3 * a + i] * (W[3 * b + j] / kpp);
+}
 
 // g, b g of node c at Gauss point gp from a staging record, either layout
 // (dN: Hex8 dN/dxi table [NGP][NPE][3], e.g. in LDS; null: computed here)
@@ -95,6 +112,7 @@ __device__ __forceinline__ void record_block(const double* rec, int a, int b, do
     const double* S = rec + St::S0 + gp * St::SS;
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], Kb);
   }
+  if (St::P0) add_condensed(rec + St::W0, a, b, Kb);
 }
 
 // Stage kernel.  OUT_LDS: the wavefront's EPW records are written to LDS and then copied to the
@@ -107,14 +125,20 @@ __device__ __forceinline__ void record_block(const double* rec, int a, int b, do
 #ifndef MPF_STAGE_LDS_TET
 #define MPF_STAGE_LDS_TET 1
 #endif
+// Q1/P0 (qKup, qKpp, qFp non-null): each element's Kup [24], Kpp and Fp also go to these arrays for
+// the pressure recovery after the solve.
 template <int NPE, int NGP, bool MIXED, int EPW, bool OUT_LDS>
-__global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double* __restrict__ GS) {
+__global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double* __restrict__ GS,
+                                              double* __restrict__ qKup, double* __restrict__ qKpp,
+                                              double* __restrict__ qFp) {
   static_assert(EPW * NGP == 64, "one wavefront holds EPW elements x NGP Gauss points");
   using St = Stage<NPE, NGP, MIXED>;
   static_assert(St::TS % 2 == 0, "16-byte records");
+  static_assert(!St::P0 || NGP == NPE, "Q1/P0: one node per Gauss-point lane in the condensation");
   __shared__ double sX[EPW][NPE][3], sU[EPW][NPE][3], sUP[MIXED ? EPW : 1][NPE][3], sP[EPW][4];
   __shared__ double sdN[NGP][3][NPE];
   __shared__ double2 sOut[OUT_LDS ? EPW * St::TS / 2 : 1];
+  __shared__ double sQ[St::P0 ? EPW * NGP * 26 : 1];  // per lane: dv g_a (24), -dv0 thetahat, -dv0 resip
   const int lane = threadIdx.x;
   const int base = blockIdx.x * EPW;
   if (lane < NGP) {
@@ -141,7 +165,8 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
       sU[k][a][c] = A.u[3 * nd + c];
       if (MIXED) sUP[k][a][c] = A.uprev[3 * nd + c];
     }
-    if (MIXED && a < 4) sP[k][a] = A.pres[nd];
+    if (St::P1 && a < 4) sP[k][a] = A.pres[nd];
+    if (St::P0 && a == 0) sP[k][0] = A.pres[e];
   }
   __syncthreads();
   const int k = lane / NGP, gp = lane % NGP;
@@ -153,10 +178,57 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   double mu, K;
   elem_material(A, writer ? e : base, mu, K);
   element_point<NPE, NGP, MIXED, St::COMPACT>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp],
-                                              gp, wgp, mu, K, A.Utype, A.consistent != 0, writer,
+                                              gp, wgp, mu, K, A.Utype, St::P0 ? false : A.consistent != 0, writer,
                                               rec + St::G0 + gp * (St::COMPACT ? 9 : NPE * St::GW),
                                               rec + St::S0 + gp * St::SS, rec + St::R0, negJ, rec + St::B0 + gp * 6);
   if (writer && negJ) flag_bad(A.bad, e);
+  if constexpr (St::P0) {
+    // Kup_a = sum_gp dv g_a (ElementBase2.cpp:500-505 with Np = 1), Kpp = -sum_gp dv0 thetahat,
+    // Fp = -sum_gp dv0 resip (:510-539), each summed in Gauss-point order by the lane of node a = gp;
+    // g_a from this lane's inverse Jacobian as element_point formed it
+    const double* Sg = rec + St::S0 + gp * St::SS;
+    double* q = sQ + (size_t)lane * 26;
+    double Jr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Jr[i] = rec[St::G0 + gp * 9 + i];
+    const double dv = Sg[4], dv0 = Sg[5];
+#pragma unroll
+    for (int a = 0; a < NPE; ++a) {
+      double g[3];
+      const double da[3] = {sdN[gp][0][a], sdN[gp][1][a], sdN[gp][2][a]};
+      grad_jinv(Jr, da, g);
+      q[3 * a] = dv * g[0]; q[3 * a + 1] = dv * g[1]; q[3 * a + 2] = dv * g[2];
+    }
+    q[24] = dv0 * Sg[6];
+    q[25] = dv0 * Sg[7];
+    __syncthreads();
+    const int a = gp;
+    const double* q0 = sQ + (size_t)k * NGP * 26;
+    double u3[3] = {0.0, 0.0, 0.0}, kpp = 0.0, fp = 0.0;
+    for (int g2 = 0; g2 < NGP; ++g2) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u3[i] += q0[g2 * 26 + 3 * a + i];
+      fp -= q0[g2 * 26 + 25];
+      kpp -= q0[g2 * 26 + 24];
+    }
+    if (writer) {
+      double* W = rec + St::W0;
+      double* R = rec + St::R0;
+      const double fps = fp / kpp;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        W[3 * a + i] = u3[i];
+        R[3 * a + i] = R[3 * a + i] - u3[i] * fps;  // the lane that wrote node a's residual (a = gp)
+        qKup[(size_t)e * 24 + 3 * a + i] = u3[i];
+      }
+      if (a == 0) {
+        W[24] = kpp;
+        W[25] = 0.0;
+        qKpp[e] = kpp;
+        qFp[e] = fp;
+      }
+    }
+  }
   if (OUT_LDS) {
     __syncthreads();
     const int nv = nElem - base < EPW ? nElem - base : EPW;
@@ -554,7 +626,7 @@ __device__ __forceinline__ void item_gps(const double* G, const double* Bv, cons
       gb[0] = q0.x; gb[1] = q0.y; gb[2] = q1.x; bb[0] = q1.y; bb[1] = q2.x; bb[2] = q2.y;
     }
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], v);
-    if (MIXED) {
+    if (St::P1) {
       double Np[4];
       tet_Np(gp0 + gp, Np);
       const double dv = S[4], dv0 = S[5], th = S[6];
@@ -580,6 +652,7 @@ __device__ __forceinline__ void item_block(const double* rec, int a, int b, doub
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
   item_gps<NPE, NGP, MIXED, NV, NGP, UNROLL, St::COMPACT>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
+  if (St::P0) add_condensed(rec + St::W0, a, b, v);
 }
 
 // Node a's residual rows of one element (v[0..2]) and, for the mixed element, its pressure row (v[3])
@@ -588,7 +661,7 @@ __device__ __forceinline__ void item_residual(const double* rec, int a, double* 
   using St = Stage<NPE, NGP, MIXED>;
 #pragma unroll
   for (int i = 0; i < 3; ++i) v[i] = rec[St::R0 + a * 3 + i];
-  if (MIXED && a < 4) {
+  if (St::P1 && a < 4) {
     double fp = 0.0;  // Fp_a -= (dv0 Np_a) resip   (ElementBase2.cpp:510, 522)
 #pragma unroll
     for (int gp = 0; gp < NGP; ++gp) {
@@ -739,8 +812,9 @@ __device__ __forceinline__ void run_heads(const FlatArgs& F, int n, int lane, in
 template <int NPE, int NGP, bool MIXED>
 __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
-  constexpr int NV = MIXED ? 16 : 9;  // uu 9 | up 3 | pu 3 | pp 1
-  constexpr int NR = MIXED ? 4 : 3;   // right-hand-side rows of a node (self items)
+  constexpr bool P1 = St::P1;        // pressure rows (Tet10/P1)
+  constexpr int NV = P1 ? 16 : 9;    // uu 9 | up 3 | pu 3 | pp 1
+  constexpr int NR = P1 ? 4 : 3;     // right-hand-side rows of a node (self items)
   constexpr int NT = NV + NR;
   __shared__ int32_t sCol[64];
   __shared__ double sPart[64 * NT];
@@ -760,14 +834,14 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
   if (self) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
   // the run head's table entry and node, loaded before the barrier (registers are the limit
   // during the block computation)
-  BlockEntry<MIXED> bt;
+  BlockEntry<P1> bt;
   bt.load(F.btab, blk);
   const int n = self ? F.fnode[w * 64 + lane] : 0;
   sCol[lane] = live ? blk : -1;
 #pragma unroll
   for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, n, lane, blk, v, sCol, sPart, bt);
+  if (live) run_heads<P1, NV, NT>(F, n, lane, blk, v, sCol, sPart, bt);
 }
 
 // Chunked matrix kernel: one workgroup per chunk (the wavefronts of one layout group of 8 nodes in
@@ -792,11 +866,14 @@ __global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
 #ifndef MPF_LDS_PAD
 #define MPF_LDS_PAD 6
 #endif
-template <int NPE, int NGP, bool MIXED, int CW, int ECAP>
+// ECAP: the chunk table's slots per chunk; SCAP (<= ECAP): the most distinct elements staged in LDS
+// (Q1/P0 records are longer, so fewer fit where two workgroups share a CU's LDS).
+template <int NPE, int NGP, bool MIXED, int CW, int ECAP, int SCAP = ECAP>
 __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
-  constexpr int NV = MIXED ? 16 : 9;
-  constexpr int NR = MIXED ? 4 : 3;
+  constexpr bool P1 = St::P1;
+  constexpr int NV = P1 ? 16 : 9;
+  constexpr int NR = P1 ? 4 : 3;
   constexpr int NT = NV + NR;
   // LDS slot: g of every node and Gauss point (Hex8: formed here from the record's inverse
   // Jacobians), then the record's B and S copied verbatim (Tet10/P1: the record's G and S verbatim)
@@ -805,7 +882,8 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   constexpr int CP2 = (St::SL - CP0) / 2;                // its 16-byte groups
   constexpr int SLP = GX + (St::SL - CP0) + MPF_LDS_PAD; // LDS stride per element slot (doubles)
   static_assert(SLP % 2 == 0 && GX % 2 == 0 && CP0 % 2 == 0, "16-byte slots");
-  constexpr int STG = ECAP * SLP, PART = CW * 64 * NT;
+  static_assert(SCAP <= ECAP, "staged elements fit the chunk table");
+  constexpr int STG = SCAP * SLP, PART = CW * 64 * NT;
   constexpr int LDSD = STG > PART ? STG : PART;  // record copies, then the run sums
   static_assert(ECAP <= 255, "slot tables");
   __shared__ double2 sMem[LDSD / 2];
@@ -815,7 +893,7 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   const int64_t c = xcd_chunk(blockIdx.x, F.nchunks);
   if (c >= F.nchunks) return;  // whole workgroup, before any barrier
   const int cnt = F.ccnt[c];
-  const bool staged = cnt <= ECAP && cnt <= F.ecap;
+  const bool staged = cnt <= SCAP && cnt <= F.ecap;
   // per-lane loads that need nothing from LDS, issued before the staging so that their latency
   // overlaps it: the item, its slot, its block's table entry, and (self items) the node and the
   // element's residual rows
@@ -827,7 +905,7 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
   const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
   const bool self = live && a == b;
   const int slot = live && staged ? F.fslot[w * 64 + lane] : 0;
-  BlockEntry<MIXED> bt;
+  BlockEntry<P1> bt;
   bt.load(F.btab, blk);
   const int n = self ? F.fnode[w * 64 + lane] : 0;
   const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
@@ -842,7 +920,7 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
     constexpr int BS = 64 * CW;
     // Hex8: one lane per (element slot, Gauss point) forms g of the 8 nodes from the record's inverse
     // Jacobian; its loads are issued before the verbatim copy, so both share one memory latency
-    static_assert(!St::COMPACT || ECAP * NGP <= BS, "one (slot, Gauss point) per lane");
+    static_assert(!St::COMPACT || SCAP * NGP <= BS, "one (slot, Gauss point) per lane");
     const bool jl = St::COMPACT && tid < cnt * NGP;
     double Jr[9];
     if (St::COMPACT) {
@@ -885,8 +963,10 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
     if (staged) {
       const double* sl = reinterpret_cast<const double*>(sMem) + slot * SLP;
       item_gps<NPE, NGP, MIXED, NV, NGP, 1>(sl, sl + GX + (St::B0 - CP0), sl + GX + (St::S0 - CP0), 0, a, b, v);
+      if (St::P0) add_condensed(sl + GX + (St::W0 - CP0), a, b, v);
     } else {
       item_gps<NPE, NGP, MIXED, NV, NGP, 1, St::COMPACT>(rec + St::G0, rec + St::B0, rec + St::S0, 0, a, b, v);
+      if (St::P0) add_condensed(rec + St::W0, a, b, v);
     }
   }
   __syncthreads();  // the record copies become the run-sum area
@@ -895,7 +975,7 @@ __global__ void __launch_bounds__(64 * CW) k_flat_chunk(FlatArgs F) {
 #pragma unroll
   for (int q = 0; q < NT; ++q) part[lane * NT + q] = v[q];
   __syncthreads();
-  if (live) run_heads<MIXED, NV, NT>(F, n, lane, blk, v, sCol + wid * 64, part, bt);
+  if (live) run_heads<P1, NV, NT>(F, n, lane, blk, v, sCol + wid * 64, part, bt);
 }
 
 // Chunk tables (once per pattern), one workgroup of 512 lanes per chunk of at most 8 wavefronts: the
@@ -976,7 +1056,7 @@ __global__ void __launch_bounds__(256) k_rhs(RhsArgs R) {
   if (n >= R.nNode) return;
   const int info = R.nodeinfo[n];
   const int mn = info & 7;
-  const bool isPn = MIXED && ((info >> 3) & 1);
+  const bool isPn = St::P1 && ((info >> 3) & 1);
   if (!mn && !isPn) return;
   const int e0 = R.nptr[n], m = R.nptr[n + 1] - e0;
   if (R.lift_only && m > 0) return;
@@ -1026,7 +1106,7 @@ __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
   __syncthreads();
   const int info = L.nodeinfo[n];
   const int mn = info & 7;
-  const bool isPn = MIXED && ((info >> 3) & 1);
+  const bool isPn = St::P1 && ((info >> 3) & 1);
   if (!mn && !isPn) return;
   const int e0 = L.nptr[n], m = L.nptr[n + 1] - e0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1045,7 +1125,7 @@ __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
     record_block<NPE, NGP, MIXED>(rec, a, b, Kb, St::COMPACT ? sdN : nullptr);
 #pragma unroll
     for (int i = 0; i < 3; ++i) acc[i] += Kb[3 * i] * db[0] + Kb[3 * i + 1] * db[1] + Kb[3 * i + 2] * db[2];
-    if (MIXED && isPn && a < 4) {
+    if (St::P1 && isPn && a < 4) {
       double pu[3] = {0.0, 0.0, 0.0};  // Kpu(a, 3b+k) = sum (dv Np_a) g_b,k
       for (int gp = 0; gp < NGP; ++gp) {
         double Np[4];
@@ -1071,8 +1151,8 @@ __global__ void __launch_bounds__(64) k_lift(LiftArgs L) {
 
 }  // namespace
 
-size_t stage_doubles_per_elem(int etype) {
-  if (etype == MPF_HEX8) return (size_t)Stage<8, 8, false>::TS;
+size_t stage_doubles_per_elem(int etype, bool p0) {
+  if (etype == MPF_HEX8) return p0 ? (size_t)Stage<8, 8, true>::TS : (size_t)Stage<8, 8, false>::TS;
   if (etype == MPF_TET10) return (size_t)Stage<10, 4, false>::TS;
   return (size_t)Stage<10, 4, true>::TS;
 }
@@ -1314,7 +1394,7 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
     MPF_HIP(hipMemcpyAsync(&P.g_nnear, nsel.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     MPF_HIP(hipStreamSynchronize(s));
   }
-  MPF_TRY(p->GS.alloc((size_t)p->nElem * stage_doubles_per_elem(p->etype)));
+  MPF_TRY(p->GS.alloc((size_t)p->nElem * stage_doubles_per_elem(p->etype, p->p0)));
   MPF_HIP(hipStreamSynchronize(s));
   P.gather_ready = true;
   return MPF_OK;
@@ -1364,13 +1444,23 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   const bool chunked = p->asm_mode != MPF_ASSEMBLY_GATHER_DIRECT && F.nchunks > 0;
   if (p->etype == MPF_TET10) {
     hipLaunchKernelGGL((k_stage<10, 4, false, 16, MPF_STAGE_LDS_TET>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A,
-                       nElem, p->GS.ptr);
+                       nElem, p->GS.ptr, nullptr, nullptr, nullptr);
     if (flat) hipLaunchKernelGGL((k_flat<10, 4, false>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<10, 4, false>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
     if (lift) hipLaunchKernelGGL((k_lift<10, 4, false>), dim3(L.nnear), dim3(64), 0, s, L);
+  } else if (p->p0) {  // Hex8 Q1/P0: condensed records, displacement rows only
+    hipLaunchKernelGGL((k_stage<8, 8, true, 8, MPF_STAGE_LDS_HEX>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A,
+                       nElem, p->GS.ptr, p->q_Kup.ptr, p->q_Kpp.ptr, p->q_Fp.ptr);
+    if (flat && chunked)
+      hipLaunchKernelGGL((k_flat_chunk<8, 8, true, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS, 28>),
+                         dim3(chunk_grid), dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
+    else if (flat)
+      hipLaunchKernelGGL((k_flat<8, 8, true>), dim3(flat_grid), dim3(64), 0, s, F);
+    hipLaunchKernelGGL((k_rhs<8, 8, true>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
+    if (lift) hipLaunchKernelGGL((k_lift<8, 8, true>), dim3(L.nnear), dim3(64), 0, s, L);
   } else if (!mixed) {
     hipLaunchKernelGGL((k_stage<8, 8, false, 8, MPF_STAGE_LDS_HEX>), dim3(grid_for(nElem, 8)), dim3(64), 0, s, A,
-                       nElem, p->GS.ptr);
+                       nElem, p->GS.ptr, nullptr, nullptr, nullptr);
     if (flat && chunked)
       hipLaunchKernelGGL((k_flat_chunk<8, 8, false, MPF_CHUNK_WAVES, MPF_CHUNK_ELEMS>),
                          dim3(chunk_grid), dim3(64 * MPF_CHUNK_WAVES), 0, s, F);
@@ -1380,7 +1470,7 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
     if (lift) hipLaunchKernelGGL((k_lift<8, 8, false>), dim3(L.nnear), dim3(64), 0, s, L);
   } else {
     hipLaunchKernelGGL((k_stage<10, 4, true, 16, MPF_STAGE_LDS_TET>), dim3(grid_for(nElem, 16)), dim3(64), 0, s, A,
-                       nElem, p->GS.ptr);
+                       nElem, p->GS.ptr, nullptr, nullptr, nullptr);
     if (flat) hipLaunchKernelGGL((k_flat<10, 4, true>), dim3(flat_grid), dim3(64), 0, s, F);
     hipLaunchKernelGGL((k_rhs<10, 4, true>), dim3(grid_for(nNode, 256)), dim3(256), 0, s, R);
     if (lift) hipLaunchKernelGGL((k_lift<10, 4, true>), dim3(L.nnear), dim3(64), 0, s, L);

@@ -217,7 +217,7 @@ struct mpf_problem {
 
 namespace mpf {
 uint64_t next_problem_uid();                        // problem.hip: unique per process
-size_t stage_doubles_per_elem(int etype);          // gather.hip
+size_t stage_doubles_per_elem(int etype, bool p0);  // gather.hip
 mpf_status ensure_gather_tables(mpf_problem* p);   // gather.hip (falls back to the scatter when unsupported)
 mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const double* dispApplied, int writeK,
                                   hipStream_t s);

@@ -501,7 +501,6 @@ mpf_status mpf_problem_create(mpf_problem** out, int32_t elem_type, int32_t nNod
   p->npres = p0 ? nElem : nNode;
   p->m = *m;
   p->m.mixed = (elem_type == MPF_HEX8 || elem_type == MPF_TET10) ? 0 : 1;
-  if (p0) p->asm_mode = MPF_ASSEMBLY_COLORED_SCATTER;  // condensed element matrices go through the slot map
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     delete p;
     return MPF_ERR_HIP;
@@ -700,7 +699,6 @@ mpf_status mpf_set_assembly_mode(mpf_problem* p, int32_t mode) {
   if (!p || p->etype == 0 || (mode != MPF_ASSEMBLY_GATHER && mode != MPF_ASSEMBLY_COLORED_SCATTER &&
              mode != MPF_ASSEMBLY_GATHER_DIRECT))
     return MPF_ERR_ARG;
-  if (p->p0 && mode != MPF_ASSEMBLY_COLORED_SCATTER) return MPF_ERR_ARG;  // Q1/P0: coloured scatter only
   p->asm_mode = mode;
   if (p->prepared && mode != MPF_ASSEMBLY_COLORED_SCATTER) MPF_TRY(ensure_gather_tables(p));
   return MPF_OK;

@@ -138,10 +138,12 @@ def test_hex8p0_relieves_locking_on_gpu(mpf, orc, gpu):
     assert normwise_rel(uz["q1p0"], uo) < 1e-9, normwise_rel(uz["q1p0"], uo)
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["gather", "colored_scatter"])
 @pytest.mark.parametrize("ksp", ["band", "cg"])
-def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp):
-    """Hex8 Q1/P0 problem on the GPU (condensed element matrices scattered colour by colour, element
-    pressures recovered after each correction) against the CPU Q1/P0 driver on the oracle's element:
+def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp, mode):
+    """Hex8 Q1/P0 problem on the GPU (condensed element matrices through the row-owner gather or
+    scattered colour by colour, element pressures recovered after each correction) against the CPU
+    Q1/P0 driver on the oracle's element:
     a near-incompressible jittered block stretched 10% in two load steps, converged displacements
     and element pressures within 1e-9."""
     n, mu, Kinv = 3, 1.0, 1e-3
@@ -151,9 +153,8 @@ def test_hex8p0_newton_vs_oracle(mpf, orc, gpu, ksp):
     P = mpf.Problem(mpf.HEX8P0, X, conn, m)
     apply_bcs(P, bcs)
     P.prepare_matrix_pattern()
-    assert P.assembly_mode() == mpf.ASSEMBLY_COLORED_SCATTER
-    with pytest.raises(mpf.MpfError):
-        P.set_assembly_mode(mpf.ASSEMBLY_GATHER)
+    assert P.assembly_mode() == mpf.ASSEMBLY_GATHER  # condensed records through the row-owner gather
+    P.set_assembly_mode(mode)
     if ksp == "band":
         opts = P.newton_opts(nsteps=2, dt=0.5, itmax=20, rtol=1e-10, ksp=mpf.KSP_BAND)
     else:
@@ -339,6 +340,44 @@ def test_global_assembly_tet_vs_oracle(mpf, orc, gpu, mode):
         assert normwise_rel(vals, ov) < TOL_K
         assert normwise_rel(rhs, orhs) < TOL_K
     assert P.assembly_mode() == mode
+
+
+@pytest.mark.parametrize("mode,cap", [(0, 32), (0, 12), (2, 32), (1, 32)],
+                         ids=["gather", "gather_partly_staged", "gather_direct", "colored_scatter"])
+def test_global_assembly_hex8p0_vs_oracle(mpf, orc, gpu, mode, cap):
+    """Hex8 Q1/P0 condensed system (Kc = Kuu - Kup Kpu / Kpp, Fc = Fu - Kup Fp / Kpp) through the
+    row-owner gather (chunk kernel staging the condensed records in LDS, some chunks reading them
+    directly, or every chunk reading directly) and through the coloured scatter: global K and R
+    against the oracle's condensed assembly, with the first-iteration lift and without; the gather
+    bitwise repeatable."""
+    nx, ny, nz = 7, 5, 6
+    X, conn = mpf.mesh_hex(nx, ny, nz, L=(1.0, 0.8, 0.9), jitter=0.1, seed=6)
+    mu, Kinv = 1.0, 1e-3
+    bcs = [(mpf.face_nodes(mpf.HEX8, nx, ny, nz, 0), -1, None), (mpf.face_nodes(mpf.HEX8, nx, ny, nz, 1), 0, 0.05)]
+    P = mpf.Problem(mpf.HEX8P0, X, conn, mpf.matl(mu, Kinv, 3))
+    O = orc.Problem(9, X, conn, mu, Kinv, 3)
+    u = random_state(len(X), 0.004, seed=41)
+    pe = np.random.default_rng(42).uniform(-0.5, 0.5, len(conn))
+    for Q in (P, O):
+        apply_bcs(Q, bcs)
+        Q.prepare_matrix_pattern()
+        Q.set_state(disp=u, disp_prev=0.5 * u, pres=pe, pres_prev=pe)
+        Q.set_boundary_conditions(0.8)
+    P.set_assembly_mode(mode)
+    P.set_chunk_stage_cap(cap)
+    for first in (True, False):
+        P.calc_stiffness_and_residual(first, 0.8)
+        vals, rhs = P.system()
+        P.calc_stiffness_and_residual(first, 0.8)
+        v2, r2 = P.system()
+        if mode != 1:
+            assert np.array_equal(vals, v2) and np.array_equal(rhs, r2)
+        ov, orhs, st = O.assemble(first, 0.8)
+        assert st == 0
+        print("Q1/P0 mode", mode, "cap", cap, "first", first, normwise_rel(vals, ov), normwise_rel(rhs, orhs),
+              flush=True)
+        assert normwise_rel(vals, ov) < TOL_K
+        assert normwise_rel(rhs, orhs) < TOL_K
 
 
 @pytest.mark.parametrize("cap", [32, 20, 0], ids=["staged", "mixed", "direct_reads"])
