@@ -253,7 +253,7 @@ def bench_direct(M, a) -> int:
         P.batch_newton_step((k + 1) / nsteps, rep=rep, **opts)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    s_solve = rep.seconds_solve / max(rep.newton_iters, 1)
+    s_solve = P.batch_time_direct(5) * 1e-3  # the Newton iterations' graphs hold assembly and solve together
     achieved = flops / s_solve / 1e12
     out = {"metric": "s per converged load step (200 Hex8 cantilevers, batched full Newton + banded LDL^T)",
            "value": round(dt / steps, 4), "unit": "s", "n_gpus": 1, "steps": steps, "warmup": warm,
@@ -268,7 +268,7 @@ def bench_direct(M, a) -> int:
                                   "(one direct solve of every beam; FP64 MFMA peak)",
                         "flops_per_solve": flops, "ms_per_solve": round(s_solve * 1e3, 3)},
            "newton": {"batch_iterations": rep.newton_iters, "graph_launches": rep.graph_launches,
-                      "seconds_assembly": round(rep.seconds_assembly, 4), "seconds_solve": round(rep.seconds_solve, 4)}}
+                      "seconds_iteration_graphs": round(rep.seconds_assembly, 4)}}
     P.close()
     print(json.dumps(out), flush=True)
     return 0

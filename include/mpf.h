@@ -355,6 +355,11 @@ mpf_status mpf_batch_autotune(mpf_problem* p, int32_t* tile, int32_t* lanes_per_
 /* The autotuner's assembly pick: MPF_ASSEMBLY_* mode, staged-element cap, ms per assembly          */
 /* (MPF_ERR_STATE before mpf_batch_autotune).                                                      */
 mpf_status mpf_batch_assembly_choice(const mpf_problem* p, int32_t* mode, int32_t* stage_cap, double* ms_per_assembly);
+/* Batched solver 1: milliseconds per direct solve of every mesh (band fill, factorisation, solve and */
+/* scatter of the last assembled system, timed with HIP events over reps solves on the problem's     */
+/* stream; the solution and the factors are those of that system).  MPF_ERR_STATE before a step.   */
+/* No counterpart in the reference (measurement only).                                              */
+mpf_status mpf_batch_time_direct(mpf_problem* p, int32_t reps, double* ms_per_solve);
 /* Newton-Raphson over nsteps load steps for every mesh at once (femSolidmechanics2.cpp:730-851  */
 /* per mesh) with Jacobi-PCG per mesh.  mesh_iters [nsteps][nmesh]: assemblies to convergence;    */
 /* mesh_relres [nmesh]: final ||R_m|| / ||R0_m||.  Either may be NULL.                           */

@@ -198,6 +198,7 @@ SIGNATURES = [
     ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
     ("mpf_set_pmg", C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_int32]),
     ("mpf_batch_assembly_choice", C.c_int, [_P, _I32P, _I32P, _DP]),
+    ("mpf_batch_time_direct", C.c_int, [_P, C.c_int32, _DP]),
     ("mpf_pmg_stats", C.c_int, [_P, C.POINTER(PmgReport)]),
     ("mpf_mesh_hex_sizes", None, [C.c_int32, C.c_int32, C.c_int32, _I32P, _I32P]),
     ("mpf_mesh_hex", None, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -732,6 +733,12 @@ class Problem:
         t, l, ms = C.c_int32(), C.c_int32(), C.c_double()
         _check(lib().mpf_batch_autotune(self._h, C.byref(t), C.byref(l), C.byref(ms)), "mpf_batch_autotune")
         return t.value, l.value, ms.value
+
+    def batch_time_direct(self, reps: int = 5) -> float:
+        """ms per direct solve of every mesh of the last assembled system (batched solver BAND)."""
+        ms = C.c_double()
+        _check(lib().mpf_batch_time_direct(self._h, reps, C.byref(ms)), "mpf_batch_time_direct")
+        return ms.value
 
     def batch_assembly_choice(self):
         """(ASSEMBLY_* mode, staged-element cap, ms per assembly) picked by batch_autotune."""

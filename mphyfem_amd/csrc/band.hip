@@ -100,10 +100,13 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // PR = kW + bmax, then kW pivots.
 //
 // Panel step at column j0 (w = min(kW, n - j0) columns, panel-relative rows 0 .. w - 1 + b):
-//   (1) the w x w diagonal block is factored by wavefront 0 in registers (lane i holds row i;
-//       the pivot column is read across lanes);
-//   (2) every row r >= w of the panel solves u_c = A(r, c) - sum_{k<c} u_k L(c, k),
-//       L(r, c) = u_c / d_c on its own (one thread per row);
+//   (1) the panel is factored right-looking, one column at a time, one thread per panel row (the
+//       diagonal block's rows included): at column c every row r > c takes
+//       A(r, c') -= A(r, c) A(c', c) / d_c for c < c' <= min(r, w - 1) and L(r, c) = A(r, c) / d_c,
+//       with the diagonal-block rows' column c published in LDS before the step (one barrier per
+//       column; measured per panel of a 167-wide band: 31.6k cycles for the earlier diagonal block
+//       in one wavefront's registers followed by the other rows' substitutions, one barrier
+//       between them);
 //   (3) the trailing b x b window takes A(r, c') -= sum_k L(r, k) d_k L(c', k) for r >= c',
 //       in 16 x 16 tiles of the lower triangle on the matrix cores (v_mfma_f64_16x16x4f64, k = 16
 //       as 4 MFMAs), kNH tiles per wavefront at a time with their old values loaded first.
@@ -113,7 +116,8 @@ __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, c
   const int blk = blockIdx.x;
   if (active && !active[(size_t)blk * astride]) return;
   const int PR = kW + bmax;
-  double* sd = sP + (size_t)PR * kW;
+  double* sd = sP + (size_t)PR * kW;  // kW pivots
+  double* sC = sd + kW;               // 2 x kW: published column entries of the diagonal-block rows
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
   double* L = A.band + A.boff[blk];
@@ -129,48 +133,39 @@ __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, c
       sP[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
     }
     __syncthreads();
-    if (wid == 0) {  // (1) diagonal block
+    {  // (1) the panel, right-looking one column at a time: thread r holds panel row r (r < w + b,
+       // the diagonal block's rows included); before step c the diagonal-block rows' final entries
+       // of column c are published in LDS, so every thread forms L(c', c) = A(c', c) / d_c itself
+       // and a step costs one barrier
+      const int r = tid;
+      const bool rowv = r < w + b && j0 + r < n;
       double a[kW];
 #pragma unroll
-      for (int c = 0; c < kW; ++c) a[c] = (lane < w && c < w && c <= lane && lane - c <= b) ? sP[c * PR + lane] : 0.0;
+      for (int c = 0; c < kW; ++c) a[c] = (rowv && c < w && c <= r && r - c <= b) ? sP[c * PR + r] : 0.0;
+      if (rowv && r < w) sC[r] = a[0];
+      __syncthreads();
 #pragma unroll
       for (int c = 0; c < kW; ++c) {
-        if (c < w) {
-          const double d = lane_value(a[c], c);
-          if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
+        if (c < w) {  // uniform
+          const double* col = sC + (c & 1) * kW;
+          const double d = col[c];
+          if (tid == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
           const double inv = 1.0 / d;
+          if (rowv && r > c) {
+            const double arc = a[c];
 #pragma unroll
-          for (int cc = c + 1; cc < kW; ++cc) {
-            const double v = lane_value(a[c], cc);  // A(cc, c)
-            if (lane >= cc) a[cc] -= a[c] * v * inv;
+            for (int cc = c + 1; cc < kW; ++cc)
+              if (cc < w && cc <= r) a[cc] -= arc * (col[cc] * inv);
+            a[c] = arc * inv;  // L(r, c)
           }
-          if (lane > c) a[c] *= inv;
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < kW; ++c) {
-        if (lane < w && c <= lane && lane - c <= b) sP[c * PR + lane] = a[c];  // L(lane, c), d on the diagonal
-        if (lane == c && c < w) sd[c] = a[c];
-      }
-    }
-    __syncthreads();
-    for (int r = w + tid; r < w + b; r += kBS) {  // (2) rows below the diagonal block
-      if (j0 + r >= n) break;
-      double u[kW];  // u_c = L(r, c) d_c
-#pragma unroll
-      for (int c = 0; c < kW; ++c) {
-        u[c] = 0.0;
-        if (c < w) {
-          double v = r - c <= b ? sP[c * PR + r] : 0.0;
-#pragma unroll
-          for (int k = 0; k < c; ++k)
-            if (c - k <= b) v -= u[k] * sP[k * PR + c];
-          u[c] = v;
+          if (rowv && r > c && r < w && c + 1 < w) sC[((c + 1) & 1) * kW + r] = a[c + 1];
+          __syncthreads();
         }
       }
 #pragma unroll
       for (int c = 0; c < kW; ++c)
-        if (c < w && r - c <= b) sP[c * PR + r] = u[c] / sd[c];
+        if (rowv && c < w && c <= r && r - c <= b) sP[c * PR + r] = a[c];  // L(r, c), d on the diagonal
+      if (rowv && r < w) sd[r] = a[r];
     }
     __syncthreads();
     for (int t = tid; t < w * B1; t += kBS) {
@@ -521,7 +516,7 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
                      astride);
   hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, P.rowptr.ptr,
                      P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
-  const size_t lds = ((size_t)(kW + W->bmax) * kW + kW) * sizeof(double);
+  const size_t lds = ((size_t)(kW + W->bmax) * kW + 3 * kW) * sizeof(double);
   hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,

@@ -1021,6 +1021,25 @@ mpf_status mpf_set_batch(mpf_problem* p, int32_t nmesh, const int32_t* node_off,
   return mpf::batch_set(p, nmesh, node_off, elem_off, mu, Kinv);
 }
 
+mpf_status mpf_batch_time_direct(mpf_problem* p, int32_t reps, double* ms_per_solve) {
+  if (!p || !p->batch || reps < 1 || !ms_per_solve) return MPF_ERR_ARG;
+  if (!p->band || !p->assembled) return MPF_ERR_STATE;
+  hipStream_t s = p->stream;
+  hipEvent_t e0, e1;
+  MPF_HIP(hipEventCreate(&e0));
+  MPF_HIP(hipEventCreate(&e1));
+  MPF_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) MPF_TRY(mpf::band_enqueue_solve(p, nullptr, 0));
+  MPF_HIP(hipEventRecord(e1, s));
+  MPF_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  MPF_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_solve = ms / reps;
+  return MPF_OK;
+}
+
 mpf_status mpf_batch_assembly_choice(const mpf_problem* p, int32_t* mode, int32_t* stage_cap, double* ms_per_assembly) {
   if (!p || !p->batch) return MPF_ERR_ARG;
   if (p->batch->asm_mode < 0) return MPF_ERR_STATE;
