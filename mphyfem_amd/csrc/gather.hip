@@ -472,6 +472,30 @@ __global__ void k_group_chunks(int64_t ngroups, int32_t CW, const int64_t* __res
   }
 }
 
+// Build step 2d (one wavefront per laid-out wavefront): the order in which the lanes compute the
+// wavefront's items.  Lane L computes the item at run-order position fperm[64 w + L]; positions are
+// ranked by (element, local index of the row node, local index of the column node), padding last.
+// The run sums stay in run order; only which lane forms which block changes.  A vector load's
+// address processing takes one pass per distinct 128-byte line among each four consecutive lanes
+// (measured on MI355X: 16-byte loads take 64 cycles per wavefront when the four lanes of every quad
+// hit four lines, 16 when each quad stays in one line); in run order the lanes of a quad read four
+// different elements' records, in this order mostly the same element's.
+__global__ void __launch_bounds__(64) k_wave_perm(int64_t nwaves, const int2* __restrict__ fitem,
+                                                  uint8_t* __restrict__ fperm) {
+  const int64_t w = blockIdx.x;
+  if (w >= nwaves) return;
+  const int lane = threadIdx.x;
+  const int2 it = fitem[w * 64 + lane];
+  const long long key = it.x >= 0 ? ((long long)(it.x >> 4) << 8) | ((it.x & 15) << 4) | (it.y & 15)
+                                   : 0x7fffffffffffffffLL;
+  int rank = 0;
+  for (int j = 0; j < 64; ++j) {
+    const long long kj = __shfl(key, j, 64);
+    rank += (kj < key || (kj == key && j < lane)) ? 1 : 0;
+  }
+  fperm[w * 64 + rank] = (uint8_t)lane;
+}
+
 // Build step 3b (one wavefront per node n): for each column block (n, j) with coupled node m, the
 // entry of block (m, n) of m's rows, which receives its transpose when n owns the pair, is copied
 // next to (n, j)'s own entry, so that a run's head reads both with one load; {-1, 0, ..} when the
@@ -678,6 +702,7 @@ struct FlatArgs {
   int64_t nwaves;
   const int2* fitem;        // {e * 16 + a, (j << 10) | (s << 4) | b}, {-1, -1} in padding slots
   const int32_t* fnode;     // node whose rows each laid-out item belongs to (-1 in padding slots)
+  const uint8_t* fperm;     // per wavefront, lane L computes the item at run-order position fperm[L]
   const int32_t* nbptr;
   const int32_t* btab;      // per column block: its CSR location, then its mirror's (2 BT ints)
   const int32_t* nda;
@@ -698,6 +723,16 @@ struct FlatArgs {
   const uint8_t* fslot;
 };
 
+// Three consecutive CSR values x0 x1 x2 at row[0..2] (the three free dofs of one column node) as one
+// 16-byte store on the 16-byte-aligned pair and one 8-byte store: two memory requests instead of
+// three.  row is 8-byte aligned (an offset into the value array).
+__device__ __forceinline__ void store3(double* row, double x0, double x1, double x2) {
+  const bool odd = (reinterpret_cast<uintptr_t>(row) & 8) != 0;
+  double2* pair = reinterpret_cast<double2*>(row + (odd ? 1 : 0));
+  *pair = odd ? make_double2(x1, x2) : make_double2(x0, x1);
+  row[odd ? 0 : 2] = odd ? x0 : x2;
+}
+
 // Writes a summed column block: K_nm (acc) into node n's rows at block blk and, through the mirror
 // table, its transpose into m's rows.  For the mixed element, acc[9..11] = Kup(n, m) (n's u rows,
 // m's pressure column), acc[12..14] = Kpu(n, m), acc[15] = Kpp(n, m); m's rows receive
@@ -714,16 +749,24 @@ __device__ __forceinline__ void write_block(const FlatArgs& F, const int* bt, co
     for (int i = 0; i < 3; ++i) {
       if (!((mn >> i) & 1)) continue;
       double* row = V + bt[0] + rank3(mn, i) * La;
+      if (mb == 7) {
+        store3(row, acc[3 * i], acc[3 * i + 1], acc[3 * i + 2]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if ((mb >> k) & 1) row[rank3(mb, k)] = acc[3 * i + k];
+        for (int k = 0; k < 3; ++k)
+          if ((mb >> k) & 1) row[rank3(mb, k)] = acc[3 * i + k];
+      }
       if (MIXED && ((masks >> 6) & 1)) row[bt[2] >> 16] = acc[9 + i];
     }
     if (MIXED && ((masks >> 7) & 1)) {
       double* row = V + bt[1];
+      if (mb == 7) {
+        store3(row, acc[12], acc[13], acc[14]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
+        for (int k = 0; k < 3; ++k)
+          if ((mb >> k) & 1) row[rank3(mb, k)] = acc[12 + k];
+      }
       if ((masks >> 6) & 1) row[bt[2] >> 16] = acc[15];
     }
   }
@@ -735,16 +778,24 @@ __device__ __forceinline__ void write_block(const FlatArgs& F, const int* bt, co
   for (int k = 0; k < 3; ++k) {
     if (!((mm >> k) & 1)) continue;
     double* row = V + mt[0] + rank3(mm, k) * La;
+    if (mnn == 7) {
+      store3(row, acc[k], acc[3 + k], acc[6 + k]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[3 * i + k];
+      for (int i = 0; i < 3; ++i)
+        if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[3 * i + k];
+    }
     if (MIXED && ((masks >> 6) & 1)) row[mt[2] >> 16] = acc[12 + k];
   }
   if (MIXED && ((masks >> 7) & 1)) {
     double* row = V + mt[1];
+    if (mnn == 7) {
+      store3(row, acc[9], acc[10], acc[11]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[9 + i];
+      for (int i = 0; i < 3; ++i)
+        if ((mnn >> i) & 1) row[rank3(mnn, i)] = acc[9 + i];
+    }
     if ((masks >> 6) & 1) row[mt[2] >> 16] = acc[15];
   }
 }
@@ -808,40 +859,61 @@ __device__ __forceinline__ void run_heads(const FlatArgs& F, int n, int lane, in
 }
 
 // Matrix kernel, one wavefront per 64 laid-out items, every item reading its element's staging record
-// from global memory.
+// from global memory.  Lane L forms the block of the item at run-order position fperm[L] (items of
+// one element on consecutive lanes) and stores it at that position in LDS.  The runs are then summed
+// and written by lanes 0 .. nruns - 1, lane r taking the r-th run: the stores of a wavefront come
+// from consecutive lanes rather than from run heads spread over the wavefront (a store's address
+// processing, like a load's, takes one pass per distinct line among each four consecutive active
+// lanes).  The sums are those of the run heads, in run order, so the values are the same bits.
 template <int NPE, int NGP, bool MIXED>
-__global__ void __launch_bounds__(64) k_flat(FlatArgs F) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_flat(FlatArgs F) {
   using St = Stage<NPE, NGP, MIXED>;
   constexpr bool P1 = St::P1;        // pressure rows (Tet10/P1)
   constexpr int NV = P1 ? 16 : 9;    // uu 9 | up 3 | pu 3 | pp 1
   constexpr int NR = P1 ? 4 : 3;     // right-hand-side rows of a node (self items)
   constexpr int NT = NV + NR;
+  constexpr int BN = P1 ? 8 : 4;     // ints of a column-block table entry
   __shared__ int32_t sCol[64];
+  __shared__ int32_t sStart[64];
   __shared__ double sPart[64 * NT];
   const int lane = threadIdx.x;
   const int64_t w = xcd_chunk(blockIdx.x, F.nwaves);
   if (w >= F.nwaves) return;
-  const int2 it = F.fitem[w * 64 + lane];
+  const int2 it = F.fitem[w * 64 + lane];  // run-order position lane
+  const int pc = F.fperm ? (int)F.fperm[w * 64 + lane] : lane;
+  const int2 ic = make_int2(__shfl(it.x, pc, 64), __shfl(it.y, pc, 64));  // the item this lane forms
   const bool live = it.x >= 0;
-  const int blk = live ? (it.y >> 4) : 0;
-  const int e = it.x >> 4, a = it.x & 15, b = it.y & 15;
-  const bool self = live && a == b;  // the element's own item of n carries its residual rows
+  const int blk = live ? (it.y >> 4) : -1;
+  const int pblk = __shfl_up(blk, 1, 64);
+  const bool head = live && (lane == 0 || pblk != blk);  // first position of its run
+  const bool self = head && (it.x & 15) == (it.y & 15);
+  const bool clive = ic.x >= 0;
+  const int e = ic.x >> 4, a = ic.x & 15, b = ic.y & 15;
+  const bool cself = clive && a == b;  // the element's own item of n carries its residual rows
   double v[NT];
 #pragma unroll
   for (int q = 0; q < NT; ++q) v[q] = 0.0;
-  const double* rec = F.GS + (size_t)(live ? e : 0) * St::TS;
-  if (live) item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
-  if (self) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
-  // the run head's table entry and node, loaded before the barrier (registers are the limit
-  // during the block computation)
-  BlockEntry<P1> bt;
-  bt.load(F.btab, blk);
+  const double* rec = F.GS + (size_t)(clive ? e : 0) * St::TS;
+  if (clive) item_block<NPE, NGP, MIXED, NV>(rec, a, b, v);
+  if (cself) item_residual<NPE, NGP, MIXED>(rec, a, v + NV);
+  // each run's table entry and node, loaded by its head before the barrier (registers are the
+  // limit during the block computation)
+  BlockEntry<P1> bt{};
+  if (head) bt.load(F.btab, blk);
   const int n = self ? F.fnode[w * 64 + lane] : 0;
-  sCol[lane] = live ? blk : -1;
+  sCol[lane] = blk;
 #pragma unroll
-  for (int q = 0; q < NT; ++q) sPart[lane * NT + q] = v[q];
+  for (int q = 0; q < NT; ++q) sPart[pc * NT + q] = v[q];
+  const unsigned long long hm = __ballot(head);
+  if (head) sStart[__popcll(hm & ((1ull << lane) - 1))] = lane;
   __syncthreads();
-  if (live) run_heads<P1, NV, NT>(F, n, lane, blk, v, sCol, sPart, bt);
+  const int nruns = __popcll(hm);
+  const int st = lane < nruns ? sStart[lane] : 0;
+  BlockEntry<P1> hb;
+#pragma unroll
+  for (int k = 0; k < BN; ++k) hb.v[k] = __shfl(bt.v[k], st, 64);
+  const int hn = __shfl(n, st, 64);
+  if (lane < nruns) run_heads<P1, NV, NT>(F, hn, st, sCol[st], sPart + st * NT, sCol, sPart, hb);
 }
 
 // Chunked matrix kernel: one workgroup per chunk (the wavefronts of one layout group of 8 nodes in
@@ -1164,6 +1236,7 @@ static mpf_status gather_unsupported(mpf_problem* p, const char* why) {
   P.g_nelA.release();
   P.g_fitem.release();
   P.g_fnode.release();
+  P.g_fperm.release();
   P.g_cwave.release();
   P.g_cnw.release();
   P.g_btab.release();
@@ -1327,6 +1400,11 @@ mpf_status ensure_gather_tables(mpf_problem* p) {
                        P.nbptr.ptr, keys.ptr, cnt.ptr, P.g_nelA.ptr, goff.ptr, P.g_fitem.ptr, P.g_fnode.ptr);
   MPF_HIP(hipGetLastError());
   P.g_nwaves = total / 64;
+  MPF_TRY(P.g_fperm.alloc(total > 0 ? (size_t)total : 64));
+  if (P.g_nwaves > 0)
+    hipLaunchKernelGGL(k_wave_perm, dim3((unsigned)P.g_nwaves), dim3(64), 0, s, P.g_nwaves, P.g_fitem.ptr,
+                       P.g_fperm.ptr);
+  MPF_HIP(hipGetLastError());
 
   // chunk tables of the chunked matrix kernel (Hex8): each group's wavefronts in chunks of <= CW
   if (p->etype == MPF_HEX8 && P.g_nwaves > 0) {
@@ -1411,6 +1489,7 @@ mpf_status launch_gather_assembly(mpf_problem* p, const ElemArgs& A, const doubl
   F.nwaves = P.g_nwaves;
   F.fitem = P.g_fitem.ptr;
   F.fnode = P.g_fnode.ptr;
+  F.fperm = P.g_fperm.ptr;
   F.nbptr = P.nbptr.ptr;
   F.btab = P.g_btab.ptr;
   F.nda = P.nda.ptr;

@@ -145,6 +145,7 @@ struct PatternBufs {
   DevBuf<int32_t> g_nelA;     // elements around each node (ascending), with the node's local index
   DevBuf<int2> g_fitem;       // laid-out (element, node) items of the owned blocks, 64 per wavefront
   DevBuf<int32_t> g_fnode;    // node whose rows each laid-out item belongs to
+  DevBuf<uint8_t> g_fperm;    // per wavefront: the run-order position of the item each lane computes
   DevBuf<int32_t> g_btab;     // CSR location of every column block (n, j) and of its mirror (m, n)
   DevBuf<int32_t> g_near;     // node has a prescribed dof in an element around it
   DevBuf<int32_t> g_nearlist; // the flagged nodes
