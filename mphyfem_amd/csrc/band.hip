@@ -8,9 +8,10 @@
 // without pivoting (K is the symmetric tangent of the consistent mode; a zero or non-finite pivot is
 // reported) and solved, one workgroup per block and kernel:
 //
-//   k_band_factor: right-looking in panels of kW columns -- the kW x kW diagonal block in one
-//   wavefront's registers, the panel's other rows one thread each, the rank-kW update of the b x b
-//   trailing triangle on the matrix cores (FP64 MFMA);
+//   k_band_factor: right-looking in panels of kW columns -- the panel factored column by column
+//   (bands up to 176 wide: by one wavefront in registers, three rows per lane; wider: one thread
+//   per panel row and one barrier per column), the rank-kW update of the b x b trailing triangle
+//   on the matrix cores (FP64 MFMA);
 //   k_band_solve: forward and backward substitution in blocks of kW rows, the unit triangle in one
 //   wavefront's registers, the coupling to the next b rows by the whole workgroup.
 //
@@ -28,7 +29,9 @@ namespace mpf {
 namespace {
 
 constexpr int kW = 16;      // panel width (columns per panel, rows per solve block)
-constexpr int kBS = 1024;   // threads per workgroup: 16 wavefronts, one per row of a backward block
+constexpr int kBS = 1024;   // threads per workgroup of the solve: 16 wavefronts, one per row of a backward block
+constexpr int kFBS = 768;   // threads per workgroup of the factorisation: 12 wavefronts, so that one
+                            // wavefront can hold three panel rows per lane in registers (170 VGPRs)
 #ifndef MPF_BAND_TILES
 #define MPF_BAND_TILES 2
 #endif
@@ -100,17 +103,18 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // PR = kW + bmax, then kW pivots.
 //
 // Panel step at column j0 (w = min(kW, n - j0) columns, panel-relative rows 0 .. w - 1 + b):
-//   (1) the panel is factored right-looking, one column at a time, one thread per panel row (the
-//       diagonal block's rows included): at column c every row r > c takes
-//       A(r, c') -= A(r, c) A(c', c) / d_c for c < c' <= min(r, w - 1) and L(r, c) = A(r, c) / d_c,
-//       with the diagonal-block rows' column c published in LDS before the step (one barrier per
-//       column; measured per panel of a 167-wide band: 31.6k cycles for the earlier diagonal block
-//       in one wavefront's registers followed by the other rows' substitutions, one barrier
-//       between them);
+//   (1) the panel is factored right-looking, one column at a time: at column c every row r > c
+//       takes A(r, c') -= A(r, c) A(c', c) / d_c for c < c' <= min(r, w - 1) and
+//       L(r, c) = A(r, c) / d_c.  Up to 192 panel rows (w + b <= 192) wavefront 0 holds three rows
+//       per lane and reads the diagonal-block rows' column c across lanes, with no barrier; wider
+//       panels give each row a thread and publish that column in LDS before each step (one
+//       barrier per column).  Measured on the longest of 200 cantilevers (12 960 dofs, b = 167,
+//       810 panels): 17.3 ms in all, of which 7.3 ms in (1), 7.3 ms in (3) and 2.7 ms loading and
+//       storing panels (timing-only builds that skip (1), (3) or both);
 //   (3) the trailing b x b window takes A(r, c') -= sum_k L(r, k) d_k L(c', k) for r >= c',
 //       in 16 x 16 tiles of the lower triangle on the matrix cores (v_mfma_f64_16x16x4f64, k = 16
 //       as 4 MFMAs), kNH tiles per wavefront at a time with their old values loaded first.
-__global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
+__global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
                                                      int32_t astride, int32_t* __restrict__ factored) {
   extern __shared__ double sP[];
   const int blk = blockIdx.x;
@@ -124,16 +128,62 @@ __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, c
   const int B1 = b + 1;
   const int T = (b + 15) >> 4;  // trailing tiles per side
   const int NT = T * (T + 1) / 2;
-  constexpr int NW = kBS / 64;
+  constexpr int NW = kFBS / 64;
 
   for (int j0 = 0; j0 < n; j0 += kW) {
     const int w = min(kW, n - j0);
-    for (int t = tid; t < w * B1; t += kBS) {  // panel (r, c), r - c <= b
+    for (int t = tid; t < w * B1; t += kFBS) {  // panel (r, c), r - c <= b
       const int c = t / B1, k = t - c * B1, r = c + k;
       sP[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
     }
     __syncthreads();
-    {  // (1) the panel, right-looking one column at a time: thread r holds panel row r (r < w + b,
+    if (w + b <= 3 * 64) {
+      // (1) narrow bands: wavefront 0 factors the whole panel in registers, lane l holding panel
+      // rows l, l + 64, l + 128, right-looking one column at a time with the pivot and the
+      // diagonal-block rows' entries of column c read across lanes: no barrier inside the panel.
+      // Same operations in the same order as the wide-band path below, so the same bits.
+      if (wid == 0) {
+        double a[3][kW];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int r = lane + 64 * s;
+          const bool rowv = r < w + b && j0 + r < n;
+#pragma unroll
+          for (int c = 0; c < kW; ++c) a[s][c] = (rowv && c < w && c <= r && r - c <= b) ? sP[c * PR + r] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < kW; ++c) {
+          if (c < w) {  // uniform
+            const double d = lane_value(a[0][c], c);
+            if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
+            const double inv = 1.0 / d;
+#pragma unroll
+            for (int cc = c + 1; cc < kW; ++cc) {
+              if (cc < w) {
+                const double m = lane_value(a[0][c], cc) * inv;  // L(cc, c)
+#pragma unroll
+                for (int s = 0; s < 3; ++s)
+                  if (lane + 64 * s >= cc) a[s][cc] -= a[s][c] * m;
+              }
+            }
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+              if (lane + 64 * s > c) a[s][c] *= inv;  // L(r, c)
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int r = lane + 64 * s;
+          const bool rowv = r < w + b && j0 + r < n;
+#pragma unroll
+          for (int c = 0; c < kW; ++c)
+            if (rowv && c < w && c <= r && r - c <= b) sP[c * PR + r] = a[s][c];  // L(r, c), d on the diagonal
+        }
+#pragma unroll
+        for (int c = 0; c < kW; ++c)
+          if (lane == c && c < w) sd[c] = a[0][c];
+      }
+    } else {  // (1) the panel, right-looking one column at a time: thread r holds panel row r (r < w + b,
        // the diagonal block's rows included); before step c the diagonal-block rows' final entries
        // of column c are published in LDS, so every thread forms L(c', c) = A(c', c) / d_c itself
        // and a step costs one barrier
@@ -168,7 +218,7 @@ __global__ void __launch_bounds__(kBS) k_band_factor(BandArgs A, int32_t bmax, c
       if (rowv && r < w) sd[r] = a[r];
     }
     __syncthreads();
-    for (int t = tid; t < w * B1; t += kBS) {
+    for (int t = tid; t < w * B1; t += kFBS) {
       const int c = t / B1, k = t - c * B1, r = c + k;
       if (j0 + r < n) L[(size_t)(j0 + c) * B1 + k] = sP[c * PR + r];
     }
@@ -483,9 +533,10 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   W.nband = boff[nblk];
   W.bw_h = bw;
   W.bstart_h = bstart;
-  const int lds = (int)(((size_t)(kW + W.bmax) * kW + kW) * sizeof(double));
+  const int lds = (int)(((size_t)(kW + W.bmax) * kW + 3 * kW) * sizeof(double));
   if (lds > 64 * 1024) return MPF_ERR_ARG;  // half-bandwidth beyond the panel's LDS (about 490 dofs)
   if (W.bmax >= kBS) return MPF_ERR_ARG;     // the solve gives each coupling row one thread
+  static_assert(kW + 493 <= kFBS, "the wide-band panel path gives each panel row one thread");
   MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
   MPF_TRY(W.iperm.upload(iperm.data(), iperm.size(), s));
   MPF_TRY(W.pblk.upload(pblk.data(), pblk.size(), s));
@@ -517,7 +568,7 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, P.rowptr.ptr,
                      P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
   const size_t lds = ((size_t)(kW + W->bmax) * kW + 3 * kW) * sizeof(double);
-  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
+  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kFBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
                      W->y.ptr, active, astride, p->du.ptr);
