@@ -98,170 +98,245 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+// Panel of w columns from band column j0 into P (column-major, row stride PR: panel row r = band
+// row j0 + r, rows past the block's end as zeros), by threads t0, t0 + ts, ...
+__device__ __forceinline__ void panel_load(double* P, const double* L, int j0, int w, int n, int B1, int PR, int t0,
+                                           int ts) {
+  for (int t = t0; t < w * B1; t += ts) {  // panel (r, c), r - c <= b
+    const int c = t / B1, k = t - c * B1, r = c + k;
+    P[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void panel_store(const double* P, double* L, int j0, int w, int n, int B1, int PR, int t0,
+                                            int ts) {
+  for (int t = t0; t < w * B1; t += ts) {
+    const int c = t / B1, k = t - c * B1, r = c + k;
+    if (j0 + r < n) L[(size_t)(j0 + c) * B1 + k] = P[c * PR + r];
+  }
+}
+
+// Panel factorisation by one wavefront (w + b <= 192 panel rows): lane l holds panel rows l, l + 64,
+// l + 128 in registers; right-looking one column at a time, the pivot and the diagonal-block rows'
+// entries of column c read across lanes, no barrier.  At column c every row r > c takes
+// A(r, c') -= A(r, c) L(c', c) for c < c' <= min(r, w - 1), L(c', c) = A(c', c) / d_c, and
+// L(r, c) = A(r, c) / d_c; P receives L below the diagonal and d on it, sd the pivots.
+__device__ __forceinline__ void panel_factor_wave(double* P, double* sd, int j0, int w, int b, int n, int PR, int lane,
+                                                  int32_t* bad) {
+  double a[3][kW];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int r = lane + 64 * s;
+    const bool rowv = r < w + b && j0 + r < n;
+#pragma unroll
+    for (int c = 0; c < kW; ++c) a[s][c] = (rowv && c < w && c <= r && r - c <= b) ? P[c * PR + r] : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < kW; ++c) {
+    if (c < w) {  // uniform
+      const double d = lane_value(a[0][c], c);
+      if (lane == 0 && (d == 0.0 || !isfinite(d))) *bad = 1;
+      const double inv = 1.0 / d;
+#pragma unroll
+      for (int cc = c + 1; cc < kW; ++cc) {
+        if (cc < w) {
+          const double m = lane_value(a[0][c], cc) * inv;  // L(cc, c)
+#pragma unroll
+          for (int s = 0; s < 3; ++s)
+            if (lane + 64 * s >= cc) a[s][cc] -= a[s][c] * m;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+        if (lane + 64 * s > c) a[s][c] *= inv;  // L(r, c)
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int r = lane + 64 * s;
+    const bool rowv = r < w + b && j0 + r < n;
+#pragma unroll
+    for (int c = 0; c < kW; ++c)
+      if (rowv && c < w && c <= r && r - c <= b) P[c * PR + r] = a[s][c];  // L(r, c), d on the diagonal
+  }
+#pragma unroll
+  for (int c = 0; c < kW; ++c)
+    if (lane == c && c < w) sd[c] = a[0][c];
+}
+
+// Product tile (R0 .., C0 ..) of the rank-w update of factored panel P (pivots sd) by one wavefront
+// on the matrix cores: lane l ends with sum_k L(R0 + 4 v + l / 16, k) d_k L(C0 + l % 16, k) in acc[v]
+// (v_mfma_f64_16x16x4f64, k = 16 as 4 steps).
+__device__ __forceinline__ double4_t update_tile(const double* P, const double* sd, int w, int b, int PR, int R0,
+                                                 int C0, int lane) {
+  const int i16 = lane & 15, k4 = lane >> 4;
+  double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < kW / 4; ++s) {
+    const int k = 4 * s + k4;
+    const int ra = R0 + i16, cb = C0 + i16;
+    const double av = (k < w && ra < w + b && ra - k <= b) ? P[k * PR + ra] : 0.0;
+    const double bv = (k < w && cb < w + b && cb - k <= b) ? P[k * PR + cb] * sd[k] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Trailing update of panel (j0, w) in the band: tiles (ti, tj) of the lower triangle of the window
+// of panel-relative rows and columns w .. w + b - 1, tj >= tj0, taken u0, u0 + us, ... in row-major
+// order by one wavefront, NH tiles at a time with their old values loaded first.
+template <int NH>
+__device__ __forceinline__ void trailing_band(const double* P, const double* sd, double* L, int j0, int w, int b,
+                                              int n, int B1, int PR, int T, int tj0, int u0, int us, int lane) {
+  const int Tq = T - tj0, NTq = Tq * (Tq + 1) / 2;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  for (int q0 = u0; q0 < NTq; q0 += NH * us) {
+    int R0[NH], C0[NH];
+    double old[NH][4];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const int q = q0 + h * us;
+      int t = 0;
+      while ((t + 1) * (t + 2) / 2 <= q) ++t;
+      R0[h] = w + 16 * (t + tj0);
+      C0[h] = w + 16 * (tj0 + q - t * (t + 1) / 2);
+      const int c = C0[h] + i16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = R0[h] + 4 * v + k4;
+        old[h][v] = (q < NTq && r >= c && r < w + b && j0 + r < n) ? L[(size_t)(j0 + c) * B1 + (r - c)] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      if (q0 + h * us >= NTq) break;
+      const double4_t acc = update_tile(P, sd, w, b, PR, R0[h], C0[h], lane);
+      const int c = C0[h] + i16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = R0[h] + 4 * v + k4;
+        if (r >= c && r < w + b && j0 + r < n) L[(size_t)(j0 + c) * B1 + (r - c)] = old[h][v] - acc[v];
+      }
+    }
+  }
+}
+
 // Factor one block per workgroup (blocks whose active[blk * astride] is 0 are skipped; active may
-// be null).  Dynamic LDS: the panel, (kW + bmax) x kW doubles, column-major with row stride
-// PR = kW + bmax, then kW pivots.
+// be null).  Dynamic LDS: bands up to 176 wide, two panel buffers of (kW + bmax) x kW doubles
+// (column-major, row stride PR = kW + bmax) each followed by its kW pivots; wider bands, one panel
+// buffer, its pivots and 2 kW doubles of published column entries.
 //
 // Panel step at column j0 (w = min(kW, n - j0) columns, panel-relative rows 0 .. w - 1 + b):
-//   (1) the panel is factored right-looking, one column at a time: at column c every row r > c
-//       takes A(r, c') -= A(r, c) A(c', c) / d_c for c < c' <= min(r, w - 1) and
-//       L(r, c) = A(r, c) / d_c.  Up to 192 panel rows (w + b <= 192) wavefront 0 holds three rows
-//       per lane and reads the diagonal-block rows' column c across lanes, with no barrier; wider
-//       panels give each row a thread and publish that column in LDS before each step (one
-//       barrier per column).  Measured on the longest of 200 cantilevers (12 960 dofs, b = 167,
-//       810 panels): 17.3 ms in all, of which 7.3 ms in (1), 7.3 ms in (3) and 2.7 ms loading and
-//       storing panels (timing-only builds that skip (1), (3) or both);
-//   (3) the trailing b x b window takes A(r, c') -= sum_k L(r, k) d_k L(c', k) for r >= c',
-//       in 16 x 16 tiles of the lower triangle on the matrix cores (v_mfma_f64_16x16x4f64, k = 16
-//       as 4 MFMAs), kNH tiles per wavefront at a time with their old values loaded first.
+//   (1) the panel is factored right-looking, one column at a time (panel_factor_wave); wider
+//       panels give each row a thread and publish the diagonal-block rows' column c in LDS before
+//       each step (one barrier per column);
+//   (2) the trailing b x b window takes A(r, c') -= sum_k L(r, k) d_k L(c', k) for r >= c', in
+//       16 x 16 tiles of the lower triangle on the matrix cores.
+// Bands up to 176 wide look one panel ahead: the next panel is loaded into the other buffer and its
+// columns (the window's first tile column) take (2) there; then wavefront 0 factors the next panel
+// while the other wavefronts store this panel and apply (2) to the rest of the window in the band.
+// Every stored value goes through the same operations in the same order either way.  Measured on
+// the longest of 200 cantilevers (12 960 dofs, b = 167, 810 panels) before the look-ahead: 17.3 ms,
+// of which 7.3 ms in (1), 7.3 ms in (2) and 2.7 ms loading and storing panels (timing-only builds
+// that skip (1), (2) or both).
 __global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
                                                      int32_t astride, int32_t* __restrict__ factored) {
   extern __shared__ double sP[];
   const int blk = blockIdx.x;
   if (active && !active[(size_t)blk * astride]) return;
   const int PR = kW + bmax;
-  double* sd = sP + (size_t)PR * kW;  // kW pivots
-  double* sC = sd + kW;               // 2 x kW: published column entries of the diagonal-block rows
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
   double* L = A.band + A.boff[blk];
   const int B1 = b + 1;
   const int T = (b + 15) >> 4;  // trailing tiles per side
-  const int NT = T * (T + 1) / 2;
   constexpr int NW = kFBS / 64;
 
-  for (int j0 = 0; j0 < n; j0 += kW) {
-    const int w = min(kW, n - j0);
-    for (int t = tid; t < w * B1; t += kFBS) {  // panel (r, c), r - c <= b
-      const int c = t / B1, k = t - c * B1, r = c + k;
-      sP[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
+  if (kW + bmax <= 3 * 64) {  // the same for every block of the launch
+    double* const buf0 = sP;
+    double* const buf1 = sP + (size_t)PR * kW + kW;
+    if (n > 0) {
+      panel_load(buf0, L, 0, min(kW, n), n, B1, PR, tid, kFBS);
+      __syncthreads();
+      if (wid == 0) panel_factor_wave(buf0, buf0 + (size_t)PR * kW, 0, min(kW, n), b, n, PR, lane, A.bad);
+      __syncthreads();
     }
-    __syncthreads();
-    if (w + b <= 3 * 64) {
-      // (1) narrow bands: wavefront 0 factors the whole panel in registers, lane l holding panel
-      // rows l, l + 64, l + 128, right-looking one column at a time with the pivot and the
-      // diagonal-block rows' entries of column c read across lanes: no barrier inside the panel.
-      // Same operations in the same order as the wide-band path below, so the same bits.
-      if (wid == 0) {
-        double a[3][kW];
+    for (int j0 = 0, q = 0; j0 < n; j0 += kW, q ^= 1) {
+      const int w = min(kW, n - j0);
+      double* cur = q ? buf1 : buf0;
+      double* nxt = q ? buf0 : buf1;
+      const double* sdc = cur + (size_t)PR * kW;
+      const int j1 = j0 + kW;
+      const int w1 = j1 < n ? min(kW, n - j1) : 0;
+      if (w1 > 0) {  // uniform
+        panel_load(nxt, L, j1, w1, n, B1, PR, tid, kFBS);
+        __syncthreads();
+        // the window's first tile column is the next panel: its tiles are updated in LDS
+        const int i16 = lane & 15, k4 = lane >> 4;
+        for (int ti = wid; ti < T; ti += NW) {
+          const double4_t acc = update_tile(cur, sdc, w, b, PR, w + 16 * ti, w, lane);
+          const int c = w + i16;
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const int r = lane + 64 * s;
-          const bool rowv = r < w + b && j0 + r < n;
-#pragma unroll
-          for (int c = 0; c < kW; ++c) a[s][c] = (rowv && c < w && c <= r && r - c <= b) ? sP[c * PR + r] : 0.0;
+          for (int v = 0; v < 4; ++v) {
+            const int r = w + 16 * ti + 4 * v + k4;
+            if (r >= c && r < w + b && j0 + r < n) nxt[(c - w) * PR + (r - w)] -= acc[v];
+          }
         }
+        __syncthreads();
+      }
+      if (wid == 0) {
+        if (w1 > 0) panel_factor_wave(nxt, nxt + (size_t)PR * kW, j1, w1, b, n, PR, lane, A.bad);
+      } else {
+        panel_store(cur, L, j0, w, n, B1, PR, tid - 64, kFBS - 64);
+        trailing_band<kNH>(cur, sdc, L, j0, w, b, n, B1, PR, T, 1, wid - 1, NW - 1, lane);
+      }
+      __syncthreads();
+    }
+  } else {
+    double* sd = sP + (size_t)PR * kW;  // kW pivots
+    double* sC = sd + kW;               // 2 x kW: published column entries of the diagonal-block rows
+    for (int j0 = 0; j0 < n; j0 += kW) {
+      const int w = min(kW, n - j0);
+      panel_load(sP, L, j0, w, n, B1, PR, tid, kFBS);
+      __syncthreads();
+      {  // (1) thread r holds panel row r (r < w + b, the diagonal block's rows included); before
+         // step c the diagonal-block rows' final entries of column c are published in LDS, so every
+         // thread forms L(c', c) = A(c', c) / d_c itself and a step costs one barrier
+        const int r = tid;
+        const bool rowv = r < w + b && j0 + r < n;
+        double a[kW];
+#pragma unroll
+        for (int c = 0; c < kW; ++c) a[c] = (rowv && c < w && c <= r && r - c <= b) ? sP[c * PR + r] : 0.0;
+        if (rowv && r < w) sC[r] = a[0];
+        __syncthreads();
 #pragma unroll
         for (int c = 0; c < kW; ++c) {
           if (c < w) {  // uniform
-            const double d = lane_value(a[0][c], c);
-            if (lane == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
+            const double* col = sC + (c & 1) * kW;
+            const double d = col[c];
+            if (tid == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
             const double inv = 1.0 / d;
+            if (rowv && r > c) {
+              const double arc = a[c];
 #pragma unroll
-            for (int cc = c + 1; cc < kW; ++cc) {
-              if (cc < w) {
-                const double m = lane_value(a[0][c], cc) * inv;  // L(cc, c)
-#pragma unroll
-                for (int s = 0; s < 3; ++s)
-                  if (lane + 64 * s >= cc) a[s][cc] -= a[s][c] * m;
-              }
+              for (int cc = c + 1; cc < kW; ++cc)
+                if (cc < w && cc <= r) a[cc] -= arc * (col[cc] * inv);
+              a[c] = arc * inv;  // L(r, c)
             }
-#pragma unroll
-            for (int s = 0; s < 3; ++s)
-              if (lane + 64 * s > c) a[s][c] *= inv;  // L(r, c)
+            if (rowv && r > c && r < w && c + 1 < w) sC[((c + 1) & 1) * kW + r] = a[c + 1];
+            __syncthreads();
           }
-        }
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const int r = lane + 64 * s;
-          const bool rowv = r < w + b && j0 + r < n;
-#pragma unroll
-          for (int c = 0; c < kW; ++c)
-            if (rowv && c < w && c <= r && r - c <= b) sP[c * PR + r] = a[s][c];  // L(r, c), d on the diagonal
         }
 #pragma unroll
         for (int c = 0; c < kW; ++c)
-          if (lane == c && c < w) sd[c] = a[0][c];
+          if (rowv && c < w && c <= r && r - c <= b) sP[c * PR + r] = a[c];  // L(r, c), d on the diagonal
+        if (rowv && r < w) sd[r] = a[r];
       }
-    } else {  // (1) the panel, right-looking one column at a time: thread r holds panel row r (r < w + b,
-       // the diagonal block's rows included); before step c the diagonal-block rows' final entries
-       // of column c are published in LDS, so every thread forms L(c', c) = A(c', c) / d_c itself
-       // and a step costs one barrier
-      const int r = tid;
-      const bool rowv = r < w + b && j0 + r < n;
-      double a[kW];
-#pragma unroll
-      for (int c = 0; c < kW; ++c) a[c] = (rowv && c < w && c <= r && r - c <= b) ? sP[c * PR + r] : 0.0;
-      if (rowv && r < w) sC[r] = a[0];
       __syncthreads();
-#pragma unroll
-      for (int c = 0; c < kW; ++c) {
-        if (c < w) {  // uniform
-          const double* col = sC + (c & 1) * kW;
-          const double d = col[c];
-          if (tid == 0 && (d == 0.0 || !isfinite(d))) *A.bad = 1;
-          const double inv = 1.0 / d;
-          if (rowv && r > c) {
-            const double arc = a[c];
-#pragma unroll
-            for (int cc = c + 1; cc < kW; ++cc)
-              if (cc < w && cc <= r) a[cc] -= arc * (col[cc] * inv);
-            a[c] = arc * inv;  // L(r, c)
-          }
-          if (rowv && r > c && r < w && c + 1 < w) sC[((c + 1) & 1) * kW + r] = a[c + 1];
-          __syncthreads();
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < kW; ++c)
-        if (rowv && c < w && c <= r && r - c <= b) sP[c * PR + r] = a[c];  // L(r, c), d on the diagonal
-      if (rowv && r < w) sd[r] = a[r];
+      panel_store(sP, L, j0, w, n, B1, PR, tid, kFBS);
+      trailing_band<kNH>(sP, sd, L, j0, w, b, n, B1, PR, T, 0, wid, NW, lane);  // (2)
+      __syncthreads();
     }
-    __syncthreads();
-    for (int t = tid; t < w * B1; t += kFBS) {
-      const int c = t / B1, k = t - c * B1, r = c + k;
-      if (j0 + r < n) L[(size_t)(j0 + c) * B1 + k] = sP[c * PR + r];
-    }
-    // (3) trailing update, tiles (ti >= tj) of panel-relative rows / columns w + 16 t ..
-    const int i16 = lane & 15, k4 = lane >> 4;
-    for (int q0 = wid; q0 < NT; q0 += kNH * NW) {
-      int R0[kNH], C0[kNH];
-      double old[kNH][4];
-#pragma unroll
-      for (int h = 0; h < kNH; ++h) {
-        const int q = q0 + h * NW;
-        int ti = 0;
-        while ((ti + 1) * (ti + 2) / 2 <= q) ++ti;
-        R0[h] = w + 16 * ti;
-        C0[h] = w + 16 * (q - ti * (ti + 1) / 2);
-        const int c = C0[h] + i16;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {  // lane holds D(4 v + lane / 16, lane % 16) in acc[v]
-          const int r = R0[h] + 4 * v + k4;
-          old[h][v] = (q < NT && r >= c && r < w + b && j0 + r < n) ? L[(size_t)(j0 + c) * B1 + (r - c)] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < kNH; ++h) {
-        if (q0 + h * NW >= NT) break;
-        double4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int s = 0; s < kW / 4; ++s) {
-          const int k = 4 * s + k4;
-          const int ra = R0[h] + i16, cb = C0[h] + i16;
-          const double av = (k < w && ra < w + b && ra - k <= b) ? sP[k * PR + ra] : 0.0;
-          const double bv = (k < w && cb < w + b && cb - k <= b) ? sP[k * PR + cb] * sd[k] : 0.0;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-        }
-        const int c = C0[h] + i16;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int r = R0[h] + 4 * v + k4;
-          if (r >= c && r < w + b && j0 + r < n) L[(size_t)(j0 + c) * B1 + (r - c)] = old[h][v] - acc[v];
-        }
-      }
-    }
-    __syncthreads();
   }
   if (tid == 0) factored[blk] = 1;  // the band now holds this block's factors
 }
@@ -567,7 +642,8 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
                      astride);
   hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, P.rowptr.ptr,
                      P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->iperm.ptr, W->pblk.ptr, A, active, astride);
-  const size_t lds = ((size_t)(kW + W->bmax) * kW + 3 * kW) * sizeof(double);
+  const size_t lds = (kW + W->bmax <= 3 * 64 ? 2 * ((size_t)(kW + W->bmax) * kW + kW)
+                                             : (size_t)(kW + W->bmax) * kW + 3 * kW) * sizeof(double);
   hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kFBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
   hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
   hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
