@@ -137,18 +137,19 @@ __device__ __forceinline__ void panel_factor_wave(double* P, double* sd, int j0,
       const double d = lane_value(a[0][c], c);
       if (lane == 0 && (d == 0.0 || !isfinite(d))) *bad = 1;
       const double inv = 1.0 / d;
+      // every row takes the update, rows above cc included: the entries that changes (above the
+      // diagonal) are never read or stored, and no test sits in the update
 #pragma unroll
       for (int cc = c + 1; cc < kW; ++cc) {
         if (cc < w) {
           const double m = lane_value(a[0][c], cc) * inv;  // L(cc, c)
 #pragma unroll
-          for (int s = 0; s < 3; ++s)
-            if (lane + 64 * s >= cc) a[s][cc] -= a[s][c] * m;
+          for (int s = 0; s < 3; ++s) a[s][cc] -= a[s][c] * m;
         }
       }
-#pragma unroll
-      for (int s = 0; s < 3; ++s)
-        if (lane + 64 * s > c) a[s][c] *= inv;  // L(r, c)
+      a[0][c] = lane > c ? a[0][c] * inv : a[0][c];  // L(r, c); the pivot row keeps d
+      a[1][c] *= inv;
+      a[2][c] *= inv;
     }
   }
 #pragma unroll
