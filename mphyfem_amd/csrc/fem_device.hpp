@@ -222,7 +222,9 @@ __device__ __forceinline__ int rank3(int mask, int i) { return __popc(mask & ((1
 // COMPACT output (the gather path's Hex8 staging records): the inverse Jacobian Bi (9 doubles,
 // row-major) at G, from which g of every node is formed with grad_jinv, and the six components
 // b00 b11 b22 b01 b02 b12 of the symmetric b at Bv, instead of g and b g of every node at G + a * 6.
-template <int NPE, int NGP, bool MIXED, bool COMPACT = false>
+// GPAD output (the gather path's Tet10 staging records): g of every node at G + a * 4 (g0 g1 g2 0)
+// and the six components of b at Bv.
+template <int NPE, int NGP, bool MIXED, bool COMPACT = false, bool GPAD = false>
 __device__ __forceinline__ void element_point(const double (*X)[3], const double (*U)[3], const double (*UP)[3],
                                               const double* Pn, const double (*d)[NPE], int gp, double wgp,
                                               double mu, double K, int Utype, bool consistent, bool writer,
@@ -297,7 +299,10 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
     const double da[3] = {d[0][a], d[1][a], d[2][a]};
     grad_jinv(&Bi[0][0], da, ga);
     const double g0 = ga[0], g1 = ga[1], g2 = ga[2];
-    if (writer && !COMPACT) {
+    if (writer && GPAD) {
+      double* Ga = G + a * 4;
+      Ga[0] = g0; Ga[1] = g1; Ga[2] = g2; Ga[3] = 0.0;
+    } else if (writer && !COMPACT) {
       double* Ga = G + a * 6;
       Ga[0] = g0; Ga[1] = g1; Ga[2] = g2;
       Ga[3] = P.b[0][0] * g0 + P.b[0][1] * g1 + P.b[0][2] * g2;
@@ -317,7 +322,7 @@ __device__ __forceinline__ void element_point(const double (*X)[3], const double
   if (writer) {
     S[0] = P.s1; S[1] = P.s2; S[2] = P.s3; S[3] = P.s4; S[4] = P.s5;
     if (MIXED) { S[4] = dv; S[5] = dv0; S[6] = thetahat; S[7] = J - Jhat - thetahat * pres; }
-    if (COMPACT) {
+    if (COMPACT || GPAD) {
       Bv[0] = P.b[0][0]; Bv[1] = P.b[1][1]; Bv[2] = P.b[2][2];
       Bv[3] = P.b[0][1]; Bv[4] = P.b[0][2]; Bv[5] = P.b[1][2];
     }

@@ -36,10 +36,10 @@ constexpr int kMaxAdj = 64;   // elements around one node
 constexpr int kMaxNB = 128;   // nodes coupled to one node (column blocks of its rows)
 
 // Staging record layout (doubles) per element:
-//   Tet10, Tet10/P1:  G [NGP][NPE][6]  g_a (3), (b g)_a (3)
-//   Hex8:      J [NGP][9]       the inverse Jacobian (g_a = grad_jinv(J, dN_a/dxi));
-//              B [NGP][6]       b00 b11 b22 b01 b02 b12   (g and b g formed on use: 184 doubles per
-//                               element instead of 448)
+//   Tet10, Tet10/P1:  G [NGP][NPE][4]  g_a (3) and a zero (two 16-byte groups per node)
+//   Hex8:      J [NGP][9]       the inverse Jacobian (g_a = grad_jinv(J, dN_a/dxi))
+//   B  [NGP][6]        b00 b11 b22 b01 b02 b12 (b g formed on use; Tet10/P1 records 248 doubles
+//                      instead of 304 with b g stored, Hex8 184 instead of 448)
 //   S  [NGP][SS]       displacement elements: s1..s5 (SS = 5); mixed elements: s1..s4, dv, dv0,
 //                      thetahat, resip (SS = 8)
 //   W  [26]            Hex8 Q1/P0 only: Kup [NPE][3], Kpp, 0 (the static condensation's coupling)
@@ -51,12 +51,12 @@ template <int NPE, int NGP, bool MIXED>
 struct Stage {
   static constexpr bool P0 = MIXED && NPE == 8;
   static constexpr bool P1 = MIXED && NPE != 8;
-  static constexpr bool COMPACT = NPE == 8;
-  static constexpr int GW = COMPACT ? 3 : 6;      // doubles per node and Gauss point of g data
+  static constexpr bool COMPACT = NPE == 8;      // g formed from the inverse Jacobian
+  static constexpr int GW = COMPACT ? 3 : 4;      // doubles per node and Gauss point of g (LDS copies, Tet10 records)
   static constexpr int SS = MIXED ? 8 : 5;
   static constexpr int G0 = 0;
   static constexpr int B0 = COMPACT ? NGP * 9 : NGP * NPE * GW;
-  static constexpr int S0 = B0 + (COMPACT ? NGP * 6 : 0);
+  static constexpr int S0 = B0 + NGP * 6;
   static constexpr int W0 = S0 + NGP * SS;
   static constexpr int WN = P0 ? 26 : 0;
   static constexpr int R0 = W0 + WN;
@@ -93,7 +93,7 @@ __device__ __forceinline__ void record_g(const double* rec, int gp, int c, doubl
   } else {
     const double* p = rec + St::G0 + (gp * NPE + c) * St::GW;
     g[0] = p[0]; g[1] = p[1]; g[2] = p[2];
-    bg[0] = p[3]; bg[1] = p[4]; bg[2] = p[5];
+    bsym_times(rec + St::B0 + gp * 6, g, bg);
   }
 }
 
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
   bool negJ = false;
   double mu, K;
   elem_material(A, writer ? e : base, mu, K);
-  element_point<NPE, NGP, MIXED, St::COMPACT>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp],
+  element_point<NPE, NGP, MIXED, St::COMPACT, !St::COMPACT>(sX[k], sU[k], MIXED ? sUP[k] : nullptr, MIXED ? sP[k] : nullptr, sdN[gp],
                                               gp, wgp, mu, K, A.Utype, St::P0 ? false : A.consistent != 0, writer,
                                               rec + St::G0 + gp * (St::COMPACT ? 9 : NPE * St::GW),
                                               rec + St::S0 + gp * St::SS, rec + St::R0, negJ, rec + St::B0 + gp * 6);
@@ -631,23 +631,23 @@ __device__ __forceinline__ void item_gps(const double* G, const double* Bv, cons
       hex8_dN_node(gp0 + gp, b, db);
       grad_jinv(Gp, da, ga);
       grad_jinv(Gp, db, gb);
-    }
-    if (St::COMPACT) {
-      if (!FROM_J)
+    } else if (St::COMPACT) {  // g formed in LDS, 3 doubles per node
 #pragma unroll
-        for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
+      for (int i = 0; i < 3; ++i) { ga[i] = Gp[a * 3 + i]; gb[i] = Gp[b * 3 + i]; }
+    } else {  // Tet10 records: g padded to 4 doubles, two 16-byte loads per node
+      const double2* ga2 = reinterpret_cast<const double2*>(Gp + a * 4);
+      const double2* gb2 = reinterpret_cast<const double2*>(Gp + b * 4);
+      const double2 p0 = ga2[0], p1 = ga2[1];
+      const double2 q0 = gb2[0], q1 = gb2[1];
+      ga[0] = p0.x; ga[1] = p0.y; ga[2] = p1.x;
+      gb[0] = q0.x; gb[1] = q0.y; gb[2] = q1.x;
+    }
+    {
       const double2* B2 = reinterpret_cast<const double2*>(Bv + gp * 6);
       const double2 c0 = B2[0], c1 = B2[1], c2 = B2[2];
       const double B6[6] = {c0.x, c0.y, c1.x, c1.y, c2.x, c2.y};
       bsym_times(B6, ga, ba);
       bsym_times(B6, gb, bb);
-    } else {
-      const double2* ga2 = reinterpret_cast<const double2*>(Gp + a * 6);
-      const double2* gb2 = reinterpret_cast<const double2*>(Gp + b * 6);
-      const double2 p0 = ga2[0], p1 = ga2[1], p2 = ga2[2];
-      const double2 q0 = gb2[0], q1 = gb2[1], q2 = gb2[2];
-      ga[0] = p0.x; ga[1] = p0.y; ga[2] = p1.x; ba[0] = p1.y; ba[1] = p2.x; ba[2] = p2.y;
-      gb[0] = q0.x; gb[1] = q0.y; gb[2] = q1.x; bb[0] = q1.y; bb[1] = q2.x; bb[2] = q2.y;
     }
     tangent_gp<!MIXED>(ga, ba, gb, bb, S[0], S[1], S[2], S[3], MIXED ? 0.0 : S[4], v);
     if (St::P1) {
