@@ -73,10 +73,12 @@ unsigned nparts_for(int64_t n) {
   return (unsigned)g;
 }
 
-// y = A x ; partial[b] = sum over the block's rows of x_r y_r (when part != null).
-template <int LPR>
+// y = A x ; partial[b] = sum over the block's rows of x_r y_r (when part != null).  V: the type the
+// matrix values are stored in (double, or float for a preconditioner's operator); products and
+// sums in double.
+template <int LPR, typename V = double>
 __global__ void __launch_bounds__(kRB) k_spmv(int64_t n, const int32_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                              const int32_t* __restrict__ colidx, const V* __restrict__ vals,
                                               const double* __restrict__ x, double* __restrict__ y,
                                               double* __restrict__ part, const double* __restrict__ sc) {
   if (sc && sc[S_DONE] != 0.0) return;
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(kRB) k_spmv(int64_t n, const int32_t* __restri
     if (r < n) {
       const int b = rowptr[r], e = rowptr[r + 1];
 #pragma unroll 4
-      for (int k = b + l; k < e; k += LPR) s += vals[k] * x[colidx[k]];
+      for (int k = b + l; k < e; k += LPR) s += (double)vals[k] * x[colidx[k]];
     }
 #pragma unroll
     for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
@@ -109,10 +111,10 @@ __global__ void __launch_bounds__(kRB) k_spmv(int64_t n, const int32_t* __restri
 // pressure row), which share one column list, so each column index and x value is read once for
 // up to three rows.  Entries go to the LPR lanes as in k_spmv (same per-row summation order);
 // part[b] = sum over the block's rows of x_r y_r (when part != null).
-template <int LPR>
+template <int LPR, typename V = double>
 __global__ void __launch_bounds__(kRB) k_spmv_grp(int64_t nblk, const int32_t* __restrict__ bs,
                                                   const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
-                                                  const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                                  const int32_t* __restrict__ colidx, const V* __restrict__ vals,
                                                   const double* __restrict__ x, double* __restrict__ y,
                                                   double* __restrict__ part, const double* __restrict__ sc) {
   if (sc && sc[S_DONE] != 0.0) return;
@@ -132,9 +134,9 @@ __global__ void __launch_bounds__(kRB) k_spmv_grp(int64_t nblk, const int32_t* _
 #pragma unroll 4
       for (int j = l; j < len; j += LPR) {
         const double xv = x[colidx[b0 + j]];
-        s0 += vals[b0 + j] * xv;
-        if (m > 1) s1 += vals[b1 + j] * xv;
-        if (m > 2) s2 += vals[b2 + j] * xv;
+        s0 += (double)vals[b0 + j] * xv;
+        if (m > 1) s1 += (double)vals[b1 + j] * xv;
+        if (m > 2) s2 += (double)vals[b2 + j] * xv;
       }
     }
 #pragma unroll
@@ -494,46 +496,62 @@ __global__ void __launch_bounds__(kRB) k_pc_apply(int64_t nblk, const int32_t* _
 }  // namespace
 
 // SpMV dispatcher: lanes per row from the mean row length.
-mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals, int64_t n, int64_t nnz,
-                const double* x, double* y, double* part, unsigned np, const double* sc, hipStream_t s) {
+template <typename V>
+static mpf_status spmv_t(const int32_t* rowptr, const int32_t* colidx, const V* vals, int64_t n, int64_t nnz,
+                         const double* x, double* y, double* part, unsigned np, const double* sc, hipStream_t s) {
   const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
   if (avg > 48) {
     constexpr int L = 32;
     const unsigned g = part ? np : grid_for(n, kRB / L);
-    hipLaunchKernelGGL(k_spmv<L>, dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv<L, V>), dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
   } else if (avg > 12) {
     constexpr int L = 16;
     const unsigned g = part ? np : grid_for(n, kRB / L);
-    hipLaunchKernelGGL(k_spmv<L>, dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv<L, V>), dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
   } else {
     constexpr int L = 4;
     const unsigned g = part ? np : grid_for(n, kRB / L);
-    hipLaunchKernelGGL(k_spmv<L>, dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv<L, V>), dim3(g), dim3(kRB), 0, s, n, rowptr, colidx, vals, x, y, part, sc);
   }
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
 
-// y = A x: the row-group kernel when the blocks share column lists, else the per-row kernel.
-// part: partial x.y per workgroup (np of them) or null.
-static mpf_status spmv_view(const CsrView& A, const BlockPcView& B, const double* x, double* y, double* part,
-                            unsigned np, const double* sc, hipStream_t s) {
+mpf_status spmv(const int32_t* rowptr, const int32_t* colidx, const double* vals, int64_t n, int64_t nnz,
+                const double* x, double* y, double* part, unsigned np, const double* sc, hipStream_t s) {
+  return spmv_t<double>(rowptr, colidx, vals, n, nnz, x, y, part, np, sc, s);
+}
+
+template <typename V>
+static mpf_status spmv_grp_t(const CsrView& A, const BlockPcView& B, const V* vals, const double* x, double* y,
+                             double* part, unsigned np, const double* sc, hipStream_t s) {
   const int64_t n = A.n, nnz = A.nnz;
-  if (!B.grp) return spmv(A.rowptr, A.colidx, A.vals, n, nnz, x, y, part, np, sc, s);
   const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
   const int32_t *bs = B.bs, *bm = B.bm, *rp = A.rowptr, *ci = A.colidx;
   if (avg > 48) {
     const unsigned g = part ? np : grid_for(B.nblk, kRB / 32);
-    hipLaunchKernelGGL(k_spmv_grp<32>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv_grp<32, V>), dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, vals, x, y, part, sc);
   } else if (avg > 12) {
     const unsigned g = part ? np : grid_for(B.nblk, kRB / 16);
-    hipLaunchKernelGGL(k_spmv_grp<16>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv_grp<16, V>), dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, vals, x, y, part, sc);
   } else {
     const unsigned g = part ? np : grid_for(B.nblk, kRB / 4);
-    hipLaunchKernelGGL(k_spmv_grp<4>, dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, A.vals, x, y, part, sc);
+    hipLaunchKernelGGL((k_spmv_grp<4, V>), dim3(g), dim3(kRB), 0, s, B.nblk, bs, bm, rp, ci, vals, x, y, part, sc);
   }
   MPF_HIP(hipGetLastError());
   return MPF_OK;
+}
+
+// y = A x: the row-group kernel when the blocks share column lists, else the per-row kernel; the
+// FP32 values when the view has them.  part: partial x.y per workgroup (np of them) or null.
+static mpf_status spmv_view(const CsrView& A, const BlockPcView& B, const double* x, double* y, double* part,
+                            unsigned np, const double* sc, hipStream_t s) {
+  if (A.vals32) {
+    if (!B.grp) return spmv_t<float>(A.rowptr, A.colidx, A.vals32, A.n, A.nnz, x, y, part, np, sc, s);
+    return spmv_grp_t<float>(A, B, A.vals32, x, y, part, np, sc, s);
+  }
+  if (!B.grp) return spmv(A.rowptr, A.colidx, A.vals, A.n, A.nnz, x, y, part, np, sc, s);
+  return spmv_grp_t<double>(A, B, A.vals, x, y, part, np, sc, s);
 }
 
 static CsrView csr_of(const mpf_problem* p) {

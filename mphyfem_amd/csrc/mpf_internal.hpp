@@ -254,6 +254,7 @@ struct CsrView {
   const int32_t* rowptr = nullptr;
   const int32_t* colidx = nullptr;
   const double* vals = nullptr;
+  const float* vals32 = nullptr;  // when set, the products read these FP32 copies of vals (sums in FP64)
 };
 struct BlockPcView {
   int64_t nblk = 0;

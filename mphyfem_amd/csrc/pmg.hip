@@ -58,6 +58,7 @@ struct PmgWork {
   DevBuf<double> rw;
   DevBuf<int32_t> crow, ccol;         // Kc pattern
   DevBuf<double> cval;
+  DevBuf<float> cval32;  // FP32 copy of cval read by the coarse CG's products
   DevBuf<int32_t> cbs, cbm, crblk;    // coarse node blocks
   DevBuf<double> cbinv;
   bool cgrp = false;
@@ -113,6 +114,15 @@ __global__ void __launch_bounds__(kT) k_galerkin(int64_t nc, const int32_t* __re
       }
     }
   }
+}
+
+// FP32 copy of the coarse matrix for the coarse CG's products: the coarse solve is a preconditioner
+// applied inside flexible GMRES (the outer residual is formed with the FP64 system), so its operator
+// can be rounded to FP32 -- it stays symmetric, the products accumulate in FP64, and the matrix
+// bytes each coarse CG iteration reads drop from 8 to 4 per value.
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // Pressure block: 1 / (Kpp_qq scale), or the node-block Schur diagonal's inverse (binv_p[9 q])
@@ -527,6 +537,7 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
   MPF_TRY(W.cbs.upload(cbs.data(), cbs.size(), s));
   MPF_TRY(W.cbm.upload(cbm.data(), cbm.size(), s));
   MPF_TRY(W.cval.alloc(W.nnzc));
+  MPF_TRY(W.cval32.alloc(W.nnzc));
   MPF_TRY(W.crblk.alloc(nc));
   MPF_TRY(W.cbinv.alloc(9 * (size_t)W.ncb));
   MPF_TRY(W.sinv.alloc(W.pn > 0 ? W.pn : 1));
@@ -554,6 +565,7 @@ CsrView coarse_csr(const PmgWork& W) {
   C.rowptr = W.crow.ptr;
   C.colidx = W.ccol.ptr;
   C.vals = W.cval.ptr;
+  C.vals32 = W.cval32.ptr;
   return C;
 }
 
@@ -581,6 +593,7 @@ mpf_status pmg_setup(mpf_problem* p, const BlockPcView& F, int64_t nub) {
                      A.rowptr, A.colidx, A.vals, (int32_t)W.d, W.pidx.ptr, W.pw.ptr, W.crow.ptr, W.ccol.ptr,
                      W.cval.ptr, W.bad.ptr);
   MPF_TRY(block_pc_values(coarse_csr(W), W.ncb, W.cbs.ptr, W.cbm.ptr, W.cbinv.ptr, s));
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(W.nnzc, kT)), dim3(kT), 0, s, W.nnzc, W.cval.ptr, W.cval32.ptr);
   if (!W.ulen_ready || W.ulen.n != (size_t)nub) {
     MPF_TRY(W.ulen.alloc(nub > 0 ? nub : 1));
     hipLaunchKernelGGL(k_ulen, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, A.rowptr, A.colidx, (int32_t)W.d,
