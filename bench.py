@@ -500,11 +500,11 @@ def main(argv=None) -> int:
             st = P.pmg_stats()
             its = max(r.iterations, 1)
             # per outer iteration: A z (12 bytes per nonzero), the V-cycle's 2 x 3 (Chebyshev degree)
-            # displacement-row sweeps (counted as full matrix reads), the coarse CG's matrix (FP32
-            # values and int32 columns, 8 bytes per nonzero) and ~18 vector streams per iteration,
-            # and the Arnoldi step's CGS2 over on average m/2 + 1 basis vectors (4 reads each) plus
-            # ~10 vector streams
-            fine = 7 * 12 * nnz
+            # displacement-row sweeps over the FP32 copy of the values (counted as full matrix reads,
+            # 8 bytes per nonzero), the coarse CG's matrix (FP32 values and int32 columns, 8 bytes per
+            # nonzero) and ~18 vector streams per iteration, and the Arnoldi step's CGS2 over on
+            # average m/2 + 1 basis vectors (4 reads each) plus ~10 vector streams
+            fine = 12 * nnz + 6 * 8 * nnz
             coarse = st.coarse_iterations / its * (8 * st.coarse_nnz + 144 * st.coarse_dofs)
             ortho = (4 * (restart / 2 + 1) + 10) * 8 * (dispDOF + presDOF)
             bpi = fine + coarse + ortho

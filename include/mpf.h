@@ -209,13 +209,14 @@ mpf_status mpf_rhs_norm(mpf_problem* p, double* norm);
 /* Chebyshev smoothing (3x3 node-block Jacobi inside) on the quadratic dofs and a coarse correction */
 /* on the vertex dofs, whose matrix is the Galerkin product P^T Kuu P (P: the Tet10 -> Tet4         */
 /* interpolation, a mid-edge node taking the mean of its two vertices) solved by block-Jacobi CG to */
-/* a loose tolerance, its products reading the coarse matrix rounded to FP32 (sums in FP64; the     */
-/* outer residual uses the FP64 system).  Pressure rows (MPF_TET10P1): block upper-triangular, the  */
-/* pressure block being diag(Kpp) (1 + 1 / (mu Kinv)) (the pressure mass matrix scaled to the Schur */
-/* complement of a nearly incompressible material; Kinv = 0: the node-block Schur diagonal).  It    */
-/* stands where petsc_options select a multigrid or field-split preconditioner (-pc_type mg / gamg */
-/* / fieldsplit) for the reference's KSPSolve (SolverPetsc.cpp:117-140, 290); the code defaults     */
-/* there are CG and ILU.  mpf_set_pmg changes its parameters, mpf_pmg_stats reports its last use.   */
+/* a loose tolerance; the smoothing and the coarse CG read their matrices rounded to FP32 (sums in  */
+/* FP64; the outer residual uses the FP64 system).  Pressure rows (MPF_TET10P1): block upper-       */
+/* triangular, the pressure block being diag(Kpp) (1 + 1 / (mu Kinv)) (the pressure mass matrix     */
+/* scaled to the Schur complement of a nearly incompressible material; Kinv = 0: the node-block     */
+/* Schur diagonal).  It stands where petsc_options select a multigrid or field-split preconditioner */
+/* (-pc_type mg / gamg / fieldsplit) for the reference's KSPSolve (SolverPetsc.cpp:117-140, 290);  */
+/* the code defaults there are CG and ILU.  mpf_set_pmg changes its parameters, mpf_pmg_stats      */
+/* reports its last use.                                                                            */
 enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1, MPF_KSP_BAND = 2 };
 enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2, MPF_PC_PMG = 3 };
 typedef struct {

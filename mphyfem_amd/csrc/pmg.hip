@@ -59,6 +59,7 @@ struct PmgWork {
   DevBuf<int32_t> crow, ccol;         // Kc pattern
   DevBuf<double> cval;
   DevBuf<float> cval32;  // FP32 copy of cval read by the coarse CG's products
+  DevBuf<float> fval32;  // FP32 copy of the system's values read by the smoothing sweeps
   DevBuf<int32_t> cbs, cbm, crblk;    // coarse node blocks
   DevBuf<double> cbinv;
   bool cgrp = false;
@@ -116,10 +117,11 @@ __global__ void __launch_bounds__(kT) k_galerkin(int64_t nc, const int32_t* __re
   }
 }
 
-// FP32 copy of the coarse matrix for the coarse CG's products: the coarse solve is a preconditioner
-// applied inside flexible GMRES (the outer residual is formed with the FP64 system), so its operator
-// can be rounded to FP32 -- it stays symmetric, the products accumulate in FP64, and the matrix
-// bytes each coarse CG iteration reads drop from 8 to 4 per value.
+// FP32 copies of the matrices the preconditioner applies (the coarse matrix in the coarse CG's
+// products, the system's values in the smoothing sweeps): the V-cycle is a preconditioner inside
+// flexible GMRES, whose residual is formed with the FP64 system, so its operators can be rounded to
+// FP32 -- they stay symmetric, the products accumulate in FP64, and the matrix bytes each product
+// reads drop from 8 to 4 per value.
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -204,10 +206,10 @@ __global__ void __launch_bounds__(kT) k_jac0(int64_t nub, const int32_t* __restr
 // one column list, so each column index and x value is read once for its up to three rows):
 //   MODE 1 (Chebyshev step): dd = c1 dd + c2 B^-1 (t - Kuu x_in);  x_out = x_in + dd
 //   MODE 2 (residual):       x_out = t - Kuu x_in
-template <int LPR, int MODE>
+template <int LPR, int MODE, typename V = double>
 __global__ void __launch_bounds__(kT) k_sweep(int64_t nub, const int32_t* __restrict__ bs,
                                               const int32_t* __restrict__ bm, const int32_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ colidx, const double* __restrict__ vals,
+                                              const int32_t* __restrict__ colidx, const V* __restrict__ vals,
                                               const int32_t* __restrict__ ulen, const double* __restrict__ binv,
                                               double c1, double c2, const double* __restrict__ t,
                                               const double* __restrict__ xin, double* __restrict__ xout,
@@ -237,9 +239,9 @@ __global__ void __launch_bounds__(kT) k_sweep(int64_t nub, const int32_t* __rest
 #pragma unroll 4
       for (int j = l; j < len; j += LPR) {
         const double xv = xin[colidx[b0 + j]];
-        s0 += vals[b0 + j] * xv;
-        if (m > 1) s1 += vals[b1 + j] * xv;
-        if (m > 2) s2 += vals[b2 + j] * xv;
+        s0 += (double)vals[b0 + j] * xv;
+        if (m > 1) s1 += (double)vals[b1 + j] * xv;
+        if (m > 2) s2 += (double)vals[b2 + j] * xv;
       }
     }
 #pragma unroll
@@ -353,18 +355,21 @@ __global__ void __launch_bounds__(kT) k_prolong(int32_t d, const int32_t* __rest
   x[f] = s;
 }
 
+// The sweeps read the FP32 copy of the matrix values (vals32): the smoother is part of the
+// preconditioner, and its products still sum in FP64.
 template <int MODE>
-void launch_sweep(int lpr, int64_t nub, const BlockPcView& F, const CsrView& A, const int32_t* ulen, double c1,
-                  double c2, const double* t, const double* xin, double* xout, double* dd, hipStream_t s) {
+void launch_sweep(int lpr, int64_t nub, const BlockPcView& F, const CsrView& A, const float* vals32,
+                  const int32_t* ulen, double c1, double c2, const double* t, const double* xin, double* xout,
+                  double* dd, hipStream_t s) {
   if (lpr == 32)
-    hipLaunchKernelGGL((k_sweep<32, MODE>), dim3(grid_for(nub, kT / 32)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
+    hipLaunchKernelGGL((k_sweep<32, MODE, float>), dim3(grid_for(nub, kT / 32)), dim3(kT), 0, s, nub, F.bs, F.bm,
+                       A.rowptr, A.colidx, vals32, ulen, F.binv, c1, c2, t, xin, xout, dd);
   else if (lpr == 16)
-    hipLaunchKernelGGL((k_sweep<16, MODE>), dim3(grid_for(nub, kT / 16)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
+    hipLaunchKernelGGL((k_sweep<16, MODE, float>), dim3(grid_for(nub, kT / 16)), dim3(kT), 0, s, nub, F.bs, F.bm,
+                       A.rowptr, A.colidx, vals32, ulen, F.binv, c1, c2, t, xin, xout, dd);
   else
-    hipLaunchKernelGGL((k_sweep<4, MODE>), dim3(grid_for(nub, kT / 4)), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr,
-                       A.colidx, A.vals, ulen, F.binv, c1, c2, t, xin, xout, dd);
+    hipLaunchKernelGGL((k_sweep<4, MODE, float>), dim3(grid_for(nub, kT / 4)), dim3(kT), 0, s, nub, F.bs, F.bm,
+                       A.rowptr, A.colidx, vals32, ulen, F.binv, c1, c2, t, xin, xout, dd);
 }
 
 CsrView fine_csr(const mpf_problem* p);
@@ -594,6 +599,8 @@ mpf_status pmg_setup(mpf_problem* p, const BlockPcView& F, int64_t nub) {
                      W.cval.ptr, W.bad.ptr);
   MPF_TRY(block_pc_values(coarse_csr(W), W.ncb, W.cbs.ptr, W.cbm.ptr, W.cbinv.ptr, s));
   hipLaunchKernelGGL(k_to_f32, dim3(grid_for(W.nnzc, kT)), dim3(kT), 0, s, W.nnzc, W.cval.ptr, W.cval32.ptr);
+  if (W.fval32.n != (size_t)A.nnz) MPF_TRY(W.fval32.alloc(A.nnz > 0 ? (size_t)A.nnz : 1));
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(A.nnz, kT)), dim3(kT), 0, s, A.nnz, A.vals, W.fval32.ptr);
   if (!W.ulen_ready || W.ulen.n != (size_t)nub) {
     MPF_TRY(W.ulen.alloc(nub > 0 ? nub : 1));
     hipLaunchKernelGGL(k_ulen, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, A.rowptr, A.colidx, (int32_t)W.d,
@@ -645,11 +652,11 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   hipLaunchKernelGGL(k_jac0, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, A.rowptr, A.colidx,
                      A.vals, W.ulen.ptr, d, W.pn > 0 ? z + d : nullptr, c2[0], v, W.t.ptr, cur, W.dd.ptr);
   for (int k = 1; k < W.degree; ++k) {
-    launch_sweep<1>(lpr, nub, F, A, W.ulen.ptr, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
+    launch_sweep<1>(lpr, nub, F, A, W.fval32.ptr, W.ulen.ptr, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
     std::swap(cur, oth);
   }
   // coarse correction
-  launch_sweep<2>(lpr, nub, F, A, W.ulen.ptr, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
+  launch_sweep<2>(lpr, nub, F, A, W.fval32.ptr, W.ulen.ptr, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
   hipLaunchKernelGGL(k_restrict, dim3(grid_for(W.nc, kT)), dim3(kT), 0, s, W.nc, W.rptr.ptr, W.rfine.ptr, W.rw.ptr,
                      W.res.ptr, W.rc.ptr);
   BlockPcView C;
@@ -667,7 +674,7 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   // post-smoothing from the corrected x, the last step into z
   for (int k = 0; k < W.degree; ++k) {
     double* out = (k == W.degree - 1) ? z : oth;
-    launch_sweep<1>(lpr, nub, F, A, W.ulen.ptr, c1[k], c2[k], t, cur, out, W.dd.ptr, s);
+    launch_sweep<1>(lpr, nub, F, A, W.fval32.ptr, W.ulen.ptr, c1[k], c2[k], t, cur, out, W.dd.ptr, s);
     oth = cur;
     cur = out;
   }
