@@ -238,25 +238,38 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* independent blocks, largest half-bandwidth, free dofs, and the factorisation's algorithmic      */
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
-/* Two-level preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before   */
+/* p-multigrid preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before */
 /* and after the coarse correction (1..8, default 3), the ratio beta / alpha of its interval        */
 /* (1 < ratio < 1e6, default 10; beta = 1.1 x the power-iteration estimate of the largest           */
 /* eigenvalue of the node-block-Jacobi-preconditioned Kuu), the coarse CG's relative tolerance      */
 /* (0 < coarse_rtol < 1, default 0.1) and iteration limit (default 1000).  No counterpart in the    */
 /* reference.                                                                                        */
 mpf_status mpf_set_pmg(mpf_problem* p, int32_t degree, double ratio, double coarse_rtol, int32_t coarse_maxit);
-/* Its coarse level and its work since the last solve began: coarse dofs and nonzeros, V-cycles   */
-/* applied, coarse CG iterations in total, the set-up time of the last solve (transfer tables on  */
-/* first use; the Galerkin product, the block inverses and the eigenvalue estimate every solve)    */
-/* and that estimate.  No counterpart in the                                                       */
-/* reference (reporting only).                                                                     */
+/* Levels of MPF_PC_PMG: 2, the vertex level solved by block-Jacobi CG to coarse_rtol; 3 (default),  */
+/* the vertex level given a V-cycle of its own -- Chebyshev smoothing with its 3x3 vertex blocks    */
+/* (same degree and ratio, its own eigenvalue estimate), a correction from aggregates of 8 vertices */
+/* (consecutive in Morton order of their coordinates, one dof per aggregate and component, the      */
+/* aggregate matrix Pa^T Kc Pa with Pa the 0/1 aggregation) solved by block-Jacobi CG to            */
+/* coarse_rtol, and the smoothing again.  Else MPF_ERR_ARG.  No counterpart in the reference.       */
+mpf_status mpf_set_pmg_levels(mpf_problem* p, int32_t levels);
+/* Its coarse levels and its work since the last solve began: vertex-level dofs and nonzeros,      */
+/* V-cycles applied, CG iterations on the lowest level in total (the vertex level with 2 levels,   */
+/* the aggregates with 3), the set-up time of the last solve (transfer tables on first use; the    */
+/* Galerkin products, the block inverses and the eigenvalue estimates every solve), the smoother's */
+/* estimate on the fine level and on the vertex level (3 levels), the aggregate level's dofs and    */
+/* nonzeros, and the number of levels.  No counterpart in the reference (reporting only).          */
 typedef struct {
   int64_t coarse_dofs;
   int64_t coarse_nnz;
   int64_t vcycles;
   int64_t coarse_iterations;
   double setup_seconds;
-  double lambda_max;    /* the smoother's eigenvalue estimate */
+  double lambda_max;    /* the fine-level smoother's eigenvalue estimate */
+  double lambda_vertex; /* the vertex-level smoother's (3 levels; else 0) */
+  int64_t agg_dofs;
+  int64_t agg_nnz;
+  int32_t levels;
+  int32_t reserved;
 } mpf_pmg_report;
 mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep);
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);

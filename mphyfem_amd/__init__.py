@@ -58,7 +58,9 @@ class KrylovOpts(C.Structure):
 
 class PmgReport(C.Structure):
     _fields_ = [("coarse_dofs", C.c_int64), ("coarse_nnz", C.c_int64), ("vcycles", C.c_int64),
-                ("coarse_iterations", C.c_int64), ("setup_seconds", C.c_double), ("lambda_max", C.c_double)]
+                ("coarse_iterations", C.c_int64), ("setup_seconds", C.c_double), ("lambda_max", C.c_double),
+                ("lambda_vertex", C.c_double), ("agg_dofs", C.c_int64), ("agg_nnz", C.c_int64),
+                ("levels", C.c_int32), ("reserved", C.c_int32)]
 
 
 class KrylovReport(C.Structure):
@@ -197,6 +199,7 @@ SIGNATURES = [
     ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
     ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
     ("mpf_set_pmg", C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_int32]),
+    ("mpf_set_pmg_levels", C.c_int, [_P, C.c_int32]),
     ("mpf_batch_assembly_choice", C.c_int, [_P, _I32P, _I32P, _DP]),
     ("mpf_batch_time_direct", C.c_int, [_P, C.c_int32, _DP]),
     ("mpf_pmg_stats", C.c_int, [_P, C.POINTER(PmgReport)]),
@@ -701,10 +704,12 @@ class Problem:
         _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
         return nb.value, bm.value, nd.value, fl.value
 
-    def set_pmg(self, degree=3, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000):
-        """Parameters of the two-level preconditioner PC_PMG (Chebyshev smoothing degree and interval
-        ratio, the coarse CG's relative tolerance and iteration limit)."""
+    def set_pmg(self, degree=3, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000, levels=3):
+        """Parameters of the p-multigrid preconditioner PC_PMG (Chebyshev smoothing degree and interval
+        ratio, the lowest level's CG relative tolerance and iteration limit, and the number of levels:
+        2, the vertex level solved by CG; 3, a V-cycle on the vertex level with vertex aggregates below)."""
         _check(lib().mpf_set_pmg(self._h, degree, ratio, coarse_rtol, coarse_maxit), "mpf_set_pmg")
+        _check(lib().mpf_set_pmg_levels(self._h, levels), "mpf_set_pmg_levels")
 
     def pmg_stats(self) -> PmgReport:
         """Coarse level size and the V-cycles / coarse CG iterations of the last PC_PMG solve."""

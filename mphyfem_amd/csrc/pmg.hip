@@ -69,8 +69,26 @@ struct PmgWork {
   bool ulen_ready = false;
   double lambda = 0.0;                // estimate of the largest eigenvalue of B^-1 Kuu
   DevBuf<double> rc, xc;              // [nc]
-  DevBuf<double> cr, cr2, cz, cp, cq, cpart, csc;  // coarse CG
+  DevBuf<double> cr, cr2, cz, cp, cq, cpart, csc;  // coarse CG (two levels) / aggregate CG (three)
   DevBuf<int32_t> bad;
+  // third level (levels == 3): the vertex level gets a V-cycle of its own -- Chebyshev smoothing
+  // with its node blocks, a correction from aggregates of 8 vertices (consecutive in Morton order,
+  // one dof per aggregate and component, unsmoothed: Ka = Pa^T Kc Pa), Ka solved by block-Jacobi
+  // CG to crtol -- in place of the CG on Kc
+  int levels = 3;
+  int64_t na = 0, nka = 0, nab = 0;        // aggregate dofs, nonzeros of Ka, aggregate blocks
+  DevBuf<int32_t> aoff;                    // [nc] aggregate dof of each vertex-level dof
+  DevBuf<int32_t> amptr, amem;             // [na + 1], [nc]: vertex-level dofs of each aggregate dof
+  DevBuf<int32_t> karow, kacol;            // Ka pattern
+  DevBuf<double> kaval;
+  DevBuf<float> kaval32;                   // FP32 copy read by the aggregate CG's products
+  DevBuf<int32_t> kabs, kabm, karblk;      // aggregate blocks (an aggregate's components)
+  DevBuf<double> kabinv;
+  bool kagrp = false;
+  DevBuf<int32_t> culen;                   // [ncb] row length of each vertex block's rows
+  DevBuf<double> cxa, cxb, cdd, cres;      // [nc] vertex-level V-cycle
+  DevBuf<double> ra, xa3;                  // [na] aggregate right-hand side and solution
+  double lambda_c = 0.0;                   // largest eigenvalue of Bc^-1 Kc
   // statistics of the current solve
   int64_t vcycles = 0, coarse_its = 0;
   double setup_seconds = 0.0;
@@ -125,6 +143,56 @@ __global__ void __launch_bounds__(kT) k_galerkin(int64_t nc, const int32_t* __re
 __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
+}
+
+// Ka = Pa^T Kc Pa with Pa the aggregation (0/1): one thread per aggregate dof row A, which sums the
+// Kc rows of its member dofs (ascending) into the columns of their aggregate dofs, in a fixed order.
+// bad[0] = 1 if some column is not in the pattern.
+__global__ void __launch_bounds__(kT) k_agg_values(int64_t na, const int32_t* __restrict__ amptr,
+                                                   const int32_t* __restrict__ amem, const int32_t* __restrict__ crow,
+                                                   const int32_t* __restrict__ ccol, const double* __restrict__ cval,
+                                                   const int32_t* __restrict__ aoff, const int32_t* __restrict__ karow,
+                                                   const int32_t* __restrict__ kacol, double* __restrict__ kaval,
+                                                   int32_t* __restrict__ bad) {
+  const int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= na) return;
+  const int kb = karow[A], ke = karow[A + 1];
+  for (int k = kb; k < ke; ++k) kaval[k] = 0.0;
+  for (int t = amptr[A]; t < amptr[A + 1]; ++t) {
+    const int i = amem[t];
+    for (int k = crow[i]; k < crow[i + 1]; ++k) {
+      const int J = aoff[ccol[k]];
+      int lo = kb, hi = ke;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (kacol[mid] < J) lo = mid + 1; else hi = mid;
+      }
+      if (lo < ke && kacol[lo] == J) kaval[lo] += cval[k];
+      else *bad = 1;
+    }
+  }
+}
+
+// ra = Pa^T r (each aggregate dof sums its members in ascending order)
+__global__ void k_agg_restrict(int64_t na, const int32_t* __restrict__ amptr, const int32_t* __restrict__ amem,
+                               const double* __restrict__ r, double* __restrict__ ra) {
+  const int64_t A = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= na) return;
+  double s = 0.0;
+  for (int t = amptr[A]; t < amptr[A + 1]; ++t) s += r[amem[t]];
+  ra[A] = s;
+}
+
+// x += Pa xa
+__global__ void k_agg_prolong(int64_t nc, const int32_t* __restrict__ aoff, const double* __restrict__ xa,
+                              double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nc) x[i] += xa[aoff[i]];
+}
+
+__global__ void k_zero(int64_t n, double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
 }
 
 // Pressure block: 1 / (Kpp_qq scale), or the node-block Schur diagonal's inverse (binv_p[9 q])
@@ -376,34 +444,32 @@ CsrView fine_csr(const mpf_problem* p);
 
 // Largest eigenvalue of B^-1 Kuu: Rayleigh quotient (s.y) / (x.s) of 20 power steps from the ones
 // vector (partial sums added on the host in workgroup order).
-mpf_status estimate_lambda(mpf_problem* p, PmgWork& W, const BlockPcView& F, int64_t nub, int lpr, double* out) {
-  hipStream_t s = p->stream;
-  const CsrView A = fine_csr(p);
-  const int32_t d = (int32_t)W.d;
+mpf_status estimate_lambda(hipStream_t s, const CsrView& A, const BlockPcView& F, const int32_t* ulen, int64_t nub,
+                           int64_t nrows, int lpr, double* xbuf, double* ybuf, double* part, double* out) {
   const unsigned G = std::min<unsigned>(grid_for(nub, kT / lpr), kMaxPartials);
-  hipLaunchKernelGGL(k_ones, dim3(grid_for(d, kT)), dim3(kT), 0, s, (int64_t)d, W.xa.ptr);
-  double* x = W.xa.ptr;
-  double* y = W.xb.ptr;
+  hipLaunchKernelGGL(k_ones, dim3(grid_for(nrows, kT)), dim3(kT), 0, s, nrows, xbuf);
+  double* x = xbuf;
+  double* y = ybuf;
   double c = 1.0, lam = 0.0;
   std::vector<double> h(3 * (size_t)G);
   for (int it = 0; it < 20; ++it) {
     if (lpr == 32)
-      hipLaunchKernelGGL(k_power<32>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
-                         c, x, y, W.cpart.ptr);
+      hipLaunchKernelGGL(k_power<32>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, ulen, F.binv,
+                         c, x, y, part);
     else if (lpr == 16)
-      hipLaunchKernelGGL(k_power<16>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
-                         c, x, y, W.cpart.ptr);
+      hipLaunchKernelGGL(k_power<16>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, ulen, F.binv,
+                         c, x, y, part);
     else
-      hipLaunchKernelGGL(k_power<4>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, W.ulen.ptr, F.binv,
-                         c, x, y, W.cpart.ptr);
+      hipLaunchKernelGGL(k_power<4>, dim3(G), dim3(kT), 0, s, nub, F.bs, F.bm, A.rowptr, A.colidx, A.vals, ulen, F.binv,
+                         c, x, y, part);
     MPF_HIP(hipGetLastError());
-    MPF_HIP(hipMemcpyAsync(h.data(), W.cpart.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPF_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     MPF_HIP(hipStreamSynchronize(s));
     double sy = 0.0, xs = 0.0, yy = 0.0;
-    for (unsigned b = 0; b < G; ++b) {
-      sy += h[b];
-      xs += h[G + b];
-      yy += h[2 * (size_t)G + b];
+    for (unsigned q = 0; q < G; ++q) {
+      sy += h[q];
+      xs += h[G + q];
+      yy += h[2 * (size_t)G + q];
     }
     if (!(xs > 0.0) || !(yy > 0.0)) break;
     lam = sy / xs;
@@ -449,7 +515,7 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
       }
     }
   // coarse numbering: free vertex dofs, vertex-major in node order
-  std::vector<int32_t> cnum((size_t)nN * 3, -1), cbs, cbm;
+  std::vector<int32_t> cnum((size_t)nN * 3, -1), cbs, cbm, bvert;
   int64_t nc = 0;
   for (int32_t v = 0; v < nN; ++v) {
     if (!isv[v]) continue;
@@ -459,6 +525,7 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
     if (nc > st) {
       cbs.push_back((int32_t)st);
       cbm.push_back((int32_t)(nc - st));
+      bvert.push_back(v);
     }
   }
   if (nc == 0 || nc > INT32_MAX / 2) return MPF_ERR_ARG;
@@ -527,11 +594,96 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
     }
     if (ccol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
   }
+  // third level: aggregates of 8 vertex blocks consecutive in Morton order of the vertices (on a
+  // lattice of about the vertex spacing), one aggregate dof per aggregate and component present,
+  // numbered aggregate by aggregate
+  const int64_t ncb = (int64_t)cbs.size();
+  std::vector<int32_t> aoff(nc), amptr, amem(nc), karow(1, 0), kacol, kabs, kabm;
+  {
+    std::vector<double> Xh((size_t)nN * 3);
+    MPF_TRY(p->X.download(Xh.data(), Xh.size(), s));
+    MPF_HIP(hipStreamSynchronize(s));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t b = 0; b < ncb; ++b)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], Xh[3 * (size_t)bvert[b] + c]);
+        hi[c] = std::max(hi[c], Xh[3 * (size_t)bvert[b] + c]);
+      }
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    double vol = 1.0;
+    for (int c = 0; c < 3; ++c) vol *= std::max(hi[c] - lo[c], ext * 1e-3);
+    double h = ncb > 0 ? std::cbrt(vol / (double)ncb) : 1.0;
+    if (!(h > 0.0)) h = 1.0;
+    h = std::max(h, ext / 2097151.0);
+    auto spread = [](uint64_t v) {
+      uint64_t o = 0;
+      for (int k = 0; k < 21; ++k) o |= ((v >> k) & 1ull) << (3 * k);
+      return o;
+    };
+    std::vector<uint64_t> code(ncb);
+    for (int64_t b = 0; b < ncb; ++b) {
+      uint64_t q[3];
+      for (int c = 0; c < 3; ++c) {
+        const double t = std::floor((Xh[3 * (size_t)bvert[b] + c] - lo[c]) / h);
+        q[c] = (uint64_t)std::min(std::max(t, 0.0), 2097151.0);
+      }
+      code[b] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+    }
+    std::vector<int64_t> order(ncb);
+    for (int64_t b = 0; b < ncb; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return code[x] < code[y]; });
+    std::vector<int64_t> agg(ncb);
+    for (int64_t r = 0; r < ncb; ++r) agg[order[r]] = r / 8;
+    const int64_t nagg = (ncb + 7) / 8;
+    std::vector<char> present((size_t)nagg * 3, 0);
+    for (int64_t b = 0; b < ncb; ++b)
+      for (int a = 0; a < 3; ++a)
+        if (cnum[3 * (size_t)bvert[b] + a] >= 0) present[3 * (size_t)agg[b] + a] = 1;
+    std::vector<int32_t> adof((size_t)nagg * 3, -1);
+    int64_t na = 0;
+    for (int64_t g = 0; g < nagg; ++g) {
+      const int64_t st = na;
+      for (int a = 0; a < 3; ++a)
+        if (present[3 * (size_t)g + a]) adof[3 * (size_t)g + a] = (int32_t)na++;
+      if (na > st) {
+        kabs.push_back((int32_t)st);
+        kabm.push_back((int32_t)(na - st));
+      }
+    }
+    for (int64_t b = 0; b < ncb; ++b)
+      for (int a = 0; a < 3; ++a) {
+        const int32_t i = cnum[3 * (size_t)bvert[b] + a];
+        if (i >= 0) aoff[i] = adof[3 * (size_t)agg[b] + a];
+      }
+    amptr.assign(na + 1, 0);
+    for (int64_t i = 0; i < nc; ++i) ++amptr[aoff[i] + 1];
+    for (int64_t A = 0; A < na; ++A) amptr[A + 1] += amptr[A];
+    std::vector<int32_t> afill(amptr.begin(), amptr.end() - 1);
+    for (int64_t i = 0; i < nc; ++i) amem[afill[aoff[i]]++] = (int32_t)i;
+    std::vector<int32_t> cols;
+    for (int64_t A = 0; A < na; ++A) {
+      cols.clear();
+      for (int32_t t = amptr[A]; t < amptr[A + 1]; ++t) {
+        const int32_t i = amem[t];
+        for (int32_t k = crow[i]; k < crow[i + 1]; ++k) cols.push_back(aoff[ccol[k]]);
+      }
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      kacol.insert(kacol.end(), cols.begin(), cols.end());
+      karow.push_back((int32_t)kacol.size());
+      if (kacol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
+    }
+    W.na = na;
+    W.nka = (int64_t)kacol.size();
+    W.nab = (int64_t)kabs.size();
+  }
+  std::vector<int32_t> culen(ncb);
+  for (int64_t b = 0; b < ncb; ++b) culen[b] = crow[cbs[b] + 1] - crow[cbs[b]];
   W.d = d;
   W.pn = p->pat.presDOF;
   W.nc = nc;
   W.nnzc = (int64_t)ccol.size();
-  W.ncb = (int64_t)cbs.size();
+  W.ncb = ncb;
   MPF_TRY(W.pidx.upload(pidx.data(), pidx.size(), s));
   MPF_TRY(W.pw.upload(pw.data(), pw.size(), s));
   MPF_TRY(W.rptr.upload(rptr.data(), rptr.size(), s));
@@ -548,6 +700,20 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
   MPF_TRY(W.sinv.alloc(W.pn > 0 ? W.pn : 1));
   for (DevBuf<double>* b : {&W.t, &W.xa, &W.xb, &W.dd, &W.res}) MPF_TRY(b->alloc(d > 0 ? d : 1));
   for (DevBuf<double>* b : {&W.rc, &W.xc, &W.cr, &W.cr2, &W.cz, &W.cp, &W.cq}) MPF_TRY(b->alloc(nc));
+  MPF_TRY(W.aoff.upload(aoff.data(), aoff.size(), s));
+  MPF_TRY(W.amptr.upload(amptr.data(), amptr.size(), s));
+  MPF_TRY(W.amem.upload(amem.data(), amem.size(), s));
+  MPF_TRY(W.karow.upload(karow.data(), karow.size(), s));
+  MPF_TRY(W.kacol.upload(kacol.data(), kacol.size(), s));
+  MPF_TRY(W.kabs.upload(kabs.data(), kabs.size(), s));
+  MPF_TRY(W.kabm.upload(kabm.data(), kabm.size(), s));
+  MPF_TRY(W.culen.upload(culen.data(), culen.size(), s));
+  MPF_TRY(W.kaval.alloc(W.nka > 0 ? W.nka : 1));
+  MPF_TRY(W.kaval32.alloc(W.nka > 0 ? W.nka : 1));
+  MPF_TRY(W.karblk.alloc(W.na > 0 ? W.na : 1));
+  MPF_TRY(W.kabinv.alloc(9 * (size_t)(W.nab > 0 ? W.nab : 1)));
+  for (DevBuf<double>* b : {&W.cxa, &W.cxb, &W.cdd, &W.cres}) MPF_TRY(b->alloc(nc));
+  for (DevBuf<double>* b : {&W.ra, &W.xa3}) MPF_TRY(b->alloc(W.na > 0 ? W.na : 1));
   MPF_TRY(W.cpart.alloc(3 * (size_t)kMaxPartials));  // coarse CG: 2 per workgroup; power steps: 3
   MPF_TRY(W.csc.alloc(kCgScalars));
   MPF_TRY(W.bad.alloc(1));
@@ -557,6 +723,12 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
   C.rowptr = W.crow.ptr;
   C.colidx = W.ccol.ptr;
   MPF_TRY(block_pc_structure(C, W.ncb, W.cbs.ptr, W.cbm.ptr, W.crblk.ptr, &W.cgrp, s));  // synchronises
+  CsrView Ka;
+  Ka.n = W.na;
+  Ka.nnz = W.nka;
+  Ka.rowptr = W.karow.ptr;
+  Ka.colidx = W.kacol.ptr;
+  if (W.na > 0) MPF_TRY(block_pc_structure(Ka, W.nab, W.kabs.ptr, W.kabm.ptr, W.karblk.ptr, &W.kagrp, s));
   W.rowptr_seen = p->pat.rowptr.ptr;
   W.built = true;
   W.ulen_ready = false;
@@ -572,6 +744,60 @@ CsrView coarse_csr(const PmgWork& W) {
   C.vals = W.cval.ptr;
   C.vals32 = W.cval32.ptr;
   return C;
+}
+
+BlockPcView coarse_blocks(const PmgWork& W) {
+  BlockPcView C;
+  C.nblk = W.ncb;
+  C.bs = W.cbs.ptr;
+  C.bm = W.cbm.ptr;
+  C.rblk = W.crblk.ptr;
+  C.binv = W.cbinv.ptr;
+  C.grp = W.cgrp;
+  return C;
+}
+
+CsrView agg_csr(const PmgWork& W) {
+  CsrView K;
+  K.n = W.na;
+  K.nnz = W.nka;
+  K.rowptr = W.karow.ptr;
+  K.colidx = W.kacol.ptr;
+  K.vals = W.kaval.ptr;
+  K.vals32 = W.kaval32.ptr;
+  return K;
+}
+
+BlockPcView agg_blocks(const PmgWork& W) {
+  BlockPcView K;
+  K.nblk = W.nab;
+  K.bs = W.kabs.ptr;
+  K.bm = W.kabm.ptr;
+  K.rblk = W.karblk.ptr;
+  K.binv = W.kabinv.ptr;
+  K.grp = W.kagrp;
+  return K;
+}
+
+int lanes_coarse(const PmgWork& W) {
+  const double avg = W.nc > 0 ? (double)W.nnzc / (double)W.nc : 0.0;
+  return avg > 48 ? 32 : (avg > 12 ? 16 : 4);
+}
+
+// Chebyshev step coefficients (three-term recurrence of the corrections) on [lambda / ratio,
+// lambda] widened by 10 %: step k takes dd = c1[k] dd + c2[k] B^-1 (t - K x), x += dd.
+void cheb_coefficients(double lambda, double ratio, int degree, double* c1, double* c2) {
+  const double beta = 1.1 * lambda, alpha = beta / ratio;
+  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha), sigma = theta / delta;
+  double rho_old = 1.0 / sigma;
+  c1[0] = 0.0;
+  c2[0] = 1.0 / theta;
+  for (int k = 1; k < degree; ++k) {
+    const double rho = 1.0 / (2.0 * sigma - rho_old);
+    c1[k] = rho * rho_old;
+    c2[k] = 2.0 * rho / delta;
+    rho_old = rho;
+  }
 }
 
 int lanes_for(const mpf_problem* p) {
@@ -607,7 +833,16 @@ mpf_status pmg_setup(mpf_problem* p, const BlockPcView& F, int64_t nub) {
                        W.ulen.ptr);
     W.ulen_ready = true;
   }
-  MPF_TRY(estimate_lambda(p, W, F, nub, lanes_for(p), &W.lambda));
+  MPF_TRY(estimate_lambda(s, A, F, W.ulen.ptr, nub, W.d, lanes_for(p), W.xa.ptr, W.xb.ptr, W.cpart.ptr, &W.lambda));
+  if (W.levels == 3 && W.na > 0) {  // aggregate matrix, its blocks, and the vertex level's smoothing interval
+    hipLaunchKernelGGL(k_agg_values, dim3(grid_for(W.na, kT)), dim3(kT), 0, s, W.na, W.amptr.ptr, W.amem.ptr,
+                       W.crow.ptr, W.ccol.ptr, W.cval.ptr, W.aoff.ptr, W.karow.ptr, W.kacol.ptr, W.kaval.ptr,
+                       W.bad.ptr);
+    MPF_TRY(block_pc_values(agg_csr(W), W.nab, W.kabs.ptr, W.kabm.ptr, W.kabinv.ptr, s));
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(W.nka, kT)), dim3(kT), 0, s, W.nka, W.kaval.ptr, W.kaval32.ptr);
+    MPF_TRY(estimate_lambda(s, coarse_csr(W), coarse_blocks(W), W.culen.ptr, W.ncb, W.nc, lanes_coarse(W), W.cxa.ptr,
+                            W.cxb.ptr, W.cpart.ptr, &W.lambda_c));
+  }
   if (W.pn > 0) {
     const double scale = (p->m.Kinv > 0.0 && p->m.mu > 0.0) ? 1.0 + 1.0 / (p->m.mu * p->m.Kinv) : 0.0;
     hipLaunchKernelGGL(k_sinv, dim3(grid_for(W.pn, kT)), dim3(kT), 0, s, W.pn, (int32_t)W.d, A.rowptr, A.colidx,
@@ -633,19 +868,8 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
     hipLaunchKernelGGL(k_pres, dim3(grid_for(W.pn, kT)), dim3(kT), 0, s, W.pn, v + d, W.sinv.ptr, z + d);
     t = W.t.ptr;
   }
-  // Chebyshev coefficients (three-term recurrence of the iteration's corrections)
-  const double beta = 1.1 * W.lambda, alpha = beta / W.ratio;
-  const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha), sigma = theta / delta;
   double c1[8], c2[8];
-  double rho_old = 1.0 / sigma;
-  c1[0] = 0.0;
-  c2[0] = 1.0 / theta;
-  for (int k = 1; k < W.degree; ++k) {
-    const double rho = 1.0 / (2.0 * sigma - rho_old);
-    c1[k] = rho * rho_old;
-    c2[k] = 2.0 * rho / delta;
-    rho_old = rho;
-  }
+  cheb_coefficients(W.lambda, W.ratio, W.degree, c1, c2);
   // pre-smoothing from x = 0
   double* cur = W.xa.ptr;
   double* oth = W.xb.ptr;
@@ -659,16 +883,38 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   launch_sweep<2>(lpr, nub, F, A, W.fval32.ptr, W.ulen.ptr, 0.0, 0.0, t, cur, W.res.ptr, nullptr, s);
   hipLaunchKernelGGL(k_restrict, dim3(grid_for(W.nc, kT)), dim3(kT), 0, s, W.nc, W.rptr.ptr, W.rfine.ptr, W.rw.ptr,
                      W.res.ptr, W.rc.ptr);
-  BlockPcView C;
-  C.nblk = W.ncb;
-  C.bs = W.cbs.ptr;
-  C.bm = W.cbm.ptr;
-  C.rblk = W.crblk.ptr;
-  C.binv = W.cbinv.ptr;
-  C.grp = W.cgrp;
   const CgScratch cw{W.cr.ptr, W.cr2.ptr, W.cz.ptr, W.cp.ptr, W.cq.ptr, W.cpart.ptr, W.csc.ptr};
   mpf_krylov_report cr{};
-  MPF_TRY(pcg_csr(coarse_csr(W), C, cw, W.rc.ptr, W.xc.ptr, W.crtol, W.cmaxit, s, &cr));
+  if (W.levels == 3 && W.na > 0) {
+    // the vertex level's own V-cycle: Chebyshev smoothing from 0, the aggregates' correction (Ka
+    // by block-Jacobi CG to crtol), Chebyshev smoothing again, the result in xc
+    const BlockPcView Cb = coarse_blocks(W);
+    const CsrView Cv = coarse_csr(W);
+    const int lc = lanes_coarse(W);
+    const int64_t nc = W.nc, ncb = W.ncb;
+    double e1[8], e2[8];
+    cheb_coefficients(W.lambda_c, W.ratio, W.degree, e1, e2);
+    double* xc0 = W.cxa.ptr;
+    double* xc1 = W.cxb.ptr;
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(nc, kT)), dim3(kT), 0, s, nc, xc0);
+    for (int k = 0; k < W.degree; ++k) {
+      launch_sweep<1>(lc, ncb, Cb, Cv, W.cval32.ptr, W.culen.ptr, e1[k], e2[k], W.rc.ptr, xc0, xc1, W.cdd.ptr, s);
+      std::swap(xc0, xc1);
+    }
+    launch_sweep<2>(lc, ncb, Cb, Cv, W.cval32.ptr, W.culen.ptr, 0.0, 0.0, W.rc.ptr, xc0, W.cres.ptr, nullptr, s);
+    hipLaunchKernelGGL(k_agg_restrict, dim3(grid_for(W.na, kT)), dim3(kT), 0, s, W.na, W.amptr.ptr, W.amem.ptr,
+                       W.cres.ptr, W.ra.ptr);
+    MPF_TRY(pcg_csr(agg_csr(W), agg_blocks(W), cw, W.ra.ptr, W.xa3.ptr, W.crtol, W.cmaxit, s, &cr));
+    hipLaunchKernelGGL(k_agg_prolong, dim3(grid_for(nc, kT)), dim3(kT), 0, s, nc, W.aoff.ptr, W.xa3.ptr, xc0);
+    for (int k = 0; k < W.degree; ++k) {
+      double* out = (k == W.degree - 1) ? W.xc.ptr : xc1;
+      launch_sweep<1>(lc, ncb, Cb, Cv, W.cval32.ptr, W.culen.ptr, e1[k], e2[k], W.rc.ptr, xc0, out, W.cdd.ptr, s);
+      xc1 = xc0;
+      xc0 = out;
+    }
+  } else {
+    MPF_TRY(pcg_csr(coarse_csr(W), coarse_blocks(W), cw, W.rc.ptr, W.xc.ptr, W.crtol, W.cmaxit, s, &cr));
+  }
   W.coarse_its += cr.iterations;
   hipLaunchKernelGGL(k_prolong, dim3(grid_for(d, kT)), dim3(kT), 0, s, d, W.pidx.ptr, W.pw.ptr, W.xc.ptr, cur);
   // post-smoothing from the corrected x, the last step into z
@@ -714,5 +960,18 @@ extern "C" mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep) {
   rep->coarse_iterations = p->pmg->coarse_its;
   rep->setup_seconds = p->pmg->setup_seconds;
   rep->lambda_max = p->pmg->lambda;
+  const bool three = p->pmg->levels == 3 && p->pmg->na > 0;
+  rep->lambda_vertex = three ? p->pmg->lambda_c : 0.0;
+  rep->agg_dofs = p->pmg->na;
+  rep->agg_nnz = p->pmg->nka;
+  rep->levels = p->pmg->levels;
+  rep->reserved = 0;
+  return MPF_OK;
+}
+
+extern "C" mpf_status mpf_set_pmg_levels(mpf_problem* p, int32_t levels) {
+  if (!p || (levels != 2 && levels != 3)) return MPF_ERR_ARG;
+  if (!p->pmg) p->pmg = new PmgWork();
+  p->pmg->levels = levels;
   return MPF_OK;
 }

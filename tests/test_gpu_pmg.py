@@ -135,3 +135,32 @@ def test_newton_tet10p1_pmg_vs_oracle(mpf, orc, gpu):
     print("Tet10/P1 Newton with PMG: u", normwise_rel(ud, uo), "p", normwise_rel(pd, po), flush=True)
     assert normwise_rel(ud, uo) < 1e-9
     assert normwise_rel(pd, po) < 1e-9
+
+
+def test_pmg_three_levels_vs_two(mpf, gpu):
+    """The vertex level's own V-cycle (aggregates of 8 vertices below) against its CG solve: both
+    FGMRES solves reach the tolerance on the assembled system; the aggregate level has between one
+    and three dofs per aggregate of 8 vertex blocks."""
+    import math
+    P, X, conn = _system(mpf, mpf.TET10P1, 8)
+    A, rhs, pat = _csr(P)
+    its = {}
+    for lv in (2, 3):
+        P.set_pmg(levels=lv)
+        r = P.factorise_and_solve(ksp=mpf.KSP_GMRES, pc=mpf.PC_PMG, rtol=1e-10, maxit=2000, restart=30)
+        st = P.pmg_stats()
+        print(f"levels {lv}: {r.iterations} its, relres {r.relres:.2e}, lowest-level CG its {st.coarse_iterations},"
+              f" agg dofs {st.agg_dofs}, vertex lambda {st.lambda_vertex:.3f}", flush=True)
+        assert r.converged and st.levels == lv
+        du = P.solution()
+        assert np.linalg.norm(A @ du - rhs) / np.linalg.norm(rhs) < 1e-9
+        its[lv] = (r.iterations, st.coarse_iterations, st.agg_dofs, st.lambda_vertex)
+    nda = np.asarray(pat.NodeDofArray).reshape(-1, 3)
+    verts = np.unique(np.asarray(conn)[:, :4])
+    nvb = int(((nda[verts] >= 0).any(axis=1)).sum())
+    nagg = math.ceil(nvb / 8)
+    assert nagg <= its[3][2] <= 3 * nagg
+    assert 1.0 < its[3][3] < 3.0  # B^-1 Kc of the vertex level: about 1.9 (scipy power iteration)
+    assert its[2][3] == 0.0
+    with pytest.raises(mpf.MpfError):
+        P.set_pmg(levels=4)
