@@ -487,8 +487,9 @@ def main(argv=None) -> int:
             restart = 100
             r = P.factorise_and_solve(ksp=M.KSP_GMRES, pc=M.PC_PMG, rtol=1e-8, maxit=a.krylov_maxit, restart=restart)
             name = "gmres"
-            pc = ("flexible GMRES(100), two-level p-multigrid (Chebyshev(3) node-block smoothing, Galerkin vertex "
-                  "level by block-Jacobi CG to 0.1), pressure block diag(Kpp)(1 + K/mu)")
+            pc = ("flexible GMRES(100), three-level p-multigrid (Chebyshev(3) node-block smoothing; Galerkin vertex "
+                  "level with Chebyshev(3) vertex-block smoothing; aggregates of 8 vertices by block-Jacobi CG to "
+                  "0.1), pressure block diag(Kpp)(1 + K/mu)")
         else:
             r = P.factorise_and_solve(ksp=M.KSP_CG, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit)
             name = "pcg"
@@ -504,15 +505,23 @@ def main(argv=None) -> int:
             # 8 bytes per nonzero), the coarse CG's matrix (FP32 values and int32 columns, 8 bytes per
             # nonzero) and ~18 vector streams per iteration, and the Arnoldi step's CGS2 over on
             # average m/2 + 1 basis vectors (4 reads each) plus ~10 vector streams
+            # with three levels the vertex level takes 2 x 3 + 1 products of its FP32 matrix per V-cycle
+            # and the CG iterations are on the aggregate matrix
             fine = 12 * nnz + 6 * 8 * nnz
-            coarse = st.coarse_iterations / its * (8 * st.coarse_nnz + 144 * st.coarse_dofs)
+            if st.levels == 3:
+                coarse = (7 * 8 * st.coarse_nnz + 7 * 64 * st.coarse_dofs
+                          + st.coarse_iterations / its * (8 * st.agg_nnz + 144 * st.agg_dofs))
+            else:
+                coarse = st.coarse_iterations / its * (8 * st.coarse_nnz + 144 * st.coarse_dofs)
             ortho = (4 * (restart / 2 + 1) + 10) * 8 * (dispDOF + presDOF)
             bpi = fine + coarse + ortho
-            out[name].update({"vcycles": st.vcycles, "coarse_dofs": st.coarse_dofs, "coarse_nnz": st.coarse_nnz,
-                              "coarse_cg_iterations": st.coarse_iterations,
-                              "coarse_cg_per_vcycle": round(st.coarse_iterations / max(st.vcycles, 1), 1),
+            out[name].update({"levels": st.levels, "vcycles": st.vcycles, "coarse_dofs": st.coarse_dofs,
+                              "coarse_nnz": st.coarse_nnz, "agg_dofs": st.agg_dofs, "agg_nnz": st.agg_nnz,
+                              "lowest_cg_iterations": st.coarse_iterations,
+                              "lowest_cg_per_vcycle": round(st.coarse_iterations / max(st.vcycles, 1), 1),
                               "setup_seconds": round(st.setup_seconds, 4),
-                              "smoother_lambda_max": round(st.lambda_max, 4)})
+                              "smoother_lambda_max": round(st.lambda_max, 4),
+                              "vertex_lambda_max": round(st.lambda_vertex, 4)})
             out[name]["roofline"] = {"bound": "hbm", "bytes_per_iteration": int(bpi),
                                      "split": {"fine": int(fine), "coarse": int(coarse), "orthogonalisation": int(ortho)},
                                      "achieved": round(bpi / (ms_it * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,

@@ -554,6 +554,19 @@ static mpf_status spmv_view(const CsrView& A, const BlockPcView& B, const double
   return spmv_grp_t<double>(A, B, A.vals, x, y, part, np, sc, s);
 }
 
+// Workgroups (and so p.q partials) of spmv_view's product with partials: as many as the product
+// runs without them (its lanes per row decide), at most kMaxParts -- a small system's product
+// then spreads over the GPU instead of looping in a few workgroups.
+static unsigned spmv_parts(const CsrView& A, const BlockPcView& B) {
+  const double avg = A.n > 0 ? (double)A.nnz / (double)A.n : 0.0;
+  const int lpr = avg > 48 ? 32 : (avg > 12 ? 16 : 4);  // spmv_view's choice for either kernel
+  const int64_t units = B.grp ? B.nblk : A.n;
+  int64_t g = (units + kRB / lpr - 1) / (kRB / lpr);
+  if (g > kMaxParts) g = kMaxParts;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
 static CsrView csr_of(const mpf_problem* p) {
   CsrView A;
   A.n = (int64_t)p->pat.dispDOF + p->pat.presDOF;
@@ -726,11 +739,12 @@ mpf_status pcg_csr(const CsrView& A, const BlockPcView& B, const CgScratch& w, c
   int it = 0;
   int chunk = 8;
   double hs[S_N];
+  const unsigned nsp = spmv_parts(A, B);
   for (;;) {
     for (int c = 0; c < chunk; ++c, ++it) {
       const int par = it & 1;
-      MPF_TRY(spmv_view(A, B, w.p, w.q, w.part, nps, sc, s));
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, w.part, (int)nps, (int)nps, 1, sc + S_PQ);
+      MPF_TRY(spmv_view(A, B, w.p, w.q, w.part, nsp, sc, s));
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, w.part, (int)nsp, (int)nsp, 1, sc + S_PQ);
       double* rb[2] = {w.r, w.r2};
       hipLaunchKernelGGL(k_cg_update_rows, dim3(nps), dim3(kRB), 0, s, n, B.rblk, B.bs, B.bm, B.binv, x, rb[par],
                          rb[1 - par], w.z, w.p, w.q, w.part, (int)nps, sc, par);
