@@ -236,8 +236,10 @@ __global__ void __launch_bounds__(kT) k_ulen(int64_t nub, const int32_t* __restr
   ulen[k] = lo - b0;
 }
 
-// First Chebyshev step from x = 0 over the node blocks: t = v - Kup z_p on the block's rows (zp
-// non-null: the pressure columns, shared by the block's rows, read once), x = dd = c2 B^-1 t.
+// First Chebyshev step from x = 0 over the node blocks, LPR lanes per block: t = v - Kup z_p on the
+// block's rows (zp non-null: the pressure columns, shared by the block's rows, each read once),
+// x = dd = c2 B^-1 t.
+template <int LPR>
 __global__ void __launch_bounds__(kT) k_jac0(int64_t nub, const int32_t* __restrict__ bs,
                                              const int32_t* __restrict__ bm, const double* __restrict__ binv,
                                              const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
@@ -245,28 +247,42 @@ __global__ void __launch_bounds__(kT) k_jac0(int64_t nub, const int32_t* __restr
                                              int32_t d, const double* __restrict__ zp, double c2,
                                              const double* __restrict__ v, double* __restrict__ t,
                                              double* __restrict__ x, double* __restrict__ dd) {
-  const int64_t k = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (k >= nub) return;
-  const int s = bs[k], m = bm[k];
-  double r[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < m; ++i) r[i] = v[s + i];
-  if (zp) {
-    const int b0 = rowptr[s], len = rowptr[s + 1] - b0;
-    const int b1 = m > 1 ? rowptr[s + 1] : b0, b2 = m > 2 ? rowptr[s + 2] : b0;
-    for (int j = ulen[k]; j < len; ++j) {
-      const double zv = zp[colidx[b0 + j] - d];
-      r[0] -= vals[b0 + j] * zv;
-      if (m > 1) r[1] -= vals[b1 + j] * zv;
-      if (m > 2) r[2] -= vals[b2 + j] * zv;
+  const int g = threadIdx.x / LPR, l = threadIdx.x % LPR;
+  constexpr int GPB = kT / LPR;
+  for (int64_t k0 = (int64_t)blockIdx.x * GPB; k0 < nub; k0 += (int64_t)gridDim.x * GPB) {
+    const int64_t k = k0 + g;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int st = 0, m = 0;
+    if (k < nub) {
+      st = bs[k];
+      m = bm[k];
+      if (zp) {  // the block's pressure columns over the lanes
+        const int b0 = rowptr[st], len = rowptr[st + 1] - b0;
+        const int b1 = m > 1 ? rowptr[st + 1] : b0, b2 = m > 2 ? rowptr[st + 2] : b0;
+        for (int j = ulen[k] + l; j < len; j += LPR) {
+          const double zv = zp[colidx[b0 + j] - d];
+          s0 += vals[b0 + j] * zv;
+          if (m > 1) s1 += vals[b1 + j] * zv;
+          if (m > 2) s2 += vals[b2 + j] * zv;
+        }
+      }
     }
-    for (int i = 0; i < m; ++i) t[s + i] = r[i];
-  }
-  const double* B = binv + 9 * k;
-  for (int i = 0; i < m; ++i) {
-    double z = 0.0;
-    for (int j = 0; j < m; ++j) z += B[3 * i + j] * r[j];
-    x[s + i] = c2 * z;
-    dd[s + i] = c2 * z;
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPR);
+      s1 += __shfl_xor(s1, off, LPR);
+      s2 += __shfl_xor(s2, off, LPR);
+    }
+    if (k < nub && l < m) {  // lane l finishes row l of the block
+      double r[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < m; ++i) r[i] = v[st + i] - (i == 0 ? s0 : (i == 1 ? s1 : s2));
+      if (zp) t[st + l] = r[l];
+      const double* B = binv + 9 * k + 3 * l;
+      double z = 0.0;
+      for (int j = 0; j < m; ++j) z += B[j] * r[j];
+      x[st + l] = c2 * z;
+      dd[st + l] = c2 * z;
+    }
   }
 }
 
@@ -873,8 +889,8 @@ mpf_status pmg_apply(mpf_problem* p, const BlockPcView& F, int64_t nub, const do
   // pre-smoothing from x = 0
   double* cur = W.xa.ptr;
   double* oth = W.xb.ptr;
-  hipLaunchKernelGGL(k_jac0, dim3(grid_for(nub, kT)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, A.rowptr, A.colidx,
-                     A.vals, W.ulen.ptr, d, W.pn > 0 ? z + d : nullptr, c2[0], v, W.t.ptr, cur, W.dd.ptr);
+  hipLaunchKernelGGL(k_jac0<16>, dim3(grid_for(nub, kT / 16)), dim3(kT), 0, s, nub, F.bs, F.bm, F.binv, A.rowptr,
+                     A.colidx, A.vals, W.ulen.ptr, d, W.pn > 0 ? z + d : nullptr, c2[0], v, W.t.ptr, cur, W.dd.ptr);
   for (int k = 1; k < W.degree; ++k) {
     launch_sweep<1>(lpr, nub, F, A, W.fval32.ptr, W.ulen.ptr, c1[k], c2[k], t, cur, oth, W.dd.ptr, s);
     std::swap(cur, oth);
