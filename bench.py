@@ -487,9 +487,9 @@ def main(argv=None) -> int:
             restart = 100
             r = P.factorise_and_solve(ksp=M.KSP_GMRES, pc=M.PC_PMG, rtol=1e-8, maxit=a.krylov_maxit, restart=restart)
             name = "gmres"
-            pc = ("flexible GMRES(100), three-level p-multigrid (Chebyshev(3) node-block smoothing; Galerkin vertex "
-                  "level with Chebyshev(3) vertex-block smoothing; aggregates of 8 vertices by block-Jacobi CG to "
-                  "0.1), pressure block diag(Kpp)(1 + K/mu)")
+            pc = ("flexible GMRES(100), three-level p-multigrid (degree-1 Chebyshev node-block smoothing; Galerkin "
+                  "vertex level with degree-1 Chebyshev vertex-block smoothing; aggregates of 8 vertices by "
+                  "block-Jacobi CG to 0.1), pressure block diag(Kpp)(1 + K/mu)")
         else:
             r = P.factorise_and_solve(ksp=M.KSP_CG, pc=M.PC_BJACOBI3, rtol=1e-8, maxit=a.krylov_maxit)
             name = "pcg"
@@ -506,16 +506,16 @@ def main(argv=None) -> int:
             out[name].update({"seconds_repeat": round(r2.seconds, 4), "setup_seconds_repeat": round(st2.setup_seconds, 4),
                               "iterations_repeat": r2.iterations})
             its = max(r.iterations, 1)
-            # per outer iteration: A z (12 bytes per nonzero), the V-cycle's 2 x 3 (Chebyshev degree)
-            # displacement-row sweeps over the FP32 copy of the values (counted as full matrix reads,
-            # 8 bytes per nonzero), the coarse CG's matrix (FP32 values and int32 columns, 8 bytes per
-            # nonzero) and ~18 vector streams per iteration, and the Arnoldi step's CGS2 over on
-            # average m/2 + 1 basis vectors (4 reads each) plus ~10 vector streams
-            # with three levels the vertex level takes 2 x 3 + 1 products of its FP32 matrix per V-cycle
-            # and the CG iterations are on the aggregate matrix
-            fine = 12 * nnz + 6 * 8 * nnz
+            # per outer iteration: A z (12 bytes per nonzero), the V-cycle's 2 x degree displacement-
+            # row sweeps over the FP32 copy of the values (counted as full matrix reads, 8 bytes per
+            # nonzero), the lowest CG's matrix (FP32 values and int32 columns, 8 bytes per nonzero)
+            # and ~18 vector streams per iteration, and the Arnoldi step's CGS2 over on average
+            # m/2 + 1 basis vectors (4 reads each) plus ~10 vector streams; with three levels the
+            # vertex level takes 2 x degree + 1 products of its FP32 matrix per V-cycle
+            deg = st.degree
+            fine = 12 * nnz + 2 * deg * 8 * nnz
             if st.levels == 3:
-                coarse = (7 * 8 * st.coarse_nnz + 7 * 64 * st.coarse_dofs
+                coarse = ((2 * deg + 1) * (8 * st.coarse_nnz + 64 * st.coarse_dofs)
                           + st.coarse_iterations / its * (8 * st.agg_nnz + 144 * st.agg_dofs))
             else:
                 coarse = st.coarse_iterations / its * (8 * st.coarse_nnz + 144 * st.coarse_dofs)

@@ -204,19 +204,19 @@ mpf_status mpf_rhs_norm(mpf_problem* p, double* norm);
 /* pc, restart, maxit, rtol are ignored;                                                          */
 /* a zero or non-finite pivot gives MPF_ERR_DIVERGED.  The half-bandwidth after renumbering must   */
 /* stay below about 490 dofs (MPF_ERR_ARG).                                                      */
-/* MPF_PC_PMG (MPF_KSP_GMRES only; MPF_TET10 and MPF_TET10P1 problems, else MPF_ERR_ARG): flexible */
-/* GMRES(m) around a two-level p-multigrid preconditioner.  Displacement rows: one V-cycle with     */
-/* Chebyshev smoothing (3x3 node-block Jacobi inside) on the quadratic dofs and a coarse correction */
-/* on the vertex dofs, whose matrix is the Galerkin product P^T Kuu P (P: the Tet10 -> Tet4         */
-/* interpolation, a mid-edge node taking the mean of its two vertices) solved by block-Jacobi CG to */
-/* a loose tolerance; the smoothing and the coarse CG read their matrices rounded to FP32 (sums in  */
-/* FP64; the outer residual uses the FP64 system).  Pressure rows (MPF_TET10P1): block upper-       */
-/* triangular, the pressure block being diag(Kpp) (1 + 1 / (mu Kinv)) (the pressure mass matrix     */
-/* scaled to the Schur complement of a nearly incompressible material; Kinv = 0: the node-block     */
-/* Schur diagonal).  It stands where petsc_options select a multigrid or field-split preconditioner */
-/* (-pc_type mg / gamg / fieldsplit) for the reference's KSPSolve (SolverPetsc.cpp:117-140, 290);  */
-/* the code defaults there are CG and ILU.  mpf_set_pmg changes its parameters, mpf_pmg_stats      */
-/* reports its last use.                                                                            */
+/* MPF_PC_PMG (MPF_KSP_GMRES only; MPF_TET10 and MPF_TET10P1 problems, else MPF_ERR_ARG): flexible   */
+/* GMRES(m) around a p-multigrid preconditioner.  Displacement rows: one V-cycle with Chebyshev      */
+/* smoothing (3x3 node-block Jacobi inside; degree 1 by default, one scaled step) on the quadratic   */
+/* dofs and a coarse correction on the vertex dofs, whose matrix is the Galerkin product P^T Kuu P   */
+/* (P: the Tet10 -> Tet4 interpolation, a mid-edge node taking the mean of its two vertices), given  */
+/* a V-cycle of its own over aggregates of 8 vertices (mpf_set_pmg_levels) or solved by block-Jacobi */
+/* CG; the smoothing and the lowest CG read their matrices rounded to FP32 (sums in FP64; the outer  */
+/* residual uses the FP64 system).  Pressure rows (MPF_TET10P1): block upper-triangular, the         */
+/* pressure block being diag(Kpp) (1 + 1 / (mu Kinv)) (the pressure mass matrix scaled to the Schur  */
+/* complement of a nearly incompressible material; Kinv = 0: the node-block Schur diagonal).  It     */
+/* stands where petsc_options select a multigrid or field-split preconditioner (-pc_type mg / gamg / */
+/* fieldsplit) for the reference's KSPSolve (SolverPetsc.cpp:117-140, 290); the code defaults there  */
+/* are CG and ILU.  mpf_set_pmg changes its parameters, mpf_pmg_stats reports its last use.          */
 enum { MPF_KSP_CG = 0, MPF_KSP_GMRES = 1, MPF_KSP_BAND = 2 };
 enum { MPF_PC_NONE = 0, MPF_PC_JACOBI = 1, MPF_PC_BJACOBI3 = 2, MPF_PC_PMG = 3 };
 typedef struct {
@@ -239,7 +239,7 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
 /* p-multigrid preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before */
-/* and after the coarse correction (1..8, default 3), the ratio beta / alpha of its interval        */
+/* and after the coarse correction (1..8, default 1), the ratio beta / alpha of its interval        */
 /* (1 < ratio < 1e6, default 10; beta = 1.1 x the power-iteration estimate of the largest           */
 /* eigenvalue of the node-block-Jacobi-preconditioned Kuu), the coarse CG's relative tolerance      */
 /* (0 < coarse_rtol < 1, default 0.1) and iteration limit (default 1000).  No counterpart in the    */
@@ -257,7 +257,8 @@ mpf_status mpf_set_pmg_levels(mpf_problem* p, int32_t levels);
 /* the aggregates with 3), the set-up time of the last solve (transfer tables on first use; the    */
 /* Galerkin products, the block inverses and the eigenvalue estimates every solve), the smoother's */
 /* estimate on the fine level and on the vertex level (3 levels), the aggregate level's dofs and    */
-/* nonzeros, and the number of levels.  No counterpart in the reference (reporting only).          */
+/* nonzeros, the number of levels and the smoothing degree.  No counterpart in the reference        */
+/* (reporting only).                                                                               */
 typedef struct {
   int64_t coarse_dofs;
   int64_t coarse_nnz;
@@ -269,7 +270,7 @@ typedef struct {
   int64_t agg_dofs;
   int64_t agg_nnz;
   int32_t levels;
-  int32_t reserved;
+  int32_t degree;       /* the Chebyshev degree of the smoothing */
 } mpf_pmg_report;
 mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep);
 mpf_status mpf_get_solution(const mpf_problem* p, double* du);

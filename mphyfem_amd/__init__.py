@@ -60,7 +60,7 @@ class PmgReport(C.Structure):
     _fields_ = [("coarse_dofs", C.c_int64), ("coarse_nnz", C.c_int64), ("vcycles", C.c_int64),
                 ("coarse_iterations", C.c_int64), ("setup_seconds", C.c_double), ("lambda_max", C.c_double),
                 ("lambda_vertex", C.c_double), ("agg_dofs", C.c_int64), ("agg_nnz", C.c_int64),
-                ("levels", C.c_int32), ("reserved", C.c_int32)]
+                ("levels", C.c_int32), ("degree", C.c_int32)]
 
 
 class KrylovReport(C.Structure):
@@ -704,7 +704,7 @@ class Problem:
         _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
         return nb.value, bm.value, nd.value, fl.value
 
-    def set_pmg(self, degree=3, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000, levels=3):
+    def set_pmg(self, degree=1, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000, levels=3):
         """Parameters of the p-multigrid preconditioner PC_PMG (Chebyshev smoothing degree and interval
         ratio, the lowest level's CG relative tolerance and iteration limit, and the number of levels:
         2, the vertex level solved by CG; 3, a V-cycle on the vertex level with vertex aggregates below)."""

@@ -44,7 +44,7 @@ namespace mpf {
 
 struct PmgWork {
   // parameters
-  int degree = 3;          // Chebyshev degree before and after the coarse correction
+  int degree = 1;          // Chebyshev degree before and after the coarse correction
   double ratio = 10.0;     // beta / alpha of the smoothing interval
   double crtol = 0.1;
   int cmaxit = 1000;
@@ -1008,7 +1008,7 @@ extern "C" mpf_status mpf_pmg_stats(const mpf_problem* p, mpf_pmg_report* rep) {
   rep->agg_dofs = p->pmg->na;
   rep->agg_nnz = p->pmg->nka;
   rep->levels = p->pmg->levels;
-  rep->reserved = 0;
+  rep->degree = p->pmg->degree;
   return MPF_OK;
 }
 
