@@ -184,7 +184,7 @@ __global__ void k_check_groups(int64_t nblk, const int32_t* __restrict__ bs, con
 __global__ void __launch_bounds__(kRB) k_finish(const double* __restrict__ part, int np, int stride, int nout,
                                                 double* __restrict__ out) {
   __shared__ double sh[kRB / 64];
-  for (int j = 0; j < nout; ++j) {
+  for (int j = blockIdx.x; j < nout; j += gridDim.x) {  // one workgroup per output
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += kRB) s += part[(size_t)j * stride + i];
     const double t = block_sum(s, sh);
@@ -828,7 +828,7 @@ static mpf_status gmres(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_rep
       // CGS2
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_multidot, dim3(nps), dim3(kRB), 0, s, n, j + 1, V, W.w.ptr, W.part.ptr, (int)nps);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRB), 0, s, W.part.ptr, (int)nps, (int)nps, j + 1, d_h);
+        hipLaunchKernelGGL(k_finish, dim3(j + 1), dim3(kRB), 0, s, W.part.ptr, (int)nps, (int)nps, j + 1, d_h);
         hipLaunchKernelGGL(k_multiaxpy, dim3(nps), dim3(kRB), 0, s, n, j + 1, V, d_h, W.w.ptr);
         MPF_HIP(hipMemcpyAsync(pass == 0 ? h.data() : h2.data(), d_h, (j + 1) * sizeof(double),
                                hipMemcpyDeviceToHost, s));
