@@ -245,7 +245,9 @@ def bench_direct(M, a) -> int:
     for k in range(warm):  # band layout, graph capture
         P.batch_newton_step((k + 1) / nsteps, **opts)
     nblk, bmax, ndof, flops = P.band_stats()
-    log(f"direct: {nblk} beams, {ndof} free dofs, half-bandwidth <= {bmax}, {flops / 1e9:.1f} GFLOP per factorisation")
+    nseg, ncut = P.band_segments()
+    log(f"direct: {nblk} beams ({ncut} cut in two, {nseg} segments), {ndof} free dofs, half-bandwidth <= {bmax}, "
+        f"{flops / 1e9:.1f} GFLOP per factorisation")
     rep = M.BatchReport()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -261,11 +263,12 @@ def bench_direct(M, a) -> int:
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "the learned workload's 200 held-out Hex8 cantilevers (H=8, W=4, L=round(U[4,12]*8)), "
                                   "plain start, load steps of the 20-step tip-shear ramp",
-                      "beams": nblk, "free_dofs": ndof, "max_half_bandwidth": bmax},
+                      "beams": nblk, "free_dofs": ndof, "max_half_bandwidth": bmax, "band_segments": nseg,
+                      "beams_cut_at_a_separator": ncut},
            "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": 78.6, "unit": "TFLOP/s",
                         "frac": round(achieved / 78.6, 4), "traffic": None,
-                        "kernel": "k_band_fill + k_band_factor + k_band_solve + k_band_scatter "
-                                  "(one direct solve of every beam; FP64 MFMA peak)",
+                        "kernel": "k_band_fill + k_band_factor + k_band_schur + k_band_solve + k_band_sep + "
+                                  "k_band_scatter (one direct solve of every beam; FP64 MFMA peak)",
                         "flops_per_solve": flops, "ms_per_solve": round(s_solve * 1e3, 3)},
            "newton": {"batch_iterations": rep.newton_iters, "graph_launches": rep.graph_launches,
                       "seconds_iteration_graphs": round(rep.seconds_assembly, 4)}}

@@ -238,6 +238,15 @@ mpf_status mpf_solve(mpf_problem* p, const mpf_krylov_opts* o, mpf_krylov_report
 /* independent blocks, largest half-bandwidth, free dofs, and the factorisation's algorithmic      */
 /* flops, sum over blocks of n b (b + 1).  No counterpart in the reference (reporting only).       */
 mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_t* bmax, int64_t* ndof, double* flops);
+/* Banded direct solver: cut each block of n >= 3 b + 64 rows in two at a separator of b .. b + 15  */
+/* rows in its middle (enable = 1, the default) or keep every block whole (0).  The two halves are  */
+/* factored and solved by workgroups of their own up to the separator, whose Schur complement then  */
+/* takes both halves' updates (elimination order [first half, second half reversed, separator]); a */
+/* long block's chain of panels is about halved.  No counterpart in the reference (a GPU layout      */
+/* choice; Eigen's SimplicialLDLT orders the whole system by AMD, SolverEigen.cpp:191-228).         */
+mpf_status mpf_set_band_split(mpf_problem* p, int32_t enable);
+/* Segments of the direct solver's layout (workgroups per launch) and blocks cut in two.            */
+mpf_status mpf_band_segments(const mpf_problem* p, int32_t* nseg, int32_t* ncut);
 /* p-multigrid preconditioner (MPF_PC_PMG) parameters: the degree of the Chebyshev smoothing before */
 /* and after the coarse correction (1..8, default 1), the ratio beta / alpha of its interval        */
 /* (1 < ratio < 1e6, default 10; beta = 1.1 x the power-iteration estimate of the largest           */

@@ -198,6 +198,8 @@ SIGNATURES = [
     ("mpf_get_assembly_mode", C.c_int, [_P, _I32P]),
     ("mpf_set_chunk_stage_cap", C.c_int, [_P, C.c_int32]),
     ("mpf_band_stats", C.c_int, [_P, _I32P, _I32P, C.POINTER(C.c_int64), _DP]),
+    ("mpf_set_band_split", C.c_int, [_P, C.c_int32]),
+    ("mpf_band_segments", C.c_int, [_P, _I32P, _I32P]),
     ("mpf_set_pmg", C.c_int, [_P, C.c_int32, C.c_double, C.c_double, C.c_int32]),
     ("mpf_set_pmg_levels", C.c_int, [_P, C.c_int32]),
     ("mpf_batch_assembly_choice", C.c_int, [_P, _I32P, _I32P, _DP]),
@@ -703,6 +705,16 @@ class Problem:
         nb, bm, nd, fl = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
         _check(lib().mpf_band_stats(self._h, C.byref(nb), C.byref(bm), C.byref(nd), C.byref(fl)), "mpf_band_stats")
         return nb.value, bm.value, nd.value, fl.value
+
+    def set_band_split(self, enable=True):
+        """Banded direct solver: cut long blocks in two at a separator (default) or keep them whole."""
+        _check(lib().mpf_set_band_split(self._h, 1 if enable else 0), "mpf_set_band_split")
+
+    def band_segments(self):
+        """(segments, blocks cut in two) of the banded direct solver's layout."""
+        ns, nc = C.c_int32(), C.c_int32()
+        _check(lib().mpf_band_segments(self._h, C.byref(ns), C.byref(nc)), "mpf_band_segments")
+        return ns.value, nc.value
 
     def set_pmg(self, degree=1, ratio=10.0, coarse_rtol=0.1, coarse_maxit=1000, levels=3):
         """Parameters of the p-multigrid preconditioner PC_PMG (Chebyshev smoothing degree and interval

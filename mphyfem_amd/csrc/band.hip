@@ -6,7 +6,16 @@
 // renumbered node by node (reverse Cuthill-McKee or slices across an axis, whichever gives the
 // smaller half-bandwidth b), which makes the block banded; the block is then factored as L D L^T
 // without pivoting (K is the symmetric tangent of the consistent mode; a zero or non-finite pivot is
-// reported) and solved, one workgroup per block and kernel:
+// reported) and solved, one workgroup per segment and kernel.
+//
+// Segments: a block long enough (n >= 3 b + 64 rows) is cut in two at a separator S of s rows
+// (b <= s < b + 16) in its middle, and eliminated in the order [A1, A2 reversed, S]: segment 0
+// holds A1 followed by S (band positions in order), segment 1 holds A2 from the block's far end
+// followed by S (positions reversed, the separator's own entries left to segment 0).  A row of A1
+// and a row of A2 are more than b apart, so the two segments factor independently up to S; S then
+// takes the sum of both partial Schur complements and is factored at the end of segment 0.  The
+// serial chain of panels of a long block is halved, and the segments (longest first) fill more of
+// the GPU's compute units than one workgroup per block.  Shorter blocks are one segment.
 //
 //   k_band_factor: right-looking in panels of kW columns -- the panel factored column by column
 //   (bands up to 176 wide: by one wavefront in registers, three rows per lane; wider: one thread
@@ -38,49 +47,69 @@ constexpr int kFBS = 768;   // threads per workgroup of the factorisation: 12 wa
 constexpr int kNH = MPF_BAND_TILES;  // trailing-update tiles per wavefront and round (old values loaded first)
 static_assert(kBS / 64 == kW, "one wavefront per panel row in the backward solve");
 
-struct BandArgs {
-  const int32_t* bstart;  // first position of each block [nblk + 1]
-  const int32_t* bw;      // half-bandwidth of each block
-  const int64_t* boff;    // band storage offset of each block
-  double* band;           // per block: column-major lower band, entry (r, c) at c * (b + 1) + (r - c)
-  double* y;              // permuted right-hand side, then the solution
-  int32_t* bad;           // set on a zero or non-finite pivot
+// One segment of a block (see the file comment).  Local row k of the segment is band position
+// base + dir * k; its band is column-major lower, entry (r, c) at boff + c * (b + 1) + (r - c), and
+// its right-hand side / solution y[yoff + k].
+struct BandSeg {
+  int64_t boff;     // band storage offset
+  int32_t yoff;     // first entry in y (also the segment's first fill column)
+  int32_t base;     // band position of local row 0
+  int32_t dir;      // +1 or -1
+  int32_t n;        // rows
+  int32_t nf;       // columns factored before the separator's Schur complement is complete
+  int32_t nend;     // columns factored in all (nf, or n for the segment that ends with the separator)
+  int32_t zc;       // rows k < zc are this segment's own (system values, solution out); rows from zc
+                    // (the separator's copy in segment 1) start at zero
+  int32_t blk;      // block
+  int32_t b;        // half-bandwidth of the block
+  int32_t partner;  // the block's other segment, -1 for a block in one segment
 };
 
-// Band of every active block from the CSR rows, zeros included, and the permuted right-hand side,
-// in one pass: one wavefront per band column j (4 per workgroup; blocks whose active[blk * astride]
-// is 0 keep their factors).  Column j holds L(i, j), i = j .. j + b, which K's symmetry puts in
-// CSR row perm[j]: the row's entries whose permuted column falls in the band are placed in an LDS
-// copy of the column, zeros elsewhere, and the column goes out with consecutive lanes.  Also
-// clears the pivot flag.  (Measured on the 200 cantilevers: 2.1 ms as a zero pass of the band
-// followed by a scatter of each CSR row into it.)
+struct BandArgs {
+  const BandSeg* seg;
+  double* band;
+  double* y;     // permuted right-hand side, then the solution, per segment
+  int32_t* bad;  // set on a zero or non-finite pivot
+};
+
+// Band of every active segment from the CSR rows, zeros included, and its right-hand side, in one
+// pass: one wavefront per band column (4 per workgroup; segments of blocks whose active[blk *
+// astride] is 0 keep their factors).  Column k of a segment holds L(i, k), i = k .. k + b, which
+// K's symmetry puts in CSR row perm[base + dir k]: the row's entries whose local row falls in the
+// band (and in the segment) are placed in an LDS copy of the column, zeros elsewhere, and the
+// column goes out with consecutive lanes.  Columns from zc are zero.  Also clears the pivot flag.
+// (Measured on the 200 cantilevers: 2.1 ms as a zero pass of the band followed by a scatter of
+// each CSR row into it, against 1.6 ms in one pass.)
 constexpr int kBandColMax = 496;  // b + 1 of the widest band the panel LDS allows
-__global__ void __launch_bounds__(256) k_band_fill(int32_t n, const int32_t* __restrict__ rowptr,
+__global__ void __launch_bounds__(256) k_band_fill(int32_t ncols, const int32_t* __restrict__ cseg,
+                                                   const int32_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ colidx,
                                                    const double* __restrict__ vals, const double* __restrict__ rhs,
                                                    const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
-                                                   const int32_t* __restrict__ pblk, BandArgs A,
-                                                   const int32_t* __restrict__ active, int32_t astride) {
+                                                   BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
   __shared__ double sCol[4][kBandColMax];
   if (blockIdx.x == 0 && threadIdx.x == 0) *A.bad = 0;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int j = blockIdx.x * 4 + w;
-  if (j >= n) return;  // the wavefronts of a workgroup share no data: no workgroup barrier below
-  const int blk = pblk[j];
-  if (active && !active[(size_t)blk * astride]) return;
-  const int s0 = A.bstart[blk], b = A.bw[blk], B1 = b + 1;
+  const int J = blockIdx.x * 4 + w;
+  if (J >= ncols) return;  // the wavefronts of a workgroup share no data: no workgroup barrier below
+  const BandSeg S = A.seg[cseg[J]];
+  if (active && !active[(size_t)S.blk * astride]) return;
+  const int k = J - S.yoff, b = S.b, B1 = b + 1;
   double* col = sCol[w];
-  for (int k = lane; k < B1; k += 64) col[k] = 0.0;
+  for (int t = lane; t < B1; t += 64) col[t] = 0.0;
   __builtin_amdgcn_wave_barrier();
-  const int r = perm[j];
-  for (int e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64) {
-    const int i = iperm[colidx[e]];
-    if (i >= j && i - j <= b) col[i - j] = vals[e];
+  int r = -1;
+  if (k < S.zc) {  // uniform
+    r = perm[S.base + S.dir * k];
+    for (int e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64) {
+      const int i = S.dir * (iperm[colidx[e]] - S.base);
+      if (i >= k && i - k <= b && i < S.n) col[i - k] = vals[e];
+    }
   }
   __builtin_amdgcn_wave_barrier();
-  double* L = A.band + A.boff[blk] + (size_t)(j - s0) * B1;
-  for (int k = lane; k < B1; k += 64) L[k] = col[k];
-  if (lane == 0) A.y[j] = rhs[r];
+  double* L = A.band + S.boff + (size_t)k * B1;
+  for (int t = lane; t < B1; t += 64) L[t] = col[t];
+  if (lane == 0) A.y[J] = r >= 0 ? rhs[r] : 0.0;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -223,12 +252,14 @@ __device__ __forceinline__ void trailing_band(const double* P, const double* sd,
   }
 }
 
-// Factor one block per workgroup (blocks whose active[blk * astride] is 0 are skipped; active may
-// be null).  Dynamic LDS: bands up to 176 wide, two panel buffers of (kW + bmax) x kW doubles
+// Factor one segment per workgroup, columns [0, nf) (phase 0) or [nf, nend) (phase 1: the
+// separator, after k_band_schur); segments of blocks whose active[blk * astride] is 0 are skipped
+// (active may be null).  Stopping at nf leaves the trailing window below the last panel -- the
+// separator's rows -- updated in the band.  Dynamic LDS: bands up to 176 wide, two panel buffers of (kW + bmax) x kW doubles
 // (column-major, row stride PR = kW + bmax) each followed by its kW pivots; wider bands, one panel
 // buffer, its pivots and 2 kW doubles of published column entries.
 //
-// Panel step at column j0 (w = min(kW, n - j0) columns, panel-relative rows 0 .. w - 1 + b):
+// Panel step at column j0 (w = min(kW, c1 - j0) columns, panel-relative rows 0 .. w - 1 + b):
 //   (1) the panel is factored right-looking, one column at a time (panel_factor_wave); wider
 //       panels give each row a thread and publish the diagonal-block rows' column c in LDS before
 //       each step (one barrier per column);
@@ -241,15 +272,18 @@ __device__ __forceinline__ void trailing_band(const double* P, const double* sd,
 // the longest of 200 cantilevers (12 960 dofs, b = 167, 810 panels) before the look-ahead: 17.3 ms,
 // of which 7.3 ms in (1), 7.3 ms in (2) and 2.7 ms loading and storing panels (timing-only builds
 // that skip (1), (2) or both).
-__global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, const int32_t* __restrict__ active,
-                                                     int32_t astride, int32_t* __restrict__ factored) {
+__global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, int32_t phase,
+                                                     const int32_t* __restrict__ active, int32_t astride,
+                                                     int32_t* __restrict__ factored) {
   extern __shared__ double sP[];
-  const int blk = blockIdx.x;
-  if (active && !active[(size_t)blk * astride]) return;
+  const BandSeg S = A.seg[blockIdx.x];
+  if (active && !active[(size_t)S.blk * astride]) return;
+  const int c0 = phase ? S.nf : 0, c1 = phase ? S.nend : S.nf;
+  if (phase && c0 >= c1) return;
   const int PR = kW + bmax;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-  const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
-  double* L = A.band + A.boff[blk];
+  const int n = S.n, b = S.b;
+  double* L = A.band + S.boff;
   const int B1 = b + 1;
   const int T = (b + 15) >> 4;  // trailing tiles per side
   constexpr int NW = kFBS / 64;
@@ -257,19 +291,19 @@ __global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, 
   if (kW + bmax <= 3 * 64) {  // the same for every block of the launch
     double* const buf0 = sP;
     double* const buf1 = sP + (size_t)PR * kW + kW;
-    if (n > 0) {
-      panel_load(buf0, L, 0, min(kW, n), n, B1, PR, tid, kFBS);
+    if (c1 > c0) {
+      panel_load(buf0, L, c0, min(kW, c1 - c0), n, B1, PR, tid, kFBS);
       __syncthreads();
-      if (wid == 0) panel_factor_wave(buf0, buf0 + (size_t)PR * kW, 0, min(kW, n), b, n, PR, lane, A.bad);
+      if (wid == 0) panel_factor_wave(buf0, buf0 + (size_t)PR * kW, c0, min(kW, c1 - c0), b, n, PR, lane, A.bad);
       __syncthreads();
     }
-    for (int j0 = 0, q = 0; j0 < n; j0 += kW, q ^= 1) {
-      const int w = min(kW, n - j0);
+    for (int j0 = c0, q = 0; j0 < c1; j0 += kW, q ^= 1) {
+      const int w = min(kW, c1 - j0);
       double* cur = q ? buf1 : buf0;
       double* nxt = q ? buf0 : buf1;
       const double* sdc = cur + (size_t)PR * kW;
       const int j1 = j0 + kW;
-      const int w1 = j1 < n ? min(kW, n - j1) : 0;
+      const int w1 = j1 < c1 ? min(kW, c1 - j1) : 0;
       if (w1 > 0) {  // uniform
         panel_load(nxt, L, j1, w1, n, B1, PR, tid, kFBS);
         __syncthreads();
@@ -290,15 +324,16 @@ __global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, 
         if (w1 > 0) panel_factor_wave(nxt, nxt + (size_t)PR * kW, j1, w1, b, n, PR, lane, A.bad);
       } else {
         panel_store(cur, L, j0, w, n, B1, PR, tid - 64, kFBS - 64);
-        trailing_band<kNH>(cur, sdc, L, j0, w, b, n, B1, PR, T, 1, wid - 1, NW - 1, lane);
+        // the last panel has no next panel in LDS: its window's first tile column goes to the band
+        trailing_band<kNH>(cur, sdc, L, j0, w, b, n, B1, PR, T, w1 > 0 ? 1 : 0, wid - 1, NW - 1, lane);
       }
       __syncthreads();
     }
   } else {
     double* sd = sP + (size_t)PR * kW;  // kW pivots
     double* sC = sd + kW;               // 2 x kW: published column entries of the diagonal-block rows
-    for (int j0 = 0; j0 < n; j0 += kW) {
-      const int w = min(kW, n - j0);
+    for (int j0 = c0; j0 < c1; j0 += kW) {
+      const int w = min(kW, c1 - j0);
       panel_load(sP, L, j0, w, n, B1, PR, tid, kFBS);
       __syncthreads();
       {  // (1) thread r holds panel row r (r < w + b, the diagonal block's rows included); before
@@ -340,26 +375,21 @@ __global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, 
       __syncthreads();
     }
   }
-  if (tid == 0) factored[blk] = 1;  // the band now holds this block's factors
+  if (tid == 0 && phase == 0) factored[S.blk] = 1;  // the band holds this block's factors once the launches end
 }
 
-// Solve L D L^T x = y in place for one block per workgroup (skipped like the factorisation), in
-// blocks of kW rows: the kW x kW unit triangle by wavefront 0 with the block's L in registers and
+// Solve L D L^T x = y in place for one segment per workgroup (skipped like the factorisation), in
+// blocks of kW rows: the kW x kW unit triangle by wavefront 0 with the segment's L in registers and
 // the solved values read across lanes; the coupling to the next b rows by the workgroup (forward)
-// or per row of the block by one wavefront with a butterfly sum (backward).
-__global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
-  __shared__ double sy[kW];
-  const int blk = blockIdx.x;
-  if (active && !active[(size_t)blk * astride]) return;
-  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
-  const int s0 = A.bstart[blk], n = A.bstart[blk + 1] - s0, b = A.bw[blk];
-  const double* L = A.band + A.boff[blk];
-  double* y = A.y + s0;
-  const int B1 = b + 1;
+// or per row of the block by one wavefront with a butterfly sum (backward).  Over columns
+// [c0, c1): forward L z = y, then z / d; backward L^T x = z, rows past c1 already solved.
 
-  // ---- forward: L z = y (unit lower) ----
-  for (int j0 = 0; j0 < n; j0 += kW) {
-    const int w = min(kW, n - j0);
+// forward substitution over columns [c0, c1) and the division by the pivots
+__device__ __forceinline__ void band_forward(const double* L, double* y, double* sy, int n, int b, int c0, int c1,
+                                             int tid, int wid, int lane) {
+  const int B1 = b + 1;
+  for (int j0 = c0; j0 < c1; j0 += kW) {
+    const int w = min(kW, c1 - j0);
     // the coupling rows' band entries and right-hand sides do not depend on this block's z: load
     // them before the block is solved (b < kBS, so one row per thread)
     const int r = j0 + w + tid;
@@ -396,13 +426,16 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
     }
     __syncthreads();
   }
-  // ---- D ----
-  for (int i = tid; i < n; i += kBS) y[i] /= L[(size_t)i * B1];
+  for (int i = c0 + tid; i < c1; i += kBS) y[i] /= L[(size_t)i * B1];
   __syncthreads();
-  // ---- backward: L^T x = z ----
-  const int nb = (n + kW - 1) / kW;
-  for (int q = nb - 1; q >= 0; --q) {
-    const int j0 = q * kW, w = min(kW, n - j0);
+}
+
+// backward substitution over columns [c0, c1) (c0 a multiple of kW)
+__device__ __forceinline__ void band_backward(const double* L, double* y, double* sy, int n, int b, int c0, int c1,
+                                              int tid, int wid, int lane) {
+  const int B1 = b + 1;
+  for (int q = (c1 - 1) / kW; c1 > c0 && q >= c0 / kW; --q) {
+    const int j0 = q * kW, w = min(kW, c1 - j0);
     double lt[kW];  // wavefront 0, lane r: L(c, r), c > r, loaded before the row sums
     if (wid == 0) {
 #pragma unroll
@@ -432,31 +465,96 @@ __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, const int32_t* _
   }
 }
 
-__global__ void k_band_scatter(int32_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ pblk,
-                               const double* __restrict__ y, const int32_t* __restrict__ active, int32_t astride,
-                               double* __restrict__ du) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const bool on = active == nullptr || active[(size_t)pblk[i] * astride] != 0;
-  du[perm[i]] = on ? y[i] : 0.0;
+// mode 0: forward and pivots over [0, nf); mode 1: the separator, [nf, nend) forward, pivots and
+// backward (after k_band_sep has summed its right-hand side); mode 2: backward over [0, nf) (after
+// k_band_sep has copied the separator's solution to segment 1).  A block in one segment: mode 0
+// and mode 2 are the whole solve.
+__global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, int32_t mode, const int32_t* __restrict__ active,
+                                                   int32_t astride) {
+  __shared__ double sy[kW];
+  const BandSeg S = A.seg[blockIdx.x];
+  if (active && !active[(size_t)S.blk * astride]) return;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const double* L = A.band + S.boff;
+  double* y = A.y + S.yoff;
+  if (mode == 0) {
+    band_forward(L, y, sy, S.n, S.b, 0, S.nf, tid, wid, lane);
+  } else if (mode == 1) {
+    if (S.nend <= S.nf) return;
+    band_forward(L, y, sy, S.n, S.b, S.nf, S.nend, tid, wid, lane);
+    band_backward(L, y, sy, S.n, S.b, S.nf, S.nend, tid, wid, lane);
+  } else {
+    band_backward(L, y, sy, S.n, S.b, 0, S.nf, tid, wid, lane);
+  }
 }
 
-// permuted right-hand side only (a solve with the factors already in the band)
-__global__ void k_band_rhs(int32_t nrows, const int32_t* __restrict__ iperm, const double* __restrict__ rhs,
-                           double* __restrict__ y) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nrows) y[iperm[r]] = rhs[r];
+// The separator of each cut block (one workgroup per segment; segment 0 of the pair acts).
+// k_band_schur: segment 0's separator triangle takes segment 1's partial Schur complement, which
+// holds the separator's entries in reversed order (separator row i is segment 0's row nf + i and
+// segment 1's row n - 1 - i; the band offset i - j is the same in both).
+__global__ void k_band_schur(BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
+  const BandSeg S = A.seg[blockIdx.x];
+  if (S.partner < 0 || S.dir < 0) return;
+  if (active && !active[(size_t)S.blk * astride]) return;
+  const BandSeg P = A.seg[S.partner];
+  const int B1 = S.b + 1, ns = S.n - S.nf;
+  double* LA = A.band + S.boff;
+  const double* LB = A.band + P.boff;
+  for (int t = threadIdx.x; t < ns * B1; t += blockDim.x) {
+    const int i = t / B1, d = t - i * B1;  // entry (i, i - d) of the separator
+    if (d <= i) LA[(size_t)(S.nf + i - d) * B1 + d] += LB[(size_t)(P.n - 1 - i) * B1 + d];
+  }
+}
+
+// k_band_sep: copy 0 -- segment 0's separator right-hand side (after both forward sweeps) takes
+// segment 1's; copy 1 -- segment 1's separator rows take the separator's solution.
+__global__ void k_band_sep(BandArgs A, int32_t copy, const int32_t* __restrict__ active, int32_t astride) {
+  const BandSeg S = A.seg[blockIdx.x];
+  if (S.partner < 0 || S.dir < 0) return;
+  if (active && !active[(size_t)S.blk * astride]) return;
+  const BandSeg P = A.seg[S.partner];
+  double* ya = A.y + S.yoff + S.nf;
+  double* yb = A.y + P.yoff + (P.n - 1);
+  for (int i = threadIdx.x; i < S.n - S.nf; i += blockDim.x) {
+    if (copy) yb[-i] = ya[i];
+    else ya[i] += yb[-i];
+  }
+}
+
+// du[perm[position]] from each segment's own rows (k < zc: every band position once); 0 for the
+// rows of blocks whose active[blk * astride] is 0
+__global__ void k_band_scatter(int32_t ncols, const int32_t* __restrict__ cseg, const int32_t* __restrict__ perm,
+                               BandArgs A, const int32_t* __restrict__ active, int32_t astride,
+                               double* __restrict__ du) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= ncols) return;
+  const BandSeg S = A.seg[cseg[J]];
+  const int k = J - S.yoff;
+  if (k >= S.zc) return;
+  const bool on = active == nullptr || active[(size_t)S.blk * astride] != 0;
+  du[perm[S.base + S.dir * k]] = on ? A.y[J] : 0.0;
+}
+
+// each segment's right-hand side only (a solve with the factors already in the band)
+__global__ void k_band_rhs(int32_t ncols, const int32_t* __restrict__ cseg, const int32_t* __restrict__ perm,
+                           const double* __restrict__ rhs, BandArgs A) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= ncols) return;
+  const BandSeg S = A.seg[cseg[J]];
+  const int k = J - S.yoff;
+  A.y[J] = k < S.zc ? rhs[perm[S.base + S.dir * k]] : 0.0;
 }
 
 }  // namespace
 
 struct BandWork {
-  uint64_t key = 0;       // problem uid and block layout it was built for
+  uint64_t key = 0;       // problem uid, block layout and cut setting it was built for
   int32_t nblk = 0, n = 0, bmax = 0;
+  int32_t nseg = 0, ncut = 0, ncols = 0;  // segments, blocks cut in two, band columns of all segments
   int64_t nband = 0;
   std::vector<int32_t> bw_h, bstart_h;
-  DevBuf<int32_t> perm, iperm, pblk, bstart, bw;
-  DevBuf<int64_t> boff;
+  DevBuf<int32_t> perm, iperm, cseg;
+  DevBuf<BandSeg> seg;
   DevBuf<double> band, y;
   DevBuf<int32_t> bad;
   DevBuf<int32_t> bok;    // per block: 1 once the band holds its factors
@@ -478,7 +576,8 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   if (!p->prepared) return MPF_ERR_STATE;
   if (P.presDOF > 0) return MPF_ERR_ARG;  // displacement systems only (no pivoting)
   const int nblk = (int)node_off.size() - 1;
-  const uint64_t key = p->uid * 1000003ull + (uint64_t)nblk * 7919ull + (uint64_t)node_off.back();
+  const uint64_t key =
+      p->uid * 1000003ull + (uint64_t)nblk * 7919ull + (uint64_t)node_off.back() * 2ull + (p->band_split ? 1 : 0);
   if (p->band && p->band->key == key) return MPF_OK;
   if (!p->band) p->band = new BandWork();
   BandWork& W = *p->band;
@@ -494,8 +593,7 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   std::vector<int32_t> nb(nbptr[nNode] > 0 ? nbptr[nNode] : 1);
   MPF_HIP(hipMemcpy(nb.data(), P.nb.ptr, (size_t)nbptr[nNode] * sizeof(int32_t), hipMemcpyDeviceToHost));
   const int n = P.dispDOF;
-  std::vector<int32_t> perm(n), iperm(n), pblk(n), bstart(nblk + 1, 0), bw(nblk, 0);
-  std::vector<int64_t> boff(nblk + 1, 0);
+  std::vector<int32_t> perm(n), iperm(n), bstart(nblk + 1, 0), bw(nblk, 0);
   std::vector<int32_t> first(nNode, -1), last(nNode, -1), mark(nNode, 0), level;
   auto deg = [&](int v) { return nbptr[v + 1] - nbptr[v]; };
   auto nfree = [&](int v) {
@@ -595,19 +693,74 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
         if (r < 0) continue;
         perm[pos] = r;
         iperm[r] = pos;
-        pblk[pos] = m;
         ++pos;
       }
     }
     bw[m] = bbest;
-    boff[m + 1] = boff[m] + (int64_t)(pos - bstart[m]) * (bbest + 1);
   }
   bstart[nblk] = pos;
   if (pos != n) return MPF_ERR_STATE;  // every free dof belongs to one block
+  // segments: a block of nb >= 3 b + 64 rows is cut at a separator of b <= ns < b + kW rows in its
+  // middle (A1 and A2 whole panels); the segments are then ordered longest first (by columns
+  // factored before the separator), so that the longest start first
+  std::vector<BandSeg> seg;
+  int ncut = 0;
+  for (int m = 0; m < nblk; ++m) {
+    const int nb = bstart[m + 1] - bstart[m], b = bw[m];
+    BandSeg g{};
+    g.blk = m;
+    g.b = b;
+    g.partner = -1;
+    if (p->band_split && nb >= 3 * b + 4 * kW) {
+      const int a1 = kW * ((nb - b) / (2 * kW)), a2 = kW * ((nb - a1 - b) / kW), ns = nb - a1 - a2;
+      BandSeg h = g;
+      g.base = bstart[m];
+      g.dir = 1;
+      g.n = a1 + ns;
+      g.nf = a1;
+      g.nend = g.zc = a1 + ns;
+      g.partner = (int)seg.size() + 1;
+      h.base = bstart[m] + nb - 1;
+      h.dir = -1;
+      h.n = a2 + ns;
+      h.nf = h.nend = h.zc = a2;
+      h.partner = (int)seg.size();
+      seg.push_back(g);
+      seg.push_back(h);
+      ++ncut;
+    } else {
+      g.base = bstart[m];
+      g.dir = 1;
+      g.n = g.nf = g.nend = g.zc = nb;
+      seg.push_back(g);
+    }
+  }
+  std::vector<int32_t> order(seg.size()), where(seg.size());
+  for (size_t i = 0; i < seg.size(); ++i) order[i] = (int32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return seg[x].nf > seg[y].nf; });
+  for (size_t i = 0; i < order.size(); ++i) where[order[i]] = (int32_t)i;
+  std::vector<BandSeg> sorted(seg.size());
+  int64_t nband = 0;
+  int32_t ncols = 0;
+  for (size_t i = 0; i < order.size(); ++i) {
+    BandSeg g = seg[order[i]];
+    if (g.partner >= 0) g.partner = where[g.partner];
+    g.boff = nband;
+    g.yoff = ncols;
+    nband += (int64_t)g.n * (g.b + 1);
+    ncols += g.n;
+    sorted[i] = g;
+  }
+  std::vector<int32_t> cseg(std::max(ncols, 1), 0);
+  for (size_t i = 0; i < sorted.size(); ++i)
+    for (int k = 0; k < sorted[i].n; ++k) cseg[sorted[i].yoff + k] = (int32_t)i;
   W.nblk = nblk;
   W.n = n;
+  W.nseg = (int32_t)sorted.size();
+  W.ncut = ncut;
+  W.ncols = ncols;
   W.bmax = nblk > 0 ? *std::max_element(bw.begin(), bw.end()) : 0;
-  W.nband = boff[nblk];
+  W.nband = nband;
   W.bw_h = bw;
   W.bstart_h = bstart;
   const int lds = (int)(((size_t)(kW + W.bmax) * kW + 3 * kW) * sizeof(double));
@@ -616,12 +769,10 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   static_assert(kW + 493 <= kFBS, "the wide-band panel path gives each panel row one thread");
   MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
   MPF_TRY(W.iperm.upload(iperm.data(), iperm.size(), s));
-  MPF_TRY(W.pblk.upload(pblk.data(), pblk.size(), s));
-  MPF_TRY(W.bstart.upload(bstart.data(), bstart.size(), s));
-  MPF_TRY(W.bw.upload(bw.data(), bw.size(), s));
-  MPF_TRY(W.boff.upload(boff.data(), boff.size(), s));
+  MPF_TRY(W.cseg.upload(cseg.data(), cseg.size(), s));
+  if (!sorted.empty()) MPF_TRY(W.seg.upload(sorted.data(), sorted.size(), s));
   MPF_TRY(W.band.alloc(W.nband > 0 ? (size_t)W.nband : 1));
-  MPF_TRY(W.y.alloc(n > 0 ? (size_t)n : 1));
+  MPF_TRY(W.y.alloc(ncols > 0 ? (size_t)ncols : 1));
   MPF_TRY(W.bad.alloc(1));
   {
     const std::vector<int32_t> zero(std::max(nblk, 1), 0);
@@ -632,24 +783,43 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   return MPF_OK;
 }
 
+// Solve phases of every segment with the factors in the band (active as for the factorisation).
+static void enqueue_band_solves(const BandWork* W, const BandArgs& A, const int32_t* active, int32_t astride,
+                                hipStream_t s) {
+  const dim3 g(std::max(W->nseg, 1));
+  hipLaunchKernelGGL(k_band_solve, g, dim3(kBS), 0, s, A, 0, active, astride);
+  if (W->ncut > 0) {
+    hipLaunchKernelGGL(k_band_sep, g, dim3(256), 0, s, A, 0, active, astride);
+    hipLaunchKernelGGL(k_band_solve, g, dim3(kBS), 0, s, A, 1, active, astride);
+    hipLaunchKernelGGL(k_band_sep, g, dim3(256), 0, s, A, 1, active, astride);
+  }
+  hipLaunchKernelGGL(k_band_solve, g, dim3(kBS), 0, s, A, 2, active, astride);
+}
+
 // Enqueue: band from the assembled K, factor and solve every block, du = K^-1 rhs per block (0 for
 // blocks whose active[m * astride] is 0; active may be null).  Allocates nothing (graph-capturable).
 mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t astride) {
   BandWork* W = p->band;
   if (!W) return MPF_ERR_STATE;
+  if (W->nseg == 0) return MPF_OK;  // no free dofs
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
-  BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
+  const BandArgs A{W->seg.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
   static_assert(kBandColMax >= 494, "the panel LDS allows half-bandwidths up to 493");
-  hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->n + 3) / 4, 1)), dim3(256), 0, s, W->n, P.rowptr.ptr,
-                     P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->perm.ptr, W->iperm.ptr, W->pblk.ptr, A, active,
+  hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->ncols + 3) / 4, 1)), dim3(256), 0, s, W->ncols, W->cseg.ptr,
+                     P.rowptr.ptr, P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->perm.ptr, W->iperm.ptr, A, active,
                      astride);
   const size_t lds = (kW + W->bmax <= 3 * 64 ? 2 * ((size_t)(kW + W->bmax) * kW + kW)
                                              : (size_t)(kW + W->bmax) * kW + 3 * kW) * sizeof(double);
-  hipLaunchKernelGGL(k_band_factor, dim3(W->nblk), dim3(kFBS), lds, s, A, W->bmax, active, astride, W->bok.ptr);
-  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, active, astride);
-  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
-                     W->y.ptr, active, astride, p->du.ptr);
+  const dim3 g(std::max(W->nseg, 1));
+  hipLaunchKernelGGL(k_band_factor, g, dim3(kFBS), lds, s, A, W->bmax, 0, active, astride, W->bok.ptr);
+  if (W->ncut > 0) {
+    hipLaunchKernelGGL(k_band_schur, g, dim3(256), 0, s, A, active, astride);
+    hipLaunchKernelGGL(k_band_factor, g, dim3(kFBS), lds, s, A, W->bmax, 1, active, astride, W->bok.ptr);
+  }
+  enqueue_band_solves(W, A, active, astride, s);
+  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(std::max(W->ncols, 1), 256)), dim3(256), 0, s, W->ncols,
+                     W->cseg.ptr, W->perm.ptr, A, active, astride, p->du.ptr);
   MPF_HIP(hipGetLastError());
   W->factored = true;
   return MPF_OK;
@@ -661,13 +831,14 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
 mpf_status band_enqueue_resolve(mpf_problem* p) {
   BandWork* W = p->band;
   if (!W || !W->factored) return MPF_ERR_STATE;
+  if (W->nseg == 0) return MPF_OK;  // no free dofs
   hipStream_t s = p->stream;
-  BandArgs A{W->bstart.ptr, W->bw.ptr, W->boff.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
-  hipLaunchKernelGGL(k_band_rhs, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->iperm.ptr, p->rhs.ptr,
-                     W->y.ptr);
-  hipLaunchKernelGGL(k_band_solve, dim3(W->nblk), dim3(kBS), 0, s, A, W->bok.ptr, 1);
-  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(W->n, 256)), dim3(256), 0, s, W->n, W->perm.ptr, W->pblk.ptr,
-                     W->y.ptr, W->bok.ptr, 1, p->du.ptr);
+  const BandArgs A{W->seg.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
+  hipLaunchKernelGGL(k_band_rhs, dim3(grid_for(std::max(W->ncols, 1), 256)), dim3(256), 0, s, W->ncols,
+                     W->cseg.ptr, W->perm.ptr, p->rhs.ptr, A);
+  enqueue_band_solves(W, A, W->bok.ptr, 1, s);
+  hipLaunchKernelGGL(k_band_scatter, dim3(grid_for(std::max(W->ncols, 1), 256)), dim3(256), 0, s, W->ncols,
+                     W->cseg.ptr, W->perm.ptr, A, W->bok.ptr, 1, p->du.ptr);
   MPF_HIP(hipGetLastError());
   return MPF_OK;
 }
@@ -705,5 +876,18 @@ extern "C" mpf_status mpf_band_stats(const mpf_problem* p, int32_t* nblk, int32_
   if (bmax) *bmax = W.bmax;
   if (ndof) *ndof = W.n;
   if (flops) *flops = f;
+  return MPF_OK;
+}
+
+extern "C" mpf_status mpf_set_band_split(mpf_problem* p, int32_t enable) {
+  if (!p || (enable != 0 && enable != 1)) return MPF_ERR_ARG;
+  p->band_split = enable;  // the layout is rebuilt at the next direct solve (its key holds the setting)
+  return MPF_OK;
+}
+
+extern "C" mpf_status mpf_band_segments(const mpf_problem* p, int32_t* nseg, int32_t* ncut) {
+  if (!p || !p->band || p->band->key == 0) return MPF_ERR_STATE;
+  if (nseg) *nseg = p->band->nseg;
+  if (ncut) *ncut = p->band->ncut;
   return MPF_OK;
 }

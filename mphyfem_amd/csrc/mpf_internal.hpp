@@ -207,6 +207,7 @@ struct mpf_problem {
   mpf::KrylovWork* kw = nullptr;
   // assembly algorithm (MPF_ASSEMBLY_*) and the gather path's staging records
   int32_t asm_mode = MPF_ASSEMBLY_GATHER;
+  int32_t band_split = 1;  // banded direct solve: cut long blocks at a separator (band.hip)
   int32_t chunk_cap = 32;  // Hex8 chunked matrix kernel: most distinct elements staged in LDS
   mpf::DevBuf<double> GS;
   mpf::DevBuf<double> scratch;  // reductions (norms)

@@ -109,6 +109,32 @@ def test_tangent_increment_solves_the_tangent_system(mpf, gpu):
     assert torch.equal(d, d2)
 
 
+def test_batched_band_cut_blocks_match_whole_blocks(mpf, gpu):
+    """Cantilevers of 32 .. 96 elements (half-bandwidth 167): every beam cut in two at a separator
+    (the default) against every beam kept whole -- a batched load step with direct Newton
+    corrections and the tangent increment from the factors left in the band."""
+    import torch
+    from mphyfem_amd import learned
+
+    beams = learned.cantilever_workload(40)
+    idx = [int(i) for i in np.argsort(beams.lengths)[[0, 20, 39]]]  # shortest, middle, longest
+    out = {}
+    for split in (False, True):
+        _, P = learned.cantilever_problem(beams, idx)
+        P.set_band_split(split)
+        iters, rel, rep = P.batch_newton_step(1.0 / learned.NSTEPS, itmax=25, rtol=1e-9,
+                                              solver=mpf.BATCH_SOLVER_BAND)  # the first step of the ramp
+        print("split", split, "iterations", iters, "relres", rel, flush=True)
+        assert np.all(rel < 1e-9), rel
+        u, _ = P.state()
+        d = torch.zeros(3 * P.nNode, dtype=torch.float64, device="cuda")
+        P.batch_tangent_increment(1.0, d)
+        out[split] = (iters, u, d.cpu().numpy(), P.band_segments())
+    assert out[True][3] == (6, 3) and out[False][3] == (3, 0)
+    assert normwise_rel(out[True][1], out[False][1]) < 1e-9
+    assert normwise_rel(out[True][2], out[False][2]) < 1e-8
+
+
 def test_tangent_increment_of_a_mesh_never_factored_is_zero(mpf, gpu):
     """A mesh with no load converges at its start and is never factored: its tangent increment is
     zero (not a solve with an empty band), the others' are finite."""

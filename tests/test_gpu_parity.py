@@ -536,6 +536,50 @@ def test_band_direct_solve_vs_oracle(mpf, orc, gpu):
     assert flops == float(ndof) * bmax * (bmax + 1)
 
 
+@pytest.mark.parametrize("shape", ["cube", "beam"])
+def test_band_cut_block_vs_oracle(mpf, orc, gpu, shape):
+    """A block cut in two at a separator (elimination order [first half, second half reversed,
+    separator]) against the oracle's banded LDL^T and against the same block kept whole: the
+    8^3 cube takes the wide-band panel path (b > 176), a 40 x 3 x 3 beam the one-wavefront panel
+    path with the look-ahead.  Different rounding from the whole block, bitwise repeatable."""
+    m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
+    if shape == "cube":
+        n = 8
+        X, conn, P = _hex_problem(mpf, n, m)
+        O = _orc_hex_problem(orc, mpf, X, conn, n, m)
+    else:
+        dims = (40, 3, 3)
+        X, conn = mpf.mesh_hex(*dims, L=(4.0, 0.3, 0.3), jitter=0.05, seed=3)
+        bcs = [(mpf.face_nodes(mpf.HEX8, *dims, 0), -1, None), (mpf.face_nodes(mpf.HEX8, *dims, 1), 0, 0.05)]
+        P = mpf.Problem(mpf.HEX8, X, conn, m)
+        O = orc.Problem(8, X, conn, m.mu, m.Kinv, m.Utype)
+        O.set_tangent(True)
+        for Q in (P, O):
+            apply_bcs(Q, bcs)
+            Q.prepare_matrix_pattern()
+    for Q in (P, O):
+        Q.set_boundary_conditions(1.0)
+    P.calc_stiffness_and_residual(True, 1.0)
+    vals, rhs = P.system()
+    x_ref, st = O.direct_solve(vals, rhs)
+    assert st == 0
+    assert P.factorise_and_solve(ksp=mpf.KSP_BAND).converged
+    cut = P.solution()
+    nblk, bmax, ndof, flops = P.band_stats()
+    assert (nblk, ndof) == (1, len(rhs)) and (bmax > 176) == (shape == "cube")
+    assert P.band_segments() == (2, 1)
+    P.factorise_and_solve(ksp=mpf.KSP_BAND)
+    assert np.array_equal(P.solution(), cut)
+    P.set_band_split(False)
+    assert P.factorise_and_solve(ksp=mpf.KSP_BAND).converged
+    whole = P.solution()
+    assert P.band_segments() == (1, 0)
+    for du in (cut, whole):
+        assert normwise_rel(du, x_ref) < 1e-10
+        assert np.linalg.norm(O.spmv(vals, du) - rhs) / np.linalg.norm(rhs) < 1e-12
+    assert normwise_rel(cut, whole) < 1e-12 and not np.array_equal(cut, whole)
+
+
 def test_band_rejects_mixed_system(mpf, gpu):
     X, conn = mpf.mesh_tet10(2, jitter=0.1, seed=2)
     P = mpf.Problem(mpf.TET10P1, X, conn, mpf.matl(1.0, 1e-3, 3, mixed=True))
