@@ -73,42 +73,46 @@ struct BandArgs {
 };
 
 // Band of every active segment from the CSR rows, zeros included, and its right-hand side, in one
-// pass: one wavefront per band column (4 per workgroup; segments of blocks whose active[blk *
+// pass: 16 lanes per band column, 16 columns per workgroup (segments of blocks whose active[blk *
 // astride] is 0 keep their factors).  Column k of a segment holds L(i, k), i = k .. k + b, which
 // K's symmetry puts in CSR row perm[base + dir k]: the row's entries whose local row falls in the
-// band (and in the segment) are placed in an LDS copy of the column, zeros elsewhere, and the
-// column goes out with consecutive lanes.  Columns from zc are zero.  Also clears the pivot flag.
-// (Measured on the 200 cantilevers: 2.1 ms as a zero pass of the band followed by a scatter of
-// each CSR row into it, against 1.6 ms in one pass.)
-constexpr int kBandColMax = 496;  // b + 1 of the widest band the panel LDS allows
-__global__ void __launch_bounds__(256) k_band_fill(int32_t ncols, const int32_t* __restrict__ cseg,
+// band (and in the segment) are placed in an LDS copy of the column (colmax >= b + 1 doubles),
+// zeros elsewhere, and the column goes out with consecutive lanes.  Columns from zc are zero.
+// Also clears the pivot flag.  A column is a chain of dependent loads (segment, row, row pointer,
+// column indices, positions), so many columns per compute unit are kept in flight.  (Measured on
+// the 200 cantilevers: 2.1 ms as a zero pass of the band followed by a scatter of each CSR row
+// into it; 1.6 ms in one pass with a wavefront per column.)
+constexpr int kFillLanes = 16;                 // lanes per band column
+constexpr int kFillCols = 256 / kFillLanes;    // columns per workgroup
+__global__ void __launch_bounds__(256) k_band_fill(int32_t ncols, int32_t colmax, const int32_t* __restrict__ cseg,
                                                    const int32_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ colidx,
                                                    const double* __restrict__ vals, const double* __restrict__ rhs,
                                                    const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
                                                    BandArgs A, const int32_t* __restrict__ active, int32_t astride) {
-  __shared__ double sCol[4][kBandColMax];
+  extern __shared__ double sFill[];
   if (blockIdx.x == 0 && threadIdx.x == 0) *A.bad = 0;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int J = blockIdx.x * 4 + w;
-  if (J >= ncols) return;  // the wavefronts of a workgroup share no data: no workgroup barrier below
+  const int grp = threadIdx.x / kFillLanes, lane = threadIdx.x % kFillLanes;
+  const int J = blockIdx.x * kFillCols + grp;
+  // a column's lanes share its LDS copy within one wavefront: no workgroup barrier below
+  if (J >= ncols) return;
   const BandSeg S = A.seg[cseg[J]];
   if (active && !active[(size_t)S.blk * astride]) return;
   const int k = J - S.yoff, b = S.b, B1 = b + 1;
-  double* col = sCol[w];
-  for (int t = lane; t < B1; t += 64) col[t] = 0.0;
+  double* col = sFill + (size_t)grp * colmax;
+  for (int t = lane; t < B1; t += kFillLanes) col[t] = 0.0;
   __builtin_amdgcn_wave_barrier();
   int r = -1;
-  if (k < S.zc) {  // uniform
+  if (k < S.zc) {
     r = perm[S.base + S.dir * k];
-    for (int e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64) {
+    for (int e = rowptr[r] + lane; e < rowptr[r + 1]; e += kFillLanes) {
       const int i = S.dir * (iperm[colidx[e]] - S.base);
       if (i >= k && i - k <= b && i < S.n) col[i - k] = vals[e];
     }
   }
   __builtin_amdgcn_wave_barrier();
   double* L = A.band + S.boff + (size_t)k * B1;
-  for (int t = lane; t < B1; t += 64) L[t] = col[t];
+  for (int t = lane; t < B1; t += kFillLanes) L[t] = col[t];
   if (lane == 0) A.y[J] = r >= 0 ? rhs[r] : 0.0;
 }
 
@@ -805,8 +809,9 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   hipStream_t s = p->stream;
   const PatternBufs& P = p->pat;
   const BandArgs A{W->seg.ptr, W->band.ptr, W->y.ptr, W->bad.ptr};
-  static_assert(kBandColMax >= 494, "the panel LDS allows half-bandwidths up to 493");
-  hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->ncols + 3) / 4, 1)), dim3(256), 0, s, W->ncols, W->cseg.ptr,
+  static_assert(kFillCols * 494 * sizeof(double) <= 64 * 1024, "the fill's LDS holds columns of b <= 493");
+  hipLaunchKernelGGL(k_band_fill, dim3(std::max((W->ncols + kFillCols - 1) / kFillCols, 1)), dim3(256),
+                     (size_t)kFillCols * (W->bmax + 1) * sizeof(double), s, W->ncols, W->bmax + 1, W->cseg.ptr,
                      P.rowptr.ptr, P.colidx.ptr, p->vals.ptr, p->rhs.ptr, W->perm.ptr, W->iperm.ptr, A, active,
                      astride);
   const size_t lds = (kW + W->bmax <= 3 * 64 ? 2 * ((size_t)(kW + W->bmax) * kW + kW)
@@ -814,7 +819,7 @@ mpf_status band_enqueue_solve(mpf_problem* p, const int32_t* active, int32_t ast
   const dim3 g(std::max(W->nseg, 1));
   hipLaunchKernelGGL(k_band_factor, g, dim3(kFBS), lds, s, A, W->bmax, 0, active, astride, W->bok.ptr);
   if (W->ncut > 0) {
-    hipLaunchKernelGGL(k_band_schur, g, dim3(256), 0, s, A, active, astride);
+    hipLaunchKernelGGL(k_band_schur, g, dim3(1024), 0, s, A, active, astride);
     hipLaunchKernelGGL(k_band_factor, g, dim3(kFBS), lds, s, A, W->bmax, 1, active, astride, W->bok.ptr);
   }
   enqueue_band_solves(W, A, active, astride, s);
