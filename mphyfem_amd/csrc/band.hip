@@ -388,27 +388,53 @@ __global__ void __launch_bounds__(kFBS) k_band_factor(BandArgs A, int32_t bmax, 
 // or per row of the block by one wavefront with a butterfly sum (backward).  Over columns
 // [c0, c1): forward L z = y, then z / d; backward L^T x = z, rows past c1 already solved.
 
-// forward substitution over columns [c0, c1) and the division by the pivots
-__device__ __forceinline__ void band_forward(const double* L, double* y, double* sy, int n, int b, int c0, int c1,
-                                             int tid, int wid, int lane) {
+// Band entries of forward block step j0 (w rows): thread gt of a group holds coupling row
+// j0 + w + gt's entries in lc (zero outside the band), the group's wavefront 0 lane r row r of the
+// unit triangle in l.
+__device__ __forceinline__ void fwd_entries(const double* L, int B1, int b, int n, int j0, int w, int gt, int gw,
+                                            int lane, double (&lc)[kW], double (&l)[kW]) {
+  const int r = j0 + w + gt;
+  const bool crow = r < min(n, j0 + w + b);
+#pragma unroll
+  for (int k = 0; k < kW; ++k)
+    lc[k] = (crow && k < w && r - (j0 + k) <= b) ? L[(size_t)(j0 + k) * B1 + (r - (j0 + k))] : 0.0;
+  if (gw == 0) {
+#pragma unroll
+    for (int c = 0; c < kW; ++c)
+      l[c] = (lane < w && c < lane && lane - c <= b) ? L[(size_t)(j0 + c) * B1 + (lane - c)] : 0.0;
+  }
+}
+
+// The rows a sweep is working on -- the b + kW rows of the window below (forward) or above
+// (backward) the current block -- are kept in an LDS ring, row r at ring[r % kRing], so that a
+// block step reads and updates them without a round trip through global memory.
+constexpr int kRing = 1024;
+static_assert(kRing >= 2 * (493 + kW), "the ring holds the window and the rows loaded ahead of it");
+
+// forward substitution over columns [c0, c1) and the division by the pivots.  The two halves of
+// the workgroup take the block steps in turn (b < kBS / 2 coupling rows each), and each half loads
+// the band entries of its next step as soon as it has applied the current one, so that they
+// arrive while the other half's step runs.  The ring starts with rows c0 .. c0 + kRing - 1; a row
+// solved at a step frees its slot for the row kRing further on, loaded then and stored one step
+// later; the rows past c1 that took updates are written back at the end.
+__device__ __forceinline__ void band_forward(const double* L, double* y, double* sy, double* ring, int n, int b,
+                                             int c0, int c1, int tid, int wid, int lane) {
   const int B1 = b + 1;
-  for (int j0 = c0; j0 < c1; j0 += kW) {
-    const int w = min(kW, c1 - j0);
-    // the coupling rows' band entries and right-hand sides do not depend on this block's z: load
-    // them before the block is solved (b < kBS, so one row per thread)
-    const int r = j0 + w + tid;
-    const bool crow = r < min(n, j0 + w + b);
-    double lc[kW];
-#pragma unroll
-    for (int k = 0; k < kW; ++k)
-      lc[k] = (crow && k < w && r - (j0 + k) <= b) ? L[(size_t)(j0 + k) * B1 + (r - (j0 + k))] : 0.0;
-    const double yr = crow ? y[r] : 0.0;
-    if (wid == 0) {
-      double l[kW];  // lane r: L(r, c), c < r
-#pragma unroll
-      for (int c = 0; c < kW; ++c)
-        l[c] = (lane < w && c < lane && lane - c <= b) ? L[(size_t)(j0 + c) * B1 + (lane - c)] : 0.0;
-      double z = lane < w ? y[j0 + lane] : 0.0;
+  const int g = tid / (kBS / 2), gt = tid % (kBS / 2), gw = gt >> 6;
+  const int nsteps = (c1 - c0 + kW - 1) / kW;
+  for (int i = tid; i < kRing; i += kBS) ring[(c0 + i) % kRing] = c0 + i < n ? y[c0 + i] : 0.0;
+  double lc[kW], l[kW];
+  if (g < nsteps) fwd_entries(L, B1, b, n, c0 + g * kW, min(kW, c1 - c0 - g * kW), gt, gw, lane, lc, l);
+  double pend = 0.0;  // wavefront 1, lanes 0 .. kW - 1: a row loaded ahead, stored into the ring one step later
+  int prow = -1;
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    const int j0 = c0 + st * kW, w = min(kW, c1 - j0);
+    const bool mine = (st & 1) == g;
+    const int r = j0 + w + gt;
+    const bool crow = mine && r < min(n, j0 + w + b);
+    if (mine && gw == 0) {
+      double z = lane < w ? ring[(j0 + lane) % kRing] : 0.0;
 #pragma unroll
       for (int c = 0; c < kW; ++c) {
         if (c < w) {
@@ -423,21 +449,55 @@ __device__ __forceinline__ void band_forward(const double* L, double* y, double*
     }
     __syncthreads();
     if (crow) {
-      double acc = yr;
+      double acc = ring[r % kRing];
       for (int k = 0; k < w; ++k)
         if (r - (j0 + k) <= b) acc -= lc[k] * sy[k];
-      y[r] = acc;
+      ring[r % kRing] = acc;
     }
+    if (wid == 1 && lane < kW) {  // the slots of the rows solved at the step before take their rows kRing on
+      if (prow >= 0) ring[prow % kRing] = pend;
+      const int nr = j0 + lane + kRing;
+      prow = (lane < w && nr < n) ? nr : -1;
+      if (prow >= 0) pend = y[nr];
+    }
+    if (mine && st + 2 < nsteps)
+      fwd_entries(L, B1, b, n, j0 + 2 * kW, min(kW, c1 - j0 - 2 * kW), gt, gw, lane, lc, l);
     __syncthreads();
   }
+  for (int r = c1 + tid; r < min(n, c1 + b); r += kBS) y[r] = ring[r % kRing];
   for (int i = c0 + tid; i < c1; i += kBS) y[i] /= L[(size_t)i * B1];
   __syncthreads();
 }
 
-// backward substitution over columns [c0, c1) (c0 a multiple of kW)
-__device__ __forceinline__ void band_backward(const double* L, double* y, double* sy, int n, int b, int c0, int c1,
-                                              int tid, int wid, int lane) {
+constexpr int kBRow = 8;  // band entries per lane of a backward row sum: b < 64 kBRow
+static_assert(64 * kBRow > 493, "the widest band the panel LDS allows");
+
+// backward row k's entries L(r, k), r = j0 + w + lane + 64 t <= min(n - 1, k + b), k = j0 + wid,
+// and its right-hand side
+__device__ __forceinline__ void bwd_entries(const double* L, const double* y, int B1, int b, int n, int j0, int w,
+                                            int wid, int lane, double (&v)[kBRow], double& yk) {
+  const int k = j0 + wid;
+#pragma unroll
+  for (int t = 0; t < kBRow; ++t) {
+    const int r = j0 + w + lane + 64 * t;
+    v[t] = (wid < w && r <= min(n - 1, k + b)) ? L[(size_t)k * B1 + (r - k)] : 0.0;
+  }
+  yk = wid < w ? y[k] : 0.0;
+}
+
+// backward substitution over columns [c0, c1) (c0 a multiple of kW); each row's band entries and
+// right-hand side for the next block step are loaded while this one is summed, and the solved
+// rows the sums read are in the ring (rows c1 .. c1 + kRing - 1 from y at the start)
+__device__ __forceinline__ void band_backward(const double* L, double* y, double* sy, double* ring, int n, int b,
+                                              int c0, int c1, int tid, int wid, int lane) {
   const int B1 = b + 1;
+  for (int i = tid; i < kRing; i += kBS) ring[(c1 + i) % kRing] = c1 + i < n ? y[c1 + i] : 0.0;
+  double v[kBRow], vn[kBRow], yk = 0.0, ykn = 0.0;
+  if (c1 > c0) {
+    const int q = (c1 - 1) / kW;
+    bwd_entries(L, y, B1, b, n, q * kW, min(kW, c1 - q * kW), wid, lane, v, yk);
+  }
+  __syncthreads();
   for (int q = (c1 - 1) / kW; c1 > c0 && q >= c0 / kW; --q) {
     const int j0 = q * kW, w = min(kW, c1 - j0);
     double lt[kW];  // wavefront 0, lane r: L(c, r), c > r, loaded before the row sums
@@ -446,12 +506,17 @@ __device__ __forceinline__ void band_backward(const double* L, double* y, double
       for (int c = 0; c < kW; ++c)
         lt[c] = (lane < w && c < w && c > lane && c - lane <= b) ? L[(size_t)(j0 + lane) * B1 + (c - lane)] : 0.0;
     }
+    if (q > c0 / kW) bwd_entries(L, y, B1, b, n, j0 - kW, kW, wid, lane, vn, ykn);
     if (wid < w) {  // row j0 + wid: minus the rows already solved beyond the block
       const int k = j0 + wid;
       double part = 0.0;
-      for (int r = j0 + w + lane; r <= min(n - 1, k + b); r += 64) part += L[(size_t)k * B1 + (r - k)] * y[r];
+#pragma unroll
+      for (int t = 0; t < kBRow; ++t) {
+        const int r = j0 + w + lane + 64 * t;
+        if (r <= min(n - 1, k + b)) part += v[t] * ring[r % kRing];
+      }
       part = wave_sum(part);
-      if (lane == 0) sy[wid] = y[k] - part;
+      if (lane == 0) sy[wid] = yk - part;
     }
     __syncthreads();
     if (wid == 0) {
@@ -463,8 +528,14 @@ __device__ __forceinline__ void band_backward(const double* L, double* y, double
           if (lane < c) x -= lt[c] * xc;
         }
       }
-      if (lane < w) y[j0 + lane] = x;
+      if (lane < w) {
+        ring[(j0 + lane) % kRing] = x;
+        y[j0 + lane] = x;
+      }
     }
+#pragma unroll
+    for (int t = 0; t < kBRow; ++t) v[t] = vn[t];
+    yk = ykn;
     __syncthreads();
   }
 }
@@ -476,19 +547,20 @@ __device__ __forceinline__ void band_backward(const double* L, double* y, double
 __global__ void __launch_bounds__(kBS) k_band_solve(BandArgs A, int32_t mode, const int32_t* __restrict__ active,
                                                    int32_t astride) {
   __shared__ double sy[kW];
+  __shared__ double ring[kRing];
   const BandSeg S = A.seg[blockIdx.x];
   if (active && !active[(size_t)S.blk * astride]) return;
   const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
   const double* L = A.band + S.boff;
   double* y = A.y + S.yoff;
   if (mode == 0) {
-    band_forward(L, y, sy, S.n, S.b, 0, S.nf, tid, wid, lane);
+    band_forward(L, y, sy, ring, S.n, S.b, 0, S.nf, tid, wid, lane);
   } else if (mode == 1) {
     if (S.nend <= S.nf) return;
-    band_forward(L, y, sy, S.n, S.b, S.nf, S.nend, tid, wid, lane);
-    band_backward(L, y, sy, S.n, S.b, S.nf, S.nend, tid, wid, lane);
+    band_forward(L, y, sy, ring, S.n, S.b, S.nf, S.nend, tid, wid, lane);
+    band_backward(L, y, sy, ring, S.n, S.b, S.nf, S.nend, tid, wid, lane);
   } else {
-    band_backward(L, y, sy, S.n, S.b, 0, S.nf, tid, wid, lane);
+    band_backward(L, y, sy, ring, S.n, S.b, 0, S.nf, tid, wid, lane);
   }
 }
 
@@ -769,7 +841,7 @@ mpf_status band_prepare(mpf_problem* p, const std::vector<int32_t>& node_off) {
   W.bstart_h = bstart;
   const int lds = (int)(((size_t)(kW + W.bmax) * kW + 3 * kW) * sizeof(double));
   if (lds > 64 * 1024) return MPF_ERR_ARG;  // half-bandwidth beyond the panel's LDS (about 490 dofs)
-  if (W.bmax >= kBS) return MPF_ERR_ARG;     // the solve gives each coupling row one thread
+  if (W.bmax >= kBS / 2) return MPF_ERR_ARG;  // each half of the solve's workgroup gives a coupling row one thread
   static_assert(kW + 493 <= kFBS, "the wide-band panel path gives each panel row one thread");
   MPF_TRY(W.perm.upload(perm.data(), perm.size(), s));
   MPF_TRY(W.iperm.upload(iperm.data(), iperm.size(), s));

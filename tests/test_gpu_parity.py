@@ -536,20 +536,21 @@ def test_band_direct_solve_vs_oracle(mpf, orc, gpu):
     assert flops == float(ndof) * bmax * (bmax + 1)
 
 
-@pytest.mark.parametrize("shape", ["cube", "beam"])
+@pytest.mark.parametrize("shape", ["cube", "beam", "wide"])
 def test_band_cut_block_vs_oracle(mpf, orc, gpu, shape):
     """A block cut in two at a separator (elimination order [first half, second half reversed,
     separator]) against the oracle's banded LDL^T and against the same block kept whole: the
     8^3 cube takes the wide-band panel path (b > 176), a 40 x 3 x 3 beam the one-wavefront panel
-    path with the look-ahead.  Different rounding from the whole block, bitwise repeatable."""
+    path with the look-ahead, a 12 x 11 x 11 block (b > 448) also the solve's second pass over
+    coupling rows.  Different rounding from the whole block, bitwise repeatable."""
     m = mpf.matl_from_E_nu(1e3, 0.3, tangent=1)
     if shape == "cube":
         n = 8
         X, conn, P = _hex_problem(mpf, n, m)
         O = _orc_hex_problem(orc, mpf, X, conn, n, m)
     else:
-        dims = (40, 3, 3)
-        X, conn = mpf.mesh_hex(*dims, L=(4.0, 0.3, 0.3), jitter=0.05, seed=3)
+        dims, size = ((40, 3, 3), (4.0, 0.3, 0.3)) if shape == "beam" else ((12, 11, 11), (1.2, 1.1, 1.1))
+        X, conn = mpf.mesh_hex(*dims, L=size, jitter=0.05, seed=3)
         bcs = [(mpf.face_nodes(mpf.HEX8, *dims, 0), -1, None), (mpf.face_nodes(mpf.HEX8, *dims, 1), 0, 0.05)]
         P = mpf.Problem(mpf.HEX8, X, conn, m)
         O = orc.Problem(8, X, conn, m.mu, m.Kinv, m.Utype)
@@ -566,7 +567,8 @@ def test_band_cut_block_vs_oracle(mpf, orc, gpu, shape):
     assert P.factorise_and_solve(ksp=mpf.KSP_BAND).converged
     cut = P.solution()
     nblk, bmax, ndof, flops = P.band_stats()
-    assert (nblk, ndof) == (1, len(rhs)) and (bmax > 176) == (shape == "cube")
+    assert (nblk, ndof) == (1, len(rhs)) and (bmax > 176) == (shape != "beam")
+    assert (bmax > 448) == (shape == "wide") and bmax <= 493
     assert P.band_segments() == (2, 1)
     P.factorise_and_solve(ksp=mpf.KSP_BAND)
     assert np.array_equal(P.solution(), cut)
