@@ -425,7 +425,7 @@ __device__ __forceinline__ void band_forward(const double* L, double* y, double*
   for (int i = tid; i < kRing; i += kBS) ring[(c0 + i) % kRing] = c0 + i < n ? y[c0 + i] : 0.0;
   double lc[kW], l[kW];
   if (g < nsteps) fwd_entries(L, B1, b, n, c0 + g * kW, min(kW, c1 - c0 - g * kW), gt, gw, lane, lc, l);
-  double pend = 0.0;  // wavefront 1, lanes 0 .. kW - 1: a row loaded ahead, stored into the ring one step later
+  double pend = 0.0;  // last wavefront, lanes 0 .. kW - 1: a row loaded ahead, stored into the ring one step later
   int prow = -1;
   __syncthreads();
   for (int st = 0; st < nsteps; ++st) {
@@ -450,11 +450,15 @@ __device__ __forceinline__ void band_forward(const double* L, double* y, double*
     __syncthreads();
     if (crow) {
       double acc = ring[r % kRing];
-      for (int k = 0; k < w; ++k)
-        if (r - (j0 + k) <= b) acc -= lc[k] * sy[k];
+#pragma unroll
+      for (int k = 0; k < kW; ++k)
+        if (k < w) acc -= lc[k] * sy[k];  // lc is 0 beyond the band
       ring[r % kRing] = acc;
     }
-    if (wid == 1 && lane < kW) {  // the slots of the rows solved at the step before take their rows kRing on
+    // the last wavefront (a coupling row only in bands wider than kBS / 2 - 64, so its wait for the row
+    // loaded ahead is not a wait for band entries): the slots of the rows solved at the step before
+    // take their rows kRing on
+    if (wid == kBS / 64 - 1 && lane < kW) {
       if (prow >= 0) ring[prow % kRing] = pend;
       const int nr = j0 + lane + kRing;
       prow = (lane < w && nr < n) ? nr : -1;
