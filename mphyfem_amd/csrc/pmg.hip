@@ -614,7 +614,9 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
   std::vector<int32_t> vel(vptr[nN]), vfill(vptr.begin(), vptr.end() - 1);
   for (int32_t e = 0; e < nE; ++e)
     for (int a = 0; a < 4; ++a) vel[vfill[conn[(size_t)e * npe + a]]++] = e;
-  std::vector<int32_t> crow(1, 0), ccol, nbr;
+  // (each list is gathered without repeats through a marker per vertex, then sorted: the same sorted
+  // sets as sorting the gathered lists with their repeats, at a fraction of the host time)
+  std::vector<int32_t> crow(1, 0), ccol, nbr, row, mark(nN, -1);
   crow.reserve(nc + 1);
   for (int32_t v = 0; v < nN; ++v) {
     if (!isv[v]) continue;
@@ -623,16 +625,20 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
     if (!nfree) continue;
     nbr.clear();
     for (int32_t k = vptr[v]; k < vptr[v + 1]; ++k)
-      for (int a = 0; a < 4; ++a) nbr.push_back(conn[(size_t)vel[k] * npe + a]);
+      for (int a = 0; a < 4; ++a) {
+        const int32_t w = conn[(size_t)vel[k] * npe + a];
+        if (mark[w] != v) {
+          mark[w] = v;
+          nbr.push_back(w);
+        }
+      }
     std::sort(nbr.begin(), nbr.end());
-    nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
-    const size_t row0 = ccol.size();
+    row.clear();
     for (int32_t w : nbr)
       for (int a = 0; a < 3; ++a)
-        if (cnum[3 * (size_t)w + a] >= 0) ccol.push_back(cnum[3 * (size_t)w + a]);
-    const size_t len = ccol.size() - row0;
+        if (cnum[3 * (size_t)w + a] >= 0) row.push_back(cnum[3 * (size_t)w + a]);
     for (int i = 0; i < nfree; ++i) {
-      if (i > 0) ccol.insert(ccol.end(), ccol.begin() + row0, ccol.begin() + row0 + len);
+      ccol.insert(ccol.end(), row.begin(), row.end());
       crow.push_back((int32_t)ccol.size());
     }
     if (ccol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
@@ -703,15 +709,20 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
     for (int64_t A = 0; A < na; ++A) amptr[A + 1] += amptr[A];
     std::vector<int32_t> afill(amptr.begin(), amptr.end() - 1);
     for (int64_t i = 0; i < nc; ++i) amem[afill[aoff[i]]++] = (int32_t)i;
-    std::vector<int32_t> cols;
+    std::vector<int32_t> cols, amark(na, -1);
     for (int64_t A = 0; A < na; ++A) {
       cols.clear();
       for (int32_t t = amptr[A]; t < amptr[A + 1]; ++t) {
         const int32_t i = amem[t];
-        for (int32_t k = crow[i]; k < crow[i + 1]; ++k) cols.push_back(aoff[ccol[k]]);
+        for (int32_t k = crow[i]; k < crow[i + 1]; ++k) {
+          const int32_t c = aoff[ccol[k]];
+          if (amark[c] != (int32_t)A) {
+            amark[c] = (int32_t)A;
+            cols.push_back(c);
+          }
+        }
       }
       std::sort(cols.begin(), cols.end());
-      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
       kacol.insert(kacol.end(), cols.begin(), cols.end());
       karow.push_back((int32_t)kacol.size());
       if (kacol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
