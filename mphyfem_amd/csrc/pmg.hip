@@ -709,8 +709,21 @@ mpf_status build_tables(mpf_problem* p, PmgWork& W) {
     for (int64_t A = 0; A < na; ++A) amptr[A + 1] += amptr[A];
     std::vector<int32_t> afill(amptr.begin(), amptr.end() - 1);
     for (int64_t i = 0; i < nc; ++i) amem[afill[aoff[i]]++] = (int32_t)i;
-    std::vector<int32_t> cols, amark(na, -1);
+    // a vertex dof's Kc row depends on its vertex only, so an aggregate dof whose members sit on
+    // the same vertices as the previous one's (another component of the aggregate) has its columns
+    std::vector<int32_t> cols, amark(na, -1), dvert(nc), mv, mvp;
+    for (int64_t bb = 0; bb < ncb; ++bb)
+      for (int t = 0; t < cbm[bb]; ++t) dvert[cbs[bb] + t] = bvert[bb];
     for (int64_t A = 0; A < na; ++A) {
+      mv.clear();
+      for (int32_t t = amptr[A]; t < amptr[A + 1]; ++t) mv.push_back(dvert[amem[t]]);
+      if (A > 0 && mv == mvp) {
+        kacol.insert(kacol.end(), cols.begin(), cols.end());
+        karow.push_back((int32_t)kacol.size());
+        if (kacol.size() > (size_t)INT32_MAX) return MPF_ERR_ARG;
+        continue;
+      }
+      mvp.swap(mv);
       cols.clear();
       for (int32_t t = amptr[A]; t < amptr[A + 1]; ++t) {
         const int32_t i = amem[t];
