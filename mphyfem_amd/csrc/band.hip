@@ -134,11 +134,22 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // Panel of w columns from band column j0 into P (column-major, row stride PR: panel row r = band
 // row j0 + r, rows past the block's end as zeros), by threads t0, t0 + ts, ...
+// (in chunks of 4 entries per thread, each chunk's loads issued before its LDS stores: a band
+// up to 176 wide is one chunk over 768 threads, one round trip to memory)
 __device__ __forceinline__ void panel_load(double* P, const double* L, int j0, int w, int n, int B1, int PR, int t0,
                                            int ts) {
-  for (int t = t0; t < w * B1; t += ts) {  // panel (r, c), r - c <= b
-    const int c = t / B1, k = t - c * B1, r = c + k;
-    P[c * PR + r] = (j0 + r < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
+  for (int t1 = t0; t1 < w * B1; t1 += 4 * ts) {  // panel (r, c), r - c <= b
+    double v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = t1 + i * ts, c = t / B1, k = t - c * B1;
+      v[i] = (t < w * B1 && j0 + c + k < n) ? L[(size_t)(j0 + c) * B1 + k] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = t1 + i * ts, c = t / B1, k = t - c * B1;
+      if (t < w * B1) P[c * PR + c + k] = v[i];
+    }
   }
 }
 
