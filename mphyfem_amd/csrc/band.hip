@@ -105,9 +105,25 @@ __global__ void __launch_bounds__(256) k_band_fill(int32_t ncols, int32_t colmax
   int r = -1;
   if (k < S.zc) {
     r = perm[S.base + S.dir * k];
-    for (int e = rowptr[r] + lane; e < rowptr[r + 1]; e += kFillLanes) {
-      const int i = S.dir * (iperm[colidx[e]] - S.base);
-      if (i >= k && i - k <= b && i < S.n) col[i - k] = vals[e];
+    const int e1 = rowptr[r + 1];
+    // in chunks of 8 entries per lane, each stage's loads issued together (a row of up to 128
+    // entries is one round trip per stage, not one per entry)
+    for (int e0 = rowptr[r] + lane; e0 < e1; e0 += 8 * kFillLanes) {
+      int ci[8], pi[8];
+      double vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * kFillLanes;
+        ci[u] = e < e1 ? colidx[e] : -1;
+        vv[u] = e < e1 ? vals[e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) pi[u] = ci[u] >= 0 ? iperm[ci[u]] : 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = S.dir * (pi[u] - S.base);
+        if (ci[u] >= 0 && i >= k && i - k <= b && i < S.n) col[i - k] = vv[u];
+      }
     }
   }
   __builtin_amdgcn_wave_barrier();
