@@ -496,7 +496,18 @@ __device__ __forceinline__ void band_forward(const double* L, double* y, double*
     __syncthreads();
   }
   for (int r = c1 + tid; r < min(n, c1 + b); r += kBS) y[r] = ring[r % kRing];
-  for (int i = c0 + tid; i < c1; i += kBS) y[i] /= L[(size_t)i * B1];
+  for (int i0 = c0 + tid; i0 < c1; i0 += 4 * kBS) {  // 4 rows per thread per round trip
+    double yv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * kBS;
+      yv[u] = i < c1 ? y[i] : 0.0;
+      dv[u] = i < c1 ? L[(size_t)i * B1] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * kBS < c1) y[i0 + u * kBS] = yv[u] / dv[u];
+  }
   __syncthreads();
 }
 
@@ -607,9 +618,20 @@ __global__ void k_band_schur(BandArgs A, const int32_t* __restrict__ active, int
   const int B1 = S.b + 1, ns = S.n - S.nf;
   double* LA = A.band + S.boff;
   const double* LB = A.band + P.boff;
-  for (int t = threadIdx.x; t < ns * B1; t += blockDim.x) {
-    const int i = t / B1, d = t - i * B1;  // entry (i, i - d) of the separator
-    if (d <= i) LA[(size_t)(S.nf + i - d) * B1 + d] += LB[(size_t)(P.n - 1 - i) * B1 + d];
+  for (int t0 = threadIdx.x; t0 < ns * B1; t0 += 8 * blockDim.x) {  // 8 entries per thread per round trip
+    double va[8], vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * blockDim.x, i = t / B1, d = t - i * B1;  // entry (i, i - d) of the separator
+      const bool on = t < ns * B1 && d <= i;
+      va[u] = on ? LA[(size_t)(S.nf + i - d) * B1 + d] : 0.0;
+      vb[u] = on ? LB[(size_t)(P.n - 1 - i) * B1 + d] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * blockDim.x, i = t / B1, d = t - i * B1;
+      if (t < ns * B1 && d <= i) LA[(size_t)(S.nf + i - d) * B1 + d] = va[u] + vb[u];
+    }
   }
 }
 
