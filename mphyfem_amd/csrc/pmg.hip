@@ -131,29 +131,54 @@ __global__ void __launch_bounds__(kGT) k_galerkin(int64_t nc, const int32_t* __r
   for (int e = rptr[I]; e < rptr[I + 1]; ++e) {
     const int f = rfine[e];
     const double wf = rw[e];
-    for (int k = rowptr[f]; k < rowptr[f + 1]; ++k) {
-      const int g = colidx[k];
-      if (g >= d) break;  // columns ascend: the pressure columns close the row
-      const double kv = wf * vals[k];
-      for (int t = 0; t < 2; ++t) {
-        const int J = pidx[2 * (size_t)g + t];
-        if (J < 0) continue;
-        if (inl) {
-          int lo = 0, hi = len;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sCol[mid][th] < J) lo = mid + 1; else hi = mid;
+    const int k1 = rowptr[f + 1];
+    // the fine row in chunks of 8 entries: columns and values loaded together, then their parents
+    // and weights together (a round trip per stage, not per entry); summed in entry order
+    bool done = false;
+    for (int k0 = rowptr[f]; k0 < k1 && !done; k0 += 8) {
+      int gc[8], jc[16];
+      double vc[8], wc[16];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        gc[u] = k0 + u < k1 ? colidx[k0 + u] : d;
+        vc[u] = k0 + u < k1 ? vals[k0 + u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          jc[2 * u + t] = gc[u] < d ? pidx[2 * (size_t)gc[u] + t] : -1;
+          wc[2 * u + t] = gc[u] < d ? pw[2 * (size_t)gc[u] + t] : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int g = gc[u];
+        if (g >= d) {  // columns ascend: the pressure columns (or the row's end) close the row
+          done = true;
+          break;
+        }
+        const double kv = wf * vc[u];
+        for (int t = 0; t < 2; ++t) {
+          const int J = jc[2 * u + t];
+          if (J < 0) continue;
+          const double pwt = wc[2 * u + t];
+          if (inl) {
+            int lo = 0, hi = len;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (sCol[mid][th] < J) lo = mid + 1; else hi = mid;
+            }
+            if (lo < len && sCol[lo][th] == J) sVal[lo][th] += kv * pwt;
+            else bad[0] = 1;
+          } else {
+            int lo = cb, hi = ce;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (ccol[mid] < J) lo = mid + 1; else hi = mid;
+            }
+            if (lo < ce && ccol[lo] == J) cval[lo] += kv * pwt;
+            else bad[0] = 1;
           }
-          if (lo < len && sCol[lo][th] == J) sVal[lo][th] += kv * pw[2 * (size_t)g + t];
-          else bad[0] = 1;
-        } else {
-          int lo = cb, hi = ce;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (ccol[mid] < J) lo = mid + 1; else hi = mid;
-          }
-          if (lo < ce && ccol[lo] == J) cval[lo] += kv * pw[2 * (size_t)g + t];
-          else bad[0] = 1;
         }
       }
     }
