@@ -155,18 +155,46 @@ __global__ void __launch_bounds__(64) k_stage(ElemArgs A, int32_t nElem, double*
 #pragma unroll
       for (int a = 0; a < NPE; ++a) sdN[lane][r][a] = d[r][a];
   }
-  for (int t = lane; t < EPW * NPE; t += 64) {
-    const int k = t / NPE, a = t % NPE;
-    const int e = (base + k < nElem) ? base + k : base;  // idle slots repeat the first element
-    const size_t nd = (size_t)A.conn[(size_t)e * NPE + a];
+  {
+    // the wavefront's (element, node) pairs, NT per lane: every connectivity load issued, then every
+    // node-data load, then the LDS stores (two round trips to memory instead of two per pair)
+    constexpr int NT = (EPW * NPE + 63) / 64;
+    size_t nd[NT];
+    int ev[NT];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      sX[k][a][c] = A.X[3 * nd + c];
-      sU[k][a][c] = A.u[3 * nd + c];
-      if (MIXED) sUP[k][a][c] = A.uprev[3 * nd + c];
+    for (int i = 0; i < NT; ++i) {
+      const int t = lane + 64 * i, k = t / NPE, a = t % NPE;
+      ev[i] = (base + k < nElem) ? base + k : base;  // idle slots repeat the first element
+      nd[i] = t < EPW * NPE ? (size_t)A.conn[(size_t)ev[i] * NPE + a] : 0;
     }
-    if (St::P1 && a < 4) sP[k][a] = A.pres[nd];
-    if (St::P0 && a == 0) sP[k][0] = A.pres[e];
+    double xv[NT][3], uv[NT][3], upv[NT][3], pv[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const int t = lane + 64 * i, a = t % NPE;
+      const bool on = t < EPW * NPE;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xv[i][c] = on ? A.X[3 * nd[i] + c] : 0.0;
+        uv[i][c] = on ? A.u[3 * nd[i] + c] : 0.0;
+        upv[i][c] = (MIXED && on) ? A.uprev[3 * nd[i] + c] : 0.0;
+      }
+      pv[i] = 0.0;
+      if (St::P1 && on && a < 4) pv[i] = A.pres[nd[i]];
+      if (St::P0 && on && a == 0) pv[i] = A.pres[ev[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const int t = lane + 64 * i, k = t / NPE, a = t % NPE;
+      if (t >= EPW * NPE) continue;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sX[k][a][c] = xv[i][c];
+        sU[k][a][c] = uv[i][c];
+        if (MIXED) sUP[k][a][c] = upv[i][c];
+      }
+      if (St::P1 && a < 4) sP[k][a] = pv[i];
+      if (St::P0 && a == 0) sP[k][0] = pv[i];
+    }
   }
   __syncthreads();
   const int k = lane / NGP, gp = lane % NGP;
