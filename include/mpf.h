@@ -351,7 +351,8 @@ typedef struct {
   int32_t tile;           /* dofs per tile (0: keep the autotuned / default value) */
   int32_t lanes_per_row;  /* SpMV lanes per row, 4, 8 or 16 (0: keep) */
   int32_t solver;         /* 0: Jacobi-PCG per mesh; 1: banded LDL^T direct solve per mesh (MPF_KSP_BAND, */
-                          /*    one workgroup per mesh; exact Newton corrections, cg_* ignored)        */
+                          /*    a workgroup per mesh, or per half of a long mesh cut at a separator;    */
+                          /*    exact Newton corrections, cg_* ignored)                                 */
 } mpf_batch_opts;
 
 typedef struct {
